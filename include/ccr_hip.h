@@ -1,0 +1,312 @@
+/*
+ * ccr_hip.h -- C ABI of libccr_hip.so, the MI355X (gfx950) exposure engine.
+ *
+ * The library replaces the hot path of the `ccr` C++17 library (svr-net/zlatko):
+ * correlated risk-factor simulation, per-scenario trade revaluation (including
+ * American Monte Carlo), CSA netting and collateral, and the fused EE/PFE/EPE/CVA/DVA
+ * reductions. Nothing C++ crosses this boundary: plain structs, pointers, sizes and
+ * int status codes. The C++17 `ccr::` mirror in zlatko_amd/host/ccr_hip.hpp builds these
+ * structs from model and trade objects and rethrows the status codes as the exception
+ * types the reference uses.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository root):
+ *   ccr::ExposureEngine::run / reportingGrid / simulationGrid
+ *       include/ccr/exposure/exposure_engine.hpp:48-66, src/exposure/exposure_engine.cpp:35-81
+ *   ccr::gpu::compile -> FusedPlan, runFusedReference -> FusedOutput, summarise -> FusedResult
+ *       include/ccr/gpu/fused_exposure.hpp:338-386 (the compile / backend / summarise split)
+ *   the backend callback of app::analyseFused   src/app/analyses.cpp:2148-2177
+ *   ccr::unilateralCva / bilateralCva / pathwiseCva   include/ccr/cva/cva.hpp:20-53
+ *   ccr::BermudanSwaption::valueWithExercise   src/instruments/bermudan_swaption.cpp:49-146
+ *   ccr::ai::fitMlp (training-loop conventions)   src/ai/surrogate.cpp:161-245
+ *
+ * Ownership: the caller owns every input and pre-sizes every output array; calls are
+ * blocking. A context owns its device buffers, its HIP stream and its kernels' scratch;
+ * one context per host thread. On failure no output is written and a message is left
+ * for ccr_hip_last_error().
+ */
+#ifndef CCR_HIP_H
+#define CCR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCR_HIP_ABI_VERSION 1
+
+/* ------------------------------------------------------------------ status codes */
+/* The C++ mirror maps them to the reference's exceptions:
+ * INVALID_ARGUMENT -> std::invalid_argument, OUT_OF_RANGE -> std::out_of_range,
+ * everything else -> std::runtime_error (exposure_engine.cpp:51,66; fused_exposure.cpp:219-247). */
+enum ccr_status {
+  CCR_OK = 0,
+  CCR_ERR_INVALID_ARGUMENT = 1,
+  CCR_ERR_OUT_OF_RANGE = 2,
+  CCR_ERR_UNSUPPORTED = 3,   /* the setup exceeds what the kernels implement (cf. gpu::limitation) */
+  CCR_ERR_OUT_OF_MEMORY = 4,
+  CCR_ERR_DEVICE = 5,        /* HIP runtime or launch failure */
+  CCR_ERR_RUNTIME = 6
+};
+
+/* ------------------------------------------------------------------ market and models */
+
+/* ccr::YieldCurve: continuously compounded zero rates, linear in time, flat outside
+ * (src/market/yield_curve.cpp:20-45). times strictly increasing, n >= 1. */
+typedef struct ccr_yield_curve {
+  const double* times;
+  const double* zero_rates;
+  int32_t size;
+} ccr_yield_curve;
+
+/* ccr::CreditCurve: piecewise-flat hazards on (pillar[i-1], pillar[i]] (src/market/credit_curve.cpp:23-41). */
+typedef struct ccr_credit_curve {
+  const double* pillars;
+  const double* hazards;
+  int32_t size;
+} ccr_credit_curve;
+
+/* ccr::HullWhite1F (G1++), include/ccr/models/hull_white.hpp:17-57. */
+typedef struct ccr_hull_white {
+  ccr_yield_curve curve;
+  double mean_reversion;
+  double volatility;
+} ccr_hull_white;
+
+/* ccr::LognormalAsset, include/ccr/models/lognormal_asset.hpp:17-41.
+ * foreign_rate < 0: the carry is the deterministic `carry` curve (the reference model).
+ * foreign_rate = f >= 0: the carry is the integrated short rate of foreign Hull-White f
+ * (multi-currency extension; `carry` is then ignored). */
+typedef struct ccr_lognormal_asset {
+  double spot;
+  double volatility;
+  ccr_yield_curve carry;
+  int32_t foreign_rate;
+} ccr_lognormal_asset;
+
+/* ccr::CirIntensity (CIR++), include/ccr/models/cir_intensity.hpp:22-53. */
+typedef struct ccr_cir_intensity {
+  ccr_credit_curve market;
+  double kappa, theta, xi, y0;
+  int32_t substeps;
+} ccr_cir_intensity;
+
+/* ccr::ScenarioGenerator, include/ccr/models/scenario_generator.hpp:64-84.
+ * Correlated drivers are ordered [domestic rate, foreign rates..., assets..., credits...]
+ * (the reference order [rate, assets..., credits...] when there is no foreign rate).
+ * Normals drawn per step: [correlated drivers | domestic HW integral | CIR bridge normals
+ * per credit | foreign HW integral per foreign rate]. correlation: row-major
+ * num_factors^2, NULL for the identity. */
+typedef struct ccr_scenario_generator {
+  ccr_hull_white rates;
+  int32_t num_foreign_rates;
+  const ccr_hull_white* foreign_rates;
+  int32_t num_assets;
+  const ccr_lognormal_asset* assets;
+  int32_t num_credits;
+  const ccr_cir_intensity* credits;
+  const double* correlation;
+} ccr_scenario_generator;
+
+/* ccr::SimulationConfig, include/ccr/models/scenario_generator.hpp:48-52. */
+typedef struct ccr_simulation_config {
+  int64_t num_paths;
+  uint64_t seed;
+  int32_t antithetic;
+} ccr_simulation_config;
+
+/* ------------------------------------------------------------------ trades */
+
+enum ccr_trade_kind {
+  CCR_TRADE_SWAP = 0,               /* ccr::InterestRateSwap (interest_rate_swap.hpp:12-66) */
+  CCR_TRADE_ASSET_FORWARD = 1,      /* ccr::AssetForward (asset_forward.hpp:11-36) */
+  CCR_TRADE_BERMUDAN_SWAPTION = 2   /* ccr::BermudanSwaption (bermudan_swaption.hpp:26-62) */
+};
+
+/* One trade. Swap (and the Bermudan's underlying): `schedule` T_0 < ... < T_n as in the
+ * reference (both legs accrue on it). A swap with a different floating-leg schedule sets
+ * float_schedule (its fixed leg accrues on `schedule`); NULL/0 means the reference's single
+ * schedule. currency 0 = domestic; f + 1 = foreign Hull-White f, valued in domestic units at
+ * the asset whose foreign_rate is f. Forward: asset, notional, strike (= fixed_rate),
+ * maturity. Bermudan: exercise_times, cash_settled, position, regression_degree. */
+typedef struct ccr_trade {
+  int32_t kind;
+  int32_t pay_fixed;
+  int32_t currency;
+  int32_t asset;
+  double notional;
+  double fixed_rate;
+  double maturity;
+  const double* schedule;
+  int32_t schedule_size;
+  const double* float_schedule;
+  int32_t float_schedule_size;
+  const double* exercise_times;
+  int32_t num_exercises;
+  int32_t cash_settled;
+  double position;
+  int32_t regression_degree;
+} ccr_trade;
+
+enum ccr_collateral_type { CCR_COLLATERAL_CASH = 0, CCR_COLLATERAL_BOND = 1, CCR_COLLATERAL_ASSET = 2 };
+
+/* ccr::CollateralAgreement, include/ccr/exposure/collateral.hpp:14-36. threshold_own = +inf
+ * is a one-way CSA. */
+typedef struct ccr_collateral {
+  double threshold_counterparty;
+  double threshold_own;
+  double minimum_transfer_amount;
+  double independent_amount;
+  double margin_period_of_risk;
+  int32_t type;
+  double haircut;
+  double bond_maturity;
+  int32_t asset_index;
+} ccr_collateral;
+
+/* ccr::NettingSet (exposure_engine.hpp:19-23) plus the credit indices the fused pipeline
+ * takes from FusedOptions::counterparty: counterparty_credit / own_credit index the
+ * generator's credits (-1: none) for the pathwise CVA / DVA increments. */
+typedef struct ccr_netting_set {
+  int32_t num_trades;
+  const ccr_trade* trades;
+  int32_t has_collateral;
+  ccr_collateral collateral;
+  int32_t counterparty_credit;
+  int32_t own_credit;
+} ccr_netting_set;
+
+/* ------------------------------------------------------------------ results */
+
+/* ccr::ExposureProfile on the reporting dates (exposure_profile.hpp:18-42) plus the fused
+ * pipeline's extra per-date statistics (fused_exposure.hpp:36-38, FusedResult). Every array
+ * has `capacity` entries allocated by the caller (query the size with
+ * ccr_hip_reporting_grid). NULL arrays are skipped. cva_increments[k] (k >= 1) is
+ * E[1/2 (D V+_{k-1} + D V+_k)(Q_{k-1} - Q_k)] with Q the counterparty's pathwise survival;
+ * dva_increments the same for -V- and the own credit. exposure_rows, if not NULL, receives
+ * max(V - C, 0) as [reporting date][path] (capacity * num_paths doubles). */
+typedef struct ccr_exposure_out {
+  int32_t capacity;
+  int32_t num_dates;  /* written */
+  double* times;
+  double* expected_value;
+  double* expected_exposure;
+  double* expected_negative_exposure;
+  double* potential_future_exposure;
+  double* discounted_expected_exposure;
+  double* discounted_expected_negative_exposure;
+  double* mean_deflator;
+  double* mean_survival;
+  double* cva_increments;
+  double* dva_increments;
+  double* exposure_rows;
+} ccr_exposure_out;
+
+/* ------------------------------------------------------------------ context */
+
+enum ccr_precision {
+  CCR_FP64 = 0,  /* FP64 everywhere: the parity mode (1e-10 relative to the FP64 oracle) */
+  CCR_FP32 = 1   /* FP32 path evolution and revaluation, compensated sums for discount
+                    integrals and exposure means; FP64 final statistics */
+};
+
+typedef struct ccr_hip_options {
+  int32_t device;      /* HIP device ordinal */
+  int32_t precision;   /* default ccr_precision of runs */
+  int32_t block_size;  /* threads per workgroup of the path kernel; 0 = autotuned */
+} ccr_hip_options;
+
+typedef struct ccr_hip_ctx ccr_hip_ctx;
+typedef struct ccr_hip_plan ccr_hip_plan;
+
+/* Kernel timings of the last execute on the context's stream, from HIP events. */
+typedef struct ccr_hip_timing {
+  double path_ms;        /* path kernel (simulation, revaluation, CSA, per-block sums) */
+  double reduce_ms;      /* fixed-order reduction of the per-block sums */
+  double pfe_ms;         /* PFE order-statistic select */
+  double total_ms;       /* first launch start to last launch end */
+  double upload_ms;      /* host-to-device copies made by compile (draws, tables) */
+  double draws_ms;       /* host generation of the normal stream (compile) */
+} ccr_hip_timing;
+
+/* Shape of a compiled plan. */
+typedef struct ccr_hip_plan_info {
+  int32_t num_sets;
+  int64_t num_paths;
+  int32_t max_sim_dates;
+  int32_t max_reporting_dates;
+  int32_t num_normals;        /* normals per step */
+  int64_t total_terms;        /* valuation terms over all sets and dates */
+  int64_t stream_normals;     /* length of the replayed normal stream */
+  int64_t revaluations;       /* sum over sets of trades x paths x reporting dates */
+  int64_t term_evaluations;   /* sum over sets, paths and valued dates of the terms evaluated */
+  int64_t hbm_bytes;          /* algorithmic HBM bytes of one execute */
+  int32_t precision;
+  int32_t block_size;
+} ccr_hip_plan_info;
+
+int32_t ccr_hip_abi_version(void);
+
+/* Creates a context on options->device (NULL: device 0, FP64, autotuned block). */
+int32_t ccr_hip_create(const ccr_hip_options* options, ccr_hip_ctx** out);
+void ccr_hip_destroy(ccr_hip_ctx* ctx);
+/* Message of the last failed call on ctx ("" after a success); ctx NULL: the last
+ * failure of ccr_hip_create on this thread. */
+const char* ccr_hip_last_error(const ccr_hip_ctx* ctx);
+
+/* Host-forced failure for tests of error handling: the next call that reaches the given
+ * stage fails with the listed code, without touching the device. 0 clears. */
+enum ccr_fault {
+  CCR_FAULT_NONE = 0,
+  CCR_FAULT_ALLOC = 1,    /* device allocation -> CCR_ERR_OUT_OF_MEMORY */
+  CCR_FAULT_LAUNCH = 2,   /* kernel launch -> CCR_ERR_DEVICE */
+  CCR_FAULT_PLAN = 3      /* plan validation -> CCR_ERR_INVALID_ARGUMENT */
+};
+int32_t ccr_hip_set_fault(ccr_hip_ctx* ctx, int32_t fault);
+
+/* ExposureEngine::reportingGrid / simulationGrid of one netting set (exposure_engine.cpp:35-48).
+ * Writes up to `capacity` times and the full size into *size. Needs no device. */
+int32_t ccr_hip_reporting_grid(const double* base_grid, int32_t base_size, const ccr_netting_set* set,
+                               double* times, int32_t capacity, int32_t* size);
+int32_t ccr_hip_simulation_grid(const double* base_grid, int32_t base_size, const ccr_netting_set* set,
+                                double* times, int32_t capacity, int32_t* size);
+
+/* gpu::compile analogue: builds the FP64 per-date tables of every netting set on the host,
+ * generates the reference normal stream (std::mt19937_64 + std::normal_distribution<double>,
+ * as ccr::NormalGenerator), and uploads tables and stream to the device. Each set is simulated
+ * on its own grid exactly as ExposureEngine::run would (one generate() per set, same seed). */
+int32_t ccr_hip_compile(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator,
+                        const ccr_simulation_config* config, const double* base_grid, int32_t base_size,
+                        const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile, int32_t precision,
+                        ccr_hip_plan** out);
+int32_t ccr_hip_plan_info_get(const ccr_hip_plan* plan, ccr_hip_plan_info* info);
+/* Runs the kernels of a compiled plan (inputs already resident) and writes one
+ * ccr_exposure_out per netting set. */
+int32_t ccr_hip_execute(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs);
+/* Same, without reading results back (for timing loops). */
+int32_t ccr_hip_execute_async(ccr_hip_ctx* ctx, ccr_hip_plan* plan);
+int32_t ccr_hip_synchronize(ccr_hip_ctx* ctx);
+int32_t ccr_hip_last_timing(const ccr_hip_ctx* ctx, ccr_hip_timing* timing);
+void ccr_hip_plan_destroy(ccr_hip_plan* plan);
+
+/* ExposureEngine::run for several netting sets: compile + execute + free. */
+int32_t ccr_hip_run_exposure(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator,
+                             const ccr_simulation_config* config, const double* base_grid, int32_t base_size,
+                             const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
+                             int32_t precision, ccr_exposure_out* outs);
+
+/* ------------------------------------------------------------------ CVA (FP64, host) */
+
+/* unilateralCva (cva.cpp:22-33) and bilateralCva (cva.cpp:34-48) on a profile. */
+double ccr_unilateral_cva(const double* times, const double* discounted_ee, int32_t size,
+                          const ccr_credit_curve* counterparty, double recovery);
+void ccr_bilateral_cva(const double* times, const double* discounted_ee, const double* discounted_ene,
+                       int32_t size, const ccr_credit_curve* counterparty, double recovery_counterparty,
+                       const ccr_credit_curve* own, double recovery_own, double* cva, double* dva);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCR_HIP_H */

@@ -1,0 +1,210 @@
+// ref_driver.cpp -- TEST INFRASTRUCTURE ONLY. A thin extern "C" driver over the reference
+// library itself, compiled (by oracle/Makefile) against the reference's headers and sources where
+// they lie, into oracle/_ref/libccr_ref.so. Tests use it to pin the FP64 oracle against the real
+// reference engine; nothing in the product loads it.
+//
+// It builds reference objects from the C structs of include/ccr_hip.h and calls
+// ccr::ExposureEngine::run (src/exposure/exposure_engine.cpp:50-81), ccr::pathwiseCva
+// (src/cva/cva.cpp:55-77) and ccr::BermudanSwaption::valueWithExercise. Only reference-
+// representable inputs are accepted: no foreign rates; a two-leg swap is expressed as
+// IRS(fixed schedule, K) + IRS(float schedule, 0) - IRS(fixed schedule, 0), all reference swaps.
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../include/ccr_hip.h"
+#include "ccr/core/math.hpp"
+#include "ccr/core/random.hpp"
+#include "ccr/cva/cva.hpp"
+#include "ccr/exposure/exposure_engine.hpp"
+#include "ccr/instruments/asset_forward.hpp"
+#include "ccr/instruments/bermudan_swaption.hpp"
+#include "ccr/instruments/interest_rate_swap.hpp"
+
+namespace {
+
+std::shared_ptr<const ccr::YieldCurve> curveOf(const ccr_yield_curve& c) {
+  return std::make_shared<ccr::YieldCurve>(std::vector<double>(c.times, c.times + c.size),
+                                           std::vector<double>(c.zero_rates, c.zero_rates + c.size));
+}
+
+std::shared_ptr<const ccr::CreditCurve> creditOf(const ccr_credit_curve& c) {
+  return std::make_shared<ccr::CreditCurve>(std::vector<double>(c.pillars, c.pillars + c.size),
+                                            std::vector<double>(c.hazards, c.hazards + c.size));
+}
+
+ccr::ScenarioGenerator generatorOf(const ccr_scenario_generator& g) {
+  if (g.num_foreign_rates != 0) throw std::invalid_argument("ref driver: the reference has no foreign rates");
+  auto hw = std::make_shared<ccr::HullWhite1F>(curveOf(g.rates.curve), g.rates.mean_reversion, g.rates.volatility);
+  std::vector<std::shared_ptr<const ccr::LognormalAsset>> assets;
+  for (int a = 0; a < g.num_assets; ++a) {
+    if (g.assets[a].foreign_rate >= 0) throw std::invalid_argument("ref driver: deterministic carry only");
+    assets.push_back(std::make_shared<ccr::LognormalAsset>("A" + std::to_string(a), g.assets[a].spot,
+                                                           g.assets[a].volatility, curveOf(g.assets[a].carry)));
+  }
+  std::vector<std::shared_ptr<const ccr::CirIntensity>> credits;
+  for (int c = 0; c < g.num_credits; ++c) {
+    const auto& m = g.credits[c];
+    credits.push_back(std::make_shared<ccr::CirIntensity>("C" + std::to_string(c), creditOf(m.market), m.kappa,
+                                                          m.theta, m.xi, m.y0, static_cast<std::size_t>(m.substeps)));
+  }
+  const std::size_t n = 1 + assets.size() + credits.size();
+  ccr::Matrix corr;
+  if (g.correlation) {
+    corr = ccr::Matrix(n, n);
+    for (std::size_t i = 0; i < n; ++i)
+      for (std::size_t j = 0; j < n; ++j) corr(i, j) = g.correlation[i * n + j];
+  }
+  return ccr::ScenarioGenerator(hw, assets, credits, corr);
+}
+
+ccr::InterestRateSwap swapOf(const std::string& id, const double* s, int n, double notional, double k, bool pay) {
+  return ccr::InterestRateSwap(id, notional, k, std::vector<double>(s, s + n),
+                               pay ? ccr::InterestRateSwap::Direction::PayFixed
+                                   : ccr::InterestRateSwap::Direction::ReceiveFixed);
+}
+
+void addTrades(const ccr_trade& d, int i, ccr::NettingSet& ns) {
+  const std::string id = "T" + std::to_string(i);
+  if (d.kind == CCR_TRADE_SWAP) {
+    if (d.currency != 0) throw std::invalid_argument("ref driver: domestic swaps only");
+    const bool pay = d.pay_fixed != 0;
+    if (d.float_schedule && d.float_schedule_size > 0) {
+      ns.trades.push_back(std::make_shared<ccr::InterestRateSwap>(
+          swapOf(id + "a", d.schedule, d.schedule_size, d.notional, d.fixed_rate, pay)));
+      ns.trades.push_back(std::make_shared<ccr::InterestRateSwap>(
+          swapOf(id + "b", d.float_schedule, d.float_schedule_size, d.notional, 0.0, pay)));
+      ns.trades.push_back(std::make_shared<ccr::InterestRateSwap>(
+          swapOf(id + "c", d.schedule, d.schedule_size, d.notional, 0.0, !pay)));
+    } else {
+      ns.trades.push_back(std::make_shared<ccr::InterestRateSwap>(
+          swapOf(id, d.schedule, d.schedule_size, d.notional, d.fixed_rate, pay)));
+    }
+  } else if (d.kind == CCR_TRADE_ASSET_FORWARD) {
+    ns.trades.push_back(
+        std::make_shared<ccr::AssetForward>(id, "A" + std::to_string(d.asset), d.notional, d.fixed_rate, d.maturity));
+  } else if (d.kind == CCR_TRADE_BERMUDAN_SWAPTION) {
+    ns.trades.push_back(std::make_shared<ccr::BermudanSwaption>(
+        id, swapOf(id + "u", d.schedule, d.schedule_size, d.notional, d.fixed_rate, d.pay_fixed != 0),
+        std::vector<double>(d.exercise_times, d.exercise_times + d.num_exercises),
+        d.cash_settled ? ccr::Settlement::Cash : ccr::Settlement::Physical, d.position,
+        static_cast<std::size_t>(d.regression_degree)));
+  } else {
+    throw std::invalid_argument("ref driver: unknown trade kind");
+  }
+}
+
+int fail(char* err, int cap, const char* what, int code) {
+  if (err && cap > 0) std::snprintf(err, static_cast<std::size_t>(cap), "%s", what);
+  return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ccr_ref_run_exposure(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                             const double* base_grid, int32_t base_size, const ccr_netting_set* set,
+                             double pfe_quantile, ccr_exposure_out* out, double* pathwise_cva, double* pathwise_dva,
+                             char* err, int32_t err_capacity) {
+  try {
+    const ccr::ScenarioGenerator gen = generatorOf(*generator);
+    const ccr::SimulationConfig cfg{static_cast<std::size_t>(config->num_paths), config->seed, config->antithetic != 0};
+    const ccr::ExposureEngine engine(gen, cfg, ccr::TimeGrid(std::vector<double>(base_grid, base_grid + base_size)));
+    ccr::NettingSet ns;
+    ns.id = "NS";
+    for (int i = 0; i < set->num_trades; ++i) addTrades(set->trades[i], i, ns);
+    if (set->has_collateral) {
+      const ccr_collateral& c = set->collateral;
+      ccr::CollateralAgreement csa;
+      csa.thresholdCounterparty = c.threshold_counterparty;
+      csa.thresholdOwn = c.threshold_own;
+      csa.minimumTransferAmount = c.minimum_transfer_amount;
+      csa.independentAmount = c.independent_amount;
+      csa.marginPeriodOfRisk = c.margin_period_of_risk;
+      csa.type = c.type == CCR_COLLATERAL_BOND    ? ccr::CollateralAgreement::Type::Bond
+                 : c.type == CCR_COLLATERAL_ASSET ? ccr::CollateralAgreement::Type::Asset
+                                                  : ccr::CollateralAgreement::Type::Cash;
+      csa.haircut = c.haircut;
+      csa.bondMaturity = c.bond_maturity;
+      csa.assetIndex = static_cast<std::size_t>(c.asset_index);
+      ns.collateral = csa;
+    }
+    const ccr::ExposureResult r = engine.run(ns, pfe_quantile);
+    const auto& pr = r.profile;
+    const std::size_t nR = pr.times.size();
+    if (out->capacity < static_cast<int32_t>(nR)) return fail(err, err_capacity, "capacity", CCR_ERR_INVALID_ARGUMENT);
+    out->num_dates = static_cast<int32_t>(nR);
+    for (std::size_t k = 0; k < nR; ++k) {
+      const std::size_t j = r.reportingIndices[k];
+      if (out->times) out->times[k] = pr.times[k];
+      if (out->expected_value) out->expected_value[k] = pr.expectedValue[k];
+      if (out->expected_exposure) out->expected_exposure[k] = pr.expectedExposure[k];
+      if (out->expected_negative_exposure) out->expected_negative_exposure[k] = pr.expectedNegativeExposure[k];
+      if (out->potential_future_exposure) out->potential_future_exposure[k] = pr.potentialFutureExposure[k];
+      if (out->discounted_expected_exposure) out->discounted_expected_exposure[k] = pr.discountedExpectedExposure[k];
+      if (out->discounted_expected_negative_exposure)
+        out->discounted_expected_negative_exposure[k] = pr.discountedExpectedNegativeExposure[k];
+      if (out->mean_deflator) {
+        double m = 0.0;
+        for (std::size_t p = 0; p < r.scenarios.numPaths; ++p) m += r.scenarios.deflator(p, j);
+        out->mean_deflator[k] = m / static_cast<double>(r.scenarios.numPaths);
+      }
+      if (out->exposure_rows)
+        for (std::size_t p = 0; p < r.scenarios.numPaths; ++p)
+          out->exposure_rows[k * r.scenarios.numPaths + p] = std::max(r.exposureValue(p, j), 0.0);
+    }
+    if (pathwise_cva)
+      *pathwise_cva = set->counterparty_credit >= 0
+                          ? ccr::pathwiseCva(r.scenarios, r.exposureValue, static_cast<std::size_t>(set->counterparty_credit),
+                                             0.0, r.reportingIndices)
+                          : 0.0;
+    if (pathwise_dva) {
+      // The reference's pathwise CVA applied to -V and the own credit (the same estimator).
+      *pathwise_dva = 0.0;
+      if (set->own_credit >= 0) {
+        ccr::Matrix neg = r.exposureValue * -1.0;
+        *pathwise_dva =
+            ccr::pathwiseCva(r.scenarios, neg, static_cast<std::size_t>(set->own_credit), 0.0, r.reportingIndices);
+      }
+    }
+    return CCR_OK;
+  } catch (const std::exception& e) {
+    return fail(err, err_capacity, e.what(), CCR_ERR_RUNTIME);
+  }
+}
+
+int32_t ccr_ref_bermudan(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                         const double* grid, int32_t grid_size, const ccr_trade* trade, double* price,
+                         int32_t* exercise_index) {
+  try {
+    const ccr::ScenarioGenerator gen = generatorOf(*generator);
+    const ccr::SimulationConfig cfg{static_cast<std::size_t>(config->num_paths), config->seed, config->antithetic != 0};
+    const auto s = gen.generate(ccr::TimeGrid(std::vector<double>(grid, grid + grid_size)), cfg);
+    ccr::NettingSet ns;
+    addTrades(*trade, 0, ns);
+    const auto* b = dynamic_cast<const ccr::BermudanSwaption*>(ns.trades[0].get());
+    const auto v = b->valueWithExercise(s);
+    *price = v.price;
+    if (exercise_index)
+      for (std::size_t p = 0; p < s.numPaths; ++p)
+        exercise_index[p] = v.exerciseIndex[p] == ccr::BermudanSwaption::npos ? -1 : static_cast<int32_t>(v.exerciseIndex[p]);
+    return CCR_OK;
+  } catch (const std::exception&) {
+    return CCR_ERR_RUNTIME;
+  }
+}
+
+void ccr_ref_normals(uint64_t seed, int64_t n, double* out) {
+  ccr::NormalGenerator g(seed);
+  g.fill(out, static_cast<std::size_t>(n));
+}
+
+double ccr_ref_quantile(const double* v, int64_t n, double p) { return ccr::quantile(std::vector<double>(v, v + n), p); }
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+"""ctypes access to the CPU checkers: the FP64 oracle (oracle/liboracle.so) and, where it was
+built, the reference engine itself (oracle/_ref/libccr_ref.so). Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from zlatko_amd import abi
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libccr_ref.so"
+
+_oracle = None
+_ref = None
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "oracle"], check=True)
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_SO.exists():
+            build_oracle()
+        lib = C.CDLL(str(ORACLE_SO))
+        D, I32, I64, P = C.c_double, C.c_int32, C.c_int64, C.c_void_p
+        lib.ccr_oracle_run_exposure.restype = I32
+        lib.ccr_oracle_run_exposure.argtypes = [P, P, P, I32, P, I32, D, I32, P, C.c_char_p, I32]
+        for n in ("ccr_oracle_discount", "ccr_oracle_instantaneous_forward", "ccr_oracle_survival",
+                  "ccr_oracle_cir_survival"):
+            getattr(lib, n).restype = D
+            getattr(lib, n).argtypes = [P, D]
+        lib.ccr_oracle_zero_bond.restype = D
+        lib.ccr_oracle_zero_bond.argtypes = [P, D, D, D]
+        lib.ccr_oracle_hw_evolve.argtypes = [P, D, D, D, D, D, D, P, P]
+        lib.ccr_oracle_quantile.restype = D
+        lib.ccr_oracle_quantile.argtypes = [P, I64, D]
+        lib.ccr_oracle_normals.argtypes = [C.c_uint64, I64, P]
+        lib.ccr_oracle_mt19937_64.argtypes = [C.c_uint64, I64, P]
+        lib.ccr_oracle_epe.restype = D
+        lib.ccr_oracle_epe.argtypes = [P, P, I32, D]
+        lib.ccr_oracle_effective_epe.restype = D
+        lib.ccr_oracle_effective_epe.argtypes = [P, P, I32, D]
+        lib.ccr_oracle_unilateral_cva.restype = D
+        lib.ccr_oracle_unilateral_cva.argtypes = [P, P, I32, P, D]
+        lib.ccr_oracle_bilateral_cva.argtypes = [P, P, P, I32, P, D, P, D, P, P]
+        lib.ccr_oracle_time_grid.argtypes = [P, I32, P, I32, P]
+        lib.ccr_oracle_uniform_grid.argtypes = [D, I32, P, I32, P]
+        lib.ccr_oracle_lagged_grid.argtypes = [P, I32, D, P, I32, P]
+        lib.ccr_oracle_index_at_or_before.argtypes = [P, I32, D]
+        lib.ccr_oracle_reporting_grid.argtypes = [P, I32, P, P, I32, P]
+        lib.ccr_oracle_simulation_grid.argtypes = [P, I32, P, P, I32, P]
+        lib.ccr_oracle_cholesky.argtypes = [P, I32, P]
+        lib.ccr_oracle_solve.argtypes = [P, P, I32, P]
+        lib.ccr_oracle_regression.argtypes = [P, P, P, I64, I32, P, P, P, P]
+        lib.ccr_oracle_collateral.argtypes = [P, P, P, I32, I32, P, P]
+        lib.ccr_oracle_profile.argtypes = [P, P, I32, I32, D, P, P, P, P, P, P]
+        lib.ccr_oracle_bermudan.argtypes = [P, P, P, I32, P, P, P]
+        lib.ccr_oracle_last_timing.argtypes = [P]
+        _oracle = lib
+    return _oracle
+
+
+def ref():
+    """The reference engine built from its own sources, or None where it is not built."""
+    global _ref
+    if _ref is None and REF_SO.exists():
+        lib = C.CDLL(str(REF_SO))
+        D, I32, I64, P = C.c_double, C.c_int32, C.c_int64, C.c_void_p
+        lib.ccr_ref_run_exposure.restype = I32
+        lib.ccr_ref_run_exposure.argtypes = [P, P, P, I32, P, D, P, P, P, C.c_char_p, I32]
+        lib.ccr_ref_bermudan.restype = I32
+        lib.ccr_ref_bermudan.argtypes = [P, P, P, I32, P, P, P]
+        lib.ccr_ref_normals.argtypes = [C.c_uint64, I64, P]
+        lib.ccr_ref_quantile.restype = D
+        lib.ccr_ref_quantile.argtypes = [P, I64, D]
+        _ref = lib
+    return _ref
+
+
+class OracleTiming(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("draws_s", "simulate_s", "pricing_s", "collateral_s", "statistics_s",
+                                          "total_s")] + [("revaluations", C.c_int64)]
+
+
+def grid_size(spec: abi.Spec, i: int) -> int:
+    n = C.c_int32()
+    oracle().ccr_oracle_reporting_grid(abi.dptr(spec.base_grid), len(spec.base_grid), C.byref(spec.sets_c[i]),
+                                       None, 0, C.byref(n))
+    return n.value
+
+
+def run_oracle(spec: abi.Spec, threads: int = 1, rows: bool = False):
+    cap = max(grid_size(spec, i) for i in range(len(spec.sets)))
+    outs, arr = abi.make_outs(len(spec.sets), cap, spec.num_paths, rows)
+    err = C.create_string_buffer(512)
+    rc = oracle().ccr_oracle_run_exposure(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                          len(spec.base_grid), spec.sets_c, len(spec.sets), spec.pfe_quantile,
+                                          threads, arr, err, 512)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
+    abi.sync_outs(outs, arr)
+    return outs
+
+
+def oracle_timing() -> OracleTiming:
+    t = OracleTiming()
+    oracle().ccr_oracle_last_timing(C.byref(t))
+    return t
+
+
+def run_ref(spec: abi.Spec, i: int = 0, rows: bool = False):
+    lib = ref()
+    cap = grid_size(spec, i)
+    outs, arr = abi.make_outs(1, cap, spec.num_paths, rows)
+    err = C.create_string_buffer(512)
+    cva, dva = C.c_double(), C.c_double()
+    rc = lib.ccr_ref_run_exposure(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                  len(spec.base_grid), C.byref(spec.sets_c[i]), spec.pfe_quantile, arr,
+                                  C.byref(cva), C.byref(dva), err, 512)
+    if rc != 0:
+        raise RuntimeError(f"reference failed ({rc}): {err.value.decode()}")
+    abi.sync_outs(outs, arr)
+    return outs[0], cva.value, dva.value
+
+
+def normals(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n)
+    oracle().ccr_oracle_normals(seed, n, out.ctypes.data_as(C.c_void_p))
+    return out

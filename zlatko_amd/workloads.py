@@ -1,0 +1,197 @@
+"""Seeded synthetic workloads: the five benchmark portfolios and small parity cases.
+
+Trades are sampled with SplitMix64 (seed per workload) so every process, here or on the
+GPU host, builds bit-identical inputs. Sizes (paths, dates) can be scaled down for parity
+tests; `full=True` gives the benchmark shapes.
+
+  c1  Plain FP64, one uncollateralised netting set of 10 vanilla IRS (semiannual fixed vs
+      quarterly float), HW1F a=0.03 sigma=0.01 on a flat 3% curve, 5,000 paths x 120 monthly
+      dates, PFE 97.5%.
+  c2  3 currencies on Hull-White plus 2 lognormal FX rates with a random SPD correlation
+      (|rho| <= 0.6); 1,000 trades (700 IRS, 300 FX forwards) over 20 netting sets under CSA
+      (threshold 0-1e7, MTA 5e5, 10-day MPoR); 50,000 paths x 260 weekly dates.
+  c3  200 Bermudan payer/receiver swaptions on HW1F (AMC), 100,000 paths x 120 dates.
+  c4  HW + 1 FX + CIR++ counterparty/own intensities with 0.3 wrong-way correlation.
+  c5  500 netting sets, Zipf(1.2) trade counts, half under CSA.
+"""
+from __future__ import annotations
+
+import math
+from typing import List
+
+import numpy as np
+
+from .abi import (CCR_TRADE_ASSET_FORWARD, CCR_TRADE_BERMUDAN_SWAPTION, CCR_TRADE_SWAP, Asset, Cir, Collateral,
+                  CreditCurve, Curve, HullWhite, Model, NettingSet, Spec, Trade, swap_schedule, uniform_grid)
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): state += 0x9e3779b97f4a7c15, then two xor-shift-multiply rounds."""
+
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / (1 << 53)))
+
+    def integer(self, lo: int, hi: int) -> int:
+        """Uniform integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1)) if hi > lo else lo
+
+    def bernoulli(self) -> bool:
+        return (self.next() >> 63) == 1
+
+
+def two_leg_swap(rng: SplitMix64, notional_lo=1e6, notional_hi=1e8, mat_lo=1.0, mat_hi=10.0, currency=0) -> Trade:
+    notional = rng.uniform(notional_lo, notional_hi)
+    pay = rng.bernoulli()
+    maturity = rng.uniform(mat_lo, mat_hi)
+    k = rng.uniform(0.02, 0.04)
+    return Trade(CCR_TRADE_SWAP, notional, k, pay, schedule=swap_schedule(0.0, maturity, 2),
+                 float_schedule=swap_schedule(0.0, maturity, 4), currency=currency)
+
+
+def random_correlation(rng: SplitMix64, n: int, max_abs: float = 0.6) -> np.ndarray:
+    """Seeded SPD correlation with |rho| <= max_abs: normalised Gram of random vectors, rejected until it fits."""
+    while True:
+        v = np.array([[rng.uniform(-1.0, 1.0) for _ in range(n + 2)] for _ in range(n)])
+        g = v @ v.T
+        d = np.sqrt(np.diag(g))
+        c = g / np.outer(d, d)
+        np.fill_diagonal(c, 1.0)
+        off = np.abs(c[~np.eye(n, dtype=bool)])
+        if off.size == 0 or off.max() <= max_abs:
+            if np.linalg.eigvalsh(c).min() > 1e-6:
+                return c
+
+
+def c1(num_paths: int = 5000, dates: int = 120, trades: int = 10, seed: int = 42) -> Spec:
+    rng = SplitMix64(1)
+    model = Model(HullWhite(Curve.flat(0.03), 0.03, 0.01))
+    ts = [two_leg_swap(rng) for _ in range(trades)]
+    return Spec(model, [NettingSet(ts)], uniform_grid(10.0, dates), num_paths, seed, True, 0.975, "c1")
+
+
+def c2_model(rng: SplitMix64) -> Model:
+    rates = [HullWhite(Curve.flat(rng.uniform(0.01, 0.04)), rng.uniform(0.01, 0.1), rng.uniform(0.005, 0.015))
+             for _ in range(3)]
+    spots = [1.1, 0.9]
+    assets = [Asset(spots[i], rng.uniform(0.08, 0.15), rates[1 + i].curve, foreign_rate=i) for i in range(2)]
+    corr = random_correlation(rng, 5, 0.6)
+    return Model(rates[0], rates[1:], assets, [], corr)
+
+
+def c2(num_paths: int = 50000, weeks: int = 260, num_sets: int = 20, num_trades: int = 1000,
+       seed: int = 42, horizon: float = 5.0) -> Spec:
+    rng = SplitMix64(2)
+    model = c2_model(rng)
+    n_irs = int(round(num_trades * 0.7))
+    trades: List[Trade] = []
+    for i in range(num_trades):
+        if i < n_irs:
+            ccy = rng.integer(0, 2)
+            trades.append(two_leg_swap(rng, mat_lo=0.5, mat_hi=horizon, currency=ccy))
+        else:
+            a = rng.integer(0, 1)
+            n = rng.uniform(1e6, 1e8) * (1.0 if rng.bernoulli() else -1.0)
+            mat = rng.uniform(0.25, horizon)
+            fwd = model.assets[a].spot * math.exp((model.rates.curve.rates[0] - model.assets[a].carry.rates[0]) * mat)
+            trades.append(Trade(CCR_TRADE_ASSET_FORWARD, n, fwd * rng.uniform(0.9, 1.1), asset=a, maturity=mat))
+    # Deal IRS and forwards round-robin so every set mixes both kinds.
+    order = [i for pair in zip(range(n_irs), range(n_irs, num_trades)) for i in pair]
+    order += [i for i in range(num_trades) if i not in set(order)]
+    sets = [NettingSet([]) for _ in range(num_sets)]
+    for k, i in enumerate(order):
+        sets[k % num_sets].trades.append(trades[i])
+    for ns in sets:
+        th = rng.uniform(0.0, 1e7)
+        ns.collateral = Collateral(threshold_counterparty=th, threshold_own=th, mta=5e5, mpr=10.0 / 250.0)
+    return Spec(model, sets, uniform_grid(horizon, weeks), num_paths, seed, True, 0.975, "c2")
+
+
+def bermudan(rng: SplitMix64, notional_hi=5e7) -> Trade:
+    notional = rng.uniform(1e6, notional_hi)
+    pay = rng.bernoulli()
+    expiry = float(rng.integer(1, 10))
+    tenor = float(rng.integer(5, 10))
+    sched = swap_schedule(expiry, tenor, 1)
+    return Trade(CCR_TRADE_BERMUDAN_SWAPTION, notional, rng.uniform(0.02, 0.04), pay, schedule=sched,
+                 exercise=sched[:-1].copy(), position=1.0, degree=3)
+
+
+def c3(num_paths: int = 100000, dates: int = 120, num_swaptions: int = 200, seed: int = 42) -> Spec:
+    rng = SplitMix64(3)
+    model = Model(HullWhite(Curve.flat(0.03), 0.03, 0.01))
+    ts = [bermudan(rng) for _ in range(num_swaptions)]
+    horizon = max(t.schedule[-1] for t in ts)
+    return Spec(model, [NettingSet(ts)], uniform_grid(horizon, dates), num_paths, seed, True, 0.975, "c3")
+
+
+def c4_model(rho_wrong_way: float = 0.3) -> Model:
+    dom = HullWhite(Curve.flat(0.03), 0.03, 0.01)
+    fx = Asset(1.1, 0.12, Curve.flat(0.02))
+    hazard = CreditCurve.flat(0.02)
+    cpty = Cir(hazard, 0.5, 0.02, 0.1, 0.02, 4)
+    own = Cir(CreditCurve.flat(0.01), 0.5, 0.01, 0.05, 0.01, 4)
+    corr = np.eye(4)
+    corr[1, 2] = corr[2, 1] = rho_wrong_way  # FX driver vs counterparty intensity
+    return Model(dom, [], [fx], [cpty, own], corr)
+
+
+def c4(num_paths: int = 100000, dates: int = 120, seed: int = 42, with_bermudans: bool = True) -> Spec:
+    rng = SplitMix64(4)
+    model = c4_model()
+    ts: List[Trade] = [two_leg_swap(rng) for _ in range(20)]
+    for _ in range(10):
+        n = rng.uniform(1e6, 1e8)
+        mat = rng.uniform(1.0, 10.0)
+        ts.append(Trade(CCR_TRADE_ASSET_FORWARD, n, 1.1 * rng.uniform(0.9, 1.1), asset=0, maturity=mat))
+    if with_bermudans:
+        ts += [bermudan(rng) for _ in range(10)]
+    horizon = max(max(t.schedule[-1] if t.schedule is not None else t.maturity for t in ts), 10.0)
+    ns = NettingSet(ts, counterparty_credit=0, own_credit=1)
+    return Spec(model, [ns], uniform_grid(horizon, dates), num_paths, seed, True, 0.975, "c4")
+
+
+def small_reference_case(num_paths: int = 64, dates: int = 12, csa: bool = False, fx: bool = False,
+                         credits: bool = False, two_leg: bool = False, seed: int = 7) -> Spec:
+    """Small reference-representable netting set for parity against a build of the reference."""
+    rng = SplitMix64(11)
+    curve = Curve([0.5, 2.0, 5.0, 10.0], [0.020, 0.025, 0.030, 0.033])
+    model = Model(HullWhite(curve, 0.05, 0.01))
+    nf = 1
+    if fx:
+        model.assets.append(Asset(1.1, 0.12, Curve.flat(0.01)))
+        nf += 1
+    if credits:
+        model.credits.append(Cir(CreditCurve([1.0, 3.0, 5.0], [0.01, 0.015, 0.02]), 0.5, 0.012, 0.08, 0.008, 4))
+        model.credits.append(Cir(CreditCurve.flat(0.01), 0.4, 0.01, 0.05, 0.01, 2))
+        nf += 2
+    if nf > 1:
+        model.correlation = random_correlation(SplitMix64(5), nf, 0.5)
+    ts = []
+    for i in range(4):
+        if two_leg:
+            ts.append(two_leg_swap(rng, mat_lo=1.0, mat_hi=5.0))
+        else:
+            mat = rng.uniform(1.0, 5.0)
+            ts.append(Trade(CCR_TRADE_SWAP, rng.uniform(1e5, 1e6), rng.uniform(0.02, 0.035), rng.bernoulli(),
+                            schedule=swap_schedule(0.0, mat, 2)))
+    if fx:
+        ts.append(Trade(CCR_TRADE_ASSET_FORWARD, 5e5, 1.05, asset=0, maturity=3.0))
+        ts.append(Trade(CCR_TRADE_ASSET_FORWARD, -3e5, 1.12, asset=0, maturity=1.5))
+    ns = NettingSet(ts)
+    if csa:
+        ns.collateral = Collateral(threshold_counterparty=2e4, threshold_own=5e4, mta=1e4, mpr=10.0 / 250.0)
+    if credits:
+        ns.counterparty_credit, ns.own_credit = 0, 1
+    return Spec(model, [ns], uniform_grid(5.0, dates), num_paths, seed, True, 0.95, "small")

@@ -1,0 +1,508 @@
+// capi.cpp -- the C ABI of libccr_hip.so (include/ccr_hip.h): contexts, plan compile/upload,
+// kernel execution with HIP-event timing, read-back into caller-owned ccr_exposure_out arrays.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/ccr_hip.h"
+#include "launch.hpp"
+#include "plan.hpp"
+
+
+using namespace ccrhip;
+
+struct ccr_hip_ctx {
+  int device = 0;
+  int precision = CCR_FP64;
+  int block = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::string err;
+  int fault = CCR_FAULT_NONE;
+  bool timing_pending = false;  // events recorded by a launch not yet read into `timing`
+  double upload_ms = 0.0, draws_ms = 0.0;
+  ccr_hip_timing timing{};
+};
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct ccr_hip_plan {
+  HostPlan host;
+  int device = 0;
+  std::vector<DevBuf> bufs;
+  GlobalDev* d_global = nullptr;
+  SetDev* d_sets = nullptr;
+  int32_t* d_block_set = nullptr;
+  void* d_stream = nullptr;
+  void* d_steps = nullptr;
+  DateRec* d_dates = nullptr;
+  void* d_date_reals = nullptr;
+  void* d_bonds = nullptr;
+  void* d_fixed = nullptr;
+  int32_t* d_fixstore = nullptr;
+  void* d_rows = nullptr;
+  double* d_partials = nullptr;
+  double* d_stats = nullptr;
+  double* d_pfe = nullptr;
+  double draws_ms = 0.0, upload_ms = 0.0;
+  ~ccr_hip_plan() {
+    for (auto& b : bufs)
+      if (b.p) (void)hipFree(b.p);
+  }
+};
+
+namespace {
+
+thread_local std::string gCreateError;
+
+void check(hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    fail(CCR_ERR_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+  if (e != hipSuccess) fail(CCR_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int32_t guarded(ccr_hip_ctx* ctx, const std::function<void()>& f) {
+  try {
+    f();
+    if (ctx) ctx->err.clear();
+    return CCR_OK;
+  } catch (const Error& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->err = "host allocation failed";
+    return CCR_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return CCR_ERR_RUNTIME;
+  }
+}
+
+void* devAlloc(ccr_hip_ctx* ctx, ccr_hip_plan* plan, size_t bytes) {
+  if (ctx->fault == CCR_FAULT_ALLOC) {
+    ctx->fault = CCR_FAULT_NONE;
+    fail(CCR_ERR_OUT_OF_MEMORY, "hipMalloc: out of memory (forced by ccr_hip_set_fault)");
+  }
+  void* p = nullptr;
+  check(hipMalloc(&p, bytes < 16 ? 16 : bytes), "hipMalloc");
+  plan->bufs.push_back({p, bytes});
+  return p;
+}
+
+template <typename T>
+T* upload(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const T* src, size_t n) {
+  T* d = static_cast<T*>(devAlloc(ctx, plan, n * sizeof(T)));
+  if (n) check(hipMemcpyAsync(d, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
+  return d;
+}
+
+void* uploadReals(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const std::vector<double>& v, bool fp32) {
+  if (!fp32) return upload(ctx, plan, v.data(), v.size());
+  std::vector<float> f(v.begin(), v.end());
+  void* d = upload(ctx, plan, f.data(), f.size());
+  check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return d;
+}
+
+// Generates the normal stream in chunks into two pinned staging buffers, overlapping generation
+// of one chunk with the upload of the other.
+void uploadStream(ccr_hip_ctx* ctx, ccr_hip_plan* plan, uint64_t seed, bool fp32) {
+  const int64_t n = plan->host.stream_len;
+  const size_t elem = fp32 ? sizeof(float) : sizeof(double);
+  plan->d_stream = devAlloc(ctx, plan, static_cast<size_t>(n) * elem);
+  if (n == 0) return;
+  const int64_t chunk = 1 << 22;
+  void* stage[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  check(hipHostMalloc(&stage[0], static_cast<size_t>(chunk) * elem, hipHostMallocDefault), "hipHostMalloc");
+  check(hipHostMalloc(&stage[1], static_cast<size_t>(chunk) * elem, hipHostMallocDefault), "hipHostMalloc");
+  check(hipEventCreate(&done[0]), "hipEventCreate");
+  check(hipEventCreate(&done[1]), "hipEventCreate");
+  std::mt19937_64 engine(seed);
+  std::normal_distribution<double> normal(0.0, 1.0);
+  double genMs = 0.0;
+  const auto t0 = std::chrono::steady_clock::now();
+  int which = 0;
+  bool pending[2] = {false, false};
+  for (int64_t off = 0; off < n; off += chunk, which ^= 1) {
+    const int64_t m = std::min(chunk, n - off);
+    if (pending[which]) check(hipEventSynchronize(done[which]), "hipEventSynchronize");
+    const auto g0 = std::chrono::steady_clock::now();
+    if (fp32) {
+      float* f = static_cast<float*>(stage[which]);
+      for (int64_t i = 0; i < m; ++i) f[i] = static_cast<float>(normal(engine));
+    } else {
+      double* d = static_cast<double*>(stage[which]);
+      for (int64_t i = 0; i < m; ++i) d[i] = normal(engine);
+    }
+    genMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
+    check(hipMemcpyAsync(static_cast<char*>(plan->d_stream) + static_cast<size_t>(off) * elem, stage[which],
+                         static_cast<size_t>(m) * elem, hipMemcpyHostToDevice, ctx->stream),
+          "hipMemcpyAsync");
+    check(hipEventRecord(done[which], ctx->stream), "hipEventRecord");
+    pending[which] = true;
+  }
+  check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  (void)hipEventDestroy(done[0]);
+  (void)hipEventDestroy(done[1]);
+  (void)hipHostFree(stage[0]);
+  (void)hipHostFree(stage[1]);
+  plan->draws_ms = genMs;
+  plan->upload_ms += total - genMs;
+}
+
+void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                 const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
+                 double pfe_quantile, int32_t precision, ccr_hip_plan** out) {
+  if (!ctx || !generator || !config || !out) invalid("null argument");
+  if (!base_grid && base_size > 0) invalid("null base grid");
+  if (precision != CCR_FP64 && precision != CCR_FP32) invalid("precision must be CCR_FP64 or CCR_FP32");
+  if (ctx->fault == CCR_FAULT_PLAN) {
+    ctx->fault = CCR_FAULT_NONE;
+    invalid("invalid plan (forced by ccr_hip_set_fault)");
+  }
+  check(hipSetDevice(ctx->device), "hipSetDevice");
+  auto plan = std::make_unique<ccr_hip_plan>();
+  plan->device = ctx->device;
+  plan->host = buildPlan(*generator, *config, base_grid, base_size, sets, num_sets, pfe_quantile, precision, ctx->block);
+  HostPlan& h = plan->host;
+  const bool fp32 = precision == CCR_FP32;
+  const auto t0 = std::chrono::steady_clock::now();
+  plan->d_global = upload(ctx, plan.get(), &h.g, 1);
+  plan->d_sets = upload(ctx, plan.get(), h.sets.data(), h.sets.size());
+  plan->d_block_set = upload(ctx, plan.get(), h.block_set.data(), h.block_set.size());
+  plan->d_steps = uploadReals(ctx, plan.get(), h.steps, fp32);
+  plan->d_dates = upload(ctx, plan.get(), h.dates.data(), h.dates.size());
+  plan->d_date_reals = uploadReals(ctx, plan.get(), h.date_reals, fp32);
+  plan->d_bonds = uploadReals(ctx, plan.get(), h.bonds, fp32);
+  plan->d_fixed = uploadReals(ctx, plan.get(), h.fixed, fp32);
+  plan->d_fixstore = upload(ctx, plan.get(), h.fixstore.data(), h.fixstore.size());
+  const size_t elem = fp32 ? sizeof(float) : sizeof(double);
+  const int64_t P = h.g.num_paths;
+  plan->d_rows = devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * P * elem);
+  plan->d_partials =
+      static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(h.total_waves) * h.max_rep * kStats * sizeof(double)));
+  plan->d_stats = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * kStats * sizeof(double)));
+  plan->d_pfe = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * sizeof(double)));
+  check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  plan->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  uploadStream(ctx, plan.get(), config->seed, fp32);
+  *out = plan.release();
+}
+
+void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
+  if (ctx->fault == CCR_FAULT_LAUNCH) {
+    ctx->fault = CCR_FAULT_NONE;
+    fail(CCR_ERR_DEVICE, "kernel launch failed (forced by ccr_hip_set_fault)");
+  }
+  check(hipSetDevice(ctx->device), "hipSetDevice");
+  const HostPlan& h = plan->host;
+  const int nsets = static_cast<int>(h.sets.size());
+  const int blocks = static_cast<int>(h.block_set.size());
+  check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
+  if (h.precision == CCR_FP32) {
+    PathArgs<float> a{plan->d_global, plan->d_sets, plan->d_block_set, static_cast<const float*>(plan->d_stream),
+                      static_cast<const float*>(plan->d_steps), plan->d_dates, static_cast<const float*>(plan->d_date_reals),
+                      static_cast<const float*>(plan->d_bonds), static_cast<const float*>(plan->d_fixed), plan->d_fixstore,
+                      static_cast<float*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.g.max_slots};
+    check(launchPath<float>(a, blocks, h.block, ctx->stream), "path kernel launch");
+  } else {
+    PathArgs<double> a{plan->d_global, plan->d_sets, plan->d_block_set, static_cast<const double*>(plan->d_stream),
+                       static_cast<const double*>(plan->d_steps), plan->d_dates, static_cast<const double*>(plan->d_date_reals),
+                       static_cast<const double*>(plan->d_bonds), static_cast<const double*>(plan->d_fixed), plan->d_fixstore,
+                       static_cast<double*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.g.max_slots};
+    check(launchPath<double>(a, blocks, h.block, ctx->stream), "path kernel launch");
+  }
+  check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
+  check(launchReduce(plan->d_sets, plan->d_partials, nsets, h.max_rep, h.total_waves / nsets,
+                     1.0 / static_cast<double>(h.g.num_paths), plan->d_stats, ctx->stream),
+        "reduce kernel launch");
+  check(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
+  if (h.precision == CCR_FP32)
+    check(launchPfe<float>(plan->d_sets, static_cast<const float*>(plan->d_rows), nsets, h.max_rep, h.g.num_paths,
+                           h.pfe_q, plan->d_pfe, ctx->stream),
+          "pfe kernel launch");
+  else
+    check(launchPfe<double>(plan->d_sets, static_cast<const double*>(plan->d_rows), nsets, h.max_rep, h.g.num_paths,
+                            h.pfe_q, plan->d_pfe, ctx->stream),
+          "pfe kernel launch");
+  check(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
+  ctx->timing_pending = true;
+  ctx->upload_ms = plan->upload_ms;
+  ctx->draws_ms = plan->draws_ms;
+}
+
+// Reads the event timings of the last launch (waits for it to finish).
+void readTimings(ccr_hip_ctx* ctx) {
+  if (!ctx->timing_pending) return;
+  ctx->timing_pending = false;
+  check(hipEventSynchronize(ctx->ev[3]), "hipEventSynchronize");
+  float a = 0, b = 0, c = 0, t = 0;
+  check(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
+  check(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
+  check(hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
+  check(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[3]), "hipEventElapsedTime");
+  ctx->timing.path_ms = a;
+  ctx->timing.reduce_ms = b;
+  ctx->timing.pfe_ms = c;
+  ctx->timing.total_ms = t;
+  ctx->timing.upload_ms = ctx->upload_ms;
+  ctx->timing.draws_ms = ctx->draws_ms;
+}
+
+void readBack(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs) {
+  const HostPlan& h = plan->host;
+  const size_t nsets = h.sets.size();
+  std::vector<double> stats(nsets * static_cast<size_t>(h.max_rep) * kStats), pfe(nsets * static_cast<size_t>(h.max_rep));
+  check(hipMemcpyAsync(stats.data(), plan->d_stats, stats.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+        "hipMemcpyAsync");
+  check(hipMemcpyAsync(pfe.data(), plan->d_pfe, pfe.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+        "hipMemcpyAsync");
+  check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  readTimings(ctx);
+  const int64_t P = h.g.num_paths;
+  for (size_t s = 0; s < nsets; ++s) {
+    const int R = h.sets[s].num_rep;
+    if (outs[s].capacity < R) invalid("output capacity smaller than the reporting grid");
+  }
+  for (size_t s = 0; s < nsets; ++s) {
+    ccr_exposure_out& o = outs[s];
+    const int R = h.sets[s].num_rep;
+    o.num_dates = R;
+    for (int r = 0; r < R; ++r) {
+      const double* st = stats.data() + (s * static_cast<size_t>(h.max_rep) + static_cast<size_t>(r)) * kStats;
+      if (o.times) o.times[r] = h.rep_times[s][static_cast<size_t>(r)];
+      if (o.expected_value) o.expected_value[r] = st[0];
+      if (o.expected_exposure) o.expected_exposure[r] = st[1];
+      if (o.expected_negative_exposure) o.expected_negative_exposure[r] = st[2];
+      if (o.discounted_expected_exposure) o.discounted_expected_exposure[r] = st[3];
+      if (o.discounted_expected_negative_exposure) o.discounted_expected_negative_exposure[r] = st[4];
+      if (o.mean_deflator) o.mean_deflator[r] = st[5];
+      if (o.mean_survival) o.mean_survival[r] = st[6];
+      if (o.cva_increments) o.cva_increments[r] = st[7];
+      if (o.dva_increments) o.dva_increments[r] = st[8];
+      if (o.potential_future_exposure) o.potential_future_exposure[r] = pfe[s * static_cast<size_t>(h.max_rep) + static_cast<size_t>(r)];
+    }
+    if (o.exposure_rows) {
+      const size_t n = static_cast<size_t>(R) * static_cast<size_t>(P);
+      const size_t off = static_cast<size_t>(h.sets[s].rows_off);
+      if (h.precision == CCR_FP32) {
+        std::vector<float> f(n);
+        check(hipMemcpy(f.data(), static_cast<float*>(plan->d_rows) + off, n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+        for (size_t i = 0; i < n; ++i) o.exposure_rows[i] = f[i];
+      } else {
+        check(hipMemcpy(o.exposure_rows, static_cast<double*>(plan->d_rows) + off, n * sizeof(double), hipMemcpyDeviceToHost),
+              "hipMemcpy");
+      }
+    }
+  }
+}
+
+int32_t writeGrid(const TimeGrid& g, double* out, int32_t cap, int32_t* size) {
+  if (size) *size = static_cast<int32_t>(g.size());
+  if (out)
+    for (size_t i = 0; i < g.size() && static_cast<int32_t>(i) < cap; ++i) out[i] = g[i];
+  return CCR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ccr_hip_abi_version(void) { return CCR_HIP_ABI_VERSION; }
+
+int32_t ccr_hip_create(const ccr_hip_options* options, ccr_hip_ctx** out) {
+  if (!out) return CCR_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto ctx = std::make_unique<ccr_hip_ctx>();
+  if (options) {
+    ctx->device = options->device;
+    ctx->precision = options->precision;
+    ctx->block = options->block_size > 0 ? options->block_size : 256;
+  }
+  if (ctx->block % 64 != 0 || ctx->block > 1024) {
+    gCreateError = "block_size must be a multiple of 64 up to 1024";
+    return CCR_ERR_INVALID_ARGUMENT;
+  }
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    gCreateError = std::string("no HIP device: ") + hipGetErrorString(e);
+    return CCR_ERR_DEVICE;
+  }
+  if (ctx->device < 0 || ctx->device >= n) {
+    gCreateError = "device ordinal out of range";
+    return CCR_ERR_OUT_OF_RANGE;
+  }
+  e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
+  if (e != hipSuccess) {
+    gCreateError = std::string("HIP initialisation failed: ") + hipGetErrorString(e);
+    return CCR_ERR_DEVICE;
+  }
+  gCreateError.clear();
+  *out = ctx.release();
+  return CCR_OK;
+}
+
+void ccr_hip_destroy(ccr_hip_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* ccr_hip_last_error(const ccr_hip_ctx* ctx) { return ctx ? ctx->err.c_str() : gCreateError.c_str(); }
+
+int32_t ccr_hip_set_fault(ccr_hip_ctx* ctx, int32_t fault) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  if (fault < CCR_FAULT_NONE || fault > CCR_FAULT_PLAN) return CCR_ERR_INVALID_ARGUMENT;
+  ctx->fault = fault;
+  return CCR_OK;
+}
+
+int32_t ccr_hip_reporting_grid(const double* base_grid, int32_t base_size, const ccr_netting_set* set, double* times,
+                               int32_t capacity, int32_t* size) {
+  return guarded(nullptr, [&] {
+    if (!set) invalid("null netting set");
+    writeGrid(reportingGrid(TimeGrid(std::vector<double>(base_grid, base_grid + std::max(0, base_size))), *set), times,
+              capacity, size);
+  });
+}
+
+int32_t ccr_hip_simulation_grid(const double* base_grid, int32_t base_size, const ccr_netting_set* set, double* times,
+                                int32_t capacity, int32_t* size) {
+  return guarded(nullptr, [&] {
+    if (!set) invalid("null netting set");
+    writeGrid(simulationGrid(TimeGrid(std::vector<double>(base_grid, base_grid + std::max(0, base_size))), *set), times,
+              capacity, size);
+  });
+}
+
+int32_t ccr_hip_compile(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                        const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
+                        double pfe_quantile, int32_t precision, ccr_hip_plan** out) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    compilePlan(ctx, generator, config, base_grid, base_size, sets, num_sets, pfe_quantile, precision, out);
+  });
+}
+
+int32_t ccr_hip_plan_info_get(const ccr_hip_plan* plan, ccr_hip_plan_info* info) {
+  if (!plan || !info) return CCR_ERR_INVALID_ARGUMENT;
+  const HostPlan& h = plan->host;
+  info->num_sets = static_cast<int32_t>(h.sets.size());
+  info->num_paths = h.g.num_paths;
+  info->max_sim_dates = h.max_dates;
+  info->max_reporting_dates = h.max_rep;
+  info->num_normals = h.g.nz;
+  info->total_terms = h.total_terms;
+  info->stream_normals = h.stream_len;
+  info->revaluations = h.revaluations;
+  info->term_evaluations = h.term_evaluations;
+  const int64_t elem = h.precision == CCR_FP32 ? 4 : 8;
+  int64_t rows = 0;
+  for (const auto& s : h.sets) rows += static_cast<int64_t>(s.num_rep) * h.g.num_paths;
+  // Replayed stream read once, exposure rows written once and read by the PFE select.
+  info->hbm_bytes = h.stream_len * elem + 2 * rows * elem;
+  info->precision = h.precision;
+  info->block_size = h.block;
+  return CCR_OK;
+}
+
+int32_t ccr_hip_execute_async(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!plan) invalid("null plan");
+    launchAll(ctx, plan);
+  });
+}
+
+int32_t ccr_hip_synchronize(ccr_hip_ctx* ctx) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] { check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); });
+}
+
+int32_t ccr_hip_execute(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!plan || !outs) invalid("null argument");
+    launchAll(ctx, plan);
+    readBack(ctx, plan, outs);
+  });
+}
+
+int32_t ccr_hip_last_timing(const ccr_hip_ctx* ctx, ccr_hip_timing* timing) {
+  if (!ctx || !timing) return CCR_ERR_INVALID_ARGUMENT;
+  ccr_hip_ctx* c = const_cast<ccr_hip_ctx*>(ctx);
+  const int32_t rc = guarded(c, [&] { readTimings(c); });
+  *timing = ctx->timing;
+  return rc;
+}
+
+void ccr_hip_plan_destroy(ccr_hip_plan* plan) {
+  if (!plan) return;
+  (void)hipSetDevice(plan->device);
+  delete plan;
+}
+
+int32_t ccr_hip_run_exposure(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                             const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
+                             double pfe_quantile, int32_t precision, ccr_exposure_out* outs) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!outs) invalid("null outputs");
+    ccr_hip_plan* raw = nullptr;
+    compilePlan(ctx, generator, config, base_grid, base_size, sets, num_sets, pfe_quantile, precision, &raw);
+    std::unique_ptr<ccr_hip_plan, void (*)(ccr_hip_plan*)> plan(raw, ccr_hip_plan_destroy);
+    launchAll(ctx, plan.get());
+    readBack(ctx, plan.get(), outs);
+  });
+}
+
+double ccr_unilateral_cva(const double* times, const double* discounted_ee, int32_t size,
+                          const ccr_credit_curve* counterparty, double recovery) {
+  // unilateralCva: src/cva/cva.cpp:11-33 (trapezoid in EE* against the default probabilities).
+  const Credit c(*counterparty);
+  double cva = 0.0;
+  for (int32_t i = 1; i < size; ++i) {
+    const double ee = 0.5 * (discounted_ee[i - 1] + discounted_ee[i]);
+    cva += (1.0 - recovery) * ee * (c.survival(times[i - 1]) - c.survival(times[i]));
+  }
+  return cva;
+}
+
+void ccr_bilateral_cva(const double* t, const double* dee, const double* dene, int32_t size,
+                       const ccr_credit_curve* counterparty, double rc, const ccr_credit_curve* own, double ro,
+                       double* cva, double* dva) {
+  // bilateralCva: src/cva/cva.cpp:34-48 (first-to-default via the other party's survival at midpoints).
+  const Credit c(*counterparty), o(*own);
+  double a = 0.0, b = 0.0;
+  for (int32_t i = 1; i < size; ++i) {
+    const double mid = 0.5 * (t[i - 1] + t[i]);
+    a += (1.0 - rc) * 0.5 * (dee[i - 1] + dee[i]) * o.survival(mid) * (c.survival(t[i - 1]) - c.survival(t[i]));
+    b += (1.0 - ro) * 0.5 * (-dene[i - 1] - dene[i]) * c.survival(mid) * (o.survival(t[i - 1]) - o.survival(t[i]));
+  }
+  if (cva) *cva = a;
+  if (dva) *dva = b;
+}
+
+}  // extern "C"

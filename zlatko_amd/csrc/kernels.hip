@@ -1,0 +1,194 @@
+// kernels.hip -- gfx950 kernels of the exposure path.
+//
+//  ccr_path_kernel<Real>   one Monte Carlo path per lane (wave64) running simulatePath
+//                          (path_body.hpp): the date loop stays in registers, per-date tables
+//                          come through uniform (scalar) loads, per-path look-back state (margin
+//                          ring, fixing states) lives in LDS columns. Reporting-date statistics
+//                          are reduced across the wavefront and written as per-wave partial sums;
+//                          max(V - C, 0) is written path-major-coalesced for the PFE.
+//  ccr_reduce_kernel       fixed-order sum of the per-wave partials (bit-reproducible).
+//  ccr_pfe_kernel<Real>    exact order statistics per (netting set, reporting date) by MSB radix
+//                          select on the IEEE bit patterns of the non-negative exposures, then
+//                          the interpolation of ccr::quantile (include/ccr/core/math.hpp:39-47).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "path_body.hpp"
+
+namespace ccrhip {
+
+__device__ inline int xcdRemap(int b, int nb) {
+  // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous range of logical
+  // workgroups so that the workgroups of one netting set share an L2.
+  const int xcd = b & 7, i = b >> 3, per = nb >> 3, rem = nb & 7;
+  return xcd < rem ? xcd * (per + 1) + i : rem * (per + 1) + (xcd - rem) * per + i;
+}
+
+__device__ inline double waveSum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) ccr_path_kernel(PathArgs<Real> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const GlobalDev& g = *a.g;
+  const int b = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  const SetDev& S = a.sets[a.block_set[b]];
+  const int bd = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
+  const int lb = b - S.block_begin;
+  const int p = lb * bd + tid;
+  const bool active = p < g.num_paths;
+  const int pe = active ? p : g.num_paths - 1;  // inactive lanes replay a valid path, contribute nothing
+  const int waveLocal = lb * (bd >> 6) + (tid >> 6);
+  double* part = a.partials + S.partial_off + static_cast<int64_t>(waveLocal) * a.max_rep * kStats;
+  Real* rows = a.rows + S.rows_off;
+  Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
+  simulatePath<Real>(a, g, S, pe, col, bd, [&](int rr, const double* st9, double vp) {
+    if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
+#pragma unroll
+    for (int k = 0; k < kStats; ++k) {
+      const double s = waveSum(active ? st9[k] : 0.0);
+      if ((tid & 63) == 0) part[rr * kStats + k] = s;
+    }
+  });
+}
+
+// Sum of the per-wave partials of every (set, reporting date, statistic), in wave order.
+__global__ void ccr_reduce_kernel(const SetDev* sets, const double* partials, int nsets, int max_rep, int waves_per_set,
+                                  double inv_paths, double* out) {
+  const int idx = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int per = max_rep * kStats;
+  if (idx >= nsets * per) return;
+  const int s = idx / per, rk = idx % per;
+  const double* p = partials + sets[s].partial_off + rk;
+  double sum = 0.0;
+  for (int w = 0; w < waves_per_set; ++w) sum += p[static_cast<int64_t>(w) * per];
+  out[idx] = sum * inv_paths;
+}
+
+template <typename Real> struct KeyOf;
+template <> struct KeyOf<double> { using T = unsigned long long; static constexpr int kBits = 64; };
+template <> struct KeyOf<float> { using T = unsigned int; static constexpr int kBits = 32; };
+
+template <typename Real>
+__device__ inline typename KeyOf<Real>::T keyOf(Real v) {
+  // Non-negative IEEE values order like their bit patterns; -0 and +0 map to key 0.
+  using K = typename KeyOf<Real>::T;
+  if (!(v > Real(0))) return K(0);
+  if constexpr (std::is_same<Real, double>::value) return static_cast<K>(__double_as_longlong(v));
+  else return static_cast<K>(__float_as_uint(v));
+}
+template <typename Real>
+__device__ inline Real valueOf(typename KeyOf<Real>::T k) {
+  if constexpr (std::is_same<Real, double>::value) return __longlong_as_double(static_cast<long long>(k));
+  else return __uint_as_float(k);
+}
+
+// One workgroup (256 threads) per (set, reporting date) row.
+template <typename Real>
+__global__ void __launch_bounds__(256) ccr_pfe_kernel(const SetDev* sets, const Real* rows, int max_rep, int num_paths,
+                                                      double level, double* out) {
+  using K = typename KeyOf<Real>::T;
+  constexpr int kBits = KeyOf<Real>::kBits;
+  __shared__ unsigned int hist[256];
+  __shared__ K sPrefix;
+  __shared__ long long sRank;
+  __shared__ unsigned int sCount;
+  __shared__ K sMin;
+  const int s = static_cast<int>(blockIdx.x) / max_rep, rr = static_cast<int>(blockIdx.x) % max_rep;
+  const SetDev& S = sets[s];
+  if (rr >= S.num_rep) return;
+  const Real* row = rows + S.rows_off + static_cast<int64_t>(rr) * num_paths;
+  const int tid = static_cast<int>(threadIdx.x);
+  const double h = level * static_cast<double>(num_paths - 1);
+  const long long lo = static_cast<long long>(floor(h));
+  const long long hi = lo + 1 < num_paths ? lo + 1 : num_paths - 1;
+
+  if (tid == 0) {
+    sPrefix = 0;
+    sRank = lo;
+  }
+  __syncthreads();
+  K mask = 0;
+  for (int shift = kBits - 8; shift >= 0; shift -= 8) {
+    hist[tid] = 0;
+    __syncthreads();
+    const K prefix = sPrefix;
+    for (int i = tid; i < num_paths; i += 256) {
+      const K k = keyOf<Real>(row[i]);
+      if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      long long rank = sRank;
+      int bin = 0;
+      for (; bin < 255; ++bin) {
+        if (rank < static_cast<long long>(hist[bin])) break;
+        rank -= hist[bin];
+      }
+      sRank = rank;
+      sPrefix = prefix | (static_cast<K>(bin) << shift);
+    }
+    mask |= static_cast<K>(0xFF) << shift;
+    __syncthreads();
+  }
+  const K kLo = sPrefix;
+  // Upper order statistic: equal to the lower one if enough keys are <= it, else the next key above.
+  if (tid == 0) {
+    sCount = 0;
+    sMin = ~K(0);
+  }
+  __syncthreads();
+  unsigned int cnt = 0;
+  K mn = ~K(0);
+  for (int i = tid; i < num_paths; i += 256) {
+    const K k = keyOf<Real>(row[i]);
+    if (k <= kLo) ++cnt;
+    else if (k < mn) mn = k;
+  }
+  atomicAdd(&sCount, cnt);
+  if constexpr (kBits == 64) atomicMin(reinterpret_cast<unsigned long long*>(&sMin), static_cast<unsigned long long>(mn));
+  else atomicMin(reinterpret_cast<unsigned int*>(&sMin), static_cast<unsigned int>(mn));
+  __syncthreads();
+  if (tid == 0) {
+    const K kHi = (hi == lo || static_cast<long long>(sCount) >= hi + 1) ? kLo : sMin;
+    const double vlo = static_cast<double>(valueOf<Real>(kLo)), vhi = static_cast<double>(valueOf<Real>(kHi));
+    out[s * max_rep + rr] = vlo + (h - static_cast<double>(lo)) * (vhi - vlo);
+  }
+}
+
+// ---------------------------------------------------------------- launchers (host side)
+
+template <typename Real>
+hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
+  const size_t lds = static_cast<size_t>(2 * args.ring + args.slots) * block * sizeof(Real);
+  hipLaunchKernelGGL(ccr_path_kernel<Real>, dim3(blocks), dim3(block), lds, stream, args);
+  return hipGetLastError();
+}
+
+hipError_t launchReduce(const SetDev* sets, const double* partials, int nsets, int max_rep, int waves_per_set,
+                        double inv_paths, double* out, hipStream_t stream) {
+  const int n = nsets * max_rep * kStats;
+  hipLaunchKernelGGL(ccr_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, sets, partials, nsets, max_rep,
+                     waves_per_set, inv_paths, out);
+  return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launchPfe(const SetDev* sets, const Real* rows, int nsets, int max_rep, int num_paths, double level,
+                     double* out, hipStream_t stream) {
+  hipLaunchKernelGGL(ccr_pfe_kernel<Real>, dim3(nsets * max_rep), dim3(256), 0, stream, sets, rows, max_rep, num_paths,
+                     level, out);
+  return hipGetLastError();
+}
+
+template hipError_t launchPath<double>(const PathArgs<double>&, int, int, hipStream_t);
+template hipError_t launchPath<float>(const PathArgs<float>&, int, int, hipStream_t);
+template hipError_t launchPfe<double>(const SetDev*, const double*, int, int, int, double, double*, hipStream_t);
+template hipError_t launchPfe<float>(const SetDev*, const float*, int, int, int, double, double*, hipStream_t);
+
+}  // namespace ccrhip

@@ -1,0 +1,86 @@
+// layout.hpp -- the device-side layout of a compiled exposure plan, shared by the host plan
+// builder and the gfx950 kernels.
+//
+// HBM layout (one plan, S netting sets, P paths, nz normals per step):
+//   stream   Real[ceil(P/2) * Lmax]    the reference normal stream in draw order: path block q
+//                                      (paths 2q, 2q+1 when antithetic) owns (T_s - 1) * nz
+//                                      consecutive normals of netting set s, step-major
+//   steps    Real[sum_s (T_s) * kStepStride]  per-date model coefficients (row j: step t_{j-1} -> t_j)
+//   dates    DateRec[sum_s T_s] + Real[sum_s T_s * kDateRealStride]  per-date valuation records
+//   bonds    Real2[...]   (c A(t,T), B(t,T)) merged per (rate, maturity) and date
+//   fixed    Real4[...]   (c R A(t,te), B(t,te), B(tk,ts) - B(tk,te), slot) per fixed floating coupon
+//   rows     Real[S][R_max][P]        max(V - C, 0) on the reporting dates (PFE input)
+//   partials double[S][waves][R_max][kStats]  per-wavefront sums, reduced in a fixed order
+#pragma once
+
+#include <cstdint>
+
+#include "../../include/ccr_hip.h"
+
+namespace ccrhip {
+
+constexpr int kMaxRates = 4;      // domestic + up to 3 foreign Hull-White rates
+constexpr int kMaxAssets = 4;     // lognormal FX / equity factors
+constexpr int kMaxCredits = 2;    // CIR++ intensities (counterparty, own)
+constexpr int kMaxCorr = kMaxRates + kMaxAssets + kMaxCredits;
+constexpr int kMaxSub = 8;        // CIR++ sub-steps per grid step
+constexpr int kStats = 9;         // E[V-C], EE, ENE, EE*, ENE*, E[D], E[Q], CVA incr., DVA incr.
+
+// Step record (Real[kStepStride]) for the step t_{j-1} -> t_j (row 0 unused):
+//   rate r:   8 r + {e, sdX, c1, c2, (1-e)/a, drift (1-e)/a, drift (dt-(1-e)/a)/a, int phi}
+//   asset a:  32 + 3 a + {int carry, vol sqrt(dt), vol^2 dt / 2}
+//   credit c: 44 + 20 c + {sqrt(dt), h, shift(t_j), bridge mean k (8), bridge sd k (8), pad}
+constexpr int kStepRate = 0;
+constexpr int kStepAsset = 32;
+constexpr int kStepCredit = 44;
+constexpr int kStepCreditStride = 20;
+constexpr int kStepStride = 84;
+
+// Per-date valuation record.
+struct DateRec {
+  int32_t flags;        // kValued | kReporting | kStoreValue
+  int32_t rep;          // reporting index (or -1)
+  int32_t call_pos;     // CSA: ring position of the margin-call date's value
+  int32_t store_pos;    // CSA: ring position this date's value is stored at (or -1)
+  int32_t bond_beg[kMaxRates + 1];   // bond terms of rate r: [bond_beg[r], bond_beg[r + 1])
+  int32_t fix_beg[kMaxRates + 1];    // fixed-coupon terms of rate r
+  int32_t fixstore_beg, fixstore_end;  // (rate, slot) pairs whose state is recorded at this date
+};
+static_assert(sizeof(DateRec) == 64, "DateRec must stay 64 bytes");
+constexpr int32_t kValued = 1, kReporting = 2, kStoreValue = 4;
+
+// Per-date reals: c0[rate] (constant bond terms), lin[asset] (deterministic-carry forwards),
+// collateral bond factor (A, B).
+constexpr int kDateC0 = 0;
+constexpr int kDateLin = kMaxRates;
+constexpr int kDateBond = kMaxRates + kMaxAssets;
+constexpr int kDateRealStride = 12;
+
+struct SetDev {
+  int64_t stream_stride;   // normals per path block: (T - 1) * nz
+  int64_t rows_off;        // first element of this set's exposure rows
+  int64_t partial_off;     // first element of this set's wave partials
+  int32_t num_dates, num_rep;
+  int32_t date_off;        // into the DateRec / date-real / step arrays
+  int32_t bond_off, fixed_off, fixstore_off;
+  int32_t block_begin, num_blocks, wave_begin;
+  int32_t ring_len, has_csa, sec_type, csa_asset;
+  int32_t cpty, own, num_slots;
+  double thr_cpty, thr_own, mta, ia, keep;  // CSA terms (thr_own < 0 encodes a one-way CSA)
+};
+
+struct GlobalDev {
+  int32_t num_paths, antithetic, nz, ncorr;
+  int32_t nrates, nassets, ncredits, nsets;
+  int32_t fx_of[kMaxRates];        // rate r >= 1: asset converting it (-1: none)
+  int32_t asset_rate[kMaxAssets];  // asset a: foreign rate group (1..), 0 = deterministic carry
+  int32_t credit_off[kMaxCredits]; // first bridge normal of credit c
+  int32_t credit_sub[kMaxCredits];
+  int32_t foreign_off;             // first foreign-integral normal
+  int32_t max_ring, max_slots, pad;
+  double spot[kMaxAssets];
+  double cir_kappa[kMaxCredits], cir_theta[kMaxCredits], cir_xi[kMaxCredits], cir_y0[kMaxCredits];
+  double chol[kMaxCorr * kMaxCorr];
+};
+
+}  // namespace ccrhip

@@ -1,0 +1,260 @@
+// path_body.hpp -- the per-path body of the fused exposure kernel, one Monte Carlo path.
+//
+// A __host__ __device__ template: ccr_path_kernel runs it with one path per lane on gfx950;
+// the test-only CPU emulator runs the same code path by path to pin the plan tables against the
+// FP64 oracle without a GPU. For one path, over the simulation dates:
+//   replayed normals (sign-flipped for the antithetic mirror, scenario_generator.cpp:94-100)
+//   -> w = L z (scenario_generator.cpp:118-124)
+//   -> exact Hull-White steps of every rate (hull_white.cpp:40-68; foreign rates with the quanto
+//      drift), deflator D *= exp(-int r) (scenario_generator.cpp:126-128)
+//   -> lognormal assets (lognormal_asset.cpp:17-22), carry from a curve or a foreign rate
+//   -> CIR++ with Brownian-bridge sub-steps and pathwise survival (cir_intensity.cpp:45-66)
+//   -> on valued dates V = sum over rate groups of FX x (c0 + sum c A e^{-B x} + fixed coupons)
+//   -> CSA (collateral.cpp:10-47) with the margin-call value read from a per-path ring
+//   -> on reporting dates, the 9 statistics of ExposureProfile::compute and pathwiseCva.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <type_traits>
+
+#include "launch.hpp"
+
+namespace ccrhip {
+
+template <typename Real>
+__host__ __device__ inline Real expr(Real v) {
+  if constexpr (std::is_same<Real, double>::value) return exp(v);
+  else return __expf(v);
+}
+template <typename Real>
+__host__ __device__ inline Real sqrtr(Real v) {
+  if constexpr (std::is_same<Real, double>::value) return sqrt(v);
+  else return __fsqrt_rn(v);
+}
+
+// col: this path's column of per-path scratch (LDS on the GPU), stride: distance between rows.
+// emit(rep, stats[kStats], positive_exposure) is called on every reporting date.
+template <typename Real, typename Emit>
+__host__ __device__ inline void simulatePath(const PathArgs<Real>& a, const GlobalDev& g, const SetDev& S, int pe,
+                                             Real* col, int stride, Emit&& emit) {
+  constexpr bool kDouble = std::is_same<Real, double>::value;
+  Real* ringV = col;
+  Real* ringP = col + a.ring * stride;
+  Real* xs = col + 2 * a.ring * stride;
+  const int nR = g.nrates, nA = g.nassets, nC = g.ncredits, nz = g.nz, nCorr = g.ncorr;
+  const int64_t q = g.antithetic ? (pe >> 1) : pe;
+  const Real sgn = (g.antithetic && (pe & 1)) ? Real(-1) : Real(1);
+  const Real* zbase = a.stream + q * S.stream_stride;
+
+  Real x[kMaxRates], S_[kMaxAssets], y[kMaxCredits], iy[kMaxCredits], Qs[kMaxCredits];
+  Real lnD = 0, lnDc = 0, D = 1;  // FP32: Kahan sum of int r; FP64: running product
+  Real lnS[kMaxAssets], lnSc[kMaxAssets];
+#pragma unroll
+  for (int r = 0; r < kMaxRates; ++r) x[r] = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxAssets; ++i) {
+    S_[i] = static_cast<Real>(g.spot[i]);
+    lnS[i] = 0;
+    lnSc[i] = 0;
+  }
+#pragma unroll
+  for (int c = 0; c < kMaxCredits; ++c) {
+    y[c] = static_cast<Real>(g.cir_y0[c]);
+    iy[c] = 0;
+    Qs[c] = 1;
+  }
+  Real units = 0;
+  double prevDvp = 0, prevDvn = 0, prevQc = 1, prevQo = 1;
+
+  const DateRec* dates = a.dates + S.date_off;
+  const Real* dreal = a.date_reals + static_cast<int64_t>(S.date_off) * kDateRealStride;
+  const Real* bonds = a.bonds + 2 * static_cast<int64_t>(S.bond_off);
+  const Real* fixed = a.fixed + 4 * static_cast<int64_t>(S.fixed_off);
+  const int32_t* fstore = a.fixstore + 2 * S.fixstore_off;
+
+  for (int j = 0; j < S.num_dates; ++j) {
+    if (j > 0) {
+      const Real* st = a.steps + static_cast<int64_t>(S.date_off + j) * kStepStride;
+      const Real* z = zbase + static_cast<int64_t>(j - 1) * nz;
+      Real w[kMaxCorr];
+      {
+        Real zc[kMaxCorr];
+#pragma unroll
+        for (int i = 0; i < kMaxCorr; ++i) zc[i] = i < nCorr ? sgn * z[i] : Real(0);
+#pragma unroll
+        for (int i = 0; i < kMaxCorr; ++i) {
+          Real sum = 0;
+#pragma unroll
+          for (int k = 0; k <= i; ++k) sum += static_cast<Real>(g.chol[i * kMaxCorr + k]) * zc[k];
+          w[i] = sum;
+        }
+      }
+      Real ir[kMaxRates];
+#pragma unroll
+      for (int r = 0; r < kMaxRates; ++r) {
+        if (r < nR) {
+          const Real* o = st + kStepRate + 8 * r;
+          const Real z2 = sgn * (r == 0 ? z[nCorr] : z[g.foreign_off + r - 1]);
+          const Real x0 = x[r];
+          x[r] = x0 * o[0] + o[1] * w[r] + o[5];
+          const Real ix = x0 * o[4] + o[2] * w[r] + o[3] * z2 + o[6];
+          ir[r] = ix + o[7];
+        } else {
+          ir[r] = 0;
+        }
+      }
+      if constexpr (kDouble) {
+        D *= exp(-ir[0]);
+      } else {
+        const Real yk = ir[0] - lnDc;  // Kahan-compensated integral of the short rate
+        const Real tk = lnD + yk;
+        lnDc = (tk - lnD) - yk;
+        lnD = tk;
+        D = expr(-lnD);
+      }
+#pragma unroll
+      for (int i = 0; i < kMaxAssets; ++i) {
+        if (i < nA) {
+          const Real* o = st + kStepAsset + 3 * i;
+          const int rr = g.asset_rate[i];
+          Real icarry = o[0];
+#pragma unroll
+          for (int r = 1; r < kMaxRates; ++r)
+            if (r == rr) icarry = ir[r];
+          const Real drift = ir[0] - icarry - o[2] + o[1] * w[nR + i];
+          if constexpr (kDouble) {
+            S_[i] = S_[i] * exp(drift);
+          } else {
+            const Real yk = drift - lnSc[i];
+            const Real tk = lnS[i] + yk;
+            lnSc[i] = (tk - lnS[i]) - yk;
+            lnS[i] = tk;
+            S_[i] = static_cast<Real>(g.spot[i]) * expr(lnS[i]);
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < kMaxCredits; ++c) {
+        if (c < nC) {
+          const Real* o = st + kStepCredit + kStepCreditStride * c;
+          const int sub = g.credit_sub[c];
+          const Real* ex = z + g.credit_off[c];
+          const Real kappa = static_cast<Real>(g.cir_kappa[c]), theta = static_cast<Real>(g.cir_theta[c]);
+          const Real xi = static_cast<Real>(g.cir_xi[c]), h = o[1];
+          Real remaining = o[0] * w[nR + nA + c];
+          Real yc = y[c], iyc = iy[c];
+          for (int k = 0; k < sub; ++k) {
+            Real dW = remaining;
+            if (k + 1 < sub) dW = o[3 + k] * remaining + o[3 + kMaxSub + k] * (sgn * ex[k]);
+            remaining -= dW;
+            const Real yp = yc > Real(0) ? yc : Real(0);
+            const Real yn = yc + kappa * (theta - yp) * h + xi * sqrtr(yp) * dW;
+            iyc += Real(0.5) * (yp + (yn > Real(0) ? yn : Real(0))) * h;
+            yc = yn;
+          }
+          y[c] = yc;
+          iy[c] = iyc;
+          Qs[c] = expr(-iyc - o[2]);
+        }
+      }
+    }
+
+    const DateRec rec = dates[j];
+    for (int i = rec.fixstore_beg; i < rec.fixstore_end; ++i) {
+      const int r = fstore[2 * i], slot = fstore[2 * i + 1];
+      Real xv = x[0];
+#pragma unroll
+      for (int k = 1; k < kMaxRates; ++k)
+        if (k == r) xv = x[k];
+      xs[slot * stride] = xv;
+    }
+    if (!(rec.flags & kValued) && !S.has_csa) continue;
+    const Real* dr = dreal + static_cast<int64_t>(j) * kDateRealStride;
+    Real V = 0;
+    if (rec.flags & kValued) {
+#pragma unroll
+      for (int r = 0; r < kMaxRates; ++r) {
+        if (r < nR) {
+          const Real xr = x[r];
+          Real acc = dr[kDateC0 + r];
+          for (int i = rec.bond_beg[r]; i < rec.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expr(-bonds[2 * i + 1] * xr);
+          for (int i = rec.fix_beg[r]; i < rec.fix_beg[r + 1]; ++i) {
+            const Real* f = fixed + 4 * static_cast<int64_t>(i);
+            const Real xk = xs[static_cast<int>(f[3]) * stride];
+            acc += f[0] * expr(-f[1] * xr - f[2] * xk);
+          }
+          Real mult = 1;
+          if (r > 0) {
+            const int fa = g.fx_of[r];
+#pragma unroll
+            for (int i = 0; i < kMaxAssets; ++i)
+              if (i == fa) mult = S_[i];
+          }
+          V += mult * acc;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kMaxAssets; ++i)
+        if (i < nA) V += S_[i] * dr[kDateLin + i];
+    }
+    Real C = 0;
+    if (S.has_csa) {
+      Real price = 1;
+      if (S.sec_type == CCR_COLLATERAL_BOND) {
+        price = dr[kDateBond] * expr(-dr[kDateBond + 1] * x[0]);
+      } else if (S.sec_type == CCR_COLLATERAL_ASSET) {
+#pragma unroll
+        for (int i = 0; i < kMaxAssets; ++i)
+          if (i == S.csa_asset) price = S_[i];
+      }
+      if (rec.store_pos >= 0) {
+        ringV[rec.store_pos * stride] = V;
+        ringP[rec.store_pos * stride] = price;
+      }
+      const Real vc = ringV[rec.call_pos * stride];
+      const Real pc = ringP[rec.call_pos * stride];
+      const Real keep = static_cast<Real>(S.keep), thc = static_cast<Real>(S.thr_cpty);
+      const Real tho = static_cast<Real>(S.thr_own);
+      Real target = 0;
+      if (vc > thc) target += vc - thc;
+      if (S.thr_own >= 0.0 && -vc > tho) target -= -vc - tho;
+      if (j == 0) units = target / (keep * pc);
+      Real diff = target - units * keep * pc;
+      diff = diff < Real(0) ? -diff : diff;
+      if (diff >= static_cast<Real>(S.mta)) units = target / (keep * pc);
+      C = units * price + static_cast<Real>(S.ia);
+    }
+    if (!(rec.flags & kReporting)) continue;
+    const double v = static_cast<double>(V) - static_cast<double>(C);
+    const double d = static_cast<double>(D);
+    const double vp = v > 0.0 ? v : 0.0, vn = v < 0.0 ? v : 0.0;
+    const double dvp = d * vp, dvn = d * (-vn);
+    double qc = 1.0, qo = 1.0;
+#pragma unroll
+    for (int c = 0; c < kMaxCredits; ++c) {
+      if (c == S.cpty) qc = static_cast<double>(Qs[c]);
+      if (c == S.own) qo = static_cast<double>(Qs[c]);
+    }
+    const int rr = rec.rep;
+    double st9[kStats];
+    st9[0] = v;
+    st9[1] = vp;
+    st9[2] = vn;
+    st9[3] = dvp;
+    st9[4] = d * vn;
+    st9[5] = d;
+    st9[6] = qc;
+    st9[7] = rr > 0 ? 0.5 * (prevDvp + dvp) * (prevQc - qc) : 0.0;
+    st9[8] = rr > 0 ? 0.5 * (prevDvn + dvn) * (prevQo - qo) : 0.0;
+    prevDvp = dvp;
+    prevDvn = dvn;
+    prevQc = qc;
+    prevQo = qo;
+    emit(rr, st9, vp);
+  }
+}
+
+}  // namespace ccrhip

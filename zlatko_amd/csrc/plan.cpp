@@ -1,0 +1,540 @@
+// plan.cpp -- host plan builder (FP64). See plan.hpp and layout.hpp.
+//
+// Valuation terms. On date t every domestic or foreign swap and every forward is a sum of
+// zero-coupon bonds P_r(t, T | x_r) = A_r(t, T) exp(-B_r(t, T) x_r) of its rate r, times the FX
+// spot of r for foreign rates (interest_rate_swap.cpp:47-78, asset_forward.cpp:13-25):
+//   unfixed period (ts >= t):   s N (P(t,ts) - P(t,te) - K tau P(t,te))
+//   fixed period  (ts <  t):    s N (P(tk,ts|x_k) / P(tk,te|x_k) - 1 - K tau) P(t,te)
+//                             = s N R A(t,te) exp(-B(t,te) x_t - dB x_k) - s N (1 + K tau) P(t,te)
+//   forward:                    N (S Q(t,T) - K P(t,T))
+// with k = indexAtOrBefore(ts), R = A(tk,ts)/A(tk,te) and dB = B(tk,ts) - B(tk,te) (A = 1, B = 0
+// where the reference's zeroBond returns 1 for T <= t). Terms of one date are merged across the
+// netting set per (rate, maturity) and per (rate, fixing date, ts, te), so the kernel evaluates one
+// exponential per distinct cash-flow date instead of 2-3 zero bonds per period per trade. The
+// x_k a fixed coupon needs is kept per path in a slot recorded on the fixing date; slots are
+// reused once their last coupon is paid (interval colouring).
+#include "plan.hpp"
+
+#include <cstring>
+#include <map>
+#include <random>
+#include <tuple>
+
+namespace ccrhip {
+
+namespace {
+
+struct Trade {
+  int kind = CCR_TRADE_SWAP;
+  int rate = 0;  // rate group of the swap (0 domestic, f + 1 foreign)
+  double sign = 1.0, notional = 0.0, K = 0.0, maturity = 0.0;
+  std::vector<double> sched, fsched;
+  int asset = 0;
+};
+
+void checkSchedule(const std::vector<double>& s) {
+  if (s.size() < 2) invalid("InterestRateSwap: schedule needs at least two dates");
+  if (s.front() < 0.0) invalid("InterestRateSwap: schedule must start at t >= 0");
+  for (std::size_t i = 1; i < s.size(); ++i)
+    if (s[i] <= s[i - 1]) invalid("InterestRateSwap: schedule must be strictly increasing");
+}
+
+std::vector<double> eventTimes(const ccr_trade& d) {
+  std::vector<double> e;
+  if (d.kind == CCR_TRADE_ASSET_FORWARD) return {d.maturity};
+  if (d.schedule && d.schedule_size > 0) e.assign(d.schedule, d.schedule + d.schedule_size);
+  if (d.float_schedule && d.float_schedule_size > 0)
+    e.insert(e.end(), d.float_schedule, d.float_schedule + d.float_schedule_size);
+  if (d.kind == CCR_TRADE_BERMUDAN_SWAPTION && d.exercise_times && d.num_exercises > 0)
+    e.insert(e.end(), d.exercise_times, d.exercise_times + d.num_exercises);
+  return e;
+}
+
+struct Models {
+  std::vector<HullWhite> rates;  // [0] domestic, [f + 1] foreign f
+  std::vector<Curve> carry;      // deterministic carry per asset
+  std::vector<int> assetRate;    // 0 = deterministic carry, f + 1 = foreign rate f
+  std::vector<double> vol, spot;
+  std::vector<CirModel> credits;
+  std::vector<int> fxOf;         // per rate group: converting asset (-1 none; [0] unused)
+  std::vector<double> quanto;    // per rate group: drift of x under the domestic measure
+  std::vector<double> chol;
+  int ncorr = 0, nz = 0, foreignOff = 0;
+  std::vector<int> creditOff;
+};
+
+Models makeModels(const ccr_scenario_generator& g) {
+  Models m;
+  if (g.num_foreign_rates < 0 || g.num_assets < 0 || g.num_credits < 0) invalid("ScenarioGenerator: negative count");
+  if (g.num_foreign_rates + 1 > kMaxRates || g.num_assets > kMaxAssets || g.num_credits > kMaxCredits)
+    fail(CCR_ERR_UNSUPPORTED, "ScenarioGenerator: at most 3 foreign rates, 4 assets and 2 credits");
+  m.rates.emplace_back(g.rates);
+  for (int f = 0; f < g.num_foreign_rates; ++f) m.rates.emplace_back(g.foreign_rates[f]);
+  const int F = g.num_foreign_rates;
+  for (int a = 0; a < g.num_assets; ++a) {
+    const ccr_lognormal_asset& as = g.assets[a];
+    if (as.spot <= 0.0) invalid("LognormalAsset: spot must be positive");
+    if (as.volatility < 0.0) invalid("LognormalAsset: volatility must be non-negative");
+    if (as.foreign_rate >= F) invalid("LognormalAsset: unknown foreign rate");
+    m.spot.push_back(as.spot);
+    m.vol.push_back(as.volatility);
+    m.assetRate.push_back(as.foreign_rate < 0 ? 0 : as.foreign_rate + 1);
+    m.carry.push_back(as.foreign_rate < 0 ? Curve(as.carry) : Curve());
+  }
+  for (int c = 0; c < g.num_credits; ++c) {
+    m.credits.emplace_back(g.credits[c]);
+    if (m.credits.back().sub > kMaxSub) fail(CCR_ERR_UNSUPPORTED, "CirIntensity: at most 8 substeps");
+  }
+  const int n = 1 + F + g.num_assets + g.num_credits;
+  m.ncorr = n;
+  std::vector<double> corr(static_cast<std::size_t>(n * n), 0.0);
+  if (g.correlation) {
+    for (int i = 0; i < n * n; ++i) corr[static_cast<std::size_t>(i)] = g.correlation[i];
+  } else {
+    for (int i = 0; i < n; ++i) corr[static_cast<std::size_t>(i * n + i)] = 1.0;
+  }
+  for (int i = 0; i < n; ++i)
+    if (std::fabs(corr[static_cast<std::size_t>(i * n + i)] - 1.0) > 1e-12)
+      invalid("ScenarioGenerator: correlation matrix must have unit diagonal");
+  m.chol = cholesky(corr, static_cast<std::size_t>(n));
+  m.nz = n + 1;
+  for (int c = 0; c < g.num_credits; ++c) {
+    m.creditOff.push_back(m.nz);
+    m.nz += m.credits[static_cast<std::size_t>(c)].sub - 1;
+  }
+  m.foreignOff = m.nz;
+  m.nz += F;
+  m.fxOf.assign(static_cast<std::size_t>(F + 1), -1);
+  m.quanto.assign(static_cast<std::size_t>(F + 1), 0.0);
+  for (int a = 0; a < g.num_assets; ++a) {
+    const int r = m.assetRate[static_cast<std::size_t>(a)];
+    if (r > 0 && m.fxOf[static_cast<std::size_t>(r)] < 0) m.fxOf[static_cast<std::size_t>(r)] = a;
+  }
+  for (int r = 1; r <= F; ++r) {
+    const int a = m.fxOf[static_cast<std::size_t>(r)];
+    if (a >= 0)
+      m.quanto[static_cast<std::size_t>(r)] =
+          -corr[static_cast<std::size_t>(r * n + (1 + F + a))] * m.rates[static_cast<std::size_t>(r)].s *
+          m.vol[static_cast<std::size_t>(a)];
+  }
+  return m;
+}
+
+Trade makeTrade(const ccr_trade& d, const Models& m) {
+  Trade t;
+  t.kind = d.kind;
+  if (d.kind == CCR_TRADE_SWAP) {
+    if (!d.schedule || d.schedule_size < 2) invalid("InterestRateSwap: schedule needs at least two dates");
+    t.sched.assign(d.schedule, d.schedule + d.schedule_size);
+    checkSchedule(t.sched);
+    if (d.float_schedule && d.float_schedule_size > 0) {
+      t.fsched.assign(d.float_schedule, d.float_schedule + d.float_schedule_size);
+      checkSchedule(t.fsched);
+    }
+    t.sign = d.pay_fixed ? 1.0 : -1.0;
+    t.notional = d.notional;
+    t.K = d.fixed_rate;
+    t.maturity = std::max(t.sched.back(), t.fsched.empty() ? t.sched.back() : t.fsched.back());
+    if (d.currency < 0 || d.currency >= static_cast<int>(m.rates.size()))
+      fail(CCR_ERR_OUT_OF_RANGE, "InterestRateSwap: unknown currency");
+    t.rate = d.currency;
+    if (t.rate > 0 && m.fxOf[static_cast<std::size_t>(t.rate)] < 0)
+      invalid("InterestRateSwap: no FX asset converts the swap currency");
+  } else if (d.kind == CCR_TRADE_ASSET_FORWARD) {
+    if (d.maturity <= 0.0) invalid("AssetForward: maturity must be positive");
+    if (d.asset < 0 || d.asset >= static_cast<int>(m.spot.size()))
+      fail(CCR_ERR_OUT_OF_RANGE, "ScenarioSet: unknown asset");
+    t.asset = d.asset;
+    t.notional = d.notional;
+    t.K = d.fixed_rate;
+    t.maturity = d.maturity;
+    const int r = m.assetRate[static_cast<std::size_t>(d.asset)];
+    if (r > 0 && m.fxOf[static_cast<std::size_t>(r)] != d.asset)
+      fail(CCR_ERR_UNSUPPORTED, "AssetForward: two assets convert the same foreign rate");
+  } else if (d.kind == CCR_TRADE_BERMUDAN_SWAPTION) {
+    fail(CCR_ERR_UNSUPPORTED, "BermudanSwaption: run it through the AMC entry point");
+  } else {
+    invalid("unknown trade kind");
+  }
+  return t;
+}
+
+// Terms of one date before A/B are attached.
+struct Collector {
+  std::map<std::pair<int, double>, double> bond;  // (rate, T) -> coefficient
+  std::map<std::tuple<int, int, double, double>, double> fix;  // (rate, k, ts, te) -> coefficient
+  double c0[kMaxRates] = {0, 0, 0, 0};
+  double lin[kMaxAssets] = {0, 0, 0, 0};
+  double t = 0.0;
+  void addBond(int r, double T, double c) {
+    if (T <= t) {
+      c0[r] += c;  // HullWhite1F::zeroBond returns 1 for T <= t
+    } else {
+      bond[{r, T}] += c;
+    }
+  }
+};
+
+void collectSwapPeriods(const Trade& tr, const TimeGrid& grid, Collector& col) {
+  const double t = col.t, sN = tr.sign * tr.notional;
+  const int r = tr.rate;
+  auto floating = [&](double ts, double te, bool withFixed) {
+    const double tau = te - ts;
+    const double fixedPart = withFixed ? tr.K * tau : 0.0;
+    if (ts >= t - kPaidTolerance) {
+      col.addBond(r, ts, sN);
+      col.addBond(r, te, -sN * (1.0 + fixedPart));
+    } else {
+      const int k = static_cast<int>(grid.indexAtOrBefore(ts));
+      col.fix[{r, k, ts, te}] += sN;
+      col.addBond(r, te, -sN * (1.0 + fixedPart));
+    }
+  };
+  if (tr.fsched.empty()) {
+    for (std::size_t i = 1; i < tr.sched.size(); ++i) {
+      const double ts = tr.sched[i - 1], te = tr.sched[i];
+      if (te <= t + kPaidTolerance) continue;
+      floating(ts, te, true);
+    }
+    return;
+  }
+  for (std::size_t i = 1; i < tr.fsched.size(); ++i) {
+    const double ts = tr.fsched[i - 1], te = tr.fsched[i];
+    if (te <= t + kPaidTolerance) continue;
+    floating(ts, te, false);
+  }
+  for (std::size_t i = 1; i < tr.sched.size(); ++i) {
+    const double ts = tr.sched[i - 1], te = tr.sched[i];
+    if (te <= t + kPaidTolerance) continue;
+    col.addBond(r, te, -sN * tr.K * (te - ts));
+  }
+}
+
+void collect(const Trade& tr, const Models& m, const TimeGrid& grid, Collector& col) {
+  const double t = col.t;
+  if (tr.kind == CCR_TRADE_SWAP) {
+    if (t >= tr.maturity - kPaidTolerance) return;
+    collectSwapPeriods(tr, grid, col);
+    return;
+  }
+  // AssetForward: N S Q(t,T) - N K P(t,T).
+  if (t >= tr.maturity - kPaidTolerance) return;
+  col.addBond(0, tr.maturity, -tr.notional * tr.K);
+  const int r = m.assetRate[static_cast<std::size_t>(tr.asset)];
+  if (r == 0) {
+    const Curve& c = m.carry[static_cast<std::size_t>(tr.asset)];
+    col.lin[tr.asset] += tr.notional * (c.discount(tr.maturity) / c.discount(t));
+  } else {
+    col.addBond(r, tr.maturity, tr.notional);
+  }
+}
+
+}  // namespace
+
+TimeGrid reportingGrid(const TimeGrid& base, const ccr_netting_set& set) {
+  std::vector<double> ev;
+  for (int i = 0; i < set.num_trades; ++i) {
+    const auto e = eventTimes(set.trades[i]);
+    ev.insert(ev.end(), e.begin(), e.end());
+  }
+  return base.merged(ev);
+}
+
+TimeGrid simulationGrid(const TimeGrid& base, const ccr_netting_set& set) {
+  TimeGrid g = reportingGrid(base, set);
+  if (set.has_collateral) g = g.lagged(set.collateral.margin_period_of_risk);
+  return g;
+}
+
+void generateStream(uint64_t seed, int64_t n, double* out) {
+  std::mt19937_64 engine(seed);
+  std::normal_distribution<double> normal(0.0, 1.0);
+  for (int64_t i = 0; i < n; ++i) out[i] = normal(engine);
+}
+
+HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
+                   int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
+                   int32_t precision, int32_t block) {
+  if (!sets || num_sets <= 0) invalid("no netting sets");
+  if (cfg.num_paths <= 0) invalid("ScenarioGenerator::generate: numPaths must be positive");
+  if (cfg.num_paths > (int64_t(1) << 31) - 1) fail(CCR_ERR_UNSUPPORTED, "at most 2^31 - 1 paths");
+  if (pfe_quantile < 0.0 || pfe_quantile > 1.0) invalid("quantile: level must be in [0, 1]");
+  if (block <= 0 || block % 64 != 0 || block > 1024) invalid("block size must be a multiple of 64 up to 1024");
+  const Models m = makeModels(gen);
+  const TimeGrid base(std::vector<double>(base_grid, base_grid + std::max(0, base_size)));
+  HostPlan plan;
+  plan.precision = precision;
+  plan.pfe_q = pfe_quantile;
+  plan.block = block;
+  GlobalDev& g = plan.g;
+  const int P = static_cast<int>(cfg.num_paths);
+  g.num_paths = P;
+  g.antithetic = cfg.antithetic ? 1 : 0;
+  g.nz = m.nz;
+  g.ncorr = m.ncorr;
+  g.nrates = static_cast<int>(m.rates.size());
+  g.nassets = static_cast<int>(m.spot.size());
+  g.ncredits = static_cast<int>(m.credits.size());
+  g.nsets = num_sets;
+  for (int r = 0; r < kMaxRates; ++r) g.fx_of[r] = r < g.nrates ? m.fxOf[static_cast<std::size_t>(r)] : -1;
+  for (int a = 0; a < kMaxAssets; ++a) {
+    g.asset_rate[a] = a < g.nassets ? m.assetRate[static_cast<std::size_t>(a)] : 0;
+    g.spot[a] = a < g.nassets ? m.spot[static_cast<std::size_t>(a)] : 0.0;
+  }
+  for (int c = 0; c < kMaxCredits; ++c) {
+    const bool on = c < g.ncredits;
+    g.credit_off[c] = on ? m.creditOff[static_cast<std::size_t>(c)] : 0;
+    g.credit_sub[c] = on ? m.credits[static_cast<std::size_t>(c)].sub : 1;
+    g.cir_kappa[c] = on ? m.credits[static_cast<std::size_t>(c)].kappa : 0.0;
+    g.cir_theta[c] = on ? m.credits[static_cast<std::size_t>(c)].theta : 0.0;
+    g.cir_xi[c] = on ? m.credits[static_cast<std::size_t>(c)].xi : 0.0;
+    g.cir_y0[c] = on ? m.credits[static_cast<std::size_t>(c)].y0 : 0.0;
+  }
+  g.foreign_off = m.foreignOff;
+  for (int i = 0; i < m.ncorr; ++i)
+    for (int k = 0; k < m.ncorr; ++k)
+      g.chol[i * kMaxCorr + k] = m.chol[static_cast<std::size_t>(i * m.ncorr + k)];
+
+  const int wavesPerBlock = block / 64;
+  const int blocksPerSet = (P + block - 1) / block;
+  int64_t maxStride = 0;
+  for (int s = 0; s < num_sets; ++s) {
+    const ccr_netting_set& ns = sets[s];
+    if (ns.num_trades <= 0 || !ns.trades) invalid("ExposureEngine::run: empty netting set");
+    if (ns.counterparty_credit >= g.ncredits || ns.own_credit >= g.ncredits)
+      fail(CCR_ERR_OUT_OF_RANGE, "credit index out of range");
+    std::vector<Trade> trades;
+    for (int i = 0; i < ns.num_trades; ++i) trades.push_back(makeTrade(ns.trades[i], m));
+    const TimeGrid rep = reportingGrid(base, ns);
+    const TimeGrid sim = simulationGrid(base, ns);
+    const int T = static_cast<int>(sim.size()), R = static_cast<int>(rep.size());
+    plan.rep_times.push_back(rep.t);
+
+    SetDev sd{};
+    sd.num_dates = T;
+    sd.num_rep = R;
+    sd.date_off = static_cast<int32_t>(plan.dates.size());
+    sd.bond_off = static_cast<int32_t>(plan.bonds.size() / 2);
+    sd.fixed_off = static_cast<int32_t>(plan.fixed.size() / 4);
+    sd.fixstore_off = static_cast<int32_t>(plan.fixstore.size() / 2);
+    sd.stream_stride = static_cast<int64_t>(T - 1) * m.nz;
+    maxStride = std::max(maxStride, sd.stream_stride);
+    sd.block_begin = s * blocksPerSet;
+    sd.num_blocks = blocksPerSet;
+    sd.wave_begin = s * blocksPerSet * wavesPerBlock;
+    sd.cpty = ns.counterparty_credit;
+    sd.own = ns.own_credit;
+
+    // Reporting slots and margin-call look-back (collateral.cpp:23-24).
+    std::vector<int> repOf(static_cast<std::size_t>(T), -1);
+    for (int r = 0; r < R; ++r) {
+      const auto j = sim.find(rep[static_cast<std::size_t>(r)]);
+      if (!j) fail(CCR_ERR_RUNTIME, "reporting date missing from the simulation grid");
+      repOf[*j] = r;
+    }
+    std::vector<int> call(static_cast<std::size_t>(T), 0);
+    std::vector<char> isCall(static_cast<std::size_t>(T), 0);
+    int ring = 1;
+    if (ns.has_collateral) {
+      const ccr_collateral& c = ns.collateral;
+      if (c.threshold_counterparty < 0.0 || c.threshold_own < 0.0 || c.minimum_transfer_amount < 0.0 ||
+          c.margin_period_of_risk < 0.0 || c.haircut < 0.0 || c.haircut >= 1.0)
+        invalid("simulateCollateral: invalid collateral agreement");
+      if (c.type == CCR_COLLATERAL_BOND && c.bond_maturity < sim.t.back() - 1e-10)
+        invalid("the collateral bond must mature after the horizon");
+      if (c.type == CCR_COLLATERAL_ASSET && (c.asset_index < 0 || c.asset_index >= g.nassets))
+        fail(CCR_ERR_OUT_OF_RANGE, "collateral asset index out of range");
+      for (int j = 0; j < T; ++j) {
+        call[static_cast<std::size_t>(j)] =
+            static_cast<int>(sim.indexAtOrBefore(std::max(sim[static_cast<std::size_t>(j)] - c.margin_period_of_risk, 0.0)));
+        isCall[static_cast<std::size_t>(call[static_cast<std::size_t>(j)])] = 1;
+        ring = std::max(ring, j - call[static_cast<std::size_t>(j)] + 1);
+      }
+      sd.has_csa = 1;
+      sd.sec_type = c.type;
+      sd.csa_asset = c.asset_index;
+      sd.thr_cpty = c.threshold_counterparty;
+      sd.thr_own = std::isfinite(c.threshold_own) ? c.threshold_own : -1.0;
+      sd.mta = c.minimum_transfer_amount;
+      sd.ia = c.independent_amount;
+      sd.keep = c.type == CCR_COLLATERAL_CASH ? 1.0 : 1.0 - c.haircut;
+    }
+    sd.ring_len = ring;
+
+    // Valuation terms per valued date.
+    std::vector<Collector> cols(static_cast<std::size_t>(T));
+    std::vector<char> valued(static_cast<std::size_t>(T), 0);
+    std::map<std::pair<int, int>, int> lastUse;  // (rate, fixing date) -> last date using it
+    for (int j = 0; j < T; ++j) {
+      valued[static_cast<std::size_t>(j)] = repOf[static_cast<std::size_t>(j)] >= 0 || isCall[static_cast<std::size_t>(j)];
+      if (!valued[static_cast<std::size_t>(j)]) continue;
+      Collector& col = cols[static_cast<std::size_t>(j)];
+      col.t = sim[static_cast<std::size_t>(j)];
+      for (const Trade& tr : trades) collect(tr, m, sim, col);
+      for (const auto& kv : col.fix) {
+        const auto key = std::make_pair(std::get<0>(kv.first), std::get<1>(kv.first));
+        auto it = lastUse.find(key);
+        if (it == lastUse.end() || it->second < j) lastUse[key] = j;
+      }
+    }
+    // Slots: interval colouring of [fixing date, last use].
+    std::map<std::pair<int, int>, int> slotOf;
+    std::vector<int> slotFreeAfter;  // last use of the interval currently in each slot
+    std::vector<std::vector<std::pair<int, int>>> storeAt(static_cast<std::size_t>(T));
+    for (const auto& kv : lastUse) {  // ordered by (rate, k); re-sort by k
+      (void)kv;
+    }
+    std::vector<std::tuple<int, int, int>> intervals;  // (k, rate, last)
+    for (const auto& kv : lastUse) intervals.emplace_back(kv.first.second, kv.first.first, kv.second);
+    std::sort(intervals.begin(), intervals.end());
+    for (const auto& iv : intervals) {
+      const int k = std::get<0>(iv), r = std::get<1>(iv), last = std::get<2>(iv);
+      int slot = -1;
+      for (std::size_t q = 0; q < slotFreeAfter.size(); ++q)
+        if (slotFreeAfter[q] < k) {
+          slot = static_cast<int>(q);
+          break;
+        }
+      if (slot < 0) {
+        slot = static_cast<int>(slotFreeAfter.size());
+        slotFreeAfter.push_back(0);
+      }
+      slotFreeAfter[static_cast<std::size_t>(slot)] = last;
+      slotOf[{r, k}] = slot;
+      storeAt[static_cast<std::size_t>(k)].emplace_back(r, slot);
+    }
+    sd.num_slots = static_cast<int32_t>(slotFreeAfter.size());
+
+    // Emit dates.
+    int64_t setTerms = 0;
+    for (int j = 0; j < T; ++j) {
+      const double t = sim[static_cast<std::size_t>(j)];
+      DateRec rec{};
+      rec.rep = repOf[static_cast<std::size_t>(j)];
+      rec.flags = (valued[static_cast<std::size_t>(j)] ? kValued : 0) | (rec.rep >= 0 ? kReporting : 0) |
+                  (isCall[static_cast<std::size_t>(j)] ? kStoreValue : 0);
+      rec.call_pos = call[static_cast<std::size_t>(j)] % ring;
+      rec.store_pos = isCall[static_cast<std::size_t>(j)] ? j % ring : -1;
+      double reals[kDateRealStride] = {0};
+      const Collector& col = cols[static_cast<std::size_t>(j)];
+      const int bondBase = static_cast<int>(plan.bonds.size() / 2), fixBase = static_cast<int>(plan.fixed.size() / 4);
+      for (int r = 0; r < kMaxRates; ++r) {
+        rec.bond_beg[r] = static_cast<int32_t>(plan.bonds.size() / 2) - sd.bond_off;
+        if (r < g.nrates) {
+          const HullWhite& hw = m.rates[static_cast<std::size_t>(r)];
+          for (auto it = col.bond.lower_bound({r, -1e300}); it != col.bond.end() && it->first.first == r; ++it) {
+            const double T_ = it->first.second;
+            plan.bonds.push_back(it->second * hw.A(t, T_));
+            plan.bonds.push_back(hw.B(t, T_));
+          }
+        }
+      }
+      rec.bond_beg[kMaxRates] = static_cast<int32_t>(plan.bonds.size() / 2) - sd.bond_off;
+      for (int r = 0; r < kMaxRates; ++r) {
+        rec.fix_beg[r] = static_cast<int32_t>(plan.fixed.size() / 4) - sd.fixed_off;
+        if (r < g.nrates) {
+          const HullWhite& hw = m.rates[static_cast<std::size_t>(r)];
+          for (const auto& kv : col.fix) {
+            if (std::get<0>(kv.first) != r) continue;
+            const int k = std::get<1>(kv.first);
+            const double ts = std::get<2>(kv.first), te = std::get<3>(kv.first);
+            const double tk = sim[static_cast<std::size_t>(k)];
+            const double As = ts <= tk ? 1.0 : hw.A(tk, ts), Bs = ts <= tk ? 0.0 : hw.B(tk, ts);
+            const double Ae = te <= tk ? 1.0 : hw.A(tk, te), Be = te <= tk ? 0.0 : hw.B(tk, te);
+            plan.fixed.push_back(kv.second * (As / Ae) * hw.A(t, te));
+            plan.fixed.push_back(hw.B(t, te));
+            plan.fixed.push_back(Bs - Be);
+            plan.fixed.push_back(static_cast<double>(slotOf.at({r, k})));
+          }
+        }
+      }
+      rec.fix_beg[kMaxRates] = static_cast<int32_t>(plan.fixed.size() / 4) - sd.fixed_off;
+      rec.fixstore_beg = static_cast<int32_t>(plan.fixstore.size() / 2) - sd.fixstore_off;
+      for (const auto& rs : storeAt[static_cast<std::size_t>(j)]) {
+        plan.fixstore.push_back(rs.first);
+        plan.fixstore.push_back(rs.second);
+      }
+      rec.fixstore_end = static_cast<int32_t>(plan.fixstore.size() / 2) - sd.fixstore_off;
+      for (int r = 0; r < kMaxRates; ++r) reals[kDateC0 + r] = col.c0[r];
+      for (int a = 0; a < kMaxAssets; ++a) reals[kDateLin + a] = col.lin[a];
+      if (ns.has_collateral && ns.collateral.type == CCR_COLLATERAL_BOND) {
+        const double T_ = ns.collateral.bond_maturity;
+        reals[kDateBond] = T_ <= t ? 1.0 : m.rates[0].A(t, T_);
+        reals[kDateBond + 1] = T_ <= t ? 0.0 : m.rates[0].B(t, T_);
+      }
+      const int nTerms = (static_cast<int>(plan.bonds.size() / 2) - bondBase) + (static_cast<int>(plan.fixed.size() / 4) - fixBase);
+      setTerms += nTerms;
+      plan.dates.push_back(rec);
+      plan.date_reals.insert(plan.date_reals.end(), reals, reals + kDateRealStride);
+
+      // Step coefficients t_{j-1} -> t_j.
+      double st[kStepStride] = {0};
+      if (j > 0) {
+        const double t0 = sim[static_cast<std::size_t>(j - 1)], t1 = t, dt = t1 - t0;
+        for (int r = 0; r < g.nrates; ++r) {
+          const HullWhite& hw = m.rates[static_cast<std::size_t>(r)];
+          const HullWhite::Step hs = hw.step(t0, t1);
+          const double mu = m.quanto[static_cast<std::size_t>(r)];
+          double* o = st + kStepRate + 8 * r;
+          o[0] = hs.e;
+          o[1] = hs.sdX;
+          o[2] = hs.c1;
+          o[3] = hs.c2;
+          o[4] = hs.xc;
+          o[5] = mu != 0.0 ? mu * (1.0 - hs.e) / hw.a : 0.0;
+          o[6] = mu != 0.0 ? mu * (dt - (1.0 - hs.e) / hw.a) / hw.a : 0.0;
+          o[7] = hs.iphi;
+        }
+        for (int a = 0; a < g.nassets; ++a) {
+          double* o = st + kStepAsset + 3 * a;
+          const double vol = m.vol[static_cast<std::size_t>(a)];
+          o[0] = m.assetRate[static_cast<std::size_t>(a)] == 0
+                     ? std::log(m.carry[static_cast<std::size_t>(a)].discount(t0) / m.carry[static_cast<std::size_t>(a)].discount(t1))
+                     : 0.0;
+          o[1] = vol * std::sqrt(dt);
+          o[2] = 0.5 * vol * vol * dt;
+        }
+        for (int c = 0; c < g.ncredits; ++c) {
+          const CirModel& cm = m.credits[static_cast<std::size_t>(c)];
+          double* o = st + kStepCredit + kStepCreditStride * c;
+          const double h = dt / static_cast<double>(cm.sub);
+          o[0] = std::sqrt(dt);
+          o[1] = h;
+          o[2] = cm.shift(t1);
+          double remT = dt;
+          for (int k = 0; k < cm.sub; ++k) {
+            if (k + 1 < cm.sub) {
+              o[3 + k] = h / remT;
+              o[3 + kMaxSub + k] = std::sqrt(h * (remT - h) / remT);
+            }
+            remT -= h;
+          }
+        }
+        if (dt <= 0.0) invalid("HullWhite1F::evolve: t1 must exceed t0");
+      }
+      plan.steps.insert(plan.steps.end(), st, st + kStepStride);
+      if (valued[static_cast<std::size_t>(j)]) plan.term_evaluations += static_cast<int64_t>(nTerms) * P;
+    }
+    plan.total_terms += setTerms;
+    plan.revaluations += static_cast<int64_t>(ns.num_trades) * P * R;
+    sd.rows_off = static_cast<int64_t>(s) * 0;  // filled below once max_rep is known
+    plan.sets.push_back(sd);
+    plan.max_dates = std::max(plan.max_dates, T);
+    plan.max_rep = std::max(plan.max_rep, R);
+    g.max_ring = std::max(g.max_ring, ring);
+    g.max_slots = std::max(g.max_slots, sd.num_slots);
+  }
+  const int64_t blocksPaths = cfg.antithetic ? (static_cast<int64_t>(P) + 1) / 2 : P;
+  plan.stream_len = blocksPaths * maxStride;
+  plan.total_waves = num_sets * blocksPerSet * wavesPerBlock;
+  for (int s = 0; s < num_sets; ++s) {
+    SetDev& sd = plan.sets[static_cast<std::size_t>(s)];
+    sd.rows_off = static_cast<int64_t>(s) * plan.max_rep * P;
+    sd.partial_off = static_cast<int64_t>(sd.wave_begin) * plan.max_rep * kStats;
+  }
+  plan.block_set.resize(static_cast<std::size_t>(num_sets * blocksPerSet));
+  for (int b = 0; b < num_sets * blocksPerSet; ++b) plan.block_set[static_cast<std::size_t>(b)] = b / blocksPerSet;
+  return plan;
+}
+
+}  // namespace ccrhip

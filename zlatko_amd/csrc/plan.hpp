@@ -1,0 +1,47 @@
+// plan.hpp -- host plan builder: turns a scenario generator, a simulation config and netting sets
+// into the flat per-date tables of layout.hpp (the analogue of ccr::gpu::compile,
+// src/gpu/fused_exposure.cpp:214-707, re-derived for one path per lane and FP64 parity).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "layout.hpp"
+#include "model.hpp"
+
+namespace ccrhip {
+
+struct HostPlan {
+  GlobalDev g{};
+  std::vector<SetDev> sets;
+  std::vector<double> steps;      // [sum T_s][kStepStride]
+  std::vector<DateRec> dates;     // [sum T_s]
+  std::vector<double> date_reals; // [sum T_s][kDateRealStride]
+  std::vector<double> bonds;      // [n][2]
+  std::vector<double> fixed;      // [n][4]
+  std::vector<int32_t> fixstore;  // [n][2] (rate, slot)
+  std::vector<int32_t> block_set; // set of each workgroup (before the XCD remap)
+  std::vector<std::vector<double>> rep_times;  // reporting dates per set
+  int64_t stream_len = 0;         // normals in the replayed stream
+  int32_t max_dates = 0, max_rep = 0, total_waves = 0;
+  int32_t block = 256;
+  int32_t precision = CCR_FP64;
+  double pfe_q = 0.95;
+  int64_t revaluations = 0;       // trades x paths x reporting dates
+  int64_t term_evaluations = 0;   // exponential terms x paths over valued dates
+  int64_t total_terms = 0;
+};
+
+// ExposureEngine::reportingGrid / simulationGrid (src/exposure/exposure_engine.cpp:35-48).
+TimeGrid reportingGrid(const TimeGrid& base, const ccr_netting_set& set);
+TimeGrid simulationGrid(const TimeGrid& base, const ccr_netting_set& set);
+
+HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
+                   int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
+                   int32_t precision, int32_t block);
+
+// The reference normal stream: std::mt19937_64(seed) + std::normal_distribution<double>, as
+// ccr::NormalGenerator (include/ccr/core/random.hpp:13-26), n draws.
+void generateStream(uint64_t seed, int64_t n, double* out);
+
+}  // namespace ccrhip

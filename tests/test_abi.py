@@ -1,0 +1,64 @@
+"""The C-ABI library: it loads, exports every function include/ccr_hip.h declares, and its
+device-free entry points (grids, CVA integrals, argument validation) agree with the oracle."""
+import ctypes as C
+import subprocess
+
+import numpy as np
+
+from tests import oracle_lib
+from zlatko_amd import abi, native
+from zlatko_amd import workloads as W
+
+
+def test_library_exports_every_declared_symbol():
+    lib = native.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", str(native.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    declared = native.header_symbols()
+    assert len(declared) >= 17
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+    assert lib.ccr_hip_abi_version() == 1
+
+
+def test_grids_match_oracle_bit_for_bit():
+    for spec in (W.small_reference_case(8, csa=True, fx=True), W.c2(num_paths=8, num_sets=3, num_trades=30)):
+        for i in range(len(spec.sets)):
+            for sim in (False, True):
+                g = native.grid(spec, i, simulation=sim)
+                fn = oracle_lib.oracle().ccr_oracle_simulation_grid if sim else oracle_lib.oracle().ccr_oracle_reporting_grid
+                n = C.c_int32()
+                ref = np.zeros(len(g) + 8)
+                fn(abi.dptr(spec.base_grid), len(spec.base_grid), C.byref(spec.sets_c[i]), abi.dptr(ref), len(ref),
+                   C.byref(n))
+                assert n.value == len(g) and np.array_equal(ref[: n.value], g)
+
+
+def test_cva_integrals_match_oracle():
+    lib = native.lib()
+    piv, haz = np.array([2.0, 5.0]), np.array([0.01, 0.03])
+    cc = abi.CreditCurveC(abi.dptr(piv), abi.dptr(haz), 2)
+    p2, h2 = np.array([1.0]), np.array([0.015])
+    own = abi.CreditCurveC(abi.dptr(p2), abi.dptr(h2), 1)
+    t = np.linspace(0.0, 5.0, 21)
+    dee = 100.0 + 10.0 * np.sin(t)
+    dene = -50.0 - 5.0 * np.cos(t)
+    a = lib.ccr_unilateral_cva(abi.dptr(t), abi.dptr(dee), 21, C.byref(cc), 0.4)
+    b = oracle_lib.oracle().ccr_oracle_unilateral_cva(abi.dptr(t), abi.dptr(dee), 21, C.byref(cc), 0.4)
+    assert a == b
+    c1, d1, c2, d2 = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    lib.ccr_bilateral_cva(abi.dptr(t), abi.dptr(dee), abi.dptr(dene), 21, C.byref(cc), 0.4, C.byref(own), 0.3,
+                          C.byref(c1), C.byref(d1))
+    oracle_lib.oracle().ccr_oracle_bilateral_cva(abi.dptr(t), abi.dptr(dee), abi.dptr(dene), 21, C.byref(cc), 0.4,
+                                                 C.byref(own), 0.3, C.byref(c2), C.byref(d2))
+    assert abs(c1.value - c2.value) <= 1e-15 * abs(c2.value) and abs(d1.value - d2.value) <= 1e-15 * abs(d2.value)
+
+
+def test_create_reports_errors_without_crashing():
+    lib = native.lib()
+    ctx = C.c_void_p()
+    bad = abi.HipOptionsC(0, 0, 96)  # not a multiple of 64
+    assert lib.ccr_hip_create(C.byref(bad), C.byref(ctx)) == abi.CCR_ERR_INVALID_ARGUMENT
+    assert b"multiple of 64" in lib.ccr_hip_last_error(None)
+    assert lib.ccr_hip_set_fault(None, 1) == abi.CCR_ERR_INVALID_ARGUMENT

@@ -58,7 +58,8 @@ def test_cva_integrals_match_oracle():
 def test_create_reports_errors_without_crashing():
     lib = native.lib()
     ctx = C.c_void_p()
-    bad = abi.HipOptionsC(0, 0, 96)  # not a multiple of 64
-    assert lib.ccr_hip_create(C.byref(bad), C.byref(ctx)) == abi.CCR_ERR_INVALID_ARGUMENT
-    assert b"multiple of 64" in lib.ccr_hip_last_error(None)
+    for block in (96, 512):  # not a multiple of 64; above the path kernel's launch bound
+        bad = abi.HipOptionsC(0, 0, block)
+        assert lib.ccr_hip_create(C.byref(bad), C.byref(ctx)) == abi.CCR_ERR_INVALID_ARGUMENT
+        assert b"multiple of 64" in lib.ccr_hip_last_error(None)
     assert lib.ccr_hip_set_fault(None, 1) == abi.CCR_ERR_INVALID_ARGUMENT

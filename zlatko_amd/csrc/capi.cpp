@@ -335,8 +335,8 @@ int32_t ccr_hip_create(const ccr_hip_options* options, ccr_hip_ctx** out) {
     ctx->precision = options->precision;
     ctx->block = options->block_size > 0 ? options->block_size : 256;
   }
-  if (ctx->block % 64 != 0 || ctx->block > 1024) {
-    gCreateError = "block_size must be a multiple of 64 up to 1024";
+  if (ctx->block % 64 != 0 || ctx->block > kMaxBlock) {
+    gCreateError = "block_size must be a multiple of 64 up to 256 (the path kernel's launch bound)";
     return CCR_ERR_INVALID_ARGUMENT;
   }
   int n = 0;

@@ -33,7 +33,7 @@ __device__ inline double waveSum(double v) {
 }
 
 template <typename Real>
-__global__ void __launch_bounds__(256) ccr_path_kernel(PathArgs<Real> a) {
+__global__ void __launch_bounds__(kMaxBlock) ccr_path_kernel(PathArgs<Real> a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const GlobalDev& g = *a.g;
   const int b = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));

@@ -25,6 +25,7 @@ constexpr int kMaxCredits = 2;    // CIR++ intensities (counterparty, own)
 constexpr int kMaxCorr = kMaxRates + kMaxAssets + kMaxCredits;
 constexpr int kMaxSub = 8;        // CIR++ sub-steps per grid step
 constexpr int kStats = 9;         // E[V-C], EE, ENE, EE*, ENE*, E[D], E[Q], CVA incr., DVA incr.
+constexpr int kMaxBlock = 256;    // launch bound of the path kernel (threads per workgroup)
 
 // Step record (Real[kStepStride]) for the step t_{j-1} -> t_j (row 0 unused):
 //   rate r:   8 r + {e, sdX, c1, c2, (1-e)/a, drift (1-e)/a, drift (dt-(1-e)/a)/a, int phi}

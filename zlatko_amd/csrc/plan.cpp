@@ -259,7 +259,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
   if (cfg.num_paths <= 0) invalid("ScenarioGenerator::generate: numPaths must be positive");
   if (cfg.num_paths > (int64_t(1) << 31) - 1) fail(CCR_ERR_UNSUPPORTED, "at most 2^31 - 1 paths");
   if (pfe_quantile < 0.0 || pfe_quantile > 1.0) invalid("quantile: level must be in [0, 1]");
-  if (block <= 0 || block % 64 != 0 || block > 1024) invalid("block size must be a multiple of 64 up to 1024");
+  if (block <= 0 || block % 64 != 0 || block > kMaxBlock) invalid("block size must be a multiple of 64 up to 256");
   const Models m = makeModels(gen);
   const TimeGrid base(std::vector<double>(base_grid, base_grid + std::max(0, base_size)));
   HostPlan plan;

@@ -54,11 +54,11 @@ def describe(spec: abi.Spec, info) -> dict:
 
 
 def time_gpu(engine, plan, steps: int, warmup: int):
-    import torch
+    # engine.synchronize() waits for the library's stream and then the whole device
+    # (hipDeviceSynchronize): the timed region is bracketed by full-device synchronisation.
     for _ in range(warmup):
         plan.execute_async()
     engine.synchronize()
-    torch.cuda.synchronize()
     path_ms, total_ms = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -67,7 +67,6 @@ def time_gpu(engine, plan, steps: int, warmup: int):
         path_ms.append(t.path_ms)
         total_ms.append(t.total_ms)
     engine.synchronize()
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return wall, float(np.mean(path_ms)), float(np.mean(total_ms)), engine.timing()
 

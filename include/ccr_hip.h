@@ -286,6 +286,7 @@ int32_t ccr_hip_plan_info_get(const ccr_hip_plan* plan, ccr_hip_plan_info* info)
 int32_t ccr_hip_execute(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs);
 /* Same, without reading results back (for timing loops). */
 int32_t ccr_hip_execute_async(ccr_hip_ctx* ctx, ccr_hip_plan* plan);
+/* Waits for the context's stream, then for the whole device. */
 int32_t ccr_hip_synchronize(ccr_hip_ctx* ctx);
 int32_t ccr_hip_last_timing(const ccr_hip_ctx* ctx, ccr_hip_timing* timing);
 void ccr_hip_plan_destroy(ccr_hip_plan* plan);

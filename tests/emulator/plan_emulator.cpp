@@ -24,9 +24,9 @@ extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr
     std::vector<double> steps = h.steps, dreal = h.date_reals, bonds = h.bonds, fixed = h.fixed;
     PathArgs<double> a{&h.g, h.sets.data(), h.block_set.data(), stream.data(), steps.data(), h.dates.data(),
                        dreal.data(), bonds.data(), fixed.data(), h.fixstore.data(), nullptr, nullptr,
-                       h.max_rep, h.g.max_ring, h.g.max_slots};
+                       h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots};
     const int P = h.g.num_paths;
-    std::vector<double> scratch(static_cast<size_t>(2 * h.g.max_ring + h.g.max_slots + 1));
+    std::vector<double> scratch(static_cast<size_t>(h.g.max_ring + h.price_rows + h.g.max_slots + 1));
     for (int s = 0; s < nsets; ++s) {
       const SetDev& S = h.sets[static_cast<size_t>(s)];
       const int R = S.num_rep;
@@ -35,7 +35,7 @@ extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr
       std::vector<double> rows(static_cast<size_t>(R) * P, 0.0);
       for (int p = 0; p < P; ++p) {
         std::fill(scratch.begin(), scratch.end(), 0.0);
-        simulatePath<double>(a, h.g, S, p, scratch.data(), 1, [&](int rr, const double* st, double vp) {
+        simulatePath<double>(a, &S, p, scratch.data(), 1, [&](int rr, const double* st, double vp) {
           for (int k = 0; k < kStats; ++k) sums[static_cast<size_t>(rr) * kStats + k] += st[k];
           rows[static_cast<size_t>(rr) * P + p] = vp;
         });
@@ -90,4 +90,9 @@ extern "C" int32_t emu_plan_stats(const ccr_scenario_generator* gen, const ccr_s
   } catch (const std::exception&) {
     return CCR_ERR_RUNTIME;
   }
+}
+
+// The product's FP64 bond-term exponential, evaluated on the host.
+extern "C" void emu_exp_bond(const double* x, int64_t n, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = expBond(x[i]);
 }

@@ -71,3 +71,14 @@ def test_plan_dimensions_of_config_2(emu):
     assert rep >= 261 and dates > rep  # weekly reporting dates plus events; lagged margin-call dates
     assert 1 < ring <= 16 and 0 < slots <= 16  # per-path LDS look-back stays small
     assert stream == 25000 * (dates - 1) * 8 or stream <= 25000 * (dates - 1) * 8
+
+
+def test_fp64_bond_exponential_is_within_one_ulp(emu):
+    """The path kernel's FP64 exp (Cody-Waite reduction + degree-13 Taylor) against numpy's exp."""
+    emu.emu_exp_bond.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+    x = np.concatenate([np.linspace(-40.0, 40.0, 400001), np.random.default_rng(0).uniform(-2.0, 2.0, 400000),
+                        [0.0, -0.0, 1e-300, -1e-17, np.log(2.0) / 2, -np.log(2.0) / 2]])
+    out = np.zeros_like(x)
+    emu.emu_exp_bond(x.ctypes.data_as(C.c_void_p), len(x), out.ctypes.data_as(C.c_void_p))
+    ref = np.exp(x)
+    assert (np.abs(out - ref) / np.spacing(ref)).max() <= 1.0

@@ -219,13 +219,13 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
     PathArgs<float> a{plan->d_global, plan->d_sets, plan->d_block_set, static_cast<const float*>(plan->d_stream),
                       static_cast<const float*>(plan->d_steps), plan->d_dates, static_cast<const float*>(plan->d_date_reals),
                       static_cast<const float*>(plan->d_bonds), static_cast<const float*>(plan->d_fixed), plan->d_fixstore,
-                      static_cast<float*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.g.max_slots};
+                      static_cast<float*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots};
     check(launchPath<float>(a, blocks, h.block, ctx->stream), "path kernel launch");
   } else {
     PathArgs<double> a{plan->d_global, plan->d_sets, plan->d_block_set, static_cast<const double*>(plan->d_stream),
                        static_cast<const double*>(plan->d_steps), plan->d_dates, static_cast<const double*>(plan->d_date_reals),
                        static_cast<const double*>(plan->d_bonds), static_cast<const double*>(plan->d_fixed), plan->d_fixstore,
-                       static_cast<double*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.g.max_slots};
+                       static_cast<double*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots};
     check(launchPath<double>(a, blocks, h.block, ctx->stream), "path kernel launch");
   }
   check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
@@ -438,7 +438,10 @@ int32_t ccr_hip_execute_async(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
 
 int32_t ccr_hip_synchronize(ccr_hip_ctx* ctx) {
   if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
-  return guarded(ctx, [&] { check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); });
+  return guarded(ctx, [&] {
+    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  });
 }
 
 int32_t ccr_hip_execute(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs) {

@@ -35,9 +35,10 @@ __device__ inline double waveSum(double v) {
 template <typename Real>
 __global__ void __launch_bounds__(kMaxBlock) ccr_path_kernel(PathArgs<Real> a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  const GlobalDev& g = *a.g;
+  const CCR_CONST GlobalDev& g = *toConst(a.g);
   const int b = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
-  const SetDev& S = a.sets[a.block_set[b]];
+  cptr<SetDev> sp = toConst(a.sets) + toConst(a.block_set)[b];
+  const CCR_CONST SetDev& S = *sp;
   const int bd = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
   const int lb = b - S.block_begin;
   const int p = lb * bd + tid;
@@ -47,7 +48,7 @@ __global__ void __launch_bounds__(kMaxBlock) ccr_path_kernel(PathArgs<Real> a) {
   double* part = a.partials + S.partial_off + static_cast<int64_t>(waveLocal) * a.max_rep * kStats;
   Real* rows = a.rows + S.rows_off;
   Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
-  simulatePath<Real>(a, g, S, pe, col, bd, [&](int rr, const double* st9, double vp) {
+  simulatePath<Real>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
     if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
 #pragma unroll
     for (int k = 0; k < kStats; ++k) {
@@ -165,7 +166,7 @@ __global__ void __launch_bounds__(256) ccr_pfe_kernel(const SetDev* sets, const 
 
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
-  const size_t lds = static_cast<size_t>(2 * args.ring + args.slots) * block * sizeof(Real);
+  const size_t lds = static_cast<size_t>(args.ring + args.price_rows + args.slots) * block * sizeof(Real);
   hipLaunchKernelGGL(ccr_path_kernel<Real>, dim3(blocks), dim3(block), lds, stream, args);
   return hipGetLastError();
 }

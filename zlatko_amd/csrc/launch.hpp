@@ -5,7 +5,23 @@
 
 #include "layout.hpp"
 
+// Plan tables are read through the constant address space on the device: AMDGPU treats such
+// loads as invariant and issues them as scalar (s_load) loads when the address is uniform, so
+// one load feeds all 64 lanes. Host code (the CPU emulator) sees plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CCR_CONST __attribute__((address_space(4)))
+#else
+#define CCR_CONST
+#endif
+
 namespace ccrhip {
+
+template <class T>
+using cptr = const CCR_CONST T*;
+template <class T>
+__host__ __device__ inline cptr<T> toConst(const T* p) {
+  return (cptr<T>)p;  // address-space cast (a C-style cast is the form clang accepts)
+}
 
 template <typename Real>
 struct PathArgs {
@@ -22,8 +38,9 @@ struct PathArgs {
   Real* rows;
   double* partials;
   int32_t max_rep;
-  int32_t ring;    // LDS rows per look-back ring
-  int32_t slots;   // LDS rows of fixing states
+  int32_t ring;        // LDS rows of the margin-call value ring
+  int32_t price_rows;  // LDS rows of the collateral price ring (0 for cash collateral)
+  int32_t slots;       // LDS rows of fixing states
 };
 
 template <typename Real>

@@ -12,6 +12,9 @@
 //   -> on valued dates V = sum over rate groups of FX x (c0 + sum c A e^{-B x} + fixed coupons)
 //   -> CSA (collateral.cpp:10-47) with the margin-call value read from a per-path ring
 //   -> on reporting dates, the 9 statistics of ExposureProfile::compute and pathwiseCva.
+//
+// The plan tables are passed as __restrict__ read-only pointers: on the GPU the per-date term
+// records are then read with scalar (uniform) loads, shared by the 64 lanes of a wavefront.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,9 +27,40 @@
 
 namespace ccrhip {
 
+// exp(a) for |a| < 700 in FP64 (no overflow / underflow handling: bond-term exponents are
+// O(1)). Cody-Waite reduction a = n ln2 + r, |r| <= ln2/2, with the two-part ln2 of fdlibm's
+// e_exp.c; exp(r) by its degree-13 Taylor polynomial (truncation < 5e-18 relative on the
+// interval) in Horner form; 2^n by ldexp. Accurate to about 1 ulp.
+__host__ __device__ inline double expBond(double a) {
+  const double n = rint(a * 1.4426950408889634074);
+  double r = fma(-n, 6.93147180369123816490e-01, a);
+  r = fma(-n, 1.90821492927058770002e-10, r);
+  double p = 1.0 / 6227020800.0;       // 1/13!
+  p = fma(p, r, 1.0 / 479001600.0);    // 1/12!
+  p = fma(p, r, 1.0 / 39916800.0);     // 1/11!
+  p = fma(p, r, 1.0 / 3628800.0);      // 1/10!
+  p = fma(p, r, 1.0 / 362880.0);       // 1/9!
+  p = fma(p, r, 1.0 / 40320.0);        // 1/8!
+  p = fma(p, r, 1.0 / 5040.0);         // 1/7!
+  p = fma(p, r, 1.0 / 720.0);          // 1/6!
+  p = fma(p, r, 1.0 / 120.0);          // 1/5!
+  p = fma(p, r, 1.0 / 24.0);           // 1/4!
+  p = fma(p, r, 1.0 / 6.0);            // 1/3!
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, static_cast<int>(n));
+}
+
 template <typename Real>
 __host__ __device__ inline Real expr(Real v) {
   if constexpr (std::is_same<Real, double>::value) return exp(v);
+  else return __expf(v);
+}
+// Exponential of a merged bond-term exponent.
+template <typename Real>
+__host__ __device__ inline Real expTerm(Real v) {
+  if constexpr (std::is_same<Real, double>::value) return expBond(v);
   else return __expf(v);
 }
 template <typename Real>
@@ -35,19 +69,24 @@ __host__ __device__ inline Real sqrtr(Real v) {
   else return __fsqrt_rn(v);
 }
 
-// col: this path's column of per-path scratch (LDS on the GPU), stride: distance between rows.
-// emit(rep, stats[kStats], positive_exposure) is called on every reporting date.
+// col: this path's column of per-path scratch (LDS on the GPU), stride: distance between rows;
+// rows [0, ring) hold the margin-call values, [ring, ring + priceRows) the collateral prices,
+// then the fixing states. emit(rep, stats[kStats], positive_exposure) is called on every
+// reporting date.
 template <typename Real, typename Emit>
-__host__ __device__ inline void simulatePath(const PathArgs<Real>& a, const GlobalDev& g, const SetDev& S, int pe,
-                                             Real* col, int stride, Emit&& emit) {
+__host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
+                                             const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
+                                             cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
+                                             cptr<int32_t> fstore, int ring, int priceRows, Real* col, int stride,
+                                             Emit&& emit) {
   constexpr bool kDouble = std::is_same<Real, double>::value;
+  const CCR_CONST GlobalDev& g = *gp;
+  const CCR_CONST SetDev& S = *sp;
   Real* ringV = col;
-  Real* ringP = col + a.ring * stride;
-  Real* xs = col + 2 * a.ring * stride;
+  Real* ringP = col + ring * stride;
+  Real* xs = col + (ring + priceRows) * stride;
   const int nR = g.nrates, nA = g.nassets, nC = g.ncredits, nz = g.nz, nCorr = g.ncorr;
-  const int64_t q = g.antithetic ? (pe >> 1) : pe;
   const Real sgn = (g.antithetic && (pe & 1)) ? Real(-1) : Real(1);
-  const Real* zbase = a.stream + q * S.stream_stride;
 
   Real x[kMaxRates], S_[kMaxAssets], y[kMaxCredits], iy[kMaxCredits], Qs[kMaxCredits];
   Real lnD = 0, lnDc = 0, D = 1;  // FP32: Kahan sum of int r; FP64: running product
@@ -69,26 +108,35 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, const Glob
   Real units = 0;
   double prevDvp = 0, prevDvn = 0, prevQc = 1, prevQo = 1;
 
-  const DateRec* dates = a.dates + S.date_off;
-  const Real* dreal = a.date_reals + static_cast<int64_t>(S.date_off) * kDateRealStride;
-  const Real* bonds = a.bonds + 2 * static_cast<int64_t>(S.bond_off);
-  const Real* fixed = a.fixed + 4 * static_cast<int64_t>(S.fixed_off);
-  const int32_t* fstore = a.fixstore + 2 * S.fixstore_off;
-
   for (int j = 0; j < S.num_dates; ++j) {
     if (j > 0) {
-      const Real* st = a.steps + static_cast<int64_t>(S.date_off + j) * kStepStride;
+      cptr<Real> st = steps + static_cast<int64_t>(j) * kStepStride;
       const Real* z = zbase + static_cast<int64_t>(j - 1) * nz;
+      // Correlated drivers at fixed factor slots: rates [0, kMaxRates), assets, credits. Row
+      // `slot` of the factor holds compact row `lim` (lower triangular: columns k <= lim).
       Real w[kMaxCorr];
       {
+        cptr<double> chol = g.chol;
+#if defined(__HIP_DEVICE_COMPILE__)
+        // Opaque per step: the factors are re-read through the scalar cache each step instead of
+        // being hoisted out of the date loop into ~100 registers.
+        asm volatile("" : "+s"(chol));
+#endif
         Real zc[kMaxCorr];
 #pragma unroll
         for (int i = 0; i < kMaxCorr; ++i) zc[i] = i < nCorr ? sgn * z[i] : Real(0);
 #pragma unroll
         for (int i = 0; i < kMaxCorr; ++i) {
+          const int lim = i < kMaxRates ? i : i < kMaxRates + kMaxAssets ? nR + (i - kMaxRates)
+                                                                          : nR + nA + (i - kMaxRates - kMaxAssets);
+          const bool used = i < kMaxRates ? i < nR : i < kMaxRates + kMaxAssets ? i - kMaxRates < nA
+                                                                                 : i - kMaxRates - kMaxAssets < nC;
           Real sum = 0;
+          if (used) {
 #pragma unroll
-          for (int k = 0; k <= i; ++k) sum += static_cast<Real>(g.chol[i * kMaxCorr + k]) * zc[k];
+            for (int k = 0; k < kMaxCorr; ++k)
+              if (k <= lim) sum += static_cast<Real>(chol[i * kMaxCorr + k]) * zc[k];
+          }
           w[i] = sum;
         }
       }
@@ -96,7 +144,7 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, const Glob
 #pragma unroll
       for (int r = 0; r < kMaxRates; ++r) {
         if (r < nR) {
-          const Real* o = st + kStepRate + 8 * r;
+          cptr<Real> o = st +kStepRate + 8 * r;
           const Real z2 = sgn * (r == 0 ? z[nCorr] : z[g.foreign_off + r - 1]);
           const Real x0 = x[r];
           x[r] = x0 * o[0] + o[1] * w[r] + o[5];
@@ -118,13 +166,13 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, const Glob
 #pragma unroll
       for (int i = 0; i < kMaxAssets; ++i) {
         if (i < nA) {
-          const Real* o = st + kStepAsset + 3 * i;
+          cptr<Real> o = st +kStepAsset + 3 * i;
           const int rr = g.asset_rate[i];
           Real icarry = o[0];
 #pragma unroll
           for (int r = 1; r < kMaxRates; ++r)
             if (r == rr) icarry = ir[r];
-          const Real drift = ir[0] - icarry - o[2] + o[1] * w[nR + i];
+          const Real drift = ir[0] - icarry - o[2] + o[1] * w[kMaxRates + i];
           if constexpr (kDouble) {
             S_[i] = S_[i] * exp(drift);
           } else {
@@ -139,12 +187,12 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, const Glob
 #pragma unroll
       for (int c = 0; c < kMaxCredits; ++c) {
         if (c < nC) {
-          const Real* o = st + kStepCredit + kStepCreditStride * c;
+          cptr<Real> o = st +kStepCredit + kStepCreditStride * c;
           const int sub = g.credit_sub[c];
           const Real* ex = z + g.credit_off[c];
           const Real kappa = static_cast<Real>(g.cir_kappa[c]), theta = static_cast<Real>(g.cir_theta[c]);
           const Real xi = static_cast<Real>(g.cir_xi[c]), h = o[1];
-          Real remaining = o[0] * w[nR + nA + c];
+          Real remaining = o[0] * w[kMaxRates + kMaxAssets + c];
           Real yc = y[c], iyc = iy[c];
           for (int k = 0; k < sub; ++k) {
             Real dW = remaining;
@@ -172,19 +220,19 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, const Glob
       xs[slot * stride] = xv;
     }
     if (!(rec.flags & kValued) && !S.has_csa) continue;
-    const Real* dr = dreal + static_cast<int64_t>(j) * kDateRealStride;
+    cptr<Real> dr = dreal + static_cast<int64_t>(j) * kDateRealStride;
     Real V = 0;
     if (rec.flags & kValued) {
 #pragma unroll
       for (int r = 0; r < kMaxRates; ++r) {
         if (r < nR) {
-          const Real xr = x[r];
+          const Real nx = -x[r];
           Real acc = dr[kDateC0 + r];
-          for (int i = rec.bond_beg[r]; i < rec.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expr(-bonds[2 * i + 1] * xr);
+          for (int i = rec.bond_beg[r]; i < rec.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expTerm(bonds[2 * i + 1] * nx);
           for (int i = rec.fix_beg[r]; i < rec.fix_beg[r + 1]; ++i) {
-            const Real* f = fixed + 4 * static_cast<int64_t>(i);
+            cptr<Real> f = fixed + 4 * static_cast<int64_t>(i);
             const Real xk = xs[static_cast<int>(f[3]) * stride];
-            acc += f[0] * expr(-f[1] * xr - f[2] * xk);
+            acc += f[0] * expTerm(f[1] * nx - f[2] * xk);
           }
           Real mult = 1;
           if (r > 0) {
@@ -210,12 +258,13 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, const Glob
         for (int i = 0; i < kMaxAssets; ++i)
           if (i == S.csa_asset) price = S_[i];
       }
+      const bool sec = S.sec_type != CCR_COLLATERAL_CASH;
       if (rec.store_pos >= 0) {
         ringV[rec.store_pos * stride] = V;
-        ringP[rec.store_pos * stride] = price;
+        if (sec) ringP[rec.store_pos * stride] = price;
       }
       const Real vc = ringV[rec.call_pos * stride];
-      const Real pc = ringP[rec.call_pos * stride];
+      const Real pc = sec ? ringP[rec.call_pos * stride] : Real(1);
       const Real keep = static_cast<Real>(S.keep), thc = static_cast<Real>(S.thr_cpty);
       const Real tho = static_cast<Real>(S.thr_own);
       Real target = 0;
@@ -255,6 +304,23 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, const Glob
     prevQo = qo;
     emit(rr, st9, vp);
   }
+}
+
+// Convenience: the tables of set S for path pe, from a PathArgs bundle.
+template <typename Real, typename Emit>
+__host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, Real* col, int stride,
+                                             Emit&& emit) {
+  const CCR_CONST SetDev& S = *sp;
+  cptr<GlobalDev> g = toConst(a.g);
+  const int64_t q = g->antithetic ? (pe >> 1) : pe;
+  simulatePath<Real>(g, sp, pe, a.stream + q * S.stream_stride,
+                     toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride,
+                     toConst(a.dates) + S.date_off,
+                     toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
+                     toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
+                     toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
+                     toConst(a.fixstore) + 2 * S.fixstore_off, a.ring, a.price_rows, col, stride,
+                     emit);
 }
 
 }  // namespace ccrhip

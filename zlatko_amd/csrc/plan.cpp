@@ -291,9 +291,15 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     g.cir_y0[c] = on ? m.credits[static_cast<std::size_t>(c)].y0 : 0.0;
   }
   g.foreign_off = m.foreignOff;
-  for (int i = 0; i < m.ncorr; ++i)
+  // Cholesky rows are stored at fixed factor slots (rates 0.., assets kMaxRates.., credits
+  // kMaxRates + kMaxAssets..) so the kernel addresses every driver with a compile-time index;
+  // columns keep the order of the drawn normals.
+  const int nRates = static_cast<int>(m.rates.size()), nAssets = static_cast<int>(m.spot.size());
+  for (int i = 0; i < m.ncorr; ++i) {
+    const int slot = i < nRates ? i : i < nRates + nAssets ? kMaxRates + (i - nRates) : kMaxRates + kMaxAssets + (i - nRates - nAssets);
     for (int k = 0; k < m.ncorr; ++k)
-      g.chol[i * kMaxCorr + k] = m.chol[static_cast<std::size_t>(i * m.ncorr + k)];
+      g.chol[slot * kMaxCorr + k] = m.chol[static_cast<std::size_t>(i * m.ncorr + k)];
+  }
 
   const int wavesPerBlock = block / 64;
   const int blocksPerSet = (P + block - 1) / block;
@@ -522,6 +528,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     plan.max_dates = std::max(plan.max_dates, T);
     plan.max_rep = std::max(plan.max_rep, R);
     g.max_ring = std::max(g.max_ring, ring);
+    if (ns.has_collateral && ns.collateral.type != CCR_COLLATERAL_CASH) plan.price_rows = std::max(plan.price_rows, ring);
     g.max_slots = std::max(g.max_slots, sd.num_slots);
   }
   const int64_t blocksPaths = cfg.antithetic ? (static_cast<int64_t>(P) + 1) / 2 : P;

@@ -25,6 +25,7 @@ struct HostPlan {
   int64_t stream_len = 0;         // normals in the replayed stream
   int32_t max_dates = 0, max_rep = 0, total_waves = 0;
   int32_t block = 256;
+  int32_t price_rows = 0;         // LDS rows of the collateral price ring (securities CSA only)
   int32_t precision = CCR_FP64;
   double pfe_q = 0.95;
   int64_t revaluations = 0;       // trades x paths x reporting dates

@@ -23,7 +23,7 @@ extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr
     generateStream(cfg->seed, h.stream_len, stream.data());
     std::vector<double> steps = h.steps, dreal = h.date_reals, bonds = h.bonds, fixed = h.fixed;
     PathArgs<double> a{&h.g, h.sets.data(), h.block_set.data(), stream.data(), steps.data(), h.dates.data(),
-                       dreal.data(), bonds.data(), fixed.data(), h.fixstore.data(), nullptr, nullptr,
+                       dreal.data(), bonds.data(), fixed.data(), h.fixstore.data(), h.poly.data(), nullptr, nullptr,
                        h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots};
     const int P = h.g.num_paths;
     std::vector<double> scratch(static_cast<size_t>(h.g.max_ring + h.price_rows + h.g.max_slots + 1));

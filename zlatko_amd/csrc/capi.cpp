@@ -54,6 +54,7 @@ struct ccr_hip_plan {
   void* d_bonds = nullptr;
   void* d_fixed = nullptr;
   int32_t* d_fixstore = nullptr;
+  double* d_poly = nullptr;
   void* d_rows = nullptr;
   double* d_partials = nullptr;
   double* d_stats = nullptr;
@@ -192,6 +193,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   plan->d_bonds = uploadReals(ctx, plan.get(), h.bonds, fp32);
   plan->d_fixed = uploadReals(ctx, plan.get(), h.fixed, fp32);
   plan->d_fixstore = upload(ctx, plan.get(), h.fixstore.data(), h.fixstore.size());
+  plan->d_poly = upload(ctx, plan.get(), h.poly.data(), h.poly.size());
   const size_t elem = fp32 ? sizeof(float) : sizeof(double);
   const int64_t P = h.g.num_paths;
   plan->d_rows = devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * P * elem);
@@ -218,13 +220,13 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
   if (h.precision == CCR_FP32) {
     PathArgs<float> a{plan->d_global, plan->d_sets, plan->d_block_set, static_cast<const float*>(plan->d_stream),
                       static_cast<const float*>(plan->d_steps), plan->d_dates, static_cast<const float*>(plan->d_date_reals),
-                      static_cast<const float*>(plan->d_bonds), static_cast<const float*>(plan->d_fixed), plan->d_fixstore,
+                      static_cast<const float*>(plan->d_bonds), static_cast<const float*>(plan->d_fixed), plan->d_fixstore, plan->d_poly,
                       static_cast<float*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots};
     check(launchPath<float>(a, blocks, h.block, ctx->stream), "path kernel launch");
   } else {
     PathArgs<double> a{plan->d_global, plan->d_sets, plan->d_block_set, static_cast<const double*>(plan->d_stream),
                        static_cast<const double*>(plan->d_steps), plan->d_dates, static_cast<const double*>(plan->d_date_reals),
-                       static_cast<const double*>(plan->d_bonds), static_cast<const double*>(plan->d_fixed), plan->d_fixstore,
+                       static_cast<const double*>(plan->d_bonds), static_cast<const double*>(plan->d_fixed), plan->d_fixstore, plan->d_poly,
                        static_cast<double*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots};
     check(launchPath<double>(a, blocks, h.block, ctx->stream), "path kernel launch");
   }

@@ -35,6 +35,7 @@ struct PathArgs {
   const Real* bonds;
   const Real* fixed;
   const int32_t* fixstore;
+  const double* poly;  // bond-polynomial coefficients (FP64 in both precisions)
   Real* rows;
   double* partials;
   int32_t max_rep;

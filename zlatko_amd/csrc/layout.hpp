@@ -46,16 +46,29 @@ struct DateRec {
   int32_t bond_beg[kMaxRates + 1];   // bond terms of rate r: [bond_beg[r], bond_beg[r + 1])
   int32_t fix_beg[kMaxRates + 1];    // fixed-coupon terms of rate r
   int32_t fixstore_beg, fixstore_end;  // (rate, slot) pairs whose state is recorded at this date
+  // Bond polynomial of rate r: coefficients [poly_beg[r], poly_beg[r] + poly_deg[r]] (-1: none).
+  int32_t poly_beg[kMaxRates];
+  int32_t poly_deg[kMaxRates];
 };
-static_assert(sizeof(DateRec) == 64, "DateRec must stay 64 bytes");
+static_assert(sizeof(DateRec) == 96, "DateRec must stay 96 bytes");
 constexpr int32_t kValued = 1, kReporting = 2, kStoreValue = 4;
 
 // Per-date reals: c0[rate] (constant bond terms), lin[asset] (deterministic-carry forwards),
-// collateral bond factor (A, B).
+// collateral bond factor (A, B), |x_r| bound of each rate's bond polynomial.
 constexpr int kDateC0 = 0;
 constexpr int kDateLin = kMaxRates;
 constexpr int kDateBond = kMaxRates + kMaxAssets;
-constexpr int kDateRealStride = 12;
+constexpr int kDateXmax = kDateBond + 2;
+constexpr int kDateRealStride = 16;
+
+// Bond polynomial. On a valued date the merged bond terms of rate r are an entire function of
+// the state, sum_T c_T A_T e^{-B_T x} = sum_k C_k x^k with C_k = sum_T c_T A_T (-B_T)^k / k!,
+// whose coefficients do not depend on the path. The host truncates the series where its tail is
+// below kPolyTolerance of sum_T |c_T A_T| for |x| <= xmax (a bound of 9 standard deviations plus
+// the drift); a wavefront with any lane beyond xmax evaluates the exponential terms instead.
+constexpr double kPolyTolerance = 1e-18;
+constexpr double kPolySigmas = 9.0;
+constexpr int kPolyMaxDegree = 48;
 
 struct SetDev {
   int64_t stream_stride;   // normals per path block: (T - 1) * nz
@@ -63,7 +76,7 @@ struct SetDev {
   int64_t partial_off;     // first element of this set's wave partials
   int32_t num_dates, num_rep;
   int32_t date_off;        // into the DateRec / date-real / step arrays
-  int32_t bond_off, fixed_off, fixstore_off;
+  int32_t bond_off, fixed_off, fixstore_off, poly_off;
   int32_t block_begin, num_blocks, wave_begin;
   int32_t ring_len, has_csa, sec_type, csa_asset;
   int32_t cpty, own, num_slots;

@@ -63,6 +63,15 @@ __host__ __device__ inline Real expTerm(Real v) {
   if constexpr (std::is_same<Real, double>::value) return expBond(v);
   else return __expf(v);
 }
+// True when the predicate holds on every lane of the wavefront (the path itself on the host).
+__host__ __device__ inline bool allLanes(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __all(p) != 0;
+#else
+  return p;
+#endif
+}
+
 template <typename Real>
 __host__ __device__ inline Real sqrtr(Real v) {
   if constexpr (std::is_same<Real, double>::value) return sqrt(v);
@@ -77,8 +86,8 @@ template <typename Real, typename Emit>
 __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
-                                             cptr<int32_t> fstore, int ring, int priceRows, Real* col, int stride,
-                                             Emit&& emit) {
+                                             cptr<int32_t> fstore, cptr<double> poly, int ring, int priceRows,
+                                             Real* col, int stride, Emit&& emit) {
   constexpr bool kDouble = std::is_same<Real, double>::value;
   const CCR_CONST GlobalDev& g = *gp;
   const CCR_CONST SetDev& S = *sp;
@@ -227,8 +236,19 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
       for (int r = 0; r < kMaxRates; ++r) {
         if (r < nR) {
           const Real nx = -x[r];
-          Real acc = dr[kDateC0 + r];
-          for (int i = rec.bond_beg[r]; i < rec.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expTerm(bonds[2 * i + 1] * nx);
+          Real acc;
+          const int deg = rec.poly_deg[r];
+          if (deg >= 0 && allLanes(static_cast<double>(x[r] < Real(0) ? -x[r] : x[r]) <= dr[kDateXmax + r])) {
+            // Merged bonds (and their constant) as a power series in x, Horner in FP64.
+            cptr<double> pc = poly + rec.poly_beg[r];
+            const double xd = static_cast<double>(x[r]);
+            double h = pc[deg];
+            for (int k = deg - 1; k >= 0; --k) h = fma(h, xd, pc[k]);
+            acc = static_cast<Real>(h);
+          } else {
+            acc = dr[kDateC0 + r];
+            for (int i = rec.bond_beg[r]; i < rec.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expTerm(bonds[2 * i + 1] * nx);
+          }
           for (int i = rec.fix_beg[r]; i < rec.fix_beg[r + 1]; ++i) {
             cptr<Real> f = fixed + 4 * static_cast<int64_t>(i);
             const Real xk = xs[static_cast<int>(f[3]) * stride];
@@ -319,8 +339,8 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDe
                      toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
                      toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
                      toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
-                     toConst(a.fixstore) + 2 * S.fixstore_off, a.ring, a.price_rows, col, stride,
-                     emit);
+                     toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off, a.ring, a.price_rows,
+                     col, stride, emit);
 }
 
 }  // namespace ccrhip

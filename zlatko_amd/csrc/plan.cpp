@@ -229,7 +229,52 @@ void collect(const Trade& tr, const Models& m, const TimeGrid& grid, Collector& 
   }
 }
 
+// Bound on |x_r(t)| used by the bond polynomial: |mean| + kPolySigmas standard deviations of the
+// Gaussian Hull-White state (constant drift `mu` under the simulation measure).
+double stateBound(const HullWhite& hw, double mu, double t) {
+  const double a = hw.a, sd = hw.s * std::sqrt((1.0 - std::exp(-2.0 * a * t)) / (2.0 * a));
+  return std::fabs(mu) * (1.0 - std::exp(-a * t)) / a + kPolySigmas * sd + 1e-12;
+}
+
 }  // namespace
+
+// Power-series form of a date's merged bond terms (cA_i, B_i), i < n, plus the constant c0:
+// coef[k] = [k == 0] c0 + sum_i cA_i (-B_i)^k / k!, truncated at the first degree K whose
+// remainder bound sum_i |cA_i| (B_i X)^{K+1} / (K+1)! e^{B_i X} is below kPolyTolerance times
+// sum_i |cA_i| for |x| <= X. Returns false (no polynomial) when no degree up to kPolyMaxDegree
+// reaches the tolerance. Accumulated in long double.
+bool bondPolynomial(const double* terms, int n, double c0, double xmax, std::vector<double>& coef) {
+  coef.clear();
+  long double gross = 0.0L, bmax = 0.0L;
+  for (int i = 0; i < n; ++i) {
+    gross += std::fabs(static_cast<long double>(terms[2 * i]));
+    bmax = std::max(bmax, static_cast<long double>(std::fabs(terms[2 * i + 1])));
+  }
+  if (n == 0) {
+    coef.push_back(c0);
+    return true;
+  }
+  std::vector<long double> pw(static_cast<std::size_t>(n));  // cA_i (-B_i)^k / k!
+  std::vector<long double> tail(static_cast<std::size_t>(n)); // |cA_i| (B_i X)^{k+1} / (k+1)! e^{B_i X}
+  for (int i = 0; i < n; ++i) {
+    pw[static_cast<std::size_t>(i)] = terms[2 * i];
+    const long double bx = std::fabs(static_cast<long double>(terms[2 * i + 1])) * xmax;
+    tail[static_cast<std::size_t>(i)] = std::fabs(static_cast<long double>(terms[2 * i])) * bx * std::exp(bx);
+  }
+  for (int k = 0; k <= kPolyMaxDegree; ++k) {
+    long double ck = k == 0 ? static_cast<long double>(c0) : 0.0L, rest = 0.0L;
+    for (int i = 0; i < n; ++i) {
+      ck += pw[static_cast<std::size_t>(i)];
+      rest += tail[static_cast<std::size_t>(i)];
+      pw[static_cast<std::size_t>(i)] *= -static_cast<long double>(terms[2 * i + 1]) / (k + 1);
+      tail[static_cast<std::size_t>(i)] *= std::fabs(static_cast<long double>(terms[2 * i + 1])) * xmax / (k + 2);
+    }
+    coef.push_back(static_cast<double>(ck));
+    if (rest <= static_cast<long double>(kPolyTolerance) * gross) return true;
+  }
+  coef.clear();
+  return false;
+}
 
 TimeGrid reportingGrid(const TimeGrid& base, const ccr_netting_set& set) {
   std::vector<double> ev;
@@ -323,6 +368,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     sd.bond_off = static_cast<int32_t>(plan.bonds.size() / 2);
     sd.fixed_off = static_cast<int32_t>(plan.fixed.size() / 4);
     sd.fixstore_off = static_cast<int32_t>(plan.fixstore.size() / 2);
+    sd.poly_off = static_cast<int32_t>(plan.poly.size());
     sd.stream_stride = static_cast<int64_t>(T - 1) * m.nz;
     maxStride = std::max(maxStride, sd.stream_stride);
     sd.block_begin = s * blocksPerSet;
@@ -436,6 +482,20 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
         }
       }
       rec.bond_beg[kMaxRates] = static_cast<int32_t>(plan.bonds.size() / 2) - sd.bond_off;
+      for (int r = 0; r < kMaxRates; ++r) {
+        rec.poly_beg[r] = static_cast<int32_t>(plan.poly.size()) - sd.poly_off;
+        rec.poly_deg[r] = -1;
+        if (r >= g.nrates || !valued[static_cast<std::size_t>(j)]) continue;
+        const int b0 = sd.bond_off + rec.bond_beg[r], b1 = sd.bond_off + rec.bond_beg[r + 1];
+        const double xmax = stateBound(m.rates[static_cast<std::size_t>(r)], m.quanto[static_cast<std::size_t>(r)], t);
+        std::vector<double> coef;
+        if (bondPolynomial(plan.bonds.data() + 2 * b0, b1 - b0, col.c0[r], xmax, coef)) {
+          rec.poly_deg[r] = static_cast<int32_t>(coef.size()) - 1;
+          plan.poly.insert(plan.poly.end(), coef.begin(), coef.end());
+          plan.poly_evaluations += static_cast<int64_t>(coef.size()) * P;
+        }
+        reals[kDateXmax + r] = xmax;
+      }
       for (int r = 0; r < kMaxRates; ++r) {
         rec.fix_beg[r] = static_cast<int32_t>(plan.fixed.size() / 4) - sd.fixed_off;
         if (r < g.nrates) {

@@ -19,6 +19,7 @@ struct HostPlan {
   std::vector<double> date_reals; // [sum T_s][kDateRealStride]
   std::vector<double> bonds;      // [n][2]
   std::vector<double> fixed;      // [n][4]
+  std::vector<double> poly;       // bond-polynomial coefficients, ascending powers per (date, rate)
   std::vector<int32_t> fixstore;  // [n][2] (rate, slot)
   std::vector<int32_t> block_set; // set of each workgroup (before the XCD remap)
   std::vector<std::vector<double>> rep_times;  // reporting dates per set
@@ -30,6 +31,7 @@ struct HostPlan {
   double pfe_q = 0.95;
   int64_t revaluations = 0;       // trades x paths x reporting dates
   int64_t term_evaluations = 0;   // exponential terms x paths over valued dates
+  int64_t poly_evaluations = 0;   // polynomial coefficients x paths over valued dates
   int64_t total_terms = 0;
 };
 
@@ -40,6 +42,9 @@ TimeGrid simulationGrid(const TimeGrid& base, const ccr_netting_set& set);
 HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
                    int32_t precision, int32_t block);
+
+// Power-series coefficients of one date's merged bond terms (see layout.hpp, kPolyTolerance).
+bool bondPolynomial(const double* terms, int n, double c0, double xmax, std::vector<double>& coef);
 
 // The reference normal stream: std::mt19937_64(seed) + std::normal_distribution<double>, as
 // ccr::NormalGenerator (include/ccr/core/random.hpp:13-26), n draws.

@@ -24,7 +24,7 @@ extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr
     std::vector<double> steps = h.steps, dreal = h.date_reals, bonds = h.bonds, fixed = h.fixed;
     PathArgs<double> a{&h.g, h.sets.data(), h.block_set.data(), stream.data(), steps.data(), h.dates.data(),
                        dreal.data(), bonds.data(), fixed.data(), h.fixstore.data(), h.poly.data(), nullptr, nullptr,
-                       h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots};
+                       h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots, h.g.nrates, h.g.nassets, h.g.ncredits};
     const int P = h.g.num_paths;
     std::vector<double> scratch(static_cast<size_t>(h.g.max_ring + h.price_rows + h.g.max_slots + 1));
     for (int s = 0; s < nsets; ++s) {
@@ -35,9 +35,13 @@ extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr
       std::vector<double> rows(static_cast<size_t>(R) * P, 0.0);
       for (int p = 0; p < P; ++p) {
         std::fill(scratch.begin(), scratch.end(), 0.0);
-        simulatePath<double>(a, &S, p, scratch.data(), 1, [&](int rr, const double* st, double vp) {
-          for (int k = 0; k < kStats; ++k) sums[static_cast<size_t>(rr) * kStats + k] += st[k];
-          rows[static_cast<size_t>(rr) * P + p] = vp;
+        // The same shape specialisation the GPU launch picks.
+        withShape(h.g.nrates, h.g.nassets, h.g.ncredits, [&](auto shape) {
+          simulatePath<double, decltype(shape)>(a, &S, p, scratch.data(), 1, [&](int rr, const double* st, double vp) {
+            for (int k = 0; k < kStats; ++k) sums[static_cast<size_t>(rr) * kStats + k] += st[k];
+            rows[static_cast<size_t>(rr) * P + p] = vp;
+          });
+          return 0;
         });
       }
       ccr_exposure_out& o = outs[s];

@@ -221,13 +221,15 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
     PathArgs<float> a{plan->d_global, plan->d_sets, plan->d_block_set, static_cast<const float*>(plan->d_stream),
                       static_cast<const float*>(plan->d_steps), plan->d_dates, static_cast<const float*>(plan->d_date_reals),
                       static_cast<const float*>(plan->d_bonds), static_cast<const float*>(plan->d_fixed), plan->d_fixstore, plan->d_poly,
-                      static_cast<float*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots};
+                      static_cast<float*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots,
+                      h.g.nrates, h.g.nassets, h.g.ncredits};
     check(launchPath<float>(a, blocks, h.block, ctx->stream), "path kernel launch");
   } else {
     PathArgs<double> a{plan->d_global, plan->d_sets, plan->d_block_set, static_cast<const double*>(plan->d_stream),
                        static_cast<const double*>(plan->d_steps), plan->d_dates, static_cast<const double*>(plan->d_date_reals),
                        static_cast<const double*>(plan->d_bonds), static_cast<const double*>(plan->d_fixed), plan->d_fixstore, plan->d_poly,
-                       static_cast<double*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots};
+                       static_cast<double*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots,
+                       h.g.nrates, h.g.nassets, h.g.ncredits};
     check(launchPath<double>(a, blocks, h.block, ctx->stream), "path kernel launch");
   }
   check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");

@@ -26,13 +26,32 @@ __device__ inline int xcdRemap(int b, int nb) {
   return xcd < rem ? xcd * (per + 1) + i : rem * (per + 1) + (xcd - rem) * per + i;
 }
 
-__device__ inline double waveSum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// DPP move of a double (two 32-bit halves) with a given control; lanes of rows outside
+// row_mask keep 0.
+template <int Ctrl, int RowMask>
+__device__ inline double dppMove(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), Ctrl, RowMask, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), Ctrl, RowMask, 0xf, false);
+  return __hiloint2double(hi, lo);
 }
 
-template <typename Real>
+// Sum over the 64 lanes of a wavefront in a fixed order (bit-reproducible), with DPP lane moves
+// only: pairs (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows (row_half_mirror),
+// rows (row_mirror), then row_bcast:15 and row_bcast:31 carry the row sums up; lane 63 ends
+// with the total, which is broadcast.
+__device__ inline double waveSum(double v) {
+  v += dppMove<0xb1, 0xf>(v);
+  v += dppMove<0x4e, 0xf>(v);
+  v += dppMove<0x141, 0xf>(v);
+  v += dppMove<0x140, 0xf>(v);
+  v += dppMove<0x142, 0xa>(v);
+  v += dppMove<0x143, 0xc>(v);
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
+template <typename Real, typename Sh>
 __global__ void __launch_bounds__(kMaxBlock) ccr_path_kernel(PathArgs<Real> a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const CCR_CONST GlobalDev& g = *toConst(a.g);
@@ -48,7 +67,7 @@ __global__ void __launch_bounds__(kMaxBlock) ccr_path_kernel(PathArgs<Real> a) {
   double* part = a.partials + S.partial_off + static_cast<int64_t>(waveLocal) * a.max_rep * kStats;
   Real* rows = a.rows + S.rows_off;
   Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
-  simulatePath<Real>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
+  simulatePath<Real, Sh>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
     if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
 #pragma unroll
     for (int k = 0; k < kStats; ++k) {
@@ -164,11 +183,19 @@ __global__ void __launch_bounds__(256) ccr_pfe_kernel(const SetDev* sets, const 
 
 // ---------------------------------------------------------------- launchers (host side)
 
+template <typename Real, typename Sh>
+hipError_t launchShape(const PathArgs<Real>& args, int blocks, int block, size_t lds, hipStream_t stream) {
+  hipLaunchKernelGGL((ccr_path_kernel<Real, Sh>), dim3(blocks), dim3(block), lds, stream, args);
+  return hipGetLastError();
+}
+
+// Picks the path-kernel specialisation for the model shape (rates, assets, credits).
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
   const size_t lds = static_cast<size_t>(args.ring + args.price_rows + args.slots) * block * sizeof(Real);
-  hipLaunchKernelGGL(ccr_path_kernel<Real>, dim3(blocks), dim3(block), lds, stream, args);
-  return hipGetLastError();
+  return withShape(args.nrates, args.nassets, args.ncredits, [&](auto shape) {
+    return launchShape<Real, decltype(shape)>(args, blocks, block, lds, stream);
+  });
 }
 
 hipError_t launchReduce(const SetDev* sets, const double* partials, int nsets, int max_rep, int waves_per_set,

@@ -42,6 +42,7 @@ struct PathArgs {
   int32_t ring;        // LDS rows of the margin-call value ring
   int32_t price_rows;  // LDS rows of the collateral price ring (0 for cash collateral)
   int32_t slots;       // LDS rows of fixing states
+  int32_t nrates, nassets, ncredits;  // model shape: selects the kernel specialisation
 };
 
 template <typename Real>

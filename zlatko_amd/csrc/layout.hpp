@@ -46,9 +46,10 @@ struct DateRec {
   int32_t bond_beg[kMaxRates + 1];   // bond terms of rate r: [bond_beg[r], bond_beg[r + 1])
   int32_t fix_beg[kMaxRates + 1];    // fixed-coupon terms of rate r
   int32_t fixstore_beg, fixstore_end;  // (rate, slot) pairs whose state is recorded at this date
-  // Bond polynomial of rate r: coefficients [poly_beg[r], poly_beg[r] + poly_deg[r]] (-1: none).
+  // Bond polynomial of rate r: poly_len[r] coefficients (a multiple of 4, highest power first,
+  // zero-padded at the top) from poly_beg[r]; 0 = none.
   int32_t poly_beg[kMaxRates];
-  int32_t poly_deg[kMaxRates];
+  int32_t poly_len[kMaxRates];
 };
 static_assert(sizeof(DateRec) == 96, "DateRec must stay 96 bytes");
 constexpr int32_t kValued = 1, kReporting = 2, kStoreValue = 4;

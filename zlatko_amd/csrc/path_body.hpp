@@ -9,12 +9,14 @@
 //      drift), deflator D *= exp(-int r) (scenario_generator.cpp:126-128)
 //   -> lognormal assets (lognormal_asset.cpp:17-22), carry from a curve or a foreign rate
 //   -> CIR++ with Brownian-bridge sub-steps and pathwise survival (cir_intensity.cpp:45-66)
-//   -> on valued dates V = sum over rate groups of FX x (c0 + sum c A e^{-B x} + fixed coupons)
+//   -> on valued dates V = sum over rate groups of FX x (bond polynomial or merged bond terms
+//      + fixed coupons) + deterministic-carry forwards
 //   -> CSA (collateral.cpp:10-47) with the margin-call value read from a per-path ring
 //   -> on reporting dates, the 9 statistics of ExposureProfile::compute and pathwiseCva.
 //
-// The plan tables are passed as __restrict__ read-only pointers: on the GPU the per-date term
-// records are then read with scalar (uniform) loads, shared by the 64 lanes of a wavefront.
+// The model shape (rates NR, assets NA, credits NC) is a template parameter: with kExact the
+// counts are exactly those, so register arrays and loops are sized at compile time; the generic
+// instantiation takes the maxima and reads the counts from the plan.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -71,45 +73,73 @@ __host__ __device__ inline bool allLanes(bool p) {
   return p;
 #endif
 }
-
 template <typename Real>
 __host__ __device__ inline Real sqrtr(Real v) {
   if constexpr (std::is_same<Real, double>::value) return sqrt(v);
   else return __fsqrt_rn(v);
 }
 
+// Model shapes the path kernel is specialised for; everything else runs the generic shape.
+template <int NR_, int NA_, int NC_, bool Exact_>
+struct Shape {
+  static constexpr int NR = NR_, NA = NA_, NC = NC_;
+  static constexpr bool kExact = Exact_;
+  static constexpr int kCorr = NR + NA + NC;  // correlated drivers
+  static constexpr int kPre = kCorr + NR;     // prefetched normals: drivers + one integral per rate
+};
+using GenericShape = Shape<kMaxRates, kMaxAssets, kMaxCredits, false>;
+
+// Calls f(Shape{}) with the specialisation for (rates, assets, credits), or the generic shape.
+template <typename F>
+inline auto withShape(int nrates, int nassets, int ncredits, F&& f) {
+  switch (nrates * 100 + nassets * 10 + ncredits) {
+    case 100: return f(Shape<1, 0, 0, true>{});
+    case 101: return f(Shape<1, 0, 1, true>{});
+    case 102: return f(Shape<1, 0, 2, true>{});
+    case 110: return f(Shape<1, 1, 0, true>{});
+    case 111: return f(Shape<1, 1, 1, true>{});
+    case 112: return f(Shape<1, 1, 2, true>{});
+    case 320: return f(Shape<3, 2, 0, true>{});
+    case 321: return f(Shape<3, 2, 1, true>{});
+    case 322: return f(Shape<3, 2, 2, true>{});
+    default: return f(GenericShape{});
+  }
+}
+
 // col: this path's column of per-path scratch (LDS on the GPU), stride: distance between rows;
 // rows [0, ring) hold the margin-call values, [ring, ring + priceRows) the collateral prices,
 // then the fixing states. emit(rep, stats[kStats], positive_exposure) is called on every
 // reporting date.
-template <typename Real, typename Emit>
+template <typename Real, typename Sh, typename Emit>
 __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
                                              cptr<int32_t> fstore, cptr<double> poly, int ring, int priceRows,
                                              Real* col, int stride, Emit&& emit) {
   constexpr bool kDouble = std::is_same<Real, double>::value;
+  constexpr int NR = Sh::NR, NA = Sh::NA, NC = Sh::NC;
   const CCR_CONST GlobalDev& g = *gp;
   const CCR_CONST SetDev& S = *sp;
   Real* ringV = col;
   Real* ringP = col + ring * stride;
   Real* xs = col + (ring + priceRows) * stride;
-  const int nR = g.nrates, nA = g.nassets, nC = g.ncredits, nz = g.nz, nCorr = g.ncorr;
+  const int nR = Sh::kExact ? NR : g.nrates, nA = Sh::kExact ? NA : g.nassets, nC = Sh::kExact ? NC : g.ncredits;
+  const int nz = g.nz, nCorr = Sh::kExact ? Sh::kCorr : g.ncorr;
   const Real sgn = (g.antithetic && (pe & 1)) ? Real(-1) : Real(1);
 
-  Real x[kMaxRates], S_[kMaxAssets], y[kMaxCredits], iy[kMaxCredits], Qs[kMaxCredits];
+  Real x[NR], S_[NA > 0 ? NA : 1], y[NC > 0 ? NC : 1], iy[NC > 0 ? NC : 1], Qs[NC > 0 ? NC : 1];
   Real lnD = 0, lnDc = 0, D = 1;  // FP32: Kahan sum of int r; FP64: running product
-  Real lnS[kMaxAssets], lnSc[kMaxAssets];
+  Real lnS[NA > 0 ? NA : 1], lnSc[NA > 0 ? NA : 1];
 #pragma unroll
-  for (int r = 0; r < kMaxRates; ++r) x[r] = 0;
+  for (int r = 0; r < NR; ++r) x[r] = 0;
 #pragma unroll
-  for (int i = 0; i < kMaxAssets; ++i) {
+  for (int i = 0; i < NA; ++i) {
     S_[i] = static_cast<Real>(g.spot[i]);
     lnS[i] = 0;
     lnSc[i] = 0;
   }
 #pragma unroll
-  for (int c = 0; c < kMaxCredits; ++c) {
+  for (int c = 0; c < NC; ++c) {
     y[c] = static_cast<Real>(g.cir_y0[c]);
     iy[c] = 0;
     Qs[c] = 1;
@@ -117,44 +147,56 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
   Real units = 0;
   double prevDvp = 0, prevDvn = 0, prevQc = 1, prevQo = 1;
 
+  // Normals of the next step, loaded one step ahead: correlated drivers, then the integral
+  // normal of each rate (domestic right after the drivers, foreign ones at foreign_off).
+  Real zn[Sh::kPre];
+  auto loadStep = [&](int s) {
+    const Real* z = zbase + static_cast<int64_t>(s) * nz;
+#pragma unroll
+    for (int i = 0; i < Sh::kCorr; ++i) zn[i] = i < nCorr ? z[i] : Real(0);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) zn[Sh::kCorr + r] = r < nR ? z[r == 0 ? nCorr : g.foreign_off + r - 1] : Real(0);
+  };
+  if (S.num_dates > 1) loadStep(0);
+
   for (int j = 0; j < S.num_dates; ++j) {
     if (j > 0) {
       cptr<Real> st = steps + static_cast<int64_t>(j) * kStepStride;
       const Real* z = zbase + static_cast<int64_t>(j - 1) * nz;
+      Real zc[Sh::kPre];
+#pragma unroll
+      for (int i = 0; i < Sh::kPre; ++i) zc[i] = sgn * zn[i];
+      if (j + 1 < S.num_dates) loadStep(j);
       // Correlated drivers at fixed factor slots: rates [0, kMaxRates), assets, credits. Row
       // `slot` of the factor holds compact row `lim` (lower triangular: columns k <= lim).
-      Real w[kMaxCorr];
+      Real w[Sh::kCorr];
       {
         cptr<double> chol = g.chol;
 #if defined(__HIP_DEVICE_COMPILE__)
         // Opaque per step: the factors are re-read through the scalar cache each step instead of
-        // being hoisted out of the date loop into ~100 registers.
+        // being hoisted out of the date loop into registers.
         asm volatile("" : "+s"(chol));
 #endif
-        Real zc[kMaxCorr];
 #pragma unroll
-        for (int i = 0; i < kMaxCorr; ++i) zc[i] = i < nCorr ? sgn * z[i] : Real(0);
-#pragma unroll
-        for (int i = 0; i < kMaxCorr; ++i) {
-          const int lim = i < kMaxRates ? i : i < kMaxRates + kMaxAssets ? nR + (i - kMaxRates)
-                                                                          : nR + nA + (i - kMaxRates - kMaxAssets);
-          const bool used = i < kMaxRates ? i < nR : i < kMaxRates + kMaxAssets ? i - kMaxRates < nA
-                                                                                 : i - kMaxRates - kMaxAssets < nC;
+        for (int i = 0; i < Sh::kCorr; ++i) {
+          const int slot = i < NR ? i : i < NR + NA ? kMaxRates + (i - NR) : kMaxRates + kMaxAssets + (i - NR - NA);
+          const int lim = i < NR ? i : i < NR + NA ? nR + (i - NR) : nR + nA + (i - NR - NA);
+          const bool used = i < NR ? i < nR : i < NR + NA ? i - NR < nA : i - NR - NA < nC;
           Real sum = 0;
           if (used) {
 #pragma unroll
-            for (int k = 0; k < kMaxCorr; ++k)
-              if (k <= lim) sum += static_cast<Real>(chol[i * kMaxCorr + k]) * zc[k];
+            for (int k = 0; k < Sh::kCorr; ++k)
+              if (k <= lim) sum += static_cast<Real>(chol[slot * kMaxCorr + k]) * zc[k];
           }
           w[i] = sum;
         }
       }
-      Real ir[kMaxRates];
+      Real ir[NR];
 #pragma unroll
-      for (int r = 0; r < kMaxRates; ++r) {
+      for (int r = 0; r < NR; ++r) {
         if (r < nR) {
-          cptr<Real> o = st +kStepRate + 8 * r;
-          const Real z2 = sgn * (r == 0 ? z[nCorr] : z[g.foreign_off + r - 1]);
+          cptr<Real> o = st + kStepRate + 8 * r;
+          const Real z2 = zc[Sh::kCorr + r];
           const Real x0 = x[r];
           x[r] = x0 * o[0] + o[1] * w[r] + o[5];
           const Real ix = x0 * o[4] + o[2] * w[r] + o[3] * z2 + o[6];
@@ -173,15 +215,15 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
         D = expr(-lnD);
       }
 #pragma unroll
-      for (int i = 0; i < kMaxAssets; ++i) {
+      for (int i = 0; i < NA; ++i) {
         if (i < nA) {
-          cptr<Real> o = st +kStepAsset + 3 * i;
+          cptr<Real> o = st + kStepAsset + 3 * i;
           const int rr = g.asset_rate[i];
           Real icarry = o[0];
 #pragma unroll
-          for (int r = 1; r < kMaxRates; ++r)
+          for (int r = 1; r < NR; ++r)
             if (r == rr) icarry = ir[r];
-          const Real drift = ir[0] - icarry - o[2] + o[1] * w[kMaxRates + i];
+          const Real drift = ir[0] - icarry - o[2] + o[1] * w[NR + i];
           if constexpr (kDouble) {
             S_[i] = S_[i] * exp(drift);
           } else {
@@ -194,14 +236,14 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
         }
       }
 #pragma unroll
-      for (int c = 0; c < kMaxCredits; ++c) {
+      for (int c = 0; c < NC; ++c) {
         if (c < nC) {
-          cptr<Real> o = st +kStepCredit + kStepCreditStride * c;
+          cptr<Real> o = st + kStepCredit + kStepCreditStride * c;
           const int sub = g.credit_sub[c];
           const Real* ex = z + g.credit_off[c];
           const Real kappa = static_cast<Real>(g.cir_kappa[c]), theta = static_cast<Real>(g.cir_theta[c]);
           const Real xi = static_cast<Real>(g.cir_xi[c]), h = o[1];
-          Real remaining = o[0] * w[kMaxRates + kMaxAssets + c];
+          Real remaining = o[0] * w[NR + NA + c];
           Real yc = y[c], iyc = iy[c];
           for (int k = 0; k < sub; ++k) {
             Real dW = remaining;
@@ -224,7 +266,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
       const int r = fstore[2 * i], slot = fstore[2 * i + 1];
       Real xv = x[0];
 #pragma unroll
-      for (int k = 1; k < kMaxRates; ++k)
+      for (int k = 1; k < NR; ++k)
         if (k == r) xv = x[k];
       xs[slot * stride] = xv;
     }
@@ -233,17 +275,24 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     Real V = 0;
     if (rec.flags & kValued) {
 #pragma unroll
-      for (int r = 0; r < kMaxRates; ++r) {
+      for (int r = 0; r < NR; ++r) {
         if (r < nR) {
           const Real nx = -x[r];
           Real acc;
-          const int deg = rec.poly_deg[r];
-          if (deg >= 0 && allLanes(static_cast<double>(x[r] < Real(0) ? -x[r] : x[r]) <= dr[kDateXmax + r])) {
-            // Merged bonds (and their constant) as a power series in x, Horner in FP64.
+          const int len = rec.poly_len[r];
+          if (len > 0 && allLanes(static_cast<double>(x[r] < Real(0) ? -x[r] : x[r]) <= dr[kDateXmax + r])) {
+            // Merged bonds (and their constant) as a power series in x: Horner in FP64 over the
+            // coefficients stored highest power first, four per scalar load.
             cptr<double> pc = poly + rec.poly_beg[r];
             const double xd = static_cast<double>(x[r]);
-            double h = pc[deg];
-            for (int k = deg - 1; k >= 0; --k) h = fma(h, xd, pc[k]);
+            double h = 0.0;
+            for (int k = 0; k < len; k += 4) {
+              const double c0 = pc[k], c1 = pc[k + 1], c2 = pc[k + 2], c3 = pc[k + 3];
+              h = fma(h, xd, c0);
+              h = fma(h, xd, c1);
+              h = fma(h, xd, c2);
+              h = fma(h, xd, c3);
+            }
             acc = static_cast<Real>(h);
           } else {
             acc = dr[kDateC0 + r];
@@ -258,14 +307,14 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
           if (r > 0) {
             const int fa = g.fx_of[r];
 #pragma unroll
-            for (int i = 0; i < kMaxAssets; ++i)
+            for (int i = 0; i < NA; ++i)
               if (i == fa) mult = S_[i];
           }
           V += mult * acc;
         }
       }
 #pragma unroll
-      for (int i = 0; i < kMaxAssets; ++i)
+      for (int i = 0; i < NA; ++i)
         if (i < nA) V += S_[i] * dr[kDateLin + i];
     }
     Real C = 0;
@@ -275,7 +324,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
         price = dr[kDateBond] * expr(-dr[kDateBond + 1] * x[0]);
       } else if (S.sec_type == CCR_COLLATERAL_ASSET) {
 #pragma unroll
-        for (int i = 0; i < kMaxAssets; ++i)
+        for (int i = 0; i < NA; ++i)
           if (i == S.csa_asset) price = S_[i];
       }
       const bool sec = S.sec_type != CCR_COLLATERAL_CASH;
@@ -303,7 +352,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     const double dvp = d * vp, dvn = d * (-vn);
     double qc = 1.0, qo = 1.0;
 #pragma unroll
-    for (int c = 0; c < kMaxCredits; ++c) {
+    for (int c = 0; c < NC; ++c) {
       if (c == S.cpty) qc = static_cast<double>(Qs[c]);
       if (c == S.own) qo = static_cast<double>(Qs[c]);
     }
@@ -327,20 +376,20 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 }
 
 // Convenience: the tables of set S for path pe, from a PathArgs bundle.
-template <typename Real, typename Emit>
+template <typename Real, typename Sh, typename Emit>
 __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, Real* col, int stride,
                                              Emit&& emit) {
   const CCR_CONST SetDev& S = *sp;
   cptr<GlobalDev> g = toConst(a.g);
   const int64_t q = g->antithetic ? (pe >> 1) : pe;
-  simulatePath<Real>(g, sp, pe, a.stream + q * S.stream_stride,
-                     toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride,
-                     toConst(a.dates) + S.date_off,
-                     toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
-                     toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
-                     toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
-                     toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off, a.ring, a.price_rows,
-                     col, stride, emit);
+  simulatePath<Real, Sh>(g, sp, pe, a.stream + q * S.stream_stride,
+                         toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride,
+                         toConst(a.dates) + S.date_off,
+                         toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
+                         toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
+                         toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
+                         toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off, a.ring, a.price_rows,
+                         col, stride, emit);
 }
 
 }  // namespace ccrhip

@@ -484,15 +484,17 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       rec.bond_beg[kMaxRates] = static_cast<int32_t>(plan.bonds.size() / 2) - sd.bond_off;
       for (int r = 0; r < kMaxRates; ++r) {
         rec.poly_beg[r] = static_cast<int32_t>(plan.poly.size()) - sd.poly_off;
-        rec.poly_deg[r] = -1;
+        rec.poly_len[r] = 0;
         if (r >= g.nrates || !valued[static_cast<std::size_t>(j)]) continue;
         const int b0 = sd.bond_off + rec.bond_beg[r], b1 = sd.bond_off + rec.bond_beg[r + 1];
         const double xmax = stateBound(m.rates[static_cast<std::size_t>(r)], m.quanto[static_cast<std::size_t>(r)], t);
         std::vector<double> coef;
         if (bondPolynomial(plan.bonds.data() + 2 * b0, b1 - b0, col.c0[r], xmax, coef)) {
-          rec.poly_deg[r] = static_cast<int32_t>(coef.size()) - 1;
-          plan.poly.insert(plan.poly.end(), coef.begin(), coef.end());
-          plan.poly_evaluations += static_cast<int64_t>(coef.size()) * P;
+          const std::size_t len = (coef.size() + 3) / 4 * 4;  // highest power first, zero-padded on top
+          plan.poly.insert(plan.poly.end(), len - coef.size(), 0.0);
+          plan.poly.insert(plan.poly.end(), coef.rbegin(), coef.rend());
+          rec.poly_len[r] = static_cast<int32_t>(len);
+          plan.poly_evaluations += static_cast<int64_t>(len) * P;
         }
         reals[kDateXmax + r] = xmax;
       }

@@ -70,19 +70,27 @@ def test_fp32_error_vs_fp64_oracle(engine):
         assert errs["expected_exposure"] < 1e-4 and errs["discounted_expected_exposure"] < 1e-4
 
 
-def test_pfe_is_exact_order_statistic(engine):
-    """The GPU PFE equals ccr::quantile of the GPU's own exposure rows, bit for bit."""
-    spec = W.small_reference_case(num_paths=777, csa=True, fx=True)
-    out = engine.run(spec, rows=True)[0]
+@pytest.mark.parametrize("paths,precision", [(1, abi.CCR_FP64), (2, abi.CCR_FP64), (3, abi.CCR_FP64),
+                                             (777, abi.CCR_FP64), (2049, abi.CCR_FP64), (5001, abi.CCR_FP64),
+                                             (777, abi.CCR_FP32), (5001, abi.CCR_FP32)])
+def test_pfe_is_exact_order_statistic(engine, paths, precision):
+    """The GPU PFE equals ccr::quantile of the GPU's own exposure rows, bit for bit. Rows include
+    all-equal values (t = 0: more ties than the LDS candidate buffer when P > 2048) and
+    all-zero rows (after the last cash flow)."""
+    spec = W.small_reference_case(num_paths=paths, csa=True, fx=True)
+    out = engine.run(spec, precision, rows=True)[0]
     rows = out.exposure_rows(spec.num_paths)
     pfe = out.get("potential_future_exposure")
-    for r in range(rows.shape[0]):
-        v = np.sort(rows[r])
-        h = spec.pfe_quantile * (len(v) - 1)
-        lo = int(np.floor(h))
-        hi = min(lo + 1, len(v) - 1)
-        expect = v[lo] + (h - lo) * (v[hi] - v[lo])
-        assert pfe[r] == expect, (r, pfe[r], expect)
+    for q_level in (spec.pfe_quantile,):
+        for r in range(rows.shape[0]):
+            v = np.sort(rows[r])
+            if precision == abi.CCR_FP32:
+                v = v.astype(np.float32).astype(np.float64)
+            h = q_level * (len(v) - 1)
+            lo = int(np.floor(h))
+            hi = min(lo + 1, len(v) - 1)
+            expect = v[lo] + (h - lo) * (v[hi] - v[lo])
+            assert pfe[r] == expect, (r, pfe[r], expect)
 
 
 def test_bit_reproducible_across_runs_and_block_sizes():

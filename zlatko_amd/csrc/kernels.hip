@@ -108,17 +108,42 @@ __device__ inline Real valueOf(typename KeyOf<Real>::T k) {
   else return __uint_as_float(k);
 }
 
-// One workgroup (256 threads) per (set, reporting date) row.
+constexpr int kPfeThreads = 256;
+constexpr int kPfeBins = 4096;   // 12-bit digits
+constexpr int kPfeCand = 2048;   // candidates finished by a bitonic sort in LDS
+
+// Finds the bin of hist[0, kPfeBins) holding rank *rank (0-based); leaves the bin in *bin and the
+// rank within it in *rank. One thread scans (the histogram is small next to a row).
+__device__ inline void pfeFindBin(const unsigned int* hist, long long* rank, int* bin) {
+  long long r = *rank;
+  int b = 0;
+  for (; b < kPfeBins - 1; ++b) {
+    if (r < static_cast<long long>(hist[b])) break;
+    r -= hist[b];
+  }
+  *rank = r;
+  *bin = b;
+}
+
+// One workgroup per (set, reporting date) row. Exact order statistics lo = floor(h), hi = lo + 1
+// of the row's keys (h = q (P - 1), ccr::quantile): a histogram of the top 12 key bits (with the
+// count of zero exposures and the smallest positive key, which settle a quantile inside the
+// zeros at once), a histogram of the next 12 bits inside the selected bin, then the keys sharing
+// the selected 24-bit prefix are gathered and bitonic-sorted in LDS; the next key above the
+// prefix, found in the same pass, gives the upper statistic when it lies outside the bin. Bins
+// with more than kPfeCand keys (ties) fall back to 8-bit radix passes over the remaining bits.
 template <typename Real>
-__global__ void __launch_bounds__(256) ccr_pfe_kernel(const SetDev* sets, const Real* rows, int max_rep, int num_paths,
-                                                      double level, double* out) {
+__global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets, const Real* rows, int max_rep,
+                                                              int num_paths, double level, double* out) {
   using K = typename KeyOf<Real>::T;
   constexpr int kBits = KeyOf<Real>::kBits;
-  __shared__ unsigned int hist[256];
-  __shared__ K sPrefix;
+  constexpr int shiftA = kBits - 12, shiftB = kBits - 24;
+  __shared__ unsigned int hist[kPfeBins];
+  __shared__ K cand[kPfeCand];
+  __shared__ K sPrefix, sMinPos, sMinAbove;
   __shared__ long long sRank;
-  __shared__ unsigned int sCount;
-  __shared__ K sMin;
+  __shared__ int sBin;
+  __shared__ unsigned int sCount, sZero;
   const int s = static_cast<int>(blockIdx.x) / max_rep, rr = static_cast<int>(blockIdx.x) % max_rep;
   const SetDev& S = sets[s];
   if (rr >= S.num_rep) return;
@@ -127,28 +152,117 @@ __global__ void __launch_bounds__(256) ccr_pfe_kernel(const SetDev* sets, const 
   const double h = level * static_cast<double>(num_paths - 1);
   const long long lo = static_cast<long long>(floor(h));
   const long long hi = lo + 1 < num_paths ? lo + 1 : num_paths - 1;
+  auto finish = [&](K kLo, K kHi) {
+    const double vlo = static_cast<double>(valueOf<Real>(kLo)), vhi = static_cast<double>(valueOf<Real>(kHi));
+    out[s * max_rep + rr] = vlo + (h - static_cast<double>(lo)) * (vhi - vlo);
+  };
 
+  // Pass A: top 12 bits, zero count, smallest positive key.
+  for (int b = tid; b < kPfeBins; b += kPfeThreads) hist[b] = 0;
   if (tid == 0) {
-    sPrefix = 0;
+    sZero = 0;
+    sMinPos = ~K(0);
     sRank = lo;
   }
   __syncthreads();
-  K mask = 0;
-  for (int shift = kBits - 8; shift >= 0; shift -= 8) {
-    hist[tid] = 0;
+  {
+    unsigned int zeros = 0;
+    K minPos = ~K(0);
+    for (int i = tid; i < num_paths; i += kPfeThreads) {
+      const K k = keyOf<Real>(row[i]);
+      if (k == 0) ++zeros;
+      else if (k < minPos) minPos = k;
+      atomicAdd(&hist[k >> shiftA], 1u);
+    }
+    atomicAdd(&sZero, zeros);
+    atomicMin(&sMinPos, minPos);
+  }
+  __syncthreads();
+  if (lo < static_cast<long long>(sZero)) {  // the quantile lies in the zero exposures
+    if (tid == 0) finish(K(0), (hi == lo || hi < static_cast<long long>(sZero)) ? K(0) : sMinPos);
+    return;
+  }
+  if (tid == 0) pfeFindBin(hist, &sRank, &sBin);
+  __syncthreads();
+  const K binA = static_cast<K>(sBin);
+
+  // Pass B: the next 12 bits inside the selected bin.
+  for (int b = tid; b < kPfeBins; b += kPfeThreads) hist[b] = 0;
+  __syncthreads();
+  for (int i = tid; i < num_paths; i += kPfeThreads) {
+    const K k = keyOf<Real>(row[i]);
+    if ((k >> shiftA) == binA) atomicAdd(&hist[(k >> shiftB) & 0xFFF], 1u);
+  }
+  __syncthreads();
+  if (tid == 0) pfeFindBin(hist, &sRank, &sBin);
+  __syncthreads();
+  const K prefix24 = (binA << 12) | static_cast<K>(sBin);
+  const unsigned int count = hist[sBin];
+
+  if (count <= static_cast<unsigned int>(kPfeCand)) {
+    // Pass C: gather the bin's keys and the smallest key above the prefix; sort in LDS.
+    if (tid == 0) {
+      sCount = 0;
+      sMinAbove = ~K(0);
+    }
+    __syncthreads();
+    K minAbove = ~K(0);
+    for (int i = tid; i < num_paths; i += kPfeThreads) {
+      const K k = keyOf<Real>(row[i]);
+      const K p = k >> shiftB;
+      if (p == prefix24) cand[atomicAdd(&sCount, 1u)] = k;
+      else if (p > prefix24 && k < minAbove) minAbove = k;
+    }
+    atomicMin(&sMinAbove, minAbove);
+    __syncthreads();
+    int n2 = 2;
+    while (n2 < static_cast<int>(count)) n2 <<= 1;
+    for (int i = static_cast<int>(count) + tid; i < n2; i += kPfeThreads) cand[i] = ~K(0);
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1)
+      for (int jj = k >> 1; jj > 0; jj >>= 1) {
+        for (int i = tid; i < n2; i += kPfeThreads) {
+          const int ixj = i ^ jj;
+          if (ixj > i) {
+            const bool asc = (i & k) == 0;
+            const K a = cand[i], b = cand[ixj];
+            if ((a > b) == asc) {
+              cand[i] = b;
+              cand[ixj] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    if (tid == 0) {
+      const long long r = sRank;
+      const K kLo = cand[r];
+      const K kHi = hi == lo ? kLo : (r + 1 < static_cast<long long>(count) ? cand[r + 1] : sMinAbove);
+      finish(kLo, kHi);
+    }
+    return;
+  }
+
+  // Fallback for bins of ties: 8-bit radix passes over the low bits, then the upper statistic.
+  if (tid == 0) sPrefix = prefix24 << shiftB;
+  __syncthreads();
+  unsigned int* hist8 = hist;
+  K mask = ~K(0) << shiftB;
+  for (int shift = shiftB - 8; shift >= 0; shift -= 8) {
+    hist8[tid] = 0;
     __syncthreads();
     const K prefix = sPrefix;
-    for (int i = tid; i < num_paths; i += 256) {
+    for (int i = tid; i < num_paths; i += kPfeThreads) {
       const K k = keyOf<Real>(row[i]);
-      if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 0xFF], 1u);
+      if ((k & mask) == prefix) atomicAdd(&hist8[(k >> shift) & 0xFF], 1u);
     }
     __syncthreads();
     if (tid == 0) {
       long long rank = sRank;
       int bin = 0;
       for (; bin < 255; ++bin) {
-        if (rank < static_cast<long long>(hist[bin])) break;
-        rank -= hist[bin];
+        if (rank < static_cast<long long>(hist8[bin])) break;
+        rank -= hist8[bin];
       }
       sRank = rank;
       sPrefix = prefix | (static_cast<K>(bin) << shift);
@@ -160,25 +274,20 @@ __global__ void __launch_bounds__(256) ccr_pfe_kernel(const SetDev* sets, const 
   // Upper order statistic: equal to the lower one if enough keys are <= it, else the next key above.
   if (tid == 0) {
     sCount = 0;
-    sMin = ~K(0);
+    sMinAbove = ~K(0);
   }
   __syncthreads();
   unsigned int cnt = 0;
   K mn = ~K(0);
-  for (int i = tid; i < num_paths; i += 256) {
+  for (int i = tid; i < num_paths; i += kPfeThreads) {
     const K k = keyOf<Real>(row[i]);
     if (k <= kLo) ++cnt;
     else if (k < mn) mn = k;
   }
   atomicAdd(&sCount, cnt);
-  if constexpr (kBits == 64) atomicMin(reinterpret_cast<unsigned long long*>(&sMin), static_cast<unsigned long long>(mn));
-  else atomicMin(reinterpret_cast<unsigned int*>(&sMin), static_cast<unsigned int>(mn));
+  atomicMin(&sMinAbove, mn);
   __syncthreads();
-  if (tid == 0) {
-    const K kHi = (hi == lo || static_cast<long long>(sCount) >= hi + 1) ? kLo : sMin;
-    const double vlo = static_cast<double>(valueOf<Real>(kLo)), vhi = static_cast<double>(valueOf<Real>(kHi));
-    out[s * max_rep + rr] = vlo + (h - static_cast<double>(lo)) * (vhi - vlo);
-  }
+  if (tid == 0) finish(kLo, (hi == lo || static_cast<long long>(sCount) >= hi + 1) ? kLo : sMinAbove);
 }
 
 // ---------------------------------------------------------------- launchers (host side)

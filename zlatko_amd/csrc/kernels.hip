@@ -154,7 +154,8 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
   const long long hi = lo + 1 < num_paths ? lo + 1 : num_paths - 1;
   auto finish = [&](K kLo, K kHi) {
     const double vlo = static_cast<double>(valueOf<Real>(kLo)), vhi = static_cast<double>(valueOf<Real>(kHi));
-    out[s * max_rep + rr] = vlo + (h - static_cast<double>(lo)) * (vhi - vlo);
+    // Rounded multiply then add, as ccr::quantile evaluates it (no fused multiply-add).
+    out[s * max_rep + rr] = __dadd_rn(vlo, __dmul_rn(h - static_cast<double>(lo), vhi - vlo));
   };
 
   // Pass A: top 12 bits, zero count, smallest positive key.

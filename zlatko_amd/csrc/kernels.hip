@@ -108,6 +108,15 @@ __device__ inline Real valueOf(typename KeyOf<Real>::T k) {
   else return __uint_as_float(k);
 }
 
+// lo + t (hi - lo) with a rounded multiply and a rounded add, as ccr::quantile evaluates it
+// (contraction into a fused multiply-add would change the last bit).
+__device__ __noinline__ double lerpUnfused(double lo, double t, double hi) {
+#pragma clang fp contract(off)
+  const double d = hi - lo;
+  const double m = t * d;
+  return lo + m;
+}
+
 constexpr int kPfeThreads = 256;
 constexpr int kPfeBins = 4096;   // 12-bit digits
 constexpr int kPfeCand = 2048;   // candidates finished by a bitonic sort in LDS
@@ -154,8 +163,7 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
   const long long hi = lo + 1 < num_paths ? lo + 1 : num_paths - 1;
   auto finish = [&](K kLo, K kHi) {
     const double vlo = static_cast<double>(valueOf<Real>(kLo)), vhi = static_cast<double>(valueOf<Real>(kHi));
-    // Rounded multiply then add, as ccr::quantile evaluates it (no fused multiply-add).
-    out[s * max_rep + rr] = __dadd_rn(vlo, __dmul_rn(h - static_cast<double>(lo), vhi - vlo));
+    out[s * max_rep + rr] = lerpUnfused(vlo, h - static_cast<double>(lo), vhi);
   };
 
   // Pass A: top 12 bits, zero count, smallest positive key.

@@ -228,6 +228,7 @@ typedef struct ccr_hip_timing {
   double total_ms;       /* first launch start to last launch end */
   double upload_ms;      /* host-to-device copies made by compile (draws, tables) */
   double draws_ms;       /* host generation of the normal stream (compile) */
+  double amc_ms;         /* American Monte Carlo: states pass and regressions (before the path kernel) */
 } ccr_hip_timing;
 
 /* Shape of a compiled plan. */
@@ -244,6 +245,10 @@ typedef struct ccr_hip_plan_info {
   int64_t hbm_bytes;          /* algorithmic HBM bytes of one execute */
   int32_t precision;
   int32_t block_size;
+  int32_t num_bermudans;      /* Bermudan swaptions valued by American Monte Carlo */
+  int32_t num_exercises;      /* their exercise dates, summed */
+  int32_t num_continuation_dates; /* (set, date) pairs with continuation regressions */
+  int32_t max_exercises;
 } ccr_hip_plan_info;
 
 int32_t ccr_hip_abi_version(void);

@@ -14,6 +14,124 @@
 
 using namespace ccrhip;
 
+namespace {
+
+// Masked polynomial regression with the moments summed in path order (the order of
+// PolynomialRegression::fit, src/core/regression.cpp:37-80), through the product's RegParams /
+// regSolve arithmetic. Returns the record center, scale, c0..c3, degree.
+void fitRecord(const std::vector<double>& x, const std::vector<double>& y, const std::vector<char>& mask, int degree,
+               double* rec) {
+  double n = 0, sx = 0, sxx = 0, sy = 0;
+  for (size_t i = 0; i < x.size(); ++i) {
+    if (!mask[i]) continue;
+    n += 1.0;
+    sx += x[i];
+    sxx += x[i] * x[i];
+    sy += y[i];
+  }
+  const RegParams r = regParams(n, sx, sxx, degree);
+  double gram[10] = {0}, xty[4] = {0};
+  if (r.deg > 0)
+    for (size_t i = 0; i < x.size(); ++i) {
+      if (!mask[i]) continue;
+      double phi[4];
+      regRow((x[i] - r.center) / r.scale, phi);
+      for (int a = 0; a <= r.deg; ++a) {
+        xty[a] += phi[a] * y[i];
+        for (int b = 0; b <= a; ++b) gram[triIndex(a, b)] += phi[a] * phi[b];
+      }
+    }
+  double coef[4];
+  regSolve(r, gram, xty, sy, coef);
+  rec[0] = r.n == 0.0 ? 0.0 : r.center;
+  rec[1] = r.scale;
+  for (int i = 0; i < 4; ++i) rec[2 + i] = coef[i];
+  rec[6] = static_cast<double>(r.deg);
+  rec[7] = 0.0;
+}
+
+// The American Monte Carlo stage on the CPU: states pass, exercise values, backward induction
+// and continuation fits, filling cont and exidx for the valuation pass.
+void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::vector<double>& cont,
+            std::vector<int16_t>& exidx, std::vector<double>& scratch) {
+  const int P = h.g.num_paths;
+  states.assign(static_cast<size_t>(2 * h.state_len), 0.0);
+  cont.assign(static_cast<size_t>(h.cont_len), 0.0);
+  exidx.assign(h.berms.size() * static_cast<size_t>(P), 0);
+  PathArgs<double> s = a;
+  s.states = states.data();
+  for (size_t si = 0; si < h.sets.size(); ++si) {
+    const SetDev& S = h.sets[si];
+    if (S.state_off < 0) continue;
+    for (int p = 0; p < P; ++p) {
+      std::fill(scratch.begin(), scratch.end(), 0.0);
+      withShape(h.g.nrates, h.g.nassets, h.g.ncredits, [&](auto shape) {
+        simulatePath<double, decltype(shape), true>(s, &S, p, scratch.data(), 1, [](int, const double*, double) {});
+        return 0;
+      });
+    }
+  }
+  const double* X = states.data();
+  const double* Dv = states.data() + h.state_len;
+  for (size_t b = 0; b < h.berms.size(); ++b) {
+    const BermDev& bm = h.berms[b];
+    const SetDev& S = h.sets[static_cast<size_t>(bm.set)];
+    const int ne = bm.ne;
+    std::vector<std::vector<double>> ev(static_cast<size_t>(ne), std::vector<double>(static_cast<size_t>(P)));
+    for (int k = 0; k < ne; ++k) {
+      const int j = h.eidx[static_cast<size_t>(bm.eidx_off + k)];
+      for (int p = 0; p < P; ++p)
+        ev[static_cast<size_t>(k)][static_cast<size_t>(p)] =
+            evalRange<double>(h.evterms[static_cast<size_t>(bm.ev_rec + k)], X[S.state_off + static_cast<int64_t>(j) * P + p],
+                              a.bonds + 2 * static_cast<int64_t>(S.bond_off), a.fixed + 4 * static_cast<int64_t>(S.fixed_off),
+                              a.poly + S.poly_off, nullptr, 0);
+    }
+    std::vector<std::vector<double>> strategy(static_cast<size_t>(ne) + 1, std::vector<double>(static_cast<size_t>(P), 0.0));
+    std::vector<double> realised(static_cast<size_t>(P), 0.0), x(static_cast<size_t>(P)), y(static_cast<size_t>(P));
+    std::vector<char> itm(static_cast<size_t>(P)), all(static_cast<size_t>(P), 1);
+    int16_t* ex = exidx.data() + b * static_cast<size_t>(P);
+    std::fill(ex, ex + P, static_cast<int16_t>(ne));
+    for (int k = ne - 1; k >= 0; --k) {
+      const int j = h.eidx[static_cast<size_t>(bm.eidx_off + k)];
+      const int64_t at = S.state_off + static_cast<int64_t>(j) * P;
+      for (int p = 0; p < P; ++p) {
+        x[static_cast<size_t>(p)] = X[at + p];
+        y[static_cast<size_t>(p)] = realised[static_cast<size_t>(p)] / Dv[at + p];
+        itm[static_cast<size_t>(p)] = ev[static_cast<size_t>(k)][static_cast<size_t>(p)] > 0.0;
+      }
+      double rec[8];
+      fitRecord(x, y, itm, bm.degree, rec);
+      for (int p = 0; p < P; ++p) {
+        const double e = ev[static_cast<size_t>(k)][static_cast<size_t>(p)];
+        if (e > 0.0 && e > regPredict(x[static_cast<size_t>(p)], rec[0], rec[1], static_cast<int>(rec[6]), rec + 2)) {
+          ex[p] = static_cast<int16_t>(k);
+          realised[static_cast<size_t>(p)] = Dv[at + p] * e;
+        }
+      }
+      strategy[static_cast<size_t>(k)] = realised;
+    }
+    for (int j = 0; j < S.num_dates; ++j) {
+      const BermDate& bd = h.bdates[static_cast<size_t>(bm.date_rec + j)];
+      double* rec = cont.data() + static_cast<size_t>(bm.date_rec + j) * kContStride;
+      rec[1] = 1.0;
+      rec[6] = -1.0;
+      if (bd.next_k >= ne || h.bdate_cd[static_cast<size_t>(bm.date_rec + j)] < 0) continue;
+      const int64_t at = S.state_off + static_cast<int64_t>(j) * P;
+      for (int p = 0; p < P; ++p) {
+        x[static_cast<size_t>(p)] = X[at + p];
+        y[static_cast<size_t>(p)] = strategy[static_cast<size_t>(bd.next_k)][static_cast<size_t>(p)] / Dv[at + p];
+      }
+      fitRecord(x, y, all, bm.degree, rec);
+    }
+  }
+  a.berms = h.berms.data();
+  a.bdates = h.bdates.data();
+  a.cont = cont.data();
+  a.exidx = exidx.data();
+}
+
+}  // namespace
+
 extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr_simulation_config* cfg,
                                     const double* base, int32_t nbase, const ccr_netting_set* sets, int32_t nsets,
                                     double q, ccr_exposure_out* outs, char* err, int32_t cap) {
@@ -22,11 +140,31 @@ extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr
     std::vector<double> stream(static_cast<size_t>(h.stream_len));
     generateStream(cfg->seed, h.stream_len, stream.data());
     std::vector<double> steps = h.steps, dreal = h.date_reals, bonds = h.bonds, fixed = h.fixed;
-    PathArgs<double> a{&h.g, h.sets.data(), h.block_set.data(), stream.data(), steps.data(), h.dates.data(),
-                       dreal.data(), bonds.data(), fixed.data(), h.fixstore.data(), h.poly.data(), nullptr, nullptr,
-                       h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots, h.g.nrates, h.g.nassets, h.g.ncredits};
+    PathArgs<double> a{};
+    a.g = &h.g;
+    a.sets = h.sets.data();
+    a.block_set = h.block_set.data();
+    a.stream = stream.data();
+    a.steps = steps.data();
+    a.dates = h.dates.data();
+    a.date_reals = dreal.data();
+    a.bonds = bonds.data();
+    a.fixed = fixed.data();
+    a.fixstore = h.fixstore.data();
+    a.poly = h.poly.data();
+    a.max_rep = h.max_rep;
+    a.ring = h.g.max_ring;
+    a.price_rows = h.price_rows;
+    a.slots = h.g.max_slots;
+    a.nrates = h.g.nrates;
+    a.nassets = h.g.nassets;
+    a.ncredits = h.g.ncredits;
+    a.state_len = h.state_len;
     const int P = h.g.num_paths;
     std::vector<double> scratch(static_cast<size_t>(h.g.max_ring + h.price_rows + h.g.max_slots + 1));
+    std::vector<double> states, cont;
+    std::vector<int16_t> exidx;
+    if (!h.berms.empty()) runAmc(h, a, states, cont, exidx, scratch);
     for (int s = 0; s < nsets; ++s) {
       const SetDev& S = h.sets[static_cast<size_t>(s)];
       const int R = S.num_rep;

@@ -29,6 +29,8 @@ CASES = {
     "credits": dict(num_paths=64, credits=True),
     "two_leg": dict(num_paths=64, two_leg=True),
     "all": dict(num_paths=128, csa=True, fx=True, credits=True, two_leg=True),
+    "bermudan_csa": dict(num_paths=200, csa=True, fx=True, bermudans=3),
+    "bermudan_cash": dict(num_paths=200, bermudans=3, cash_settled=True),
 }
 
 

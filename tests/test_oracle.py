@@ -249,7 +249,9 @@ def test_oracle_bermudan_matches_reference_golden_vectors():
 
 @pytest.mark.skipif(oracle_lib.ref() is None, reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("kw", [dict(num_paths=200, csa=True, fx=True, credits=True),
-                                dict(num_paths=150, two_leg=True, csa=True)])
+                                dict(num_paths=150, two_leg=True, csa=True),
+                                dict(num_paths=120, csa=True, fx=True, credits=True, bermudans=4),
+                                dict(num_paths=100, bermudans=2, cash_settled=True)])
 def test_oracle_matches_live_reference_build(kw):
     spec = W.small_reference_case(**kw)
     o = oracle_lib.run_oracle(spec, threads=3)[0]

@@ -45,7 +45,15 @@ CASES = {
     "c1": lambda: W.c1(num_paths=100),
     "c2_shape": lambda: W.c2(num_paths=48, num_sets=4, num_trades=60, weeks=52, horizon=1.0),
     "c4_plain": lambda: W.c4(num_paths=48, dates=24, with_bermudans=False),
+    "bermudan": lambda: bermudan_case(200),
+    "c3_small": lambda: W.c3(num_paths=96, dates=30, num_swaptions=12),
+    "c4_bermudan": lambda: W.c4(num_paths=64, dates=24),
 }
+
+
+def bermudan_case(num_paths):
+    from tests.golden.make_golden import bermudan_case as case
+    return case(num_paths)
 
 
 @pytest.mark.parametrize("case", sorted(CASES))

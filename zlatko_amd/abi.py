@@ -104,14 +104,17 @@ class HipOptionsC(C.Structure):
 
 
 class HipTimingC(C.Structure):
-    _fields_ = [(n, C.c_double) for n in ("path_ms", "reduce_ms", "pfe_ms", "total_ms", "upload_ms", "draws_ms")]
+    _fields_ = [(n, C.c_double) for n in ("path_ms", "reduce_ms", "pfe_ms", "total_ms", "upload_ms", "draws_ms",
+                                          "amc_ms")]
 
 
 class HipPlanInfoC(C.Structure):
     _fields_ = [("num_sets", C.c_int32), ("num_paths", C.c_int64), ("max_sim_dates", C.c_int32),
                 ("max_reporting_dates", C.c_int32), ("num_normals", C.c_int32), ("total_terms", C.c_int64),
                 ("stream_normals", C.c_int64), ("revaluations", C.c_int64), ("term_evaluations", C.c_int64),
-                ("hbm_bytes", C.c_int64), ("precision", C.c_int32), ("block_size", C.c_int32)]
+                ("hbm_bytes", C.c_int64), ("precision", C.c_int32), ("block_size", C.c_int32),
+                ("num_bermudans", C.c_int32), ("num_exercises", C.c_int32), ("num_continuation_dates", C.c_int32),
+                ("max_exercises", C.c_int32)]
 
 
 PROFILE_FIELDS = ("times", "expected_value", "expected_exposure", "expected_negative_exposure",

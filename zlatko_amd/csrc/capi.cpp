@@ -23,7 +23,7 @@ struct ccr_hip_ctx {
   int precision = CCR_FP64;
   int block = 256;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // start, AMC, path, reduce, PFE
   std::string err;
   int fault = CCR_FAULT_NONE;
   bool timing_pending = false;  // events recorded by a launch not yet read into `timing`
@@ -59,6 +59,13 @@ struct ccr_hip_plan {
   double* d_partials = nullptr;
   double* d_stats = nullptr;
   double* d_pfe = nullptr;
+  // American Monte Carlo (null when no set holds a Bermudan swaption).
+  BermDev* d_berms = nullptr;
+  BermDate* d_bdates = nullptr;
+  double* d_cont = nullptr;
+  int16_t* d_exidx = nullptr;
+  double* d_states = nullptr;
+  AmcArgs amc{};
   double draws_ms = 0.0, upload_ms = 0.0;
   ~ccr_hip_plan() {
     for (auto& b : bufs)
@@ -201,10 +208,102 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
       static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(h.total_waves) * h.max_rep * kStats * sizeof(double)));
   plan->d_stats = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * kStats * sizeof(double)));
   plan->d_pfe = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * sizeof(double)));
+  if (!h.berms.empty()) {
+    auto dalloc = [&](int64_t n) { return static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(n) * sizeof(double))); };
+    const int nB = static_cast<int>(h.berms.size());
+    const int ncd = static_cast<int>(h.cdates.size() / 2);
+    const int ncols = static_cast<int>(h.exlist.size());
+    const int nchunks = static_cast<int>((P + kAmcChunk - 1) / kAmcChunk);
+    plan->d_berms = upload(ctx, plan.get(), h.berms.data(), h.berms.size());
+    plan->d_bdates = upload(ctx, plan.get(), h.bdates.data(), h.bdates.size());
+    plan->d_cont = dalloc(h.cont_len);
+    plan->d_exidx = static_cast<int16_t*>(devAlloc(ctx, plan.get(), static_cast<size_t>(nB) * P * sizeof(int16_t)));
+    plan->d_states = dalloc(2 * h.state_len);
+    AmcArgs& a = plan->amc;
+    a.sets = plan->d_sets;
+    a.berms = plan->d_berms;
+    a.bdates = plan->d_bdates;
+    a.evterms = upload(ctx, plan.get(), h.evterms.data(), h.evterms.size());
+    a.eidx = upload(ctx, plan.get(), h.eidx.data(), h.eidx.size());
+    a.exlist = upload(ctx, plan.get(), h.exlist.data(), h.exlist.size());
+    a.cdates = ncd > 0 ? upload(ctx, plan.get(), h.cdates.data(), h.cdates.size()) : nullptr;
+    a.bdate_berm = upload(ctx, plan.get(), h.bdate_berm.data(), h.bdate_berm.size());
+    a.bdate_cd = upload(ctx, plan.get(), h.bdate_cd.data(), h.bdate_cd.size());
+    a.poly = plan->d_poly;
+    a.states = plan->d_states;
+    a.ev = dalloc(h.ev_len);
+    a.strategy = dalloc(h.ev_len);
+    a.part = dalloc(static_cast<int64_t>(nchunks) * std::max(nB * kRegMom2, ncd * 10));
+    a.regp = dalloc(static_cast<int64_t>(nB) * 8);
+    a.coef = dalloc(static_cast<int64_t>(nB) * 4);
+    a.csum = dalloc(std::max(1, ncd) * 3);
+    a.gram = dalloc(std::max(1, ncd) * 10);
+    a.W = ncd > 0 ? dalloc(static_cast<int64_t>(ncd) * 4 * P) : nullptr;
+    a.C = ncd > 0 ? dalloc(static_cast<int64_t>(ncd) * 4 * ncols) : nullptr;
+    a.cont = plan->d_cont;
+    a.exidx = plan->d_exidx;
+    a.paths = P;
+    a.state_len = h.state_len;
+    a.nberms = nB;
+    a.ncols = ncols;
+    a.ncdates = ncd;
+    a.nbdates = static_cast<int>(h.bdates.size());
+    a.nchunks = nchunks;
+  }
   check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   plan->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   uploadStream(ctx, plan.get(), config->seed, fp32);
   *out = plan.release();
+}
+
+template <typename Real>
+PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
+  const HostPlan& h = plan->host;
+  PathArgs<Real> a{};
+  a.g = plan->d_global;
+  a.sets = plan->d_sets;
+  a.block_set = plan->d_block_set;
+  a.stream = static_cast<const Real*>(plan->d_stream);
+  a.steps = static_cast<const Real*>(plan->d_steps);
+  a.dates = plan->d_dates;
+  a.date_reals = static_cast<const Real*>(plan->d_date_reals);
+  a.bonds = static_cast<const Real*>(plan->d_bonds);
+  a.fixed = static_cast<const Real*>(plan->d_fixed);
+  a.fixstore = plan->d_fixstore;
+  a.poly = plan->d_poly;
+  a.berms = plan->d_berms;
+  a.bdates = plan->d_bdates;
+  a.cont = plan->d_cont;
+  a.exidx = plan->d_exidx;
+  a.states = nullptr;
+  a.rows = static_cast<Real*>(plan->d_rows);
+  a.partials = plan->d_partials;
+  a.max_rep = h.max_rep;
+  a.ring = h.g.max_ring;
+  a.price_rows = h.price_rows;
+  a.slots = h.g.max_slots;
+  a.nrates = h.g.nrates;
+  a.nassets = h.g.nassets;
+  a.ncredits = h.g.ncredits;
+  a.state_len = h.state_len;
+  return a;
+}
+
+// States pass and regressions (sets with Bermudan swaptions), then the path kernel; events 1 and
+// 2 close the two stages.
+template <typename Real>
+void launchPathStages(ccr_hip_ctx* ctx, ccr_hip_plan* plan, int blocks) {
+  const HostPlan& h = plan->host;
+  PathArgs<Real> a = pathArgs<Real>(plan);
+  if (!h.berms.empty()) {
+    PathArgs<Real> s = a;
+    s.states = plan->d_states;
+    check(launchStates<Real>(s, blocks, h.block, ctx->stream), "states kernel launch");
+    check(launchAmc<Real>(plan->amc, a.bonds, a.fixed, h.max_exercises, ctx->stream), "regression kernel launch");
+  }
+  check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
+  check(launchPath<Real>(a, blocks, h.block, ctx->stream), "path kernel launch");
+  check(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
 }
 
 void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
@@ -217,26 +316,12 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
   const int nsets = static_cast<int>(h.sets.size());
   const int blocks = static_cast<int>(h.block_set.size());
   check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
-  if (h.precision == CCR_FP32) {
-    PathArgs<float> a{plan->d_global, plan->d_sets, plan->d_block_set, static_cast<const float*>(plan->d_stream),
-                      static_cast<const float*>(plan->d_steps), plan->d_dates, static_cast<const float*>(plan->d_date_reals),
-                      static_cast<const float*>(plan->d_bonds), static_cast<const float*>(plan->d_fixed), plan->d_fixstore, plan->d_poly,
-                      static_cast<float*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots,
-                      h.g.nrates, h.g.nassets, h.g.ncredits};
-    check(launchPath<float>(a, blocks, h.block, ctx->stream), "path kernel launch");
-  } else {
-    PathArgs<double> a{plan->d_global, plan->d_sets, plan->d_block_set, static_cast<const double*>(plan->d_stream),
-                       static_cast<const double*>(plan->d_steps), plan->d_dates, static_cast<const double*>(plan->d_date_reals),
-                       static_cast<const double*>(plan->d_bonds), static_cast<const double*>(plan->d_fixed), plan->d_fixstore, plan->d_poly,
-                       static_cast<double*>(plan->d_rows), plan->d_partials, h.max_rep, h.g.max_ring, h.price_rows, h.g.max_slots,
-                       h.g.nrates, h.g.nassets, h.g.ncredits};
-    check(launchPath<double>(a, blocks, h.block, ctx->stream), "path kernel launch");
-  }
-  check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
+  if (h.precision == CCR_FP32) launchPathStages<float>(ctx, plan, blocks);
+  else launchPathStages<double>(ctx, plan, blocks);
   check(launchReduce(plan->d_sets, plan->d_partials, nsets, h.max_rep, h.total_waves / nsets,
                      1.0 / static_cast<double>(h.g.num_paths), plan->d_stats, ctx->stream),
         "reduce kernel launch");
-  check(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
+  check(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
   if (h.precision == CCR_FP32)
     check(launchPfe<float>(plan->d_sets, static_cast<const float*>(plan->d_rows), nsets, h.max_rep, h.g.num_paths,
                            h.pfe_q, plan->d_pfe, ctx->stream),
@@ -245,7 +330,7 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
     check(launchPfe<double>(plan->d_sets, static_cast<const double*>(plan->d_rows), nsets, h.max_rep, h.g.num_paths,
                             h.pfe_q, plan->d_pfe, ctx->stream),
           "pfe kernel launch");
-  check(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
+  check(hipEventRecord(ctx->ev[4], ctx->stream), "hipEventRecord");
   ctx->timing_pending = true;
   ctx->upload_ms = plan->upload_ms;
   ctx->draws_ms = plan->draws_ms;
@@ -255,12 +340,14 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
 void readTimings(ccr_hip_ctx* ctx) {
   if (!ctx->timing_pending) return;
   ctx->timing_pending = false;
-  check(hipEventSynchronize(ctx->ev[3]), "hipEventSynchronize");
-  float a = 0, b = 0, c = 0, t = 0;
-  check(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
-  check(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
-  check(hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
-  check(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[3]), "hipEventElapsedTime");
+  check(hipEventSynchronize(ctx->ev[4]), "hipEventSynchronize");
+  float m = 0, a = 0, b = 0, c = 0, t = 0;
+  check(hipEventElapsedTime(&m, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
+  check(hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
+  check(hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
+  check(hipEventElapsedTime(&c, ctx->ev[3], ctx->ev[4]), "hipEventElapsedTime");
+  check(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[4]), "hipEventElapsedTime");
+  ctx->timing.amc_ms = m;
   ctx->timing.path_ms = a;
   ctx->timing.reduce_ms = b;
   ctx->timing.pfe_ms = c;
@@ -355,7 +442,7 @@ int32_t ccr_hip_create(const ccr_hip_options* options, ccr_hip_ctx** out) {
   }
   e = hipSetDevice(ctx->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-  for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
+  for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
   if (e != hipSuccess) {
     gCreateError = std::string("HIP initialisation failed: ") + hipGetErrorString(e);
     return CCR_ERR_DEVICE;
@@ -429,6 +516,10 @@ int32_t ccr_hip_plan_info_get(const ccr_hip_plan* plan, ccr_hip_plan_info* info)
   info->hbm_bytes = h.stream_len * elem + 2 * rows * elem;
   info->precision = h.precision;
   info->block_size = h.block;
+  info->num_bermudans = static_cast<int32_t>(h.berms.size());
+  info->num_exercises = static_cast<int32_t>(h.exlist.size());
+  info->num_continuation_dates = static_cast<int32_t>(h.cdates.size() / 2);
+  info->max_exercises = h.max_exercises;
   return CCR_OK;
 }
 

@@ -16,6 +16,7 @@
 #include <type_traits>
 
 #include "path_body.hpp"
+#include "wave.hpp"
 
 namespace ccrhip {
 
@@ -24,31 +25,6 @@ __device__ inline int xcdRemap(int b, int nb) {
   // workgroups so that the workgroups of one netting set share an L2.
   const int xcd = b & 7, i = b >> 3, per = nb >> 3, rem = nb & 7;
   return xcd < rem ? xcd * (per + 1) + i : rem * (per + 1) + (xcd - rem) * per + i;
-}
-
-// DPP move of a double (two 32-bit halves) with a given control; lanes of rows outside
-// row_mask keep 0.
-template <int Ctrl, int RowMask>
-__device__ inline double dppMove(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), Ctrl, RowMask, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), Ctrl, RowMask, 0xf, false);
-  return __hiloint2double(hi, lo);
-}
-
-// Sum over the 64 lanes of a wavefront in a fixed order (bit-reproducible), with DPP lane moves
-// only: pairs (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows (row_half_mirror),
-// rows (row_mirror), then row_bcast:15 and row_bcast:31 carry the row sums up; lane 63 ends
-// with the total, which is broadcast.
-__device__ inline double waveSum(double v) {
-  v += dppMove<0xb1, 0xf>(v);
-  v += dppMove<0x4e, 0xf>(v);
-  v += dppMove<0x141, 0xf>(v);
-  v += dppMove<0x140, 0xf>(v);
-  v += dppMove<0x142, 0xa>(v);
-  v += dppMove<0x143, 0xc>(v);
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
-  return __hiloint2double(hi, lo);
 }
 
 template <typename Real, typename Sh>
@@ -75,6 +51,22 @@ __global__ void __launch_bounds__(kMaxBlock) ccr_path_kernel(PathArgs<Real> a) {
       if ((tid & 63) == 0) part[rr * kStats + k] = s;
     }
   });
+}
+
+// States-only pass for American Monte Carlo: the same simulation, recording the domestic state
+// and the deflator of every date of the sets that hold Bermudan swaptions.
+template <typename Real, typename Sh>
+__global__ void __launch_bounds__(kMaxBlock) ccr_states_kernel(PathArgs<Real> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int b = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  cptr<SetDev> sp = toConst(a.sets) + toConst(a.block_set)[b];
+  if (sp->state_off < 0) return;
+  const int bd = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
+  const int p = (b - sp->block_begin) * bd + tid;
+  const int paths = toConst(a.g)->num_paths;
+  const int pe = p < paths ? p : paths - 1;
+  Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
+  simulatePath<Real, Sh, true>(a, sp, pe, col, bd, [](int, const double*, double) {});
 }
 
 // Sum of the per-wave partials of every (set, reporting date, statistic), in wave order.
@@ -315,6 +307,16 @@ hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStre
     return launchShape<Real, decltype(shape)>(args, blocks, block, lds, stream);
   });
 }
+
+template <typename Real>
+hipError_t launchStates(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
+  return withShape(args.nrates, args.nassets, args.ncredits, [&](auto shape) {
+    hipLaunchKernelGGL((ccr_states_kernel<Real, decltype(shape)>), dim3(blocks), dim3(block), 0, stream, args);
+    return hipGetLastError();
+  });
+}
+template hipError_t launchStates<double>(const PathArgs<double>&, int, int, hipStream_t);
+template hipError_t launchStates<float>(const PathArgs<float>&, int, int, hipStream_t);
 
 hipError_t launchReduce(const SetDev* sets, const double* partials, int nsets, int max_rep, int waves_per_set,
                         double inv_paths, double* out, hipStream_t stream) {

@@ -36,6 +36,11 @@ struct PathArgs {
   const Real* fixed;
   const int32_t* fixstore;
   const double* poly;  // bond-polynomial coefficients (FP64 in both precisions)
+  const BermDev* berms;     // American Monte Carlo: swaptions, per-date records, fitted continuation
+  const BermDate* bdates;
+  const double* cont;
+  const int16_t* exidx;     // [Bermudan][path] first exercise index (ne: never)
+  double* states;           // states-only pass: [set dates][path] x then D (null otherwise)
   Real* rows;
   double* partials;
   int32_t max_rep;
@@ -43,10 +48,47 @@ struct PathArgs {
   int32_t price_rows;  // LDS rows of the collateral price ring (0 for cash collateral)
   int32_t slots;       // LDS rows of fixing states
   int32_t nrates, nassets, ncredits;  // model shape: selects the kernel specialisation
+  int64_t state_len;                  // elements of x in `states` (D follows)
+};
+
+// American Monte Carlo regressions (amc.hip). Layouts: states x then D, [set dates][path];
+// ev and strategy [global exercise][path]; part [item][chunk][moments]; W [cdate][4][path];
+// C [cdate x 4][global exercise]; cont [Bermudan date record][kContStride].
+struct AmcArgs {
+  const SetDev* sets;
+  const BermDev* berms;
+  const BermDate* bdates;
+  const TermRange* evterms;
+  const int32_t* eidx;        // exercise date index per (Bermudan, exercise)
+  const int32_t* exlist;      // Bermudan of each global exercise
+  const int32_t* cdates;      // (set, date) per continuation date
+  const int32_t* bdate_berm;  // Bermudan of each date record
+  const int32_t* bdate_cd;    // continuation date of each date record (-1: none)
+  const double* poly;
+  const double* states;
+  double* ev;
+  double* strategy;
+  double* part;
+  double* regp;               // [Bermudan][8]: n, center, scale, degree, sum y
+  double* coef;               // [Bermudan][4]
+  double* csum;               // [cdate][3]: n, sum x, sum x^2
+  double* gram;               // [cdate][10]
+  double* W;
+  double* C;
+  double* cont;
+  int16_t* exidx;             // [Bermudan][path]
+  int64_t paths, state_len;
+  int32_t nberms, ncols, ncdates, nbdates, nchunks;
 };
 
 template <typename Real>
+hipError_t launchAmc(const AmcArgs& args, const Real* bonds, const Real* fixed, int max_exercises, hipStream_t stream);
+
+template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);
+// The states-only pass of the path kernel (AMC): writes x and D of every date into args.states.
+template <typename Real>
+hipError_t launchStates(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);
 hipError_t launchReduce(const SetDev* sets, const double* partials, int nsets, int max_rep, int waves_per_set,
                         double inv_paths, double* out, hipStream_t stream);
 template <typename Real>

@@ -71,6 +71,43 @@ constexpr double kPolyTolerance = 1e-18;
 constexpr double kPolySigmas = 9.0;
 constexpr int kPolyMaxDegree = 48;
 
+// ---------------------------------------------------------------- American Monte Carlo
+// Bermudan swaptions are valued by Longstaff-Schwartz regressions on the domestic state
+// (src/instruments/bermudan_swaption.cpp:49-146): a states-only pass of the path kernel records
+// x and D per date, batched regression kernels fit the exercise policy backwards over the exercise
+// dates and the continuation value on every valued date, and the path kernel then values each
+// swaption per path: the underlying swap once exercised (physical; 0 for cash), else
+// max(continuation, 0), times the position.
+constexpr int kMaxExercises = 64;
+constexpr int kAmcChunk = 256;   // paths per workgroup of the regression reductions
+constexpr int kRegMom1 = 4;      // masked moments: n, sum x, sum x^2, sum y
+constexpr int kRegMom2 = 14;     // Gram lower triangle (10) + X'y (4)
+constexpr int kContStride = 8;   // per (Bermudan, date): center, scale, c0..c3, degree, pad
+
+// Value terms of one instrument on one date: bond polynomial (or its exponential terms) plus
+// fixed coupons; indices into the plan's global poly / bonds / fixed arrays.
+struct TermRange {
+  int32_t poly_beg, poly_len;
+  int32_t bond_beg, bond_end;
+  int32_t fix_beg, fix_end;
+  double c0, xmax;
+};
+
+struct BermDev {
+  int32_t set, ne, degree, cash;
+  double position;
+  int32_t ev_rec;     // first of ne exercise-value TermRanges
+  int32_t date_rec;   // first of the set's T BermDate records (one per simulation date)
+  int32_t eidx_off;   // first of ne exercise date indices
+  int32_t ev_off;     // first of ne x P exercise values / strategy values (x P)
+};
+
+struct BermDate {
+  int32_t next_k;     // first exercise strictly after t_j (ne: none left)
+  int32_t has_swap;   // the underlying's value terms are present (physical settlement, exercisable)
+  TermRange swap;     // value of the underlying swap at t_j
+};
+
 struct SetDev {
   int64_t stream_stride;   // normals per path block: (T - 1) * nz
   int64_t rows_off;        // first element of this set's exposure rows
@@ -81,6 +118,8 @@ struct SetDev {
   int32_t block_begin, num_blocks, wave_begin;
   int32_t ring_len, has_csa, sec_type, csa_asset;
   int32_t cpty, own, num_slots;
+  int32_t berm_beg, berm_end;  // Bermudan swaptions of this set
+  int64_t state_off;           // first element of this set's [date][path] states (AMC), -1 without
   double thr_cpty, thr_own, mta, ia, keep;  // CSA terms (thr_own < 0 encodes a one-way CSA)
 };
 

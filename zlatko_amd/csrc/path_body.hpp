@@ -25,9 +25,19 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "amc.hpp"
 #include "launch.hpp"
 
 namespace ccrhip {
+
+// American Monte Carlo inputs of the valuation pass.
+struct AmcView {
+  cptr<BermDev> berms;
+  cptr<BermDate> bdates;
+  cptr<double> cont;
+  const int16_t* exidx;  // [Bermudan][path]
+  int64_t paths;
+};
 
 // exp(a) for |a| < 700 in FP64 (no overflow / underflow handling: bond-term exponents are
 // O(1)). Cody-Waite reduction a = n ln2 + r, |r| <= ln2/2, with the two-part ln2 of fdlibm's
@@ -79,6 +89,37 @@ __host__ __device__ inline Real sqrtr(Real v) {
   else return __fsqrt_rn(v);
 }
 
+// Value of a TermRange (an instrument's terms on one date, indices relative to its set's tables)
+// at domestic state x: the bond polynomial when every lane is within its bound, else the
+// exponential terms; plus the fixed coupons, whose fixing states come from the slot column xs.
+template <typename Real>
+__host__ __device__ inline Real evalRange(const CCR_CONST TermRange& tr, Real x, cptr<Real> bonds, cptr<Real> fixed,
+                                          cptr<double> poly, const Real* xs, int stride) {
+  Real acc;
+  if (tr.poly_len > 0 && allLanes(static_cast<double>(x < Real(0) ? -x : x) <= tr.xmax)) {
+    cptr<double> pc = poly + tr.poly_beg;
+    const double xd = static_cast<double>(x);
+    double h = 0.0;
+    for (int k = 0; k < tr.poly_len; k += 4) {
+      const double c0 = pc[k], c1 = pc[k + 1], c2 = pc[k + 2], c3 = pc[k + 3];
+      h = fma(h, xd, c0);
+      h = fma(h, xd, c1);
+      h = fma(h, xd, c2);
+      h = fma(h, xd, c3);
+    }
+    acc = static_cast<Real>(h);
+  } else {
+    acc = static_cast<Real>(tr.c0);
+    for (int i = tr.bond_beg; i < tr.bond_end; ++i) acc += bonds[2 * i] * expTerm(-bonds[2 * i + 1] * x);
+  }
+  for (int i = tr.fix_beg; i < tr.fix_end; ++i) {
+    cptr<Real> f = fixed + 4 * static_cast<int64_t>(i);
+    const Real xk = xs[static_cast<int>(f[3]) * stride];
+    acc += f[0] * expTerm(-f[1] * x - f[2] * xk);
+  }
+  return acc;
+}
+
 // Model shapes the path kernel is specialised for; everything else runs the generic shape.
 template <int NR_, int NA_, int NC_, bool Exact_>
 struct Shape {
@@ -110,12 +151,13 @@ inline auto withShape(int nrates, int nassets, int ncredits, F&& f) {
 // rows [0, ring) hold the margin-call values, [ring, ring + priceRows) the collateral prices,
 // then the fixing states. emit(rep, stats[kStats], positive_exposure) is called on every
 // reporting date.
-template <typename Real, typename Sh, typename Emit>
+// kStatesOnly: the AMC states pass, which only calls onState(j, x, D) on every date.
+template <typename Real, typename Sh, bool kStatesOnly, typename Emit, typename OnState>
 __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
-                                             cptr<int32_t> fstore, cptr<double> poly, int ring, int priceRows,
-                                             Real* col, int stride, Emit&& emit) {
+                                             cptr<int32_t> fstore, cptr<double> poly, AmcView amc, int ring,
+                                             int priceRows, Real* col, int stride, Emit&& emit, OnState&& onState) {
   constexpr bool kDouble = std::is_same<Real, double>::value;
   constexpr int NR = Sh::NR, NA = Sh::NA, NC = Sh::NC;
   const CCR_CONST GlobalDev& g = *gp;
@@ -261,6 +303,10 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
       }
     }
 
+    if constexpr (kStatesOnly) {
+      onState(j, x[0], D);
+      continue;
+    }
     const DateRec rec = dates[j];
     for (int i = rec.fixstore_beg; i < rec.fixstore_end; ++i) {
       const int r = fstore[2 * i], slot = fstore[2 * i + 1];
@@ -316,6 +362,28 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 #pragma unroll
       for (int i = 0; i < NA; ++i)
         if (i < nA) V += S_[i] * dr[kDateLin + i];
+      // Bermudan swaptions (bermudan_swaption.cpp:104-140): the underlying once exercised before
+      // the next exercise date (physical; 0 for cash), else max(continuation, 0).
+      for (int b = S.berm_beg; b < S.berm_end; ++b) {
+        const CCR_CONST BermDev& bm = amc.berms[b];
+        const CCR_CONST BermDate& bd = amc.bdates[bm.date_rec + j];
+        const int kx = amc.exidx[static_cast<int64_t>(b) * amc.paths + pe];
+        const bool exercised = kx < bd.next_k;
+        Real v = 0;
+        if (bd.next_k < bm.ne) {
+          cptr<double> c = amc.cont + static_cast<int64_t>(bm.date_rec + j) * kContStride;
+          const double coef[4] = {c[2], c[3], c[4], c[5]};
+          const double pred = regPredict(static_cast<double>(x[0]), c[0], c[1], static_cast<int>(c[6]), coef);
+          v = static_cast<Real>(pred > 0.0 ? pred : 0.0);
+        }
+        if (bd.has_swap && !allLanes(!exercised)) {
+          const Real sv = evalRange<Real>(bd.swap, x[0], bonds, fixed, poly, xs, stride);
+          if (exercised) v = sv;
+        } else if (exercised) {
+          v = 0;
+        }
+        V += static_cast<Real>(bm.position) * v;
+      }
     }
     Real C = 0;
     if (S.has_csa) {
@@ -375,21 +443,33 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
   }
 }
 
-// Convenience: the tables of set S for path pe, from a PathArgs bundle.
-template <typename Real, typename Sh, typename Emit>
+// Convenience: the tables of set S for path pe, from a PathArgs bundle. kStatesOnly: the AMC
+// states pass, writing x and D of every date into a.states ([state_off + j P + path], D after
+// a.state_len elements).
+template <typename Real, typename Sh, bool kStatesOnly = false, typename Emit>
 __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, Real* col, int stride,
                                              Emit&& emit) {
   const CCR_CONST SetDev& S = *sp;
   cptr<GlobalDev> g = toConst(a.g);
   const int64_t q = g->antithetic ? (pe >> 1) : pe;
-  simulatePath<Real, Sh>(g, sp, pe, a.stream + q * S.stream_stride,
-                         toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride,
-                         toConst(a.dates) + S.date_off,
-                         toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
-                         toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
-                         toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
-                         toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off, a.ring, a.price_rows,
-                         col, stride, emit);
+  const AmcView amc{toConst(a.berms), toConst(a.bdates), toConst(a.cont), a.exidx, g->num_paths};
+  double* states = a.states;
+  const int64_t stateOff = S.state_off, paths = g->num_paths, stateLen = a.state_len;
+  auto onState = [&](int j, Real x, Real D) {
+    if (states && stateOff >= 0 && pe < paths) {
+      const int64_t at = stateOff + static_cast<int64_t>(j) * paths + pe;
+      states[at] = static_cast<double>(x);
+      states[stateLen + at] = static_cast<double>(D);
+    }
+  };
+  simulatePath<Real, Sh, kStatesOnly>(g, sp, pe, a.stream + q * S.stream_stride,
+                                      toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride,
+                                      toConst(a.dates) + S.date_off,
+                                      toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
+                                      toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
+                                      toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
+                                      toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off, amc,
+                                      a.ring, a.price_rows, col, stride, emit, onState);
 }
 
 }  // namespace ccrhip

@@ -30,6 +30,11 @@ struct Trade {
   double sign = 1.0, notional = 0.0, K = 0.0, maturity = 0.0;
   std::vector<double> sched, fsched;
   int asset = 0;
+  // Bermudan swaption (the swap fields hold the underlying).
+  std::vector<double> exercise;
+  bool cash = false;
+  double position = 1.0;
+  int degree = 3;
 };
 
 void checkSchedule(const std::vector<double>& s) {
@@ -152,7 +157,30 @@ Trade makeTrade(const ccr_trade& d, const Models& m) {
     if (r > 0 && m.fxOf[static_cast<std::size_t>(r)] != d.asset)
       fail(CCR_ERR_UNSUPPORTED, "AssetForward: two assets convert the same foreign rate");
   } else if (d.kind == CCR_TRADE_BERMUDAN_SWAPTION) {
-    fail(CCR_ERR_UNSUPPORTED, "BermudanSwaption: run it through the AMC entry point");
+    // BermudanSwaption ctor: src/instruments/bermudan_swaption.cpp:17-32.
+    if (!d.schedule || d.schedule_size < 2) invalid("InterestRateSwap: schedule needs at least two dates");
+    t.sched.assign(d.schedule, d.schedule + d.schedule_size);
+    checkSchedule(t.sched);
+    if (d.float_schedule && d.float_schedule_size > 0) invalid("BermudanSwaption: the underlying has a single schedule");
+    if (d.currency != 0) fail(CCR_ERR_UNSUPPORTED, "BermudanSwaption: domestic underlying only");
+    if (!d.exercise_times || d.num_exercises <= 0) invalid("BermudanSwaption: no exercise dates");
+    if (d.num_exercises > kMaxExercises) fail(CCR_ERR_UNSUPPORTED, "BermudanSwaption: at most 64 exercise dates");
+    t.exercise.assign(d.exercise_times, d.exercise_times + d.num_exercises);
+    std::sort(t.exercise.begin(), t.exercise.end());
+    for (double e : t.exercise) {
+      const bool on = std::any_of(t.sched.begin(), t.sched.end() - 1, [e](double x) { return std::fabs(x - e) < 1e-8; });
+      if (!on) invalid("BermudanSwaption: exercise dates must be start dates of the underlying periods");
+    }
+    if (d.regression_degree < 0 || d.regression_degree > 3)
+      fail(CCR_ERR_UNSUPPORTED, "BermudanSwaption: regression degree 0..3");
+    t.sign = d.pay_fixed ? 1.0 : -1.0;
+    t.notional = d.notional;
+    t.K = d.fixed_rate;
+    t.maturity = t.sched.back();
+    t.rate = 0;
+    t.cash = d.cash_settled != 0;
+    t.position = d.position;
+    t.degree = d.regression_degree;
   } else {
     invalid("unknown trade kind");
   }
@@ -354,8 +382,11 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     if (ns.num_trades <= 0 || !ns.trades) invalid("ExposureEngine::run: empty netting set");
     if (ns.counterparty_credit >= g.ncredits || ns.own_credit >= g.ncredits)
       fail(CCR_ERR_OUT_OF_RANGE, "credit index out of range");
-    std::vector<Trade> trades;
-    for (int i = 0; i < ns.num_trades; ++i) trades.push_back(makeTrade(ns.trades[i], m));
+    std::vector<Trade> trades, berms;
+    for (int i = 0; i < ns.num_trades; ++i) {
+      Trade t = makeTrade(ns.trades[i], m);
+      (t.kind == CCR_TRADE_BERMUDAN_SWAPTION ? berms : trades).push_back(std::move(t));
+    }
     const TimeGrid rep = reportingGrid(base, ns);
     const TimeGrid sim = simulationGrid(base, ns);
     const int T = static_cast<int>(sim.size()), R = static_cast<int>(rep.size());
@@ -427,6 +458,44 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
         const auto key = std::make_pair(std::get<0>(kv.first), std::get<1>(kv.first));
         auto it = lastUse.find(key);
         if (it == lastUse.end() || it->second < j) lastUse[key] = j;
+      }
+    }
+    // Bermudan swaptions: exercise dates on the grid, the next exercise per date, the underlying's
+    // value terms where an exercised path holds it (bermudan_swaption.cpp:55-62, 104-140).
+    const int nB = static_cast<int>(berms.size());
+    std::vector<std::vector<int>> bEidx(static_cast<std::size_t>(nB)), bNext(static_cast<std::size_t>(nB));
+    std::vector<std::vector<Collector>> bCols(static_cast<std::size_t>(nB));
+    std::vector<std::vector<char>> bHas(static_cast<std::size_t>(nB));
+    for (int b = 0; b < nB; ++b) {
+      const Trade& tr = berms[static_cast<std::size_t>(b)];
+      for (double e : tr.exercise) {
+        const auto idx = sim.find(e, 1e-8);
+        if (!idx) invalid("BermudanSwaption: exercise date not on the simulation grid");
+        bEidx[static_cast<std::size_t>(b)].push_back(static_cast<int>(*idx));
+      }
+      Trade und = tr;
+      und.kind = CCR_TRADE_SWAP;
+      auto& nxt = bNext[static_cast<std::size_t>(b)];
+      auto& bc = bCols[static_cast<std::size_t>(b)];
+      auto& has = bHas[static_cast<std::size_t>(b)];
+      nxt.assign(static_cast<std::size_t>(T), 0);
+      bc.resize(static_cast<std::size_t>(T));
+      has.assign(static_cast<std::size_t>(T), 0);
+      for (int j = 0; j < T; ++j) {
+        const double t = sim[static_cast<std::size_t>(j)];
+        int k = 0;
+        while (k < static_cast<int>(tr.exercise.size()) && tr.exercise[static_cast<std::size_t>(k)] <= t + kPaidTolerance) ++k;
+        nxt[static_cast<std::size_t>(j)] = k;
+        if (!valued[static_cast<std::size_t>(j)] || tr.cash || k == 0) continue;
+        Collector& c = bc[static_cast<std::size_t>(j)];
+        c.t = t;
+        collect(und, m, sim, c);
+        has[static_cast<std::size_t>(j)] = 1;
+        for (const auto& kv : c.fix) {
+          const auto key = std::make_pair(std::get<0>(kv.first), std::get<1>(kv.first));
+          auto it = lastUse.find(key);
+          if (it == lastUse.end() || it->second < j) lastUse[key] = j;
+        }
       }
     }
     // Slots: interval colouring of [fixing date, last use].
@@ -584,6 +653,111 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       if (valued[static_cast<std::size_t>(j)]) plan.term_evaluations += static_cast<int64_t>(nTerms) * P;
     }
     plan.total_terms += setTerms;
+
+    // Bermudan records: exercise values (tail swap at each exercise date) and, per date, the next
+    // exercise and the underlying's value terms.
+    auto emitRange = [&](const Collector& col, double t, double xmax, bool allowFixed) {
+      TermRange tr{};
+      const HullWhite& hw = m.rates[0];
+      tr.bond_beg = static_cast<int32_t>(plan.bonds.size() / 2);
+      for (const auto& kv : col.bond) {
+        plan.bonds.push_back(kv.second * hw.A(t, kv.first.second));
+        plan.bonds.push_back(hw.B(t, kv.first.second));
+      }
+      tr.bond_end = static_cast<int32_t>(plan.bonds.size() / 2);
+      tr.fix_beg = static_cast<int32_t>(plan.fixed.size() / 4);
+      for (const auto& kv : col.fix) {
+        if (!allowFixed) fail(CCR_ERR_UNSUPPORTED, "BermudanSwaption: exercise value with a fixed coupon");
+        const int k = std::get<1>(kv.first);
+        const double ts = std::get<2>(kv.first), te = std::get<3>(kv.first);
+        const double tk = sim[static_cast<std::size_t>(k)];
+        const double As = ts <= tk ? 1.0 : hw.A(tk, ts), Bs = ts <= tk ? 0.0 : hw.B(tk, ts);
+        const double Ae = te <= tk ? 1.0 : hw.A(tk, te), Be = te <= tk ? 0.0 : hw.B(tk, te);
+        plan.fixed.push_back(kv.second * (As / Ae) * hw.A(t, te));
+        plan.fixed.push_back(hw.B(t, te));
+        plan.fixed.push_back(Bs - Be);
+        plan.fixed.push_back(static_cast<double>(slotOf.at({0, k})));
+      }
+      tr.fix_end = static_cast<int32_t>(plan.fixed.size() / 4);
+      tr.c0 = col.c0[0];
+      tr.xmax = xmax;
+      tr.poly_beg = static_cast<int32_t>(plan.poly.size());
+      std::vector<double> coef;
+      if (bondPolynomial(plan.bonds.data() + 2 * tr.bond_beg, tr.bond_end - tr.bond_beg, tr.c0, xmax, coef)) {
+        const std::size_t len = (coef.size() + 3) / 4 * 4;
+        plan.poly.insert(plan.poly.end(), len - coef.size(), 0.0);
+        plan.poly.insert(plan.poly.end(), coef.rbegin(), coef.rend());
+        tr.poly_len = static_cast<int32_t>(len);
+      }
+      // Indices relative to the set's tables, like the date records.
+      tr.bond_beg -= sd.bond_off;
+      tr.bond_end -= sd.bond_off;
+      tr.fix_beg -= sd.fixed_off;
+      tr.fix_end -= sd.fixed_off;
+      tr.poly_beg -= sd.poly_off;
+      return tr;
+    };
+    sd.berm_beg = static_cast<int32_t>(plan.berms.size());
+    sd.state_off = nB > 0 ? plan.state_len : -1;
+    if (nB > 0) plan.state_len += static_cast<int64_t>(T) * P;
+    // Continuation regressions run on every valued date of a set with Bermudans.
+    std::vector<int32_t> cdOf(static_cast<std::size_t>(T), -1);
+    if (nB > 0)
+      for (int j = 0; j < T; ++j)
+        if (valued[static_cast<std::size_t>(j)]) {
+          cdOf[static_cast<std::size_t>(j)] = static_cast<int32_t>(plan.cdates.size() / 2);
+          plan.cdates.push_back(s);
+          plan.cdates.push_back(j);
+        }
+    for (int b = 0; b < nB; ++b) {
+      const Trade& tr = berms[static_cast<std::size_t>(b)];
+      const int ne = static_cast<int>(tr.exercise.size());
+      BermDev bd{};
+      bd.set = s;
+      bd.ne = ne;
+      bd.degree = tr.degree;
+      bd.cash = tr.cash ? 1 : 0;
+      bd.position = tr.position;
+      bd.ev_rec = static_cast<int32_t>(plan.evterms.size());
+      bd.date_rec = static_cast<int32_t>(plan.bdates.size());
+      bd.eidx_off = static_cast<int32_t>(plan.eidx.size());
+      bd.ev_off = static_cast<int32_t>(plan.ev_len / P);
+      for (int k = 0; k < ne; ++k) {
+        const int je = bEidx[static_cast<std::size_t>(b)][static_cast<std::size_t>(k)];
+        const double te = sim[static_cast<std::size_t>(je)];
+        Trade tail = tr;  // InterestRateSwap::tail (interest_rate_swap.cpp:97-103)
+        tail.kind = CCR_TRADE_SWAP;
+        tail.sched.clear();
+        for (double x : tr.sched)
+          if (x >= tr.exercise[static_cast<std::size_t>(k)] - kPaidTolerance) tail.sched.push_back(x);
+        if (tail.sched.size() < 2) invalid("InterestRateSwap::tail: no period starts after fromTime");
+        tail.maturity = tail.sched.back();
+        Collector ec;
+        ec.t = te;
+        collect(tail, m, sim, ec);
+        plan.eidx.push_back(je);
+        plan.evterms.push_back(emitRange(ec, te, stateBound(m.rates[0], 0.0, te), false));
+        plan.exlist.push_back(static_cast<int32_t>(plan.berms.size()));
+      }
+      for (int j = 0; j < T; ++j) {
+        BermDate rec{};
+        rec.next_k = bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(j)];
+        rec.has_swap = bHas[static_cast<std::size_t>(b)][static_cast<std::size_t>(j)];
+        if (rec.has_swap) {
+          const double t = sim[static_cast<std::size_t>(j)];
+          rec.swap = emitRange(bCols[static_cast<std::size_t>(b)][static_cast<std::size_t>(j)], t,
+                               stateBound(m.rates[0], 0.0, t), true);
+        }
+        plan.bdates.push_back(rec);
+        plan.bdate_berm.push_back(static_cast<int32_t>(plan.berms.size()));
+        plan.bdate_cd.push_back(cdOf[static_cast<std::size_t>(j)]);
+      }
+      plan.ev_len += static_cast<int64_t>(ne) * P;
+      plan.cont_len += static_cast<int64_t>(T) * kContStride;
+      plan.max_exercises = std::max(plan.max_exercises, ne);
+      plan.berms.push_back(bd);
+    }
+    sd.berm_end = static_cast<int32_t>(plan.berms.size());
     plan.revaluations += static_cast<int64_t>(ns.num_trades) * P * R;
     sd.rows_off = static_cast<int64_t>(s) * 0;  // filled below once max_rep is known
     plan.sets.push_back(sd);

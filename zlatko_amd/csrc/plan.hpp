@@ -33,6 +33,19 @@ struct HostPlan {
   int64_t term_evaluations = 0;   // exponential terms x paths over valued dates
   int64_t poly_evaluations = 0;   // polynomial coefficients x paths over valued dates
   int64_t total_terms = 0;
+  // American Monte Carlo.
+  std::vector<BermDev> berms;
+  std::vector<BermDate> bdates;     // per Bermudan, one per simulation date of its set
+  std::vector<TermRange> evterms;   // per Bermudan exercise: the tail swap at the exercise date
+  std::vector<int32_t> eidx;        // exercise date indices
+  int64_t state_len = 0;            // elements of the x / D state arrays
+  int64_t ev_len = 0;               // elements of the exercise-value / strategy arrays
+  int32_t max_exercises = 0;
+  int64_t cont_len = 0;             // kContStride x sum over Bermudans of their set's dates
+  std::vector<int32_t> exlist;      // per exercise (global order): its Bermudan
+  std::vector<int32_t> cdates;      // continuation dates: (set, date) pairs
+  std::vector<int32_t> bdate_berm;  // per BermDate record: its Bermudan
+  std::vector<int32_t> bdate_cd;    // per BermDate record: its continuation date (-1: none)
 };
 
 // ExposureEngine::reportingGrid / simulationGrid (src/exposure/exposure_engine.cpp:35-48).

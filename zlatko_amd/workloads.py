@@ -163,7 +163,8 @@ def c4(num_paths: int = 100000, dates: int = 120, seed: int = 42, with_bermudans
 
 
 def small_reference_case(num_paths: int = 64, dates: int = 12, csa: bool = False, fx: bool = False,
-                         credits: bool = False, two_leg: bool = False, seed: int = 7) -> Spec:
+                         credits: bool = False, two_leg: bool = False, seed: int = 7, bermudans: int = 0,
+                         cash_settled: bool = False) -> Spec:
     """Small reference-representable netting set for parity against a build of the reference."""
     rng = SplitMix64(11)
     curve = Curve([0.5, 2.0, 5.0, 10.0], [0.020, 0.025, 0.030, 0.033])
@@ -189,6 +190,13 @@ def small_reference_case(num_paths: int = 64, dates: int = 12, csa: bool = False
     if fx:
         ts.append(Trade(CCR_TRADE_ASSET_FORWARD, 5e5, 1.05, asset=0, maturity=3.0))
         ts.append(Trade(CCR_TRADE_ASSET_FORWARD, -3e5, 1.12, asset=0, maturity=1.5))
+    for i in range(bermudans):
+        # 1-2y into 2-3y, semiannual exercise on the underlying's period starts.
+        expiry = 1.0 + 0.5 * rng.integer(0, 2)
+        sched = swap_schedule(expiry, 2.0 + 0.5 * rng.integer(0, 2), 2)
+        ts.append(Trade(CCR_TRADE_BERMUDAN_SWAPTION, rng.uniform(1e5, 1e6), rng.uniform(0.02, 0.035), rng.bernoulli(),
+                        schedule=sched, exercise=sched[:-1].copy(), position=-1.0 if i % 2 else 1.0,
+                        degree=1 + i % 3, cash_settled=cash_settled))
     ns = NettingSet(ts)
     if csa:
         ns.collateral = Collateral(threshold_counterparty=2e4, threshold_own=5e4, mta=1e4, mpr=10.0 / 250.0)

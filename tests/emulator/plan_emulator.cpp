@@ -53,7 +53,7 @@ void fitRecord(const std::vector<double>& x, const std::vector<double>& y, const
 // The American Monte Carlo stage on the CPU: states pass, exercise values, backward induction
 // and continuation fits, filling cont and exidx for the valuation pass.
 void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::vector<double>& cont,
-            std::vector<int16_t>& exidx, std::vector<double>& scratch) {
+            std::vector<int16_t>& exidx, std::vector<double>& bermv, std::vector<double>& scratch) {
   const int P = h.g.num_paths;
   states.assign(static_cast<size_t>(2 * h.state_len), 0.0);
   cont.assign(static_cast<size_t>(h.cont_len), 0.0);
@@ -84,7 +84,7 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
         ev[static_cast<size_t>(k)][static_cast<size_t>(p)] =
             evalRange<double>(h.evterms[static_cast<size_t>(bm.ev_rec + k)], X[S.state_off + static_cast<int64_t>(j) * P + p],
                               a.bonds + 2 * static_cast<int64_t>(S.bond_off), a.fixed + 4 * static_cast<int64_t>(S.fixed_off),
-                              a.poly + S.poly_off, nullptr, 0);
+                              a.poly + S.poly_off, [](int) { return 0.0; });
     }
     std::vector<std::vector<double>> strategy(static_cast<size_t>(ne) + 1, std::vector<double>(static_cast<size_t>(P), 0.0));
     std::vector<double> realised(static_cast<size_t>(P), 0.0), x(static_cast<size_t>(P)), y(static_cast<size_t>(P));
@@ -124,10 +124,18 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
       fitRecord(x, y, all, bm.degree, rec);
     }
   }
-  a.berms = h.berms.data();
-  a.bdates = h.bdates.data();
-  a.cont = cont.data();
-  a.exidx = exidx.data();
+  const int ncd = static_cast<int>(h.cdates.size() / 2);
+  bermv.assign(static_cast<size_t>(ncd) * P, 0.0);
+  for (int d = 0; d < ncd; ++d) {
+    const SetDev& S = h.sets[static_cast<size_t>(h.cdates[2 * static_cast<size_t>(d)])];
+    const int j = h.cdates[2 * static_cast<size_t>(d) + 1];
+    for (int p = 0; p < P; ++p)
+      bermv[static_cast<size_t>(d) * P + p] =
+          bermudanSum<double>(S, j, p, P, h.berms.data(), h.bdates.data(), cont.data(), exidx.data(), states.data(),
+                              a.bonds + 2 * static_cast<int64_t>(S.bond_off), a.fixed + 4 * static_cast<int64_t>(S.fixed_off),
+                              a.poly + S.poly_off);
+  }
+  a.bermv = bermv.data();
 }
 
 }  // namespace
@@ -162,9 +170,9 @@ extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr
     a.state_len = h.state_len;
     const int P = h.g.num_paths;
     std::vector<double> scratch(static_cast<size_t>(h.g.max_ring + h.price_rows + h.g.max_slots + 1));
-    std::vector<double> states, cont;
+    std::vector<double> states, cont, bermv;
     std::vector<int16_t> exidx;
-    if (!h.berms.empty()) runAmc(h, a, states, cont, exidx, scratch);
+    if (!h.berms.empty()) runAmc(h, a, states, cont, exidx, bermv, scratch);
     for (int s = 0; s < nsets; ++s) {
       const SetDev& S = h.sets[static_cast<size_t>(s)];
       const int R = S.num_rep;

@@ -1,4 +1,4 @@
-// amc.hip -- American Monte Carlo (Longstaff-Schwartz) kernels for Bermudan swaptions, FP64.
+// amc.hip -- American Monte Carlo (Longstaff-Schwartz) kernels for Bermudan swaptions.
 //
 // Follows BermudanSwaption::valueWithExercise (src/instruments/bermudan_swaption.cpp:49-146):
 //  1. exercise values: the tail swap at each exercise date, on the recorded states;
@@ -6,12 +6,15 @@
 //     regress realised / D on x (masked moments -> standardisation and degree rules -> Gram and
 //     X'y -> ridge + elimination with partial pivoting), exercise where e > 0 and e > prediction,
 //     record the strategy value realised by the policy;
-//  3. continuation on every valued date: the standardised design of x is shared by all swaptions
-//     of a date, so the right-hand sides of every (date, basis function) against every
-//     (swaption, exercise) strategy column are one GEMM  C = W S^T  with K = paths
-//     (W[date, a][p] = z_p^a / D_p), followed by one small solve per (swaption, date).
-// All reductions are per-workgroup DPP sums combined in a fixed order, so results are
-// bit-reproducible.
+//  3. continuation on every valued date: the standardised design of x (W[date, a][p] =
+//     z_p^a / D_p) and its Gram are shared by all swaptions of the date; within an exercise epoch
+//     every swaption regresses the same strategy column, so the right-hand sides of a tile of
+//     dates x swaptions are one gathered GEMM over the paths (split in chunks, summed in chunk
+//     order), followed by one small solve per (swaption, date);
+//  4. valuation: per (valued date, path) the sum over the set's swaptions of position x value,
+//     read by the path kernel.
+// Regressions run in FP64 in both precisions. All reductions combine per-wavefront DPP sums in a
+// fixed order, so results are bit-reproducible.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -43,10 +46,11 @@ __global__ void __launch_bounds__(kAmcChunk) amc_ev_kernel(AmcArgs a, const Real
   const int64_t p = p0 < a.paths ? p0 : a.paths - 1;
   const int j = toConst(a.eidx)[bm.eidx_off + k];
   const double x = a.states[S.state_off + static_cast<int64_t>(j) * a.paths + p];
+  // The tail swap starts at the exercise date: no coupon is fixed yet.
   const Real v = evalRange<Real>(toConst(a.evterms)[bm.ev_rec + k], static_cast<Real>(x),
                                  toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off),
                                  toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(a.poly) + S.poly_off,
-                                 nullptr, 0);
+                                 [](int) { return Real(0); });
   if (p0 < a.paths) a.ev[static_cast<int64_t>(e) * a.paths + p0] = static_cast<double>(v);
 }
 
@@ -64,6 +68,22 @@ __device__ inline ExerciseSample exerciseSample(const AmcArgs& a, const CCR_CONS
   s.e = a.ev[static_cast<int64_t>(bm.ev_off + k) * a.paths + p];
   s.prev = k + 1 < bm.ne ? a.strategy[static_cast<int64_t>(bm.ev_off + k + 1) * a.paths + p] : 0.0;
   return s;
+}
+
+// Sums N moments of item `item` over the path chunks in a fixed order with one wavefront: lane l
+// takes chunks l, l + 64, ... in order, then a DPP wavefront sum. Valid in every lane.
+template <int N>
+__device__ inline void chunkSum(const double* part, int64_t item, int nchunks, double (&s)[N]) {
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+#pragma unroll
+  for (int i = 0; i < N; ++i) s[i] = 0.0;
+  for (int c = lane; c < nchunks; c += 64) {
+    const double* q = part + (item * nchunks + c) * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) s[i] += q[i];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) s[i] = waveSum(s[i]);
 }
 
 // Masked moments n, sum x, sum x^2, sum y of the in-the-money paths (regression.cpp:37-46).
@@ -89,14 +109,15 @@ __global__ void __launch_bounds__(kAmcChunk) amc_mom1_kernel(AmcArgs a, int k) {
     for (int i = 0; i < kRegMom1; ++i) a.part[(static_cast<int64_t>(b) * a.nchunks + blockIdx.x) * kRegMom1 + i] = v[i];
 }
 
-__global__ void amc_fin1_kernel(AmcArgs a, int k) {
-  const int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+// One wavefront per swaption: standardisation and effective degree.
+__global__ void __launch_bounds__(256) amc_fin1_kernel(AmcArgs a, int k) {
+  const int b = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (b >= a.nberms) return;
   const BermDev& bm = a.berms[b];
   if (k >= bm.ne) return;
-  double s[kRegMom1] = {0, 0, 0, 0};
-  for (int c = 0; c < a.nchunks; ++c)
-    for (int i = 0; i < kRegMom1; ++i) s[i] += a.part[(static_cast<int64_t>(b) * a.nchunks + c) * kRegMom1 + i];
+  double s[kRegMom1];
+  chunkSum<kRegMom1>(a.part, b, a.nchunks, s);
+  if ((threadIdx.x & 63) != 0) return;
   const RegParams r = regParams(s[0], s[1], s[2], bm.degree);
   double* o = a.regp + static_cast<int64_t>(b) * 8;
   o[0] = r.n;
@@ -139,8 +160,9 @@ __global__ void __launch_bounds__(kAmcChunk) amc_mom2_kernel(AmcArgs a, int k) {
     for (int i = 0; i < kRegMom2; ++i) a.part[(static_cast<int64_t>(b) * a.nchunks + blockIdx.x) * kRegMom2 + i] = v[i];
 }
 
-__global__ void amc_solve_kernel(AmcArgs a, int k) {
-  const int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+// One wavefront per swaption: normal equations with the ridge, elimination with pivoting.
+__global__ void __launch_bounds__(256) amc_solve_kernel(AmcArgs a, int k) {
+  const int b = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (b >= a.nberms) return;
   const BermDev& bm = a.berms[b];
   if (k >= bm.ne) return;
@@ -151,10 +173,12 @@ __global__ void amc_solve_kernel(AmcArgs a, int k) {
   r.scale = rp[2];
   r.deg = static_cast<int>(rp[3]);
   double s[kRegMom2];
-  for (int i = 0; i < kRegMom2; ++i) s[i] = 0.0;
-  if (r.deg > 0)
-    for (int c = 0; c < a.nchunks; ++c)
-      for (int i = 0; i < kRegMom2; ++i) s[i] += a.part[(static_cast<int64_t>(b) * a.nchunks + c) * kRegMom2 + i];
+  if (r.deg > 0) {
+    chunkSum<kRegMom2>(a.part, b, a.nchunks, s);
+  } else {
+    for (int i = 0; i < kRegMom2; ++i) s[i] = 0.0;
+  }
+  if ((threadIdx.x & 63) != 0) return;
   double coef[4];
   regSolve(r, s, s + 10, rp[4], coef);
   for (int i = 0; i < 4; ++i) a.coef[static_cast<int64_t>(b) * 4 + i] = coef[i];
@@ -197,12 +221,12 @@ __global__ void __launch_bounds__(kAmcChunk) amc_cmom1_kernel(AmcArgs a) {
     for (int i = 0; i < 3; ++i) a.part[(static_cast<int64_t>(d) * a.nchunks + blockIdx.x) * 3 + i] = v[i];
 }
 
-__global__ void amc_cfin1_kernel(AmcArgs a) {
-  const int d = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(256) amc_cfin1_kernel(AmcArgs a) {
+  const int d = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (d >= a.ncdates) return;
-  double s[3] = {0, 0, 0};
-  for (int c = 0; c < a.nchunks; ++c)
-    for (int i = 0; i < 3; ++i) s[i] += a.part[(static_cast<int64_t>(d) * a.nchunks + c) * 3 + i];
+  double s[3];
+  chunkSum<3>(a.part, d, a.nchunks, s);
+  if ((threadIdx.x & 63) != 0) return;
   for (int i = 0; i < 3; ++i) a.csum[3 * d + i] = s[i];
 }
 
@@ -233,65 +257,74 @@ __global__ void __launch_bounds__(kAmcChunk) amc_design_kernel(AmcArgs a) {
     for (int i = 0; i < 10; ++i) a.part[(static_cast<int64_t>(d) * a.nchunks + blockIdx.x) * 10 + i] = v[i];
 }
 
-__global__ void amc_gram_kernel(AmcArgs a) {
-  const int d = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(256) amc_gram_kernel(AmcArgs a) {
+  const int d = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (d >= a.ncdates) return;
   double s[10];
-  for (int i = 0; i < 10; ++i) s[i] = 0.0;
-  for (int c = 0; c < a.nchunks; ++c)
-    for (int i = 0; i < 10; ++i) s[i] += a.part[(static_cast<int64_t>(d) * a.nchunks + c) * 10 + i];
+  chunkSum<10>(a.part, d, a.nchunks, s);
+  if ((threadIdx.x & 63) != 0) return;
   for (int i = 0; i < 10; ++i) a.gram[10 * d + i] = s[i];
 }
 
-// C[m][n] = sum_p A[m][p] B[n][p] (FP64, K = paths), 64 x 64 tile per workgroup, 4 x 4 per
-// thread, K staged through LDS in chunks of 32; each output sums its K in a fixed order.
-constexpr int kGT = 64, kGK = 32;
-__global__ void __launch_bounds__(256) amc_gemm_nt_kernel(const double* A, const double* B, double* C, int M, int N,
-                                                          int64_t K) {
-  __shared__ double As[kGK][kGT + 1];
-  __shared__ double Bs[kGK][kGT + 1];
-  const int tx = static_cast<int>(threadIdx.x) % 16, ty = static_cast<int>(threadIdx.x) / 16;
-  const int m0 = static_cast<int>(blockIdx.y) * kGT, n0 = static_cast<int>(blockIdx.x) * kGT;
-  double acc[4][4];
+// Continuation right-hand sides of one tile over one chunk of paths: rows (date dd, basis a) of
+// W against the strategy column of each swaption in the tile. Wavefront w owns rows 8w .. 8w+7,
+// lane c swaption c; paths are staged 64 at a time in LDS (W rows are read as broadcasts).
+constexpr int kGStep = 64;
+__global__ void __launch_bounds__(256) amc_cgemm_kernel(AmcArgs a) {
+  __shared__ double Ws[4 * kGemmDates][kGStep];
+  __shared__ double Ss[kGemmCols][kGStep + 1];
+  const GemmTile t = a.tiles[blockIdx.x];
+  const int kc = static_cast<int>(blockIdx.y);
+  const int tid = static_cast<int>(threadIdx.x), w = tid >> 6, c = tid & 63;
+  const int64_t kBeg = static_cast<int64_t>(kc) * kGemmK;
+  const int64_t kEnd = kBeg + kGemmK < a.paths ? kBeg + kGemmK : a.paths;
+  const int rows = 4 * t.nd;
+  // Loader roles: W row tid / 8, 8 paths from (tid % 8) * 8; S column tid / 4, 16 paths from (tid % 4) * 16.
+  const int wr = tid >> 3, wp = (tid & 7) * 8;
+  const int sc = tid >> 2, spo = (tid & 3) * 16;
+  const double* wsrc = wr < rows ? a.W + (static_cast<int64_t>(t.d0) * 4 + wr) * a.paths : nullptr;
+  const int col = sc < t.nb ? a.gcols[t.col_off + sc] : -1;
+  const double* ssrc = col >= 0 ? a.strategy + static_cast<int64_t>(col) * a.paths : nullptr;
+  double acc[8];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 8; ++i) acc[i] = 0.0;
+  for (int64_t k0 = kBeg; k0 < kEnd; k0 += kGStep) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[i][c] = 0.0;
-  for (int64_t k0 = 0; k0 < K; k0 += kGK) {
-    for (int t = static_cast<int>(threadIdx.x); t < kGT * kGK; t += 256) {
-      const int r = t / kGK, kk = t % kGK;
-      const int64_t k = k0 + kk;
-      As[kk][r] = (m0 + r < M && k < K) ? A[static_cast<int64_t>(m0 + r) * K + k] : 0.0;
-      Bs[kk][r] = (n0 + r < N && k < K) ? B[static_cast<int64_t>(n0 + r) * K + k] : 0.0;
+    for (int i = 0; i < 8; ++i) {
+      const int64_t k = k0 + wp + i;
+      Ws[wr][wp + i] = (wsrc && k < kEnd) ? wsrc[k] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int64_t k = k0 + spo + i;
+      Ss[sc][spo + i] = (ssrc && k < kEnd) ? ssrc[k] : 0.0;
     }
     __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < kGK; ++kk) {
-      double av[4], bv[4];
+#pragma unroll 8
+    for (int kk = 0; kk < kGStep; ++kk) {
+      const double s = Ss[c][kk];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        av[i] = As[kk][ty * 4 + i];
-        bv[i] = Bs[kk][tx * 4 + i];
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[i][c] = fma(av[i], bv[c], acc[i][c]);
+      for (int i = 0; i < 8; ++i) acc[i] = fma(Ws[8 * w + i][kk], s, acc[i]);
     }
     __syncthreads();
   }
+  if (c >= t.nb) return;
+  const int cc = a.gcols[t.col_off + c];
+  if (cc < 0) return;
+  const int date_rec = a.berms[t.b0 + c].date_rec;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int m = m0 + ty * 4 + i, n = n0 + tx * 4 + c;
-      if (m < M && n < N) C[static_cast<int64_t>(m) * N + n] = acc[i][c];
-    }
+  for (int i = 0; i < 8; ++i) {
+    const int row = 8 * w + i, dd = row >> 2, basis = row & 3;
+    if (dd >= t.nd) continue;
+    const int r = date_rec + a.cdates[2 * (t.d0 + dd) + 1];
+    a.gpart[(static_cast<int64_t>(kc) * a.nbdates + r) * 4 + basis] = acc[i];
+  }
 }
 
 // Continuation fit of every (Bermudan, date) record: target strategy[next exercise] / D on all
-// paths (bermudan_swaption.cpp:110-116); records without one get degree -1 (prediction 0).
-__global__ void amc_csolve_kernel(AmcArgs a) {
+// paths (bermudan_swaption.cpp:110-116); the path-chunk partials of X'y are summed in chunk
+// order. Records without an exercise left get degree -1 (prediction 0).
+__global__ void __launch_bounds__(256) amc_csolve_kernel(AmcArgs a) {
   const int r = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (r >= a.nbdates) return;
   const BermDev& bm = a.berms[a.bdate_berm[r]];
@@ -302,16 +335,34 @@ __global__ void amc_csolve_kernel(AmcArgs a) {
   out[1] = 1.0;
   out[6] = -1.0;
   if (d < 0 || bd.next_k >= bm.ne) return;
+  double xty[4] = {0, 0, 0, 0};
+  for (int kc = 0; kc < a.nkc; ++kc)
+    for (int i = 0; i < 4; ++i) xty[i] += a.gpart[(static_cast<int64_t>(kc) * a.nbdates + r) * 4 + i];
   const RegParams rp = regParams(a.csum[3 * d], a.csum[3 * d + 1], a.csum[3 * d + 2], bm.degree);
-  const int col = bm.ev_off + bd.next_k;
-  double xty[4];
-  for (int i = 0; i < 4; ++i) xty[i] = a.C[static_cast<int64_t>(4 * d + i) * a.ncols + col];
   double coef[4];
   regSolve(rp, a.gram + 10 * d, xty, xty[0], coef);
   out[0] = rp.center;
   out[1] = rp.scale;
   for (int i = 0; i < 4; ++i) out[2 + i] = coef[i];
   out[6] = static_cast<double>(rp.deg);
+}
+
+// Per (valued date of a set with Bermudans, path): the sum over the set's swaptions of
+// position x value. Workgroups of one path chunk land on one XCD (the grid's x extent is a
+// multiple of 8), so the chunk's exercise indices are read from that XCD's L2.
+template <typename Real>
+__global__ void __launch_bounds__(kAmcChunk) amc_value_kernel(AmcArgs a, const Real* bonds, const Real* fixed) {
+  const int chunk = static_cast<int>(blockIdx.x);
+  if (chunk >= a.nchunks) return;
+  const int d = static_cast<int>(blockIdx.y);
+  const CCR_CONST SetDev& S = toConst(a.sets)[toConst(a.cdates)[2 * d]];
+  const int j = toConst(a.cdates)[2 * d + 1];
+  const int64_t p0 = static_cast<int64_t>(chunk) * kAmcChunk + threadIdx.x;
+  const int64_t p = p0 < a.paths ? p0 : a.paths - 1;
+  const Real v = bermudanSum<Real>(S, j, p, a.paths, toConst(a.berms), toConst(a.bdates), toConst(a.cont), a.exidx,
+                                   a.states, toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off),
+                                   toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(a.poly) + S.poly_off);
+  if (p0 < a.paths) a.bermv[static_cast<int64_t>(d) * a.paths + p0] = static_cast<double>(v);
 }
 
 hipError_t last() { return hipGetLastError(); }
@@ -327,7 +378,8 @@ hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int
   if (hipError_t e = last()) return e;
   hipLaunchKernelGGL((amc_ev_kernel<Real>), dim3(a.nchunks, a.ncols), blk, 0, stream, a, bonds, fixed);
   if (hipError_t e = last()) return e;
-  const dim3 gb(a.nchunks, a.nberms), gs((a.nberms + 255) / 256);
+  // One wavefront per item in the finishing kernels.
+  const dim3 gb(a.nchunks, a.nberms), gs((a.nberms + 3) / 4);
   for (int k = max_exercises - 1; k >= 0; --k) {
     hipLaunchKernelGGL(amc_mom1_kernel, gb, blk, 0, stream, a, k);
     hipLaunchKernelGGL(amc_fin1_kernel, gs, dim3(256), 0, stream, a, k);
@@ -337,17 +389,19 @@ hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int
     if (hipError_t e = last()) return e;
   }
   if (a.ncdates > 0) {
-    const dim3 gd(a.nchunks, a.ncdates), gds((a.ncdates + 255) / 256);
+    const dim3 gd(a.nchunks, a.ncdates), gds((a.ncdates + 3) / 4);
     hipLaunchKernelGGL(amc_cmom1_kernel, gd, blk, 0, stream, a);
     hipLaunchKernelGGL(amc_cfin1_kernel, gds, dim3(256), 0, stream, a);
     hipLaunchKernelGGL(amc_design_kernel, gd, blk, 0, stream, a);
     hipLaunchKernelGGL(amc_gram_kernel, gds, dim3(256), 0, stream, a);
-    const int M = 4 * a.ncdates, N = a.ncols;
-    hipLaunchKernelGGL(amc_gemm_nt_kernel, dim3((N + kGT - 1) / kGT, (M + kGT - 1) / kGT), dim3(256), 0, stream, a.W,
-                       a.strategy, a.C, M, N, P);
+    if (a.ntiles > 0)
+      hipLaunchKernelGGL(amc_cgemm_kernel, dim3(a.ntiles, a.nkc), dim3(256), 0, stream, a);
     if (hipError_t e = last()) return e;
   }
   hipLaunchKernelGGL(amc_csolve_kernel, dim3((a.nbdates + 255) / 256), dim3(256), 0, stream, a);
+  if (a.ncdates > 0)
+    hipLaunchKernelGGL((amc_value_kernel<Real>), dim3((a.nchunks + 7) / 8 * 8, a.ncdates), blk, 0, stream, a, bonds,
+                       fixed);
   return last();
 }
 

@@ -65,6 +65,7 @@ struct ccr_hip_plan {
   double* d_cont = nullptr;
   int16_t* d_exidx = nullptr;
   double* d_states = nullptr;
+  double* d_bermv = nullptr;
   AmcArgs amc{};
   double draws_ms = 0.0, upload_ms = 0.0;
   ~ccr_hip_plan() {
@@ -238,10 +239,15 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.coef = dalloc(static_cast<int64_t>(nB) * 4);
     a.csum = dalloc(std::max(1, ncd) * 3);
     a.gram = dalloc(std::max(1, ncd) * 10);
+    const int nkc = static_cast<int>((P + kGemmK - 1) / kGemmK);
     a.W = ncd > 0 ? dalloc(static_cast<int64_t>(ncd) * 4 * P) : nullptr;
-    a.C = ncd > 0 ? dalloc(static_cast<int64_t>(ncd) * 4 * ncols) : nullptr;
+    a.tiles = h.tiles.empty() ? nullptr : upload(ctx, plan.get(), h.tiles.data(), h.tiles.size());
+    a.gcols = h.gcols.empty() ? nullptr : upload(ctx, plan.get(), h.gcols.data(), h.gcols.size());
+    a.gpart = dalloc(static_cast<int64_t>(nkc) * static_cast<int64_t>(h.bdates.size()) * 4);
     a.cont = plan->d_cont;
     a.exidx = plan->d_exidx;
+    a.bermv = ncd > 0 ? dalloc(static_cast<int64_t>(ncd) * P) : nullptr;
+    plan->d_bermv = a.bermv;
     a.paths = P;
     a.state_len = h.state_len;
     a.nberms = nB;
@@ -249,6 +255,8 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.ncdates = ncd;
     a.nbdates = static_cast<int>(h.bdates.size());
     a.nchunks = nchunks;
+    a.ntiles = static_cast<int>(h.tiles.size());
+    a.nkc = nkc;
   }
   check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   plan->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -271,10 +279,7 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   a.fixed = static_cast<const Real*>(plan->d_fixed);
   a.fixstore = plan->d_fixstore;
   a.poly = plan->d_poly;
-  a.berms = plan->d_berms;
-  a.bdates = plan->d_bdates;
-  a.cont = plan->d_cont;
-  a.exidx = plan->d_exidx;
+  a.bermv = plan->d_bermv;
   a.states = nullptr;
   a.rows = static_cast<Real*>(plan->d_rows);
   a.partials = plan->d_partials;

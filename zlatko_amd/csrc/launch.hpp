@@ -36,10 +36,7 @@ struct PathArgs {
   const Real* fixed;
   const int32_t* fixstore;
   const double* poly;  // bond-polynomial coefficients (FP64 in both precisions)
-  const BermDev* berms;     // American Monte Carlo: swaptions, per-date records, fitted continuation
-  const BermDate* bdates;
-  const double* cont;
-  const int16_t* exidx;     // [Bermudan][path] first exercise index (ne: never)
+  const double* bermv;      // American Monte Carlo: [continuation date][path] sum of the set's swaptions
   double* states;           // states-only pass: [set dates][path] x then D (null otherwise)
   Real* rows;
   double* partials;
@@ -53,7 +50,8 @@ struct PathArgs {
 
 // American Monte Carlo regressions (amc.hip). Layouts: states x then D, [set dates][path];
 // ev and strategy [global exercise][path]; part [item][chunk][moments]; W [cdate][4][path];
-// C [cdate x 4][global exercise]; cont [Bermudan date record][kContStride].
+// gpart [path chunk][date record][4] (continuation X'y partials); cont [date record][kContStride];
+// bermv [cdate][path].
 struct AmcArgs {
   const SetDev* sets;
   const BermDev* berms;
@@ -74,11 +72,14 @@ struct AmcArgs {
   double* csum;               // [cdate][3]: n, sum x, sum x^2
   double* gram;               // [cdate][10]
   double* W;
-  double* C;
+  const GemmTile* tiles;
+  const int32_t* gcols;       // strategy column per (tile, swaption), -1: none
+  double* gpart;
   double* cont;
   int16_t* exidx;             // [Bermudan][path]
+  double* bermv;
   int64_t paths, state_len;
-  int32_t nberms, ncols, ncdates, nbdates, nchunks;
+  int32_t nberms, ncols, ncdates, nbdates, nchunks, ntiles, nkc;
 };
 
 template <typename Real>

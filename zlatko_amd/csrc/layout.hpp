@@ -75,17 +75,25 @@ constexpr int kPolyMaxDegree = 48;
 // Bermudan swaptions are valued by Longstaff-Schwartz regressions on the domestic state
 // (src/instruments/bermudan_swaption.cpp:49-146): a states-only pass of the path kernel records
 // x and D per date, batched regression kernels fit the exercise policy backwards over the exercise
-// dates and the continuation value on every valued date, and the path kernel then values each
-// swaption per path: the underlying swap once exercised (physical; 0 for cash), else
-// max(continuation, 0), times the position.
+// dates and the continuation value on every valued date, a valuation kernel sums each set's
+// swaptions per (valued date, path) -- the underlying swap once exercised (physical; 0 for cash),
+// else max(continuation, 0), times the position -- and the path kernel adds that sum to the
+// netting-set value.
 constexpr int kMaxExercises = 64;
 constexpr int kAmcChunk = 256;   // paths per workgroup of the regression reductions
 constexpr int kRegMom1 = 4;      // masked moments: n, sum x, sum x^2, sum y
 constexpr int kRegMom2 = 14;     // Gram lower triangle (10) + X'y (4)
 constexpr int kContStride = 8;   // per (Bermudan, date): center, scale, c0..c3, degree, pad
+// Continuation right-hand sides X'y for every (Bermudan, valued date): a gathered GEMM per
+// exercise epoch (run of valued dates on which every swaption of the set has the same next
+// exercise), tiles of kGemmDates dates x kGemmCols swaptions, split over paths in kGemmK chunks.
+constexpr int kGemmDates = 8;
+constexpr int kGemmCols = 64;
+constexpr int kGemmK = 4096;
 
 // Value terms of one instrument on one date: bond polynomial (or its exponential terms) plus
-// fixed coupons; indices into the plan's global poly / bonds / fixed arrays.
+// fixed coupons; indices relative to the set's poly / bonds / fixed tables. The fixed coupons of
+// a TermRange carry the fixing date index (states row) in their fourth element.
 struct TermRange {
   int32_t poly_beg, poly_len;
   int32_t bond_beg, bond_end;
@@ -108,6 +116,15 @@ struct BermDate {
   TermRange swap;     // value of the underlying swap at t_j
 };
 
+// One tile of the continuation GEMM: dates [d0, d0 + nd) of one epoch (continuation-date
+// indices), swaptions [b0, b0 + nb), and the strategy column of each swaption in the epoch
+// (cols[i] = exercise-value row, -1: no exercise left).
+struct GemmTile {
+  int32_t d0, nd, b0, nb;
+  int32_t col_off;    // first of nb entries in the plan's column table
+  int32_t pad[3];
+};
+
 struct SetDev {
   int64_t stream_stride;   // normals per path block: (T - 1) * nz
   int64_t rows_off;        // first element of this set's exposure rows
@@ -119,6 +136,7 @@ struct SetDev {
   int32_t ring_len, has_csa, sec_type, csa_asset;
   int32_t cpty, own, num_slots;
   int32_t berm_beg, berm_end;  // Bermudan swaptions of this set
+  int32_t cd_base;             // first continuation date (one per valued date) of this set's rows
   int64_t state_off;           // first element of this set's [date][path] states (AMC), -1 without
   double thr_cpty, thr_own, mta, ia, keep;  // CSA terms (thr_own < 0 encodes a one-way CSA)
 };

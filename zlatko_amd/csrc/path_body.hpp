@@ -30,15 +30,6 @@
 
 namespace ccrhip {
 
-// American Monte Carlo inputs of the valuation pass.
-struct AmcView {
-  cptr<BermDev> berms;
-  cptr<BermDate> bdates;
-  cptr<double> cont;
-  const int16_t* exidx;  // [Bermudan][path]
-  int64_t paths;
-};
-
 // exp(a) for |a| < 700 in FP64 (no overflow / underflow handling: bond-term exponents are
 // O(1)). Cody-Waite reduction a = n ln2 + r, |r| <= ln2/2, with the two-part ln2 of fdlibm's
 // e_exp.c; exp(r) by its degree-13 Taylor polynomial (truncation < 5e-18 relative on the
@@ -91,10 +82,11 @@ __host__ __device__ inline Real sqrtr(Real v) {
 
 // Value of a TermRange (an instrument's terms on one date, indices relative to its set's tables)
 // at domestic state x: the bond polynomial when every lane is within its bound, else the
-// exponential terms; plus the fixed coupons, whose fixing states come from the slot column xs.
-template <typename Real>
+// exponential terms; plus the fixed coupons, whose fixing states fixing(k) come from the states
+// row k named in each coupon's fourth element.
+template <typename Real, typename Fixing>
 __host__ __device__ inline Real evalRange(const CCR_CONST TermRange& tr, Real x, cptr<Real> bonds, cptr<Real> fixed,
-                                          cptr<double> poly, const Real* xs, int stride) {
+                                          cptr<double> poly, Fixing&& fixing) {
   Real acc;
   if (tr.poly_len > 0 && allLanes(static_cast<double>(x < Real(0) ? -x : x) <= tr.xmax)) {
     cptr<double> pc = poly + tr.poly_beg;
@@ -114,10 +106,46 @@ __host__ __device__ inline Real evalRange(const CCR_CONST TermRange& tr, Real x,
   }
   for (int i = tr.fix_beg; i < tr.fix_end; ++i) {
     cptr<Real> f = fixed + 4 * static_cast<int64_t>(i);
-    const Real xk = xs[static_cast<int>(f[3]) * stride];
+    const Real xk = fixing(static_cast<int>(f[3]));
     acc += f[0] * expTerm(-f[1] * x - f[2] * xk);
   }
   return acc;
+}
+
+// Sum over the Bermudan swaptions of set S of position x value on date j of path p
+// (bermudan_swaption.cpp:104-140): the underlying swap once exercised before the next exercise
+// date (physical; 0 for cash), else max(continuation, 0). states holds x then D ([date][path]
+// from S.state_off, D after state_len); bonds / fixed / poly are the set's tables.
+template <typename Real>
+__host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, int64_t p, int64_t paths,
+                                            cptr<BermDev> berms, cptr<BermDate> bdates, cptr<double> cont,
+                                            const int16_t* exidx, const double* states, cptr<Real> bonds,
+                                            cptr<Real> fixed, cptr<double> poly) {
+  const double* xrow = states + S.state_off;
+  const double xd = xrow[static_cast<int64_t>(j) * paths + p];
+  const Real x = static_cast<Real>(xd);
+  auto fixing = [&](int k) { return static_cast<Real>(xrow[static_cast<int64_t>(k) * paths + p]); };
+  Real V = 0;
+  for (int b = S.berm_beg; b < S.berm_end; ++b) {
+    const CCR_CONST BermDev& bm = berms[b];
+    const CCR_CONST BermDate& bd = bdates[bm.date_rec + j];
+    const bool exercised = exidx[static_cast<int64_t>(b) * paths + p] < bd.next_k;
+    Real v = 0;
+    if (bd.next_k < bm.ne) {
+      cptr<double> c = cont + static_cast<int64_t>(bm.date_rec + j) * kContStride;
+      const double coef[4] = {c[2], c[3], c[4], c[5]};
+      const double pred = regPredict(xd, c[0], c[1], static_cast<int>(c[6]), coef);
+      v = static_cast<Real>(pred > 0.0 ? pred : 0.0);
+    }
+    if (bd.has_swap && !allLanes(!exercised)) {
+      const Real sv = evalRange<Real>(bd.swap, x, bonds, fixed, poly, fixing);
+      if (exercised) v = sv;
+    } else if (exercised) {
+      v = 0;
+    }
+    V += static_cast<Real>(bm.position) * v;
+  }
+  return V;
 }
 
 // Model shapes the path kernel is specialised for; everything else runs the generic shape.
@@ -156,7 +184,7 @@ template <typename Real, typename Sh, bool kStatesOnly, typename Emit, typename 
 __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
-                                             cptr<int32_t> fstore, cptr<double> poly, AmcView amc, int ring,
+                                             cptr<int32_t> fstore, cptr<double> poly, const double* bermv, int ring,
                                              int priceRows, Real* col, int stride, Emit&& emit, OnState&& onState) {
   constexpr bool kDouble = std::is_same<Real, double>::value;
   constexpr int NR = Sh::NR, NA = Sh::NA, NC = Sh::NC;
@@ -187,6 +215,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     Qs[c] = 1;
   }
   Real units = 0;
+  const double* bermRow = bermv ? bermv + static_cast<int64_t>(S.cd_base) * g.num_paths + pe : nullptr;
   double prevDvp = 0, prevDvn = 0, prevQc = 1, prevQo = 1;
 
   // Normals of the next step, loaded one step ahead: correlated drivers, then the integral
@@ -362,27 +391,11 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 #pragma unroll
       for (int i = 0; i < NA; ++i)
         if (i < nA) V += S_[i] * dr[kDateLin + i];
-      // Bermudan swaptions (bermudan_swaption.cpp:104-140): the underlying once exercised before
-      // the next exercise date (physical; 0 for cash), else max(continuation, 0).
-      for (int b = S.berm_beg; b < S.berm_end; ++b) {
-        const CCR_CONST BermDev& bm = amc.berms[b];
-        const CCR_CONST BermDate& bd = amc.bdates[bm.date_rec + j];
-        const int kx = amc.exidx[static_cast<int64_t>(b) * amc.paths + pe];
-        const bool exercised = kx < bd.next_k;
-        Real v = 0;
-        if (bd.next_k < bm.ne) {
-          cptr<double> c = amc.cont + static_cast<int64_t>(bm.date_rec + j) * kContStride;
-          const double coef[4] = {c[2], c[3], c[4], c[5]};
-          const double pred = regPredict(static_cast<double>(x[0]), c[0], c[1], static_cast<int>(c[6]), coef);
-          v = static_cast<Real>(pred > 0.0 ? pred : 0.0);
-        }
-        if (bd.has_swap && !allLanes(!exercised)) {
-          const Real sv = evalRange<Real>(bd.swap, x[0], bonds, fixed, poly, xs, stride);
-          if (exercised) v = sv;
-        } else if (exercised) {
-          v = 0;
-        }
-        V += static_cast<Real>(bm.position) * v;
+      // Bermudan swaptions: their per-path sum on this valued date, from the valuation kernel
+      // (one row per valued date of the set, in date order).
+      if (bermRow) {
+        V += static_cast<Real>(*bermRow);
+        bermRow += g.num_paths;
       }
     }
     Real C = 0;
@@ -452,7 +465,7 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDe
   const CCR_CONST SetDev& S = *sp;
   cptr<GlobalDev> g = toConst(a.g);
   const int64_t q = g->antithetic ? (pe >> 1) : pe;
-  const AmcView amc{toConst(a.berms), toConst(a.bdates), toConst(a.cont), a.exidx, g->num_paths};
+  const double* bermv = S.berm_end > S.berm_beg ? a.bermv : nullptr;
   double* states = a.states;
   const int64_t stateOff = S.state_off, paths = g->num_paths, stateLen = a.state_len;
   auto onState = [&](int j, Real x, Real D) {
@@ -468,7 +481,7 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDe
                                       toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
                                       toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
                                       toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
-                                      toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off, amc,
+                                      toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off, bermv,
                                       a.ring, a.price_rows, col, stride, emit, onState);
 }
 

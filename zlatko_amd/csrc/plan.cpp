@@ -491,11 +491,6 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
         c.t = t;
         collect(und, m, sim, c);
         has[static_cast<std::size_t>(j)] = 1;
-        for (const auto& kv : c.fix) {
-          const auto key = std::make_pair(std::get<0>(kv.first), std::get<1>(kv.first));
-          auto it = lastUse.find(key);
-          if (it == lastUse.end() || it->second < j) lastUse[key] = j;
-        }
       }
     }
     // Slots: interval colouring of [fixing date, last use].
@@ -676,7 +671,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
         plan.fixed.push_back(kv.second * (As / Ae) * hw.A(t, te));
         plan.fixed.push_back(hw.B(t, te));
         plan.fixed.push_back(Bs - Be);
-        plan.fixed.push_back(static_cast<double>(slotOf.at({0, k})));
+        plan.fixed.push_back(static_cast<double>(k));  // fixing date: a row of the recorded states
       }
       tr.fix_end = static_cast<int32_t>(plan.fixed.size() / 4);
       tr.c0 = col.c0[0];
@@ -702,6 +697,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     if (nB > 0) plan.state_len += static_cast<int64_t>(T) * P;
     // Continuation regressions run on every valued date of a set with Bermudans.
     std::vector<int32_t> cdOf(static_cast<std::size_t>(T), -1);
+    sd.cd_base = static_cast<int32_t>(plan.cdates.size() / 2);
     if (nB > 0)
       for (int j = 0; j < T; ++j)
         if (valued[static_cast<std::size_t>(j)]) {
@@ -758,6 +754,46 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       plan.berms.push_back(bd);
     }
     sd.berm_end = static_cast<int32_t>(plan.berms.size());
+    // Continuation GEMM tiles: runs of valued dates with the same next exercise for every swaption
+    // (an epoch), cut into kGemmDates dates x kGemmCols swaptions; epochs past every swaption's
+    // last exercise need no regression.
+    if (nB > 0) {
+      std::vector<int> vdates;
+      for (int j = 0; j < T; ++j)
+        if (valued[static_cast<std::size_t>(j)]) vdates.push_back(j);
+      auto sameEpoch = [&](int ja, int jb) {
+        for (int b = 0; b < nB; ++b)
+          if (bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(ja)] !=
+              bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(jb)])
+            return false;
+        return true;
+      };
+      std::size_t i0 = 0;
+      while (i0 < vdates.size()) {
+        std::size_t i1 = i0 + 1;
+        while (i1 < vdates.size() && i1 - i0 < static_cast<std::size_t>(kGemmDates) && sameEpoch(vdates[i0], vdates[i1])) ++i1;
+        const int j0 = vdates[i0];
+        bool any = false;
+        for (int b = 0; b < nB; ++b)
+          any = any || bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(j0)] <
+                           static_cast<int>(berms[static_cast<std::size_t>(b)].exercise.size());
+        for (int b0 = 0; any && b0 < nB; b0 += kGemmCols) {
+          GemmTile tile{};
+          tile.d0 = sd.cd_base + static_cast<int32_t>(i0);
+          tile.nd = static_cast<int32_t>(i1 - i0);
+          tile.b0 = sd.berm_beg + b0;
+          tile.nb = std::min(kGemmCols, nB - b0);
+          tile.col_off = static_cast<int32_t>(plan.gcols.size());
+          for (int b = b0; b < b0 + tile.nb; ++b) {
+            const BermDev& bm = plan.berms[static_cast<std::size_t>(sd.berm_beg + b)];
+            const int k = bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(j0)];
+            plan.gcols.push_back(k < bm.ne ? bm.ev_off + k : -1);
+          }
+          plan.tiles.push_back(tile);
+        }
+        i0 = i1;
+      }
+    }
     plan.revaluations += static_cast<int64_t>(ns.num_trades) * P * R;
     sd.rows_off = static_cast<int64_t>(s) * 0;  // filled below once max_rep is known
     plan.sets.push_back(sd);

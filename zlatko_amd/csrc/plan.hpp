@@ -46,6 +46,8 @@ struct HostPlan {
   std::vector<int32_t> cdates;      // continuation dates: (set, date) pairs
   std::vector<int32_t> bdate_berm;  // per BermDate record: its Bermudan
   std::vector<int32_t> bdate_cd;    // per BermDate record: its continuation date (-1: none)
+  std::vector<GemmTile> tiles;      // continuation GEMM tiles
+  std::vector<int32_t> gcols;       // strategy column per (tile, swaption), -1: none
 };
 
 // ExposureEngine::reportingGrid / simulationGrid (src/exposure/exposure_engine.cpp:35-48).

@@ -121,15 +121,20 @@ typedef struct ccr_simulation_config {
 enum ccr_trade_kind {
   CCR_TRADE_SWAP = 0,               /* ccr::InterestRateSwap (interest_rate_swap.hpp:12-66) */
   CCR_TRADE_ASSET_FORWARD = 1,      /* ccr::AssetForward (asset_forward.hpp:11-36) */
-  CCR_TRADE_BERMUDAN_SWAPTION = 2   /* ccr::BermudanSwaption (bermudan_swaption.hpp:26-62) */
+  CCR_TRADE_BERMUDAN_SWAPTION = 2,  /* ccr::BermudanSwaption (bermudan_swaption.hpp:26-62) */
+  CCR_TRADE_EUROPEAN_OPTION = 3     /* ccr::EuropeanOption (european_option.hpp:18-41) */
 };
+
+enum ccr_option_type { CCR_OPTION_CALL = 0, CCR_OPTION_PUT = 1 };  /* ccr::OptionType (math.hpp:12) */
 
 /* One trade. Swap (and the Bermudan's underlying): `schedule` T_0 < ... < T_n as in the
  * reference (both legs accrue on it). A swap with a different floating-leg schedule sets
  * float_schedule (its fixed leg accrues on `schedule`); NULL/0 means the reference's single
  * schedule. currency 0 = domestic; f + 1 = foreign Hull-White f, valued in domestic units at
  * the asset whose foreign_rate is f. Forward: asset, notional, strike (= fixed_rate),
- * maturity. Bermudan: exercise_times, cash_settled, position, regression_degree. */
+ * maturity. Bermudan: exercise_times, cash_settled, position, regression_degree. European
+ * option: asset (deterministic carry), option_type, notional (negative: short), strike
+ * (= fixed_rate), expiry (= maturity). */
 typedef struct ccr_trade {
   int32_t kind;
   int32_t pay_fixed;
@@ -147,6 +152,7 @@ typedef struct ccr_trade {
   int32_t cash_settled;
   double position;
   int32_t regression_degree;
+  int32_t option_type;
 } ccr_trade;
 
 enum ccr_collateral_type { CCR_COLLATERAL_CASH = 0, CCR_COLLATERAL_BOND = 1, CCR_COLLATERAL_ASSET = 2 };

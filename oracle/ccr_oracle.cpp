@@ -183,6 +183,17 @@ double quantile(Vec v, double p) {
   return v[lo] + (h - static_cast<double>(lo)) * (v[hi] - v[lo]);
 }
 
+// normalCdf and blackFormula: include/ccr/core/math.hpp:15-31. N(x) = erfc(-x / sqrt 2) / 2;
+// discount times E[(s (F_T - K))^+] with total log-sd stdDev, intrinsic value when degenerate.
+double normalCdf(double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); }
+double blackFormula(bool call, double forward, double strike, double stdDev, double discount) {
+  const double sign = call ? 1.0 : -1.0;
+  if (stdDev <= 1e-14 || forward <= 0.0 || strike <= 0.0) return discount * std::max(sign * (forward - strike), 0.0);
+  const double d1 = std::log(forward / strike) / stdDev + 0.5 * stdDev;
+  const double d2 = d1 - stdDev;
+  return discount * sign * (forward * normalCdf(sign * d1) - strike * normalCdf(sign * d2));
+}
+
 // ---------------------------------------------------------------- time grid
 // src/core/time_grid.cpp:10-66
 constexpr double kTimeTol = 1e-9;
@@ -626,9 +637,10 @@ double swapValue(const Swap& sw, const HW& m, const Scenarios& s, const Mat& X, 
 struct Trade {
   int kind = CCR_TRADE_SWAP;
   Swap swap;
-  // forward
+  // forward / European option (maturity = expiry)
   int asset = 0;
   double notional = 0.0, strike = 0.0, maturity = 0.0;
+  int optionType = CCR_OPTION_CALL;
   // Bermudan
   Vec exercise;
   bool cash = false;
@@ -636,7 +648,7 @@ struct Trade {
   std::size_t degree = 3;
 
   Vec eventTimes() const {
-    if (kind == CCR_TRADE_ASSET_FORWARD) return {maturity};
+    if (kind == CCR_TRADE_ASSET_FORWARD || kind == CCR_TRADE_EUROPEAN_OPTION) return {maturity};
     Vec e = swap.sched;
     e.insert(e.end(), swap.fsched.begin(), swap.fsched.end());
     if (kind == CCR_TRADE_BERMUDAN_SWAPTION) e.insert(e.end(), exercise.begin(), exercise.end());
@@ -672,6 +684,19 @@ Trade makeTrade(const ccr_trade& d, const Generator& g) {
     t.notional = d.notional;
     t.strike = d.fixed_rate;
     t.maturity = d.maturity;
+  } else if (d.kind == CCR_TRADE_EUROPEAN_OPTION) {
+    // EuropeanOption ctor: src/instruments/european_option.cpp:9-18
+    if (d.maturity <= 0.0 || d.fixed_rate <= 0.0) invalid("EuropeanOption: invalid expiry or strike");
+    if (d.asset < 0 || d.asset >= static_cast<int>(g.assets.size()))
+      throw Failure(CCR_ERR_OUT_OF_RANGE, "ScenarioSet: unknown asset");
+    if (g.assets[static_cast<std::size_t>(d.asset)].foreign >= 0)
+      throw Failure(CCR_ERR_UNSUPPORTED, "EuropeanOption: the asset needs a deterministic carry curve");
+    if (d.option_type != CCR_OPTION_CALL && d.option_type != CCR_OPTION_PUT) invalid("EuropeanOption: unknown type");
+    t.asset = d.asset;
+    t.notional = d.notional;
+    t.strike = d.fixed_rate;
+    t.maturity = d.maturity;
+    t.optionType = d.option_type;
   } else {
     invalid("unknown trade kind");
   }
@@ -784,8 +809,18 @@ double pathwiseValue(const Trade& tr, const Generator& g, const Scenarios& s, st
     const double vf = swapValue(tr.swap, g.foreign[f], s, s.xf[f], p, j);
     return vf * s.S[static_cast<std::size_t>(g.fxOf[f])](p, j);
   }
-  // AssetForward::valueAtTime: src/instruments/asset_forward.cpp:13-25
   const double t = s.grid[j];
+  if (tr.kind == CCR_TRADE_EUROPEAN_OPTION) {
+    // EuropeanOption::valueAtTime: src/instruments/european_option.cpp:20-35
+    if (t >= tr.maturity - 1e-10) return 0.0;
+    const Asset& as = g.assets[static_cast<std::size_t>(tr.asset)];
+    const double carry = as.carryDiscount(t, tr.maturity);
+    const double stdDev = as.vol * std::sqrt(tr.maturity - t);
+    const double df = g.rates.zeroBond(t, tr.maturity, s.x(p, j));
+    const double forward = s.S[static_cast<std::size_t>(tr.asset)](p, j) * carry / df;
+    return tr.notional * blackFormula(tr.optionType == CCR_OPTION_CALL, forward, tr.strike, stdDev, df);
+  }
+  // AssetForward::valueAtTime: src/instruments/asset_forward.cpp:13-25
   if (t >= tr.maturity - 1e-10) return 0.0;
   const Asset& as = g.assets[static_cast<std::size_t>(tr.asset)];
   const double carry = as.foreign < 0 ? as.carryDiscount(t, tr.maturity)
@@ -1005,7 +1040,7 @@ std::vector<Trade> tradesOf(const ccr_netting_set* set, const Generator* g) {
     Trade t;
     const ccr_trade& d = set->trades[i];
     t.kind = d.kind;
-    if (d.kind == CCR_TRADE_ASSET_FORWARD) {
+    if (d.kind == CCR_TRADE_ASSET_FORWARD || d.kind == CCR_TRADE_EUROPEAN_OPTION) {
       t.maturity = d.maturity;
     } else {
       if (d.schedule && d.schedule_size > 0) t.swap.sched.assign(d.schedule, d.schedule + d.schedule_size);

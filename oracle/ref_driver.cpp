@@ -24,6 +24,7 @@
 #include "ccr/exposure/exposure_engine.hpp"
 #include "ccr/instruments/asset_forward.hpp"
 #include "ccr/instruments/bermudan_swaption.hpp"
+#include "ccr/instruments/european_option.hpp"
 #include "ccr/instruments/interest_rate_swap.hpp"
 
 namespace {
@@ -88,6 +89,10 @@ void addTrades(const ccr_trade& d, int i, ccr::NettingSet& ns) {
   } else if (d.kind == CCR_TRADE_ASSET_FORWARD) {
     ns.trades.push_back(
         std::make_shared<ccr::AssetForward>(id, "A" + std::to_string(d.asset), d.notional, d.fixed_rate, d.maturity));
+  } else if (d.kind == CCR_TRADE_EUROPEAN_OPTION) {
+    ns.trades.push_back(std::make_shared<ccr::EuropeanOption>(
+        id, "A" + std::to_string(d.asset), d.option_type == CCR_OPTION_CALL ? ccr::OptionType::Call : ccr::OptionType::Put,
+        d.notional, d.fixed_rate, d.maturity));
   } else if (d.kind == CCR_TRADE_BERMUDAN_SWAPTION) {
     ns.trades.push_back(std::make_shared<ccr::BermudanSwaption>(
         id, swapOf(id + "u", d.schedule, d.schedule_size, d.notional, d.fixed_rate, d.pay_fixed != 0),

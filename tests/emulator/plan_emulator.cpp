@@ -160,6 +160,7 @@ extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr
     a.fixed = fixed.data();
     a.fixstore = h.fixstore.data();
     a.poly = h.poly.data();
+    a.opts = h.opts.data();
     a.max_rep = h.max_rep;
     a.ring = h.g.max_ring;
     a.price_rows = h.price_rows;

@@ -31,6 +31,7 @@ CASES = {
     "all": dict(num_paths=128, csa=True, fx=True, credits=True, two_leg=True),
     "bermudan_csa": dict(num_paths=200, csa=True, fx=True, bermudans=3),
     "bermudan_cash": dict(num_paths=200, bermudans=3, cash_settled=True),
+    "options": dict(num_paths=200, fx=True, csa=True, options=5),
 }
 
 

@@ -45,20 +45,6 @@ def test_bermudan_golden_vectors_fp64(engine):
         assert err <= FP64_TOL, f
 
 
-@pytest.mark.parametrize("case", ["bermudan_csa", "bermudan_cash"])
-def test_bermudan_sets_golden_vectors_fp64(engine, case):
-    """Bermudans (physical under CSA with FX forwards; cash-settled) next to swaps: the reference
-    engine's profiles and pathwise CVA inputs."""
-    g = GOLDEN["cases"][case]
-    spec = W.small_reference_case(**g["params"])
-    out = engine.run(spec)[0]
-    scale = max(np.abs(np.array(g["profile"]["expected_exposure"])).max(), 1.0)
-    for f, v in g["profile"].items():
-        err = np.abs(out.get(f) - np.array(v)).max() / (1.0 if f == "times" else scale)
-        print(case, f, f"{err:.2e}", flush=True)
-        assert err <= (0.0 if f == "times" else FP64_TOL), f
-
-
 def test_c3_reduced_fp64(engine):
     """Config 3 shape (Bermudans 1-10y into 5-10y, annual exercise, degree 3), fewer paths."""
     spec = W.c3(num_paths=4000, dates=60, num_swaptions=40)

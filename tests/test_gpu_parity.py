@@ -3,6 +3,9 @@
 FP64 bar: every profile field within 1e-10 scale-relative (see compare.py), reporting dates
 bit-identical. FP32 runs state their error against the FP64 oracle.
 """
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -21,6 +24,7 @@ SMALL = {
     "credits": dict(credits=True),
     "two_leg": dict(two_leg=True),
     "all": dict(csa=True, fx=True, credits=True, two_leg=True),
+    "options": dict(fx=True, csa=True, credits=True, options=6),
 }
 
 
@@ -36,6 +40,26 @@ def _check(spec, outs, ref, tol=FP64_TOL):
 def test_small_cases_fp64(engine, case):
     spec = W.small_reference_case(num_paths=256, **SMALL[case])
     _check(spec, engine.run(spec), oracle_lib.run_oracle(spec, threads=4))
+
+
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_vectors.json").read_text())
+
+
+@pytest.mark.parametrize("case", sorted(GOLDEN["cases"]))
+def test_reference_golden_vectors_fp64(engine, case):
+    """The profiles a build of the reference engine produced on the same inputs (swaps, CSA, FX
+    forwards, CIR++ credits, two-leg swaps, Bermudans physical and cash, FX options)."""
+    g = GOLDEN["cases"][case]
+    spec = W.small_reference_case(**g["params"])
+    out = engine.run(spec)[0]
+    scale = max(np.abs(np.array(g["profile"]["expected_exposure"])).max(), 1.0)
+    for f, v in g["profile"].items():
+        err = np.abs(out.get(f) - np.array(v)).max() / (1.0 if f == "times" else scale)
+        print(case, f, f"{err:.2e}", flush=True)
+        assert err <= (0.0 if f == "times" else FP64_TOL), f
+    if "pathwise_cva" in g and spec.sets[0].counterparty_credit >= 0:
+        cva = out.get("cva_increments")[1:].sum()
+        assert abs(cva - g["pathwise_cva"]) <= FP64_TOL * max(1.0, abs(g["pathwise_cva"]))
 
 
 def test_c1_full_size_fp64(engine):
@@ -68,6 +92,16 @@ def test_fp32_error_vs_fp64_oracle(engine):
         print("fp32", s, {k: f"{v:.2e}" for k, v in errs.items()})
         assert errs["times"] == 0.0
         assert errs["expected_exposure"] < 1e-4 and errs["discounted_expected_exposure"] < 1e-4
+
+
+def test_fp32_options_error_vs_fp64_oracle(engine):
+    """FX options (Black in FP32: logf, erfcf) next to swaps and forwards."""
+    spec = W.small_reference_case(num_paths=2000, fx=True, csa=True, credits=True, options=6)
+    ref = oracle_lib.run_oracle(spec, threads=8)
+    out = engine.run(spec, abi.CCR_FP32)
+    errs = compare.field_errors(out[0], ref[0])
+    print("fp32 options", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert errs["expected_exposure"] < 1e-4 and errs["discounted_expected_exposure"] < 1e-4
 
 
 @pytest.mark.parametrize("paths,precision", [(1, abi.CCR_FP64), (2, abi.CCR_FP64), (3, abi.CCR_FP64),

@@ -251,7 +251,8 @@ def test_oracle_bermudan_matches_reference_golden_vectors():
 @pytest.mark.parametrize("kw", [dict(num_paths=200, csa=True, fx=True, credits=True),
                                 dict(num_paths=150, two_leg=True, csa=True),
                                 dict(num_paths=120, csa=True, fx=True, credits=True, bermudans=4),
-                                dict(num_paths=100, bermudans=2, cash_settled=True)])
+                                dict(num_paths=100, bermudans=2, cash_settled=True),
+                                dict(num_paths=150, fx=True, csa=True, credits=True, options=6)])
 def test_oracle_matches_live_reference_build(kw):
     spec = W.small_reference_case(**kw)
     o = oracle_lib.run_oracle(spec, threads=3)[0]

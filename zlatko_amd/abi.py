@@ -25,6 +25,9 @@ CCR_ERR_RUNTIME = 6
 CCR_TRADE_SWAP = 0
 CCR_TRADE_ASSET_FORWARD = 1
 CCR_TRADE_BERMUDAN_SWAPTION = 2
+CCR_TRADE_EUROPEAN_OPTION = 3
+CCR_OPTION_CALL = 0
+CCR_OPTION_PUT = 1
 
 CCR_COLLATERAL_CASH = 0
 CCR_COLLATERAL_BOND = 1
@@ -76,7 +79,8 @@ class TradeC(C.Structure):
                 ("notional", C.c_double), ("fixed_rate", C.c_double), ("maturity", C.c_double),
                 ("schedule", DP), ("schedule_size", C.c_int32), ("float_schedule", DP),
                 ("float_schedule_size", C.c_int32), ("exercise_times", DP), ("num_exercises", C.c_int32),
-                ("cash_settled", C.c_int32), ("position", C.c_double), ("regression_degree", C.c_int32)]
+                ("cash_settled", C.c_int32), ("position", C.c_double), ("regression_degree", C.c_int32),
+                ("option_type", C.c_int32)]
 
 
 class CollateralC(C.Structure):
@@ -190,6 +194,7 @@ class Trade:
     cash_settled: bool = False
     position: float = 1.0
     degree: int = 3
+    option_type: int = 0
 
 
 @dataclass
@@ -313,6 +318,7 @@ class Spec:
             e = self._arr(t.exercise)
             c.exercise_times, c.num_exercises = dptr(e), len(e)
         c.cash_settled, c.position, c.regression_degree = 1 if t.cash_settled else 0, t.position, t.degree
+        c.option_type = t.option_type
         return c
 
     def _set(self, ns: NettingSet) -> NettingSetC:

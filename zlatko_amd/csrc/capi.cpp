@@ -55,6 +55,7 @@ struct ccr_hip_plan {
   void* d_fixed = nullptr;
   int32_t* d_fixstore = nullptr;
   double* d_poly = nullptr;
+  void* d_opts = nullptr;
   void* d_rows = nullptr;
   double* d_partials = nullptr;
   double* d_stats = nullptr;
@@ -202,6 +203,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   plan->d_fixed = uploadReals(ctx, plan.get(), h.fixed, fp32);
   plan->d_fixstore = upload(ctx, plan.get(), h.fixstore.data(), h.fixstore.size());
   plan->d_poly = upload(ctx, plan.get(), h.poly.data(), h.poly.size());
+  plan->d_opts = h.opts.empty() ? nullptr : uploadReals(ctx, plan.get(), h.opts, fp32);
   const size_t elem = fp32 ? sizeof(float) : sizeof(double);
   const int64_t P = h.g.num_paths;
   plan->d_rows = devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * P * elem);
@@ -279,6 +281,7 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   a.fixed = static_cast<const Real*>(plan->d_fixed);
   a.fixstore = plan->d_fixstore;
   a.poly = plan->d_poly;
+  a.opts = static_cast<const Real*>(plan->d_opts);
   a.bermv = plan->d_bermv;
   a.states = nullptr;
   a.rows = static_cast<Real*>(plan->d_rows);

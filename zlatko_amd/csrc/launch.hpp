@@ -36,6 +36,7 @@ struct PathArgs {
   const Real* fixed;
   const int32_t* fixstore;
   const double* poly;  // bond-polynomial coefficients (FP64 in both precisions)
+  const Real* opts;    // European option terms
   const double* bermv;      // American Monte Carlo: [continuation date][path] sum of the set's swaptions
   double* states;           // states-only pass: [set dates][path] x then D (null otherwise)
   Real* rows;

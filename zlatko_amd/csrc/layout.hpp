@@ -50,8 +50,14 @@ struct DateRec {
   // zero-padded at the top) from poly_beg[r]; 0 = none.
   int32_t poly_beg[kMaxRates];
   int32_t poly_len[kMaxRates];
+  int32_t opt_beg, opt_end;          // European options alive at this date: [opt_beg, opt_end)
+  int32_t pad[2];
 };
-static_assert(sizeof(DateRec) == 96, "DateRec must stay 96 bytes");
+static_assert(sizeof(DateRec) == 112, "DateRec must stay 112 bytes");
+
+// European option term (Real[kOptStride]) on one date: asset, +1 call / -1 put, notional,
+// strike, A(t,T), B(t,T) of the domestic zero bond to expiry, carry Q(t,T), vol sqrt(T - t).
+constexpr int kOptStride = 8;
 constexpr int32_t kValued = 1, kReporting = 2, kStoreValue = 4;
 
 // Per-date reals: c0[rate] (constant bond terms), lin[asset] (deterministic-carry forwards),
@@ -137,6 +143,7 @@ struct SetDev {
   int32_t cpty, own, num_slots;
   int32_t berm_beg, berm_end;  // Bermudan swaptions of this set
   int32_t cd_base;             // first continuation date (one per valued date) of this set's rows
+  int32_t opt_off;             // into the option-term table
   int64_t state_off;           // first element of this set's [date][path] states (AMC), -1 without
   double thr_cpty, thr_own, mta, ia, keep;  // CSA terms (thr_own < 0 encodes a one-way CSA)
 };

@@ -80,6 +80,28 @@ __host__ __device__ inline Real sqrtr(Real v) {
   else return __fsqrt_rn(v);
 }
 
+// normalCdf and blackFormula (include/ccr/core/math.hpp:15-31): N(x) = erfc(-x / sqrt 2) / 2;
+// discount x sign x (F N(sign d1) - K N(sign d2)), the discounted intrinsic value when the
+// total standard deviation, the forward or the strike degenerates. sign: +1 call, -1 put.
+template <typename Real>
+__host__ __device__ inline Real normalCdf(Real x) {
+  if constexpr (std::is_same<Real, double>::value) return 0.5 * erfc(-x / 1.4142135623730951);
+  else return 0.5f * erfcf(-x / 1.41421356f);
+}
+template <typename Real>
+__host__ __device__ inline Real blackFormula(Real sign, Real forward, Real strike, Real stdDev, Real discount) {
+  if (stdDev <= Real(1e-14) || forward <= Real(0) || strike <= Real(0)) {
+    const Real intrinsic = sign * (forward - strike);
+    return discount * (intrinsic > Real(0) ? intrinsic : Real(0));
+  }
+  Real lr;
+  if constexpr (std::is_same<Real, double>::value) lr = log(forward / strike);
+  else lr = logf(forward / strike);
+  const Real d1 = lr / stdDev + Real(0.5) * stdDev;
+  const Real d2 = d1 - stdDev;
+  return discount * sign * (forward * normalCdf(sign * d1) - strike * normalCdf(sign * d2));
+}
+
 // Value of a TermRange (an instrument's terms on one date, indices relative to its set's tables)
 // at domestic state x: the bond polynomial when every lane is within its bound, else the
 // exponential terms; plus the fixed coupons, whose fixing states fixing(k) come from the states
@@ -184,7 +206,8 @@ template <typename Real, typename Sh, bool kStatesOnly, typename Emit, typename 
 __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
-                                             cptr<int32_t> fstore, cptr<double> poly, const double* bermv, int ring,
+                                             cptr<int32_t> fstore, cptr<double> poly, cptr<Real> opts,
+                                             const double* bermv, int ring,
                                              int priceRows, Real* col, int stride, Emit&& emit, OnState&& onState) {
   constexpr bool kDouble = std::is_same<Real, double>::value;
   constexpr int NR = Sh::NR, NA = Sh::NA, NC = Sh::NC;
@@ -391,6 +414,18 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 #pragma unroll
       for (int i = 0; i < NA; ++i)
         if (i < nA) V += S_[i] * dr[kDateLin + i];
+      if constexpr (NA > 0) {
+        for (int i = rec.opt_beg; i < rec.opt_end; ++i) {
+          cptr<Real> o = opts + kOptStride * static_cast<int64_t>(i);
+          const int oa = static_cast<int>(o[0]);
+          Real Sa = S_[0];
+#pragma unroll
+          for (int k = 1; k < NA; ++k)
+            if (k == oa) Sa = S_[k];
+          const Real df = o[4] * expr(-o[5] * x[0]);
+          V += o[2] * blackFormula(o[1], Sa * o[6] / df, o[3], o[7], df);
+        }
+      }
       // Bermudan swaptions: their per-path sum on this valued date, from the valuation kernel
       // (one row per valued date of the set, in date order).
       if (bermRow) {
@@ -481,7 +516,8 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDe
                                       toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
                                       toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
                                       toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
-                                      toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off, bermv,
+                                      toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off,
+                                      toConst(a.opts) + kOptStride * static_cast<int64_t>(S.opt_off), bermv,
                                       a.ring, a.price_rows, col, stride, emit, onState);
 }
 

@@ -35,6 +35,7 @@ struct Trade {
   bool cash = false;
   double position = 1.0;
   int degree = 3;
+  int optionType = CCR_OPTION_CALL;  // European option (asset, notional, K = strike, maturity = expiry)
 };
 
 void checkSchedule(const std::vector<double>& s) {
@@ -46,7 +47,7 @@ void checkSchedule(const std::vector<double>& s) {
 
 std::vector<double> eventTimes(const ccr_trade& d) {
   std::vector<double> e;
-  if (d.kind == CCR_TRADE_ASSET_FORWARD) return {d.maturity};
+  if (d.kind == CCR_TRADE_ASSET_FORWARD || d.kind == CCR_TRADE_EUROPEAN_OPTION) return {d.maturity};
   if (d.schedule && d.schedule_size > 0) e.assign(d.schedule, d.schedule + d.schedule_size);
   if (d.float_schedule && d.float_schedule_size > 0)
     e.insert(e.end(), d.float_schedule, d.float_schedule + d.float_schedule_size);
@@ -181,6 +182,19 @@ Trade makeTrade(const ccr_trade& d, const Models& m) {
     t.cash = d.cash_settled != 0;
     t.position = d.position;
     t.degree = d.regression_degree;
+  } else if (d.kind == CCR_TRADE_EUROPEAN_OPTION) {
+    // EuropeanOption ctor: src/instruments/european_option.cpp:9-18.
+    if (d.maturity <= 0.0 || d.fixed_rate <= 0.0) invalid("EuropeanOption: invalid expiry or strike");
+    if (d.asset < 0 || d.asset >= static_cast<int>(m.spot.size()))
+      fail(CCR_ERR_OUT_OF_RANGE, "ScenarioSet: unknown asset");
+    if (m.assetRate[static_cast<std::size_t>(d.asset)] != 0)
+      fail(CCR_ERR_UNSUPPORTED, "EuropeanOption: the asset needs a deterministic carry curve");
+    if (d.option_type != CCR_OPTION_CALL && d.option_type != CCR_OPTION_PUT) invalid("EuropeanOption: unknown type");
+    t.asset = d.asset;
+    t.notional = d.notional;
+    t.K = d.fixed_rate;
+    t.maturity = d.maturity;
+    t.optionType = d.option_type;
   } else {
     invalid("unknown trade kind");
   }
@@ -193,6 +207,7 @@ struct Collector {
   std::map<std::tuple<int, int, double, double>, double> fix;  // (rate, k, ts, te) -> coefficient
   double c0[kMaxRates] = {0, 0, 0, 0};
   double lin[kMaxAssets] = {0, 0, 0, 0};
+  std::vector<const Trade*> options;  // European options alive at t
   double t = 0.0;
   void addBond(int r, double T, double c) {
     if (T <= t) {
@@ -245,8 +260,12 @@ void collect(const Trade& tr, const Models& m, const TimeGrid& grid, Collector& 
     collectSwapPeriods(tr, grid, col);
     return;
   }
-  // AssetForward: N S Q(t,T) - N K P(t,T).
   if (t >= tr.maturity - kPaidTolerance) return;
+  if (tr.kind == CCR_TRADE_EUROPEAN_OPTION) {
+    col.options.push_back(&tr);
+    return;
+  }
+  // AssetForward: N S Q(t,T) - N K P(t,T).
   col.addBond(0, tr.maturity, -tr.notional * tr.K);
   const int r = m.assetRate[static_cast<std::size_t>(tr.asset)];
   if (r == 0) {
@@ -400,6 +419,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     sd.fixed_off = static_cast<int32_t>(plan.fixed.size() / 4);
     sd.fixstore_off = static_cast<int32_t>(plan.fixstore.size() / 2);
     sd.poly_off = static_cast<int32_t>(plan.poly.size());
+    sd.opt_off = static_cast<int32_t>(plan.opts.size() / kOptStride);
     sd.stream_stride = static_cast<int64_t>(T - 1) * m.nz;
     maxStride = std::max(maxStride, sd.stream_stride);
     sd.block_begin = s * blocksPerSet;
@@ -581,6 +601,19 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
         }
       }
       rec.fix_beg[kMaxRates] = static_cast<int32_t>(plan.fixed.size() / 4) - sd.fixed_off;
+      // European options (european_option.cpp:20-35): Black on the forward S Q(t,T) / P(t,T).
+      rec.opt_beg = static_cast<int32_t>(plan.opts.size() / kOptStride) - sd.opt_off;
+      for (const Trade* op : col.options) {
+        const HullWhite& hw = m.rates[0];
+        const double T_ = op->maturity;
+        const Curve& carry = m.carry[static_cast<std::size_t>(op->asset)];
+        const double o[kOptStride] = {static_cast<double>(op->asset), op->optionType == CCR_OPTION_CALL ? 1.0 : -1.0,
+                                      op->notional, op->K, hw.A(t, T_), hw.B(t, T_),
+                                      carry.discount(T_) / carry.discount(t),
+                                      m.vol[static_cast<std::size_t>(op->asset)] * std::sqrt(T_ - t)};
+        plan.opts.insert(plan.opts.end(), o, o + kOptStride);
+      }
+      rec.opt_end = static_cast<int32_t>(plan.opts.size() / kOptStride) - sd.opt_off;
       rec.fixstore_beg = static_cast<int32_t>(plan.fixstore.size() / 2) - sd.fixstore_off;
       for (const auto& rs : storeAt[static_cast<std::size_t>(j)]) {
         plan.fixstore.push_back(rs.first);

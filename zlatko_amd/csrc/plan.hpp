@@ -20,6 +20,7 @@ struct HostPlan {
   std::vector<double> bonds;      // [n][2]
   std::vector<double> fixed;      // [n][4]
   std::vector<double> poly;       // bond-polynomial coefficients, ascending powers per (date, rate)
+  std::vector<double> opts;       // European option terms [n][kOptStride]
   std::vector<int32_t> fixstore;  // [n][2] (rate, slot)
   std::vector<int32_t> block_set; // set of each workgroup (before the XCD remap)
   std::vector<std::vector<double>> rep_times;  // reporting dates per set

@@ -21,7 +21,8 @@ from typing import List
 
 import numpy as np
 
-from .abi import (CCR_TRADE_ASSET_FORWARD, CCR_TRADE_BERMUDAN_SWAPTION, CCR_TRADE_SWAP, Asset, Cir, Collateral,
+from .abi import (CCR_OPTION_CALL, CCR_OPTION_PUT, CCR_TRADE_ASSET_FORWARD, CCR_TRADE_BERMUDAN_SWAPTION,
+                  CCR_TRADE_EUROPEAN_OPTION, CCR_TRADE_SWAP, Asset, Cir, Collateral,
                   CreditCurve, Curve, HullWhite, Model, NettingSet, Spec, Trade, swap_schedule, uniform_grid)
 
 MASK64 = (1 << 64) - 1
@@ -164,7 +165,7 @@ def c4(num_paths: int = 100000, dates: int = 120, seed: int = 42, with_bermudans
 
 def small_reference_case(num_paths: int = 64, dates: int = 12, csa: bool = False, fx: bool = False,
                          credits: bool = False, two_leg: bool = False, seed: int = 7, bermudans: int = 0,
-                         cash_settled: bool = False) -> Spec:
+                         cash_settled: bool = False, options: int = 0) -> Spec:
     """Small reference-representable netting set for parity against a build of the reference."""
     rng = SplitMix64(11)
     curve = Curve([0.5, 2.0, 5.0, 10.0], [0.020, 0.025, 0.030, 0.033])
@@ -190,6 +191,11 @@ def small_reference_case(num_paths: int = 64, dates: int = 12, csa: bool = False
     if fx:
         ts.append(Trade(CCR_TRADE_ASSET_FORWARD, 5e5, 1.05, asset=0, maturity=3.0))
         ts.append(Trade(CCR_TRADE_ASSET_FORWARD, -3e5, 1.12, asset=0, maturity=1.5))
+    for i in range(options if fx else 0):
+        # FX calls and puts, long and short, 0.5-3y expiries.
+        ts.append(Trade(CCR_TRADE_EUROPEAN_OPTION, rng.uniform(1e5, 1e6) * (-1.0 if i % 3 == 2 else 1.0),
+                        rng.uniform(0.95, 1.25), asset=0, maturity=rng.uniform(0.5, 3.0),
+                        option_type=CCR_OPTION_PUT if i % 2 else CCR_OPTION_CALL))
     for i in range(bermudans):
         # 1-2y into 2-3y, semiannual exercise on the underlying's period starts.
         expiry = 1.0 + 0.5 * rng.integer(0, 2)

@@ -213,8 +213,9 @@ typedef struct ccr_exposure_out {
 
 enum ccr_precision {
   CCR_FP64 = 0,  /* FP64 everywhere: the parity mode (1e-10 relative to the FP64 oracle) */
-  CCR_FP32 = 1   /* FP32 path evolution and revaluation, compensated sums for discount
+  CCR_FP32 = 1,  /* FP32 path evolution and revaluation, compensated sums for discount
                     integrals and exposure means; FP64 final statistics */
+  CCR_BF16 = 2   /* MLP surrogate only: BF16 matrix-core products, FP32 accumulation */
 };
 
 typedef struct ccr_hip_options {
@@ -256,6 +257,34 @@ typedef struct ccr_hip_plan_info {
   int32_t num_continuation_dates; /* (set, date) pairs with continuation regressions */
   int32_t max_exercises;
 } ccr_hip_plan_info;
+
+/* ------------------------------------------------------------------ MLP surrogate */
+
+/* Multilayer perceptron: `inputs` -> hidden_layers x `hidden` (activation) -> `outputs` (linear),
+ * as ccr::ai::Mlp (include/ccr/ai/surrogate.hpp:44-50; the reference has two tanh hidden
+ * layers). Parameters are flattened layer by layer: W_l [fan_out][fan_in] row-major, then
+ * b_l [fan_out]. Standardisers (ccr::ai::Standardiser, surrogate.hpp:29-34) are stored as
+ * x_mean[inputs], x_sd[inputs], y_mean[outputs], y_sd[outputs]. */
+enum ccr_activation { CCR_ACT_TANH = 0, CCR_ACT_SILU = 1 };
+typedef struct ccr_mlp_desc {
+  int32_t inputs;
+  int32_t hidden;
+  int32_t hidden_layers;
+  int32_t outputs;
+  int32_t activation;
+} ccr_mlp_desc;
+
+/* ccr::ai::MlpOptions (surrogate.hpp:51-56): full-batch Adam (beta 0.9 / 0.999, eps 1e-8,
+ * bias-corrected, weight decay on weights only). precision: CCR_FP32 (vector units, any shape)
+ * or CCR_BF16 (BF16 matrix cores with FP32 accumulation, hidden = 128). */
+typedef struct ccr_mlp_train_options {
+  int32_t epochs;
+  double learning_rate;
+  double weight_decay;
+  uint32_t seed;
+  int32_t precision;
+  int32_t warm;            /* nonzero: start from the parameters and standardisers passed in */
+} ccr_mlp_train_options;
 
 int32_t ccr_hip_abi_version(void);
 

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../include/ccr_hip.h"
+#include "ccr/ai/surrogate.hpp"
 #include "ccr/core/math.hpp"
 #include "ccr/core/random.hpp"
 #include "ccr/cva/cva.hpp"
@@ -211,5 +212,37 @@ void ccr_ref_normals(uint64_t seed, int64_t n, double* out) {
 }
 
 double ccr_ref_quantile(const double* v, int64_t n, double p) { return ccr::quantile(std::vector<double>(v, v + n), p); }
+
+// ccr::ai::fitMlp on rows X [n][din], Y [n][dout]; writes W0, B0, W1, B1, W2, B2 (row-major) and
+// the standardisers x_mean, x_sd, y_mean, y_sd.
+int32_t ccr_ref_fit_mlp(const double* X, const double* Y, int64_t n, int32_t din, int32_t dout, int32_t hidden,
+                        int32_t epochs, double lr, double wd, uint32_t seed, double* params, double* stdz) {
+  try {
+    ccr::ai::Rows rx, ry;
+    for (int64_t r = 0; r < n; ++r) {
+      rx.emplace_back(X + r * din, X + (r + 1) * din);
+      ry.emplace_back(Y + r * dout, Y + (r + 1) * dout);
+    }
+    ccr::ai::MlpOptions o;
+    o.hidden = static_cast<std::size_t>(hidden);
+    o.epochs = static_cast<std::size_t>(epochs);
+    o.learningRate = lr;
+    o.weightDecay = wd;
+    o.seed = seed;
+    const ccr::ai::Mlp m = ccr::ai::fitMlp(rx, ry, o);
+    for (std::size_t l = 0; l < 3; ++l) {
+      for (const auto& row : m.W[l])
+        for (double w : row) *params++ = w;
+      for (double b : m.B[l]) *params++ = b;
+    }
+    for (double v : m.sx.mean) *stdz++ = v;
+    for (double v : m.sx.sd) *stdz++ = v;
+    for (double v : m.sy.mean) *stdz++ = v;
+    for (double v : m.sy.sd) *stdz++ = v;
+    return CCR_OK;
+  } catch (const std::exception&) {
+    return CCR_ERR_RUNTIME;
+  }
+}
 
 }  // extern "C"

@@ -72,6 +72,15 @@ def main():
     normals = np.zeros(64)
     ref.ccr_ref_normals(42, 64, normals.ctypes.data_as(C.c_void_p))
     out["normals_seed42"] = normals.tolist()
+    # ccr::ai::fitMlp on the synthetic rows of oracle_lib.mlp_dataset (reference 5-16-16-6 tanh).
+    mlp = dict(n=160, din=5, dout=6, hidden=16, epochs=300, lr=0.01, wd=1e-6, seed=1)
+    X, Y = oracle_lib.mlp_dataset(mlp["n"], mlp["din"], mlp["dout"])
+    params = np.zeros(mlp["din"] * 16 + 16 + 16 * 16 + 16 + 16 * mlp["dout"] + mlp["dout"])
+    stdz = np.zeros(2 * (mlp["din"] + mlp["dout"]))
+    rc = ref.ccr_ref_fit_mlp(abi.dptr(X), abi.dptr(Y), mlp["n"], mlp["din"], mlp["dout"], mlp["hidden"],
+                             mlp["epochs"], mlp["lr"], mlp["wd"], mlp["seed"], abi.dptr(params), abi.dptr(stdz))
+    assert rc == 0
+    out["mlp"] = dict(mlp, params=params.tolist(), stdz=stdz.tolist())
     path = Path(__file__).resolve().parent / "reference_vectors.json"
     path.write_text(json.dumps(out, indent=1))
     print("wrote", path)

@@ -63,6 +63,14 @@ def oracle():
         lib.ccr_oracle_profile.argtypes = [P, P, I32, I32, D, P, P, P, P, P, P]
         lib.ccr_oracle_bermudan.argtypes = [P, P, P, I32, P, P, P]
         lib.ccr_oracle_last_timing.argtypes = [P]
+        lib.ccr_oracle_mlp_param_count.restype = I64
+        lib.ccr_oracle_mlp_param_count.argtypes = [P]
+        lib.ccr_oracle_standardiser_fit.argtypes = [P, I64, I32, P, P]
+        lib.ccr_oracle_mlp_init.argtypes = [P, C.c_uint32, P]
+        lib.ccr_oracle_mlp_gradient.restype = D
+        lib.ccr_oracle_mlp_gradient.argtypes = [P, P, P, P, I64, P]
+        lib.ccr_oracle_fit_mlp.argtypes = [P, P, P, I64, P, P, P]
+        lib.ccr_oracle_mlp_predict.argtypes = [P, P, P, P, I64, P]
         _oracle = lib
     return _oracle
 
@@ -80,8 +88,40 @@ def ref():
         lib.ccr_ref_normals.argtypes = [C.c_uint64, I64, P]
         lib.ccr_ref_quantile.restype = D
         lib.ccr_ref_quantile.argtypes = [P, I64, D]
+        lib.ccr_ref_fit_mlp.restype = I32
+        lib.ccr_ref_fit_mlp.argtypes = [P, P, I64, I32, I32, I32, I32, D, D, C.c_uint32, P, P]
         _ref = lib
     return _ref
+
+
+def mlp_dataset(n: int, din: int = 5, dout: int = 6, seed: int = 9):
+    """Smooth synthetic regression rows (SplitMix64 inputs in a box, nonlinear targets)."""
+    from zlatko_amd.workloads import SplitMix64
+    rng = SplitMix64(seed)
+    X = np.array([[rng.uniform(-1.0, 1.0) * (1 + j) for j in range(din)] for _ in range(n)])
+    W = np.array([[rng.uniform(-1.0, 1.0) for _ in range(din)] for _ in range(dout)])
+    Y = np.sin(X @ W.T) + 0.3 * (X @ W.T) ** 2 + 2.0
+    return np.ascontiguousarray(X), np.ascontiguousarray(Y)
+
+
+def oracle_fit_mlp(desc: abi.MlpDescC, X: np.ndarray, Y: np.ndarray, epochs: int, lr: float = 0.01,
+                   wd: float = 1e-6, seed: int = 1, params=None, stdz=None):
+    """fitMlp restated in FP64; params / stdz given: warm start."""
+    n = X.shape[0]
+    warm = params is not None
+    params = np.zeros(abi.mlp_param_count(desc)) if params is None else params.copy()
+    stdz = np.zeros(2 * (desc.inputs + desc.outputs)) if stdz is None else stdz.copy()
+    o = abi.MlpTrainOptionsC(epochs, lr, wd, seed, abi.CCR_FP64, 1 if warm else 0)
+    oracle().ccr_oracle_fit_mlp(C.byref(desc), abi.dptr(X), abi.dptr(Y), n, C.byref(o), abi.dptr(params),
+                                abi.dptr(stdz))
+    return params, stdz
+
+
+def oracle_mlp_gradient(desc: abi.MlpDescC, params: np.ndarray, xs: np.ndarray, ys: np.ndarray):
+    g = np.zeros_like(params)
+    loss = oracle().ccr_oracle_mlp_gradient(C.byref(desc), abi.dptr(params), abi.dptr(xs), abi.dptr(ys),
+                                            xs.shape[0], abi.dptr(g))
+    return loss, g
 
 
 class OracleTiming(C.Structure):

@@ -121,6 +121,26 @@ class HipPlanInfoC(C.Structure):
                 ("max_exercises", C.c_int32)]
 
 
+CCR_BF16 = 2
+CCR_ACT_TANH = 0
+CCR_ACT_SILU = 1
+
+
+class MlpDescC(C.Structure):
+    _fields_ = [("inputs", C.c_int32), ("hidden", C.c_int32), ("hidden_layers", C.c_int32), ("outputs", C.c_int32),
+                ("activation", C.c_int32)]
+
+
+class MlpTrainOptionsC(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("learning_rate", C.c_double), ("weight_decay", C.c_double),
+                ("seed", C.c_uint32), ("precision", C.c_int32), ("warm", C.c_int32)]
+
+
+def mlp_param_count(d: MlpDescC) -> int:
+    sizes = [d.inputs] + [d.hidden] * d.hidden_layers + [d.outputs]
+    return sum(sizes[i + 1] * sizes[i] + sizes[i + 1] for i in range(len(sizes) - 1))
+
+
 PROFILE_FIELDS = ("times", "expected_value", "expected_exposure", "expected_negative_exposure",
                   "potential_future_exposure", "discounted_expected_exposure",
                   "discounted_expected_negative_exposure", "mean_deflator", "mean_survival", "cva_increments",

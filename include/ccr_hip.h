@@ -286,6 +286,21 @@ typedef struct ccr_mlp_train_options {
   int32_t warm;            /* nonzero: start from the parameters and standardisers passed in */
 } ccr_mlp_train_options;
 
+/* fitMlp (src/ai/surrogate.cpp:143-245) on the GPU: rows X [n][inputs], Y [n][outputs]. Unless
+ * options->warm, fits the standardisers (into stdz) and initialises params from options->seed;
+ * then options->epochs full-batch Adam steps. Writes the trained params, the loss (MSE on
+ * standardised rows) after training, and the training time in milliseconds. */
+int32_t ccr_hip_mlp_train(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, const ccr_mlp_train_options* options,
+                          const double* X, const double* Y, int64_t n, double* params, double* stdz, double* loss,
+                          double* train_ms);
+/* Loss and its gradient with respect to params on standardised rows xs, ys (one forward and
+ * one backward pass in the given precision: CCR_FP32 or CCR_BF16). */
+int32_t ccr_hip_mlp_gradient(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, int32_t precision, const double* params,
+                             const double* xs, const double* ys, int64_t n, double* grad, double* loss);
+/* Mlp::predict (surrogate.cpp:157) for rows X [n][inputs] into Y [n][outputs]. */
+int32_t ccr_hip_mlp_predict(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, int32_t precision, const double* params,
+                            const double* stdz, const double* X, int64_t n, double* Y);
+
 int32_t ccr_hip_abi_version(void);
 
 /* Creates a context on options->device (NULL: device 0, FP64, autotuned block). */

@@ -13,6 +13,7 @@
 
 #include "../../include/ccr_hip.h"
 #include "launch.hpp"
+#include "mlp_host.hpp"
 #include "plan.hpp"
 
 
@@ -609,6 +610,77 @@ void ccr_bilateral_cva(const double* t, const double* dee, const double* dene, i
   }
   if (cva) *cva = a;
   if (dva) *dva = b;
+}
+
+int32_t ccr_hip_mlp_train(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, const ccr_mlp_train_options* o, const double* X,
+                          const double* Y, int64_t n, double* params, double* stdz, double* loss, double* train_ms) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!desc || !o || !X || !Y || !params || !stdz) invalid("null argument");
+    if (o->epochs < 0) invalid("Mlp: negative epochs");
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    const int din = desc->inputs, dout = desc->outputs;
+    MlpRun run(*desc, o->precision, n, ctx->stream);
+    double* xm = stdz;
+    double* xsd = stdz + din;
+    double* ym = stdz + 2 * din;
+    double* ysd = stdz + 2 * din + dout;
+    if (!o->warm) {
+      standardiserFit(X, n, din, xm, xsd);
+      standardiserFit(Y, n, dout, ym, ysd);
+      mlpInit(run.device().sh, o->seed, params);
+    }
+    std::vector<double> xs(static_cast<size_t>(n * din)), ys(static_cast<size_t>(n * dout));
+    for (int64_t r = 0; r < n; ++r) {
+      for (int j = 0; j < din; ++j) xs[static_cast<size_t>(r * din + j)] = (X[r * din + j] - xm[j]) / xsd[j];
+      for (int k = 0; k < dout; ++k) ys[static_cast<size_t>(r * dout + k)] = (Y[r * dout + k] - ym[k]) / ysd[k];
+    }
+    run.setParams(params);
+    run.setInputs(xs.data());
+    run.setTargets(ys.data());
+    const auto t0 = std::chrono::steady_clock::now();
+    run.train(o->epochs, o->learning_rate, o->weight_decay);
+    if (train_ms) *train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (loss) *loss = run.loss();
+    run.getParams(params);
+  });
+}
+
+int32_t ccr_hip_mlp_gradient(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, int32_t precision, const double* params,
+                             const double* xs, const double* ys, int64_t n, double* grad, double* loss) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!desc || !params || !xs || !ys || !grad) invalid("null argument");
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    MlpRun run(*desc, precision, n, ctx->stream);
+    run.setParams(params);
+    run.setInputs(xs);
+    run.setTargets(ys);
+    const double l = run.lossAndGradient(grad);
+    if (loss) *loss = l;
+  });
+}
+
+int32_t ccr_hip_mlp_predict(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, int32_t precision, const double* params,
+                            const double* stdz, const double* X, int64_t n, double* Y) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!desc || !params || !stdz || !X || !Y) invalid("null argument");
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    const int din = desc->inputs, dout = desc->outputs;
+    MlpRun run(*desc, precision, n, ctx->stream);
+    std::vector<double> xs(static_cast<size_t>(n * din));
+    for (int64_t r = 0; r < n; ++r)
+      for (int j = 0; j < din; ++j) xs[static_cast<size_t>(r * din + j)] = (X[r * din + j] - stdz[j]) / stdz[din + j];
+    run.setParams(params);
+    run.setInputs(xs.data());
+    run.forward();
+    std::vector<double> z(static_cast<size_t>(n * dout));
+    run.outputs(z.data());
+    for (int64_t r = 0; r < n; ++r)
+      for (int k = 0; k < dout; ++k)
+        Y[r * dout + k] = z[static_cast<size_t>(r * dout + k)] * stdz[2 * din + dout + k] + stdz[2 * din + k];
+  });
 }
 
 }  // extern "C"

@@ -60,6 +60,12 @@ def lib():
         L.ccr_unilateral_cva.restype = D
         L.ccr_unilateral_cva.argtypes = [P, P, I32, P, D]
         L.ccr_bilateral_cva.argtypes = [P, P, P, I32, P, D, P, D, P, P]
+        L.ccr_hip_mlp_train.restype = I32
+        L.ccr_hip_mlp_train.argtypes = [P, P, P, P, P, I64, P, P, P, P]
+        L.ccr_hip_mlp_gradient.restype = I32
+        L.ccr_hip_mlp_gradient.argtypes = [P, P, I32, P, P, P, I64, P, P]
+        L.ccr_hip_mlp_predict.restype = I32
+        L.ccr_hip_mlp_predict.argtypes = [P, P, I32, P, P, P, I64, P]
         _lib = L
     return _lib
 
@@ -152,6 +158,41 @@ class Engine:
 
     def set_fault(self, kind: int):
         self._check(lib().ccr_hip_set_fault(self.ctx, kind))
+
+    # ------------------------------------------------------------ MLP surrogate
+
+    def mlp_train(self, desc: abi.MlpDescC, X: np.ndarray, Y: np.ndarray, epochs: int, lr: float = 0.01,
+                  wd: float = 1e-6, seed: int = 1, precision: int = abi.CCR_FP32, params=None, stdz=None):
+        """fitMlp on the GPU; returns (params, stdz, loss, train_ms). params/stdz given: warm start."""
+        X = np.ascontiguousarray(X, np.float64)
+        Y = np.ascontiguousarray(Y, np.float64)
+        warm = params is not None
+        params = np.zeros(abi.mlp_param_count(desc)) if params is None else np.array(params, np.float64)
+        stdz = np.zeros(2 * (desc.inputs + desc.outputs)) if stdz is None else np.array(stdz, np.float64)
+        o = abi.MlpTrainOptionsC(epochs, lr, wd, seed, precision, 1 if warm else 0)
+        loss, ms = C.c_double(), C.c_double()
+        self._check(lib().ccr_hip_mlp_train(self.ctx, C.byref(desc), C.byref(o), abi.dptr(X), abi.dptr(Y), X.shape[0],
+                                            abi.dptr(params), abi.dptr(stdz), C.byref(loss), C.byref(ms)))
+        return params, stdz, loss.value, ms.value
+
+    def mlp_gradient(self, desc: abi.MlpDescC, precision: int, params: np.ndarray, xs: np.ndarray, ys: np.ndarray):
+        xs = np.ascontiguousarray(xs, np.float64)
+        ys = np.ascontiguousarray(ys, np.float64)
+        params = np.ascontiguousarray(params, np.float64)
+        g = np.zeros_like(params)
+        loss = C.c_double()
+        self._check(lib().ccr_hip_mlp_gradient(self.ctx, C.byref(desc), precision, abi.dptr(params), abi.dptr(xs),
+                                               abi.dptr(ys), xs.shape[0], abi.dptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def mlp_predict(self, desc: abi.MlpDescC, precision: int, params: np.ndarray, stdz: np.ndarray, X: np.ndarray):
+        X = np.ascontiguousarray(X, np.float64)
+        Y = np.zeros((X.shape[0], desc.outputs))
+        self._check(lib().ccr_hip_mlp_predict(self.ctx, C.byref(desc), precision,
+                                              abi.dptr(np.ascontiguousarray(params, np.float64)),
+                                              abi.dptr(np.ascontiguousarray(stdz, np.float64)), abi.dptr(X),
+                                              X.shape[0], abi.dptr(Y)))
+        return Y
 
     def close(self):
         if self.ctx:

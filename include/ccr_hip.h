@@ -284,6 +284,8 @@ typedef struct ccr_mlp_train_options {
   uint32_t seed;
   int32_t precision;
   int32_t warm;            /* nonzero: start from the parameters and standardisers passed in */
+  int32_t batch;           /* 0: full batch (the reference); else Adam steps over consecutive
+                              mini-batches of this many rows (BF16 only, a multiple of 64) */
 } ccr_mlp_train_options;
 
 /* fitMlp (src/ai/surrogate.cpp:143-245) on the GPU: rows X [n][inputs], Y [n][outputs]. Unless
@@ -300,6 +302,50 @@ int32_t ccr_hip_mlp_gradient(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, int32_t
 /* Mlp::predict (surrogate.cpp:157) for rows X [n][inputs] into Y [n][outputs]. */
 int32_t ccr_hip_mlp_predict(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, int32_t precision, const double* params,
                             const double* stdz, const double* X, int64_t n, double* Y);
+
+/* ------------------------------------------------------------------ exposure surrogate
+ * For trades without a closed form (Bermudan swaptions): per netting set, an MLP (BF16 matrix
+ * cores, SiLU) maps the per-date state (t, x, running max and min of x along the path) to the
+ * set's Bermudan value. It is trained on the pathwise values the American Monte Carlo method
+ * produced on a training plan, then judged on an independent hold-out plan: a set keeps the
+ * surrogate only if its hold-out EE stays within gate_threshold (relative to the peak plain EE);
+ * otherwise that set falls back to the regression method. No counterpart in the reference. */
+typedef struct ccr_hip_surrogate ccr_hip_surrogate;
+
+typedef struct ccr_surrogate_options {
+  int32_t train_paths;     /* paths of the training plan used as samples (0: all) */
+  int32_t hidden;          /* 128 */
+  int32_t hidden_layers;   /* e.g. 3 */
+  int32_t epochs;
+  double learning_rate;
+  double weight_decay;
+  uint32_t seed;
+  double gate_threshold;   /* e.g. 0.005 */
+  int32_t batch;           /* Adam mini-batch rows (a multiple of 64; 0: full batch); rows are shuffled */
+} ccr_surrogate_options;
+
+typedef struct ccr_surrogate_report {
+  int32_t candidate_sets;  /* netting sets holding Bermudan swaptions */
+  int32_t enabled_sets;    /* of which the gate kept the surrogate */
+  int64_t samples;         /* training rows over all candidate sets */
+  double train_ms;         /* GPU training time (all sets) */
+  double loss;             /* largest final training loss (standardised MSE) */
+  double max_ee_error;     /* worst hold-out EE error (relative to the peak plain EE) */
+} ccr_surrogate_report;
+
+/* Trains on train_plan (executed plain) and gates on holdout_plan (executed plain and with the
+ * surrogate). Both plans must hold the same netting sets (other paths / seeds). */
+int32_t ccr_hip_surrogate_train(ccr_hip_ctx* ctx, ccr_hip_plan* train_plan, ccr_hip_plan* holdout_plan,
+                                const ccr_surrogate_options* options, ccr_hip_surrogate** out,
+                                ccr_surrogate_report* report);
+/* Gate outcome for netting set `set`: enabled (1/0; -1: not a candidate) and hold-out EE error. */
+int32_t ccr_hip_surrogate_set_info(const ccr_hip_surrogate* surrogate, int32_t set, int32_t* enabled,
+                                   double* ee_error);
+/* ccr_hip_execute / _async with the surrogate valuing the enabled sets' Bermudans. */
+int32_t ccr_hip_execute_surrogate(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* surrogate,
+                                  ccr_exposure_out* outs);
+int32_t ccr_hip_execute_surrogate_async(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* surrogate);
+void ccr_hip_surrogate_destroy(ccr_hip_surrogate* surrogate);
 
 int32_t ccr_hip_abi_version(void);
 

@@ -133,7 +133,18 @@ class MlpDescC(C.Structure):
 
 class MlpTrainOptionsC(C.Structure):
     _fields_ = [("epochs", C.c_int32), ("learning_rate", C.c_double), ("weight_decay", C.c_double),
-                ("seed", C.c_uint32), ("precision", C.c_int32), ("warm", C.c_int32)]
+                ("seed", C.c_uint32), ("precision", C.c_int32), ("warm", C.c_int32), ("batch", C.c_int32)]
+
+
+class SurrogateOptionsC(C.Structure):
+    _fields_ = [("train_paths", C.c_int32), ("hidden", C.c_int32), ("hidden_layers", C.c_int32), ("epochs", C.c_int32),
+                ("learning_rate", C.c_double), ("weight_decay", C.c_double), ("seed", C.c_uint32),
+                ("gate_threshold", C.c_double), ("batch", C.c_int32)]
+
+
+class SurrogateReportC(C.Structure):
+    _fields_ = [("candidate_sets", C.c_int32), ("enabled_sets", C.c_int32), ("samples", C.c_int64),
+                ("train_ms", C.c_double), ("loss", C.c_double), ("max_ee_error", C.c_double)]
 
 
 def mlp_param_count(d: MlpDescC) -> int:

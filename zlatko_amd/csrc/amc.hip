@@ -365,9 +365,63 @@ __global__ void __launch_bounds__(kAmcChunk) amc_value_kernel(AmcArgs a, const R
   if (p0 < a.paths) a.bermv[static_cast<int64_t>(d) * a.paths + p0] = static_cast<double>(v);
 }
 
+// Surrogate inputs per path: the date loop of one set; running extremes of the domestic state and
+// its value at each exercise date already passed (0 before it). The exercise-date states are
+// kept in registers (compile-time indexed).
+__global__ void __launch_bounds__(256) amc_features_kernel(const double* states, int64_t state_off, int64_t paths,
+                                                           const DateRec* dates, const double* ctimes, int num_dates,
+                                                           int cd_base, SurrogateExercises ex, float* feats) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= paths) return;
+  const int inputs = kSurrogateBase + ex.count;
+  double hi = -1e300, lo = 1e300;
+  float xe[kSurrogateMaxExercises];
+#pragma unroll
+  for (int m = 0; m < kSurrogateMaxExercises; ++m) xe[m] = 0.0f;
+  int64_t row = 0;
+  for (int j = 0; j < num_dates; ++j) {
+    const double x = states[state_off + static_cast<int64_t>(j) * paths + p];
+    hi = x > hi ? x : hi;
+    lo = x < lo ? x : lo;
+#pragma unroll
+    for (int m = 0; m < kSurrogateMaxExercises; ++m)
+      if (m < ex.count && ex.date[m] == j) xe[m] = static_cast<float>(x);
+    if (!(dates[j].flags & kValued)) continue;
+    float* f = feats + (row * paths + p) * inputs;
+    f[0] = static_cast<float>(ctimes[cd_base + row]);
+    f[1] = static_cast<float>(x);
+    f[2] = static_cast<float>(hi);
+    f[3] = static_cast<float>(lo);
+#pragma unroll
+    for (int m = 0; m < kSurrogateMaxExercises; ++m)
+      if (m < ex.count) f[kSurrogateBase + m] = xe[m];
+    ++row;
+  }
+}
+
+__global__ void amc_scatter_kernel(const float* y, int64_t n, double mean, double sd, double* out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = static_cast<double>(y[i]) * sd + mean;
+}
+
 hipError_t last() { return hipGetLastError(); }
 
 }  // namespace
+
+hipError_t launchSurrogateFeatures(const double* states, int64_t state_off, int64_t paths, const DateRec* dates,
+                                   const double* ctimes, int num_dates, int cd_base, const SurrogateExercises& ex,
+                                   float* feats, hipStream_t stream) {
+  if (state_off < 0 || ex.count < 0 || ex.count > kSurrogateMaxExercises) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(amc_features_kernel, dim3(static_cast<unsigned>((paths + 255) / 256)), dim3(256), 0, stream, states,
+                     state_off, paths, dates, ctimes, num_dates, cd_base, ex, feats);
+  return last();
+}
+
+hipError_t launchSurrogateScatter(const float* y, int64_t n, double mean, double sd, double* out, hipStream_t stream) {
+  hipLaunchKernelGGL(amc_scatter_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, y, n, mean,
+                     sd, out);
+  return last();
+}
 
 template <typename Real>
 hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int max_exercises, hipStream_t stream) {

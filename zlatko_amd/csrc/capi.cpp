@@ -2,6 +2,7 @@
 // kernel execution with HIP-event timing, read-back into caller-owned ccr_exposure_out arrays.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -68,6 +69,9 @@ struct ccr_hip_plan {
   int16_t* d_exidx = nullptr;
   double* d_states = nullptr;
   double* d_bermv = nullptr;
+  double* d_ctimes = nullptr;
+  float* d_feats = nullptr;   // surrogate inputs (allocated on first surrogate execute)
+  float* d_ysur = nullptr;
   AmcArgs amc{};
   double draws_ms = 0.0, upload_ms = 0.0;
   ~ccr_hip_plan() {
@@ -251,6 +255,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.exidx = plan->d_exidx;
     a.bermv = ncd > 0 ? dalloc(static_cast<int64_t>(ncd) * P) : nullptr;
     plan->d_bermv = a.bermv;
+    plan->d_ctimes = ncd > 0 ? upload(ctx, plan.get(), h.ctimes.data(), h.ctimes.size()) : nullptr;
     a.paths = P;
     a.state_len = h.state_len;
     a.nberms = nB;
@@ -298,24 +303,106 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   return a;
 }
 
-// States pass and regressions (sets with Bermudan swaptions), then the path kernel; events 1 and
-// 2 close the two stages.
+int setRows(const HostPlan& h, int s) {
+  int rows = 0;
+  for (size_t i = 0; i < h.cdates.size() / 2; ++i)
+    if (h.cdates[2 * i] == s) ++rows;
+  return rows;
+}
+
+// The distinct exercise dates of set s's Bermudans (grid indices, ascending, at most
+// kSurrogateMaxExercises).
+SurrogateExercises exercisesOf(const HostPlan& h, int s) {
+  const SetDev& S = h.sets[static_cast<size_t>(s)];
+  std::vector<int32_t> d;
+  for (int b = S.berm_beg; b < S.berm_end; ++b) {
+    const BermDev& bm = h.berms[static_cast<size_t>(b)];
+    for (int k = 0; k < bm.ne; ++k) d.push_back(h.eidx[static_cast<size_t>(bm.eidx_off + k)]);
+  }
+  std::sort(d.begin(), d.end());
+  d.erase(std::unique(d.begin(), d.end()), d.end());
+  SurrogateExercises ex{};
+  ex.count = static_cast<int32_t>(std::min<size_t>(d.size(), kSurrogateMaxExercises));
+  for (int m = 0; m < ex.count; ++m) ex.date[m] = d[static_cast<size_t>(m)];
+  return ex;
+}
+
+}  // namespace
+
+struct ccr_hip_surrogate {
+  struct Net {
+    int set = -1;
+    int nberms = 0;
+    bool enabled = true;
+    double ee_error = 0.0;
+    std::unique_ptr<MlpRun> run;  // trained weights (inference only)
+    int inputs = kSurrogateBase;
+    double xmean[kSurrogateBase + kSurrogateMaxExercises], xsd[kSurrogateBase + kSurrogateMaxExercises];
+    double ymean = 0.0, ysd = 1.0;
+  };
+  int device = 0;
+  int num_sets = 0;
+  std::vector<Net> nets;
+};
+
+namespace {
+
+// Surrogate inputs, inference and Bermudan rows of every enabled set of `plan`.
+void surrogateRows(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* sur) {
+  const HostPlan& h = plan->host;
+  const int64_t P = h.g.num_paths;
+  if (!plan->d_feats) {
+    int maxRows = 0;
+    for (int s = 0; s < static_cast<int>(h.sets.size()); ++s) maxRows = std::max(maxRows, setRows(h, s));
+    plan->d_feats = static_cast<float*>(
+        devAlloc(ctx, plan, static_cast<size_t>(maxRows) * P * (kSurrogateBase + kSurrogateMaxExercises) * sizeof(float)));
+    plan->d_ysur = static_cast<float*>(devAlloc(ctx, plan, static_cast<size_t>(maxRows) * P * sizeof(float)));
+  }
+  for (const auto& net : sur->nets) {
+    if (!net.enabled) continue;
+    const SetDev& S = h.sets[static_cast<size_t>(net.set)];
+    const int rows = setRows(h, net.set);
+    const SurrogateExercises ex = exercisesOf(h, net.set);
+    if (kSurrogateBase + ex.count != net.inputs) invalid("surrogate: trained for other exercise dates");
+    check(launchSurrogateFeatures(plan->d_states, S.state_off, P, plan->d_dates + S.date_off, plan->d_ctimes, S.num_dates,
+                                  S.cd_base, ex, plan->d_feats, ctx->stream),
+          "surrogate features launch");
+    check(mlpInfer(net.run->device(), plan->d_feats, static_cast<int64_t>(rows) * P, net.xmean, net.xsd, plan->d_ysur,
+                   ctx->stream),
+          "surrogate inference launch");
+    check(launchSurrogateScatter(plan->d_ysur, static_cast<int64_t>(rows) * P, net.ymean, net.ysd,
+                                 plan->d_bermv + static_cast<int64_t>(S.cd_base) * P, ctx->stream),
+          "surrogate scatter launch");
+  }
+}
+
+// States pass and regressions (sets with Bermudan swaptions) or the surrogate, then the path
+// kernel; events 1 and 2 close the two stages.
 template <typename Real>
-void launchPathStages(ccr_hip_ctx* ctx, ccr_hip_plan* plan, int blocks) {
+void launchPathStages(ccr_hip_ctx* ctx, ccr_hip_plan* plan, int blocks, const ccr_hip_surrogate* sur) {
   const HostPlan& h = plan->host;
   PathArgs<Real> a = pathArgs<Real>(plan);
   if (!h.berms.empty()) {
     PathArgs<Real> s = a;
     s.states = plan->d_states;
     check(launchStates<Real>(s, blocks, h.block, ctx->stream), "states kernel launch");
-    check(launchAmc<Real>(plan->amc, a.bonds, a.fixed, h.max_exercises, ctx->stream), "regression kernel launch");
+    bool allSurrogate = sur != nullptr;
+    for (int si = 0; sur && si < static_cast<int>(h.sets.size()); ++si) {
+      if (h.sets[static_cast<size_t>(si)].berm_end == h.sets[static_cast<size_t>(si)].berm_beg) continue;
+      bool on = false;
+      for (const auto& net : sur->nets) on = on || (net.set == si && net.enabled);
+      allSurrogate = allSurrogate && on;
+    }
+    if (!allSurrogate)
+      check(launchAmc<Real>(plan->amc, a.bonds, a.fixed, h.max_exercises, ctx->stream), "regression kernel launch");
+    if (sur) surrogateRows(ctx, plan, sur);
   }
   check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
   check(launchPath<Real>(a, blocks, h.block, ctx->stream), "path kernel launch");
   check(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
 }
 
-void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
+void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* sur = nullptr) {
   if (ctx->fault == CCR_FAULT_LAUNCH) {
     ctx->fault = CCR_FAULT_NONE;
     fail(CCR_ERR_DEVICE, "kernel launch failed (forced by ccr_hip_set_fault)");
@@ -325,8 +412,8 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
   const int nsets = static_cast<int>(h.sets.size());
   const int blocks = static_cast<int>(h.block_set.size());
   check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
-  if (h.precision == CCR_FP32) launchPathStages<float>(ctx, plan, blocks);
-  else launchPathStages<double>(ctx, plan, blocks);
+  if (h.precision == CCR_FP32) launchPathStages<float>(ctx, plan, blocks, sur);
+  else launchPathStages<double>(ctx, plan, blocks, sur);
   check(launchReduce(plan->d_sets, plan->d_partials, nsets, h.max_rep, h.total_waves / nsets,
                      1.0 / static_cast<double>(h.g.num_paths), plan->d_stats, ctx->stream),
         "reduce kernel launch");
@@ -411,6 +498,22 @@ void readBack(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs) {
       }
     }
   }
+}
+
+// Expected exposure per set and reporting date of the last launch on `plan`.
+std::vector<std::vector<double>> expectedExposure(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
+  const HostPlan& h = plan->host;
+  const size_t ns = h.sets.size();
+  std::vector<std::vector<double>> ee(ns, std::vector<double>(static_cast<size_t>(h.max_rep), 0.0));
+  std::vector<ccr_exposure_out> outs(ns);
+  for (size_t s = 0; s < ns; ++s) {
+    outs[s] = ccr_exposure_out{};
+    outs[s].capacity = h.max_rep;
+    outs[s].expected_exposure = ee[s].data();
+  }
+  readBack(ctx, plan, outs.data());
+  for (size_t s = 0; s < ns; ++s) ee[s].resize(static_cast<size_t>(outs[s].num_dates));
+  return ee;
 }
 
 int32_t writeGrid(const TimeGrid& g, double* out, int32_t cap, int32_t* size) {
@@ -639,11 +742,176 @@ int32_t ccr_hip_mlp_train(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, const ccr_
     run.setInputs(xs.data());
     run.setTargets(ys.data());
     const auto t0 = std::chrono::steady_clock::now();
-    run.train(o->epochs, o->learning_rate, o->weight_decay);
+    run.train(o->epochs, o->learning_rate, o->weight_decay, o->batch);
     if (train_ms) *train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (loss) *loss = run.loss();
     run.getParams(params);
   });
+}
+
+int32_t ccr_hip_surrogate_train(ccr_hip_ctx* ctx, ccr_hip_plan* train, ccr_hip_plan* holdout,
+                                const ccr_surrogate_options* o, ccr_hip_surrogate** out, ccr_surrogate_report* report) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!train || !holdout || !o || !out) invalid("null argument");
+    if (train == holdout) invalid("surrogate: the hold-out plan must differ from the training plan");
+    const HostPlan& ht = train->host;
+    const HostPlan& hh = holdout->host;
+    if (ht.sets.size() != hh.sets.size()) invalid("surrogate: the plans hold different netting sets");
+    for (size_t s = 0; s < ht.sets.size(); ++s)
+      if (ht.sets[s].berm_end - ht.sets[s].berm_beg != hh.sets[s].berm_end - hh.sets[s].berm_beg)
+        invalid("surrogate: the plans hold different netting sets");
+    if (ht.berms.empty()) invalid("surrogate: no netting set holds a Bermudan swaption");
+    if (o->epochs < 0 || o->gate_threshold < 0.0) invalid("surrogate: invalid options");
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    auto sur = std::make_unique<ccr_hip_surrogate>();
+    sur->device = ctx->device;
+    sur->num_sets = static_cast<int>(ht.sets.size());
+    ccr_surrogate_report rep{};
+    // Method-generated pathwise values: the training plan executed with the regressions.
+    launchAll(ctx, train);
+    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    const int64_t P = ht.g.num_paths;
+    const int64_t nTrain = o->train_paths > 0 ? std::min<int64_t>(o->train_paths, P) : P;
+    int maxRows = 0;
+    for (int s = 0; s < static_cast<int>(ht.sets.size()); ++s) maxRows = std::max(maxRows, setRows(ht, s));
+    constexpr int kMaxIn = kSurrogateBase + kSurrogateMaxExercises;
+    float* feats = nullptr;
+    check(hipMalloc(reinterpret_cast<void**>(&feats), static_cast<size_t>(maxRows) * P * kMaxIn * sizeof(float)),
+          "hipMalloc");
+    std::unique_ptr<float, hipError_t (*)(void*)> featsOwner(feats, hipFree);
+    for (int s = 0; s < static_cast<int>(ht.sets.size()); ++s) {
+      const SetDev& S = ht.sets[static_cast<size_t>(s)];
+      if (S.berm_end == S.berm_beg) continue;
+      const int rows = setRows(ht, s);
+      const SurrogateExercises ex = exercisesOf(ht, s);
+      const int in = kSurrogateBase + ex.count;
+      check(launchSurrogateFeatures(train->d_states, S.state_off, P, train->d_dates + S.date_off, train->d_ctimes,
+                                    S.num_dates, S.cd_base, ex, feats, ctx->stream),
+            "surrogate features launch");
+      std::vector<float> f(static_cast<size_t>(rows) * P * in);
+      std::vector<double> v(static_cast<size_t>(rows) * P);
+      check(hipMemcpyAsync(f.data(), feats, f.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+      check(hipMemcpyAsync(v.data(), train->d_bermv + static_cast<int64_t>(S.cd_base) * P, v.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream),
+            "hipMemcpyAsync");
+      check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+      const int64_t n = static_cast<int64_t>(rows) * nTrain;
+      std::vector<double> X(static_cast<size_t>(n * in)), Y(static_cast<size_t>(n));
+      int64_t k = 0;
+      for (int r = 0; r < rows; ++r)
+        for (int64_t p = 0; p < nTrain; ++p, ++k) {
+          for (int i = 0; i < in; ++i) X[static_cast<size_t>(k * in + i)] = f[static_cast<size_t>((r * P + p) * in + i)];
+          Y[static_cast<size_t>(k)] = v[static_cast<size_t>(r * P + p)];
+        }
+      ccr_hip_surrogate::Net net;
+      net.set = s;
+      net.nberms = S.berm_end - S.berm_beg;
+      net.inputs = in;
+      standardiserFit(X.data(), n, in, net.xmean, net.xsd);
+      standardiserFit(Y.data(), n, 1, &net.ymean, &net.ysd);
+      for (int64_t i = 0; i < n; ++i) {
+        for (int j = 0; j < in; ++j)
+          X[static_cast<size_t>(i * in + j)] = (X[static_cast<size_t>(i * in + j)] - net.xmean[j]) / net.xsd[j];
+        Y[static_cast<size_t>(i)] = (Y[static_cast<size_t>(i)] - net.ymean) / net.ysd;
+      }
+      if (o->batch > 0) {  // mini-batches of consecutive rows: shuffle the rows once
+        std::vector<int64_t> perm(static_cast<size_t>(n));
+        for (int64_t i = 0; i < n; ++i) perm[static_cast<size_t>(i)] = i;
+        std::mt19937_64 rng(o->seed);
+        std::shuffle(perm.begin(), perm.end(), rng);
+        std::vector<double> X2(X.size()), Y2(Y.size());
+        for (int64_t i = 0; i < n; ++i) {
+          const int64_t q = perm[static_cast<size_t>(i)];
+          for (int j = 0; j < in; ++j) X2[static_cast<size_t>(i * in + j)] = X[static_cast<size_t>(q * in + j)];
+          Y2[static_cast<size_t>(i)] = Y[static_cast<size_t>(q)];
+        }
+        X.swap(X2);
+        Y.swap(Y2);
+      }
+      const ccr_mlp_desc desc{in, o->hidden, o->hidden_layers, 1, CCR_ACT_SILU};
+      std::vector<double> params(static_cast<size_t>(mlpShape(desc).count));
+      {
+        MlpRun trainer(desc, CCR_BF16, n, ctx->stream);
+        mlpInit(trainer.device().sh, o->seed, params.data());
+        trainer.setParams(params.data());
+        trainer.setInputs(X.data());
+        trainer.setTargets(Y.data());
+        const auto t0 = std::chrono::steady_clock::now();
+        trainer.train(o->epochs, o->learning_rate, o->weight_decay, o->batch);
+        rep.train_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        rep.loss = std::max(rep.loss, trainer.loss());
+        trainer.getParams(params.data());
+      }
+      net.run = std::make_unique<MlpRun>(desc, CCR_BF16, kMlpRows, ctx->stream);
+      net.run->setParams(params.data());
+      rep.samples += n;
+      rep.candidate_sets += 1;
+      sur->nets.push_back(std::move(net));
+    }
+    // The gate: hold-out EE with the surrogate against the regression method.
+    launchAll(ctx, holdout);
+    const auto plain = expectedExposure(ctx, holdout);
+    launchAll(ctx, holdout, sur.get());
+    const auto approx = expectedExposure(ctx, holdout);
+    for (auto& net : sur->nets) {
+      const auto& a = plain[static_cast<size_t>(net.set)];
+      const auto& b = approx[static_cast<size_t>(net.set)];
+      double peak = 0.0, err = 0.0;
+      for (size_t r = 0; r < a.size(); ++r) {
+        peak = std::max(peak, std::fabs(a[r]));
+        err = std::max(err, std::fabs(b[r] - a[r]));
+      }
+      net.ee_error = peak > 0.0 ? err / peak : err;
+      net.enabled = net.ee_error <= o->gate_threshold;
+      rep.max_ee_error = std::max(rep.max_ee_error, net.ee_error);
+      rep.enabled_sets += net.enabled ? 1 : 0;
+    }
+    if (report) *report = rep;
+    *out = sur.release();
+  });
+}
+
+int32_t ccr_hip_surrogate_set_info(const ccr_hip_surrogate* sur, int32_t set, int32_t* enabled, double* ee_error) {
+  if (!sur || set < 0 || set >= sur->num_sets) return CCR_ERR_INVALID_ARGUMENT;
+  if (enabled) *enabled = -1;
+  if (ee_error) *ee_error = 0.0;
+  for (const auto& net : sur->nets)
+    if (net.set == set) {
+      if (enabled) *enabled = net.enabled ? 1 : 0;
+      if (ee_error) *ee_error = net.ee_error;
+    }
+  return CCR_OK;
+}
+
+int32_t ccr_hip_execute_surrogate_async(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* sur) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!plan || !sur) invalid("null argument");
+    if (static_cast<int>(plan->host.sets.size()) != sur->num_sets || sur->device != ctx->device)
+      invalid("surrogate: trained for other netting sets or another device");
+    for (const auto& net : sur->nets) {
+      const SetDev& S = plan->host.sets[static_cast<size_t>(net.set)];
+      if (S.berm_end - S.berm_beg != net.nberms) invalid("surrogate: trained for other netting sets");
+    }
+    launchAll(ctx, plan, sur);
+  });
+}
+
+int32_t ccr_hip_execute_surrogate(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* sur,
+                                  ccr_exposure_out* outs) {
+  const int32_t rc = ccr_hip_execute_surrogate_async(ctx, plan, sur);
+  if (rc != CCR_OK) return rc;
+  return guarded(ctx, [&] {
+    if (!outs) invalid("null outputs");
+    readBack(ctx, plan, outs);
+  });
+}
+
+void ccr_hip_surrogate_destroy(ccr_hip_surrogate* sur) {
+  if (!sur) return;
+  (void)hipSetDevice(sur->device);
+  delete sur;
 }
 
 int32_t ccr_hip_mlp_gradient(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, int32_t precision, const double* params,
@@ -668,6 +936,27 @@ int32_t ccr_hip_mlp_predict(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, int32_t 
     if (!desc || !params || !stdz || !X || !Y) invalid("null argument");
     check(hipSetDevice(ctx->device), "hipSetDevice");
     const int din = desc->inputs, dout = desc->outputs;
+    if (precision == CCR_BF16) {
+      // The fused inference kernel (weights in LDS, no per-sample buffers) on raw rows.
+      MlpRun run(*desc, CCR_BF16, kMlpRows, ctx->stream);
+      run.setParams(params);
+      std::vector<float> xf(static_cast<size_t>(n * din));
+      for (size_t i = 0; i < xf.size(); ++i) xf[i] = static_cast<float>(X[i]);
+      float *dx = nullptr, *dy = nullptr;
+      check(hipMalloc(reinterpret_cast<void**>(&dx), xf.size() * sizeof(float)), "hipMalloc");
+      std::unique_ptr<float, hipError_t (*)(void*)> ownX(dx, hipFree);
+      check(hipMalloc(reinterpret_cast<void**>(&dy), static_cast<size_t>(n * dout) * sizeof(float)), "hipMalloc");
+      std::unique_ptr<float, hipError_t (*)(void*)> ownY(dy, hipFree);
+      check(hipMemcpyAsync(dx, xf.data(), xf.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
+      check(mlpInfer(run.device(), dx, n, stdz, stdz + din, dy, ctx->stream), "MLP inference launch");
+      std::vector<float> yf(static_cast<size_t>(n * dout));
+      check(hipMemcpyAsync(yf.data(), dy, yf.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+      check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+      for (int64_t r = 0; r < n; ++r)
+        for (int k = 0; k < dout; ++k)
+          Y[r * dout + k] = yf[static_cast<size_t>(r * dout + k)] * stdz[2 * din + dout + k] + stdz[2 * din + k];
+      return;
+    }
     MlpRun run(*desc, precision, n, ctx->stream);
     std::vector<double> xs(static_cast<size_t>(n * din));
     for (int64_t r = 0; r < n; ++r)

@@ -86,6 +86,22 @@ struct AmcArgs {
 template <typename Real>
 hipError_t launchAmc(const AmcArgs& args, const Real* bonds, const Real* fixed, int max_exercises, hipStream_t stream);
 
+// Surrogate features of one set with Bermudans on every valued date and path, from the recorded
+// states: rows [valued date][path][kSurrogateBase + count] = (t, x, running max x, running min x,
+// x at each of the set's exercise dates once passed, else 0) (FP32).
+constexpr int kSurrogateBase = 4;
+constexpr int kSurrogateMaxExercises = 28;  // BF16 network inputs are at most 32
+struct SurrogateExercises {
+  int32_t count;                           // distinct exercise dates used (the earliest ones)
+  int32_t date[kSurrogateMaxExercises];    // their indices on the set's simulation grid
+};
+// `dates` are the set's date records (num_dates of them); ctimes the plan's continuation-date times.
+hipError_t launchSurrogateFeatures(const double* states, int64_t state_off, int64_t paths, const DateRec* dates,
+                                   const double* ctimes, int num_dates, int cd_base, const SurrogateExercises& ex,
+                                   float* feats, hipStream_t stream);
+// bermv rows [cd_base, cd_base + rows) x paths = y * sd + mean.
+hipError_t launchSurrogateScatter(const float* y, int64_t n, double mean, double sd, double* out, hipStream_t stream);
+
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);
 // The states-only pass of the path kernel (AMC): writes x and D of every date into args.states.

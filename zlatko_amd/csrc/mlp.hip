@@ -12,6 +12,7 @@
 // Adam follows fitMlp (src/ai/surrogate.cpp:223-240) in FP32.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -243,7 +244,8 @@ __global__ void __launch_bounds__(256) bf16_bwd_kernel(const __bf16* dnext, cons
 // Weight-gradient partial of one 16 x 16 tile of dW_l over one chunk of samples (one wavefront):
 // dW[o][j] = sum_s D_{l+1}[o][s] A_l[j][s] from the feature-major copies; tiles of the first column
 // also sum the bias gradient. Writes only real (unpadded) entries.
-__global__ void __launch_bounds__(64) bf16_wgrad_kernel(const __bf16* dfm, const __bf16* afm, int64_t n_pad, int fi,
+__global__ void __launch_bounds__(64) bf16_wgrad_kernel(const __bf16* dfm, const __bf16* afm, int64_t n_pad, int64_t ld,
+                                                        int fi,
                                                         int fo, int pi, int64_t offW, int64_t offB, int64_t count,
                                                         float* partial) {
   const int lane = static_cast<int>(threadIdx.x);
@@ -252,8 +254,8 @@ __global__ void __launch_bounds__(64) bf16_wgrad_kernel(const __bf16* dfm, const
   const int r = lane & 15, g = lane >> 4;
   const int64_t c = blockIdx.y;
   const int64_t k0 = c * kMlpChunk, k1 = k0 + kMlpChunk < n_pad ? k0 + kMlpChunk : n_pad;
-  const __bf16* arow = dfm + static_cast<int64_t>(mt * 16 + r) * n_pad + 8 * g;
-  const __bf16* brow = afm + static_cast<int64_t>(nt * 16 + r) * n_pad + 8 * g;
+  const __bf16* arow = dfm + static_cast<int64_t>(mt * 16 + r) * ld + 8 * g;
+  const __bf16* brow = afm + static_cast<int64_t>(nt * 16 + r) * ld + 8 * g;
   f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
   float bsum = 0.0f;
   for (int64_t k = k0; k < k1; k += 32) {
@@ -278,16 +280,141 @@ __global__ void __launch_bounds__(64) bf16_wgrad_kernel(const __bf16* dfm, const
   }
 }
 
+// Fused inference (hidden = 128): every layer of a 16-sample tile per wavefront, weights staged
+// in LDS, each layer's output turned into the next layer's A operand through a per-wavefront LDS
+// tile. Inputs x [n][inputs] (raw) are standardised on load with (mean, sd); writes the
+// standardised outputs y [n][outputs]. Grid-stride over tiles.
+constexpr int kInferWaves = 8;
+constexpr int kActStride = 128 + 8;  // BF16 elements per activation row in LDS (padding vs bank conflicts)
+struct InferArgs {
+  const __bf16* w[kMlpMaxLayers];
+  const float* b[kMlpMaxLayers];
+  const float* x;
+  float* y;
+  float mean[32], sd[32];
+  int64_t n;
+  int layers, inputs, outputs, act;
+};
+
+__device__ inline void stageWeights(__bf16* dst, const __bf16* src, int elems) {
+  for (int i = static_cast<int>(threadIdx.x) * 8; i < elems; i += static_cast<int>(blockDim.x) * 8)
+    *reinterpret_cast<bf16x8*>(dst + i) = load8(src + i);
+}
+
+__global__ void __launch_bounds__(64 * kInferWaves) bf16_infer_kernel(InferArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __bf16* wl = reinterpret_cast<__bf16*>(lds_raw);
+  // Layers in order: first 128 x 32, middle 128 x 128, last 32 x 128 (32 x 32 alone).
+  int wtotal = 0;
+  for (int l = 0; l < a.layers; ++l) {
+    const int fi = l == 0 ? 32 : 128, fo = l + 1 == a.layers ? 32 : 128;
+    stageWeights(wl + wtotal, a.w[l], fi * fo);
+    wtotal += fi * fo;
+  }
+  __bf16* act = wl + wtotal + (static_cast<int>(threadIdx.x) >> 6) * 16 * kActStride;
+  __syncthreads();
+  const int lane = static_cast<int>(threadIdx.x) & 63, r = lane & 15, g = lane >> 4;
+  const int64_t tiles = (a.n + 15) / 16;
+  for (int64_t tile = static_cast<int64_t>(blockIdx.x) * kInferWaves + (threadIdx.x >> 6); tile < tiles;
+       tile += static_cast<int64_t>(gridDim.x) * kInferWaves) {
+    const int64_t s0 = tile * 16;
+    // Layer-0 A operand: sample s0 + r, features 8 g .. 8 g + 7.
+    bf16x8 a0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int j = 8 * g + i;
+      const int64_t s = s0 + r;
+      float v = 0.0f;
+      if (j < a.inputs && s < a.n) v = (a.x[s * a.inputs + j] - a.mean[j]) / a.sd[j];
+      a0[i] = static_cast<__bf16>(v);
+    }
+    int wo = 0;
+    for (int l = 0; l < a.layers; ++l) {
+      const bool lastL = l + 1 == a.layers;
+      const int fi = l == 0 ? 32 : 128, nt = lastL ? 2 : 8;
+      const __bf16* w = wl + wo;
+      wo += fi * nt * 16;
+      bf16x8 afr[4];
+      if (l == 0) {
+        afr[0] = a0;
+      } else {
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) afr[ks] = *reinterpret_cast<const bf16x8*>(act + r * kActStride + ks * 32 + 8 * g);
+      }
+      for (int t = 0; t < nt; ++t) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (l == 0) {
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[0], *reinterpret_cast<const bf16x8*>(w + (t * 16 + r) * fi + 8 * g),
+                                                        acc, 0, 0, 0);
+        } else {
+#pragma unroll
+          for (int ks = 0; ks < 4; ++ks)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                afr[ks], *reinterpret_cast<const bf16x8*>(w + (t * 16 + r) * fi + ks * 32 + 8 * g), acc, 0, 0, 0);
+        }
+        const int o = t * 16 + r;
+        const float bo = a.b[l][o];
+        if (lastL) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int64_t s = s0 + 4 * g + i;
+            if (o < a.outputs && s < a.n) a.y[s * a.outputs + o] = acc[i] + bo;
+          }
+        } else {
+          // This wavefront's tile rows 4 g + i, column o (all lanes have read the previous
+          // layer's rows into afr before any lane writes: wavefront-synchronous).
+#pragma unroll
+          for (int i = 0; i < 4; ++i) act[(4 * g + i) * kActStride + o] = static_cast<__bf16>(actF(a.act, acc[i] + bo));
+        }
+      }
+    }
+  }
+}
+
 hipError_t last() { return hipGetLastError(); }
 
 unsigned blocksFor(int64_t n, int per) { return static_cast<unsigned>((n + per - 1) / per); }
 
 }  // namespace
 
+hipError_t mlpInfer(const MlpDevice& m, const float* x, int64_t n, const double* mean, const double* sd, float* y,
+                    hipStream_t stream) {
+  const MlpShape& sh = m.sh;
+  if (m.precision != CCR_BF16 || sh.pad[0] != 32 || sh.pad[sh.layers] != 32) return hipErrorInvalidValue;
+  for (int l = 1; l < sh.layers; ++l)
+    if (sh.size[l] != 128) return hipErrorInvalidValue;
+  InferArgs a{};
+  for (int l = 0; l < sh.layers; ++l) {
+    a.w[l] = static_cast<const __bf16*>(m.w16[l]);
+    a.b[l] = m.bpad[l];
+  }
+  for (int j = 0; j < sh.size[0]; ++j) {
+    a.mean[j] = static_cast<float>(mean[j]);
+    a.sd[j] = static_cast<float>(sd[j]);
+  }
+  a.x = x;
+  a.y = y;
+  a.n = n;
+  a.layers = sh.layers;
+  a.inputs = sh.size[0];
+  a.outputs = sh.size[sh.layers];
+  a.act = sh.act;
+  int welems = 0;
+  for (int l = 0; l < sh.layers; ++l) welems += sh.pad[l] * sh.pad[l + 1];
+  const size_t lds = static_cast<size_t>(welems) * 2 + static_cast<size_t>(kInferWaves) * 16 * kActStride * 2;
+  const int64_t tiles = (n + 15) / 16;
+  const unsigned grid = static_cast<unsigned>(std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves, 2048));
+  if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&bf16_infer_kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)))
+    return e;
+  hipLaunchKernelGGL(bf16_infer_kernel, dim3(grid), dim3(64 * kInferWaves), lds, stream, a);
+  return last();
+}
+
 hipError_t mlpLoadInputs(const MlpDevice& m, const float* xs, hipStream_t stream) {
   if (m.precision != CCR_BF16) return hipSuccess;
   hipLaunchKernelGGL(bf16_inputs_kernel, dim3(blocksFor(m.n_pad * m.sh.pad[0], 256)), dim3(256), 0, stream, xs,
-                     m.sh.size[0], m.sh.pad[0], m.n, m.n_pad, static_cast<__bf16*>(m.a_sm[0]),
+                     m.sh.size[0], m.sh.pad[0], m.n, m.ld, static_cast<__bf16*>(m.a_sm[0]),
                      static_cast<__bf16*>(m.a_fm[0]));
   return last();
 }
@@ -321,13 +448,13 @@ hipError_t mlpForward(const MlpDevice& m, hipStream_t stream) {
     const bool lastL = l + 1 == L;
     const int pi = sh.pad[l], po = sh.pad[l + 1];
     if (pi == 32 && po == 128 && !lastL)
-      hipLaunchKernelGGL((bf16_fwd_kernel<32, 128, false>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.n_pad, sh.act);
+      hipLaunchKernelGGL((bf16_fwd_kernel<32, 128, false>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.ld, sh.act);
     else if (pi == 128 && po == 128 && !lastL)
-      hipLaunchKernelGGL((bf16_fwd_kernel<128, 128, false>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.n_pad, sh.act);
+      hipLaunchKernelGGL((bf16_fwd_kernel<128, 128, false>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.ld, sh.act);
     else if (pi == 128 && po == 32 && lastL)
-      hipLaunchKernelGGL((bf16_fwd_kernel<128, 32, true>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.n_pad, sh.act);
+      hipLaunchKernelGGL((bf16_fwd_kernel<128, 32, true>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.ld, sh.act);
     else if (pi == 32 && po == 32 && lastL)
-      hipLaunchKernelGGL((bf16_fwd_kernel<32, 32, true>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.n_pad, sh.act);
+      hipLaunchKernelGGL((bf16_fwd_kernel<32, 32, true>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.ld, sh.act);
     else
       return hipErrorInvalidValue;
   }
@@ -343,7 +470,7 @@ hipError_t mlpLossGrad(const MlpDevice& m, hipStream_t stream) {
                        static_cast<float*>(m.d_sm[L]), m.n, nout, m.loss_part);
     return last();
   }
-  hipLaunchKernelGGL(bf16_loss_kernel, dim3(blocks), dim3(256), 0, stream, m.z[L], sh.pad[L], m.ys, nout, m.n, m.n_pad,
+  hipLaunchKernelGGL(bf16_loss_kernel, dim3(blocks), dim3(256), 0, stream, m.z[L], sh.pad[L], m.ys, nout, m.n, m.ld,
                      static_cast<__bf16*>(m.d_sm[L]), static_cast<__bf16*>(m.d_fm[L]), m.loss_part);
   return last();
 }
@@ -369,9 +496,9 @@ hipError_t mlpBackward(const MlpDevice& m, hipStream_t stream) {
       __bf16* dfm = static_cast<__bf16*>(m.d_fm[l]);
       const int pi = sh.pad[l], po = sh.pad[l + 1];
       if (pi == 128 && po == 32)
-        hipLaunchKernelGGL((bf16_bwd_kernel<128, 32>), grid, blk, 0, stream, dn, wt, m.z[l], dsm, dfm, m.n_pad, sh.act);
+        hipLaunchKernelGGL((bf16_bwd_kernel<128, 32>), grid, blk, 0, stream, dn, wt, m.z[l], dsm, dfm, m.ld, sh.act);
       else if (pi == 128 && po == 128)
-        hipLaunchKernelGGL((bf16_bwd_kernel<128, 128>), grid, blk, 0, stream, dn, wt, m.z[l], dsm, dfm, m.n_pad, sh.act);
+        hipLaunchKernelGGL((bf16_bwd_kernel<128, 128>), grid, blk, 0, stream, dn, wt, m.z[l], dsm, dfm, m.ld, sh.act);
       else
         return hipErrorInvalidValue;
     }
@@ -379,7 +506,7 @@ hipError_t mlpBackward(const MlpDevice& m, hipStream_t stream) {
       hipLaunchKernelGGL(bf16_wgrad_kernel, dim3(static_cast<unsigned>((sh.pad[l + 1] / 16) * (sh.pad[l] / 16)),
                                                  static_cast<unsigned>(m.nchunks)),
                          dim3(64), 0, stream, static_cast<const __bf16*>(m.d_fm[l + 1]),
-                         static_cast<const __bf16*>(m.a_fm[l]), m.n_pad, sh.size[l], sh.size[l + 1], sh.pad[l],
+                         static_cast<const __bf16*>(m.a_fm[l]), m.n_pad, m.ld, sh.size[l], sh.size[l + 1], sh.pad[l],
                          sh.offW[l], sh.offB[l], sh.count, m.partial);
   }
   hipLaunchKernelGGL(reduce_kernel, dim3(blocksFor(sh.count, 256)), dim3(256), 0, stream, m.partial, sh.count, m.nchunks,

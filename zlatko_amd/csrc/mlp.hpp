@@ -43,6 +43,7 @@ struct MlpDevice {
   MlpShape sh;
   int precision = CCR_FP32;
   int64_t n = 0, n_pad = 0, nchunks = 0;
+  int64_t ld = 0;            // row length of the feature-major arrays (the full sample count, padded)
   float* params = nullptr;   // FP32 master parameters [count]
   float* grad = nullptr;     // [count]
   float* mom = nullptr;
@@ -73,5 +74,9 @@ hipError_t mlpAdam(const MlpDevice& m, float lr, float wd, float c1, float c2, h
 hipError_t mlpRefreshBf16(const MlpDevice& m, hipStream_t stream);      // BF16 weight copies from params
 // BF16 inputs from FP32 standardised rows (the BF16 path's layer-0 activations).
 hipError_t mlpLoadInputs(const MlpDevice& m, const float* xs, hipStream_t stream);
+// Fused BF16 inference with m's weights (hidden = 128): raw rows x [n][inputs], standardised on
+// load with (mean, sd); standardised outputs y [n][outputs]. Needs no per-sample buffers.
+hipError_t mlpInfer(const MlpDevice& m, const float* x, int64_t n, const double* mean, const double* sd, float* y,
+                    hipStream_t stream);
 
 }  // namespace ccrhip

@@ -103,6 +103,7 @@ MlpRun::MlpRun(const ccr_mlp_desc& d, int precision, int64_t n, hipStream_t stre
   m_.precision = precision;
   m_.n = n;
   m_.n_pad = precision == CCR_BF16 ? (n + kMlpRows - 1) / kMlpRows * kMlpRows : n;
+  m_.ld = m_.n_pad;
   const int64_t np = m_.n_pad;
   m_.nchunks = (np + kMlpChunk - 1) / kMlpChunk;
   try {
@@ -219,17 +220,46 @@ double MlpRun::lossAndGradient(double* grad) {
   return l;
 }
 
-void MlpRun::train(int epochs, double lr, double wd) {
-  for (int e = 0; e < epochs; ++e) {
+// Samples [b0, b0 + count) of m as a device of their own (BF16 path; b0 a multiple of kMlpRows).
+MlpDevice batchView(const MlpDevice& m, int64_t b0, int64_t count) {
+  MlpDevice v = m;
+  v.n = count;
+  v.n_pad = (count + kMlpRows - 1) / kMlpRows * kMlpRows;
+  v.nchunks = (v.n_pad + kMlpChunk - 1) / kMlpChunk;
+  v.loss_blocks = static_cast<int>((count * m.sh.size[m.sh.layers] + 255) / 256);
+  for (int l = 0; l <= m.sh.layers; ++l) {
+    const int64_t sm = b0 * m.sh.pad[l];
+    if (v.a_sm[l]) v.a_sm[l] = static_cast<__bf16*>(v.a_sm[l]) + sm;
+    if (v.a_fm[l]) v.a_fm[l] = static_cast<__bf16*>(v.a_fm[l]) + b0;
+    if (v.z[l]) v.z[l] += sm;
+    if (v.d_sm[l]) v.d_sm[l] = static_cast<__bf16*>(v.d_sm[l]) + sm;
+    if (v.d_fm[l]) v.d_fm[l] = static_cast<__bf16*>(v.d_fm[l]) + b0;
+  }
+  v.ys += b0 * m.sh.size[m.sh.layers];
+  return v;
+}
+
+void MlpRun::train(int epochs, double lr, double wd, int64_t batch) {
+  if (batch > 0 && m_.precision != CCR_BF16) invalid("Mlp: mini-batches need the BF16 path");
+  if (batch > 0 && batch % kMlpRows != 0) invalid("Mlp: the batch size must be a multiple of 64");
+  const bool full = batch <= 0 || batch >= m_.n;
+  auto step = [&](const MlpDevice& d) {
     ++epoch_;
-    forward();
-    hipCheck(mlpLossGrad(m_, stream_), "MLP loss");
-    hipCheck(mlpBackward(m_, stream_), "MLP backward");
+    hipCheck(mlpForward(d, stream_), "MLP forward");
+    hipCheck(mlpLossGrad(d, stream_), "MLP loss");
+    hipCheck(mlpBackward(d, stream_), "MLP backward");
     const double c1 = 1.0 - std::pow(0.9, static_cast<double>(epoch_));
     const double c2 = 1.0 - std::pow(0.999, static_cast<double>(epoch_));
-    hipCheck(mlpAdam(m_, static_cast<float>(lr), static_cast<float>(wd), static_cast<float>(c1), static_cast<float>(c2),
+    hipCheck(mlpAdam(d, static_cast<float>(lr), static_cast<float>(wd), static_cast<float>(c1), static_cast<float>(c2),
                      stream_),
              "MLP Adam");
+  };
+  for (int e = 0; e < epochs; ++e) {
+    if (full) {
+      step(m_);
+      continue;
+    }
+    for (int64_t b0 = 0; b0 < m_.n; b0 += batch) step(batchView(m_, b0, std::min(batch, m_.n - b0)));
   }
   hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }

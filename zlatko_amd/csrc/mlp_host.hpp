@@ -29,7 +29,9 @@ class MlpRun {
   void setInputs(const double* xs);             // standardised rows [n][inputs]
   void setTargets(const double* ys);            // standardised rows [n][outputs]
   double lossAndGradient(double* grad);         // forward + backward at the current parameters
-  void train(int epochs, double lr, double wd); // Adam steps (epoch counter continues across calls)
+  // Adam: epochs passes over the samples, in steps of `batch` consecutive samples (0: full
+  // batch, the reference's scheme); the step counter continues across calls.
+  void train(int epochs, double lr, double wd, int64_t batch = 0);
   double loss();                                // forward + loss at the current parameters
   void outputs(double* ys) const;               // standardised network outputs [n][outputs] (after a forward)
   void forward();

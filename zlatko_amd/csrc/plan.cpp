@@ -737,6 +737,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
           cdOf[static_cast<std::size_t>(j)] = static_cast<int32_t>(plan.cdates.size() / 2);
           plan.cdates.push_back(s);
           plan.cdates.push_back(j);
+          plan.ctimes.push_back(sim[static_cast<std::size_t>(j)]);
         }
     for (int b = 0; b < nB; ++b) {
       const Trade& tr = berms[static_cast<std::size_t>(b)];

@@ -45,6 +45,7 @@ struct HostPlan {
   int64_t cont_len = 0;             // kContStride x sum over Bermudans of their set's dates
   std::vector<int32_t> exlist;      // per exercise (global order): its Bermudan
   std::vector<int32_t> cdates;      // continuation dates: (set, date) pairs
+  std::vector<double> ctimes;       // time of each continuation date
   std::vector<int32_t> bdate_berm;  // per BermDate record: its Bermudan
   std::vector<int32_t> bdate_cd;    // per BermDate record: its continuation date (-1: none)
   std::vector<GemmTile> tiles;      // continuation GEMM tiles

@@ -66,6 +66,15 @@ def lib():
         L.ccr_hip_mlp_gradient.argtypes = [P, P, I32, P, P, P, I64, P, P]
         L.ccr_hip_mlp_predict.restype = I32
         L.ccr_hip_mlp_predict.argtypes = [P, P, I32, P, P, P, I64, P]
+        L.ccr_hip_surrogate_train.restype = I32
+        L.ccr_hip_surrogate_train.argtypes = [P, P, P, P, P, P]
+        L.ccr_hip_surrogate_set_info.restype = I32
+        L.ccr_hip_surrogate_set_info.argtypes = [P, I32, P, P]
+        L.ccr_hip_execute_surrogate.restype = I32
+        L.ccr_hip_execute_surrogate.argtypes = [P, P, P, P]
+        L.ccr_hip_execute_surrogate_async.restype = I32
+        L.ccr_hip_execute_surrogate_async.argtypes = [P, P, P]
+        L.ccr_hip_surrogate_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -106,9 +115,42 @@ class Plan:
     def execute_async(self):
         self.engine._check(lib().ccr_hip_execute_async(self.engine.ctx, self.handle))
 
+    def execute_surrogate(self, surrogate: "Surrogate", rows: bool = False):
+        info = self.info()
+        outs, arr = abi.make_outs(len(self.spec.sets), info.max_reporting_dates, self.spec.num_paths, rows)
+        self.engine._check(lib().ccr_hip_execute_surrogate(self.engine.ctx, self.handle, surrogate.handle, arr))
+        abi.sync_outs(outs, arr)
+        return outs
+
+    def execute_surrogate_async(self, surrogate: "Surrogate"):
+        self.engine._check(lib().ccr_hip_execute_surrogate_async(self.engine.ctx, self.handle, surrogate.handle))
+
     def close(self):
         if self.handle:
             lib().ccr_hip_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Surrogate:
+    """A trained exposure surrogate (per netting set with Bermudans, gated on hold-out paths)."""
+
+    def __init__(self, handle: C.c_void_p, report: abi.SurrogateReportC):
+        self.handle, self.report = handle, report
+
+    def set_info(self, s: int):
+        en, err = C.c_int32(), C.c_double()
+        lib().ccr_hip_surrogate_set_info(self.handle, s, C.byref(en), C.byref(err))
+        return en.value, err.value
+
+    def close(self):
+        if self.handle:
+            lib().ccr_hip_surrogate_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -161,15 +203,25 @@ class Engine:
 
     # ------------------------------------------------------------ MLP surrogate
 
+    def surrogate_train(self, train: Plan, holdout: Plan, train_paths: int = 2000, hidden: int = 128,
+                        hidden_layers: int = 3, epochs: int = 300, lr: float = 0.003, wd: float = 1e-6,
+                        seed: int = 1, gate: float = 0.005, batch: int = 0) -> Surrogate:
+        o = abi.SurrogateOptionsC(train_paths, hidden, hidden_layers, epochs, lr, wd, seed, gate, batch)
+        h, rep = C.c_void_p(), abi.SurrogateReportC()
+        self._check(lib().ccr_hip_surrogate_train(self.ctx, train.handle, holdout.handle, C.byref(o), C.byref(h),
+                                                  C.byref(rep)))
+        return Surrogate(h, rep)
+
     def mlp_train(self, desc: abi.MlpDescC, X: np.ndarray, Y: np.ndarray, epochs: int, lr: float = 0.01,
-                  wd: float = 1e-6, seed: int = 1, precision: int = abi.CCR_FP32, params=None, stdz=None):
+                  wd: float = 1e-6, seed: int = 1, precision: int = abi.CCR_FP32, params=None, stdz=None,
+                  batch: int = 0):
         """fitMlp on the GPU; returns (params, stdz, loss, train_ms). params/stdz given: warm start."""
         X = np.ascontiguousarray(X, np.float64)
         Y = np.ascontiguousarray(Y, np.float64)
         warm = params is not None
         params = np.zeros(abi.mlp_param_count(desc)) if params is None else np.array(params, np.float64)
         stdz = np.zeros(2 * (desc.inputs + desc.outputs)) if stdz is None else np.array(stdz, np.float64)
-        o = abi.MlpTrainOptionsC(epochs, lr, wd, seed, precision, 1 if warm else 0)
+        o = abi.MlpTrainOptionsC(epochs, lr, wd, seed, precision, 1 if warm else 0, batch)
         loss, ms = C.c_double(), C.c_double()
         self._check(lib().ccr_hip_mlp_train(self.ctx, C.byref(desc), C.byref(o), abi.dptr(X), abi.dptr(Y), X.shape[0],
                                             abi.dptr(params), abi.dptr(stdz), C.byref(loss), C.byref(ms)))

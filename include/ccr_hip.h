@@ -398,6 +398,25 @@ int32_t ccr_hip_run_exposure(ccr_hip_ctx* ctx, const ccr_scenario_generator* gen
                              const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
                              int32_t precision, ccr_exposure_out* outs);
 
+/* BermudanSwaption::valueWithExercise (src/instruments/bermudan_swaption.cpp:49-146) for one
+ * Bermudan on its own scenario set: the simulation grid is `grid` merged with the trade's event
+ * dates (as ExposureEngine::simulationGrid does), the regressions run on the GPU in `precision`.
+ * Writes the price (mean of the time-0 value, times the position) and, if exercise_index is not
+ * NULL, each path's first exercise index (-1: never exercised). */
+int32_t ccr_hip_bermudan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                         const double* grid, int32_t grid_size, const ccr_trade* trade, int32_t precision,
+                         double* price, int32_t* exercise_index);
+
+/* ------------------------------------------------------------------ profile statistics (FP64, host) */
+
+/* ExposureProfile::expectedPositiveExposure (src/exposure/exposure_profile.cpp:52-65): the
+ * right-point time average of EE over (0, horizon]. */
+double ccr_expected_positive_exposure(const double* times, const double* ee, int32_t size, double horizon);
+/* ExposureProfile::effectiveExpectedExposure (:67-75): the running maximum of EE, into eff[size]. */
+void ccr_effective_expected_exposure(const double* ee, int32_t size, double* eff);
+/* ExposureProfile::effectiveExpectedPositiveExposure (:77-79). */
+double ccr_effective_expected_positive_exposure(const double* times, const double* ee, int32_t size, double horizon);
+
 /* ------------------------------------------------------------------ CVA (FP64, host) */
 
 /* unilateralCva (cva.cpp:22-33) and bilateralCva (cva.cpp:34-48) on a profile. */

@@ -2,6 +2,7 @@
 device-free entry points (grids, CVA integrals, argument validation) agree with the oracle."""
 import ctypes as C
 import subprocess
+from pathlib import Path
 
 import numpy as np
 
@@ -55,6 +56,24 @@ def test_cva_integrals_match_oracle():
     assert abs(c1.value - c2.value) <= 1e-15 * abs(c2.value) and abs(d1.value - d2.value) <= 1e-15 * abs(d2.value)
 
 
+def test_epe_and_effective_ee_match_oracle():
+    lib = native.lib()
+    t = np.array([0.0, 0.25, 0.5, 1.0, 1.5, 2.0, 3.0, 5.0])
+    ee = np.array([3.0, 5.0, 4.0, 7.0, 6.5, 2.0, 8.0, 1.0])
+    for h in (0.0, 0.4, 1.0, 2.0, 10.0):
+        a = lib.ccr_expected_positive_exposure(abi.dptr(t), abi.dptr(ee), 8, h)
+        b = oracle_lib.oracle().ccr_oracle_epe(abi.dptr(t), abi.dptr(ee), 8, h)
+        assert a == b, h
+        a = lib.ccr_effective_expected_positive_exposure(abi.dptr(t), abi.dptr(ee), 8, h)
+        b = oracle_lib.oracle().ccr_oracle_effective_epe(abi.dptr(t), abi.dptr(ee), 8, h)
+        assert a == b, h
+    eff = np.zeros(8)
+    lib.ccr_effective_expected_exposure(abi.dptr(ee), 8, abi.dptr(eff))
+    assert np.array_equal(eff, np.maximum.accumulate(ee))
+    # Hand value: EPE over (0, 1] = (5 * 0.25 + 4 * 0.25 + 7 * 0.5) / 1.
+    assert abs(lib.ccr_expected_positive_exposure(abi.dptr(t), abi.dptr(ee), 8, 1.0) - 5.75) < 1e-15
+
+
 def test_create_reports_errors_without_crashing():
     lib = native.lib()
     ctx = C.c_void_p()
@@ -63,3 +82,19 @@ def test_create_reports_errors_without_crashing():
         assert lib.ccr_hip_create(C.byref(bad), C.byref(ctx)) == abi.CCR_ERR_INVALID_ARGUMENT
         assert b"multiple of 64" in lib.ccr_hip_last_error(None)
     assert lib.ccr_hip_set_fault(None, 1) == abi.CCR_ERR_INVALID_ARGUMENT
+
+
+def build_wrapper_smoke(tmp_path):
+    """Compiles tests/cpp/wrapper_smoke.cpp (the C++ wrapper, include/ccr_hip.hpp) against the library."""
+    root = Path(__file__).resolve().parents[1]
+    exe = tmp_path / "wrapper_smoke"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", f"-I{root / 'include'}",
+                    str(root / "tests" / "cpp" / "wrapper_smoke.cpp"), f"-L{root / 'zlatko_amd'}", "-lccr_hip",
+                    f"-Wl,-rpath,{root / 'zlatko_amd'}", "-o", str(exe)], check=True)
+    return exe
+
+
+def test_cpp_wrapper_compiles_and_maps_errors_to_exceptions(tmp_path):
+    out = subprocess.run([str(build_wrapper_smoke(tmp_path))], capture_output=True, text=True, check=True).stdout
+    # Without a GPU the context constructor throws std::runtime_error with the library's message.
+    assert out.startswith("error: no HIP device") or out.splitlines()[-1].startswith("EPE")

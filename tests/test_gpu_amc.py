@@ -45,6 +45,20 @@ def test_bermudan_golden_vectors_fp64(engine):
         assert err <= FP64_TOL, f
 
 
+def test_bermudan_price_and_exercise_indices_match_reference(engine):
+    """ccr_hip_bermudan (valueWithExercise) against the reference's prices and per-path exercise
+    indices for three Bermudans on 400 paths."""
+    g = GOLDEN["bermudan"]
+    spec = bermudan_case(g["num_paths"])
+    grid = np.array(g["grid"])
+    for t, ref in enumerate(g["trades"]):
+        price, ex = engine.bermudan(spec, 0, t, grid)
+        rel = abs(price - ref["price"]) / abs(ref["price"])
+        flips = int(np.sum(ex != np.array(ref["exercise_index"])))
+        print("bermudan", t, f"price {price:.10e} ref {ref['price']:.10e} rel {rel:.2e} flips {flips}", flush=True)
+        assert rel <= FP64_TOL and flips == 0
+
+
 def test_c3_reduced_fp64(engine):
     """Config 3 shape (Bermudans 1-10y into 5-10y, annual exercise, degree 3), fewer paths."""
     spec = W.c3(num_paths=4000, dates=60, num_swaptions=40)

@@ -94,6 +94,33 @@ def test_fp32_error_vs_fp64_oracle(engine):
         assert errs["expected_exposure"] < 1e-4 and errs["discounted_expected_exposure"] < 1e-4
 
 
+def test_cpp_wrapper_run_matches_oracle(tmp_path):
+    """ccr::hip::ExposureEngine::run (include/ccr_hip.hpp) through tests/cpp/wrapper_smoke.cpp:
+    EE and EPE against the FP64 oracle on the same inputs."""
+    import subprocess
+    from tests.test_abi import build_wrapper_smoke
+    out = subprocess.run([str(build_wrapper_smoke(tmp_path))], capture_output=True, text=True, check=True).stdout
+    lines = out.splitlines()
+    assert not out.startswith("error"), out
+    t = np.array([float(l.split()[0]) for l in lines[:-1]])
+    ee = np.array([float(l.split()[1]) for l in lines[:-1]])
+    epe = float(lines[-1].split()[1])
+    model = abi.Model(abi.HullWhite(abi.Curve([1.0, 3.0, 10.0], [0.02, 0.025, 0.03]), 0.05, 0.01),
+                      assets=[abi.Asset(1.1, 0.12, abi.Curve.flat(0.01))],
+                      correlation=np.array([[1.0, 0.3], [0.3, 1.0]]))
+    trades = [abi.Trade(abi.CCR_TRADE_SWAP, 1e6, 0.025, True, schedule=[0.0, 0.5, 1.0, 1.5, 2.0]),
+              abi.Trade(abi.CCR_TRADE_SWAP, 5e5, 0.03, False, schedule=[0.0, 0.5, 1.0, 1.5, 2.0, 2.5, 3.0]),
+              abi.Trade(abi.CCR_TRADE_ASSET_FORWARD, 4e5, 1.1, asset=0, maturity=2.5)]
+    spec = abi.Spec(model, [abi.NettingSet(trades)], np.array([0.25 * i for i in range(13)]), 2000, 42, True, 0.95,
+                    "wrapper")
+    ref = oracle_lib.run_oracle(spec)[0]
+    assert np.array_equal(t, ref.get("times"))
+    scale = np.abs(ref.get("expected_exposure")).max()
+    assert np.abs(ee - ref.get("expected_exposure")).max() <= FP64_TOL * scale
+    ref_epe = oracle_lib.oracle().ccr_oracle_epe(abi.dptr(t), abi.dptr(ref.get("expected_exposure")), len(t), 3.0)
+    assert abs(epe - ref_epe) <= FP64_TOL * scale
+
+
 def test_fp32_options_error_vs_fp64_oracle(engine):
     """FX options (Black in FP32: logf, erfcf) next to swaps and forwards."""
     spec = W.small_reference_case(num_paths=2000, fx=True, csa=True, credits=True, options=6)

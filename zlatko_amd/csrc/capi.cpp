@@ -688,6 +688,66 @@ int32_t ccr_hip_run_exposure(ccr_hip_ctx* ctx, const ccr_scenario_generator* gen
   });
 }
 
+int32_t ccr_hip_bermudan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                         const double* grid, int32_t grid_size, const ccr_trade* trade, int32_t precision, double* price,
+                         int32_t* exercise_index) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!trade || !price) invalid("null argument");
+    if (trade->kind != CCR_TRADE_BERMUDAN_SWAPTION) invalid("ccr_hip_bermudan: the trade is not a Bermudan swaption");
+    ccr_netting_set ns{};
+    ns.num_trades = 1;
+    ns.trades = trade;
+    ns.counterparty_credit = -1;
+    ns.own_credit = -1;
+    ccr_hip_plan* raw = nullptr;
+    compilePlan(ctx, generator, config, grid, grid_size, &ns, 1, 0.95, precision, &raw);
+    std::unique_ptr<ccr_hip_plan, void (*)(ccr_hip_plan*)> plan(raw, ccr_hip_plan_destroy);
+    launchAll(ctx, plan.get());
+    const HostPlan& h = plan->host;
+    const int64_t P = h.g.num_paths;
+    // Date 0 is always valued (the grid starts at 0): its row is the first continuation row.
+    std::vector<double> v0(static_cast<size_t>(P));
+    check(hipMemcpyAsync(v0.data(), plan->d_bermv, v0.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+          "hipMemcpyAsync");
+    std::vector<int16_t> ex(static_cast<size_t>(P));
+    check(hipMemcpyAsync(ex.data(), plan->d_exidx, ex.size() * sizeof(int16_t), hipMemcpyDeviceToHost, ctx->stream),
+          "hipMemcpyAsync");
+    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    double sum = 0.0;
+    for (double v : v0) sum += v;
+    *price = sum / static_cast<double>(P);
+    if (exercise_index) {
+      const int ne = h.berms[0].ne;
+      for (int64_t p = 0; p < P; ++p) exercise_index[p] = ex[static_cast<size_t>(p)] >= ne ? -1 : ex[static_cast<size_t>(p)];
+    }
+  });
+}
+
+double ccr_expected_positive_exposure(const double* times, const double* ee, int32_t size, double horizon) {
+  // ExposureProfile::timeAverage: src/exposure/exposure_profile.cpp:52-61.
+  double integral = 0.0, end = 0.0;
+  for (int32_t j = 1; j < size && times[j] <= horizon + 1e-10; ++j) {
+    integral += ee[j] * (times[j] - times[j - 1]);
+    end = times[j];
+  }
+  return end > 0.0 ? integral / end : (size > 0 ? ee[0] : 0.0);
+}
+
+void ccr_effective_expected_exposure(const double* ee, int32_t size, double* eff) {
+  double running = 0.0;
+  for (int32_t j = 0; j < size; ++j) {
+    running = std::max(running, ee[j]);
+    eff[j] = running;
+  }
+}
+
+double ccr_effective_expected_positive_exposure(const double* times, const double* ee, int32_t size, double horizon) {
+  std::vector<double> eff(static_cast<size_t>(std::max(size, 0)));
+  ccr_effective_expected_exposure(ee, size, eff.data());
+  return ccr_expected_positive_exposure(times, eff.data(), size, horizon);
+}
+
 double ccr_unilateral_cva(const double* times, const double* discounted_ee, int32_t size,
                           const ccr_credit_curve* counterparty, double recovery) {
   // unilateralCva: src/cva/cva.cpp:11-33 (trapezoid in EE* against the default probabilities).

@@ -75,6 +75,13 @@ def lib():
         L.ccr_hip_execute_surrogate_async.restype = I32
         L.ccr_hip_execute_surrogate_async.argtypes = [P, P, P]
         L.ccr_hip_surrogate_destroy.argtypes = [P]
+        L.ccr_hip_bermudan.restype = I32
+        L.ccr_hip_bermudan.argtypes = [P, P, P, P, I32, P, I32, P, P]
+        L.ccr_expected_positive_exposure.restype = D
+        L.ccr_expected_positive_exposure.argtypes = [P, P, I32, D]
+        L.ccr_effective_expected_exposure.argtypes = [P, I32, P]
+        L.ccr_effective_expected_positive_exposure.restype = D
+        L.ccr_effective_expected_positive_exposure.argtypes = [P, P, I32, D]
         _lib = L
     return _lib
 
@@ -200,6 +207,18 @@ class Engine:
 
     def set_fault(self, kind: int):
         self._check(lib().ccr_hip_set_fault(self.ctx, kind))
+
+    def bermudan(self, spec: abi.Spec, set_index: int, trade_index: int, grid: np.ndarray,
+                 precision: int = abi.CCR_FP64):
+        """BermudanSwaption::valueWithExercise on the GPU: (price, exercise index per path, -1 never)."""
+        grid = np.ascontiguousarray(grid, np.float64)
+        price = C.c_double()
+        ex = np.zeros(spec.num_paths, np.int32)
+        trade = spec.sets_c[set_index].trades[trade_index]
+        self._check(lib().ccr_hip_bermudan(self.ctx, C.byref(spec.generator), C.byref(spec.config), abi.dptr(grid),
+                                           len(grid), C.byref(trade), precision, C.byref(price),
+                                           ex.ctypes.data_as(C.c_void_p)))
+        return price.value, ex
 
     # ------------------------------------------------------------ MLP surrogate
 

@@ -36,12 +36,29 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() else \
     "trade·path·date revals/sec"
+# Workload name -> (spec builder taking a path count, default path count).
+WORKLOADS = {
+    "c1": (lambda p: W.c1(num_paths=p), 5000),
+    "c2": (lambda p: W.c2(num_paths=p), 50000),
+    "c3": (lambda p: W.c3(num_paths=p), 100000),
+    "c4": (lambda p: W.c4(num_paths=p), 100000),
+}
+
+
+WORKLOAD_TEXT = {
+    "c1": "c1: 10 two-leg IRS (semiannual fixed, quarterly float), 1-factor Hull-White, 5,000 paths x 120 monthly dates",
+    "c2": "c2: 3-ccy Hull-White + 2 FX, 1000 trades (700 IRS, 300 FX forwards), 20 CSA netting sets, "
+          "50,000 paths x 260 weekly dates",
+    "c3": "c3: 200 Bermudan swaptions (1-10y into 5-10y, annual exercise, degree-3 regressions), 1-factor "
+          "Hull-White, 100,000 paths x 120 dates",
+    "c4": "c4: 20 two-leg IRS + 10 FX forwards + 10 Bermudans, HW + FX + CIR++ counterparty / own intensities "
+          "(0.3 wrong-way), 100,000 paths x 120 dates",
+}
 
 
 def describe(spec: abi.Spec, info) -> dict:
     return {
-        "workload": "c2: 3-ccy Hull-White + 2 FX, 1000 trades (700 IRS, 300 FX forwards), 20 CSA netting sets, "
-                    "50,000 paths x 260 weekly dates" if spec.name == "c2" else spec.name,
+        "workload": WORKLOAD_TEXT.get(spec.name, spec.name),
         "paths": int(info.num_paths),
         "netting_sets": int(info.num_sets),
         "trades": int(sum(len(s.trades) for s in spec.sets)),
@@ -102,7 +119,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--paths", type=int, default=50000)
+    ap.add_argument("--paths", type=int, default=0, help="0: the workload's own path count")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2",
+                    help="c2 (default, the metric's workload); c1, c3 (Bermudan AMC), c4 (credit + Bermudans)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -112,7 +131,9 @@ def main():
               "(the path-sharded multi-GPU design is described, not built)", file=sys.stderr)
         sys.exit(2)
 
-    spec = W.c2(num_paths=args.paths)
+    make, default_paths = WORKLOADS[args.workload]
+    spec_fn = lambda p: make(p)  # noqa: E731
+    spec = spec_fn(args.paths or default_paths)
     prec = abi.CCR_FP32 if args.precision == "fp32" else abi.CCR_FP64
     engine = native.Engine()
     plan = engine.compile(spec, prec)
@@ -142,7 +163,8 @@ def main():
         "dtype": "f64" if prec == abi.CCR_FP64 else "f32",
         "data": "synthetic (seeded SplitMix64 portfolio, replayed std::mt19937_64 normal stream)",
         "config": describe(spec, info),
-        "kernel_ms": {"path": path_ms, "total": total_ms, "reduce": last.reduce_ms, "pfe": last.pfe_ms},
+        "kernel_ms": {"path": path_ms, "total": total_ms, "amc": last.amc_ms, "reduce": last.reduce_ms,
+                      "pfe": last.pfe_ms},
         "host_ms": {"draws": last.draws_ms, "upload": last.upload_ms},
         "roofline": {
             "bound": "hbm",
@@ -168,7 +190,7 @@ def main():
                           "path_kernel_ms": pm32, "ee_rel_err_vs_fp64": ee_err}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        result["cpu_baseline"] = cpu_baseline(lambda p: W.c2(num_paths=p), args.cpu_seconds, threads)
+        result["cpu_baseline"] = cpu_baseline(spec_fn, args.cpu_seconds, threads)
     engine.close()
     print(json.dumps(result), flush=True)
 

@@ -299,6 +299,7 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   a.nrates = h.g.nrates;
   a.nassets = h.g.nassets;
   a.ncredits = h.g.ncredits;
+  a.has_options = h.opts.empty() ? 0 : 1;
   a.state_len = h.state_len;
   return a;
 }

@@ -27,7 +27,7 @@ __device__ inline int xcdRemap(int b, int nb) {
   return xcd < rem ? xcd * (per + 1) + i : rem * (per + 1) + (xcd - rem) * per + i;
 }
 
-template <typename Real, typename Sh>
+template <typename Real, typename Sh, bool kOptions>
 __global__ void __launch_bounds__(kMaxBlock) ccr_path_kernel(PathArgs<Real> a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const CCR_CONST GlobalDev& g = *toConst(a.g);
@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(kMaxBlock) ccr_path_kernel(PathArgs<Real> a) {
   double* part = a.partials + S.partial_off + static_cast<int64_t>(waveLocal) * a.max_rep * kStats;
   Real* rows = a.rows + S.rows_off;
   Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
-  simulatePath<Real, Sh>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
+  simulatePath<Real, Sh, false, kOptions>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
     if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
 #pragma unroll
     for (int k = 0; k < kStats; ++k) {
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(kMaxBlock) ccr_states_kernel(PathArgs<Real> a)
   const int paths = toConst(a.g)->num_paths;
   const int pe = p < paths ? p : paths - 1;
   Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
-  simulatePath<Real, Sh, true>(a, sp, pe, col, bd, [](int, const double*, double) {});
+  simulatePath<Real, Sh, true, false>(a, sp, pe, col, bd, [](int, const double*, double) {});
 }
 
 // Sum of the per-wave partials of every (set, reporting date, statistic), in wave order.
@@ -295,11 +295,18 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
 
 template <typename Real, typename Sh>
 hipError_t launchShape(const PathArgs<Real>& args, int blocks, int block, size_t lds, hipStream_t stream) {
-  hipLaunchKernelGGL((ccr_path_kernel<Real, Sh>), dim3(blocks), dim3(block), lds, stream, args);
+  if constexpr (Sh::NA > 0) {
+    if (args.has_options) {
+      hipLaunchKernelGGL((ccr_path_kernel<Real, Sh, true>), dim3(blocks), dim3(block), lds, stream, args);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((ccr_path_kernel<Real, Sh, false>), dim3(blocks), dim3(block), lds, stream, args);
   return hipGetLastError();
 }
 
-// Picks the path-kernel specialisation for the model shape (rates, assets, credits).
+// Picks the path-kernel specialisation for the model shape (rates, assets, credits) and for the
+// presence of European options.
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
   const size_t lds = static_cast<size_t>(args.ring + args.price_rows + args.slots) * block * sizeof(Real);

@@ -46,6 +46,7 @@ struct PathArgs {
   int32_t price_rows;  // LDS rows of the collateral price ring (0 for cash collateral)
   int32_t slots;       // LDS rows of fixing states
   int32_t nrates, nassets, ncredits;  // model shape: selects the kernel specialisation
+  int32_t has_options;                // European options present: selects the kernel variant
   int64_t state_len;                  // elements of x in `states` (D follows)
 };
 

@@ -202,7 +202,9 @@ inline auto withShape(int nrates, int nassets, int ncredits, F&& f) {
 // then the fixing states. emit(rep, stats[kStats], positive_exposure) is called on every
 // reporting date.
 // kStatesOnly: the AMC states pass, which only calls onState(j, x, D) on every date.
-template <typename Real, typename Sh, bool kStatesOnly, typename Emit, typename OnState>
+// kOptions: the plan holds European options (the Black evaluation is compiled in only then, so
+// option-free plans keep a compact loop).
+template <typename Real, typename Sh, bool kStatesOnly, bool kOptions, typename Emit, typename OnState>
 __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
@@ -414,7 +416,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 #pragma unroll
       for (int i = 0; i < NA; ++i)
         if (i < nA) V += S_[i] * dr[kDateLin + i];
-      if constexpr (NA > 0) {
+      if constexpr (NA > 0 && kOptions) {
         for (int i = rec.opt_beg; i < rec.opt_end; ++i) {
           cptr<Real> o = opts + kOptStride * static_cast<int64_t>(i);
           const int oa = static_cast<int>(o[0]);
@@ -494,7 +496,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 // Convenience: the tables of set S for path pe, from a PathArgs bundle. kStatesOnly: the AMC
 // states pass, writing x and D of every date into a.states ([state_off + j P + path], D after
 // a.state_len elements).
-template <typename Real, typename Sh, bool kStatesOnly = false, typename Emit>
+template <typename Real, typename Sh, bool kStatesOnly = false, bool kOptions = true, typename Emit>
 __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, Real* col, int stride,
                                              Emit&& emit) {
   const CCR_CONST SetDev& S = *sp;
@@ -510,7 +512,7 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDe
       states[stateLen + at] = static_cast<double>(D);
     }
   };
-  simulatePath<Real, Sh, kStatesOnly>(g, sp, pe, a.stream + q * S.stream_stride,
+  simulatePath<Real, Sh, kStatesOnly, kOptions>(g, sp, pe, a.stream + q * S.stream_stride,
                                       toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride,
                                       toConst(a.dates) + S.date_off,
                                       toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,

@@ -143,10 +143,11 @@ def main():
     value = info.revaluations / (wall / args.steps)
 
     elem = 4 if prec == abi.CCR_FP32 else 8
-    # Path kernel algorithmic bytes: the replayed stream read once + max(V - C, 0) rows written once.
+    # Path kernel algorithmic bytes: every netting set's slice of the replayed normal stream read
+    # once (P/2 x (T_s - 1) x nz per set) + max(V - C, 0) rows written once.
     out = plan.execute()
     rows_elems = sum(o.c.num_dates for o in out) * info.num_paths
-    path_bytes = info.stream_normals * elem + rows_elems * elem
+    path_bytes = info.normals_read * elem + rows_elems * elem
     achieved_gbs = path_bytes / (path_ms * 1e-3) / 1e9
 
     result = {
@@ -173,6 +174,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": None,
+            "algorithmic_bytes": path_bytes,
         },
     }
     plan.close()

@@ -249,13 +249,16 @@ typedef struct ccr_hip_plan_info {
   int64_t stream_normals;     /* length of the replayed normal stream */
   int64_t revaluations;       /* sum over sets of trades x paths x reporting dates */
   int64_t term_evaluations;   /* sum over sets, paths and valued dates of the terms evaluated */
-  int64_t hbm_bytes;          /* algorithmic HBM bytes of one execute */
+  int64_t hbm_bytes;          /* algorithmic HBM bytes of one execute: normals replayed by every
+                                 set + exposure rows written and read back by the PFE select */
   int32_t precision;
   int32_t block_size;
   int32_t num_bermudans;      /* Bermudan swaptions valued by American Monte Carlo */
   int32_t num_exercises;      /* their exercise dates, summed */
   int32_t num_continuation_dates; /* (set, date) pairs with continuation regressions */
   int32_t max_exercises;
+  int64_t normals_read;       /* normals the path kernel reads: every set replays its own slice of
+                                 the stream, (paths / 2) x (T_s - 1) x nz per set with antithetics */
 } ccr_hip_plan_info;
 
 /* ------------------------------------------------------------------ MLP surrogate */

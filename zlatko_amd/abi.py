@@ -118,7 +118,7 @@ class HipPlanInfoC(C.Structure):
                 ("stream_normals", C.c_int64), ("revaluations", C.c_int64), ("term_evaluations", C.c_int64),
                 ("hbm_bytes", C.c_int64), ("precision", C.c_int32), ("block_size", C.c_int32),
                 ("num_bermudans", C.c_int32), ("num_exercises", C.c_int32), ("num_continuation_dates", C.c_int32),
-                ("max_exercises", C.c_int32)]
+                ("max_exercises", C.c_int32), ("normals_read", C.c_int64)]
 
 
 CCR_BF16 = 2

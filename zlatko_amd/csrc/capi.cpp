@@ -623,10 +623,15 @@ int32_t ccr_hip_plan_info_get(const ccr_hip_plan* plan, ccr_hip_plan_info* info)
   info->revaluations = h.revaluations;
   info->term_evaluations = h.term_evaluations;
   const int64_t elem = h.precision == CCR_FP32 ? 4 : 8;
-  int64_t rows = 0;
-  for (const auto& s : h.sets) rows += static_cast<int64_t>(s.num_rep) * h.g.num_paths;
-  // Replayed stream read once, exposure rows written once and read by the PFE select.
-  info->hbm_bytes = h.stream_len * elem + 2 * rows * elem;
+  int64_t rows = 0, normals = 0;
+  const int64_t blocks = h.g.antithetic ? (static_cast<int64_t>(h.g.num_paths) + 1) / 2 : h.g.num_paths;
+  for (const auto& s : h.sets) {
+    rows += static_cast<int64_t>(s.num_rep) * h.g.num_paths;
+    normals += blocks * s.stream_stride;  // each set replays its own slice of the stream
+  }
+  // Normals read by the path kernel, exposure rows written once and read by the PFE select.
+  info->normals_read = normals;
+  info->hbm_bytes = normals * elem + 2 * rows * elem;
   info->precision = h.precision;
   info->block_size = h.block;
   info->num_bermudans = static_cast<int32_t>(h.berms.size());

@@ -45,10 +45,21 @@ __global__ void __launch_bounds__(kMaxBlock) ccr_path_kernel(PathArgs<Real> a) {
   Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
   simulatePath<Real, Sh, false, kOptions>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
     if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
+    // Without intensities E[Q] is the count of active lanes and the CVA / DVA increments are 0:
+    // only the first six statistics need wavefront sums.
+    constexpr int kSummed = (Sh::kExact && Sh::NC == 0) ? 6 : kStats;
 #pragma unroll
-    for (int k = 0; k < kStats; ++k) {
+    for (int k = 0; k < kSummed; ++k) {
       const double s = waveSum(active ? st9[k] : 0.0);
       if ((tid & 63) == 0) part[rr * kStats + k] = s;
+    }
+    if constexpr (kSummed < kStats) {
+      const double lanes = static_cast<double>(__popcll(__ballot(active)));
+      if ((tid & 63) == 0) {
+        part[rr * kStats + 6] = lanes;
+        part[rr * kStats + 7] = 0.0;
+        part[rr * kStats + 8] = 0.0;
+      }
     }
   });
 }

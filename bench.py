@@ -42,6 +42,7 @@ WORKLOADS = {
     "c2": (lambda p: W.c2(num_paths=p), 50000),
     "c3": (lambda p: W.c3(num_paths=p), 100000),
     "c4": (lambda p: W.c4(num_paths=p), 100000),
+    "c5": (lambda p: W.c5(num_paths=p), 20000),
 }
 
 
@@ -53,6 +54,9 @@ WORKLOAD_TEXT = {
           "Hull-White, 100,000 paths x 120 dates",
     "c4": "c4: 20 two-leg IRS + 10 FX forwards + 10 Bermudans, HW + FX + CIR++ counterparty / own intensities "
           "(0.3 wrong-way), 100,000 paths x 120 dates",
+    "c5": "c5: 500 netting sets, Zipf(1.2) trade counts 1..5000 (20,744 trades: 3-ccy IRS, FX forwards, 1,009 "
+          "Bermudans), half under CSA, 3-ccy HW + 2 FX + shared CIR++, 120 dates (paths reduced to fit one "
+          "plan in HBM)",
 }
 
 
@@ -107,7 +111,7 @@ def cpu_baseline(spec_fn, budget_s: float, threads: int):
         "unit": "revals/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP64 oracle (ExposureEngine restated), same 20 netting sets and grids, {paths} paths, "
+        "sample": f"FP64 oracle (ExposureEngine restated), same {len(spec.sets)} netting sets and grids, {paths} paths, "
                   f"{wall:.1f} s wall (draws {tm.draws_s:.2f} s, pricing {tm.pricing_s:.1f} s, "
                   f"collateral {tm.collateral_s:.2f} s, statistics {tm.statistics_s:.2f} s)",
     }
@@ -121,7 +125,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--paths", type=int, default=0, help="0: the workload's own path count")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2",
-                    help="c2 (default, the metric's workload); c1, c3 (Bermudan AMC), c4 (credit + Bermudans)")
+                    help="c2 (default, the metric's workload); c1, c3 (Bermudan AMC), c4 (credit + Bermudans), c5 (500 uneven sets)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")

@@ -401,6 +401,14 @@ int32_t ccr_hip_run_exposure(ccr_hip_ctx* ctx, const ccr_scenario_generator* gen
                              const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
                              int32_t precision, ccr_exposure_out* outs);
 
+/* Autotuner: compiles the workload with min(config->num_paths, sample_paths) paths for every
+ * workgroup size of the path kernel (64, 128, 192, 256), times each execute (best of 3, HIP
+ * events) and makes the fastest the context's workgroup size for later compiles. Writes the
+ * chosen size and the four path-kernel times (ms). Results do not depend on the choice. */
+int32_t ccr_hip_autotune(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                         const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
+                         int32_t precision, int64_t sample_paths, int32_t* block_size, double* times_ms);
+
 /* BermudanSwaption::valueWithExercise (src/instruments/bermudan_swaption.cpp:49-146) for one
  * Bermudan on its own scenario set: the simulation grid is `grid` merged with the trade's event
  * dates (as ExposureEngine::simulationGrid does), the regressions run on the GPU in `precision`.

@@ -83,6 +83,32 @@ def test_c4_plain_credit_fp64(engine):
     assert outs[0].get("cva_increments")[1:].sum() > 0
 
 
+def test_c5_uneven_portfolio_fp64(engine):
+    """Config 5 shape: 500 netting sets with Zipf(1.2) trade counts (1 .. 5,000; ~20,700 trades:
+    IRS in 3 currencies, FX forwards, ~1,000 Bermudans), half under CSA, shared CIR++ credit, all
+    in one plan and one launch per stage; every set against the oracle."""
+    spec = W.c5(num_paths=64)
+    outs = engine.run(spec)
+    ref = oracle_lib.run_oracle(spec, threads=16)
+    worst = max(compare.worst(compare.field_errors(a, b)) for a, b in zip(outs, ref))
+    print("c5", len(spec.sets), "sets, worst field error", f"{worst:.2e}", flush=True)
+    assert worst <= FP64_TOL
+
+
+def test_autotuner_picks_a_size_and_keeps_results(engine):
+    from zlatko_amd import native
+    spec = W.c2(num_paths=4000, num_sets=4, num_trades=80, weeks=52, horizon=1.0)
+    e = native.Engine()
+    before = e.run(spec)[0].as_dict()
+    block, times = e.autotune(spec, sample_paths=4000)
+    print("autotune", block, times, flush=True)
+    assert block in (64, 128, 192, 256) and np.all(times > 0)
+    after = e.run(spec)[0].as_dict()
+    e.close()
+    for k in abi.PROFILE_FIELDS:
+        assert np.array_equal(before[k], after[k]), k
+
+
 def test_fp32_error_vs_fp64_oracle(engine):
     spec = W.c2(num_paths=1000, num_sets=2, num_trades=40, weeks=52, horizon=1.0)
     ref = oracle_lib.run_oracle(spec, threads=8)

@@ -49,6 +49,7 @@ CASES = {
     "c3_small": lambda: W.c3(num_paths=96, dates=30, num_swaptions=12),
     "c4_bermudan": lambda: W.c4(num_paths=64, dates=24),
     "options": lambda: W.small_reference_case(120, fx=True, credits=True, options=6, bermudans=2),
+    "c5_uneven": lambda: W.c5(num_paths=16),
 }
 
 

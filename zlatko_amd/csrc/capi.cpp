@@ -694,6 +694,42 @@ int32_t ccr_hip_run_exposure(ccr_hip_ctx* ctx, const ccr_scenario_generator* gen
   });
 }
 
+int32_t ccr_hip_autotune(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                         const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
+                         int32_t precision, int64_t sample_paths, int32_t* block_size, double* times_ms) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  const int keep = ctx->block;
+  const int32_t rc = guarded(ctx, [&] {
+    if (!config || sample_paths <= 0) invalid("ccr_hip_autotune: invalid arguments");
+    ccr_simulation_config sample = *config;
+    sample.num_paths = std::min<int64_t>(config->num_paths, sample_paths);
+    const int candidates[4] = {64, 128, 192, 256};
+    int best = keep;
+    double bestMs = 1e300;
+    for (int i = 0; i < 4; ++i) {
+      ctx->block = candidates[i];
+      ccr_hip_plan* raw = nullptr;
+      compilePlan(ctx, generator, &sample, base_grid, base_size, sets, num_sets, 0.95, precision, &raw);
+      std::unique_ptr<ccr_hip_plan, void (*)(ccr_hip_plan*)> plan(raw, ccr_hip_plan_destroy);
+      double ms = 1e300;
+      for (int rep = 0; rep < 4; ++rep) {
+        launchAll(ctx, plan.get());
+        readTimings(ctx);
+        if (rep > 0) ms = std::min(ms, ctx->timing.path_ms);  // the first execute warms up
+      }
+      if (times_ms) times_ms[i] = ms;
+      if (ms < bestMs) {
+        bestMs = ms;
+        best = candidates[i];
+      }
+    }
+    ctx->block = best;
+    if (block_size) *block_size = best;
+  });
+  if (rc != CCR_OK) ctx->block = keep;
+  return rc;
+}
+
 int32_t ccr_hip_bermudan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
                          const double* grid, int32_t grid_size, const ccr_trade* trade, int32_t precision, double* price,
                          int32_t* exercise_index) {

@@ -75,6 +75,8 @@ def lib():
         L.ccr_hip_execute_surrogate_async.restype = I32
         L.ccr_hip_execute_surrogate_async.argtypes = [P, P, P]
         L.ccr_hip_surrogate_destroy.argtypes = [P]
+        L.ccr_hip_autotune.restype = I32
+        L.ccr_hip_autotune.argtypes = [P, P, P, P, I32, P, I32, I32, I64, P, P]
         L.ccr_hip_bermudan.restype = I32
         L.ccr_hip_bermudan.argtypes = [P, P, P, P, I32, P, I32, P, P]
         L.ccr_expected_positive_exposure.restype = D
@@ -207,6 +209,15 @@ class Engine:
 
     def set_fault(self, kind: int):
         self._check(lib().ccr_hip_set_fault(self.ctx, kind))
+
+    def autotune(self, spec: abi.Spec, precision: int = abi.CCR_FP64, sample_paths: int = 16384):
+        """Picks the path kernel's workgroup size for this workload; returns (size, times in ms)."""
+        block = C.c_int32()
+        times = np.zeros(4)
+        self._check(lib().ccr_hip_autotune(self.ctx, C.byref(spec.generator), C.byref(spec.config),
+                                           abi.dptr(spec.base_grid), len(spec.base_grid), spec.sets_c,
+                                           len(spec.sets), precision, sample_paths, C.byref(block), abi.dptr(times)))
+        return block.value, times
 
     def bermudan(self, spec: abi.Spec, set_index: int, trade_index: int, grid: np.ndarray,
                  precision: int = abi.CCR_FP64):

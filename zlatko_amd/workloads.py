@@ -163,6 +163,46 @@ def c4(num_paths: int = 100000, dates: int = 120, seed: int = 42, with_bermudans
     return Spec(model, [ns], uniform_grid(horizon, dates), num_paths, seed, True, 0.975, "c4")
 
 
+def c5(num_paths: int = 20000, dates: int = 120, num_sets: int = 500, largest: int = 5000, seed: int = 42,
+       bermudan_share: float = 0.05) -> Spec:
+    """Batched uneven portfolio: num_sets netting sets with round(largest / rank^1.2) trades each
+    (Zipf s = 1.2, clipped to [1, largest]; ~20,000 trades at 500 sets), a mix of two-leg IRS in
+    3 currencies, FX forwards and Bermudan swaptions, every other set under CSA; 3-currency
+    Hull-White + 2 FX + one CIR++ intensity shared by the counterparties."""
+    rng = SplitMix64(5)
+    base = c2_model(rng)
+    corr = np.eye(6)
+    corr[:5, :5] = base.correlation
+    cir = Cir(CreditCurve.flat(0.02), 0.5, 0.02, 0.1, 0.02, 4)
+    model = Model(base.rates, base.foreign, base.assets, [cir], corr)
+    sets = []
+    for i in range(1, num_sets + 1):
+        n = max(1, min(largest, int(round(largest / i ** 1.2))))
+        ts: List[Trade] = []
+        for _ in range(n):
+            u = rng.uniform()
+            if u < bermudan_share:
+                ts.append(bermudan(rng))
+            elif u < bermudan_share + 0.3:
+                a = rng.integer(0, 1)
+                notional = rng.uniform(1e6, 1e8) * (1.0 if rng.bernoulli() else -1.0)
+                mat = 0.25 * rng.integer(1, 40)
+                fwd = model.assets[a].spot * math.exp((model.rates.curve.rates[0] - model.assets[a].carry.rates[0]) * mat)
+                ts.append(Trade(CCR_TRADE_ASSET_FORWARD, notional, fwd * rng.uniform(0.9, 1.1), asset=a, maturity=mat))
+            else:
+                mat = 0.25 * rng.integer(2, 40)
+                ts.append(Trade(CCR_TRADE_SWAP, rng.uniform(1e6, 1e8), rng.uniform(0.02, 0.04), rng.bernoulli(),
+                                schedule=swap_schedule(0.0, mat, 2), float_schedule=swap_schedule(0.0, mat, 4),
+                                currency=rng.integer(0, 2)))
+        ns = NettingSet(ts, counterparty_credit=0)
+        if i % 2 == 0:
+            th = rng.uniform(0.0, 1e7)
+            ns.collateral = Collateral(threshold_counterparty=th, threshold_own=th, mta=5e5, mpr=10.0 / 250.0)
+        sets.append(ns)
+    horizon = 20.0
+    return Spec(model, sets, uniform_grid(horizon, dates), num_paths, seed, True, 0.975, "c5")
+
+
 def small_reference_case(num_paths: int = 64, dates: int = 12, csa: bool = False, fx: bool = False,
                          credits: bool = False, two_leg: bool = False, seed: int = 7, bermudans: int = 0,
                          cash_settled: bool = False, options: int = 0) -> Spec:

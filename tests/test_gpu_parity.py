@@ -157,14 +157,20 @@ def test_fp32_options_error_vs_fp64_oracle(engine):
     assert errs["expected_exposure"] < 1e-4 and errs["discounted_expected_exposure"] < 1e-4
 
 
-@pytest.mark.parametrize("paths,precision", [(1, abi.CCR_FP64), (2, abi.CCR_FP64), (3, abi.CCR_FP64),
-                                             (777, abi.CCR_FP64), (2049, abi.CCR_FP64), (5001, abi.CCR_FP64),
-                                             (777, abi.CCR_FP32), (5001, abi.CCR_FP32)])
-def test_pfe_is_exact_order_statistic(engine, paths, precision):
+@pytest.mark.parametrize("paths,precision,level", [
+    (1, abi.CCR_FP64, None), (2, abi.CCR_FP64, None), (3, abi.CCR_FP64, None), (777, abi.CCR_FP64, None),
+    (2049, abi.CCR_FP64, None), (4096, abi.CCR_FP64, None), (4097, abi.CCR_FP64, None), (5001, abi.CCR_FP64, None),
+    (20000, abi.CCR_FP64, 0.999), (20000, abi.CCR_FP64, 0.001), (20000, abi.CCR_FP64, 0.3),
+    (70000, abi.CCR_FP64, 0.5), (777, abi.CCR_FP32, None), (5001, abi.CCR_FP32, None),
+    (70000, abi.CCR_FP32, 0.5)])
+def test_pfe_is_exact_order_statistic(engine, paths, precision, level):
     """The GPU PFE equals ccr::quantile of the GPU's own exposure rows, bit for bit. Rows include
-    all-equal values (t = 0: more ties than the LDS candidate buffer when P > 2048) and
-    all-zero rows (after the last cash flow)."""
+    all-equal values (t = 0: ties at the splitters), all-zero rows (after the last cash flow),
+    quantiles whose sample splitter falls off either end of the sample, and P = 70,000 at the
+    median, where more keys lie between the splitters than LDS holds (the radix-select path)."""
     spec = W.small_reference_case(num_paths=paths, csa=True, fx=True)
+    if level is not None:
+        spec.pfe_quantile = level
     out = engine.run(spec, precision, rows=True)[0]
     rows = out.exposure_rows(spec.num_paths)
     pfe = out.get("potential_future_exposure")

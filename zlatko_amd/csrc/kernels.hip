@@ -7,9 +7,11 @@
 //                          are reduced across the wavefront and written as per-wave partial sums;
 //                          max(V - C, 0) is written path-major-coalesced for the PFE.
 //  ccr_reduce_kernel       fixed-order sum of the per-wave partials (bit-reproducible).
-//  ccr_pfe_kernel<Real>    exact order statistics per (netting set, reporting date) by MSB radix
-//                          select on the IEEE bit patterns of the non-negative exposures, then
-//                          the interpolation of ccr::quantile (include/ccr/core/math.hpp:39-47).
+//  ccr_pfe_kernel<Real>    exact order statistics per (netting set, reporting date) on the IEEE
+//                          bit patterns of the non-negative exposures: splitters from a sorted
+//                          sample, one counting / gathering pass, an LDS sort of the keys between
+//                          the splitters (MSB radix select as the fallback), then the
+//                          interpolation of ccr::quantile (include/ccr/core/math.hpp:39-47).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -27,8 +29,18 @@ __device__ inline int xcdRemap(int b, int nb) {
   return xcd < rem ? xcd * (per + 1) + i : rem * (per + 1) + (xcd - rem) * per + i;
 }
 
+// Minimum waves per SIMD the register allocator is held to, per specialisation: 5 (at most 96
+// VGPRs) where that fits without scratch (three rates, two assets, no credit, no options in
+// FP64: 97 VGPRs unconstrained), else no constraint.
 template <typename Real, typename Sh, bool kOptions>
-__global__ void __launch_bounds__(kMaxBlock) ccr_path_kernel(PathArgs<Real> a) {
+constexpr int pathWaveHint() {
+  return (std::is_same<Real, double>::value && Sh::kExact && Sh::NR == 3 && Sh::NA == 2 && Sh::NC == 0 && !kOptions)
+             ? 5 : 1;
+}
+
+template <typename Real, typename Sh, bool kOptions>
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(pathWaveHint<Real, Sh, kOptions>())))
+ccr_path_kernel(PathArgs<Real> a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const CCR_CONST GlobalDev& g = *toConst(a.g);
   const int b = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
@@ -121,46 +133,125 @@ __device__ __noinline__ double lerpUnfused(double lo, double t, double hi) {
 }
 
 constexpr int kPfeThreads = 256;
-constexpr int kPfeBins = 4096;   // 12-bit digits
-constexpr int kPfeCand = 2048;   // candidates finished by a bitonic sort in LDS
+constexpr int kPfeWaves = kPfeThreads / 64;
+constexpr int kPfeKeys = 4096;   // LDS keys: the sample, then the keys between the splitters
+constexpr int kPfeBins = 4096;   // 12-bit digits of the radix fallback (histogram aliases the keys)
+constexpr int kPfeUnroll = 4;    // row elements in flight per thread
 
-// Finds the bin of hist[0, kPfeBins) holding rank *rank (0-based); leaves the bin in *bin and the
-// rank within it in *rank. One thread scans (the histogram is small next to a row).
-__device__ inline void pfeFindBin(const unsigned int* hist, long long* rank, int* bin) {
-  long long r = *rank;
-  int b = 0;
-  for (; b < kPfeBins - 1; ++b) {
-    if (r < static_cast<long long>(hist[b])) break;
-    r -= hist[b];
-  }
-  *rank = r;
-  *bin = b;
+// Ascending bitonic sort of keys[0, n2), n2 a power of two, by the whole workgroup: each step
+// visits the n2 / 2 compare-exchange pairs once.
+template <typename K>
+__device__ inline void bitonicSort(K* keys, int n2) {
+  const int tid = static_cast<int>(threadIdx.x);
+  for (int k = 2; k <= n2; k <<= 1)
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int p = tid; p < (n2 >> 1); p += kPfeThreads) {
+        const int i = ((p & ~(jj - 1)) << 1) | (p & (jj - 1));
+        const int ixj = i + jj;
+        const bool asc = (i & k) == 0;
+        const K a = keys[i], b = keys[ixj];
+        if ((a > b) == asc) {
+          keys[i] = b;
+          keys[ixj] = a;
+        }
+      }
+      __syncthreads();
+    }
 }
 
-// One workgroup per (set, reporting date) row. Exact order statistics lo = floor(h), hi = lo + 1
-// of the row's keys (h = q (P - 1), ccr::quantile): a histogram of the top 12 key bits (with the
-// count of zero exposures and the smallest positive key, which settle a quantile inside the
-// zeros at once), a histogram of the next 12 bits inside the selected bin, then the keys sharing
-// the selected 24-bit prefix are gathered and bitonic-sorted in LDS; the next key above the
-// prefix, found in the same pass, gives the upper statistic when it lies outside the bin. Bins
-// with more than kPfeCand keys (ties) fall back to 8-bit radix passes over the remaining bits.
+__device__ inline unsigned int waveInclusiveScan(unsigned int v) {
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned int t = __shfl_up(v, d);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+__device__ inline unsigned int waveTotal(unsigned int v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+// Appends the keys of the lanes with `take` to keys[*count ...) (at most kPfeKeys are stored;
+// *count keeps counting past that). One LDS atomic per wavefront. Call with the whole wave active.
+template <typename K>
+__device__ inline void appendKeys(bool take, K k, K* keys, unsigned int* count) {
+  const unsigned long long m = __ballot(take);
+  if (m == 0) return;
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  const int leader = __builtin_ctzll(m);
+  unsigned int base = 0;
+  if (lane == leader) base = atomicAdd(count, static_cast<unsigned int>(__popcll(m)));
+  base = __shfl(base, leader);
+  if (take) {
+    const unsigned int pos = base + static_cast<unsigned int>(__popcll(m & ((1ull << lane) - 1ull)));
+    if (pos < static_cast<unsigned int>(kPfeKeys)) keys[pos] = k;
+  }
+}
+
+// Finds the bin of hist[0, kPfeBins) holding rank `rank` (0-based, below the histogram's total):
+// each thread owns 16 consecutive bins, a workgroup prefix sum of the per-thread counts names the
+// owner, which scans its bins. Leaves the bin and the rank within it in *bin / *binRank; ends with
+// a barrier.
+__device__ inline void pfeFindBin(const unsigned int* hist, long long rank, unsigned int* waveTot, int* bin,
+                                  long long* binRank) {
+  constexpr int per = kPfeBins / kPfeThreads;
+  const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, w = tid >> 6;
+  unsigned int c[per], tot = 0;
+#pragma unroll
+  for (int k = 0; k < per; ++k) {
+    c[k] = hist[tid * per + k];
+    tot += c[k];
+  }
+  const unsigned int incl = waveInclusiveScan(tot);
+  if (lane == 63) waveTot[w] = incl;
+  __syncthreads();
+  long long excl = static_cast<long long>(incl) - tot;
+  for (int k = 0; k < w; ++k) excl += waveTot[k];
+  if (rank >= excl && rank < excl + tot) {
+    long long r = rank - excl;
+    int b = 0;
+    for (; b < per - 1; ++b) {
+      if (r < static_cast<long long>(c[b])) break;
+      r -= c[b];
+    }
+    *bin = tid * per + b;
+    *binRank = r;
+  }
+  __syncthreads();
+}
+
+// One workgroup per (set, reporting date) row: exact order statistics lo = floor(h), hi = lo + 1
+// of the row's keys (h = q (P - 1)), then the interpolation of ccr::quantile.
+//  1. The first min(P, 4096) keys of the row (paths are exchangeable) are sorted in LDS; a row
+//     that short is done. Otherwise two sample keys a <= b, 4.5 binomial standard deviations (+2)
+//     either side of the sample's q-quantile, bracket the row's statistics with high probability.
+//  2. One pass over the row counts keys < a, == a and == b, keeps the smallest key > b and
+//     gathers the keys strictly between a and b into LDS (wave-aggregated slots).
+//  3. If lo falls in [a, b] and the gathered keys fit, they are sorted and the statistics read off
+//     the virtual sorted list (a x #a, the gathered keys, b x #b, then the smallest key > b). Ties
+//     at the splitters (all-equal rows, rows of zeros) cost nothing.
+// Otherwise (a sample miss, or too many keys between the splitters: large P with a central
+// quantile) an MSB radix select over 12-bit digits narrows the prefix until its keys fit in LDS.
 template <typename Real>
 __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets, const Real* rows, int max_rep,
                                                               int num_paths, double level, double* out) {
   using K = typename KeyOf<Real>::T;
   constexpr int kBits = KeyOf<Real>::kBits;
-  constexpr int shiftA = kBits - 12, shiftB = kBits - 24;
-  __shared__ unsigned int hist[kPfeBins];
-  __shared__ K cand[kPfeCand];
-  __shared__ K sPrefix, sMinPos, sMinAbove;
+  __shared__ K keys[kPfeKeys];
+  unsigned int* hist = reinterpret_cast<unsigned int*>(keys);
+  __shared__ unsigned int sBelow, sEqA, sEqB, sOpen, sWave[kPfeWaves];
+  __shared__ K sMinAbove;
   __shared__ long long sRank;
   __shared__ int sBin;
-  __shared__ unsigned int sCount, sZero;
   const int s = static_cast<int>(blockIdx.x) / max_rep, rr = static_cast<int>(blockIdx.x) % max_rep;
   const SetDev& S = sets[s];
   if (rr >= S.num_rep) return;
   const Real* row = rows + S.rows_off + static_cast<int64_t>(rr) * num_paths;
-  const int tid = static_cast<int>(threadIdx.x);
+  const int tid = static_cast<int>(threadIdx.x), lane = tid & 63;
   const double h = level * static_cast<double>(num_paths - 1);
   const long long lo = static_cast<long long>(floor(h));
   const long long hi = lo + 1 < num_paths ? lo + 1 : num_paths - 1;
@@ -169,137 +260,156 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
     out[s * max_rep + rr] = lerpUnfused(vlo, h - static_cast<double>(lo), vhi);
   };
 
-  // Pass A: top 12 bits, zero count, smallest positive key.
-  for (int b = tid; b < kPfeBins; b += kPfeThreads) hist[b] = 0;
+  // 1. Sample.
+  const int ns = num_paths < kPfeKeys ? num_paths : kPfeKeys;
+  int n2 = 1;
+  while (n2 < ns) n2 <<= 1;
+  for (int i = tid; i < n2; i += kPfeThreads) keys[i] = i < ns ? keyOf<Real>(row[i]) : ~K(0);
+  __syncthreads();
+  bitonicSort(keys, n2);
+  if (ns == num_paths) {
+    if (tid == 0) finish(keys[lo], keys[hi]);
+    return;
+  }
+  const long long ls = static_cast<long long>(floor(level * static_cast<double>(ns - 1)));
+  const long long margin = static_cast<long long>(ceil(4.5 * sqrt(ns * level * (1.0 - level)))) + 2;
+  const K a = ls - margin >= 0 ? keys[ls - margin] : K(0);
+  const K b = ls + 1 + margin < ns ? keys[ls + 1 + margin] : ~K(0);  // ~0: no upper splitter
+  const bool twoEnds = b != a;
   if (tid == 0) {
-    sZero = 0;
-    sMinPos = ~K(0);
-    sRank = lo;
+    sBelow = sEqA = sEqB = sOpen = 0;
+    sMinAbove = ~K(0);
+  }
+  __syncthreads();  // the splitters are read before the key buffer is reused
+
+  // 2. One pass over the row.
+  {
+    unsigned int below = 0, eqA = 0, eqB = 0;
+    K minAbove = ~K(0);
+    for (int base = 0; base < num_paths; base += kPfeThreads * kPfeUnroll) {
+      K kk[kPfeUnroll];
+#pragma unroll
+      for (int u = 0; u < kPfeUnroll; ++u) {
+        const int i = base + u * kPfeThreads + tid;
+        kk[u] = i < num_paths ? keyOf<Real>(row[i]) : ~K(0);  // ~0: past the row, counted nowhere
+      }
+#pragma unroll
+      for (int u = 0; u < kPfeUnroll; ++u) {
+        const K k = kk[u];
+        const bool valid = k != ~K(0);
+        below += (valid && k < a) ? 1u : 0u;
+        eqA += (valid && k == a) ? 1u : 0u;
+        eqB += (valid && twoEnds && k == b) ? 1u : 0u;
+        if (valid && k > b && k < minAbove) minAbove = k;
+        appendKeys(valid && k > a && k < b, k, keys, &sOpen);
+      }
+    }
+    below = waveTotal(below);
+    eqA = waveTotal(eqA);
+    eqB = waveTotal(eqB);
+    if (lane == 0) {
+      atomicAdd(&sBelow, below);
+      atomicAdd(&sEqA, eqA);
+      atomicAdd(&sEqB, eqB);
+    }
+    atomicMin(&sMinAbove, minAbove);
   }
   __syncthreads();
   {
-    unsigned int zeros = 0;
-    K minPos = ~K(0);
+    const long long nBelow = sBelow, nA = sEqA, nB = sEqB;
+    const unsigned int nOpen = sOpen;
+    if (nOpen <= static_cast<unsigned int>(kPfeKeys) && lo >= nBelow && lo < nBelow + nA + nOpen + nB) {
+      // 3. Sort the gathered keys and read the statistics off the virtual list.
+      int m2 = 1;
+      while (m2 < static_cast<int>(nOpen)) m2 <<= 1;
+      for (int i = static_cast<int>(nOpen) + tid; i < m2; i += kPfeThreads) keys[i] = ~K(0);
+      __syncthreads();
+      bitonicSort(keys, m2);
+      if (tid == 0) {
+        auto at = [&](long long r, K* v) {
+          long long i = r - nBelow;
+          if (i < nA) {
+            *v = a;
+            return true;
+          }
+          i -= nA;
+          if (i < static_cast<long long>(nOpen)) {
+            *v = keys[i];
+            return true;
+          }
+          i -= nOpen;
+          if (i < nB) {
+            *v = b;
+            return true;
+          }
+          return false;
+        };
+        K kLo = 0, kHi = 0;
+        at(lo, &kLo);
+        if (hi == lo) kHi = kLo;
+        else if (!at(hi, &kHi)) kHi = sMinAbove;  // hi is the first key past b
+        finish(kLo, kHi);
+      }
+      return;
+    }
+  }
+
+  // Fallback: MSB radix select over 12-bit digits (the last digit takes the remaining bits).
+  K prefix = 0, mask = 0;
+  long long rank = lo;
+  for (int shift = kBits; shift > 0;) {
+    const int width = shift >= 12 ? 12 : shift;
+    shift -= width;
+    const K digit = (K(1) << width) - 1;
+    __syncthreads();
+    for (int i = tid; i < kPfeBins; i += kPfeThreads) hist[i] = 0;
+    __syncthreads();
     for (int i = tid; i < num_paths; i += kPfeThreads) {
       const K k = keyOf<Real>(row[i]);
-      if (k == 0) ++zeros;
-      else if (k < minPos) minPos = k;
-      atomicAdd(&hist[k >> shiftA], 1u);
+      if ((k & mask) == prefix) atomicAdd(&hist[static_cast<int>((k >> shift) & digit)], 1u);
     }
-    atomicAdd(&sZero, zeros);
-    atomicMin(&sMinPos, minPos);
-  }
-  __syncthreads();
-  if (lo < static_cast<long long>(sZero)) {  // the quantile lies in the zero exposures
-    if (tid == 0) finish(K(0), (hi == lo || hi < static_cast<long long>(sZero)) ? K(0) : sMinPos);
-    return;
-  }
-  if (tid == 0) pfeFindBin(hist, &sRank, &sBin);
-  __syncthreads();
-  const K binA = static_cast<K>(sBin);
-
-  // Pass B: the next 12 bits inside the selected bin.
-  for (int b = tid; b < kPfeBins; b += kPfeThreads) hist[b] = 0;
-  __syncthreads();
-  for (int i = tid; i < num_paths; i += kPfeThreads) {
-    const K k = keyOf<Real>(row[i]);
-    if ((k >> shiftA) == binA) atomicAdd(&hist[(k >> shiftB) & 0xFFF], 1u);
-  }
-  __syncthreads();
-  if (tid == 0) pfeFindBin(hist, &sRank, &sBin);
-  __syncthreads();
-  const K prefix24 = (binA << 12) | static_cast<K>(sBin);
-  const unsigned int count = hist[sBin];
-
-  if (count <= static_cast<unsigned int>(kPfeCand)) {
-    // Pass C: gather the bin's keys and the smallest key above the prefix; sort in LDS.
+    __syncthreads();
+    pfeFindBin(hist, rank, sWave, &sBin, &sRank);
+    const int bin = sBin;
+    const unsigned int cnt = hist[bin];
+    rank = sRank;
+    prefix |= static_cast<K>(bin) << shift;
+    mask |= digit << shift;
+    if (shift > 0 && cnt > static_cast<unsigned int>(kPfeKeys)) continue;
+    // The keys under the prefix (all equal once shift is 0) fit in LDS: gather them with the
+    // smallest key above the prefix, sort, read off the statistics.
+    __syncthreads();
     if (tid == 0) {
-      sCount = 0;
+      sOpen = 0;
       sMinAbove = ~K(0);
     }
     __syncthreads();
     K minAbove = ~K(0);
-    for (int i = tid; i < num_paths; i += kPfeThreads) {
-      const K k = keyOf<Real>(row[i]);
-      const K p = k >> shiftB;
-      if (p == prefix24) cand[atomicAdd(&sCount, 1u)] = k;
-      else if (p > prefix24 && k < minAbove) minAbove = k;
+    for (int base = 0; base < num_paths; base += kPfeThreads) {
+      const int i = base + tid;
+      const K k = i < num_paths ? keyOf<Real>(row[i]) : K(0);
+      const bool in = i < num_paths && (k & mask) == prefix;
+      if (i < num_paths && (k & mask) > prefix && k < minAbove) minAbove = k;
+      appendKeys(in, k, keys, &sOpen);
     }
     atomicMin(&sMinAbove, minAbove);
     __syncthreads();
-    int n2 = 2;
-    while (n2 < static_cast<int>(count)) n2 <<= 1;
-    for (int i = static_cast<int>(count) + tid; i < n2; i += kPfeThreads) cand[i] = ~K(0);
+    if (cnt > static_cast<unsigned int>(kPfeKeys)) {  // shift is 0: every key under the prefix equals it
+      if (tid == 0) finish(prefix, (hi == lo || rank + 1 < static_cast<long long>(cnt)) ? prefix : sMinAbove);
+      return;
+    }
+    int m2 = 1;
+    while (m2 < static_cast<int>(cnt)) m2 <<= 1;
+    for (int i = static_cast<int>(cnt) + tid; i < m2; i += kPfeThreads) keys[i] = ~K(0);
     __syncthreads();
-    for (int k = 2; k <= n2; k <<= 1)
-      for (int jj = k >> 1; jj > 0; jj >>= 1) {
-        for (int i = tid; i < n2; i += kPfeThreads) {
-          const int ixj = i ^ jj;
-          if (ixj > i) {
-            const bool asc = (i & k) == 0;
-            const K a = cand[i], b = cand[ixj];
-            if ((a > b) == asc) {
-              cand[i] = b;
-              cand[ixj] = a;
-            }
-          }
-        }
-        __syncthreads();
-      }
+    bitonicSort(keys, m2);
     if (tid == 0) {
-      const long long r = sRank;
-      const K kLo = cand[r];
-      const K kHi = hi == lo ? kLo : (r + 1 < static_cast<long long>(count) ? cand[r + 1] : sMinAbove);
+      const K kLo = keys[rank];
+      const K kHi = hi == lo ? kLo : (rank + 1 < static_cast<long long>(cnt) ? keys[rank + 1] : sMinAbove);
       finish(kLo, kHi);
     }
     return;
   }
-
-  // Fallback for bins of ties: 8-bit radix passes over the low bits, then the upper statistic.
-  if (tid == 0) sPrefix = prefix24 << shiftB;
-  __syncthreads();
-  unsigned int* hist8 = hist;
-  K mask = ~K(0) << shiftB;
-  for (int shift = shiftB - 8; shift >= 0; shift -= 8) {
-    hist8[tid] = 0;
-    __syncthreads();
-    const K prefix = sPrefix;
-    for (int i = tid; i < num_paths; i += kPfeThreads) {
-      const K k = keyOf<Real>(row[i]);
-      if ((k & mask) == prefix) atomicAdd(&hist8[(k >> shift) & 0xFF], 1u);
-    }
-    __syncthreads();
-    if (tid == 0) {
-      long long rank = sRank;
-      int bin = 0;
-      for (; bin < 255; ++bin) {
-        if (rank < static_cast<long long>(hist8[bin])) break;
-        rank -= hist8[bin];
-      }
-      sRank = rank;
-      sPrefix = prefix | (static_cast<K>(bin) << shift);
-    }
-    mask |= static_cast<K>(0xFF) << shift;
-    __syncthreads();
-  }
-  const K kLo = sPrefix;
-  // Upper order statistic: equal to the lower one if enough keys are <= it, else the next key above.
-  if (tid == 0) {
-    sCount = 0;
-    sMinAbove = ~K(0);
-  }
-  __syncthreads();
-  unsigned int cnt = 0;
-  K mn = ~K(0);
-  for (int i = tid; i < num_paths; i += kPfeThreads) {
-    const K k = keyOf<Real>(row[i]);
-    if (k <= kLo) ++cnt;
-    else if (k < mn) mn = k;
-  }
-  atomicAdd(&sCount, cnt);
-  atomicMin(&sMinAbove, mn);
-  __syncthreads();
-  if (tid == 0) finish(kLo, (hi == lo || static_cast<long long>(sCount) >= hi + 1) ? kLo : sMinAbove);
 }
 
 // ---------------------------------------------------------------- launchers (host side)

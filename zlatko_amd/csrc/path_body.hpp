@@ -30,28 +30,51 @@
 
 namespace ccrhip {
 
+// Fused multiply-adds whose named operand is wavefront-uniform (a constant, or a value loaded
+// through the scalar cache): one v_fma_f64 that reads it from SGPRs. Left to itself the compiler
+// selects the two-address v_fmac_f64 and first copies such an operand into the destination
+// VGPRs (two v_mov_b32 per use) or keeps constants resident in VGPRs. Same rounding as fma().
+// Only for operands that are uniform across the wavefront.
+__host__ __device__ inline double fmaU3(double a, double b, double c) {  // c uniform
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+  return d;
+#else
+  return fma(a, b, c);
+#endif
+}
+__host__ __device__ inline double fmaU2(double a, double b, double c) {  // b uniform
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+  return d;
+#else
+  return fma(a, b, c);
+#endif
+}
+
 // exp(a) for |a| < 700 in FP64 (no overflow / underflow handling: bond-term exponents are
 // O(1)). Cody-Waite reduction a = n ln2 + r, |r| <= ln2/2, with the two-part ln2 of fdlibm's
 // e_exp.c; exp(r) by its degree-13 Taylor polynomial (truncation < 5e-18 relative on the
 // interval) in Horner form; 2^n by ldexp. Accurate to about 1 ulp.
 __host__ __device__ inline double expBond(double a) {
   const double n = rint(a * 1.4426950408889634074);
-  double r = fma(-n, 6.93147180369123816490e-01, a);
-  r = fma(-n, 1.90821492927058770002e-10, r);
-  double p = 1.0 / 6227020800.0;       // 1/13!
-  p = fma(p, r, 1.0 / 479001600.0);    // 1/12!
-  p = fma(p, r, 1.0 / 39916800.0);     // 1/11!
-  p = fma(p, r, 1.0 / 3628800.0);      // 1/10!
-  p = fma(p, r, 1.0 / 362880.0);       // 1/9!
-  p = fma(p, r, 1.0 / 40320.0);        // 1/8!
-  p = fma(p, r, 1.0 / 5040.0);         // 1/7!
-  p = fma(p, r, 1.0 / 720.0);          // 1/6!
-  p = fma(p, r, 1.0 / 120.0);          // 1/5!
-  p = fma(p, r, 1.0 / 24.0);           // 1/4!
-  p = fma(p, r, 1.0 / 6.0);            // 1/3!
-  p = fma(p, r, 0.5);
-  p = fma(p, r, 1.0);
-  p = fma(p, r, 1.0);
+  double r = fmaU2(n, -6.93147180369123816490e-01, a);
+  r = fmaU2(n, -1.90821492927058770002e-10, r);
+  double p = fmaU3(r, 1.0 / 6227020800.0, 1.0 / 479001600.0);  // 1/13!, 1/12!
+  p = fmaU3(p, r, 1.0 / 39916800.0);     // 1/11!
+  p = fmaU3(p, r, 1.0 / 3628800.0);      // 1/10!
+  p = fmaU3(p, r, 1.0 / 362880.0);       // 1/9!
+  p = fmaU3(p, r, 1.0 / 40320.0);        // 1/8!
+  p = fmaU3(p, r, 1.0 / 5040.0);         // 1/7!
+  p = fmaU3(p, r, 1.0 / 720.0);          // 1/6!
+  p = fmaU3(p, r, 1.0 / 120.0);          // 1/5!
+  p = fmaU3(p, r, 1.0 / 24.0);           // 1/4!
+  p = fmaU3(p, r, 1.0 / 6.0);            // 1/3!
+  p = fmaU3(p, r, 0.5);
+  p = fmaU3(p, r, 1.0);
+  p = fmaU3(p, r, 1.0);
   return ldexp(p, static_cast<int>(n));
 }
 
@@ -388,10 +411,10 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
             double h = 0.0;
             for (int k = 0; k < len; k += 4) {
               const double c0 = pc[k], c1 = pc[k + 1], c2 = pc[k + 2], c3 = pc[k + 3];
-              h = fma(h, xd, c0);
-              h = fma(h, xd, c1);
-              h = fma(h, xd, c2);
-              h = fma(h, xd, c3);
+              h = fmaU3(h, xd, c0);  // the coefficients are uniform: the date loop is
+              h = fmaU3(h, xd, c1);  // uniform across the wavefront
+              h = fmaU3(h, xd, c2);
+              h = fmaU3(h, xd, c3);
             }
             acc = static_cast<Real>(h);
           } else {

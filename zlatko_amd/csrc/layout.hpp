@@ -46,10 +46,11 @@ struct DateRec {
   int32_t bond_beg[kMaxRates + 1];   // bond terms of rate r: [bond_beg[r], bond_beg[r + 1])
   int32_t fix_beg[kMaxRates + 1];    // fixed-coupon terms of rate r
   int32_t fixstore_beg, fixstore_end;  // (rate, slot) pairs whose state is recorded at this date
-  // Bond polynomial of rate r: poly_len[r] coefficients (a multiple of 4, highest power first,
-  // zero-padded at the top) from poly_beg[r]; 0 = none.
-  int32_t poly_beg[kMaxRates];
-  int32_t poly_len[kMaxRates];
+  // Bond polynomials of all rates, interleaved: poly_rows rows (a multiple of 4) of nrates
+  // coefficients from poly_beg; row k holds power poly_rows - 1 - k of every rate (zero-padded at
+  // the top). 0 rows: none (the exponential terms are evaluated).
+  int32_t poly_beg, poly_rows;
+  int32_t pad_poly[2 * kMaxRates - 2];
   int32_t opt_beg, opt_end;          // European options alive at this date: [opt_beg, opt_end)
   int32_t pad[2];
 };

@@ -397,26 +397,44 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     cptr<Real> dr = dreal + static_cast<int64_t>(j) * kDateRealStride;
     Real V = 0;
     if (rec.flags & kValued) {
+      // Merged bonds (and their constants) of every rate as power series in its state: Horner
+      // in FP64 over the interleaved coefficient rows (highest power first, one coefficient per
+      // rate per row), the rates' chains side by side and four rows per batch of scalar loads.
+      // The coefficients are uniform: the date loop is uniform across the wavefront.
+      const int rows = rec.poly_rows;
+      bool usePoly = rows > 0;
+      if (usePoly) {
+        bool inside = true;
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          if (r < nR) inside = inside && static_cast<double>(x[r] < Real(0) ? -x[r] : x[r]) <= dr[kDateXmax + r];
+        usePoly = allLanes(inside);
+      }
+      double hp[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) hp[r] = 0.0;
+      if (usePoly) {
+        cptr<double> pc = poly + rec.poly_beg;
+        for (int k = 0; k < rows; k += 4) {
+          double c[4][NR];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) c[u][r] = r < nR ? pc[(k + u) * nR + r] : 0.0;
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+              if (r < nR) hp[r] = fmaU3(hp[r], static_cast<double>(x[r]), c[u][r]);
+        }
+      }
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         if (r < nR) {
           const Real nx = -x[r];
           Real acc;
-          const int len = rec.poly_len[r];
-          if (len > 0 && allLanes(static_cast<double>(x[r] < Real(0) ? -x[r] : x[r]) <= dr[kDateXmax + r])) {
-            // Merged bonds (and their constant) as a power series in x: Horner in FP64 over the
-            // coefficients stored highest power first, four per scalar load.
-            cptr<double> pc = poly + rec.poly_beg[r];
-            const double xd = static_cast<double>(x[r]);
-            double h = 0.0;
-            for (int k = 0; k < len; k += 4) {
-              const double c0 = pc[k], c1 = pc[k + 1], c2 = pc[k + 2], c3 = pc[k + 3];
-              h = fmaU3(h, xd, c0);  // the coefficients are uniform: the date loop is
-              h = fmaU3(h, xd, c1);  // uniform across the wavefront
-              h = fmaU3(h, xd, c2);
-              h = fmaU3(h, xd, c3);
-            }
-            acc = static_cast<Real>(h);
+          if (usePoly) {
+            acc = static_cast<Real>(hp[r]);
           } else {
             acc = dr[kDateC0 + r];
             for (int i = rec.bond_beg[r]; i < rec.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expTerm(bonds[2 * i + 1] * nx);

@@ -566,21 +566,32 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
         }
       }
       rec.bond_beg[kMaxRates] = static_cast<int32_t>(plan.bonds.size() / 2) - sd.bond_off;
-      for (int r = 0; r < kMaxRates; ++r) {
-        rec.poly_beg[r] = static_cast<int32_t>(plan.poly.size()) - sd.poly_off;
-        rec.poly_len[r] = 0;
-        if (r >= g.nrates || !valued[static_cast<std::size_t>(j)]) continue;
-        const int b0 = sd.bond_off + rec.bond_beg[r], b1 = sd.bond_off + rec.bond_beg[r + 1];
-        const double xmax = stateBound(m.rates[static_cast<std::size_t>(r)], m.quanto[static_cast<std::size_t>(r)], t);
-        std::vector<double> coef;
-        if (bondPolynomial(plan.bonds.data() + 2 * b0, b1 - b0, col.c0[r], xmax, coef)) {
-          const std::size_t len = (coef.size() + 3) / 4 * 4;  // highest power first, zero-padded on top
-          plan.poly.insert(plan.poly.end(), len - coef.size(), 0.0);
-          plan.poly.insert(plan.poly.end(), coef.rbegin(), coef.rend());
-          rec.poly_len[r] = static_cast<int32_t>(len);
-          plan.poly_evaluations += static_cast<int64_t>(len) * P;
+      // Bond polynomials of every rate, interleaved row by row (highest power first, shorter
+      // series zero-padded on top); none unless every rate has one.
+      rec.poly_beg = static_cast<int32_t>(plan.poly.size()) - sd.poly_off;
+      rec.poly_rows = 0;
+      if (valued[static_cast<std::size_t>(j)]) {
+        std::vector<std::vector<double>> coef(static_cast<std::size_t>(g.nrates));
+        bool all = true;
+        std::size_t rows = 0;
+        for (int r = 0; r < g.nrates; ++r) {
+          const int b0 = sd.bond_off + rec.bond_beg[r], b1 = sd.bond_off + rec.bond_beg[r + 1];
+          const double xmax =
+              stateBound(m.rates[static_cast<std::size_t>(r)], m.quanto[static_cast<std::size_t>(r)], t);
+          reals[kDateXmax + r] = xmax;
+          auto& c = coef[static_cast<std::size_t>(r)];
+          all = all && bondPolynomial(plan.bonds.data() + 2 * b0, b1 - b0, col.c0[r], xmax, c);
+          rows = std::max(rows, c.size());
         }
-        reals[kDateXmax + r] = xmax;
+        if (all) {
+          rows = (rows + 3) / 4 * 4;
+          for (std::size_t k = 0; k < rows; ++k) {
+            const std::size_t power = rows - 1 - k;
+            for (const auto& c : coef) plan.poly.push_back(power < c.size() ? c[power] : 0.0);
+          }
+          rec.poly_rows = static_cast<int32_t>(rows);
+          plan.poly_evaluations += static_cast<int64_t>(rows) * g.nrates * P;
+        }
       }
       for (int r = 0; r < kMaxRates; ++r) {
         rec.fix_beg[r] = static_cast<int32_t>(plan.fixed.size() / 4) - sd.fixed_off;

@@ -221,7 +221,9 @@ enum ccr_precision {
 typedef struct ccr_hip_options {
   int32_t device;      /* HIP device ordinal */
   int32_t precision;   /* default ccr_precision of runs */
-  int32_t block_size;  /* threads per workgroup of the path kernel; 0 = autotuned */
+  int32_t block_size;  /* threads per workgroup of the path kernel (64..256, a multiple of 64); 0 = per
+                          plan, the size with the most resident waves per CU (ccr_hip_autotune
+                          measures the candidates instead) */
 } ccr_hip_options;
 
 typedef struct ccr_hip_ctx ccr_hip_ctx;

@@ -23,7 +23,7 @@ using namespace ccrhip;
 struct ccr_hip_ctx {
   int device = 0;
   int precision = CCR_FP64;
-  int block = 256;
+  int block = 0;  // path-kernel workgroup size; 0: chosen per plan by occupancy
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // start, AMC, path, reduce, PFE
   std::string err;
@@ -181,6 +181,28 @@ void uploadStream(ccr_hip_ctx* ctx, ccr_hip_plan* plan, uint64_t seed, bool fp32
   plan->upload_ms += total - genMs;
 }
 
+// Workgroup size of the path kernel with the most waves resident per CU for this plan (its LDS
+// rows per path and its kernel specialisation's registers); ties go to the larger workgroup.
+int occupancyBlock(const HostPlan& h) {
+  const int ldsRows = h.g.max_ring + h.price_rows + h.g.max_slots;
+  int best = kMaxBlock, bestWaves = -1;
+  for (int block = kMaxBlock; block >= 64; block -= 64) {
+    int wg = 0;
+    const hipError_t e = h.precision == CCR_FP32
+                             ? pathOccupancy<float>(h.g.nrates, h.g.nassets, h.g.ncredits, !h.opts.empty(), ldsRows,
+                                                    block, &wg)
+                             : pathOccupancy<double>(h.g.nrates, h.g.nassets, h.g.ncredits, !h.opts.empty(), ldsRows,
+                                                     block, &wg);
+    check(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    const int waves = wg * (block / 64);
+    if (waves > bestWaves) {
+      bestWaves = waves;
+      best = block;
+    }
+  }
+  return best;
+}
+
 void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
                  const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
                  double pfe_quantile, int32_t precision, ccr_hip_plan** out) {
@@ -194,8 +216,10 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   check(hipSetDevice(ctx->device), "hipSetDevice");
   auto plan = std::make_unique<ccr_hip_plan>();
   plan->device = ctx->device;
-  plan->host = buildPlan(*generator, *config, base_grid, base_size, sets, num_sets, pfe_quantile, precision, ctx->block);
+  plan->host = buildPlan(*generator, *config, base_grid, base_size, sets, num_sets, pfe_quantile, precision,
+                         ctx->block > 0 ? ctx->block : kMaxBlock);
   HostPlan& h = plan->host;
+  if (ctx->block == 0) assignBlocks(h, occupancyBlock(h));
   const bool fp32 = precision == CCR_FP32;
   const auto t0 = std::chrono::steady_clock::now();
   plan->d_global = upload(ctx, plan.get(), &h.g, 1);
@@ -537,9 +561,9 @@ int32_t ccr_hip_create(const ccr_hip_options* options, ccr_hip_ctx** out) {
   if (options) {
     ctx->device = options->device;
     ctx->precision = options->precision;
-    ctx->block = options->block_size > 0 ? options->block_size : 256;
+    ctx->block = options->block_size;
   }
-  if (ctx->block % 64 != 0 || ctx->block > kMaxBlock) {
+  if (ctx->block < 0 || ctx->block % 64 != 0 || ctx->block > kMaxBlock) {
     gCreateError = "block_size must be a multiple of 64 up to 256 (the path kernel's launch bound)";
     return CCR_ERR_INVALID_ARGUMENT;
   }

@@ -437,6 +437,21 @@ hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStre
 }
 
 template <typename Real>
+hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu) {
+  const size_t lds = static_cast<size_t>(ldsRows) * block * sizeof(Real);
+  return withShape(nrates, nassets, ncredits, [&](auto shape) -> hipError_t {
+    using Sh = decltype(shape);
+    if constexpr (Sh::NA > 0) {
+      if (options)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(wgPerCu, ccr_path_kernel<Real, Sh, true>, block, lds);
+    }
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wgPerCu, ccr_path_kernel<Real, Sh, false>, block, lds);
+  });
+}
+template hipError_t pathOccupancy<double>(int, int, int, bool, int, int, int*);
+template hipError_t pathOccupancy<float>(int, int, int, bool, int, int, int*);
+
+template <typename Real>
 hipError_t launchStates(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
   return withShape(args.nrates, args.nassets, args.ncredits, [&](auto shape) {
     hipLaunchKernelGGL((ccr_states_kernel<Real, decltype(shape)>), dim3(blocks), dim3(block), 0, stream, args);

@@ -105,6 +105,10 @@ hipError_t launchSurrogateScatter(const float* y, int64_t n, double mean, double
 
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);
+// Workgroups of the path kernel resident per CU (registers, LDS, workgroup slots) for a model
+// shape, the option variant, ldsRows LDS rows per thread and a workgroup size.
+template <typename Real>
+hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu);
 // The states-only pass of the path kernel (AMC): writes x and D of every date into args.states.
 template <typename Real>
 hipError_t launchStates(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);

@@ -393,8 +393,6 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       g.chol[slot * kMaxCorr + k] = m.chol[static_cast<std::size_t>(i * m.ncorr + k)];
   }
 
-  const int wavesPerBlock = block / 64;
-  const int blocksPerSet = (P + block - 1) / block;
   int64_t maxStride = 0;
   for (int s = 0; s < num_sets; ++s) {
     const ccr_netting_set& ns = sets[s];
@@ -422,9 +420,6 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     sd.opt_off = static_cast<int32_t>(plan.opts.size() / kOptStride);
     sd.stream_stride = static_cast<int64_t>(T - 1) * m.nz;
     maxStride = std::max(maxStride, sd.stream_stride);
-    sd.block_begin = s * blocksPerSet;
-    sd.num_blocks = blocksPerSet;
-    sd.wave_begin = s * blocksPerSet * wavesPerBlock;
     sd.cpty = ns.counterparty_credit;
     sd.own = ns.own_credit;
 
@@ -850,15 +845,28 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
   }
   const int64_t blocksPaths = cfg.antithetic ? (static_cast<int64_t>(P) + 1) / 2 : P;
   plan.stream_len = blocksPaths * maxStride;
-  plan.total_waves = num_sets * blocksPerSet * wavesPerBlock;
+  for (int s = 0; s < num_sets; ++s)
+    plan.sets[static_cast<std::size_t>(s)].rows_off = static_cast<int64_t>(s) * plan.max_rep * P;
+  assignBlocks(plan, block);
+  return plan;
+}
+
+void assignBlocks(HostPlan& plan, int block) {
+  if (block <= 0 || block % 64 != 0 || block > kMaxBlock) invalid("block size must be a multiple of 64 up to 256");
+  const int P = plan.g.num_paths, num_sets = static_cast<int>(plan.sets.size());
+  const int wavesPerBlock = block / 64;
+  const int blocksPerSet = (P + block - 1) / block;
+  plan.block = block;
   for (int s = 0; s < num_sets; ++s) {
     SetDev& sd = plan.sets[static_cast<std::size_t>(s)];
-    sd.rows_off = static_cast<int64_t>(s) * plan.max_rep * P;
+    sd.block_begin = s * blocksPerSet;
+    sd.num_blocks = blocksPerSet;
+    sd.wave_begin = s * blocksPerSet * wavesPerBlock;
     sd.partial_off = static_cast<int64_t>(sd.wave_begin) * plan.max_rep * kStats;
   }
+  plan.total_waves = num_sets * blocksPerSet * wavesPerBlock;
   plan.block_set.resize(static_cast<std::size_t>(num_sets * blocksPerSet));
   for (int b = 0; b < num_sets * blocksPerSet; ++b) plan.block_set[static_cast<std::size_t>(b)] = b / blocksPerSet;
-  return plan;
 }
 
 }  // namespace ccrhip

@@ -60,6 +60,10 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
                    int32_t precision, int32_t block);
 
+// Workgroup size of the path kernel: the workgroups of every set, their waves and the offsets of
+// the per-wave partial sums.
+void assignBlocks(HostPlan& plan, int block);
+
 // Power-series coefficients of one date's merged bond terms (see layout.hpp, kPolyTolerance).
 bool bondPolynomial(const double* terms, int n, double c0, double xmax, std::vector<double>& coef);
 

@@ -428,6 +428,41 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
               if (r < nR) hp[r] = fmaU3(hp[r], static_cast<double>(x[r]), c[u][r]);
         }
       }
+      // Fixed floating coupons of every rate (contiguous over the rates), two per batch so that
+      // their record loads, fixing-state reads and exponentials overlap; the plan pads the coupon
+      // table with one zero record, so the second record of a batch always exists.
+      Real fsum[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) fsum[r] = 0;
+      {
+        const int fe = rec.fix_beg[kMaxRates];
+        for (int i = rec.fix_beg[0]; i < fe; i += 2) {
+          cptr<Real> f = fixed + 4 * static_cast<int64_t>(i);
+          const Real a0 = f[0], b0 = f[1], c0 = f[2], a1 = f[4], b1 = f[5], c1 = f[6];
+          const int s0 = static_cast<int>(f[3]), s1 = static_cast<int>(f[7]);
+          const bool two = i + 1 < fe;
+          int r0 = 0, r1 = 0;
+#pragma unroll
+          for (int r = 1; r < NR; ++r) {
+            if (i >= rec.fix_beg[r]) r0 = r;
+            if (i + 1 >= rec.fix_beg[r]) r1 = r;
+          }
+          Real x0 = x[0], x1 = x[0];
+#pragma unroll
+          for (int r = 1; r < NR; ++r) {
+            if (r == r0) x0 = x[r];
+            if (r == r1) x1 = x[r];
+          }
+          const Real xk0 = xs[s0 * stride], xk1 = xs[s1 * stride];
+          const Real t0 = a0 * expTerm(-b0 * x0 - c0 * xk0);
+          const Real t1 = a1 * expTerm(-b1 * x1 - c1 * xk1);
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            if (r == r0) fsum[r] += t0;
+            if (two && r == r1) fsum[r] += t1;
+          }
+        }
+      }
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         if (r < nR) {
@@ -439,11 +474,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
             acc = dr[kDateC0 + r];
             for (int i = rec.bond_beg[r]; i < rec.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expTerm(bonds[2 * i + 1] * nx);
           }
-          for (int i = rec.fix_beg[r]; i < rec.fix_beg[r + 1]; ++i) {
-            cptr<Real> f = fixed + 4 * static_cast<int64_t>(i);
-            const Real xk = xs[static_cast<int>(f[3]) * stride];
-            acc += f[0] * expTerm(f[1] * nx - f[2] * xk);
-          }
+          acc += fsum[r];
           Real mult = 1;
           if (r > 0) {
             const int fa = g.fx_of[r];

@@ -847,6 +847,8 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
   plan.stream_len = blocksPaths * maxStride;
   for (int s = 0; s < num_sets; ++s)
     plan.sets[static_cast<std::size_t>(s)].rows_off = static_cast<int64_t>(s) * plan.max_rep * P;
+  // One zero coupon record past the end: the path kernel reads coupons two at a time.
+  plan.fixed.insert(plan.fixed.end(), 4, 0.0);
   assignBlocks(plan, block);
   return plan;
 }

@@ -46,7 +46,7 @@ struct DateRec {
   int32_t bond_beg[kMaxRates + 1];   // bond terms of rate r: [bond_beg[r], bond_beg[r + 1])
   int32_t fix_beg[kMaxRates + 1];    // fixed-coupon terms of rate r
   int32_t fixstore_beg, fixstore_end;  // (rate, slot) pairs whose state is recorded at this date
-  // Bond polynomials of all rates, interleaved: poly_rows rows (a multiple of 4) of nrates
+  // Bond polynomials of all rates, interleaved: poly_rows rows (a multiple of kPolyBatch) of nrates
   // coefficients from poly_beg; row k holds power poly_rows - 1 - k of every rate (zero-padded at
   // the top). 0 rows: none (the exponential terms are evaluated).
   int32_t poly_beg, poly_rows;
@@ -75,6 +75,7 @@ constexpr int kDateRealStride = 16;
 // below kPolyTolerance of sum_T |c_T A_T| for |x| <= xmax (a bound of 9 standard deviations plus
 // the drift); a wavefront with any lane beyond xmax evaluates the exponential terms instead.
 constexpr double kPolyTolerance = 1e-18;
+constexpr int kPolyBatch = 8;  // coefficient rows per batch of scalar loads in the path kernel
 constexpr double kPolySigmas = 9.0;
 constexpr int kPolyMaxDegree = 48;
 

@@ -399,7 +399,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     if (rec.flags & kValued) {
       // Merged bonds (and their constants) of every rate as power series in its state: Horner
       // in FP64 over the interleaved coefficient rows (highest power first, one coefficient per
-      // rate per row), the rates' chains side by side and four rows per batch of scalar loads.
+      // rate per row), the rates' chains side by side and kPolyBatch rows per batch of scalar loads.
       // The coefficients are uniform: the date loop is uniform across the wavefront.
       const int rows = rec.poly_rows;
       bool usePoly = rows > 0;
@@ -415,14 +415,14 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
       for (int r = 0; r < NR; ++r) hp[r] = 0.0;
       if (usePoly) {
         cptr<double> pc = poly + rec.poly_beg;
-        for (int k = 0; k < rows; k += 4) {
-          double c[4][NR];
+        for (int k = 0; k < rows; k += kPolyBatch) {
+          double c[kPolyBatch][NR];
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+          for (int u = 0; u < kPolyBatch; ++u)
 #pragma unroll
             for (int r = 0; r < NR; ++r) c[u][r] = r < nR ? pc[(k + u) * nR + r] : 0.0;
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+          for (int u = 0; u < kPolyBatch; ++u)
 #pragma unroll
             for (int r = 0; r < NR; ++r)
               if (r < nR) hp[r] = fmaU3(hp[r], static_cast<double>(x[r]), c[u][r]);

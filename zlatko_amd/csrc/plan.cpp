@@ -579,7 +579,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
           rows = std::max(rows, c.size());
         }
         if (all) {
-          rows = (rows + 3) / 4 * 4;
+          rows = (rows + kPolyBatch - 1) / kPolyBatch * kPolyBatch;
           for (std::size_t k = 0; k < rows; ++k) {
             const std::size_t power = rows - 1 - k;
             for (const auto& c : coef) plan.poly.push_back(power < c.size() ? c[power] : 0.0);

@@ -57,22 +57,18 @@ ccr_path_kernel(PathArgs<Real> a) {
   Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
   simulatePath<Real, Sh, false, kOptions>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
     if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
-    // Without intensities E[Q] is the count of active lanes and the CVA / DVA increments are 0:
-    // only the first six statistics need wavefront sums.
-    constexpr int kSummed = (Sh::kExact && Sh::NC == 0) ? 6 : kStats;
+    // The first eight statistics in one transposed wavefront sum (lanes 0..7 store one each);
+    // the DVA increment is 0 without intensities (Q = 1), else a wavefront sum of its own
+    // (stored by lane 8 either way).
+    double vals[8];
 #pragma unroll
-    for (int k = 0; k < kSummed; ++k) {
-      const double s = waveSum(active ? st9[k] : 0.0);
-      if ((tid & 63) == 0) part[rr * kStats + k] = s;
-    }
-    if constexpr (kSummed < kStats) {
-      const double lanes = static_cast<double>(__popcll(__ballot(active)));
-      if ((tid & 63) == 0) {
-        part[rr * kStats + 6] = lanes;
-        part[rr * kStats + 7] = 0.0;
-        part[rr * kStats + 8] = 0.0;
-      }
-    }
+    for (int k = 0; k < 8; ++k) vals[k] = active ? st9[k] : 0.0;
+    int idx;
+    const double s = waveSumMany<8>(vals, &idx);
+    double dva = 0.0;
+    if constexpr (!(Sh::kExact && Sh::NC == 0)) dva = waveSum(active ? st9[8] : 0.0);
+    const int lane = tid & 63;
+    if (lane < 9) part[rr * kStats + (lane < 8 ? idx : 8)] = lane < 8 ? s : dva;
   });
 }
 

@@ -31,6 +31,44 @@ __device__ inline double waveSum(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// Wavefront sums of N values per lane, in a fixed order, for the cost of about one: each of
+// log2(NP) transpose steps (NP = N rounded up to a power of two) exchanges half of the values a
+// lane still carries with lane ^ m and keeps the other half (lanes with bit m set keep the upper
+// half), so the values per lane halve while the lanes sharing a value index double; the one value
+// left is then summed across the remaining lane bits. Lane L ends with the total of value index
+// bitrev(L mod NP) (log2(NP) bits), which is written to *index (-1 past the padding values).
+// Every lane of a value index holds the same total (xor butterflies add commutatively).
+template <int N>
+__device__ inline double waveSumMany(const double (&in)[N], int* index) {
+  constexpr int NP = N <= 1 ? 1 : N <= 2 ? 2 : N <= 4 ? 4 : N <= 8 ? 8 : 16;
+  constexpr int LB = NP == 1 ? 0 : NP == 2 ? 1 : NP == 4 ? 2 : NP == 8 ? 3 : 4;
+  static_assert(N <= 16, "waveSumMany: at most 16 values");
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  double v[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) v[i] = i < N ? in[i] : 0.0;
+#pragma unroll
+  for (int l = 0; l < LB; ++l) {
+    const int m = 1 << l, half = NP >> (l + 1);
+    const bool hi = (lane & m) != 0;
+#pragma unroll
+    for (int i = 0; i < half; ++i) {
+      const double keep = hi ? v[i + half] : v[i];
+      const double send = hi ? v[i] : v[i + half];
+      v[i] = keep + __shfl_xor(send, m);
+    }
+  }
+  double s = v[0];
+#pragma unroll
+  for (int m = NP; m < 64; m <<= 1) s += __shfl_xor(s, m);
+  int idx = 0;
+#pragma unroll
+  for (int l = 0; l < LB; ++l)
+    if (lane & (1 << l)) idx |= NP >> (l + 1);
+  *index = idx < N ? idx : -1;
+  return s;
+}
+
 // Sums N values over a workgroup of `waves` wavefronts (<= 16) in a fixed order: DPP within
 // each wave, then wave 0, 1, ... in order. `lds` holds waves x N doubles. The totals are valid
 // in thread 0.

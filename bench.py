@@ -60,6 +60,19 @@ WORKLOAD_TEXT = {
 }
 
 
+def measured_traffic(workload: str, precision: str):
+    """HBM bytes per path-kernel launch from the committed counter summary of this workload and
+    precision (FETCH_SIZE and WRITE_SIZE passes of rocprofv3, each scaled by the copy-kernel
+    calibration stated there), or None when no summary holds the line 'traffic_bytes: N'."""
+    f = ROOT / "profiles" / f"path_kernel_{workload}_{precision}.md"
+    if not f.exists():
+        return None, None
+    for line in f.read_text().splitlines():
+        if line.strip().startswith("traffic_bytes:"):
+            return float(line.split(":", 1)[1].split()[0]), str(f.relative_to(ROOT))
+    return None, None
+
+
 def describe(spec: abi.Spec, info) -> dict:
     return {
         "workload": WORKLOAD_TEXT.get(spec.name, spec.name),
@@ -181,6 +194,10 @@ def main():
             "algorithmic_bytes": path_bytes,
         },
     }
+    traffic, source = measured_traffic(args.workload, args.precision)
+    if traffic is not None:
+        result["roofline"]["traffic"] = traffic
+        result["roofline"]["traffic_source"] = source
     plan.close()
     if not args.no_fp32 and prec == abi.CCR_FP64:
         p32 = engine.compile(spec, abi.CCR_FP32)

@@ -131,6 +131,7 @@ __device__ __noinline__ double lerpUnfused(double lo, double t, double hi) {
 constexpr int kPfeThreads = 256;
 constexpr int kPfeWaves = kPfeThreads / 64;
 constexpr int kPfeKeys = 4096;   // LDS keys: the sample, then the keys between the splitters
+constexpr int kPfeSample = 2048; // smaller sample of a row longer than kPfeKeys
 constexpr int kPfeBins = 4096;   // 12-bit digits of the radix fallback (histogram aliases the keys)
 constexpr int kPfeUnroll = 4;    // row elements in flight per thread
 
@@ -222,9 +223,10 @@ __device__ inline void pfeFindBin(const unsigned int* hist, long long rank, unsi
 
 // One workgroup per (set, reporting date) row: exact order statistics lo = floor(h), hi = lo + 1
 // of the row's keys (h = q (P - 1)), then the interpolation of ccr::quantile.
-//  1. The first min(P, 4096) keys of the row (paths are exchangeable) are sorted in LDS; a row
-//     that short is done. Otherwise two sample keys a <= b, 4.5 binomial standard deviations (+2)
-//     either side of the sample's q-quantile, bracket the row's statistics with high probability.
+//  1. A row of at most 4096 keys is sorted in LDS and done. Otherwise its first 2048 keys (4096
+//     when 2048 would leave too many keys between the splitters; paths are exchangeable) are
+//     sorted, and two sample keys a <= b, 4.5 binomial standard deviations (+2) either side of the
+//     sample's q-quantile, bracket the row's statistics with high probability.
 //  2. One pass over the row counts keys < a, == a and == b, keeps the smallest key > b and
 //     gathers the keys strictly between a and b into LDS (wave-aggregated slots).
 //  3. If lo falls in [a, b] and the gathered keys fit, they are sorted and the statistics read off
@@ -257,7 +259,10 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
   };
 
   // 1. Sample.
-  const int ns = num_paths < kPfeKeys ? num_paths : kPfeKeys;
+  // The smaller sample unless its splitters would bracket more than 60% of the key buffer.
+  auto bracketed = [&](double n) { return 2.0 * (4.5 * sqrt(n * level * (1.0 - level)) + 2.0) / n * num_paths; };
+  const int ns = num_paths <= kPfeKeys ? num_paths
+                 : bracketed(kPfeSample) <= 0.6 * kPfeKeys ? kPfeSample : kPfeKeys;
   int n2 = 1;
   while (n2 < ns) n2 <<= 1;
   for (int i = tid; i < n2; i += kPfeThreads) keys[i] = i < ns ? keyOf<Real>(row[i]) : ~K(0);

@@ -63,13 +63,7 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
   for (size_t si = 0; si < h.sets.size(); ++si) {
     const SetDev& S = h.sets[si];
     if (S.state_off < 0) continue;
-    for (int p = 0; p < P; ++p) {
-      std::fill(scratch.begin(), scratch.end(), 0.0);
-      withShape(h.g.nrates, h.g.nassets, h.g.ncredits, [&](auto shape) {
-        simulatePath<double, decltype(shape), true>(s, &S, p, scratch.data(), 1, [](int, const double*, double) {});
-        return 0;
-      });
-    }
+    for (int p = 0; p < P; ++p) simulateStates<double>(s, &S, p, true);
   }
   const double* X = states.data();
   const double* Dv = states.data() + h.state_len;

@@ -55,7 +55,7 @@ ccr_path_kernel(PathArgs<Real> a) {
   double* part = a.partials + S.partial_off + static_cast<int64_t>(waveLocal) * a.max_rep * kStats;
   Real* rows = a.rows + S.rows_off;
   Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
-  simulatePath<Real, Sh, false, kOptions>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
+  simulatePath<Real, Sh, kOptions>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
     if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
     // The first eight statistics in one transposed wavefront sum (lanes 0..7 store one each);
     // the DVA increment is 0 without intensities (Q = 1), else a wavefront sum of its own
@@ -72,20 +72,17 @@ ccr_path_kernel(PathArgs<Real> a) {
   });
 }
 
-// States-only pass for American Monte Carlo: the same simulation, recording the domestic state
-// and the deflator of every date of the sets that hold Bermudan swaptions.
-template <typename Real, typename Sh>
+// States pass for American Monte Carlo: the domestic state and the deflator of every date of the
+// sets that hold Bermudan swaptions (simulateStates: the domestic rate only).
+template <typename Real>
 __global__ void __launch_bounds__(kMaxBlock) ccr_states_kernel(PathArgs<Real> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
   const int b = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
   cptr<SetDev> sp = toConst(a.sets) + toConst(a.block_set)[b];
   if (sp->state_off < 0) return;
   const int bd = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
   const int p = (b - sp->block_begin) * bd + tid;
   const int paths = toConst(a.g)->num_paths;
-  const int pe = p < paths ? p : paths - 1;
-  Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
-  simulatePath<Real, Sh, true, false>(a, sp, pe, col, bd, [](int, const double*, double) {});
+  simulateStates<Real>(a, sp, p < paths ? p : paths - 1, p < paths);  // lanes past the last path replay it, write nothing
 }
 
 // Sum of the per-wave partials of every (set, reporting date, statistic), in wave order.
@@ -454,10 +451,8 @@ template hipError_t pathOccupancy<float>(int, int, int, bool, int, int, int*);
 
 template <typename Real>
 hipError_t launchStates(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
-  return withShape(args.nrates, args.nassets, args.ncredits, [&](auto shape) {
-    hipLaunchKernelGGL((ccr_states_kernel<Real, decltype(shape)>), dim3(blocks), dim3(block), 0, stream, args);
-    return hipGetLastError();
-  });
+  hipLaunchKernelGGL(ccr_states_kernel<Real>, dim3(blocks), dim3(block), 0, stream, args);
+  return hipGetLastError();
 }
 template hipError_t launchStates<double>(const PathArgs<double>&, int, int, hipStream_t);
 template hipError_t launchStates<float>(const PathArgs<float>&, int, int, hipStream_t);

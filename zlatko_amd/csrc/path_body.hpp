@@ -224,16 +224,15 @@ inline auto withShape(int nrates, int nassets, int ncredits, F&& f) {
 // rows [0, ring) hold the margin-call values, [ring, ring + priceRows) the collateral prices,
 // then the fixing states. emit(rep, stats[kStats], positive_exposure) is called on every
 // reporting date.
-// kStatesOnly: the AMC states pass, which only calls onState(j, x, D) on every date.
 // kOptions: the plan holds European options (the Black evaluation is compiled in only then, so
 // option-free plans keep a compact loop).
-template <typename Real, typename Sh, bool kStatesOnly, bool kOptions, typename Emit, typename OnState>
+template <typename Real, typename Sh, bool kOptions, typename Emit>
 __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
                                              cptr<int32_t> fstore, cptr<double> poly, cptr<Real> opts,
                                              const double* bermv, int ring,
-                                             int priceRows, Real* col, int stride, Emit&& emit, OnState&& onState) {
+                                             int priceRows, Real* col, int stride, Emit&& emit) {
   constexpr bool kDouble = std::is_same<Real, double>::value;
   constexpr int NR = Sh::NR, NA = Sh::NA, NC = Sh::NC;
   const CCR_CONST GlobalDev& g = *gp;
@@ -380,10 +379,6 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
       }
     }
 
-    if constexpr (kStatesOnly) {
-      onState(j, x[0], D);
-      continue;
-    }
     const DateRec rec = dates[j];
     for (int i = rec.fixstore_beg; i < rec.fixstore_end; ++i) {
       const int r = fstore[2 * i], slot = fstore[2 * i + 1];
@@ -565,34 +560,73 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
   }
 }
 
-// Convenience: the tables of set S for path pe, from a PathArgs bundle. kStatesOnly: the AMC
-// states pass, writing x and D of every date into a.states ([state_off + j P + path], D after
-// a.state_len elements).
-template <typename Real, typename Sh, bool kStatesOnly = false, bool kOptions = true, typename Emit>
+// Convenience: the tables of set S for path pe, from a PathArgs bundle.
+template <typename Real, typename Sh, bool kOptions = true, typename Emit>
 __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, Real* col, int stride,
                                              Emit&& emit) {
   const CCR_CONST SetDev& S = *sp;
   cptr<GlobalDev> g = toConst(a.g);
   const int64_t q = g->antithetic ? (pe >> 1) : pe;
   const double* bermv = S.berm_end > S.berm_beg ? a.bermv : nullptr;
-  double* states = a.states;
-  const int64_t stateOff = S.state_off, paths = g->num_paths, stateLen = a.state_len;
-  auto onState = [&](int j, Real x, Real D) {
-    if (states && stateOff >= 0 && pe < paths) {
-      const int64_t at = stateOff + static_cast<int64_t>(j) * paths + pe;
-      states[at] = static_cast<double>(x);
-      states[stateLen + at] = static_cast<double>(D);
+  simulatePath<Real, Sh, kOptions>(g, sp, pe, a.stream + q * S.stream_stride,
+                                   toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride,
+                                   toConst(a.dates) + S.date_off,
+                                   toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
+                                   toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
+                                   toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
+                                   toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off,
+                                   toConst(a.opts) + kOptStride * static_cast<int64_t>(S.opt_off), bermv, a.ring,
+                                   a.price_rows, col, stride, emit);
+}
+
+// The AMC states pass for path pe of set S: the domestic state x and the deflator D of every
+// date, written to a.states ([state_off + j P + pe], D after a.state_len elements) when write
+// is set. Only the domestic rate is simulated: its driver is the first correlated normal (the
+// Cholesky factor's first row has one entry) plus its integral normal, and x and D depend on
+// nothing else; the arithmetic is that of simulatePath's domestic rate, so the values are the
+// ones the exposure pass sees.
+template <typename Real>
+__host__ __device__ inline void simulateStates(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, bool write) {
+  constexpr bool kDouble = std::is_same<Real, double>::value;
+  const CCR_CONST SetDev& S = *sp;
+  const CCR_CONST GlobalDev& g = *toConst(a.g);
+  if (S.state_off < 0 || !a.states) return;
+  const int64_t paths = g.num_paths;
+  const int64_t q = g.antithetic ? (pe >> 1) : pe;
+  const Real* zb = a.stream + q * S.stream_stride;
+  cptr<Real> steps = toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride;
+  const int nz = g.nz, nCorr = g.ncorr;
+  const Real sgn = (g.antithetic && (pe & 1)) ? Real(-1) : Real(1);
+  const Real c00 = static_cast<Real>(g.chol[0]);
+  double* xs = a.states + S.state_off + pe;
+  double* ds = xs + a.state_len;
+  Real x = 0, D = 1, lnD = 0, lnDc = 0;
+  for (int j = 0; j < S.num_dates; ++j) {
+    if (j > 0) {
+      cptr<Real> o = steps + static_cast<int64_t>(j) * kStepStride + kStepRate;
+      const Real* z = zb + static_cast<int64_t>(j - 1) * nz;
+      const Real z1 = sgn * z[0], z2 = sgn * z[nCorr];
+      Real w = 0;
+      w += c00 * z1;
+      const Real x0 = x;
+      x = x0 * o[0] + o[1] * w + o[5];
+      const Real ix = x0 * o[4] + o[2] * w + o[3] * z2 + o[6];
+      const Real ir = ix + o[7];
+      if constexpr (kDouble) {
+        D *= exp(-ir);
+      } else {
+        const Real yk = ir - lnDc;
+        const Real tk = lnD + yk;
+        lnDc = (tk - lnD) - yk;
+        lnD = tk;
+        D = expr(-lnD);
+      }
     }
-  };
-  simulatePath<Real, Sh, kStatesOnly, kOptions>(g, sp, pe, a.stream + q * S.stream_stride,
-                                      toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride,
-                                      toConst(a.dates) + S.date_off,
-                                      toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
-                                      toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
-                                      toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
-                                      toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off,
-                                      toConst(a.opts) + kOptStride * static_cast<int64_t>(S.opt_off), bermv,
-                                      a.ring, a.price_rows, col, stride, emit, onState);
+    if (write) {
+      xs[static_cast<int64_t>(j) * paths] = static_cast<double>(x);
+      ds[static_cast<int64_t>(j) * paths] = static_cast<double>(D);
+    }
+  }
 }
 
 }  // namespace ccrhip

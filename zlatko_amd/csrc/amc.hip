@@ -230,6 +230,8 @@ __global__ void __launch_bounds__(256) amc_cfin1_kernel(AmcArgs a) {
   for (int i = 0; i < 3; ++i) a.csum[3 * d + i] = s[i];
 }
 
+// Gram partials of the shared design phi(z) per continuation date and path chunk (the design
+// itself, W = phi(z) / D, is formed where the continuation GEMM loads it).
 __global__ void __launch_bounds__(kAmcChunk) amc_design_kernel(AmcArgs a) {
   __shared__ double lds[kWaves * 10];
   const int d = static_cast<int>(blockIdx.y);
@@ -241,16 +243,13 @@ __global__ void __launch_bounds__(kAmcChunk) amc_design_kernel(AmcArgs a) {
 #pragma unroll
   for (int i = 0; i < 10; ++i) v[i] = 0.0;
   if (p < a.paths) {
-    const int64_t at = S.state_off + static_cast<int64_t>(j) * a.paths + p;
-    const double x = a.states[at], D = a.states[a.state_len + at];
+    const double x = a.states[S.state_off + static_cast<int64_t>(j) * a.paths + p];
     double phi[4];
     regRow((x - r.center) / r.scale, phi);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      a.W[(static_cast<int64_t>(d) * 4 + i) * a.paths + p] = phi[i] / D;
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int c = 0; c <= i; ++c) v[triIndex(i, c)] = phi[i] * phi[c];
-    }
   }
   blockSum<10>(v, lds, kWaves);
   if (threadIdx.x == 0)
@@ -267,8 +266,9 @@ __global__ void __launch_bounds__(256) amc_gram_kernel(AmcArgs a) {
 }
 
 // Continuation right-hand sides of one tile over one chunk of paths: rows (date dd, basis a) of
-// W against the strategy column of each swaption in the tile. Wavefront w owns rows 8w .. 8w+7,
-// lane c swaption c; paths are staged 64 at a time in LDS (W rows are read as broadcasts).
+// the design W = phi(z) / D against the strategy column of each swaption in the tile. Wavefront
+// w owns rows 8w .. 8w+7, lane c swaption c; paths are staged 64 at a time in LDS, the design
+// rows formed there from the recorded states (x, D) of the tile's dates.
 constexpr int kGStep = 64;
 __global__ void __launch_bounds__(256) amc_cgemm_kernel(AmcArgs a) {
   __shared__ double Ws[4 * kGemmDates][kGStep];
@@ -278,11 +278,17 @@ __global__ void __launch_bounds__(256) amc_cgemm_kernel(AmcArgs a) {
   const int tid = static_cast<int>(threadIdx.x), w = tid >> 6, c = tid & 63;
   const int64_t kBeg = static_cast<int64_t>(kc) * kGemmK;
   const int64_t kEnd = kBeg + kGemmK < a.paths ? kBeg + kGemmK : a.paths;
-  const int rows = 4 * t.nd;
-  // Loader roles: W row tid / 8, 8 paths from (tid % 8) * 8; S column tid / 4, 16 paths from (tid % 4) * 16.
-  const int wr = tid >> 3, wp = (tid & 7) * 8;
+  // Loader roles: design date tid / 32 (its 4 rows), paths 2 (tid % 32) and 2 (tid % 32) + 1 of
+  // each step; S column tid / 4, 16 paths from (tid % 4) * 16.
+  const int wd = tid >> 5, wp = (tid & 31) * 2;
   const int sc = tid >> 2, spo = (tid & 3) * 16;
-  const double* wsrc = wr < rows ? a.W + (static_cast<int64_t>(t.d0) * 4 + wr) * a.paths : nullptr;
+  const double* xsrc = nullptr;
+  RegParams rp;
+  if (wd < t.nd) {
+    const int d = t.d0 + wd;
+    xsrc = a.states + toConst(a.sets)[a.cdates[2 * d]].state_off + static_cast<int64_t>(a.cdates[2 * d + 1]) * a.paths;
+    rp = regParams(a.csum[3 * d], a.csum[3 * d + 1], a.csum[3 * d + 2], 3);
+  }
   const int col = sc < t.nb ? a.gcols[t.col_off + sc] : -1;
   const double* ssrc = col >= 0 ? a.strategy + static_cast<int64_t>(col) * a.paths : nullptr;
   double acc[8];
@@ -290,9 +296,16 @@ __global__ void __launch_bounds__(256) amc_cgemm_kernel(AmcArgs a) {
   for (int i = 0; i < 8; ++i) acc[i] = 0.0;
   for (int64_t k0 = kBeg; k0 < kEnd; k0 += kGStep) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < 2; ++i) {
       const int64_t k = k0 + wp + i;
-      Ws[wr][wp + i] = (wsrc && k < kEnd) ? wsrc[k] : 0.0;
+      double phi[4] = {0.0, 0.0, 0.0, 0.0};
+      double D = 1.0;
+      if (xsrc && k < kEnd) {
+        regRow((xsrc[k] - rp.center) / rp.scale, phi);
+        D = xsrc[a.state_len + k];
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) Ws[4 * wd + b][wp + i] = phi[b] / D;
     }
 #pragma unroll
     for (int i = 0; i < 16; ++i) {

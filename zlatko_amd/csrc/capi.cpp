@@ -271,7 +271,6 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.csum = dalloc(std::max(1, ncd) * 3);
     a.gram = dalloc(std::max(1, ncd) * 10);
     const int nkc = static_cast<int>((P + kGemmK - 1) / kGemmK);
-    a.W = ncd > 0 ? dalloc(static_cast<int64_t>(ncd) * 4 * P) : nullptr;
     a.tiles = h.tiles.empty() ? nullptr : upload(ctx, plan.get(), h.tiles.data(), h.tiles.size());
     a.gcols = h.gcols.empty() ? nullptr : upload(ctx, plan.get(), h.gcols.data(), h.gcols.size());
     a.gpart = dalloc(static_cast<int64_t>(nkc) * static_cast<int64_t>(h.bdates.size()) * 4);

@@ -73,7 +73,6 @@ struct AmcArgs {
   double* coef;               // [Bermudan][4]
   double* csum;               // [cdate][3]: n, sum x, sum x^2
   double* gram;               // [cdate][10]
-  double* W;
   const GemmTile* tiles;
   const int32_t* gcols;       // strategy column per (tile, swaption), -1: none
   double* gpart;

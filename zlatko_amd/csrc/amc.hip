@@ -202,30 +202,34 @@ __global__ void __launch_bounds__(kAmcChunk) amc_update_kernel(AmcArgs a, int k)
   a.strategy[static_cast<int64_t>(bm.ev_off + k) * a.paths + p] = realised;
 }
 
-// Continuation dates: moments of x over all paths, the shared design W, the shared Gram.
+// Continuation dates: moments of x over all paths and the Gram of the shared design. A
+// workgroup covers kContGroup consecutive path chunks of one date (each thread sums its path of
+// every chunk in order), so the grid stays small for plans with many continuation dates.
 __global__ void __launch_bounds__(kAmcChunk) amc_cmom1_kernel(AmcArgs a) {
   __shared__ double lds[kWaves * 3];
   const int d = static_cast<int>(blockIdx.y);
   const CCR_CONST SetDev& S = toConst(a.sets)[a.cdates[2 * d]];
-  const int j = a.cdates[2 * d + 1];
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kAmcChunk + threadIdx.x;
+  const double* xs = a.states + S.state_off + static_cast<int64_t>(a.cdates[2 * d + 1]) * a.paths;
   double v[3] = {0, 0, 0};
-  if (p < a.paths) {
-    const double x = a.states[S.state_off + static_cast<int64_t>(j) * a.paths + p];
-    v[0] = 1.0;
-    v[1] = x;
-    v[2] = x * x;
+  for (int g = 0; g < kContGroup; ++g) {
+    const int64_t p = (static_cast<int64_t>(blockIdx.x) * kContGroup + g) * kAmcChunk + threadIdx.x;
+    if (p < a.paths) {
+      const double x = xs[p];
+      v[0] += 1.0;
+      v[1] += x;
+      v[2] += x * x;
+    }
   }
   blockSum<3>(v, lds, kWaves);
   if (threadIdx.x == 0)
-    for (int i = 0; i < 3; ++i) a.part[(static_cast<int64_t>(d) * a.nchunks + blockIdx.x) * 3 + i] = v[i];
+    for (int i = 0; i < 3; ++i) a.part[(static_cast<int64_t>(d) * a.ncchunks + blockIdx.x) * 3 + i] = v[i];
 }
 
 __global__ void __launch_bounds__(256) amc_cfin1_kernel(AmcArgs a) {
   const int d = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (d >= a.ncdates) return;
   double s[3];
-  chunkSum<3>(a.part, d, a.nchunks, s);
+  chunkSum<3>(a.part, d, a.ncchunks, s);
   if ((threadIdx.x & 63) != 0) return;
   for (int i = 0; i < 3; ++i) a.csum[3 * d + i] = s[i];
 }
@@ -238,29 +242,31 @@ __global__ void __launch_bounds__(kAmcChunk) amc_design_kernel(AmcArgs a) {
   const CCR_CONST SetDev& S = toConst(a.sets)[a.cdates[2 * d]];
   const int j = a.cdates[2 * d + 1];
   const RegParams r = regParams(a.csum[3 * d], a.csum[3 * d + 1], a.csum[3 * d + 2], 3);
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kAmcChunk + threadIdx.x;
+  const double* xs = a.states + S.state_off + static_cast<int64_t>(j) * a.paths;
   double v[10];
 #pragma unroll
   for (int i = 0; i < 10; ++i) v[i] = 0.0;
-  if (p < a.paths) {
-    const double x = a.states[S.state_off + static_cast<int64_t>(j) * a.paths + p];
-    double phi[4];
-    regRow((x - r.center) / r.scale, phi);
+  for (int g = 0; g < kContGroup; ++g) {
+    const int64_t p = (static_cast<int64_t>(blockIdx.x) * kContGroup + g) * kAmcChunk + threadIdx.x;
+    if (p < a.paths) {
+      double phi[4];
+      regRow((xs[p] - r.center) / r.scale, phi);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int c = 0; c <= i; ++c) v[triIndex(i, c)] = phi[i] * phi[c];
+        for (int c = 0; c <= i; ++c) v[triIndex(i, c)] += phi[i] * phi[c];
+    }
   }
   blockSum<10>(v, lds, kWaves);
   if (threadIdx.x == 0)
-    for (int i = 0; i < 10; ++i) a.part[(static_cast<int64_t>(d) * a.nchunks + blockIdx.x) * 10 + i] = v[i];
+    for (int i = 0; i < 10; ++i) a.part[(static_cast<int64_t>(d) * a.ncchunks + blockIdx.x) * 10 + i] = v[i];
 }
 
 __global__ void __launch_bounds__(256) amc_gram_kernel(AmcArgs a) {
   const int d = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (d >= a.ncdates) return;
   double s[10];
-  chunkSum<10>(a.part, d, a.nchunks, s);
+  chunkSum<10>(a.part, d, a.ncchunks, s);
   if ((threadIdx.x & 63) != 0) return;
   for (int i = 0; i < 10; ++i) a.gram[10 * d + i] = s[i];
 }
@@ -456,7 +462,7 @@ hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int
     if (hipError_t e = last()) return e;
   }
   if (a.ncdates > 0) {
-    const dim3 gd(a.nchunks, a.ncdates), gds((a.ncdates + 3) / 4);
+    const dim3 gd(a.ncchunks, a.ncdates), gds((a.ncdates + 3) / 4);
     hipLaunchKernelGGL(amc_cmom1_kernel, gd, blk, 0, stream, a);
     hipLaunchKernelGGL(amc_cfin1_kernel, gds, dim3(256), 0, stream, a);
     hipLaunchKernelGGL(amc_design_kernel, gd, blk, 0, stream, a);

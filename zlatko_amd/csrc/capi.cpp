@@ -286,6 +286,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.ncdates = ncd;
     a.nbdates = static_cast<int>(h.bdates.size());
     a.nchunks = nchunks;
+    a.ncchunks = (nchunks + kContGroup - 1) / kContGroup;
     a.ntiles = static_cast<int>(h.tiles.size());
     a.nkc = nkc;
   }

@@ -81,6 +81,7 @@ struct AmcArgs {
   double* bermv;
   int64_t paths, state_len;
   int32_t nberms, ncols, ncdates, nbdates, nchunks, ntiles, nkc;
+  int32_t ncchunks;           // continuation-date partials per date: ceil(nchunks / kContGroup)
 };
 
 template <typename Real>

@@ -89,6 +89,7 @@ constexpr int kPolyMaxDegree = 48;
 // netting-set value.
 constexpr int kMaxExercises = 64;
 constexpr int kAmcChunk = 256;   // paths per workgroup of the regression reductions
+constexpr int kContGroup = 16;   // path chunks per workgroup of the continuation-date reductions
 constexpr int kRegMom1 = 4;      // masked moments: n, sum x, sum x^2, sum y
 constexpr int kRegMom2 = 14;     // Gram lower triangle (10) + X'y (4)
 constexpr int kContStride = 8;   // per (Bermudan, date): center, scale, c0..c3, degree, pad

@@ -50,6 +50,7 @@ struct ccr_hip_plan {
   SetDev* d_sets = nullptr;
   int32_t* d_block_set = nullptr;
   void* d_stream = nullptr;
+  void* d_dom = nullptr;     // domestic normals of the AMC states pass
   void* d_steps = nullptr;
   DateRec* d_dates = nullptr;
   void* d_date_reals = nullptr;
@@ -203,6 +204,9 @@ int occupancyBlock(const HostPlan& h) {
   return best;
 }
 
+template <typename Real>
+PathArgs<Real> pathArgs(const ccr_hip_plan* plan);
+
 void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
                  const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
                  double pfe_quantile, int32_t precision, ccr_hip_plan** out) {
@@ -293,6 +297,16 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   plan->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   uploadStream(ctx, plan.get(), config->seed, fp32);
+  if (h.dom_len > 0) {
+    plan->d_dom = devAlloc(ctx, plan.get(), static_cast<size_t>(h.dom_len) * elem);
+    const int n = static_cast<int>(h.sets.size());
+    const hipError_t e = fp32 ? launchDomesticNormals<float>(pathArgs<float>(plan.get()), h.sets.data(), n,
+                                                             static_cast<float*>(plan->d_dom), ctx->stream)
+                              : launchDomesticNormals<double>(pathArgs<double>(plan.get()), h.sets.data(), n,
+                                                              static_cast<double*>(plan->d_dom), ctx->stream);
+    check(e, "domestic normals gather");
+    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  }
   *out = plan.release();
 }
 
@@ -304,6 +318,7 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   a.sets = plan->d_sets;
   a.block_set = plan->d_block_set;
   a.stream = static_cast<const Real*>(plan->d_stream);
+  a.dom = static_cast<const Real*>(plan->d_dom);
   a.steps = static_cast<const Real*>(plan->d_steps);
   a.dates = plan->d_dates;
   a.date_reals = static_cast<const Real*>(plan->d_date_reals);

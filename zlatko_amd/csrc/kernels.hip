@@ -16,6 +16,7 @@
 
 #include <cstdint>
 #include <type_traits>
+#include <vector>
 
 #include "path_body.hpp"
 #include "wave.hpp"
@@ -448,6 +449,57 @@ hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, in
 }
 template hipError_t pathOccupancy<double>(int, int, int, bool, int, int, int*);
 template hipError_t pathOccupancy<float>(int, int, int, bool, int, int, int*);
+
+// dom[dom_off + (2 s + i) Q + q] = stream[q L + s nz + (i ? ncorr : 0)] for every set with a
+// domestic-normals slice, step s < T - 1 and path block q < Q (blockIdx.z: the set's position
+// among those sets).
+template <typename Real>
+__global__ void __launch_bounds__(256) ccr_domestic_normals_kernel(const SetDev* sets, const int32_t* which,
+                                                                    const Real* stream, Real* dom, int64_t Q, int nz,
+                                                                    int ncorr) {
+  const SetDev& S = sets[which[blockIdx.z]];
+  const int s = static_cast<int>(blockIdx.y);
+  if (s >= S.num_dates - 1) return;
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (q >= Q) return;
+  const Real* z = stream + q * S.stream_stride + static_cast<int64_t>(s) * nz;
+  Real* d = dom + S.dom_off + static_cast<int64_t>(2 * s) * Q + q;
+  d[0] = z[0];
+  d[Q] = z[ncorr];
+}
+
+template <typename Real>
+hipError_t launchDomesticNormals(const PathArgs<Real>& args, const SetDev* hostSets, int nsets, Real* dom,
+                                 hipStream_t stream) {
+  std::vector<int32_t> which;
+  int maxDates = 0;
+  for (int s = 0; s < nsets; ++s)
+    if (hostSets[s].dom_off >= 0) {
+      which.push_back(s);
+      maxDates = hostSets[s].num_dates > maxDates ? hostSets[s].num_dates : maxDates;
+    }
+  if (which.empty() || maxDates < 2) return hipSuccess;
+  GlobalDev g;
+  hipError_t e = hipMemcpy(&g, args.g, sizeof(GlobalDev), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
+  const int64_t Q = g.antithetic ? (static_cast<int64_t>(g.num_paths) + 1) / 2 : g.num_paths;
+  int32_t* dwhich = nullptr;
+  if ((e = hipMalloc(&dwhich, which.size() * sizeof(int32_t))) != hipSuccess) return e;
+  e = hipMemcpy(dwhich, which.data(), which.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    for (size_t b = 0; b < which.size() && e == hipSuccess; b += 65535) {
+      const unsigned nz3 = static_cast<unsigned>(which.size() - b < 65535 ? which.size() - b : 65535);
+      hipLaunchKernelGGL(ccr_domestic_normals_kernel<Real>, dim3(static_cast<unsigned>((Q + 255) / 256), maxDates - 1, nz3),
+                         dim3(256), 0, stream, args.sets, dwhich + b, args.stream, dom, Q, g.nz, g.ncorr);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  }
+  const hipError_t f = hipFree(dwhich);
+  return e != hipSuccess ? e : f;
+}
+template hipError_t launchDomesticNormals<double>(const PathArgs<double>&, const SetDev*, int, double*, hipStream_t);
+template hipError_t launchDomesticNormals<float>(const PathArgs<float>&, const SetDev*, int, float*, hipStream_t);
 
 template <typename Real>
 hipError_t launchStates(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {

@@ -29,6 +29,7 @@ struct PathArgs {
   const SetDev* sets;
   const int32_t* block_set;
   const Real* stream;
+  const Real* dom;     // AMC states pass: per set [step][2][path block] the domestic driver and integral normals
   const Real* steps;
   const DateRec* dates;
   const Real* date_reals;
@@ -105,6 +106,11 @@ hipError_t launchSurrogateScatter(const float* y, int64_t n, double mean, double
 
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);
+// Gathers the two normals per step the AMC states pass reads (the first correlated driver and the
+// domestic integral normal) of every set with Bermudans from the stream into args.dom.
+template <typename Real>
+hipError_t launchDomesticNormals(const PathArgs<Real>& args, const SetDev* hostSets, int nsets, Real* dom,
+                                 hipStream_t stream);
 // Workgroups of the path kernel resident per CU (registers, LDS, workgroup slots) for a model
 // shape, the option variant, ldsRows LDS rows per thread and a workgroup size.
 template <typename Real>

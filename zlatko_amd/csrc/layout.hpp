@@ -148,6 +148,7 @@ struct SetDev {
   int32_t cd_base;             // first continuation date (one per valued date) of this set's rows
   int32_t opt_off;             // into the option-term table
   int64_t state_off;           // first element of this set's [date][path] states (AMC), -1 without
+  int64_t dom_off;             // first element of its domestic normals [step][2][path block] (AMC), -1 without
   double thr_cpty, thr_own, mta, ia, keep;  // CSA terms (thr_own < 0 encodes a one-way CSA)
 };
 

@@ -593,9 +593,13 @@ __host__ __device__ inline void simulateStates(const PathArgs<Real>& a, cptr<Set
   if (S.state_off < 0 || !a.states) return;
   const int64_t paths = g.num_paths;
   const int64_t q = g.antithetic ? (pe >> 1) : pe;
-  const Real* zb = a.stream + q * S.stream_stride;
+  // The two normals of a step: from the gathered domestic normals ([step][2][path block]) when
+  // the plan holds them, else from the stream itself.
+  const int64_t Q = g.antithetic ? (paths + 1) / 2 : paths;
+  const bool gathered = a.dom && S.dom_off >= 0;
+  const Real* zb = gathered ? a.dom + S.dom_off + q : a.stream + q * S.stream_stride;
+  const int64_t zStep = gathered ? 2 * Q : g.nz, zSecond = gathered ? Q : g.ncorr;
   cptr<Real> steps = toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride;
-  const int nz = g.nz, nCorr = g.ncorr;
   const Real sgn = (g.antithetic && (pe & 1)) ? Real(-1) : Real(1);
   const Real c00 = static_cast<Real>(g.chol[0]);
   double* xs = a.states + S.state_off + pe;
@@ -604,8 +608,8 @@ __host__ __device__ inline void simulateStates(const PathArgs<Real>& a, cptr<Set
   for (int j = 0; j < S.num_dates; ++j) {
     if (j > 0) {
       cptr<Real> o = steps + static_cast<int64_t>(j) * kStepStride + kStepRate;
-      const Real* z = zb + static_cast<int64_t>(j - 1) * nz;
-      const Real z1 = sgn * z[0], z2 = sgn * z[nCorr];
+      const Real* z = zb + static_cast<int64_t>(j - 1) * zStep;
+      const Real z1 = sgn * z[0], z2 = sgn * z[zSecond];
       Real w = 0;
       w += c00 * z1;
       const Real x0 = x;

@@ -733,6 +733,8 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     };
     sd.berm_beg = static_cast<int32_t>(plan.berms.size());
     sd.state_off = nB > 0 ? plan.state_len : -1;
+    sd.dom_off = nB > 0 ? plan.dom_len : -1;
+    if (nB > 0) plan.dom_len += static_cast<int64_t>(T - 1) * 2 * (cfg.antithetic ? (static_cast<int64_t>(P) + 1) / 2 : P);
     if (nB > 0) plan.state_len += static_cast<int64_t>(T) * P;
     // Continuation regressions run on every valued date of a set with Bermudans.
     std::vector<int32_t> cdOf(static_cast<std::size_t>(T), -1);

@@ -40,6 +40,7 @@ struct HostPlan {
   std::vector<TermRange> evterms;   // per Bermudan exercise: the tail swap at the exercise date
   std::vector<int32_t> eidx;        // exercise date indices
   int64_t state_len = 0;            // elements of the x / D state arrays
+  int64_t dom_len = 0;              // domestic normals of the AMC states pass (both per step and path block)
   int64_t ev_len = 0;               // elements of the exercise-value / strategy arrays
   int32_t max_exercises = 0;
   int64_t cont_len = 0;             // kContStride x sum over Bermudans of their set's dates

@@ -105,20 +105,36 @@ def time_gpu(engine, plan, steps: int, warmup: int):
     return wall, float(np.mean(path_ms)), float(np.mean(total_ms)), engine.timing()
 
 
+def host_cpu() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return f"{line.split(':', 1)[1].strip()}, {os.cpu_count()} logical CPUs"
+    except OSError:
+        pass
+    return f"{os.cpu_count()} logical CPUs"
+
+
 def cpu_baseline(spec_fn, budget_s: float, threads: int):
-    """FP64 oracle on a bounded path sample of the same workload (all netting sets)."""
+    """FP64 oracle on a bounded path sample of the same workload (all netting sets): all host
+    threads (the reported value), then one thread on a smaller sample."""
     from tests import oracle_lib
-    probe = spec_fn(8)
-    t0 = time.perf_counter()
-    oracle_lib.run_oracle(probe, threads=threads)
-    per_path = (time.perf_counter() - t0) / 8
-    paths = int(max(8, min(2000, budget_s / max(per_path, 1e-9))))
-    paths -= paths % 2
-    spec = spec_fn(paths)
-    t0 = time.perf_counter()
-    oracle_lib.run_oracle(spec, threads=threads)
-    wall = time.perf_counter() - t0
-    tm = oracle_lib.oracle_timing()
+
+    def timed(nthreads: int, budget: float, probe_paths: int):
+        probe = spec_fn(probe_paths)
+        t0 = time.perf_counter()
+        oracle_lib.run_oracle(probe, threads=nthreads)
+        per_path = (time.perf_counter() - t0) / probe_paths
+        paths = int(max(probe_paths, min(2000, budget / max(per_path, 1e-9))))
+        paths -= paths % 2
+        spec = spec_fn(paths)
+        t0 = time.perf_counter()
+        oracle_lib.run_oracle(spec, threads=nthreads)
+        wall = time.perf_counter() - t0
+        return spec, paths, wall, oracle_lib.oracle_timing()
+
+    spec, paths, wall, tm = timed(threads, budget_s, 8)
+    _, paths1, wall1, tm1 = timed(1, budget_s / 3, 2)
     return {
         "value": tm.revaluations / wall,
         "unit": "revals/s",
@@ -126,7 +142,9 @@ def cpu_baseline(spec_fn, budget_s: float, threads: int):
         "kind": "port",
         "sample": f"FP64 oracle (ExposureEngine restated), same {len(spec.sets)} netting sets and grids, {paths} paths, "
                   f"{wall:.1f} s wall (draws {tm.draws_s:.2f} s, pricing {tm.pricing_s:.1f} s, "
-                  f"collateral {tm.collateral_s:.2f} s, statistics {tm.statistics_s:.2f} s)",
+                  f"collateral {tm.collateral_s:.2f} s, statistics {tm.statistics_s:.2f} s); host {host_cpu()}",
+        "value_1thread": tm1.revaluations / wall1,
+        "sample_1thread": f"same netting sets, {paths1} paths, {wall1:.1f} s wall on one thread",
     }
 
 

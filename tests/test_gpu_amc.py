@@ -110,3 +110,20 @@ def test_amc_bit_reproducible_across_block_sizes():
     for r in results[1:]:
         for k in abi.PROFILE_FIELDS:
             assert np.array_equal(r[k], results[0][k]), k
+
+
+@pytest.mark.parametrize("paths", [1, 2, 3, 5, 257])
+def test_amc_tiny_path_counts_fp64(engine, paths):
+    """Bermudans under CSA with 1..257 paths: one partial path chunk, a mirror path without its
+    pair, regressions whose degree the sample size lowers (down to the mean)."""
+    spec = W.small_reference_case(num_paths=paths, csa=True, bermudans=2)
+    _check(spec, engine.run(spec), oracle_lib.run_oracle(spec))
+
+
+@pytest.mark.parametrize("bermudans", [0, 2])
+def test_without_antithetics_fp64(engine, bermudans):
+    """Every path owns its normals (no mirror): the stream, the gathered domestic normals of the
+    states pass and the PFE rows are indexed by path instead of by path pair."""
+    spec = W.small_reference_case(num_paths=777, csa=True, fx=True, credits=True, bermudans=bermudans)
+    spec.config.antithetic = 0
+    _check(spec, engine.run(spec), oracle_lib.run_oracle(spec, threads=8))

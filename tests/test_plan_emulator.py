@@ -50,7 +50,13 @@ CASES = {
     "c4_bermudan": lambda: W.c4(num_paths=64, dates=24),
     "options": lambda: W.small_reference_case(120, fx=True, credits=True, options=6, bermudans=2),
     "c5_uneven": lambda: W.c5(num_paths=16),
+    "no_antithetic": lambda: no_antithetic(W.small_reference_case(77, csa=True, fx=True, credits=True, bermudans=2)),
 }
+
+
+def no_antithetic(spec):
+    spec.config.antithetic = 0
+    return spec
 
 
 def bermudan_case(num_paths):

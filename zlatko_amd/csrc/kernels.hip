@@ -6,7 +6,8 @@
 //                          ring, fixing states) lives in LDS columns. Reporting-date statistics
 //                          are reduced across the wavefront and written as per-wave partial sums;
 //                          max(V - C, 0) is written path-major-coalesced for the PFE.
-//  ccr_reduce_kernel       fixed-order sum of the per-wave partials (bit-reproducible).
+//  ccr_reduce_kernel       fixed-order sum of the per-wave partials (bit-reproducible across runs
+//                          and workgroup sizes).
 //  ccr_pfe_kernel<Real>    exact order statistics per (netting set, reporting date) on the IEEE
 //                          bit patterns of the non-negative exposures: splitters from a sorted
 //                          sample, one counting / gathering pass, an LDS sort of the keys between
@@ -86,17 +87,33 @@ __global__ void __launch_bounds__(kMaxBlock) ccr_states_kernel(PathArgs<Real> a)
   simulateStates<Real>(a, sp, p < paths ? p : paths - 1, p < paths);  // lanes past the last path replay it, write nothing
 }
 
-// Sum of the per-wave partials of every (set, reporting date, statistic), in wave order.
-__global__ void ccr_reduce_kernel(const SetDev* sets, const double* partials, int nsets, int max_rep, int waves_per_set,
-                                  double inv_paths, double* out) {
-  const int idx = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+// Sum of the per-wave partials of every (set, reporting date, statistic) in a fixed order that
+// does not depend on the path kernel's workgroup size: a workgroup takes 64 consecutive (date,
+// statistic) items of one set (coalesced loads), its 4 wavefronts sum the waves w = g mod 4 in
+// order, then the 4 partial sums are added in order g = 0..3. Waves past the last path hold
+// zeros, which leave every sum unchanged.
+constexpr int kReduceGroups = 4;
+__global__ void __launch_bounds__(64 * kReduceGroups) ccr_reduce_kernel(const SetDev* sets, const double* partials,
+                                                                        int nsets, int max_rep, int waves_per_set,
+                                                                        double inv_paths, double* out) {
+  __shared__ double lds[kReduceGroups][64];
   const int per = max_rep * kStats;
-  if (idx >= nsets * per) return;
-  const int s = idx / per, rk = idx % per;
-  const double* p = partials + sets[s].partial_off + rk;
+  const int blocksPerSet = (per + 63) / 64;
+  const int s = static_cast<int>(blockIdx.x) / blocksPerSet;
+  const int rk = (static_cast<int>(blockIdx.x) % blocksPerSet) * 64 + (static_cast<int>(threadIdx.x) & 63);
+  const int g = static_cast<int>(threadIdx.x) >> 6;
   double sum = 0.0;
-  for (int w = 0; w < waves_per_set; ++w) sum += p[static_cast<int64_t>(w) * per];
-  out[idx] = sum * inv_paths;
+  if (s < nsets && rk < per) {
+    const double* p = partials + sets[s].partial_off + rk;
+    for (int w = g; w < waves_per_set; w += kReduceGroups) sum += p[static_cast<int64_t>(w) * per];
+  }
+  lds[g][threadIdx.x & 63] = sum;
+  __syncthreads();
+  if (g == 0 && s < nsets && rk < per) {
+    double total = lds[0][threadIdx.x];
+    for (int k = 1; k < kReduceGroups; ++k) total += lds[k][threadIdx.x];
+    out[s * per + rk] = total * inv_paths;
+  }
 }
 
 template <typename Real> struct KeyOf;
@@ -511,9 +528,9 @@ template hipError_t launchStates<float>(const PathArgs<float>&, int, int, hipStr
 
 hipError_t launchReduce(const SetDev* sets, const double* partials, int nsets, int max_rep, int waves_per_set,
                         double inv_paths, double* out, hipStream_t stream) {
-  const int n = nsets * max_rep * kStats;
-  hipLaunchKernelGGL(ccr_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, sets, partials, nsets, max_rep,
-                     waves_per_set, inv_paths, out);
+  const int blocks = nsets * ((max_rep * kStats + 63) / 64);
+  hipLaunchKernelGGL(ccr_reduce_kernel, dim3(blocks), dim3(64 * kReduceGroups), 0, stream, sets, partials, nsets,
+                     max_rep, waves_per_set, inv_paths, out);
   return hipGetLastError();
 }
 

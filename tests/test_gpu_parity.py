@@ -223,3 +223,28 @@ def test_forced_failures_leave_engine_usable(engine):
             engine.run(spec)
         assert e.value.code == code
     _check(spec, engine.run(spec), oracle_lib.run_oracle(spec))
+
+
+def test_plans_sharing_a_context_stream_match_fresh_contexts():
+    """Plans of one context replay prefixes of one device stream per seed (generated once; FP32
+    plans narrow it on the device). Interleaved compiles of different lengths, seeds and
+    precisions give the same profiles as plans compiled on fresh contexts."""
+    from zlatko_amd import native
+    short = W.small_reference_case(num_paths=300, csa=True, fx=True)
+    long_ = W.small_reference_case(num_paths=900, csa=True, fx=True, bermudans=2)
+    other = W.small_reference_case(num_paths=300, csa=True, fx=True)
+    other.config.seed = 7
+    order = [(short, abi.CCR_FP64), (long_, abi.CCR_FP32), (other, abi.CCR_FP64), (short, abi.CCR_FP32),
+             (long_, abi.CCR_FP64), (short, abi.CCR_FP64)]
+    e = native.Engine()
+    plans = [e.compile(s, p) for s, p in order]
+    shared = [pl.execute()[0].as_dict() for pl in plans]
+    for pl in plans:
+        pl.close()
+    e.close()
+    for (s, p), got in zip(order, shared):
+        f = native.Engine()
+        want = f.run(s, p)[0].as_dict()
+        f.close()
+        for k in abi.PROFILE_FIELDS:
+            assert np.array_equal(got[k], want[k]), (s.num_paths, p, k)

@@ -20,6 +20,22 @@
 
 using namespace ccrhip;
 
+namespace {
+
+// One replayed normal stream in FP64 on the device (the reference's draws for a seed, first n),
+// shared by every plan of the context that replays a prefix of it.
+struct StreamBuf {
+  uint64_t seed = 0;
+  int64_t n = 0;
+  double* d = nullptr;
+  double draws_ms = 0.0, upload_ms = 0.0;
+  ~StreamBuf() {
+    if (d) (void)hipFree(d);
+  }
+};
+
+}  // namespace
+
 struct ccr_hip_ctx {
   int device = 0;
   int precision = CCR_FP64;
@@ -31,6 +47,9 @@ struct ccr_hip_ctx {
   bool timing_pending = false;  // events recorded by a launch not yet read into `timing`
   double upload_ms = 0.0, draws_ms = 0.0;
   ccr_hip_timing timing{};
+  // Normal streams recently generated on this context (at most two kept; a plan holds its own
+  // reference, so an evicted stream lives as long as the plans replaying it).
+  std::vector<std::shared_ptr<StreamBuf>> streams;
 };
 
 namespace {
@@ -49,7 +68,8 @@ struct ccr_hip_plan {
   GlobalDev* d_global = nullptr;
   SetDev* d_sets = nullptr;
   int32_t* d_block_set = nullptr;
-  void* d_stream = nullptr;
+  void* d_stream = nullptr;   // the stream in the plan's precision (FP64: the shared buffer itself)
+  std::shared_ptr<StreamBuf> stream64;
   void* d_dom = nullptr;     // domestic normals of the AMC states pass
   void* d_steps = nullptr;
   DateRec* d_dates = nullptr;
@@ -134,18 +154,23 @@ void* uploadReals(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const std::vector<double
   return d;
 }
 
-// Generates the normal stream in chunks into two pinned staging buffers, overlapping generation
-// of one chunk with the upload of the other.
-void uploadStream(ccr_hip_ctx* ctx, ccr_hip_plan* plan, uint64_t seed, bool fp32) {
-  const int64_t n = plan->host.stream_len;
-  const size_t elem = fp32 ? sizeof(float) : sizeof(double);
-  plan->d_stream = devAlloc(ctx, plan, static_cast<size_t>(n) * elem);
-  if (n == 0) return;
+// Generates the first n normals of the stream for `seed` in chunks into two pinned staging
+// buffers, overlapping the generation of one chunk with the upload of the other.
+std::shared_ptr<StreamBuf> generateStream64(ccr_hip_ctx* ctx, uint64_t seed, int64_t n) {
+  auto buf = std::make_shared<StreamBuf>();
+  buf->seed = seed;
+  buf->n = n;
+  if (ctx->fault == CCR_FAULT_ALLOC) {
+    ctx->fault = CCR_FAULT_NONE;
+    fail(CCR_ERR_OUT_OF_MEMORY, "hipMalloc: out of memory (forced by ccr_hip_set_fault)");
+  }
+  check(hipMalloc(reinterpret_cast<void**>(&buf->d), static_cast<size_t>(n > 0 ? n : 1) * sizeof(double)), "hipMalloc");
+  if (n == 0) return buf;
   const int64_t chunk = 1 << 22;
   void* stage[2] = {nullptr, nullptr};
   hipEvent_t done[2] = {nullptr, nullptr};
-  check(hipHostMalloc(&stage[0], static_cast<size_t>(chunk) * elem, hipHostMallocDefault), "hipHostMalloc");
-  check(hipHostMalloc(&stage[1], static_cast<size_t>(chunk) * elem, hipHostMallocDefault), "hipHostMalloc");
+  check(hipHostMalloc(&stage[0], static_cast<size_t>(chunk) * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+  check(hipHostMalloc(&stage[1], static_cast<size_t>(chunk) * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
   check(hipEventCreate(&done[0]), "hipEventCreate");
   check(hipEventCreate(&done[1]), "hipEventCreate");
   std::mt19937_64 engine(seed);
@@ -158,16 +183,11 @@ void uploadStream(ccr_hip_ctx* ctx, ccr_hip_plan* plan, uint64_t seed, bool fp32
     const int64_t m = std::min(chunk, n - off);
     if (pending[which]) check(hipEventSynchronize(done[which]), "hipEventSynchronize");
     const auto g0 = std::chrono::steady_clock::now();
-    if (fp32) {
-      float* f = static_cast<float*>(stage[which]);
-      for (int64_t i = 0; i < m; ++i) f[i] = static_cast<float>(normal(engine));
-    } else {
-      double* d = static_cast<double*>(stage[which]);
-      for (int64_t i = 0; i < m; ++i) d[i] = normal(engine);
-    }
+    double* d = static_cast<double*>(stage[which]);
+    for (int64_t i = 0; i < m; ++i) d[i] = normal(engine);
     genMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
-    check(hipMemcpyAsync(static_cast<char*>(plan->d_stream) + static_cast<size_t>(off) * elem, stage[which],
-                         static_cast<size_t>(m) * elem, hipMemcpyHostToDevice, ctx->stream),
+    check(hipMemcpyAsync(buf->d + off, stage[which], static_cast<size_t>(m) * sizeof(double), hipMemcpyHostToDevice,
+                         ctx->stream),
           "hipMemcpyAsync");
     check(hipEventRecord(done[which], ctx->stream), "hipEventRecord");
     pending[which] = true;
@@ -178,8 +198,34 @@ void uploadStream(ccr_hip_ctx* ctx, ccr_hip_plan* plan, uint64_t seed, bool fp32
   (void)hipEventDestroy(done[1]);
   (void)hipHostFree(stage[0]);
   (void)hipHostFree(stage[1]);
-  plan->draws_ms = genMs;
-  plan->upload_ms += total - genMs;
+  buf->draws_ms = genMs;
+  buf->upload_ms = total - genMs;
+  return buf;
+}
+
+// The plan's replayed stream: a prefix of a stream this context already holds for the seed, or
+// a newly generated one; FP32 plans narrow it on the device (round to nearest, as the host cast).
+void uploadStream(ccr_hip_ctx* ctx, ccr_hip_plan* plan, uint64_t seed, bool fp32) {
+  const int64_t n = plan->host.stream_len;
+  std::shared_ptr<StreamBuf> src;
+  for (const auto& s : ctx->streams)
+    if (s->seed == seed && s->n >= n) src = s;
+  plan->draws_ms = 0.0;
+  if (!src) {
+    src = generateStream64(ctx, seed, n);
+    ctx->streams.push_back(src);
+    if (ctx->streams.size() > 2) ctx->streams.erase(ctx->streams.begin());
+    plan->draws_ms = src->draws_ms;
+    plan->upload_ms += src->upload_ms;
+  }
+  plan->stream64 = src;
+  if (!fp32) {
+    plan->d_stream = src->d;
+    return;
+  }
+  plan->d_stream = devAlloc(ctx, plan, static_cast<size_t>(n) * sizeof(float));
+  if (n > 0) check(launchNarrow(src->d, static_cast<float*>(plan->d_stream), n, ctx->stream), "stream narrowing");
+  check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
 // Workgroup size of the path kernel with the most waves resident per CU for this plan (its LDS

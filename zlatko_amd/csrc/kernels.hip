@@ -467,6 +467,19 @@ hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, in
 template hipError_t pathOccupancy<double>(int, int, int, bool, int, int, int*);
 template hipError_t pathOccupancy<float>(int, int, int, bool, int, int, int*);
 
+__global__ void __launch_bounds__(256) ccr_narrow_kernel(const double* in, float* out, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * 256)
+    out[i] = static_cast<float>(in[i]);
+}
+
+hipError_t launchNarrow(const double* in, float* out, int64_t n, hipStream_t stream) {
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(ccr_narrow_kernel, dim3(static_cast<unsigned>(blocks < 65536 ? blocks : 65536)), dim3(256), 0,
+                     stream, in, out, n);
+  return hipGetLastError();
+}
+
 // dom[dom_off + (2 s + i) Q + q] = stream[q L + s nz + (i ? ncorr : 0)] for every set with a
 // domestic-normals slice, step s < T - 1 and path block q < Q (blockIdx.z: the set's position
 // among those sets).

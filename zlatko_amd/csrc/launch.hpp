@@ -106,6 +106,8 @@ hipError_t launchSurrogateScatter(const float* y, int64_t n, double mean, double
 
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);
+// out[i] = (float)in[i], i < n (round to nearest even, as a host static_cast).
+hipError_t launchNarrow(const double* in, float* out, int64_t n, hipStream_t stream);
 // Gathers the two normals per step the AMC states pass reads (the first correlated driver and the
 // domestic integral normal) of every set with Bermudans from the stream into args.dom.
 template <typename Real>

@@ -371,6 +371,16 @@ enum ccr_fault {
 };
 int32_t ccr_hip_set_fault(ccr_hip_ctx* ctx, int32_t fault);
 
+/* No counterpart in the reference (its engine materialises per netting set and frees on return).
+   Plans compiled after ccr_hip_set_shared_workspace(ctx, 1) keep their per-execute buffers
+   (exposure rows, wave partials, American Monte Carlo states, values and regressions) in one
+   workspace of the context, sized to the largest such plan, instead of owning them: a portfolio
+   split into waves of netting sets (one plan per wave) then needs HBM for one wave's buffers.
+   The buffers of such a plan are valid from its execute until the next execute of any plan of
+   the context (ccr_hip_execute copies the profiles out before returning); their plans cannot
+   train a surrogate. 0 restores owned buffers for later compiles. */
+int32_t ccr_hip_set_shared_workspace(ccr_hip_ctx* ctx, int32_t enable);
+
 /* ExposureEngine::reportingGrid / simulationGrid of one netting set (exposure_engine.cpp:35-48).
  * Writes up to `capacity` times and the full size into *size. Needs no device. */
 int32_t ccr_hip_reporting_grid(const double* base_grid, int32_t base_size, const ccr_netting_set* set,

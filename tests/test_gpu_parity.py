@@ -248,3 +248,27 @@ def test_plans_sharing_a_context_stream_match_fresh_contexts():
         f.close()
         for k in abi.PROFILE_FIELDS:
             assert np.array_equal(got[k], want[k]), (s.num_paths, p, k)
+
+
+def test_shared_workspace_waves_match_owned_buffers():
+    """Waves of netting sets as plans sharing one context workspace (grown when a larger wave
+    executes) give the profiles of plans that own their buffers; a surrogate refuses them."""
+    from zlatko_amd import native
+    a = W.small_reference_case(num_paths=500, csa=True, fx=True, credits=True, bermudans=2)
+    b = W.c5(num_paths=400)
+    e = native.Engine()
+    e.set_shared_workspace(True)
+    pa, pb = e.compile(a), e.compile(b)
+    got = [pa.execute(), pb.execute(), pa.execute()]
+    with pytest.raises(native.CcrError):
+        e.surrogate_train(pa, pb, train_paths=64, epochs=1)
+    pa.close()
+    pb.close()
+    e.close()
+    f = native.Engine()
+    want_a, want_b = f.run(a), f.run(b)
+    f.close()
+    for outs, want in ((got[0], want_a), (got[1], want_b), (got[2], want_a)):
+        for o, w in zip(outs, want):
+            for k in abi.PROFILE_FIELDS:
+                assert np.array_equal(o.get(k), w.get(k)), k

@@ -50,6 +50,11 @@ struct ccr_hip_ctx {
   // Normal streams recently generated on this context (at most two kept; a plan holds its own
   // reference, so an evicted stream lives as long as the plans replaying it).
   std::vector<std::shared_ptr<StreamBuf>> streams;
+  // Shared workspace: plans compiled while share_ws is set keep their per-execute buffers
+  // (exposure rows, wave partials, AMC states and values) here, sized to the largest of them.
+  bool share_ws = false;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
 };
 
 namespace {
@@ -95,6 +100,9 @@ struct ccr_hip_plan {
   float* d_ysur = nullptr;
   AmcArgs amc{};
   double draws_ms = 0.0, upload_ms = 0.0;
+  // Per-execute buffers placed in the context's shared workspace: (field, byte offset).
+  std::vector<std::pair<void**, size_t>> ws_slots;
+  size_t ws_need = 0;
   ~ccr_hip_plan() {
     for (auto& b : bufs)
       if (b.p) (void)hipFree(b.p);
@@ -126,6 +134,39 @@ int32_t guarded(ccr_hip_ctx* ctx, const std::function<void()>& f) {
     if (ctx) ctx->err = e.what();
     return CCR_ERR_RUNTIME;
   }
+}
+
+void* devAlloc(ccr_hip_ctx* ctx, ccr_hip_plan* plan, size_t bytes);
+
+// A per-execute buffer: the plan's own allocation, or a slot of the context's shared workspace
+// (bound to an address by bindWorkspace before every execute).
+template <typename T>
+void execAlloc(ccr_hip_ctx* ctx, ccr_hip_plan* plan, T*& field, size_t bytes) {
+  if (!ctx->share_ws) {
+    field = static_cast<T*>(devAlloc(ctx, plan, bytes));
+    return;
+  }
+  plan->ws_slots.push_back({reinterpret_cast<void**>(&field), plan->ws_need});
+  plan->ws_need += (bytes + 255) / 256 * 256;
+  field = nullptr;
+}
+
+void bindWorkspace(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
+  if (plan->ws_slots.empty()) return;
+  if (ctx->ws_bytes < plan->ws_need) {
+    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (ctx->ws) check(hipFree(ctx->ws), "hipFree");
+    ctx->ws = nullptr;
+    ctx->ws_bytes = 0;
+    check(hipMalloc(&ctx->ws, plan->ws_need), "hipMalloc (shared workspace)");
+    ctx->ws_bytes = plan->ws_need;
+  }
+  for (auto& s : plan->ws_slots) *s.first = static_cast<char*>(ctx->ws) + s.second;
+  AmcArgs& a = plan->amc;
+  a.states = plan->d_states;
+  a.cont = plan->d_cont;
+  a.exidx = plan->d_exidx;
+  plan->d_bermv = a.bermv;
 }
 
 void* devAlloc(ccr_hip_ctx* ctx, ccr_hip_plan* plan, size_t bytes) {
@@ -285,9 +326,8 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   plan->d_opts = h.opts.empty() ? nullptr : uploadReals(ctx, plan.get(), h.opts, fp32);
   const size_t elem = fp32 ? sizeof(float) : sizeof(double);
   const int64_t P = h.g.num_paths;
-  plan->d_rows = devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * P * elem);
-  plan->d_partials =
-      static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(h.total_waves) * h.max_rep * kStats * sizeof(double)));
+  execAlloc(ctx, plan.get(), plan->d_rows, static_cast<size_t>(num_sets) * h.max_rep * P * elem);
+  execAlloc(ctx, plan.get(), plan->d_partials, static_cast<size_t>(h.total_waves) * h.max_rep * kStats * sizeof(double));
   plan->d_stats = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * kStats * sizeof(double)));
   plan->d_pfe = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * sizeof(double)));
   if (!h.berms.empty()) {
@@ -298,9 +338,9 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     const int nchunks = static_cast<int>((P + kAmcChunk - 1) / kAmcChunk);
     plan->d_berms = upload(ctx, plan.get(), h.berms.data(), h.berms.size());
     plan->d_bdates = upload(ctx, plan.get(), h.bdates.data(), h.bdates.size());
-    plan->d_cont = dalloc(h.cont_len);
-    plan->d_exidx = static_cast<int16_t*>(devAlloc(ctx, plan.get(), static_cast<size_t>(nB) * P * sizeof(int16_t)));
-    plan->d_states = dalloc(2 * h.state_len);
+    execAlloc(ctx, plan.get(), plan->d_cont, static_cast<size_t>(h.cont_len) * sizeof(double));
+    execAlloc(ctx, plan.get(), plan->d_exidx, static_cast<size_t>(nB) * P * sizeof(int16_t));
+    execAlloc(ctx, plan.get(), plan->d_states, static_cast<size_t>(2 * h.state_len) * sizeof(double));
     AmcArgs& a = plan->amc;
     a.sets = plan->d_sets;
     a.berms = plan->d_berms;
@@ -313,9 +353,10 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.bdate_cd = upload(ctx, plan.get(), h.bdate_cd.data(), h.bdate_cd.size());
     a.poly = plan->d_poly;
     a.states = plan->d_states;
-    a.ev = dalloc(h.ev_len);
-    a.strategy = dalloc(h.ev_len);
-    a.part = dalloc(static_cast<int64_t>(nchunks) * std::max(nB * kRegMom2, ncd * 10));
+    execAlloc(ctx, plan.get(), a.ev, static_cast<size_t>(h.ev_len) * sizeof(double));
+    execAlloc(ctx, plan.get(), a.strategy, static_cast<size_t>(h.ev_len) * sizeof(double));
+    execAlloc(ctx, plan.get(), a.part,
+              static_cast<size_t>(nchunks) * std::max(nB * kRegMom2, ncd * 10) * sizeof(double));
     a.regp = dalloc(static_cast<int64_t>(nB) * 8);
     a.coef = dalloc(static_cast<int64_t>(nB) * 4);
     a.csum = dalloc(std::max(1, ncd) * 3);
@@ -323,10 +364,10 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     const int nkc = static_cast<int>((P + kGemmK - 1) / kGemmK);
     a.tiles = h.tiles.empty() ? nullptr : upload(ctx, plan.get(), h.tiles.data(), h.tiles.size());
     a.gcols = h.gcols.empty() ? nullptr : upload(ctx, plan.get(), h.gcols.data(), h.gcols.size());
-    a.gpart = dalloc(static_cast<int64_t>(nkc) * static_cast<int64_t>(h.bdates.size()) * 4);
+    execAlloc(ctx, plan.get(), a.gpart, static_cast<size_t>(nkc) * h.bdates.size() * 4 * sizeof(double));
     a.cont = plan->d_cont;
     a.exidx = plan->d_exidx;
-    a.bermv = ncd > 0 ? dalloc(static_cast<int64_t>(ncd) * P) : nullptr;
+    if (ncd > 0) execAlloc(ctx, plan.get(), a.bermv, static_cast<size_t>(ncd) * P * sizeof(double));
     plan->d_bermv = a.bermv;
     plan->d_ctimes = ncd > 0 ? upload(ctx, plan.get(), h.ctimes.data(), h.ctimes.size()) : nullptr;
     a.paths = P;
@@ -494,6 +535,7 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* su
     fail(CCR_ERR_DEVICE, "kernel launch failed (forced by ccr_hip_set_fault)");
   }
   check(hipSetDevice(ctx->device), "hipSetDevice");
+  bindWorkspace(ctx, plan);
   const HostPlan& h = plan->host;
   const int nsets = static_cast<int>(h.sets.size());
   const int blocks = static_cast<int>(h.block_set.size());
@@ -656,10 +698,17 @@ void ccr_hip_destroy(ccr_hip_ctx* ctx) {
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->ws) (void)hipFree(ctx->ws);
   delete ctx;
 }
 
 const char* ccr_hip_last_error(const ccr_hip_ctx* ctx) { return ctx ? ctx->err.c_str() : gCreateError.c_str(); }
+
+int32_t ccr_hip_set_shared_workspace(ccr_hip_ctx* ctx, int32_t enable) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  ctx->share_ws = enable != 0;
+  return CCR_OK;
+}
 
 int32_t ccr_hip_set_fault(ccr_hip_ctx* ctx, int32_t fault) {
   if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
@@ -942,6 +991,8 @@ int32_t ccr_hip_surrogate_train(ccr_hip_ctx* ctx, ccr_hip_plan* train, ccr_hip_p
   return guarded(ctx, [&] {
     if (!train || !holdout || !o || !out) invalid("null argument");
     if (train == holdout) invalid("surrogate: the hold-out plan must differ from the training plan");
+    if (!train->ws_slots.empty() || !holdout->ws_slots.empty())
+      invalid("surrogate: plans compiled with the shared workspace cannot train a surrogate");
     const HostPlan& ht = train->host;
     const HostPlan& hh = holdout->host;
     if (ht.sets.size() != hh.sets.size()) invalid("surrogate: the plans hold different netting sets");

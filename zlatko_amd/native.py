@@ -42,6 +42,7 @@ def lib():
         L.ccr_hip_last_error.argtypes = [P]
         L.ccr_hip_last_error.restype = C.c_char_p
         L.ccr_hip_set_fault.argtypes = [P, I32]
+        L.ccr_hip_set_shared_workspace.argtypes = [P, I32]
         L.ccr_hip_reporting_grid.argtypes = [P, I32, P, P, I32, P]
         L.ccr_hip_simulation_grid.argtypes = [P, I32, P, P, I32, P]
         L.ccr_hip_compile.argtypes = [P, P, P, P, I32, P, I32, D, I32, P]
@@ -209,6 +210,11 @@ class Engine:
 
     def set_fault(self, kind: int):
         self._check(lib().ccr_hip_set_fault(self.ctx, kind))
+
+    def set_shared_workspace(self, enable: bool):
+        """Plans compiled from now on keep their per-execute buffers in one workspace of the
+        context (valid until the next execute of any plan), so waves of netting sets fit in HBM."""
+        self._check(lib().ccr_hip_set_shared_workspace(self.ctx, 1 if enable else 0))
 
     def autotune(self, spec: abi.Spec, precision: int = abi.CCR_FP64, sample_paths: int = 16384):
         """Picks the path kernel's workgroup size for this workload; returns (size, times in ms)."""

@@ -42,7 +42,7 @@ WORKLOADS = {
     "c2": (lambda p: W.c2(num_paths=p), 50000),
     "c3": (lambda p: W.c3(num_paths=p), 100000),
     "c4": (lambda p: W.c4(num_paths=p), 100000),
-    "c5": (lambda p: W.c5(num_paths=p), 20000),
+    "c5": (lambda p: W.c5(num_paths=p), 200000),
 }
 
 
@@ -55,8 +55,8 @@ WORKLOAD_TEXT = {
     "c4": "c4: 20 two-leg IRS + 10 FX forwards + 10 Bermudans, HW + FX + CIR++ counterparty / own intensities "
           "(0.3 wrong-way), 100,000 paths x 120 dates",
     "c5": "c5: 500 netting sets, Zipf(1.2) trade counts 1..5000 (20,744 trades: 3-ccy IRS, FX forwards, 1,009 "
-          "Bermudans), half under CSA, 3-ccy HW + 2 FX + shared CIR++, 120 dates (paths reduced to fit one "
-          "plan in HBM)",
+          "Bermudans), half under CSA, 3-ccy HW + 2 FX + shared CIR++, 120 dates; netting-set waves (one plan "
+          "each, one shared workspace) once sets x paths exceeds 1e7",
 }
 
 
@@ -73,36 +73,58 @@ def measured_traffic(workload: str, precision: str):
     return None, None
 
 
-def describe(spec: abi.Spec, info) -> dict:
-    return {
+def describe(spec: abi.Spec, infos, waves: int) -> dict:
+    d = {
         "workload": WORKLOAD_TEXT.get(spec.name, spec.name),
-        "paths": int(info.num_paths),
-        "netting_sets": int(info.num_sets),
+        "paths": int(infos[0].num_paths),
+        "netting_sets": int(sum(i.num_sets for i in infos)),
         "trades": int(sum(len(s.trades) for s in spec.sets)),
-        "max_simulation_dates": int(info.max_sim_dates),
-        "max_reporting_dates": int(info.max_reporting_dates),
-        "normals_per_step": int(info.num_normals),
-        "merged_terms": int(info.total_terms),
+        "max_simulation_dates": int(max(i.max_sim_dates for i in infos)),
+        "max_reporting_dates": int(max(i.max_reporting_dates for i in infos)),
+        "normals_per_step": int(infos[0].num_normals),
+        "merged_terms": int(sum(i.total_terms for i in infos)),
         "pfe_quantile": spec.pfe_quantile,
     }
+    if waves > 1:
+        d["netting_set_waves"] = waves
+    return d
 
 
-def time_gpu(engine, plan, steps: int, warmup: int):
+def split_waves(spec: abi.Spec, wave_sets: int):
+    """The netting sets in waves of wave_sets (one plan each, executed in order in one step),
+    largest sets first so that the first wave's stream (its longest grid) serves the others."""
+    if wave_sets <= 0 or wave_sets >= len(spec.sets):
+        return [spec]
+    order = sorted(spec.sets, key=lambda s: -len(s.trades))
+    return [abi.Spec(spec.model, order[i:i + wave_sets], spec.base_grid, spec.num_paths, int(spec.config.seed),
+                     bool(spec.config.antithetic), spec.pfe_quantile, spec.name)
+            for i in range(0, len(order), wave_sets)]
+
+
+def time_gpu(engine, plans, steps: int, warmup: int):
     # engine.synchronize() waits for the library's stream and then the whole device
     # (hipDeviceSynchronize): the timed region is bracketed by full-device synchronisation.
     for _ in range(warmup):
-        plan.execute_async()
+        for plan in plans:
+            plan.execute_async()
     engine.synchronize()
-    path_ms, total_ms = [], []
+    sums = {"path": [], "total": [], "amc": [], "reduce": [], "pfe": []}
     t0 = time.perf_counter()
     for _ in range(steps):
-        plan.execute_async()
-        t = engine.timing()  # waits for this step's last kernel, reads its HIP events
-        path_ms.append(t.path_ms)
-        total_ms.append(t.total_ms)
+        step = dict.fromkeys(sums, 0.0)
+        for plan in plans:
+            plan.execute_async()
+            t = engine.timing()  # waits for this wave's last kernel, reads its HIP events
+            step["path"] += t.path_ms
+            step["total"] += t.total_ms
+            step["amc"] += t.amc_ms
+            step["reduce"] += t.reduce_ms
+            step["pfe"] += t.pfe_ms
+        for k in sums:
+            sums[k].append(step[k])
     engine.synchronize()
     wall = time.perf_counter() - t0
-    return wall, float(np.mean(path_ms)), float(np.mean(total_ms)), engine.timing()
+    return wall, {k: float(np.mean(v)) for k, v in sums.items()}, engine.timing()
 
 
 def host_cpu() -> str:
@@ -160,6 +182,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--wave-sets", type=int, default=0,
+                    help="netting sets per plan (0: all in one plan unless sets x paths exceeds 1e7)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -169,21 +193,30 @@ def main():
     make, default_paths = WORKLOADS[args.workload]
     spec_fn = lambda p: make(p)  # noqa: E731
     spec = spec_fn(args.paths or default_paths)
+    # Netting-set waves: portfolios whose per-execute buffers would not fit in HBM at once run as
+    # one plan per wave, the waves sharing one context workspace (sized to the largest wave).
+    wave_sets = args.wave_sets
+    if wave_sets == 0 and len(spec.sets) * spec.num_paths > 10_000_000:
+        wave_sets = max(1, 10_000_000 // spec.num_paths)
+    specs = split_waves(spec, wave_sets)
     prec = abi.CCR_FP32 if args.precision == "fp32" else abi.CCR_FP64
     engine = native.Engine()
-    plan = engine.compile(spec, prec)
-    info = plan.info()
-    wall, path_ms, total_ms, last = time_gpu(engine, plan, args.steps, args.warmup)
+    if len(specs) > 1:
+        engine.set_shared_workspace(True)
+    plans = [engine.compile(s, prec) for s in specs]
+    infos = [p.info() for p in plans]
+    revals = sum(i.revaluations for i in infos)
+    wall, kms, last = time_gpu(engine, plans, args.steps, args.warmup)
     ms_per_step = 1000.0 * wall / args.steps
-    value = info.revaluations / (wall / args.steps)
+    value = revals / (wall / args.steps)
 
     elem = 4 if prec == abi.CCR_FP32 else 8
     # Path kernel algorithmic bytes: every netting set's slice of the replayed normal stream read
     # once (P/2 x (T_s - 1) x nz per set) + max(V - C, 0) rows written once.
-    out = plan.execute()
-    rows_elems = sum(o.c.num_dates for o in out) * info.num_paths
-    path_bytes = info.normals_read * elem + rows_elems * elem
-    achieved_gbs = path_bytes / (path_ms * 1e-3) / 1e9
+    outs = [o for p in plans for o in p.execute()]
+    rows_elems = sum(o.c.num_dates for o in outs) * infos[0].num_paths
+    path_bytes = sum(i.normals_read for i in infos) * elem + rows_elems * elem
+    achieved_gbs = path_bytes / (kms["path"] * 1e-3) / 1e9
 
     result = {
         "metric": METRIC,
@@ -198,9 +231,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == abi.CCR_FP64 else "f32",
         "data": "synthetic (seeded SplitMix64 portfolio, replayed std::mt19937_64 normal stream)",
-        "config": describe(spec, info),
-        "kernel_ms": {"path": path_ms, "total": total_ms, "amc": last.amc_ms, "reduce": last.reduce_ms,
-                      "pfe": last.pfe_ms},
+        "config": describe(spec, infos, len(specs)),
+        "kernel_ms": kms,
         "host_ms": {"draws": last.draws_ms, "upload": last.upload_ms},
         "roofline": {
             "bound": "hbm",
@@ -213,22 +245,21 @@ def main():
         },
     }
     traffic, source = measured_traffic(args.workload, args.precision)
-    if traffic is not None:
+    if traffic is not None and len(specs) == 1 and not args.paths:
         result["roofline"]["traffic"] = traffic
         result["roofline"]["traffic_source"] = source
-    plan.close()
+    for p in plans:
+        p.close()
     if not args.no_fp32 and prec == abi.CCR_FP64:
-        p32 = engine.compile(spec, abi.CCR_FP32)
-        w32, pm32, tm32, _ = time_gpu(engine, p32, args.steps, args.warmup)
-        o32 = p32.execute()
-        p32.close()
-        p64 = engine.compile(spec, abi.CCR_FP64)
-        o64 = p64.execute()
-        p64.close()
+        p32 = [engine.compile(s, abi.CCR_FP32) for s in specs]
+        w32, k32, _ = time_gpu(engine, p32, args.steps, args.warmup)
+        o32 = [o for p in p32 for o in p.execute()]
+        for p in p32:
+            p.close()
         ee_err = max(float(np.abs(a.get("expected_exposure") - b.get("expected_exposure")).max()
-                           / np.abs(b.get("expected_exposure")).max()) for a, b in zip(o32, o64))
-        result["fp32"] = {"value": info.revaluations / (w32 / args.steps), "ms_per_step": 1000.0 * w32 / args.steps,
-                          "path_kernel_ms": pm32, "ee_rel_err_vs_fp64": ee_err}
+                           / np.abs(b.get("expected_exposure")).max()) for a, b in zip(o32, outs))
+        result["fp32"] = {"value": revals / (w32 / args.steps), "ms_per_step": 1000.0 * w32 / args.steps,
+                          "path_kernel_ms": k32["path"], "ee_rel_err_vs_fp64": ee_err}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(spec_fn, args.cpu_seconds, threads)

@@ -161,13 +161,14 @@ def test_fp32_options_error_vs_fp64_oracle(engine):
     (1, abi.CCR_FP64, None), (2, abi.CCR_FP64, None), (3, abi.CCR_FP64, None), (777, abi.CCR_FP64, None),
     (2049, abi.CCR_FP64, None), (4096, abi.CCR_FP64, None), (4097, abi.CCR_FP64, None), (5001, abi.CCR_FP64, None),
     (20000, abi.CCR_FP64, 0.999), (20000, abi.CCR_FP64, 0.001), (20000, abi.CCR_FP64, 0.3),
-    (70000, abi.CCR_FP64, 0.5), (777, abi.CCR_FP32, None), (5001, abi.CCR_FP32, None),
-    (70000, abi.CCR_FP32, 0.5)])
+    (70000, abi.CCR_FP64, 0.5), (120000, abi.CCR_FP64, None), (120000, abi.CCR_FP64, 0.5),
+    (777, abi.CCR_FP32, None), (5001, abi.CCR_FP32, None), (70000, abi.CCR_FP32, 0.5), (120000, abi.CCR_FP32, None)])
 def test_pfe_is_exact_order_statistic(engine, paths, precision, level):
     """The GPU PFE equals ccr::quantile of the GPU's own exposure rows, bit for bit. Rows include
     all-equal values (t = 0: ties at the splitters), all-zero rows (after the last cash flow),
-    quantiles whose sample splitter falls off either end of the sample, and P = 70,000 at the
-    median, where more keys lie between the splitters than LDS holds (the radix-select path)."""
+    quantiles whose sample splitter falls off either end of the sample, P = 70,000 at the median,
+    where more keys lie between the splitters than LDS holds (the radix-select path), and P =
+    120,000 (the 8,192-key buffer)."""
     spec = W.small_reference_case(num_paths=paths, csa=True, fx=True)
     if level is not None:
         spec.pfe_quantile = level

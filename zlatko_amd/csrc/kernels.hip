@@ -145,8 +145,7 @@ __device__ __noinline__ double lerpUnfused(double lo, double t, double hi) {
 
 constexpr int kPfeThreads = 256;
 constexpr int kPfeWaves = kPfeThreads / 64;
-constexpr int kPfeKeys = 4096;   // LDS keys: the sample, then the keys between the splitters
-constexpr int kPfeSample = 2048; // smaller sample of a row longer than kPfeKeys
+constexpr int kPfeSample = 2048; // smallest sample of a row longer than the key buffer
 constexpr int kPfeBins = 4096;   // 12-bit digits of the radix fallback (histogram aliases the keys)
 constexpr int kPfeUnroll = 4;    // row elements in flight per thread
 
@@ -187,9 +186,9 @@ __device__ inline unsigned int waveTotal(unsigned int v) {
   return v;
 }
 
-// Appends the keys of the lanes with `take` to keys[*count ...) (at most kPfeKeys are stored;
-// *count keeps counting past that). One LDS atomic per wavefront. Call with the whole wave active.
-template <typename K>
+// Appends the keys of the lanes with `take` to keys[*count ...) (at most kCap are stored; *count
+// keeps counting past that). One LDS atomic per wavefront. Call with the whole wave active.
+template <int kCap, typename K>
 __device__ inline void appendKeys(bool take, K k, K* keys, unsigned int* count) {
   const unsigned long long m = __ballot(take);
   if (m == 0) return;
@@ -200,7 +199,7 @@ __device__ inline void appendKeys(bool take, K k, K* keys, unsigned int* count) 
   base = __shfl(base, leader);
   if (take) {
     const unsigned int pos = base + static_cast<unsigned int>(__popcll(m & ((1ull << lane) - 1ull)));
-    if (pos < static_cast<unsigned int>(kPfeKeys)) keys[pos] = k;
+    if (pos < static_cast<unsigned int>(kCap)) keys[pos] = k;
   }
 }
 
@@ -238,10 +237,11 @@ __device__ inline void pfeFindBin(const unsigned int* hist, long long rank, unsi
 
 // One workgroup per (set, reporting date) row: exact order statistics lo = floor(h), hi = lo + 1
 // of the row's keys (h = q (P - 1)), then the interpolation of ccr::quantile.
-//  1. A row of at most 4096 keys is sorted in LDS and done. Otherwise its first 2048 keys (4096
-//     when 2048 would leave too many keys between the splitters; paths are exchangeable) are
-//     sorted, and two sample keys a <= b, 4.5 binomial standard deviations (+2) either side of the
-//     sample's q-quantile, bracket the row's statistics with high probability.
+//  1. A row no longer than the key buffer (4096 keys, 8192 past 100,000 paths) is sorted in LDS
+//     and done. Otherwise its first 2048 keys (more when 2048 would leave too many keys between
+//     the splitters; paths are exchangeable) are sorted, and two sample keys a <= b, 4.5 binomial
+//     standard deviations (+2) either side of the sample's q-quantile, bracket the row's
+//     statistics with high probability.
 //  2. One pass over the row counts keys < a, == a and == b, keeps the smallest key > b and
 //     gathers the keys strictly between a and b into LDS (wave-aggregated slots).
 //  3. If lo falls in [a, b] and the gathered keys fit, they are sorted and the statistics read off
@@ -249,7 +249,7 @@ __device__ inline void pfeFindBin(const unsigned int* hist, long long rank, unsi
 //     at the splitters (all-equal rows, rows of zeros) cost nothing.
 // Otherwise (a sample miss, or too many keys between the splitters: large P with a central
 // quantile) an MSB radix select over 12-bit digits narrows the prefix until its keys fit in LDS.
-template <typename Real>
+template <typename Real, int kPfeKeys>
 __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets, const Real* rows, int max_rep,
                                                               int num_paths, double level, double* out) {
   using K = typename KeyOf<Real>::T;
@@ -274,10 +274,13 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
   };
 
   // 1. Sample.
-  // The smaller sample unless its splitters would bracket more than 60% of the key buffer.
+  // The smallest sample (2048, 4096, ... keys) whose splitters bracket at most 60% of the buffer.
   auto bracketed = [&](double n) { return 2.0 * (4.5 * sqrt(n * level * (1.0 - level)) + 2.0) / n * num_paths; };
-  const int ns = num_paths <= kPfeKeys ? num_paths
-                 : bracketed(kPfeSample) <= 0.6 * kPfeKeys ? kPfeSample : kPfeKeys;
+  int ns = num_paths;
+  if (num_paths > kPfeKeys) {
+    ns = kPfeSample;
+    while (ns < kPfeKeys && bracketed(ns) > 0.6 * kPfeKeys) ns <<= 1;
+  }
   int n2 = 1;
   while (n2 < ns) n2 <<= 1;
   for (int i = tid; i < n2; i += kPfeThreads) keys[i] = i < ns ? keyOf<Real>(row[i]) : ~K(0);
@@ -317,7 +320,7 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
         eqA += (valid && k == a) ? 1u : 0u;
         eqB += (valid && twoEnds && k == b) ? 1u : 0u;
         if (valid && k > b && k < minAbove) minAbove = k;
-        appendKeys(valid && k > a && k < b, k, keys, &sOpen);
+        appendKeys<kPfeKeys>(valid && k > a && k < b, k, keys, &sOpen);
       }
     }
     below = waveTotal(below);
@@ -406,7 +409,7 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
       const K k = i < num_paths ? keyOf<Real>(row[i]) : K(0);
       const bool in = i < num_paths && (k & mask) == prefix;
       if (i < num_paths && (k & mask) > prefix && k < minAbove) minAbove = k;
-      appendKeys(in, k, keys, &sOpen);
+      appendKeys<kPfeKeys>(in, k, keys, &sOpen);
     }
     atomicMin(&sMinAbove, minAbove);
     __syncthreads();
@@ -550,8 +553,14 @@ hipError_t launchReduce(const SetDev* sets, const double* partials, int nsets, i
 template <typename Real>
 hipError_t launchPfe(const SetDev* sets, const Real* rows, int nsets, int max_rep, int num_paths, double level,
                      double* out, hipStream_t stream) {
-  hipLaunchKernelGGL(ccr_pfe_kernel<Real>, dim3(nsets * max_rep), dim3(256), 0, stream, sets, rows, max_rep, num_paths,
-                     level, out);
+  // Rows beyond 100,000 paths take an 8,192-key buffer, so the keys between the splitters of a
+  // 4,096-key sample still fit.
+  if (num_paths > 100000)
+    hipLaunchKernelGGL((ccr_pfe_kernel<Real, 8192>), dim3(nsets * max_rep), dim3(256), 0, stream, sets, rows, max_rep,
+                       num_paths, level, out);
+  else
+    hipLaunchKernelGGL((ccr_pfe_kernel<Real, 4096>), dim3(nsets * max_rep), dim3(256), 0, stream, sets, rows, max_rep,
+                       num_paths, level, out);
   return hipGetLastError();
 }
 

@@ -340,6 +340,70 @@ __global__ void __launch_bounds__(256) amc_cgemm_kernel(AmcArgs a) {
   }
 }
 
+// Narrow tiles (sets with at most kNarrowCols swaptions): up to kNarrowDates dates x nb
+// swaptions. Thread (dd, b) of each half of the workgroup accumulates the 4 basis rows of date dd
+// against swaption b's strategy column; the two halves take the two halves of every 64-path step
+// and their sums are added in a fixed order at the end.
+__global__ void __launch_bounds__(256) amc_cgemm_narrow_kernel(AmcArgs a) {
+  __shared__ double Ws[4 * kNarrowDates][kGStep + 1];
+  __shared__ double Ss[kNarrowCols][kGStep + 1];
+  __shared__ double red[128][4];
+  const GemmTile t = a.narrow[blockIdx.x];
+  const int kc = static_cast<int>(blockIdx.y);
+  const int tid = static_cast<int>(threadIdx.x), half = tid >> 7, dd = (tid >> 3) & 15, b = tid & 7;
+  const int64_t kBeg = static_cast<int64_t>(kc) * kGemmK;
+  const int64_t kEnd = kBeg + kGemmK < a.paths ? kBeg + kGemmK : a.paths;
+  // Loader roles: design date tid / 16, paths 4 (tid % 16) .. +3 of each step; S column tid / 32,
+  // paths 2 (tid % 32) and +1.
+  const int wd = tid >> 4, wp = (tid & 15) * 4;
+  const int sc = tid >> 5, spo = (tid & 31) * 2;
+  const double* xsrc = nullptr;
+  RegParams rp;
+  if (wd < t.nd) {
+    const int d = t.d0 + wd;
+    xsrc = a.states + toConst(a.sets)[a.cdates[2 * d]].state_off + static_cast<int64_t>(a.cdates[2 * d + 1]) * a.paths;
+    rp = regParams(a.csum[3 * d], a.csum[3 * d + 1], a.csum[3 * d + 2], 3);
+  }
+  const int col = sc < t.nb ? a.gcols[t.col_off + sc] : -1;
+  const double* ssrc = col >= 0 ? a.strategy + static_cast<int64_t>(col) * a.paths : nullptr;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t k0 = kBeg; k0 < kEnd; k0 += kGStep) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t k = k0 + wp + i;
+      double phi[4] = {0.0, 0.0, 0.0, 0.0};
+      double D = 1.0;
+      if (xsrc && k < kEnd) {
+        regRow((xsrc[k] - rp.center) / rp.scale, phi);
+        D = xsrc[a.state_len + k];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) Ws[4 * wd + c][wp + i] = phi[c] / D;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int64_t k = k0 + spo + i;
+      Ss[sc][spo + i] = (ssrc && k < kEnd) ? ssrc[k] : 0.0;
+    }
+    __syncthreads();
+    const int kk0 = half * (kGStep / 2);
+#pragma unroll 8
+    for (int kk = kk0; kk < kk0 + kGStep / 2; ++kk) {
+      const double s = Ss[b][kk];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] = fma(Ws[4 * dd + c][kk], s, acc[c]);
+    }
+    __syncthreads();
+  }
+  if (half == 1)
+    for (int c = 0; c < 4; ++c) red[tid - 128][c] = acc[c];
+  __syncthreads();
+  if (half == 1 || dd >= t.nd || b >= t.nb) return;
+  if (a.gcols[t.col_off + b] < 0) return;
+  const int r = a.berms[t.b0 + b].date_rec + a.cdates[2 * (t.d0 + dd) + 1];
+  for (int c = 0; c < 4; ++c) a.gpart[(static_cast<int64_t>(kc) * a.nbdates + r) * 4 + c] = acc[c] + red[tid][c];
+}
+
 // Continuation fit of every (Bermudan, date) record: target strategy[next exercise] / D on all
 // paths (bermudan_swaption.cpp:110-116); the path-chunk partials of X'y are summed in chunk
 // order. Records without an exercise left get degree -1 (prediction 0).
@@ -469,6 +533,8 @@ hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int
     hipLaunchKernelGGL(amc_gram_kernel, gds, dim3(256), 0, stream, a);
     if (a.ntiles > 0)
       hipLaunchKernelGGL(amc_cgemm_kernel, dim3(a.ntiles, a.nkc), dim3(256), 0, stream, a);
+    if (a.nnarrow > 0)
+      hipLaunchKernelGGL(amc_cgemm_narrow_kernel, dim3(a.nnarrow, a.nkc), dim3(256), 0, stream, a);
     if (hipError_t e = last()) return e;
   }
   hipLaunchKernelGGL(amc_csolve_kernel, dim3((a.nbdates + 255) / 256), dim3(256), 0, stream, a);

@@ -363,6 +363,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.gram = dalloc(std::max(1, ncd) * 10);
     const int nkc = static_cast<int>((P + kGemmK - 1) / kGemmK);
     a.tiles = h.tiles.empty() ? nullptr : upload(ctx, plan.get(), h.tiles.data(), h.tiles.size());
+    a.narrow = h.ntiles.empty() ? nullptr : upload(ctx, plan.get(), h.ntiles.data(), h.ntiles.size());
     a.gcols = h.gcols.empty() ? nullptr : upload(ctx, plan.get(), h.gcols.data(), h.gcols.size());
     execAlloc(ctx, plan.get(), a.gpart, static_cast<size_t>(nkc) * h.bdates.size() * 4 * sizeof(double));
     a.cont = plan->d_cont;
@@ -379,6 +380,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.nchunks = nchunks;
     a.ncchunks = (nchunks + kContGroup - 1) / kContGroup;
     a.ntiles = static_cast<int>(h.tiles.size());
+    a.nnarrow = static_cast<int>(h.ntiles.size());
     a.nkc = nkc;
   }
   check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");

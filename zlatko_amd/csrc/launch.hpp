@@ -75,6 +75,7 @@ struct AmcArgs {
   double* csum;               // [cdate][3]: n, sum x, sum x^2
   double* gram;               // [cdate][10]
   const GemmTile* tiles;
+  const GemmTile* narrow;     // narrow tiles
   const int32_t* gcols;       // strategy column per (tile, swaption), -1: none
   double* gpart;
   double* cont;
@@ -82,6 +83,7 @@ struct AmcArgs {
   double* bermv;
   int64_t paths, state_len;
   int32_t nberms, ncols, ncdates, nbdates, nchunks, ntiles, nkc;
+  int32_t nnarrow;
   int32_t ncchunks;           // continuation-date partials per date: ceil(nchunks / kContGroup)
 };
 

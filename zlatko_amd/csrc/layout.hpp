@@ -99,6 +99,10 @@ constexpr int kContStride = 8;   // per (Bermudan, date): center, scale, c0..c3,
 constexpr int kGemmDates = 8;
 constexpr int kGemmCols = 64;
 constexpr int kGemmK = 4096;
+// Sets with at most kNarrowCols Bermudans use narrow tiles instead: up to kNarrowDates dates of
+// one epoch x their (<= 8) swaptions, one lane per (date, swaption).
+constexpr int kNarrowDates = 16;
+constexpr int kNarrowCols = 8;
 
 // Value terms of one instrument on one date: bond polynomial (or its exponential terms) plus
 // fixed coupons; indices relative to the set's poly / bonds / fixed tables. The fixed coupons of

@@ -797,7 +797,8 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     }
     sd.berm_end = static_cast<int32_t>(plan.berms.size());
     // Continuation GEMM tiles: runs of valued dates with the same next exercise for every swaption
-    // (an epoch), cut into kGemmDates dates x kGemmCols swaptions; epochs past every swaption's
+    // (an epoch), cut into kGemmDates dates x kGemmCols swaptions (kNarrowDates dates x all
+    // swaptions in narrow tiles for sets with at most kNarrowCols); epochs past every swaption's
     // last exercise need no regression.
     if (nB > 0) {
       std::vector<int> vdates;
@@ -813,7 +814,9 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       std::size_t i0 = 0;
       while (i0 < vdates.size()) {
         std::size_t i1 = i0 + 1;
-        while (i1 < vdates.size() && i1 - i0 < static_cast<std::size_t>(kGemmDates) && sameEpoch(vdates[i0], vdates[i1])) ++i1;
+        const bool narrow = nB <= kNarrowCols;
+        const std::size_t maxDates = static_cast<std::size_t>(narrow ? kNarrowDates : kGemmDates);
+        while (i1 < vdates.size() && i1 - i0 < maxDates && sameEpoch(vdates[i0], vdates[i1])) ++i1;
         const int j0 = vdates[i0];
         bool any = false;
         for (int b = 0; b < nB; ++b)
@@ -831,7 +834,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
             const int k = bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(j0)];
             plan.gcols.push_back(k < bm.ne ? bm.ev_off + k : -1);
           }
-          plan.tiles.push_back(tile);
+          (narrow ? plan.ntiles : plan.tiles).push_back(tile);
         }
         i0 = i1;
       }

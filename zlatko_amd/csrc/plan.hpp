@@ -50,6 +50,7 @@ struct HostPlan {
   std::vector<int32_t> bdate_berm;  // per BermDate record: its Bermudan
   std::vector<int32_t> bdate_cd;    // per BermDate record: its continuation date (-1: none)
   std::vector<GemmTile> tiles;      // continuation GEMM tiles
+  std::vector<GemmTile> ntiles;     // narrow continuation GEMM tiles (sets with few Bermudans)
   std::vector<int32_t> gcols;       // strategy column per (tile, swaption), -1: none
 };
 

@@ -96,12 +96,15 @@ __host__ __device__ inline bool regSolve(const RegParams& p, const double* gram,
   return true;
 }
 
-// PolynomialRegression::predict (regression.cpp:85-90).
-__host__ __device__ inline double regPredict(double x, double center, double scale, int deg, const double* coef) {
-  const double z = (x - center) / scale;
+// PolynomialRegression::predict (regression.cpp:85-90), from the standardised state z.
+__host__ __device__ inline double regPredictZ(double z, int deg, const double* coef) {
   double r = 0.0;
   for (int d = deg < 0 ? 0 : deg; d >= 0; --d) r = r * z + coef[d];
   return r;
+}
+// The same from the state x: z = (x - center) / scale.
+__host__ __device__ inline double regPredict(double x, double center, double scale, int deg, const double* coef) {
+  return regPredictZ((x - center) / scale, deg, coef);
 }
 
 }  // namespace ccrhip

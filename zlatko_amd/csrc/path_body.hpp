@@ -134,15 +134,17 @@ __host__ __device__ inline Real evalRange(const CCR_CONST TermRange& tr, Real x,
                                           cptr<double> poly, Fixing&& fixing) {
   Real acc;
   if (tr.poly_len > 0 && allLanes(static_cast<double>(x < Real(0) ? -x : x) <= tr.xmax)) {
+    // Horner over kPolyBatch coefficients per batch of scalar loads (the range is uniform
+    // across the wavefront wherever this runs: one exercise or one valued date per workgroup).
     cptr<double> pc = poly + tr.poly_beg;
     const double xd = static_cast<double>(x);
     double h = 0.0;
-    for (int k = 0; k < tr.poly_len; k += 4) {
-      const double c0 = pc[k], c1 = pc[k + 1], c2 = pc[k + 2], c3 = pc[k + 3];
-      h = fma(h, xd, c0);
-      h = fma(h, xd, c1);
-      h = fma(h, xd, c2);
-      h = fma(h, xd, c3);
+    for (int k = 0; k < tr.poly_len; k += kPolyBatch) {
+      double c[kPolyBatch];
+#pragma unroll
+      for (int u = 0; u < kPolyBatch; ++u) c[u] = pc[k + u];
+#pragma unroll
+      for (int u = 0; u < kPolyBatch; ++u) h = fmaU3(h, xd, c[u]);
     }
     acc = static_cast<Real>(h);
   } else {
@@ -171,6 +173,10 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
   const Real x = static_cast<Real>(xd);
   auto fixing = [&](int k) { return static_cast<Real>(xrow[static_cast<int64_t>(k) * paths + p]); };
   Real V = 0;
+  // The continuation fits of one date standardise x alike (the date's mean and deviation over
+  // all paths): z is computed once and again only if a record holds other constants.
+  double zCenter = 0.0, zScale = 0.0, z = 0.0;
+  bool haveZ = false;
   for (int b = S.berm_beg; b < S.berm_end; ++b) {
     const CCR_CONST BermDev& bm = berms[b];
     const CCR_CONST BermDate& bd = bdates[bm.date_rec + j];
@@ -179,7 +185,13 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
     if (bd.next_k < bm.ne) {
       cptr<double> c = cont + static_cast<int64_t>(bm.date_rec + j) * kContStride;
       const double coef[4] = {c[2], c[3], c[4], c[5]};
-      const double pred = regPredict(xd, c[0], c[1], static_cast<int>(c[6]), coef);
+      if (!haveZ || c[0] != zCenter || c[1] != zScale) {
+        zCenter = c[0];
+        zScale = c[1];
+        z = (xd - zCenter) / zScale;
+        haveZ = true;
+      }
+      const double pred = regPredictZ(z, static_cast<int>(c[6]), coef);
       v = static_cast<Real>(pred > 0.0 ? pred : 0.0);
     }
     if (bd.has_swap && !allLanes(!exercised)) {

@@ -718,7 +718,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       tr.poly_beg = static_cast<int32_t>(plan.poly.size());
       std::vector<double> coef;
       if (bondPolynomial(plan.bonds.data() + 2 * tr.bond_beg, tr.bond_end - tr.bond_beg, tr.c0, xmax, coef)) {
-        const std::size_t len = (coef.size() + 3) / 4 * 4;
+        const std::size_t len = (coef.size() + kPolyBatch - 1) / kPolyBatch * kPolyBatch;
         plan.poly.insert(plan.poly.end(), len - coef.size(), 0.0);
         plan.poly.insert(plan.poly.end(), coef.rbegin(), coef.rend());
         tr.poly_len = static_cast<int32_t>(len);

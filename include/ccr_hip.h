@@ -449,6 +449,41 @@ void ccr_bilateral_cva(const double* times, const double* discounted_ee, const d
                        int32_t size, const ccr_credit_curve* counterparty, double recovery_counterparty,
                        const ccr_credit_curve* own, double recovery_own, double* cva, double* dva);
 
+/* ------------------------------------------------------------------ XVA on a profile (FP64, host) */
+
+/* ccr::XvaParameters (include/ccr/xva/xva.hpp:12-21); ccr_xva_defaults fills the reference's
+ * defaults (borrow 1%, lend 0, margin funding 1%, hurdle 10%, capital ratio 8%, risk weight 1,
+ * alpha 1.4). */
+typedef struct ccr_xva_parameters {
+  double funding_spread_borrow, funding_spread_lend, margin_funding_spread;
+  double hurdle_rate, capital_ratio, risk_weight, alpha;
+} ccr_xva_parameters;
+void ccr_xva_defaults(ccr_xva_parameters* parameters);
+
+/* ccr::Xva (xva.hpp:33-42): every adjustment as a positive cost or benefit. */
+typedef struct ccr_xva_result {
+  double cva, dva, fca, fba, mva, kva;
+} ccr_xva_result;
+
+/* exposureXva (src/xva/xva.cpp:32-43): CVA, FCA and KVA of a discounted EE profile into
+ * cva_fca_kva[3]. Returns CCR_OK, or CCR_ERR_INVALID_ARGUMENT for a null pointer. */
+int32_t ccr_exposure_xva(const double* times, const double* discounted_ee, int32_t size,
+                         const ccr_credit_curve* counterparty, double recovery_counterparty,
+                         const ccr_credit_curve* own, const ccr_xva_parameters* parameters, double* cva_fca_kva);
+/* fundingBenefit (xva.cpp:45-51) from the discounted ENE (<= 0). */
+double ccr_funding_benefit(const double* times, const double* discounted_ene, int32_t size,
+                           const ccr_credit_curve* counterparty, const ccr_credit_curve* own,
+                           double funding_spread_lend);
+/* marginValuationAdjustment (xva.cpp:53-57) from a discounted initial-margin profile. */
+double ccr_margin_valuation_adjustment(const double* times, const double* discounted_initial_margin,
+                                       int32_t size, const ccr_credit_curve* counterparty,
+                                       const ccr_credit_curve* own, double margin_funding_spread);
+/* xva (xva.cpp:78-96) of a profile; discounted_initial_margin may be NULL (MVA = 0). */
+int32_t ccr_xva(const double* times, const double* discounted_ee, const double* discounted_ene, int32_t size,
+                const double* discounted_initial_margin, const ccr_credit_curve* counterparty,
+                double recovery_counterparty, const ccr_credit_curve* own, double recovery_own,
+                const ccr_xva_parameters* parameters, ccr_xva_result* out);
+
 #ifdef __cplusplus
 }
 #endif

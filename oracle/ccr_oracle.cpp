@@ -1256,6 +1256,40 @@ void ccr_oracle_bilateral_cva(const double* t, const double* epe, const double* 
   *dva = b;
 }
 
+// xva (src/xva/xva.cpp:16-96): CVA/DVA first-to-default with the other party's survival at the
+// mid-points; FCA, FBA, MVA and KVA as survival-weighted trapezoid integrals (survivals of both
+// parties at the mid-points); FBA on the magnitude of ENE*. out = cva, dva, fca, fba, mva, kva.
+// params = borrow, lend, margin funding, hurdle, capital ratio, risk weight, alpha.
+void ccr_oracle_xva(const double* t, const double* dee, const double* dene, int32_t n, const double* dim,
+                    const ccr_credit_curve* cpty, double rc, const ccr_credit_curve* own, double ro,
+                    const double* params, double* out) {
+  const Credit c(*cpty), o(*own);
+  auto weighted = [&](const Vec& f) {
+    double total = 0.0;
+    for (int32_t i = 1; i < n; ++i) {
+      const double mid = 0.5 * (t[i - 1] + t[i]);
+      total += 0.5 * (f[static_cast<std::size_t>(i - 1)] + f[static_cast<std::size_t>(i)]) * (t[i] - t[i - 1]) *
+               c.survival(mid) * o.survival(mid);
+    }
+    return total;
+  };
+  Vec ee(dee, dee + n), mag(static_cast<std::size_t>(n));
+  for (int32_t i = 0; i < n; ++i) mag[static_cast<std::size_t>(i)] = -dene[i];
+  double cva = 0.0, dva = 0.0;
+  for (int32_t i = 1; i < n; ++i) {
+    const double mid = 0.5 * (t[i - 1] + t[i]);
+    cva += (1.0 - rc) * 0.5 * (dee[i - 1] + dee[i]) * o.survival(mid) * c.defaultProbability(t[i - 1], t[i]);
+    dva += (1.0 - ro) * 0.5 * (-dene[i - 1] - dene[i]) * c.survival(mid) * o.defaultProbability(t[i - 1], t[i]);
+  }
+  const double w = weighted(ee);
+  out[0] = cva;
+  out[1] = dva;
+  out[2] = params[0] * w;
+  out[3] = params[1] * weighted(mag);
+  out[4] = dim ? params[2] * weighted(Vec(dim, dim + n)) : 0.0;
+  out[5] = params[3] * params[4] * params[5] * params[6] * w;
+}
+
 int32_t ccr_oracle_bermudan(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
                             const double* grid, int32_t grid_size, const ccr_trade* trade, double* price,
                             int32_t* exercise_index) {

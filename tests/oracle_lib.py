@@ -50,6 +50,7 @@ def oracle():
         lib.ccr_oracle_unilateral_cva.restype = D
         lib.ccr_oracle_unilateral_cva.argtypes = [P, P, I32, P, D]
         lib.ccr_oracle_bilateral_cva.argtypes = [P, P, P, I32, P, D, P, D, P, P]
+        lib.ccr_oracle_xva.argtypes = [P, P, P, I32, P, P, D, P, D, P, P]
         lib.ccr_oracle_time_grid.argtypes = [P, I32, P, I32, P]
         lib.ccr_oracle_uniform_grid.argtypes = [D, I32, P, I32, P]
         lib.ccr_oracle_lagged_grid.argtypes = [P, I32, D, P, I32, P]

@@ -98,3 +98,82 @@ def test_cpp_wrapper_compiles_and_maps_errors_to_exceptions(tmp_path):
     out = subprocess.run([str(build_wrapper_smoke(tmp_path))], capture_output=True, text=True, check=True).stdout
     # Without a GPU the context constructor throws std::runtime_error with the library's message.
     assert out.startswith("error: no HIP device") or out.splitlines()[-1].startswith("EPE")
+
+
+def _xva_inputs():
+    t = np.array([0.25 * i for i in range(21)])
+    dee = 1e6 * 0.02 * np.sqrt(t) * (5.0 - t) / 5.0
+    dene = -0.5 * dee
+    return t, dee, dene
+
+
+def _curve(hazard):
+    piv, haz = np.array([1.0]), np.array([hazard])
+    return abi.CreditCurveC(abi.dptr(piv), abi.dptr(haz), 1), (piv, haz)
+
+
+def test_xva_constant_exposure_closed_forms():
+    """fva_of_constant_exposure_and_symmetry (tests/test_xva.cpp:42-65): without default risk
+    FCA = s_b EE T, FBA = s_l |ENE| T, MVA = s_m IM T; KVA is linear in the risk weight."""
+    lib = native.lib()
+    t = np.array([0.5 * i for i in range(9)])
+    ee, ene = np.full(9, 100.0), np.full(9, -100.0)
+    none, keep = _curve(0.0)
+    p = abi.XvaParametersC()
+    lib.ccr_xva_defaults(C.byref(p))
+    p.funding_spread_borrow, p.funding_spread_lend = 0.02, 0.005
+    out = np.zeros(3)
+    assert lib.ccr_exposure_xva(abi.dptr(t), abi.dptr(ee), 9, C.byref(none), C.c_double(0.4), C.byref(none),
+                                C.byref(p), abi.dptr(out)) == 0
+    assert abs(out[1] - 0.02 * 100.0 * 4.0) <= 1e-12 and abs(out[0]) <= 1e-15
+    lib.ccr_funding_benefit.restype = C.c_double
+    lib.ccr_margin_valuation_adjustment.restype = C.c_double
+    fba = lib.ccr_funding_benefit(abi.dptr(t), abi.dptr(ene), 9, C.byref(none), C.byref(none), C.c_double(0.005))
+    mva = lib.ccr_margin_valuation_adjustment(abi.dptr(t), abi.dptr(ee), 9, C.byref(none), C.byref(none),
+                                              C.c_double(0.01))
+    assert abs(fba - 0.005 * 100.0 * 4.0) <= 1e-12 and abs(mva - 0.01 * 100.0 * 4.0) <= 1e-12
+    q = abi.XvaParametersC.from_buffer_copy(p)
+    q.risk_weight = 2.0
+    out2 = np.zeros(3)
+    lib.ccr_exposure_xva(abi.dptr(t), abi.dptr(ee), 9, C.byref(none), C.c_double(0.4), C.byref(none), C.byref(q),
+                         abi.dptr(out2))
+    assert abs(out2[2] - 2.0 * out[2]) <= 1e-12
+    risky, keep2 = _curve(0.05)
+    out3 = np.zeros(3)
+    lib.ccr_exposure_xva(abi.dptr(t), abi.dptr(ee), 9, C.byref(risky), C.c_double(0.4), C.byref(none), C.byref(p),
+                         abi.dptr(out3))
+    assert out3[1] < out[1]
+
+
+def test_xva_reduces_to_bilateral_cva_and_matches_oracle():
+    """xva_reduces_to_cva_dva_without_funding_and_capital (tests/test_xva.cpp:29-40) and every
+    adjustment of the humped profile against the oracle's restatement of xva.cpp."""
+    lib = native.lib()
+    t, dee, dene = _xva_inputs()
+    cpty, k1 = _curve(0.02)
+    own, k2 = _curve(0.01)
+    none = abi.XvaParametersC()
+    r = abi.XvaResultC()
+    assert lib.ccr_xva(abi.dptr(t), abi.dptr(dee), abi.dptr(dene), len(t), None, C.byref(cpty), C.c_double(0.4),
+                       C.byref(own), C.c_double(0.4), C.byref(none), C.byref(r)) == 0
+    c, d = C.c_double(), C.c_double()
+    lib.ccr_bilateral_cva(abi.dptr(t), abi.dptr(dee), abi.dptr(dene), len(t), C.byref(cpty), C.c_double(0.4),
+                          C.byref(own), C.c_double(0.4), C.byref(c), C.byref(d))
+    assert abs(r.cva - c.value) <= 1e-9 and abs(r.dva - d.value) <= 1e-9
+    assert r.fca == 0.0 and r.fba == 0.0 and r.mva == 0.0 and r.kva == 0.0
+    p = abi.XvaParametersC()
+    lib.ccr_xva_defaults(C.byref(p))
+    p.funding_spread_lend = 0.004
+    im = np.full(len(t), 5e4)
+    im[0] = 0.0
+    assert lib.ccr_xva(abi.dptr(t), abi.dptr(dee), abi.dptr(dene), len(t), abi.dptr(im), C.byref(cpty),
+                       C.c_double(0.4), C.byref(own), C.c_double(0.3), C.byref(p), C.byref(r)) == 0
+    got = np.array([r.cva, r.dva, r.fca, r.fba, r.mva, r.kva])
+    want = np.zeros(6)
+    params = np.array([p.funding_spread_borrow, p.funding_spread_lend, p.margin_funding_spread, p.hurdle_rate,
+                       p.capital_ratio, p.risk_weight, p.alpha])
+    oracle_lib.oracle().ccr_oracle_xva(abi.dptr(t), abi.dptr(dee), abi.dptr(dene), len(t), abi.dptr(im),
+                                       C.byref(cpty), C.c_double(0.4), C.byref(own), C.c_double(0.3),
+                                       abi.dptr(params), abi.dptr(want))
+    assert np.all(got > 0.0)
+    assert np.all(np.abs(got - want) <= 1e-14 * np.abs(want))

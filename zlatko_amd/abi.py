@@ -50,6 +50,17 @@ class CreditCurveC(C.Structure):
     _fields_ = [("pillars", DP), ("hazards", DP), ("size", C.c_int32)]
 
 
+class XvaParametersC(C.Structure):
+    _fields_ = [("funding_spread_borrow", C.c_double), ("funding_spread_lend", C.c_double),
+                ("margin_funding_spread", C.c_double), ("hurdle_rate", C.c_double), ("capital_ratio", C.c_double),
+                ("risk_weight", C.c_double), ("alpha", C.c_double)]
+
+
+class XvaResultC(C.Structure):
+    _fields_ = [("cva", C.c_double), ("dva", C.c_double), ("fca", C.c_double), ("fba", C.c_double),
+                ("mva", C.c_double), ("kva", C.c_double)]
+
+
 class HullWhiteC(C.Structure):
     _fields_ = [("curve", YieldCurveC), ("mean_reversion", C.c_double), ("volatility", C.c_double)]
 

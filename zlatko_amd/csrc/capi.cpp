@@ -953,6 +953,75 @@ void ccr_bilateral_cva(const double* t, const double* dee, const double* dene, i
   if (dva) *dva = b;
 }
 
+void ccr_xva_defaults(ccr_xva_parameters* p) {
+  if (!p) return;
+  *p = ccr_xva_parameters{0.0100, 0.0, 0.0100, 0.10, 0.08, 1.0, 1.4};
+}
+
+namespace {
+
+// int f(t) Q_c(t) Q_own(t) dt: trapezoid in f, both survivals at the interval mid-points
+// (xva.cpp:18-27).
+double survivalWeighted(const double* t, const double* f, int32_t n, const Credit& c, const Credit& o, double sign) {
+  double total = 0.0;
+  for (int32_t i = 1; i < n; ++i) {
+    const double mid = 0.5 * (t[i - 1] + t[i]);
+    total += 0.5 * (sign * f[i - 1] + sign * f[i]) * (t[i] - t[i - 1]) * c.survival(mid) * o.survival(mid);
+  }
+  return total;
+}
+
+}  // namespace
+
+int32_t ccr_exposure_xva(const double* t, const double* dee, int32_t n, const ccr_credit_curve* counterparty,
+                         double rc, const ccr_credit_curve* own, const ccr_xva_parameters* p, double* out) {
+  if (!t || !dee || !counterparty || !own || !p || !out || n < 0) return CCR_ERR_INVALID_ARGUMENT;
+  const Credit c(*counterparty), o(*own);
+  double cva = 0.0;
+  for (int32_t i = 1; i < n; ++i)
+    cva += (1.0 - rc) * 0.5 * (dee[i - 1] + dee[i]) * o.survival(0.5 * (t[i - 1] + t[i])) *
+           (c.survival(t[i - 1]) - c.survival(t[i]));
+  const double weighted = survivalWeighted(t, dee, n, c, o, 1.0);
+  out[0] = cva;
+  out[1] = p->funding_spread_borrow * weighted;
+  out[2] = p->hurdle_rate * p->capital_ratio * p->risk_weight * p->alpha * weighted;
+  return CCR_OK;
+}
+
+double ccr_funding_benefit(const double* t, const double* dene, int32_t n, const ccr_credit_curve* counterparty,
+                           const ccr_credit_curve* own, double spread) {
+  if (!t || !dene || !counterparty || !own) return 0.0;
+  return spread * survivalWeighted(t, dene, n, Credit(*counterparty), Credit(*own), -1.0);
+}
+
+double ccr_margin_valuation_adjustment(const double* t, const double* dim, int32_t n,
+                                       const ccr_credit_curve* counterparty, const ccr_credit_curve* own,
+                                       double spread) {
+  if (!t || !dim || !counterparty || !own) return 0.0;
+  return spread * survivalWeighted(t, dim, n, Credit(*counterparty), Credit(*own), 1.0);
+}
+
+int32_t ccr_xva(const double* t, const double* dee, const double* dene, int32_t n, const double* dim,
+                const ccr_credit_curve* counterparty, double rc, const ccr_credit_curve* own, double ro,
+                const ccr_xva_parameters* p, ccr_xva_result* out) {
+  if (!dene || !out) return CCR_ERR_INVALID_ARGUMENT;
+  double e[3];
+  const int32_t rc2 = ccr_exposure_xva(t, dee, n, counterparty, rc, own, p, e);
+  if (rc2 != CCR_OK) return rc2;
+  const Credit c(*counterparty), o(*own);
+  ccr_xva_result r{};
+  r.cva = e[0];
+  r.fca = e[1];
+  r.kva = e[2];
+  for (int32_t i = 1; i < n; ++i)
+    r.dva += (1.0 - ro) * 0.5 * (-dene[i - 1] - dene[i]) * c.survival(0.5 * (t[i - 1] + t[i])) *
+             (o.survival(t[i - 1]) - o.survival(t[i]));
+  r.fba = ccr_funding_benefit(t, dene, n, counterparty, own, p->funding_spread_lend);
+  if (dim) r.mva = ccr_margin_valuation_adjustment(t, dim, n, counterparty, own, p->margin_funding_spread);
+  *out = r;
+  return CCR_OK;
+}
+
 int32_t ccr_hip_mlp_train(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, const ccr_mlp_train_options* o, const double* X,
                           const double* Y, int64_t n, double* params, double* stdz, double* loss, double* train_ms) {
   if (!ctx) return CCR_ERR_INVALID_ARGUMENT;

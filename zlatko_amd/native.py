@@ -43,6 +43,16 @@ def lib():
         L.ccr_hip_last_error.restype = C.c_char_p
         L.ccr_hip_set_fault.argtypes = [P, I32]
         L.ccr_hip_set_shared_workspace.argtypes = [P, I32]
+        L.ccr_xva_defaults.argtypes = [P]
+        L.ccr_xva_defaults.restype = None
+        L.ccr_exposure_xva.argtypes = [P, P, I32, P, D, P, P, P]
+        L.ccr_exposure_xva.restype = I32
+        L.ccr_funding_benefit.argtypes = [P, P, I32, P, P, D]
+        L.ccr_funding_benefit.restype = D
+        L.ccr_margin_valuation_adjustment.argtypes = [P, P, I32, P, P, D]
+        L.ccr_margin_valuation_adjustment.restype = D
+        L.ccr_xva.argtypes = [P, P, P, I32, P, P, D, P, D, P, P]
+        L.ccr_xva.restype = I32
         L.ccr_hip_reporting_grid.argtypes = [P, I32, P, P, I32, P]
         L.ccr_hip_simulation_grid.argtypes = [P, I32, P, P, I32, P]
         L.ccr_hip_compile.argtypes = [P, P, P, P, I32, P, I32, D, I32, P]

@@ -904,8 +904,11 @@ double secondsSince(std::chrono::steady_clock::time_point t0) {
 
 // ExposureEngine::run: src/exposure/exposure_engine.cpp:50-81 (plus the fused pipeline's per-date
 // extras: E[D], E[Q], pathwise CVA/DVA increments on the reporting dates).
+// im (optional): initialMarginProfile (src/xva/xva.cpp:59-76) of the netted value on the
+// simulation grid, imQ its quantile level, imMpor the margin period of risk.
 void runSet(const Generator& g, const ccr_simulation_config& cfg, const Grid& base, const ccr_netting_set& ns,
-            double q, int threads, ccr_exposure_out* out) {
+            double q, int threads, ccr_exposure_out* out, Vec* im = nullptr, double imQ = 0.99,
+            double imMpor = 10.0 / 250.0) {
   if (ns.num_trades <= 0 || !ns.trades) invalid("ExposureEngine::run: empty netting set");
   std::vector<Trade> trades;
   for (int i = 0; i < ns.num_trades; ++i) trades.push_back(makeTrade(ns.trades[i], g));
@@ -942,6 +945,19 @@ void runSet(const Generator& g, const ccr_simulation_config& cfg, const Grid& ba
   });
   gTiming.pricing_s += secondsSince(t0);
   gTiming.revaluations += static_cast<int64_t>(trades.size()) * static_cast<int64_t>(nP) * static_cast<int64_t>(nR);
+  if (im) {
+    im->assign(nT, 0.0);
+    for (std::size_t k = 1; k < nT; ++k) {
+      Vec change(nP);
+      double meanDeflator = 0.0;
+      for (std::size_t p = 0; p < nP; ++p) {
+        change[p] = std::fabs(net(p, k) - net(p, k - 1));
+        meanDeflator += s.D(p, k);
+      }
+      const double scale = std::sqrt(imMpor / (s.grid[k] - s.grid[k - 1]));
+      (*im)[k] = quantile(std::move(change), imQ) * scale * meanDeflator / static_cast<double>(nP);
+    }
+  }
 
   t0 = std::chrono::steady_clock::now();
   Mat coll(nP, nT, 0.0);
@@ -1072,6 +1088,23 @@ int32_t ccr_oracle_run_exposure(const ccr_scenario_generator* generator, const c
     const Grid base(Vec(base_grid, base_grid + std::max(0, base_size)));
     for (int i = 0; i < num_sets; ++i) runSet(g, *config, base, sets[i], pfe_quantile, threads, &outs[i]);
     gTiming.total_s = secondsSince(t0);
+  });
+}
+
+// initialMarginProfile of one netting set (its simulation grid) next to its exposure profile.
+int32_t ccr_oracle_initial_margin(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                                  const double* base_grid, int32_t base_size, const ccr_netting_set* set,
+                                  double quantile_level, double mpor, int32_t threads, ccr_exposure_out* out,
+                                  double* margin, int32_t capacity, int32_t* size, char* err, int32_t err_capacity) {
+  return guard(err, err_capacity, [&] {
+    if (!generator || !config || !set || !out || !margin) invalid("null argument");
+    const Generator g(*generator);
+    const Grid base(Vec(base_grid, base_grid + std::max(0, base_size)));
+    Vec im;
+    runSet(g, *config, base, *set, 0.95, threads, out, &im, quantile_level, mpor);
+    if (size) *size = static_cast<int32_t>(im.size());
+    if (capacity < static_cast<int32_t>(im.size())) invalid("capacity smaller than the simulation grid");
+    std::copy(im.begin(), im.end(), margin);
   });
 }
 

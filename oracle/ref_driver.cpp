@@ -5,7 +5,8 @@
 //
 // It builds reference objects from the C structs of include/ccr_hip.h and calls
 // ccr::ExposureEngine::run (src/exposure/exposure_engine.cpp:50-81), ccr::pathwiseCva
-// (src/cva/cva.cpp:55-77) and ccr::BermudanSwaption::valueWithExercise. Only reference-
+// (src/cva/cva.cpp:55-77), ccr::initialMarginProfile and ccr::xva (src/xva/xva.cpp:59-96) and
+// ccr::BermudanSwaption::valueWithExercise. Only reference-
 // representable inputs are accepted: no foreign rates; a two-leg swap is expressed as
 // IRS(fixed schedule, K) + IRS(float schedule, 0) - IRS(fixed schedule, 0), all reference swaps.
 
@@ -27,6 +28,7 @@
 #include "ccr/instruments/bermudan_swaption.hpp"
 #include "ccr/instruments/european_option.hpp"
 #include "ccr/instruments/interest_rate_swap.hpp"
+#include "ccr/xva/xva.hpp"
 
 namespace {
 
@@ -110,9 +112,83 @@ int fail(char* err, int cap, const char* what, int code) {
   return code;
 }
 
+ccr::NettingSet nettingSetOf(const ccr_netting_set& set) {
+  ccr::NettingSet ns;
+  ns.id = "NS";
+  for (int i = 0; i < set.num_trades; ++i) addTrades(set.trades[i], i, ns);
+  if (set.has_collateral) {
+    const ccr_collateral& c = set.collateral;
+    ccr::CollateralAgreement csa;
+    csa.thresholdCounterparty = c.threshold_counterparty;
+    csa.thresholdOwn = c.threshold_own;
+    csa.minimumTransferAmount = c.minimum_transfer_amount;
+    csa.independentAmount = c.independent_amount;
+    csa.marginPeriodOfRisk = c.margin_period_of_risk;
+    csa.type = c.type == CCR_COLLATERAL_BOND    ? ccr::CollateralAgreement::Type::Bond
+               : c.type == CCR_COLLATERAL_ASSET ? ccr::CollateralAgreement::Type::Asset
+                                                : ccr::CollateralAgreement::Type::Cash;
+    csa.haircut = c.haircut;
+    csa.bondMaturity = c.bond_maturity;
+    csa.assetIndex = static_cast<std::size_t>(c.asset_index);
+    ns.collateral = csa;
+  }
+  return ns;
+}
+
 }  // namespace
 
 extern "C" {
+
+// ccr::initialMarginProfile of the netted value (before collateral) of one netting set on its
+// simulation grid, after ExposureEngine::run.
+int32_t ccr_ref_initial_margin(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                               const double* base_grid, int32_t base_size, const ccr_netting_set* set,
+                               double quantile_level, double mpor, double* margin, int32_t capacity, int32_t* size,
+                               char* err, int32_t err_capacity) {
+  try {
+    const ccr::ScenarioGenerator gen = generatorOf(*generator);
+    const ccr::SimulationConfig cfg{static_cast<std::size_t>(config->num_paths), config->seed, config->antithetic != 0};
+    const ccr::ExposureEngine engine(gen, cfg, ccr::TimeGrid(std::vector<double>(base_grid, base_grid + base_size)));
+    const ccr::ExposureResult r = engine.run(nettingSetOf(*set), 0.95);
+    const std::vector<double> im =
+        ccr::initialMarginProfile(r.nettedValue, r.scenarios.grid, r.scenarios.deflator, quantile_level, mpor);
+    if (size) *size = static_cast<int32_t>(im.size());
+    if (capacity < static_cast<int32_t>(im.size())) return fail(err, err_capacity, "capacity", CCR_ERR_INVALID_ARGUMENT);
+    for (std::size_t k = 0; k < im.size(); ++k) margin[k] = im[k];
+    return CCR_OK;
+  } catch (const std::exception& e) {
+    return fail(err, err_capacity, e.what(), CCR_ERR_RUNTIME);
+  }
+}
+
+// ccr::xva of a profile given by its times, discounted EE and discounted ENE.
+int32_t ccr_ref_xva(const double* times, const double* discounted_ee, const double* discounted_ene, int32_t size,
+                    const double* discounted_im, const ccr_credit_curve* counterparty, double recovery_counterparty,
+                    const ccr_credit_curve* own, double recovery_own, const ccr_xva_parameters* parameters,
+                    ccr_xva_result* out) {
+  try {
+    ccr::ExposureProfile profile;
+    profile.times.assign(times, times + size);
+    profile.discountedExpectedExposure.assign(discounted_ee, discounted_ee + size);
+    profile.discountedExpectedNegativeExposure.assign(discounted_ene, discounted_ene + size);
+    ccr::XvaParameters p;
+    p.fundingSpreadBorrow = parameters->funding_spread_borrow;
+    p.fundingSpreadLend = parameters->funding_spread_lend;
+    p.marginFundingSpread = parameters->margin_funding_spread;
+    p.hurdleRate = parameters->hurdle_rate;
+    p.capitalRatio = parameters->capital_ratio;
+    p.riskWeight = parameters->risk_weight;
+    p.alpha = parameters->alpha;
+    const std::vector<double> im = discounted_im ? std::vector<double>(discounted_im, discounted_im + size)
+                                                 : std::vector<double>();
+    const ccr::Xva x = ccr::xva(profile, im, *creditOf(*counterparty), recovery_counterparty, *creditOf(*own),
+                                recovery_own, p);
+    *out = ccr_xva_result{x.cva, x.dva, x.fca, x.fba, x.mva, x.kva};
+    return CCR_OK;
+  } catch (const std::exception&) {
+    return CCR_ERR_RUNTIME;
+  }
+}
 
 int32_t ccr_ref_run_exposure(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
                              const double* base_grid, int32_t base_size, const ccr_netting_set* set,
@@ -122,26 +198,7 @@ int32_t ccr_ref_run_exposure(const ccr_scenario_generator* generator, const ccr_
     const ccr::ScenarioGenerator gen = generatorOf(*generator);
     const ccr::SimulationConfig cfg{static_cast<std::size_t>(config->num_paths), config->seed, config->antithetic != 0};
     const ccr::ExposureEngine engine(gen, cfg, ccr::TimeGrid(std::vector<double>(base_grid, base_grid + base_size)));
-    ccr::NettingSet ns;
-    ns.id = "NS";
-    for (int i = 0; i < set->num_trades; ++i) addTrades(set->trades[i], i, ns);
-    if (set->has_collateral) {
-      const ccr_collateral& c = set->collateral;
-      ccr::CollateralAgreement csa;
-      csa.thresholdCounterparty = c.threshold_counterparty;
-      csa.thresholdOwn = c.threshold_own;
-      csa.minimumTransferAmount = c.minimum_transfer_amount;
-      csa.independentAmount = c.independent_amount;
-      csa.marginPeriodOfRisk = c.margin_period_of_risk;
-      csa.type = c.type == CCR_COLLATERAL_BOND    ? ccr::CollateralAgreement::Type::Bond
-                 : c.type == CCR_COLLATERAL_ASSET ? ccr::CollateralAgreement::Type::Asset
-                                                  : ccr::CollateralAgreement::Type::Cash;
-      csa.haircut = c.haircut;
-      csa.bondMaturity = c.bond_maturity;
-      csa.assetIndex = static_cast<std::size_t>(c.asset_index);
-      ns.collateral = csa;
-    }
-    const ccr::ExposureResult r = engine.run(ns, pfe_quantile);
+    const ccr::ExposureResult r = engine.run(nettingSetOf(*set), pfe_quantile);
     const auto& pr = r.profile;
     const std::size_t nR = pr.times.size();
     if (out->capacity < static_cast<int32_t>(nR)) return fail(err, err_capacity, "capacity", CCR_ERR_INVALID_ARGUMENT);

@@ -32,6 +32,8 @@ def oracle():
         D, I32, I64, P = C.c_double, C.c_int32, C.c_int64, C.c_void_p
         lib.ccr_oracle_run_exposure.restype = I32
         lib.ccr_oracle_run_exposure.argtypes = [P, P, P, I32, P, I32, D, I32, P, C.c_char_p, I32]
+        lib.ccr_oracle_initial_margin.restype = I32
+        lib.ccr_oracle_initial_margin.argtypes = [P, P, P, I32, P, D, D, I32, P, P, I32, P, C.c_char_p, I32]
         for n in ("ccr_oracle_discount", "ccr_oracle_instantaneous_forward", "ccr_oracle_survival",
                   "ccr_oracle_cir_survival"):
             getattr(lib, n).restype = D
@@ -91,6 +93,11 @@ def ref():
         lib.ccr_ref_quantile.argtypes = [P, I64, D]
         lib.ccr_ref_fit_mlp.restype = I32
         lib.ccr_ref_fit_mlp.argtypes = [P, P, I64, I32, I32, I32, I32, D, D, C.c_uint32, P, P]
+        if hasattr(lib, "ccr_ref_initial_margin"):
+            lib.ccr_ref_initial_margin.restype = I32
+            lib.ccr_ref_initial_margin.argtypes = [P, P, P, I32, P, D, D, P, I32, P, C.c_char_p, I32]
+            lib.ccr_ref_xva.restype = I32
+            lib.ccr_ref_xva.argtypes = [P, P, P, I32, P, P, D, P, D, P, P]
         _ref = lib
     return _ref
 
@@ -148,6 +155,43 @@ def run_oracle(spec: abi.Spec, threads: int = 1, rows: bool = False):
         raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
     abi.sync_outs(outs, arr)
     return outs
+
+
+def sim_grid_size(spec: abi.Spec, i: int) -> int:
+    n = C.c_int32()
+    oracle().ccr_oracle_simulation_grid(abi.dptr(spec.base_grid), len(spec.base_grid), C.byref(spec.sets_c[i]),
+                                        None, 0, C.byref(n))
+    return n.value
+
+
+def oracle_initial_margin(spec: abi.Spec, i: int, quantile: float, mpor: float = 10.0 / 250.0,
+                          threads: int = 4) -> np.ndarray:
+    """initialMarginProfile of netting set i's netted value on its simulation grid."""
+    cap = sim_grid_size(spec, i)
+    outs, arr = abi.make_outs(1, grid_size(spec, i), spec.num_paths, False)
+    margin = np.zeros(cap)
+    n = C.c_int32()
+    err = C.create_string_buffer(512)
+    rc = oracle().ccr_oracle_initial_margin(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                            len(spec.base_grid), C.byref(spec.sets_c[i]), quantile, mpor, threads,
+                                            arr, abi.dptr(margin), cap, C.byref(n), err, 512)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
+    return margin[:n.value]
+
+
+def ref_initial_margin(spec: abi.Spec, i: int, quantile: float, mpor: float = 10.0 / 250.0) -> np.ndarray:
+    lib = ref()
+    cap = sim_grid_size(spec, i)
+    margin = np.zeros(cap)
+    n = C.c_int32()
+    err = C.create_string_buffer(512)
+    rc = lib.ccr_ref_initial_margin(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                    len(spec.base_grid), C.byref(spec.sets_c[i]), quantile, mpor, abi.dptr(margin),
+                                    cap, C.byref(n), err, 512)
+    if rc != 0:
+        raise RuntimeError(f"reference failed ({rc}): {err.value.decode()}")
+    return margin[:n.value]
 
 
 def oracle_timing() -> OracleTiming:

@@ -263,6 +263,75 @@ def test_oracle_matches_live_reference_build(kw):
     assert compare.worst(errs) <= (1e-14 if kw.get("two_leg") else 0.0), errs
 
 
+def _ref_has(name):
+    r = oracle_lib.ref()
+    return r is not None and hasattr(r, name)
+
+
+@pytest.mark.skipif(not _ref_has("ccr_ref_initial_margin"), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("kw", [dict(num_paths=200, fx=True, credits=True),
+                                dict(num_paths=150, csa=True, fx=True, credits=True),
+                                dict(num_paths=120, csa=True, bermudans=3),
+                                dict(num_paths=150, fx=True, options=6)])
+def test_oracle_initial_margin_matches_live_reference_build(kw):
+    """initialMarginProfile (src/xva/xva.cpp:59-76) of the reference's own netted value and
+    deflator against the oracle's, on the simulation grid (CSA lag dates included)."""
+    spec = W.small_reference_case(**kw)
+    for q, mpor in ((0.99, 10.0 / 250.0), (0.5, 0.05)):
+        o = oracle_lib.oracle_initial_margin(spec, 0, q, mpor, threads=3)
+        r = oracle_lib.ref_initial_margin(spec, 0, q, mpor)
+        assert len(o) == len(r) == oracle_lib.sim_grid_size(spec, 0)
+        assert o[0] == 0.0 and r.max() > 0.0
+        assert np.array_equal(o, r), np.abs(o - r).max()
+
+
+def test_oracle_initial_margin_scaling_and_extremes():
+    """Properties that fix the formula's terms: margin[0] = 0, margin proportional to
+    sqrt(mpor), the 100% quantile is the largest change, and a lower level never gives more."""
+    spec = W.small_reference_case(num_paths=300, csa=True, fx=True)
+    a = oracle_lib.oracle_initial_margin(spec, 0, 0.99, 0.04)
+    b = oracle_lib.oracle_initial_margin(spec, 0, 0.99, 0.16)
+    live = a > 0.0  # dates after the last trade matures have no value change
+    assert a[0] == 0.0 and b[0] == 0.0 and live.sum() >= len(a) - 3
+    assert np.allclose(b, 2.0 * a, rtol=1e-14, atol=0.0)
+    top = oracle_lib.oracle_initial_margin(spec, 0, 1.0, 0.04)
+    low = oracle_lib.oracle_initial_margin(spec, 0, 0.5, 0.04)
+    assert (top >= a).all() and (a >= low).all() and (top[live] > low[live]).all()
+
+
+@pytest.mark.skipif(not _ref_has("ccr_ref_xva"), reason="reference build (oracle/_ref) not present")
+def test_xva_matches_live_reference_build():
+    """xva (src/xva/xva.cpp:78-96) of the reference against the oracle's restatement and the C ABI's
+    ccr_xva, with and without an initial-margin profile."""
+    from zlatko_amd import native
+    t = np.array([0.0, 0.1, 0.35, 0.5, 1.0, 1.7, 2.0, 3.0, 4.5, 5.0])
+    dee = 1e6 * 0.02 * np.sqrt(t) * (5.0 - t) / 5.0
+    dene = -0.4 * dee - 1e3 * t
+    im = 3e4 * np.sqrt(t)
+    cp, ch = np.array([1.0, 3.0, 5.0]), np.array([0.012, 0.018, 0.025])
+    op, oh = np.array([2.0, 6.0]), np.array([0.006, 0.009])
+    cpty, own = abi.CreditCurveC(abi.dptr(cp), abi.dptr(ch), 3), abi.CreditCurveC(abi.dptr(op), abi.dptr(oh), 2)
+    p = abi.XvaParametersC()
+    native.lib().ccr_xva_defaults(C.byref(p))
+    p.funding_spread_lend = 0.004
+    for margin in (None, im):
+        mp = abi.dptr(margin) if margin is not None else None
+        want, got, mine = abi.XvaResultC(), abi.XvaResultC(), np.zeros(6)
+        assert oracle_lib.ref().ccr_ref_xva(abi.dptr(t), abi.dptr(dee), abi.dptr(dene), len(t), mp, C.byref(cpty),
+                                            0.4, C.byref(own), 0.35, C.byref(p), C.byref(want)) == 0
+        assert native.lib().ccr_xva(abi.dptr(t), abi.dptr(dee), abi.dptr(dene), len(t), mp, C.byref(cpty),
+                                    C.c_double(0.4), C.byref(own), C.c_double(0.35), C.byref(p), C.byref(got)) == 0
+        params = np.array([p.funding_spread_borrow, p.funding_spread_lend, p.margin_funding_spread, p.hurdle_rate,
+                           p.capital_ratio, p.risk_weight, p.alpha])
+        O().ccr_oracle_xva(abi.dptr(t), abi.dptr(dee), abi.dptr(dene), len(t), mp, C.byref(cpty), 0.4,
+                           C.byref(own), 0.35, abi.dptr(params), abi.dptr(mine))
+        w = np.array([want.cva, want.dva, want.fca, want.fba, want.mva, want.kva])
+        g = np.array([got.cva, got.dva, got.fca, got.fba, got.mva, got.kva])
+        assert (w[:4] > 0).all() and (w[5] > 0) and (w[4] > 0) == (margin is not None)
+        assert np.allclose(g, w, rtol=1e-13, atol=1e-9), (g, w)
+        assert np.allclose(mine, w, rtol=1e-13, atol=1e-9), (mine, w)
+
+
 # ------------------------------------------------------------ invariants (oracle-only extensions included)
 
 def test_threads_do_not_change_bits():

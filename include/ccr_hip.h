@@ -381,6 +381,16 @@ int32_t ccr_hip_set_fault(ccr_hip_ctx* ctx, int32_t fault);
    train a surrogate. 0 restores owned buffers for later compiles. */
 int32_t ccr_hip_set_shared_workspace(ccr_hip_ctx* ctx, int32_t enable);
 
+/* initialMarginProfile (src/xva/xva.cpp:59-76) of every netting set, on the GPU: plans compiled
+ * after ccr_hip_set_initial_margin(ctx, quantile, mpor) with quantile in (0, 1] also write
+ * |V(t_k) - V(t_k-1)| of the netted value on every simulation date and select its quantile over
+ * the paths with the PFE kernel; quantile <= 0 switches it off for later compiles. After an
+ * execute, ccr_hip_initial_margin gives one value per simulation date of netting set `set`
+ * (ccr_hip_simulation_grid; 0 at t = 0): quantile x sqrt(mpor / dt) x the mean deflator. */
+int32_t ccr_hip_set_initial_margin(ccr_hip_ctx* ctx, double quantile, double margin_period_of_risk);
+int32_t ccr_hip_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set, double* margin,
+                               int32_t capacity, int32_t* size);
+
 /* ExposureEngine::reportingGrid / simulationGrid of one netting set (exposure_engine.cpp:35-48).
  * Writes up to `capacity` times and the full size into *size. Needs no device. */
 int32_t ccr_hip_reporting_grid(const double* base_grid, int32_t base_size, const ccr_netting_set* set,

@@ -273,3 +273,85 @@ def test_shared_workspace_waves_match_owned_buffers():
         for o, w in zip(outs, want):
             for k in abi.PROFILE_FIELDS:
                 assert np.array_equal(o.get(k), w.get(k)), k
+
+
+IM_CASES = {
+    "csa_fx_credits": dict(csa=True, fx=True, credits=True),
+    "two_leg": dict(two_leg=True),
+    "bermudans": dict(csa=True, fx=True, bermudans=3),
+    "options": dict(fx=True, credits=True, options=6),
+}
+
+
+def _im_check(plan, spec, quantile, mpor, tol):
+    worst = 0.0
+    for s in range(len(spec.sets)):
+        got = plan.initial_margin(s)
+        want = oracle_lib.oracle_initial_margin(spec, s, quantile, mpor)
+        assert len(got) == len(want) and got[0] == 0.0
+        err = np.abs(got - want).max() / max(np.abs(want).max(), 1.0)
+        worst = max(worst, err)
+        assert err <= tol, (s, err)
+    return worst
+
+
+@pytest.mark.parametrize("case", sorted(IM_CASES))
+def test_initial_margin_matches_oracle_fp64(case):
+    """initialMarginProfile (src/xva/xva.cpp:59-76) from the path kernel's |V(t_k) - V(t_k-1)|
+    rows and per-wave deflator sums, against the oracle on the same paths (700 paths: several
+    wavefronts per set); the exposure profile of the same execute is unchanged."""
+    from zlatko_amd import native
+    spec = W.small_reference_case(num_paths=700, **IM_CASES[case])
+    e = native.Engine()
+    e.set_initial_margin(0.99, 10.0 / 250.0)
+    plan = e.compile(spec)
+    _check(spec, plan.execute(), oracle_lib.run_oracle(spec, threads=4))
+    print(case, f"{_im_check(plan, spec, 0.99, 10.0 / 250.0, FP64_TOL):.2e}", flush=True)
+    plan.close()
+    e.close()
+
+
+def test_initial_margin_many_sets_fp64_and_fp32():
+    """Several netting sets of uneven size in one plan (FP64 within 1e-10; the FP32 plan's error
+    against the FP64 oracle is printed and bounded by 1e-4 of the largest margin)."""
+    from zlatko_amd import native
+    spec = W.c2(num_paths=1000, num_sets=4, num_trades=80, weeks=104, horizon=2.0)
+    e = native.Engine()
+    e.set_initial_margin(0.975, 0.05)
+    p64, p32 = e.compile(spec, abi.CCR_FP64), e.compile(spec, abi.CCR_FP32)
+    p64.execute()
+    p32.execute()
+    _im_check(p64, spec, 0.975, 0.05, FP64_TOL)
+    print("fp32 initial margin error", f"{_im_check(p32, spec, 0.975, 0.05, 1e-4):.2e}", flush=True)
+    p64.close()
+    p32.close()
+    e.close()
+
+
+def test_initial_margin_errors():
+    """A plan compiled without it, or not executed yet, refuses; quantile 0 switches it off."""
+    from zlatko_amd import native
+    spec = W.small_reference_case(num_paths=128, csa=True)
+    e = native.Engine()
+    plain = e.compile(spec)
+    plain.execute()
+    with pytest.raises(native.CcrError):
+        plain.initial_margin(0)
+    e.set_initial_margin(0.99)
+    fresh = e.compile(spec)
+    with pytest.raises(native.CcrError):
+        fresh.initial_margin(0)
+    with pytest.raises(native.CcrError):
+        e.set_initial_margin(0.99, -1.0)
+    fresh.execute()
+    with pytest.raises(native.CcrError):
+        fresh.initial_margin(1)
+    assert fresh.initial_margin(0)[0] == 0.0
+    e.set_initial_margin(0.0)
+    off = e.compile(spec)
+    off.execute()
+    with pytest.raises(native.CcrError):
+        off.initial_margin(0)
+    for p in (plain, fresh, off):
+        p.close()
+    e.close()

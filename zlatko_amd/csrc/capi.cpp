@@ -55,6 +55,8 @@ struct ccr_hip_ctx {
   bool share_ws = false;
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  // Initial-margin proxy for plans compiled while im_quantile > 0.
+  double im_quantile = 0.0, im_mpor = 10.0 / 250.0;
 };
 
 namespace {
@@ -103,6 +105,15 @@ struct ccr_hip_plan {
   // Per-execute buffers placed in the context's shared workspace: (field, byte offset).
   std::vector<std::pair<void**, size_t>> ws_slots;
   size_t ws_need = 0;
+  // Initial-margin proxy (null / 0 when off): |dV| rows, per-wave deflator sums, their quantiles
+  // and sums per (set, simulation date), and the last execute's read-back.
+  double im_quantile = 0.0, im_mpor = 0.0;
+  void* d_im_rows = nullptr;
+  double* d_im_part = nullptr;
+  double* d_im_q = nullptr;
+  double* d_im_dsum = nullptr;
+  std::vector<double> im_q, im_dsum;
+  bool im_ready = false;
   ~ccr_hip_plan() {
     for (auto& b : bufs)
       if (b.p) (void)hipFree(b.p);
@@ -328,6 +339,26 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   const int64_t P = h.g.num_paths;
   execAlloc(ctx, plan.get(), plan->d_rows, static_cast<size_t>(num_sets) * h.max_rep * P * elem);
   execAlloc(ctx, plan.get(), plan->d_partials, static_cast<size_t>(h.total_waves) * h.max_rep * kStats * sizeof(double));
+  if (ctx->im_quantile > 0.0) {
+    // The proxy needs the netted value on every simulation date (true of every plan: without a
+    // CSA the simulation grid is the reporting grid, with one every date is valued).
+    for (const DateRec& d : h.dates)
+      if (!(d.flags & kValued)) invalid("initial margin: a simulation date is not valued");
+    plan->im_quantile = ctx->im_quantile;
+    plan->im_mpor = ctx->im_mpor;
+    const int64_t dates = h.max_dates, wavesPerSet = h.total_waves / num_sets;
+    for (int s = 0; s < num_sets; ++s) {
+      h.sets[static_cast<size_t>(s)].im_off = static_cast<int64_t>(s) * (dates - 1) * P;
+      h.sets[static_cast<size_t>(s)].im_part_off = static_cast<int64_t>(s) * wavesPerSet * dates;
+    }
+    execAlloc(ctx, plan.get(), plan->d_im_rows, static_cast<size_t>(num_sets) * (dates > 1 ? dates - 1 : 1) * P * elem);
+    execAlloc(ctx, plan.get(), plan->d_im_part, static_cast<size_t>(h.total_waves) * dates * sizeof(double));
+    plan->d_im_q = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * dates * sizeof(double)));
+    plan->d_im_dsum = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * dates * sizeof(double)));
+    check(hipMemcpyAsync(plan->d_sets, h.sets.data(), h.sets.size() * sizeof(SetDev), hipMemcpyHostToDevice,
+                         ctx->stream),
+          "hipMemcpyAsync");
+  }
   plan->d_stats = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * kStats * sizeof(double)));
   plan->d_pfe = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * sizeof(double)));
   if (!h.berms.empty()) {
@@ -428,6 +459,9 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   a.nassets = h.g.nassets;
   a.ncredits = h.g.ncredits;
   a.has_options = h.opts.empty() ? 0 : 1;
+  a.im_rows = static_cast<Real*>(plan->d_im_rows);
+  a.im_part = plan->d_im_part;
+  a.im_dates = h.max_dates;
   a.state_len = h.state_len;
   return a;
 }
@@ -556,6 +590,23 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* su
     check(launchPfe<double>(plan->d_sets, static_cast<const double*>(plan->d_rows), nsets, h.max_rep, h.g.num_paths,
                             h.pfe_q, plan->d_pfe, ctx->stream),
           "pfe kernel launch");
+  if (plan->d_im_q) {
+    const int dates = h.max_dates;
+    if (dates > 1) {
+      if (h.precision == CCR_FP32)
+        check(launchPfe<float>(plan->d_sets, static_cast<const float*>(plan->d_im_rows), nsets, dates - 1,
+                               h.g.num_paths, plan->im_quantile, plan->d_im_q, ctx->stream, 1),
+              "initial-margin select launch");
+      else
+        check(launchPfe<double>(plan->d_sets, static_cast<const double*>(plan->d_im_rows), nsets, dates - 1,
+                                h.g.num_paths, plan->im_quantile, plan->d_im_q, ctx->stream, 1),
+              "initial-margin select launch");
+    }
+    check(launchImReduce(plan->d_sets, plan->d_im_part, nsets, dates, h.total_waves / nsets, plan->d_im_dsum,
+                         ctx->stream),
+          "initial-margin reduce launch");
+    plan->im_ready = false;
+  }
   check(hipEventRecord(ctx->ev[4], ctx->stream), "hipEventRecord");
   ctx->timing_pending = true;
   ctx->upload_ms = plan->upload_ms;
@@ -590,7 +641,18 @@ void readBack(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs) {
         "hipMemcpyAsync");
   check(hipMemcpyAsync(pfe.data(), plan->d_pfe, pfe.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
         "hipMemcpyAsync");
+  if (plan->d_im_q) {
+    const size_t n = nsets * static_cast<size_t>(h.max_dates);
+    plan->im_q.resize(n);
+    plan->im_dsum.resize(n);
+    check(hipMemcpyAsync(plan->im_q.data(), plan->d_im_q, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+          "hipMemcpyAsync");
+    check(hipMemcpyAsync(plan->im_dsum.data(), plan->d_im_dsum, n * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream),
+          "hipMemcpyAsync");
+  }
   check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  plan->im_ready = plan->d_im_q != nullptr;
   readTimings(ctx);
   const int64_t P = h.g.num_paths;
   for (size_t s = 0; s < nsets; ++s) {
@@ -705,6 +767,45 @@ void ccr_hip_destroy(ccr_hip_ctx* ctx) {
 }
 
 const char* ccr_hip_last_error(const ccr_hip_ctx* ctx) { return ctx ? ctx->err.c_str() : gCreateError.c_str(); }
+
+int32_t ccr_hip_set_initial_margin(ccr_hip_ctx* ctx, double quantile, double margin_period_of_risk) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  if (quantile > 1.0 || (quantile > 0.0 && !(margin_period_of_risk > 0.0))) {
+    ctx->err = "initial margin: quantile in (0, 1] and a positive margin period of risk";
+    return CCR_ERR_INVALID_ARGUMENT;
+  }
+  ctx->im_quantile = quantile > 0.0 ? quantile : 0.0;
+  ctx->im_mpor = margin_period_of_risk;
+  return CCR_OK;
+}
+
+int32_t ccr_hip_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set, double* margin,
+                               int32_t capacity, int32_t* size) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!plan || !margin) invalid("null argument");
+    if (!plan->d_im_q) invalid("initial margin: the plan was compiled without ccr_hip_set_initial_margin");
+    if (!plan->im_ready) invalid("initial margin: execute the plan (ccr_hip_execute) first");
+    const HostPlan& h = plan->host;
+    if (set < 0 || set >= static_cast<int32_t>(h.sets.size())) fail(CCR_ERR_OUT_OF_RANGE, "netting set index out of range");
+    const std::vector<double>& t = h.sim_times[static_cast<size_t>(set)];
+    const int32_t n = static_cast<int32_t>(t.size());
+    if (size) *size = n;
+    if (capacity < n) invalid("initial margin: capacity smaller than the simulation grid");
+    // initialMarginProfile (src/xva/xva.cpp:59-76): q-quantile of |V(t_k) - V(t_k-1)| over the
+    // paths x sqrt(mpor / dt) x the sum of the deflators / P; 0 at t = 0.
+    // Quantiles: one per date after the first, (max_dates - 1) per set; deflator sums: one per
+    // date, max_dates per set.
+    const size_t qBase = static_cast<size_t>(set) * static_cast<size_t>(h.max_dates - 1);
+    const size_t dBase = static_cast<size_t>(set) * static_cast<size_t>(h.max_dates);
+    const double paths = static_cast<double>(h.g.num_paths);
+    margin[0] = 0.0;
+    for (int32_t k = 1; k < n; ++k) {
+      const double scale = std::sqrt(plan->im_mpor / (t[static_cast<size_t>(k)] - t[static_cast<size_t>(k - 1)]));
+      margin[k] = plan->im_q[qBase + static_cast<size_t>(k - 1)] * scale * plan->im_dsum[dBase + static_cast<size_t>(k)] / paths;
+    }
+  });
+}
 
 int32_t ccr_hip_set_shared_workspace(ccr_hip_ctx* ctx, int32_t enable) {
   if (!ctx) return CCR_ERR_INVALID_ARGUMENT;

@@ -57,6 +57,16 @@ ccr_path_kernel(PathArgs<Real> a) {
   double* part = a.partials + S.partial_off + static_cast<int64_t>(waveLocal) * a.max_rep * kStats;
   Real* rows = a.rows + S.rows_off;
   Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
+  // Initial-margin proxy (uniform switch): |V_j - V_j-1| per path and the wavefront's deflator
+  // sum on every valued date.
+  Real vPrev = 0;
+  auto onValue = [&](int j, Real V, Real D) {
+    if (!a.im_rows) return;
+    if (j > 0 && active) a.im_rows[S.im_off + static_cast<int64_t>(j - 1) * g.num_paths + p] = V > vPrev ? V - vPrev : vPrev - V;
+    vPrev = V;
+    const double sD = waveSum(active ? static_cast<double>(D) : 0.0);
+    if ((tid & 63) == 0) a.im_part[S.im_part_off + static_cast<int64_t>(waveLocal) * a.im_dates + j] = sD;
+  };
   simulatePath<Real, Sh, kOptions>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
     if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
     // The first eight statistics in one transposed wavefront sum (lanes 0..7 store one each);
@@ -71,7 +81,7 @@ ccr_path_kernel(PathArgs<Real> a) {
     if constexpr (!(Sh::kExact && Sh::NC == 0)) dva = waveSum(active ? st9[8] : 0.0);
     const int lane = tid & 63;
     if (lane < 9) part[rr * kStats + (lane < 8 ? idx : 8)] = lane < 8 ? s : dva;
-  });
+  }, onValue);
 }
 
 // States pass for American Monte Carlo: the domestic state and the deflator of every date of the
@@ -251,7 +261,8 @@ __device__ inline void pfeFindBin(const unsigned int* hist, long long rank, unsi
 // quantile) an MSB radix select over 12-bit digits narrows the prefix until its keys fit in LDS.
 template <typename Real, int kPfeKeys>
 __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets, const Real* rows, int max_rep,
-                                                              int num_paths, double level, double* out) {
+                                                              int num_paths, double level, double* out,
+                                                              int which) {
   using K = typename KeyOf<Real>::T;
   constexpr int kBits = KeyOf<Real>::kBits;
   __shared__ K keys[kPfeKeys];
@@ -262,8 +273,10 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
   __shared__ int sBin;
   const int s = static_cast<int>(blockIdx.x) / max_rep, rr = static_cast<int>(blockIdx.x) % max_rep;
   const SetDev& S = sets[s];
-  if (rr >= S.num_rep) return;
-  const Real* row = rows + S.rows_off + static_cast<int64_t>(rr) * num_paths;
+  // which 0: the exposure rows (one per reporting date); 1: the initial-margin proxy's
+  // |V_j - V_j-1| rows (one per simulation date after the first).
+  if (rr >= (which ? S.num_dates - 1 : S.num_rep)) return;
+  const Real* row = rows + (which ? S.im_off : S.rows_off) + static_cast<int64_t>(rr) * num_paths;
   const int tid = static_cast<int>(threadIdx.x), lane = tid & 63;
   const double h = level * static_cast<double>(num_paths - 1);
   const long long lo = static_cast<long long>(floor(h));
@@ -552,21 +565,41 @@ hipError_t launchReduce(const SetDev* sets, const double* partials, int nsets, i
 
 template <typename Real>
 hipError_t launchPfe(const SetDev* sets, const Real* rows, int nsets, int max_rep, int num_paths, double level,
-                     double* out, hipStream_t stream) {
+                     double* out, hipStream_t stream, int which) {
   // Rows beyond 100,000 paths take an 8,192-key buffer, so the keys between the splitters of a
   // 4,096-key sample still fit.
   if (num_paths > 100000)
     hipLaunchKernelGGL((ccr_pfe_kernel<Real, 8192>), dim3(nsets * max_rep), dim3(256), 0, stream, sets, rows, max_rep,
-                       num_paths, level, out);
+                       num_paths, level, out, which);
   else
     hipLaunchKernelGGL((ccr_pfe_kernel<Real, 4096>), dim3(nsets * max_rep), dim3(256), 0, stream, sets, rows, max_rep,
-                       num_paths, level, out);
+                       num_paths, level, out, which);
+  return hipGetLastError();
+}
+
+// Sum of the per-wave deflator sums of every (set, simulation date), in wave order.
+__global__ void ccr_im_reduce_kernel(const SetDev* sets, const double* part, int nsets, int dates, int waves_per_set,
+                                     double* out) {
+  const int idx = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (idx >= nsets * dates) return;
+  const int s = idx / dates, j = idx % dates;
+  const double* p = part + sets[s].im_part_off + j;
+  double sum = 0.0;
+  for (int w = 0; w < waves_per_set; ++w) sum += p[static_cast<int64_t>(w) * dates];
+  out[idx] = sum;
+}
+
+hipError_t launchImReduce(const SetDev* sets, const double* part, int nsets, int dates, int waves_per_set, double* out,
+                          hipStream_t stream) {
+  const int n = nsets * dates;
+  hipLaunchKernelGGL(ccr_im_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, sets, part, nsets, dates,
+                     waves_per_set, out);
   return hipGetLastError();
 }
 
 template hipError_t launchPath<double>(const PathArgs<double>&, int, int, hipStream_t);
 template hipError_t launchPath<float>(const PathArgs<float>&, int, int, hipStream_t);
-template hipError_t launchPfe<double>(const SetDev*, const double*, int, int, int, double, double*, hipStream_t);
-template hipError_t launchPfe<float>(const SetDev*, const float*, int, int, int, double, double*, hipStream_t);
+template hipError_t launchPfe<double>(const SetDev*, const double*, int, int, int, double, double*, hipStream_t, int);
+template hipError_t launchPfe<float>(const SetDev*, const float*, int, int, int, double, double*, hipStream_t, int);
 
 }  // namespace ccrhip

@@ -49,6 +49,11 @@ struct PathArgs {
   int32_t nrates, nassets, ncredits;  // model shape: selects the kernel specialisation
   int32_t has_options;                // European options present: selects the kernel variant
   int64_t state_len;                  // elements of x in `states` (D follows)
+  // Initial-margin proxy (null when off): |V_j - V_j-1| rows per set from SetDev::im_off and
+  // per-wave deflator sums from SetDev::im_part_off, im_dates dates per row block.
+  Real* im_rows;
+  double* im_part;
+  int32_t im_dates;
 };
 
 // American Monte Carlo regressions (amc.hip). Layouts: states x then D, [set dates][path];
@@ -126,6 +131,9 @@ hipError_t launchReduce(const SetDev* sets, const double* partials, int nsets, i
                         double inv_paths, double* out, hipStream_t stream);
 template <typename Real>
 hipError_t launchPfe(const SetDev* sets, const Real* rows, int nsets, int max_rep, int num_paths, double level,
-                     double* out, hipStream_t stream);
+                     double* out, hipStream_t stream, int which = 0);
+// Per (set, simulation date) deflator sums of the initial-margin proxy from its per-wave sums.
+hipError_t launchImReduce(const SetDev* sets, const double* part, int nsets, int dates, int waves_per_set, double* out,
+                          hipStream_t stream);
 
 }  // namespace ccrhip

@@ -153,6 +153,8 @@ struct SetDev {
   int32_t opt_off;             // into the option-term table
   int64_t state_off;           // first element of this set's [date][path] states (AMC), -1 without
   int64_t dom_off;             // first element of its domestic normals [step][2][path block] (AMC), -1 without
+  int64_t im_off;              // initial-margin proxy: first element of its |V_j - V_j-1| rows [date][path]
+  int64_t im_part_off;         // and of its per-wave deflator sums [wave][date]
   double thr_cpty, thr_own, mta, ia, keep;  // CSA terms (thr_own < 0 encodes a one-way CSA)
 };
 

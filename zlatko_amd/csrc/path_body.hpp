@@ -237,14 +237,16 @@ inline auto withShape(int nrates, int nassets, int ncredits, F&& f) {
 // then the fixing states. emit(rep, stats[kStats], positive_exposure) is called on every
 // reporting date.
 // kOptions: the plan holds European options (the Black evaluation is compiled in only then, so
-// option-free plans keep a compact loop).
-template <typename Real, typename Sh, bool kOptions, typename Emit>
+// option-free plans keep a compact loop). onValue(j, V, D) is called on every valued date with
+// the netted value before collateral and the deflator.
+template <typename Real, typename Sh, bool kOptions, typename Emit, typename OnValue>
 __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
                                              cptr<int32_t> fstore, cptr<double> poly, cptr<Real> opts,
                                              const double* bermv, int ring,
-                                             int priceRows, Real* col, int stride, Emit&& emit) {
+                                             int priceRows, Real* col, int stride, Emit&& emit,
+                                             OnValue&& onValue) {
   constexpr bool kDouble = std::is_same<Real, double>::value;
   constexpr int NR = Sh::NR, NA = Sh::NA, NC = Sh::NC;
   const CCR_CONST GlobalDev& g = *gp;
@@ -542,6 +544,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
       if (diff >= static_cast<Real>(S.mta)) units = target / (keep * pc);
       C = units * price + static_cast<Real>(S.ia);
     }
+    if (rec.flags & kValued) onValue(j, V, D);
     if (!(rec.flags & kReporting)) continue;
     const double v = static_cast<double>(V) - static_cast<double>(C);
     const double d = static_cast<double>(D);
@@ -573,9 +576,9 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 }
 
 // Convenience: the tables of set S for path pe, from a PathArgs bundle.
-template <typename Real, typename Sh, bool kOptions = true, typename Emit>
+template <typename Real, typename Sh, bool kOptions = true, typename Emit, typename OnValue>
 __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, Real* col, int stride,
-                                             Emit&& emit) {
+                                             Emit&& emit, OnValue&& onValue) {
   const CCR_CONST SetDev& S = *sp;
   cptr<GlobalDev> g = toConst(a.g);
   const int64_t q = g->antithetic ? (pe >> 1) : pe;
@@ -588,7 +591,12 @@ __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDe
                                    toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
                                    toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off,
                                    toConst(a.opts) + kOptStride * static_cast<int64_t>(S.opt_off), bermv, a.ring,
-                                   a.price_rows, col, stride, emit);
+                                   a.price_rows, col, stride, emit, onValue);
+}
+template <typename Real, typename Sh, bool kOptions = true, typename Emit>
+__host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, Real* col, int stride,
+                                             Emit&& emit) {
+  simulatePath<Real, Sh, kOptions>(a, sp, pe, col, stride, emit, [](int, Real, Real) {});
 }
 
 // The AMC states pass for path pe of set S: the domestic state x and the deflator D of every

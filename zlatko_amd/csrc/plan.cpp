@@ -408,6 +408,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     const TimeGrid sim = simulationGrid(base, ns);
     const int T = static_cast<int>(sim.size()), R = static_cast<int>(rep.size());
     plan.rep_times.push_back(rep.t);
+    plan.sim_times.push_back(sim.t);
 
     SetDev sd{};
     sd.num_dates = T;

@@ -24,6 +24,7 @@ struct HostPlan {
   std::vector<int32_t> fixstore;  // [n][2] (rate, slot)
   std::vector<int32_t> block_set; // set of each workgroup (before the XCD remap)
   std::vector<std::vector<double>> rep_times;  // reporting dates per set
+  std::vector<std::vector<double>> sim_times;  // simulation dates per set
   int64_t stream_len = 0;         // normals in the replayed stream
   int32_t max_dates = 0, max_rep = 0, total_waves = 0;
   int32_t block = 256;

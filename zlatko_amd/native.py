@@ -43,6 +43,8 @@ def lib():
         L.ccr_hip_last_error.restype = C.c_char_p
         L.ccr_hip_set_fault.argtypes = [P, I32]
         L.ccr_hip_set_shared_workspace.argtypes = [P, I32]
+        L.ccr_hip_set_initial_margin.argtypes = [P, D, D]
+        L.ccr_hip_initial_margin.argtypes = [P, P, I32, P, I32, P]
         L.ccr_xva_defaults.argtypes = [P]
         L.ccr_xva_defaults.restype = None
         L.ccr_exposure_xva.argtypes = [P, P, I32, P, D, P, P, P]
@@ -135,6 +137,16 @@ class Plan:
     def execute_async(self):
         self.engine._check(lib().ccr_hip_execute_async(self.engine.ctx, self.handle))
 
+    def initial_margin(self, s: int = 0) -> np.ndarray:
+        """initialMarginProfile (src/xva/xva.cpp:59-76) of netting set `s` from the last execute,
+        one value per simulation date."""
+        info = self.info()
+        out = np.zeros(max(1, info.max_sim_dates))
+        n = C.c_int32()
+        self.engine._check(lib().ccr_hip_initial_margin(self.engine.ctx, self.handle, s, abi.dptr(out), len(out),
+                                                        C.byref(n)))
+        return out[:n.value].copy()
+
     def execute_surrogate(self, surrogate: "Surrogate", rows: bool = False):
         info = self.info()
         outs, arr = abi.make_outs(len(self.spec.sets), info.max_reporting_dates, self.spec.num_paths, rows)
@@ -225,6 +237,11 @@ class Engine:
         """Plans compiled from now on keep their per-execute buffers in one workspace of the
         context (valid until the next execute of any plan), so waves of netting sets fit in HBM."""
         self._check(lib().ccr_hip_set_shared_workspace(self.ctx, 1 if enable else 0))
+
+    def set_initial_margin(self, quantile: float, margin_period_of_risk: float = 10.0 / 250.0):
+        """Plans compiled from now on also produce the initial-margin profile of every netting set
+        (quantile <= 0 switches it off)."""
+        self._check(lib().ccr_hip_set_initial_margin(self.ctx, quantile, margin_period_of_risk))
 
     def autotune(self, spec: abi.Spec, precision: int = abi.CCR_FP64, sample_paths: int = 16384):
         """Picks the path kernel's workgroup size for this workload; returns (size, times in ms)."""

@@ -906,9 +906,15 @@ double secondsSince(std::chrono::steady_clock::time_point t0) {
 // extras: E[D], E[Q], pathwise CVA/DVA increments on the reporting dates).
 // im (optional): initialMarginProfile (src/xva/xva.cpp:59-76) of the netted value on the
 // simulation grid, imQ its quantile level, imMpor the margin period of risk.
+// tstats (optional): per trade i, kTradeStatCount rows of one value per reporting date
+// ([i][m][r] in tstats[i][m * nR + r]): the statistics of gpu per-trade mode
+// (src/gpu/fused_kernels.cpp:174-183) -- E[D V_i 1{V>0}], E[V_i 1{V>0}], E[D (V+ - (V - V_i)+)],
+// E[D V_i+], E[V_i+] with V the netted value before collateral, the estimators of
+// marginalExposureContributions and incrementalExposure (src/exposure/allocation.cpp:8-45).
+constexpr int kTradeStatCount = 5;
 void runSet(const Generator& g, const ccr_simulation_config& cfg, const Grid& base, const ccr_netting_set& ns,
             double q, int threads, ccr_exposure_out* out, Vec* im = nullptr, double imQ = 0.99,
-            double imMpor = 10.0 / 250.0) {
+            double imMpor = 10.0 / 250.0, std::vector<Vec>* tstats = nullptr) {
   if (ns.num_trades <= 0 || !ns.trades) invalid("ExposureEngine::run: empty netting set");
   std::vector<Trade> trades;
   for (int i = 0; i < ns.num_trades; ++i) trades.push_back(makeTrade(ns.trades[i], g));
@@ -931,6 +937,7 @@ void runSet(const Generator& g, const ccr_simulation_config& cfg, const Grid& ba
   for (std::size_t i = 0; i < trades.size(); ++i)
     if (trades[i].kind == CCR_TRADE_BERMUDAN_SWAPTION) berm[i] = std::make_unique<BermudanValuation>(bermudan(trades[i], g, s));
   Mat net(nP, nT, 0.0);
+  std::vector<Mat> tv(tstats ? trades.size() : 0, Mat(tstats ? nP : 0, tstats ? nT : 0, 0.0));
   // Under a CSA every simulation date is needed (margin-call look-back); otherwise the
   // simulation grid is the reporting grid.
   parallelFor(nP, threads, [&](std::size_t p) {
@@ -938,11 +945,31 @@ void runSet(const Generator& g, const ccr_simulation_config& cfg, const Grid& ba
       double total = 0.0;
       for (std::size_t i = 0; i < trades.size(); ++i) {
         const double v = berm[i] ? berm[i]->value(p, j) : pathwiseValue(trades[i], g, s, p, j);
+        if (tstats) tv[i](p, j) = v;
         total = i == 0 ? v : total + v;
       }
       net(p, j) = total;
     }
   });
+  if (tstats) {
+    tstats->assign(trades.size(), Vec(static_cast<std::size_t>(kTradeStatCount) * nR, 0.0));
+    for (std::size_t i = 0; i < trades.size(); ++i)
+      for (std::size_t r = 0; r < nR; ++r) {
+        const std::size_t j = ri[r];
+        double sum[kTradeStatCount] = {0, 0, 0, 0, 0};
+        for (std::size_t p = 0; p < nP; ++p) {
+          const double v = net(p, j), vi = tv[i](p, j), d = s.D(p, j);
+          const double on = v > 0.0 ? 1.0 : 0.0;
+          sum[0] += d * vi * on;
+          sum[1] += vi * on;
+          sum[2] += d * (std::max(v, 0.0) - std::max(v - vi, 0.0));
+          sum[3] += d * std::max(vi, 0.0);
+          sum[4] += std::max(vi, 0.0);
+        }
+        for (int m = 0; m < kTradeStatCount; ++m)
+          (*tstats)[i][static_cast<std::size_t>(m) * nR + r] = sum[m] / static_cast<double>(nP);
+      }
+  }
   gTiming.pricing_s += secondsSince(t0);
   gTiming.revaluations += static_cast<int64_t>(trades.size()) * static_cast<int64_t>(nP) * static_cast<int64_t>(nR);
   if (im) {
@@ -1088,6 +1115,28 @@ int32_t ccr_oracle_run_exposure(const ccr_scenario_generator* generator, const c
     const Grid base(Vec(base_grid, base_grid + std::max(0, base_size)));
     for (int i = 0; i < num_sets; ++i) runSet(g, *config, base, sets[i], pfe_quantile, threads, &outs[i]);
     gTiming.total_s = secondsSince(t0);
+  });
+}
+
+// Per-trade statistics of one netting set next to its exposure profile: stats[(i * 5 + m) *
+// capacity + r] for trade i, statistic m, reporting date r.
+int32_t ccr_oracle_trade_statistics(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                                    const double* base_grid, int32_t base_size, const ccr_netting_set* set,
+                                    int32_t threads, ccr_exposure_out* out, double* stats, int32_t capacity, char* err,
+                                    int32_t err_capacity) {
+  return guard(err, err_capacity, [&] {
+    if (!generator || !config || !set || !out || !stats) invalid("null argument");
+    const Generator g(*generator);
+    const Grid base(Vec(base_grid, base_grid + std::max(0, base_size)));
+    std::vector<Vec> ts;
+    runSet(g, *config, base, *set, 0.95, threads, out, nullptr, 0.99, 10.0 / 250.0, &ts);
+    const std::size_t nR = static_cast<std::size_t>(out->num_dates);
+    if (capacity < out->num_dates) invalid("capacity smaller than the reporting grid");
+    for (std::size_t i = 0; i < ts.size(); ++i)
+      for (int m = 0; m < kTradeStatCount; ++m)
+        for (std::size_t r = 0; r < nR; ++r)
+          stats[(i * kTradeStatCount + static_cast<std::size_t>(m)) * static_cast<std::size_t>(capacity) + r] =
+              ts[i][static_cast<std::size_t>(m) * nR + r];
   });
 }
 

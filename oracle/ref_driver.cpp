@@ -23,6 +23,7 @@
 #include "ccr/core/math.hpp"
 #include "ccr/core/random.hpp"
 #include "ccr/cva/cva.hpp"
+#include "ccr/exposure/allocation.hpp"
 #include "ccr/exposure/exposure_engine.hpp"
 #include "ccr/instruments/asset_forward.hpp"
 #include "ccr/instruments/bermudan_swaption.hpp"
@@ -155,6 +156,52 @@ int32_t ccr_ref_initial_margin(const ccr_scenario_generator* generator, const cc
     if (size) *size = static_cast<int32_t>(im.size());
     if (capacity < static_cast<int32_t>(im.size())) return fail(err, err_capacity, "capacity", CCR_ERR_INVALID_ARGUMENT);
     for (std::size_t k = 0; k < im.size(); ++k) margin[k] = im[k];
+    return CCR_OK;
+  } catch (const std::exception& e) {
+    return fail(err, err_capacity, e.what(), CCR_ERR_RUNTIME);
+  }
+}
+
+// Per-trade statistics of one netting set on its reporting dates, stats[(i * 5 + m) * capacity + k]
+// for trade i: m = 0 / 1 marginalExposureContributions with / without the deflator, 2
+// incrementalExposure of trade i added to the other trades' net (with the deflator), 3 / 4 the
+// trade's own discounted / undiscounted EE (ExposureProfile::compute of its values).
+int32_t ccr_ref_trade_statistics(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                                 const double* base_grid, int32_t base_size, const ccr_netting_set* set,
+                                 double* stats, int32_t capacity, char* err, int32_t err_capacity) {
+  try {
+    const ccr::ScenarioGenerator gen = generatorOf(*generator);
+    const ccr::SimulationConfig cfg{static_cast<std::size_t>(config->num_paths), config->seed, config->antithetic != 0};
+    const ccr::ExposureEngine engine(gen, cfg, ccr::TimeGrid(std::vector<double>(base_grid, base_grid + base_size)));
+    const ccr::ExposureResult r = engine.run(nettingSetOf(*set), 0.95);
+    const std::vector<std::size_t>& idx = r.reportingIndices;
+    const std::size_t nP = r.scenarios.numPaths, nR = idx.size();
+    if (capacity < static_cast<int32_t>(nR)) return fail(err, err_capacity, "capacity", CCR_ERR_INVALID_ARGUMENT);
+    auto columns = [&](const ccr::Matrix& m) {
+      ccr::Matrix out(nP, nR, 0.0);
+      for (std::size_t p = 0; p < nP; ++p)
+        for (std::size_t k = 0; k < nR; ++k) out(p, k) = m(p, idx[k]);
+      return out;
+    };
+    const ccr::Matrix net = columns(r.nettedValue), deflator = columns(r.scenarios.deflator);
+    std::vector<ccr::Matrix> values;
+    for (const ccr::Matrix& m : r.tradeValues) values.push_back(columns(m));
+    const auto discounted = ccr::marginalExposureContributions(values, net, &deflator);
+    const auto plain = ccr::marginalExposureContributions(values, net, nullptr);
+    std::vector<double> times;
+    for (std::size_t k = 0; k < nR; ++k) times.push_back(r.scenarios.grid[idx[k]]);
+    const ccr::TimeGrid grid(times);
+    for (std::size_t i = 0; i < values.size(); ++i) {
+      ccr::Matrix rest = net;
+      for (std::size_t p = 0; p < nP; ++p)
+        for (std::size_t k = 0; k < nR; ++k) rest(p, k) = net(p, k) - values[i](p, k);
+      const auto inc = ccr::incrementalExposure(rest, values[i], &deflator);
+      const ccr::ExposureProfile own = ccr::ExposureProfile::compute(values[i], grid, deflator);
+      const std::vector<double>* rows[5] = {&discounted[i], &plain[i], &inc, &own.discountedExpectedExposure,
+                                            &own.expectedExposure};
+      for (int m = 0; m < 5; ++m)
+        for (std::size_t k = 0; k < nR; ++k) stats[(i * 5 + static_cast<std::size_t>(m)) * capacity + k] = (*rows[m])[k];
+    }
     return CCR_OK;
   } catch (const std::exception& e) {
     return fail(err, err_capacity, e.what(), CCR_ERR_RUNTIME);

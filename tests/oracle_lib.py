@@ -33,6 +33,8 @@ def oracle():
         lib.ccr_oracle_run_exposure.restype = I32
         lib.ccr_oracle_run_exposure.argtypes = [P, P, P, I32, P, I32, D, I32, P, C.c_char_p, I32]
         lib.ccr_oracle_initial_margin.restype = I32
+        lib.ccr_oracle_trade_statistics.restype = I32
+        lib.ccr_oracle_trade_statistics.argtypes = [P, P, P, I32, P, I32, P, P, I32, C.c_char_p, I32]
         lib.ccr_oracle_initial_margin.argtypes = [P, P, P, I32, P, D, D, I32, P, P, I32, P, C.c_char_p, I32]
         for n in ("ccr_oracle_discount", "ccr_oracle_instantaneous_forward", "ccr_oracle_survival",
                   "ccr_oracle_cir_survival"):
@@ -97,6 +99,8 @@ def ref():
             lib.ccr_ref_initial_margin.restype = I32
             lib.ccr_ref_initial_margin.argtypes = [P, P, P, I32, P, D, D, P, I32, P, C.c_char_p, I32]
             lib.ccr_ref_xva.restype = I32
+            lib.ccr_ref_trade_statistics.restype = I32
+            lib.ccr_ref_trade_statistics.argtypes = [P, P, P, I32, P, P, I32, C.c_char_p, I32]
             lib.ccr_ref_xva.argtypes = [P, P, P, I32, P, P, D, P, D, P, P]
         _ref = lib
     return _ref
@@ -178,6 +182,35 @@ def oracle_initial_margin(spec: abi.Spec, i: int, quantile: float, mpor: float =
     if rc != 0:
         raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
     return margin[:n.value]
+
+
+def oracle_trade_statistics(spec: abi.Spec, i: int, threads: int = 4):
+    """Per-trade statistics of netting set i: (profile, {name: [trades][reporting dates]})."""
+    cap = grid_size(spec, i)
+    outs, arr = abi.make_outs(1, cap, spec.num_paths, False)
+    n = spec.sets_c[i].num_trades
+    stats = np.zeros((n, len(abi.TRADE_STATS), cap))
+    err = C.create_string_buffer(512)
+    rc = oracle().ccr_oracle_trade_statistics(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                              len(spec.base_grid), C.byref(spec.sets_c[i]), threads, arr,
+                                              abi.dptr(stats), cap, err, 512)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
+    abi.sync_outs(outs, arr)
+    return outs[0], {m: stats[:, k, :] for k, m in enumerate(abi.TRADE_STATS)}
+
+
+def ref_trade_statistics(spec: abi.Spec, i: int):
+    """The reference's allocation functions on its own engine's values (swaps, forwards, options)."""
+    cap = grid_size(spec, i)
+    n = spec.sets_c[i].num_trades
+    stats = np.zeros((n, len(abi.TRADE_STATS), cap))
+    err = C.create_string_buffer(512)
+    rc = ref().ccr_ref_trade_statistics(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                        len(spec.base_grid), C.byref(spec.sets_c[i]), abi.dptr(stats), cap, err, 512)
+    if rc != 0:
+        raise RuntimeError(f"reference failed ({rc}): {err.value.decode()}")
+    return {m: stats[:, k, :] for k, m in enumerate(abi.TRADE_STATS)}
 
 
 def ref_initial_margin(spec: abi.Spec, i: int, quantile: float, mpor: float = 10.0 / 250.0) -> np.ndarray:

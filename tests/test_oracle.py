@@ -299,6 +299,38 @@ def test_oracle_initial_margin_scaling_and_extremes():
     assert (top >= a).all() and (a >= low).all() and (top[live] > low[live]).all()
 
 
+@pytest.mark.skipif(not _ref_has("ccr_ref_trade_statistics"), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("kw", [dict(num_paths=200),
+                                dict(num_paths=150, csa=True, fx=True, credits=True),
+                                dict(num_paths=150, fx=True, options=6)])
+def test_oracle_trade_statistics_match_live_reference_build(kw):
+    """marginalExposureContributions (with and without the deflator), incrementalExposure of
+    each trade against the rest and each trade's own EE / EE* (ExposureProfile::compute) of the
+    reference's engine values (src/exposure/allocation.cpp:8-45) against the oracle's per-trade
+    statistics, on the reporting dates (CSA sets: before collateral)."""
+    spec = W.small_reference_case(**kw)
+    _, o = oracle_lib.oracle_trade_statistics(spec, 0, threads=3)
+    r = oracle_lib.ref_trade_statistics(spec, 0)
+    scale = max(np.abs(r["standalone"]).max(), 1.0)
+    for m in abi.TRADE_STATS:
+        err = np.abs(o[m] - r[m]).max() / scale
+        assert err <= (0.0 if m.startswith("marginal") else 1e-13), (m, err)
+
+
+def test_oracle_trade_statistics_invariants():
+    """Euler contributions add up to the uncollateralised EE and EE* (the reference's
+    gpu_per_trade_statistics check), and per trade marginal <= incremental-free bounds:
+    E[V_i 1{V>0}] <= E[V_i+] and E[D (V+ - (V - V_i)+)] <= E[D V_i+]."""
+    spec = W.small_reference_case(num_paths=300, fx=True, credits=True, options=4)
+    prof, o = oracle_lib.oracle_trade_statistics(spec, 0)
+    scale = np.abs(prof.get("expected_exposure")).max()
+    assert np.abs(o["marginal"].sum(axis=0) - prof.get("expected_exposure")).max() <= 1e-12 * scale
+    assert np.abs(o["marginal_discounted"].sum(axis=0) - prof.get("discounted_expected_exposure")).max() <= 1e-12 * scale
+    assert (o["marginal"] <= o["standalone"] + 1e-9).all()
+    assert (o["incremental_discounted"] <= o["standalone_discounted"] + 1e-9).all()
+    assert (o["standalone"] >= 0.0).all() and o["standalone"].max() > 0.0
+
+
 @pytest.mark.skipif(not _ref_has("ccr_ref_xva"), reason="reference build (oracle/_ref) not present")
 def test_xva_matches_live_reference_build():
     """xva (src/xva/xva.cpp:78-96) of the reference against the oracle's restatement and the C ABI's

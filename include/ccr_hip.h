@@ -391,6 +391,28 @@ int32_t ccr_hip_set_initial_margin(ccr_hip_ctx* ctx, double quantile, double mar
 int32_t ccr_hip_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set, double* margin,
                                int32_t capacity, int32_t* size);
 
+/* Per-trade statistics (gpu::FusedOptions::perTrade and FusedResult::TradeStats,
+ * include/ccr/gpu/fused_exposure.hpp:251-256, 299; the estimators of marginalExposureContributions
+ * and incrementalExposure, src/exposure/allocation.cpp:8-45): plans compiled after
+ * ccr_hip_set_trade_statistics(ctx, 1) value every trade of a netting set on its own (the set's
+ * value is their sum in trade order) and give, per trade and reporting date, the expectations
+ * below, with V the set's value before collateral and D the deflator. At most 16 trades per
+ * netting set and no Bermudan swaptions (CCR_ERR_UNSUPPORTED at compile). */
+typedef struct ccr_trade_statistics {
+  int32_t capacity;                /* doubles each non-null array can hold (>= reporting dates) */
+  int32_t num_dates;               /* out: reporting dates written */
+  double* marginal_discounted;     /* E[D V_i 1{V > 0}]: Euler contribution to EE* */
+  double* marginal;                /* E[V_i 1{V > 0}]: Euler contribution to EE */
+  double* incremental_discounted;  /* E[D (V+ - (V - V_i)+)]: EE* the trade adds */
+  double* standalone_discounted;   /* E[D V_i+] */
+  double* standalone;              /* E[V_i+] */
+} ccr_trade_statistics;
+int32_t ccr_hip_set_trade_statistics(ccr_hip_ctx* ctx, int32_t enable);
+/* After an execute: the statistics of trade `trade` (its index in the set) of netting set `set`;
+ * null arrays are skipped. */
+int32_t ccr_hip_trade_statistics(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set, int32_t trade,
+                                 ccr_trade_statistics* out);
+
 /* ExposureEngine::reportingGrid / simulationGrid of one netting set (exposure_engine.cpp:35-48).
  * Writes up to `capacity` times and the full size into *size. Needs no device. */
 int32_t ccr_hip_reporting_grid(const double* base_grid, int32_t base_size, const ccr_netting_set* set,

@@ -219,6 +219,74 @@ extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr
   }
 }
 
+// Per-trade plan of one netting set, run through the kernel body's per-trade path (the generic
+// shape with per-trade values, as the GPU launch picks): stats[(i * kTradeStats + m) * capacity + r]
+// are the per-trade statistics summed in path order and divided by the paths.
+extern "C" int32_t emu_run_trade_statistics(const ccr_scenario_generator* gen, const ccr_simulation_config* cfg,
+                                            const double* base, int32_t nbase, const ccr_netting_set* set,
+                                            double* stats, int32_t capacity, char* err, int32_t cap) {
+  try {
+    HostPlan h = buildPlan(*gen, *cfg, base, nbase, set, 1, 0.95, CCR_FP64, 256, true);
+    std::vector<double> stream(static_cast<size_t>(h.stream_len));
+    generateStream(cfg->seed, h.stream_len, stream.data());
+    PathArgs<double> a{};
+    a.g = &h.g;
+    a.sets = h.sets.data();
+    a.block_set = h.block_set.data();
+    a.stream = stream.data();
+    a.steps = h.steps.data();
+    a.dates = h.dates.data();
+    a.date_reals = h.date_reals.data();
+    a.bonds = h.bonds.data();
+    a.fixed = h.fixed.data();
+    a.fixstore = h.fixstore.data();
+    a.poly = h.poly.data();
+    a.opts = h.opts.data();
+    a.max_rep = h.max_rep;
+    a.ring = h.g.max_ring;
+    a.price_rows = h.price_rows;
+    a.slots = h.g.max_slots;
+    a.nrates = h.g.nrates;
+    a.nassets = h.g.nassets;
+    a.ncredits = h.g.ncredits;
+    a.tstride = kTradeStats * h.max_groups;
+    a.tdates = h.tdates.data();
+    a.tdreals = h.tdreals.data();
+    const SetDev& S = h.sets[0];
+    const int P = h.g.num_paths, R = S.num_rep, G = S.num_groups;
+    if (capacity < R) throw Error(CCR_ERR_INVALID_ARGUMENT, "capacity");
+    std::vector<double> sums(static_cast<size_t>(G) * kTradeStats * R, 0.0);
+    std::vector<double> scratch(static_cast<size_t>(h.g.max_ring + h.price_rows + h.g.max_slots + G + 1));
+    for (int p = 0; p < P; ++p) {
+      std::fill(scratch.begin(), scratch.end(), 0.0);
+      simulatePath<double, GenericShape, true, true>(
+          a, &S, p, scratch.data(), 1, [](int, const double*, double) {}, [](int, double, double) {},
+          [&](int rr, double V, double D, const double* tv, int stride) {
+            const double on = V > 0.0 ? 1.0 : 0.0, vp = V > 0.0 ? V : 0.0;
+            for (int i = 0; i < G; ++i) {
+              const double vi = tv[i * stride], rest = V - vi, vip = vi > 0.0 ? vi : 0.0;
+              const double v5[kTradeStats] = {D * vi * on, vi * on, D * (vp - (rest > 0.0 ? rest : 0.0)), D * vip, vip};
+              for (int m = 0; m < kTradeStats; ++m)
+                sums[(static_cast<size_t>(i) * kTradeStats + static_cast<size_t>(m)) * R + static_cast<size_t>(rr)] += v5[m];
+            }
+          });
+    }
+    for (int i = 0; i < G; ++i)
+      for (int m = 0; m < kTradeStats; ++m)
+        for (int r = 0; r < R; ++r)
+          stats[(static_cast<size_t>(i) * kTradeStats + static_cast<size_t>(m)) * capacity + static_cast<size_t>(r)] =
+              sums[(static_cast<size_t>(i) * kTradeStats + static_cast<size_t>(m)) * R + static_cast<size_t>(r)] / P;
+    if (err && cap > 0) err[0] = 0;
+    return CCR_OK;
+  } catch (const Error& e) {
+    if (err && cap > 0) std::snprintf(err, static_cast<size_t>(cap), "%s", e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    if (err && cap > 0) std::snprintf(err, static_cast<size_t>(cap), "%s", e.what());
+    return CCR_ERR_RUNTIME;
+  }
+}
+
 // Plan dimensions (no execution): max_ring, max_slots, max_dates, max_rep, total_terms, stream_len.
 extern "C" int32_t emu_plan_stats(const ccr_scenario_generator* gen, const ccr_simulation_config* cfg,
                                   const double* base, int32_t nbase, const ccr_netting_set* sets, int32_t nsets,

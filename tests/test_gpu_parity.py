@@ -355,3 +355,91 @@ def test_initial_margin_errors():
     for p in (plain, fresh, off):
         p.close()
     e.close()
+
+
+TRADE_CASES = {
+    "swaps": dict(),
+    "csa_fx_credits": dict(csa=True, fx=True, credits=True),
+    "options": dict(fx=True, credits=True, options=6),
+    "two_leg": dict(two_leg=True, csa=True),
+}
+
+
+def _trade_check(plan, spec, tol):
+    worst = 0.0
+    for s in range(len(spec.sets)):
+        _, want = oracle_lib.oracle_trade_statistics(spec, s)
+        n = spec.sets_c[s].num_trades
+        scale = max(np.abs(want["standalone"]).max(), 1.0)
+        for i in range(n):
+            got = plan.trade_statistics(s, i)
+            for m in abi.TRADE_STATS:
+                assert len(got[m]) == want[m].shape[1]
+                err = np.abs(got[m] - want[m][i]).max() / scale
+                worst = max(worst, err)
+                assert err <= tol, (s, i, m, err)
+    return worst
+
+
+@pytest.mark.parametrize("case", sorted(TRADE_CASES))
+def test_trade_statistics_match_oracle_fp64(case):
+    """Per-trade mode (each trade valued on its own, netted in trade order): Euler contributions
+    with and without the deflator, incremental EE* and standalone EE / EE* per trade against the
+    oracle (allocation.cpp's estimators); the plan's exposure profile stays within 1e-10."""
+    from zlatko_amd import native
+    spec = W.small_reference_case(num_paths=700, **TRADE_CASES[case])
+    e = native.Engine()
+    e.set_trade_statistics(True)
+    plan = e.compile(spec)
+    _check(spec, plan.execute(), oracle_lib.run_oracle(spec, threads=4))
+    print(case, f"{_trade_check(plan, spec, FP64_TOL):.2e}", flush=True)
+    plan.close()
+    e.close()
+
+
+def test_trade_statistics_many_sets_fp64_and_fp32():
+    """Several CSA netting sets with foreign swaps and FX forwards (FP32 error against the FP64
+    oracle printed, bounded by 1e-4 of the largest standalone EE)."""
+    from zlatko_amd import native
+    spec = W.c2(num_paths=800, num_sets=3, num_trades=30, weeks=52, horizon=1.0)
+    assert max(spec.sets_c[s].num_trades for s in range(3)) <= 16
+    e = native.Engine()
+    e.set_trade_statistics(True)
+    p64, p32 = e.compile(spec, abi.CCR_FP64), e.compile(spec, abi.CCR_FP32)
+    p64.execute()
+    p32.execute()
+    _trade_check(p64, spec, FP64_TOL)
+    print("fp32 per-trade error", f"{_trade_check(p32, spec, 1e-4):.2e}", flush=True)
+    p64.close()
+    p32.close()
+    e.close()
+
+
+def test_trade_statistics_errors():
+    """More than 16 trades or a Bermudan swaption refuse to compile (CCR_ERR_UNSUPPORTED); a plan
+    compiled without per-trade mode, or not executed, refuses the query; indices are checked."""
+    from zlatko_amd import native
+    e = native.Engine()
+    e.set_trade_statistics(True)
+    for spec in (W.small_reference_case(num_paths=64, fx=True, options=11),  # 17 trades
+                 W.small_reference_case(num_paths=64, bermudans=1)):
+        with pytest.raises(native.CcrError) as err:
+            e.compile(spec)
+        assert err.value.code == abi.CCR_ERR_UNSUPPORTED
+    spec = W.small_reference_case(num_paths=128)
+    fresh = e.compile(spec)
+    with pytest.raises(native.CcrError):
+        fresh.trade_statistics(0, 0)
+    fresh.execute()
+    for s, i in ((0, 4), (1, 0), (0, -1)):
+        with pytest.raises(native.CcrError) as err:
+            fresh.trade_statistics(s, i)
+        assert err.value.code == abi.CCR_ERR_OUT_OF_RANGE
+    e.set_trade_statistics(False)
+    off = e.compile(spec)
+    off.execute()
+    with pytest.raises(native.CcrError):
+        off.trade_statistics(0, 0)
+    for p in (fresh, off):
+        p.close()
+    e.close()

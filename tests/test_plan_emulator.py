@@ -23,6 +23,8 @@ def emu():
     lib.emu_run_exposure.argtypes = [C.c_void_p] * 3 + [C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
                                                          C.c_char_p, C.c_int32]
     lib.emu_plan_stats.argtypes = [C.c_void_p] * 3 + [C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]
+    lib.emu_run_trade_statistics.argtypes = [C.c_void_p] * 3 + [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                                                                 C.c_char_p, C.c_int32]
     return lib
 
 
@@ -98,3 +100,23 @@ def test_fp64_bond_exponential_is_within_one_ulp(emu):
     emu.emu_exp_bond(x.ctypes.data_as(C.c_void_p), len(x), out.ctypes.data_as(C.c_void_p))
     ref = np.exp(x)
     assert (np.abs(out - ref) / np.spacing(ref)).max() <= 1.0
+
+
+@pytest.mark.parametrize("kw", [dict(num_paths=96), dict(num_paths=80, csa=True, fx=True, credits=True, options=4),
+                                dict(num_paths=64, two_leg=True, csa=True)])
+def test_emulated_trade_statistics_match_oracle(emu, kw):
+    """Per-trade plans (each trade's own records, netted in trade order) through the kernel body's
+    per-trade path on the CPU, against the oracle's per-trade statistics."""
+    spec = W.small_reference_case(**kw)
+    _, want = oracle_lib.oracle_trade_statistics(spec, 0)
+    cap = want["standalone"].shape[1]
+    n = spec.sets_c[0].num_trades
+    got = np.zeros((n, len(abi.TRADE_STATS), cap))
+    err = C.create_string_buffer(512)
+    rc = emu.emu_run_trade_statistics(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                      len(spec.base_grid), C.byref(spec.sets_c[0]), abi.dptr(got), cap, err, 512)
+    assert rc == 0, err.value
+    scale = max(np.abs(want["standalone"]).max(), 1.0)
+    for k, m in enumerate(abi.TRADE_STATS):
+        e = np.abs(got[:, k, :] - want[m]).max() / scale
+        assert e <= 1e-12, (m, e)

@@ -61,6 +61,13 @@ class XvaResultC(C.Structure):
                 ("mva", C.c_double), ("kva", C.c_double)]
 
 
+TRADE_STATS = ("marginal_discounted", "marginal", "incremental_discounted", "standalone_discounted", "standalone")
+
+
+class TradeStatisticsC(C.Structure):
+    _fields_ = [("capacity", C.c_int32), ("num_dates", C.c_int32)] + [(n, DP) for n in TRADE_STATS]
+
+
 class HullWhiteC(C.Structure):
     _fields_ = [("curve", YieldCurveC), ("mean_reversion", C.c_double), ("volatility", C.c_double)]
 

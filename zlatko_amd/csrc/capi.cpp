@@ -57,6 +57,8 @@ struct ccr_hip_ctx {
   size_t ws_bytes = 0;
   // Initial-margin proxy for plans compiled while im_quantile > 0.
   double im_quantile = 0.0, im_mpor = 10.0 / 250.0;
+  // Per-trade statistics for plans compiled while set.
+  bool per_trade = false;
 };
 
 namespace {
@@ -114,6 +116,15 @@ struct ccr_hip_plan {
   double* d_im_dsum = nullptr;
   std::vector<double> im_q, im_dsum;
   bool im_ready = false;
+  // Per-trade statistics (tstride = 0 when off): per-(date, trade) records and reals, per-wave
+  // partials, the reduced statistics [set][R_max][tstride] and the last execute's read-back.
+  int32_t tstride = 0;
+  DateRec* d_tdates = nullptr;
+  void* d_tdreals = nullptr;
+  double* d_tpart = nullptr;
+  double* d_tstats = nullptr;
+  std::vector<double> tstats;
+  bool tstats_ready = false;
   ~ccr_hip_plan() {
     for (auto& b : bufs)
       if (b.p) (void)hipFree(b.p);
@@ -283,15 +294,16 @@ void uploadStream(ccr_hip_ctx* ctx, ccr_hip_plan* plan, uint64_t seed, bool fp32
 // Workgroup size of the path kernel with the most waves resident per CU for this plan (its LDS
 // rows per path and its kernel specialisation's registers); ties go to the larger workgroup.
 int occupancyBlock(const HostPlan& h) {
-  const int ldsRows = h.g.max_ring + h.price_rows + h.g.max_slots;
+  const int ldsRows = h.g.max_ring + h.price_rows + h.g.max_slots + h.max_groups;
+  const bool trades = h.max_groups > 0;
   int best = kMaxBlock, bestWaves = -1;
   for (int block = kMaxBlock; block >= 64; block -= 64) {
     int wg = 0;
     const hipError_t e = h.precision == CCR_FP32
                              ? pathOccupancy<float>(h.g.nrates, h.g.nassets, h.g.ncredits, !h.opts.empty(), ldsRows,
-                                                    block, &wg)
+                                                    block, &wg, trades)
                              : pathOccupancy<double>(h.g.nrates, h.g.nassets, h.g.ncredits, !h.opts.empty(), ldsRows,
-                                                     block, &wg);
+                                                     block, &wg, trades);
     check(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     const int waves = wg * (block / 64);
     if (waves > bestWaves) {
@@ -319,7 +331,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   auto plan = std::make_unique<ccr_hip_plan>();
   plan->device = ctx->device;
   plan->host = buildPlan(*generator, *config, base_grid, base_size, sets, num_sets, pfe_quantile, precision,
-                         ctx->block > 0 ? ctx->block : kMaxBlock);
+                         ctx->block > 0 ? ctx->block : kMaxBlock, ctx->per_trade);
   HostPlan& h = plan->host;
   if (ctx->block == 0) assignBlocks(h, occupancyBlock(h));
   const bool fp32 = precision == CCR_FP32;
@@ -355,6 +367,18 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     execAlloc(ctx, plan.get(), plan->d_im_part, static_cast<size_t>(h.total_waves) * dates * sizeof(double));
     plan->d_im_q = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * dates * sizeof(double)));
     plan->d_im_dsum = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * dates * sizeof(double)));
+    check(hipMemcpyAsync(plan->d_sets, h.sets.data(), h.sets.size() * sizeof(SetDev), hipMemcpyHostToDevice,
+                         ctx->stream),
+          "hipMemcpyAsync");
+  }
+  if (h.max_groups > 0) {
+    plan->tstride = kTradeStats * h.max_groups;
+    const int64_t per = static_cast<int64_t>(h.max_rep) * plan->tstride;
+    for (SetDev& sd : h.sets) sd.tpart_off = static_cast<int64_t>(sd.wave_begin) * per;
+    plan->d_tdates = upload(ctx, plan.get(), h.tdates.data(), h.tdates.size());
+    plan->d_tdreals = uploadReals(ctx, plan.get(), h.tdreals, fp32);
+    execAlloc(ctx, plan.get(), plan->d_tpart, static_cast<size_t>(h.total_waves) * per * sizeof(double));
+    plan->d_tstats = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * per * sizeof(double)));
     check(hipMemcpyAsync(plan->d_sets, h.sets.data(), h.sets.size() * sizeof(SetDev), hipMemcpyHostToDevice,
                          ctx->stream),
           "hipMemcpyAsync");
@@ -462,6 +486,10 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   a.im_rows = static_cast<Real*>(plan->d_im_rows);
   a.im_part = plan->d_im_part;
   a.im_dates = h.max_dates;
+  a.tstride = plan->tstride;
+  a.tdates = plan->d_tdates;
+  a.tdreals = static_cast<const Real*>(plan->d_tdreals);
+  a.tpart = plan->d_tpart;
   a.state_len = h.state_len;
   return a;
 }
@@ -581,6 +609,12 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* su
   check(launchReduce(plan->d_sets, plan->d_partials, nsets, h.max_rep, h.total_waves / nsets,
                      1.0 / static_cast<double>(h.g.num_paths), plan->d_stats, ctx->stream),
         "reduce kernel launch");
+  if (plan->tstride > 0) {
+    check(launchTradeReduce(plan->d_sets, plan->d_tpart, nsets, h.max_rep, plan->tstride, h.total_waves / nsets,
+                            h.g.num_paths, plan->d_tstats, ctx->stream),
+          "per-trade reduce kernel launch");
+    plan->tstats_ready = false;
+  }
   check(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
   if (h.precision == CCR_FP32)
     check(launchPfe<float>(plan->d_sets, static_cast<const float*>(plan->d_rows), nsets, h.max_rep, h.g.num_paths,
@@ -651,8 +685,15 @@ void readBack(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs) {
                          ctx->stream),
           "hipMemcpyAsync");
   }
+  if (plan->tstride > 0) {
+    plan->tstats.resize(nsets * static_cast<size_t>(h.max_rep) * static_cast<size_t>(plan->tstride));
+    check(hipMemcpyAsync(plan->tstats.data(), plan->d_tstats, plan->tstats.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, ctx->stream),
+          "hipMemcpyAsync");
+  }
   check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   plan->im_ready = plan->d_im_q != nullptr;
+  plan->tstats_ready = plan->tstride > 0;
   readTimings(ctx);
   const int64_t P = h.g.num_paths;
   for (size_t s = 0; s < nsets; ++s) {
@@ -803,6 +844,37 @@ int32_t ccr_hip_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32
     for (int32_t k = 1; k < n; ++k) {
       const double scale = std::sqrt(plan->im_mpor / (t[static_cast<size_t>(k)] - t[static_cast<size_t>(k - 1)]));
       margin[k] = plan->im_q[qBase + static_cast<size_t>(k - 1)] * scale * plan->im_dsum[dBase + static_cast<size_t>(k)] / paths;
+    }
+  });
+}
+
+int32_t ccr_hip_set_trade_statistics(ccr_hip_ctx* ctx, int32_t enable) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  ctx->per_trade = enable != 0;
+  return CCR_OK;
+}
+
+int32_t ccr_hip_trade_statistics(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set, int32_t trade,
+                                 ccr_trade_statistics* out) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!plan || !out) invalid("null argument");
+    if (plan->tstride == 0) invalid("per-trade statistics: the plan was compiled without ccr_hip_set_trade_statistics");
+    if (!plan->tstats_ready) invalid("per-trade statistics: execute the plan (ccr_hip_execute) first");
+    const HostPlan& h = plan->host;
+    if (set < 0 || set >= static_cast<int32_t>(h.sets.size())) fail(CCR_ERR_OUT_OF_RANGE, "netting set index out of range");
+    const SetDev& S = h.sets[static_cast<size_t>(set)];
+    if (trade < 0 || trade >= S.num_groups) fail(CCR_ERR_OUT_OF_RANGE, "trade index out of range");
+    if (out->capacity < S.num_rep) invalid("output capacity smaller than the reporting grid");
+    out->num_dates = S.num_rep;
+    double* dst[kTradeStats] = {out->marginal_discounted, out->marginal, out->incremental_discounted,
+                                out->standalone_discounted, out->standalone};
+    const size_t base = static_cast<size_t>(set) * static_cast<size_t>(h.max_rep) * static_cast<size_t>(plan->tstride);
+    for (int m = 0; m < kTradeStats; ++m) {
+      if (!dst[m]) continue;
+      for (int r = 0; r < S.num_rep; ++r)
+        dst[m][r] = plan->tstats[base + static_cast<size_t>(r) * static_cast<size_t>(plan->tstride) +
+                                 static_cast<size_t>(trade) * kTradeStats + static_cast<size_t>(m)];
     }
   });
 }

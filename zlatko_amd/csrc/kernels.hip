@@ -40,7 +40,7 @@ constexpr int pathWaveHint() {
              ? 5 : 1;
 }
 
-template <typename Real, typename Sh, bool kOptions>
+template <typename Real, typename Sh, bool kOptions, bool kTrades = false>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(pathWaveHint<Real, Sh, kOptions>())))
 ccr_path_kernel(PathArgs<Real> a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -67,7 +67,28 @@ ccr_path_kernel(PathArgs<Real> a) {
     const double sD = waveSum(active ? static_cast<double>(D) : 0.0);
     if ((tid & 63) == 0) a.im_part[S.im_part_off + static_cast<int64_t>(waveLocal) * a.im_dates + j] = sD;
   };
-  simulatePath<Real, Sh, kOptions>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
+  // Per-trade statistics (kTrades): on every reporting date, per trade i, D V_i 1{V>0},
+  // V_i 1{V>0}, D (V+ - (V - V_i)+), D V_i+ and V_i+ summed over the wavefront (one transposed
+  // sum per trade) into this wave's partials.
+  auto onTrades = [&](int rr, Real V, Real D, const Real* tv, int stride) {
+    double* tp = a.tpart + S.tpart_off + (static_cast<int64_t>(waveLocal) * a.max_rep + rr) * a.tstride;
+    const double v = static_cast<double>(V), d = static_cast<double>(D);
+    const double on = v > 0.0 ? 1.0 : 0.0, vp = v > 0.0 ? v : 0.0;
+    for (int i = 0; i < S.num_groups; ++i) {
+      const double vi = static_cast<double>(tv[i * stride]);
+      const double rest = v - vi, vip = vi > 0.0 ? vi : 0.0;
+      double vals[kTradeStats];
+      vals[0] = active ? d * vi * on : 0.0;
+      vals[1] = active ? vi * on : 0.0;
+      vals[2] = active ? d * (vp - (rest > 0.0 ? rest : 0.0)) : 0.0;
+      vals[3] = active ? d * vip : 0.0;
+      vals[4] = active ? vip : 0.0;
+      int idx;
+      const double s = waveSumMany<kTradeStats>(vals, &idx);
+      if ((tid & 63) < 8 && idx >= 0) tp[i * kTradeStats + idx] = s;
+    }
+  };
+  simulatePath<Real, Sh, kOptions, kTrades>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
     if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
     // The first eight statistics in one transposed wavefront sum (lanes 0..7 store one each);
     // the DVA increment is 0 without intensities (Q = 1), else a wavefront sum of its own
@@ -81,7 +102,7 @@ ccr_path_kernel(PathArgs<Real> a) {
     if constexpr (!(Sh::kExact && Sh::NC == 0)) dva = waveSum(active ? st9[8] : 0.0);
     const int lane = tid & 63;
     if (lane < 9) part[rr * kStats + (lane < 8 ? idx : 8)] = lane < 8 ? s : dva;
-  }, onValue);
+  }, onValue, onTrades);
 }
 
 // States pass for American Monte Carlo: the domestic state and the deflator of every date of the
@@ -460,8 +481,16 @@ hipError_t launchShape(const PathArgs<Real>& args, int blocks, int block, size_t
 
 // Picks the path-kernel specialisation for the model shape (rates, assets, credits) and for the
 // presence of European options.
+// Per-trade plans run the generic shape with options and per-trade values compiled in; their
+// LDS column adds one row per trade.
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
+  if (args.tstride > 0) {
+    const size_t lds = static_cast<size_t>(args.ring + args.price_rows + args.slots + args.tstride / kTradeStats) *
+                       block * sizeof(Real);
+    hipLaunchKernelGGL((ccr_path_kernel<Real, GenericShape, true, true>), dim3(blocks), dim3(block), lds, stream, args);
+    return hipGetLastError();
+  }
   const size_t lds = static_cast<size_t>(args.ring + args.price_rows + args.slots) * block * sizeof(Real);
   return withShape(args.nrates, args.nassets, args.ncredits, [&](auto shape) {
     return launchShape<Real, decltype(shape)>(args, blocks, block, lds, stream);
@@ -469,8 +498,12 @@ hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStre
 }
 
 template <typename Real>
-hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu) {
+hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu,
+                         bool trades) {
   const size_t lds = static_cast<size_t>(ldsRows) * block * sizeof(Real);
+  if (trades)
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(wgPerCu, ccr_path_kernel<Real, GenericShape, true, true>, block,
+                                                        lds);
   return withShape(nrates, nassets, ncredits, [&](auto shape) -> hipError_t {
     using Sh = decltype(shape);
     if constexpr (Sh::NA > 0) {
@@ -480,8 +513,8 @@ hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, in
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(wgPerCu, ccr_path_kernel<Real, Sh, false>, block, lds);
   });
 }
-template hipError_t pathOccupancy<double>(int, int, int, bool, int, int, int*);
-template hipError_t pathOccupancy<float>(int, int, int, bool, int, int, int*);
+template hipError_t pathOccupancy<double>(int, int, int, bool, int, int, int*, bool);
+template hipError_t pathOccupancy<float>(int, int, int, bool, int, int, int*, bool);
 
 __global__ void __launch_bounds__(256) ccr_narrow_kernel(const double* in, float* out, int64_t n) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
@@ -594,6 +627,29 @@ hipError_t launchImReduce(const SetDev* sets, const double* part, int nsets, int
   const int n = nsets * dates;
   hipLaunchKernelGGL(ccr_im_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, sets, part, nsets, dates,
                      waves_per_set, out);
+  return hipGetLastError();
+}
+
+// Per-trade statistics: the fixed-order sum over a set's waves of every (reporting date, value)
+// of its partials, divided by the paths.
+__global__ void ccr_trade_reduce_kernel(const SetDev* sets, const double* part, int nsets, int max_rep, int tstride,
+                                        int waves_per_set, double inv_paths, double* out) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t per = static_cast<int64_t>(max_rep) * tstride;
+  if (idx >= nsets * per) return;
+  const int s = static_cast<int>(idx / per);
+  const int64_t e = idx - s * per;
+  const double* p = part + sets[s].tpart_off + e;
+  double sum = 0.0;
+  for (int w = 0; w < waves_per_set; ++w) sum += p[static_cast<int64_t>(w) * per];
+  out[idx] = sum * inv_paths;
+}
+
+hipError_t launchTradeReduce(const SetDev* sets, const double* part, int nsets, int max_rep, int tstride,
+                             int waves_per_set, int num_paths, double* out, hipStream_t stream) {
+  const int64_t n = static_cast<int64_t>(nsets) * max_rep * tstride;
+  hipLaunchKernelGGL(ccr_trade_reduce_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, sets,
+                     part, nsets, max_rep, tstride, waves_per_set, 1.0 / static_cast<double>(num_paths), out);
   return hipGetLastError();
 }
 

@@ -54,6 +54,12 @@ struct PathArgs {
   Real* im_rows;
   double* im_part;
   int32_t im_dates;
+  // Per-trade plans (tstride > 0): per-(date, trade) valuation records and reals, and per-wave
+  // partials [set waves][R_max][tstride] of the kTradeStats statistics of every trade.
+  int32_t tstride;
+  const DateRec* tdates;
+  const Real* tdreals;
+  double* tpart;
 };
 
 // American Monte Carlo regressions (amc.hip). Layouts: states x then D, [set dates][path];
@@ -123,7 +129,8 @@ hipError_t launchDomesticNormals(const PathArgs<Real>& args, const SetDev* hostS
 // Workgroups of the path kernel resident per CU (registers, LDS, workgroup slots) for a model
 // shape, the option variant, ldsRows LDS rows per thread and a workgroup size.
 template <typename Real>
-hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu);
+hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu,
+                         bool trades = false);
 // The states-only pass of the path kernel (AMC): writes x and D of every date into args.states.
 template <typename Real>
 hipError_t launchStates(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);
@@ -133,6 +140,8 @@ template <typename Real>
 hipError_t launchPfe(const SetDev* sets, const Real* rows, int nsets, int max_rep, int num_paths, double level,
                      double* out, hipStream_t stream, int which = 0);
 // Per (set, simulation date) deflator sums of the initial-margin proxy from its per-wave sums.
+hipError_t launchTradeReduce(const SetDev* sets, const double* part, int nsets, int max_rep, int tstride,
+                             int waves_per_set, int num_paths, double* out, hipStream_t stream);
 hipError_t launchImReduce(const SetDev* sets, const double* part, int nsets, int dates, int waves_per_set, double* out,
                           hipStream_t stream);
 

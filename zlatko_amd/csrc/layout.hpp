@@ -26,6 +26,11 @@ constexpr int kMaxCorr = kMaxRates + kMaxAssets + kMaxCredits;
 constexpr int kMaxSub = 8;        // CIR++ sub-steps per grid step
 constexpr int kStats = 9;         // E[V-C], EE, ENE, EE*, ENE*, E[D], E[Q], CVA incr., DVA incr.
 constexpr int kMaxBlock = 256;    // launch bound of the path kernel (threads per workgroup)
+// Per-trade statistics (allocation): at most kMaxTradeGroups trades per netting set, kTradeStats
+// per trade and reporting date -- E[D V_i 1{V>0}], E[V_i 1{V>0}], E[D (V+ - (V - V_i)+)],
+// E[D V_i+], E[V_i+] with V the netting set's value before collateral.
+constexpr int kMaxTradeGroups = 16;
+constexpr int kTradeStats = 5;
 
 // Step record (Real[kStepStride]) for the step t_{j-1} -> t_j (row 0 unused):
 //   rate r:   8 r + {e, sdX, c1, c2, (1-e)/a, drift (1-e)/a, drift (dt-(1-e)/a)/a, int phi}
@@ -155,6 +160,9 @@ struct SetDev {
   int64_t dom_off;             // first element of its domestic normals [step][2][path block] (AMC), -1 without
   int64_t im_off;              // initial-margin proxy: first element of its |V_j - V_j-1| rows [date][path]
   int64_t im_part_off;         // and of its per-wave deflator sums [wave][date]
+  int64_t tpart_off;           // per-trade statistics: first element of its wave partials
+  int32_t num_groups;          // trades with their own value (per-trade plans), else 0
+  int32_t tdate_off;           // first of its per-(date, trade) valuation records
   double thr_cpty, thr_own, mta, ia, keep;  // CSA terms (thr_own < 0 encodes a one-way CSA)
 };
 

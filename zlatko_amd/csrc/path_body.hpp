@@ -239,14 +239,19 @@ inline auto withShape(int nrates, int nassets, int ncredits, F&& f) {
 // kOptions: the plan holds European options (the Black evaluation is compiled in only then, so
 // option-free plans keep a compact loop). onValue(j, V, D) is called on every valued date with
 // the netted value before collateral and the deflator.
-template <typename Real, typename Sh, bool kOptions, typename Emit, typename OnValue>
+// kTrades (per-trade plans): each of the set's S.num_groups trades is valued from its own records
+// (tdates / tdreals, [j G + i]) into row i of tv, the netted value is their sum in trade order,
+// and onTrades(rep, V, D, tv, stride) is called on every reporting date.
+template <typename Real, typename Sh, bool kOptions, bool kTrades, typename Emit, typename OnValue,
+          typename OnTrades>
 __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
                                              cptr<int32_t> fstore, cptr<double> poly, cptr<Real> opts,
                                              const double* bermv, int ring,
                                              int priceRows, Real* col, int stride, Emit&& emit,
-                                             OnValue&& onValue) {
+                                             OnValue&& onValue, cptr<DateRec> tdates, cptr<Real> tdreals,
+                                             Real* tv, OnTrades&& onTrades) {
   constexpr bool kDouble = std::is_same<Real, double>::value;
   constexpr int NR = Sh::NR, NA = Sh::NA, NC = Sh::NC;
   const CCR_CONST GlobalDev& g = *gp;
@@ -404,26 +409,27 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     }
     if (!(rec.flags & kValued) && !S.has_csa) continue;
     cptr<Real> dr = dreal + static_cast<int64_t>(j) * kDateRealStride;
-    Real V = 0;
-    if (rec.flags & kValued) {
+    // Value of the terms of one valuation record on this date.
+    auto valueTerms = [&](const DateRec& vr, cptr<Real> vdr) -> Real {
+      Real Vt = 0;
       // Merged bonds (and their constants) of every rate as power series in its state: Horner
       // in FP64 over the interleaved coefficient rows (highest power first, one coefficient per
       // rate per row), the rates' chains side by side and kPolyBatch rows per batch of scalar loads.
       // The coefficients are uniform: the date loop is uniform across the wavefront.
-      const int rows = rec.poly_rows;
+      const int rows = vr.poly_rows;
       bool usePoly = rows > 0;
       if (usePoly) {
         bool inside = true;
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-          if (r < nR) inside = inside && static_cast<double>(x[r] < Real(0) ? -x[r] : x[r]) <= dr[kDateXmax + r];
+          if (r < nR) inside = inside && static_cast<double>(x[r] < Real(0) ? -x[r] : x[r]) <= vdr[kDateXmax + r];
         usePoly = allLanes(inside);
       }
       double hp[NR];
 #pragma unroll
       for (int r = 0; r < NR; ++r) hp[r] = 0.0;
       if (usePoly) {
-        cptr<double> pc = poly + rec.poly_beg;
+        cptr<double> pc = poly + vr.poly_beg;
         for (int k = 0; k < rows; k += kPolyBatch) {
           double c[kPolyBatch][NR];
 #pragma unroll
@@ -444,8 +450,8 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 #pragma unroll
       for (int r = 0; r < NR; ++r) fsum[r] = 0;
       {
-        const int fe = rec.fix_beg[kMaxRates];
-        for (int i = rec.fix_beg[0]; i < fe; i += 2) {
+        const int fe = vr.fix_beg[kMaxRates];
+        for (int i = vr.fix_beg[0]; i < fe; i += 2) {
           cptr<Real> f = fixed + 4 * static_cast<int64_t>(i);
           const Real a0 = f[0], b0 = f[1], c0 = f[2], a1 = f[4], b1 = f[5], c1 = f[6];
           const int s0 = static_cast<int>(f[3]), s1 = static_cast<int>(f[7]);
@@ -453,8 +459,8 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
           int r0 = 0, r1 = 0;
 #pragma unroll
           for (int r = 1; r < NR; ++r) {
-            if (i >= rec.fix_beg[r]) r0 = r;
-            if (i + 1 >= rec.fix_beg[r]) r1 = r;
+            if (i >= vr.fix_beg[r]) r0 = r;
+            if (i + 1 >= vr.fix_beg[r]) r1 = r;
           }
           Real x0 = x[0], x1 = x[0];
 #pragma unroll
@@ -480,8 +486,8 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
           if (usePoly) {
             acc = static_cast<Real>(hp[r]);
           } else {
-            acc = dr[kDateC0 + r];
-            for (int i = rec.bond_beg[r]; i < rec.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expTerm(bonds[2 * i + 1] * nx);
+            acc = vdr[kDateC0 + r];
+            for (int i = vr.bond_beg[r]; i < vr.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expTerm(bonds[2 * i + 1] * nx);
           }
           acc += fsum[r];
           Real mult = 1;
@@ -491,14 +497,14 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
             for (int i = 0; i < NA; ++i)
               if (i == fa) mult = S_[i];
           }
-          V += mult * acc;
+          Vt += mult * acc;
         }
       }
 #pragma unroll
       for (int i = 0; i < NA; ++i)
-        if (i < nA) V += S_[i] * dr[kDateLin + i];
+        if (i < nA) Vt += S_[i] * vdr[kDateLin + i];
       if constexpr (NA > 0 && kOptions) {
-        for (int i = rec.opt_beg; i < rec.opt_end; ++i) {
+        for (int i = vr.opt_beg; i < vr.opt_end; ++i) {
           cptr<Real> o = opts + kOptStride * static_cast<int64_t>(i);
           const int oa = static_cast<int>(o[0]);
           Real Sa = S_[0];
@@ -506,7 +512,23 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
           for (int k = 1; k < NA; ++k)
             if (k == oa) Sa = S_[k];
           const Real df = o[4] * expr(-o[5] * x[0]);
-          V += o[2] * blackFormula(o[1], Sa * o[6] / df, o[3], o[7], df);
+          Vt += o[2] * blackFormula(o[1], Sa * o[6] / df, o[3], o[7], df);
+        }
+      }
+      return Vt;
+    };
+    Real V = 0;
+    if (rec.flags & kValued) {
+      V = valueTerms(rec, dr);
+      if constexpr (kTrades) {
+        const int G = S.num_groups;
+        cptr<DateRec> trecs = tdates + static_cast<int64_t>(j) * G;
+        cptr<Real> tdr = tdreals + static_cast<int64_t>(j) * G * kDateRealStride;
+        for (int i = 0; i < G; ++i) {
+          const DateRec tr = trecs[i];
+          const Real vi = valueTerms(tr, tdr + i * kDateRealStride);
+          tv[i * stride] = vi;
+          V += vi;
         }
       }
       // Bermudan swaptions: their per-path sum on this valued date, from the valuation kernel
@@ -572,26 +594,34 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     prevQc = qc;
     prevQo = qo;
     emit(rr, st9, vp);
+    if constexpr (kTrades) onTrades(rr, V, D, tv, stride);
   }
 }
 
-// Convenience: the tables of set S for path pe, from a PathArgs bundle.
-template <typename Real, typename Sh, bool kOptions = true, typename Emit, typename OnValue>
+// Convenience: the tables of set S for path pe, from a PathArgs bundle. Per-trade values take the
+// LDS rows after the fixing states.
+template <typename Real, typename Sh, bool kOptions = true, bool kTrades = false, typename Emit, typename OnValue,
+          typename OnTrades>
 __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, Real* col, int stride,
-                                             Emit&& emit, OnValue&& onValue) {
+                                             Emit&& emit, OnValue&& onValue, OnTrades&& onTrades) {
   const CCR_CONST SetDev& S = *sp;
   cptr<GlobalDev> g = toConst(a.g);
   const int64_t q = g->antithetic ? (pe >> 1) : pe;
   const double* bermv = S.berm_end > S.berm_beg ? a.bermv : nullptr;
-  simulatePath<Real, Sh, kOptions>(g, sp, pe, a.stream + q * S.stream_stride,
-                                   toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride,
-                                   toConst(a.dates) + S.date_off,
-                                   toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
-                                   toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off),
-                                   toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
-                                   toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off,
-                                   toConst(a.opts) + kOptStride * static_cast<int64_t>(S.opt_off), bermv, a.ring,
-                                   a.price_rows, col, stride, emit, onValue);
+  simulatePath<Real, Sh, kOptions, kTrades>(
+      g, sp, pe, a.stream + q * S.stream_stride, toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride,
+      toConst(a.dates) + S.date_off, toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride,
+      toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off), toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off),
+      toConst(a.fixstore) + 2 * S.fixstore_off, toConst(a.poly) + S.poly_off,
+      toConst(a.opts) + kOptStride * static_cast<int64_t>(S.opt_off), bermv, a.ring, a.price_rows, col, stride, emit,
+      onValue, toConst(a.tdates) + S.tdate_off, toConst(a.tdreals) + static_cast<int64_t>(S.tdate_off) * kDateRealStride,
+      col + (a.ring + a.price_rows + a.slots) * stride, onTrades);
+}
+template <typename Real, typename Sh, bool kOptions = true, typename Emit, typename OnValue>
+__host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, Real* col, int stride,
+                                             Emit&& emit, OnValue&& onValue) {
+  simulatePath<Real, Sh, kOptions, false>(a, sp, pe, col, stride, emit, onValue,
+                                          [](int, Real, Real, const Real*, int) {});
 }
 template <typename Real, typename Sh, bool kOptions = true, typename Emit>
 __host__ __device__ inline void simulatePath(const PathArgs<Real>& a, cptr<SetDev> sp, int pe, Real* col, int stride,

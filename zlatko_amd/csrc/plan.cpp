@@ -346,7 +346,7 @@ void generateStream(uint64_t seed, int64_t n, double* out) {
 
 HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
-                   int32_t precision, int32_t block) {
+                   int32_t precision, int32_t block, bool per_trade) {
   if (!sets || num_sets <= 0) invalid("no netting sets");
   if (cfg.num_paths <= 0) invalid("ScenarioGenerator::generate: numPaths must be positive");
   if (cfg.num_paths > (int64_t(1) << 31) - 1) fail(CCR_ERR_UNSUPPORTED, "at most 2^31 - 1 paths");
@@ -537,19 +537,21 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     }
     sd.num_slots = static_cast<int32_t>(slotFreeAfter.size());
 
-    // Emit dates.
-    int64_t setTerms = 0;
-    for (int j = 0; j < T; ++j) {
-      const double t = sim[static_cast<std::size_t>(j)];
-      DateRec rec{};
-      rec.rep = repOf[static_cast<std::size_t>(j)];
-      rec.flags = (valued[static_cast<std::size_t>(j)] ? kValued : 0) | (rec.rep >= 0 ? kReporting : 0) |
-                  (isCall[static_cast<std::size_t>(j)] ? kStoreValue : 0);
-      rec.call_pos = call[static_cast<std::size_t>(j)] % ring;
-      rec.store_pos = isCall[static_cast<std::size_t>(j)] ? j % ring : -1;
-      double reals[kDateRealStride] = {0};
-      const Collector& col = cols[static_cast<std::size_t>(j)];
-      const int bondBase = static_cast<int>(plan.bonds.size() / 2), fixBase = static_cast<int>(plan.fixed.size() / 4);
+    // Per-trade plans: every trade of the set gets its own valuation records (its terms alone);
+    // the set's records then carry none, and the kernel nets the trades' values in trade order.
+    const int G = per_trade ? ns.num_trades : 0;
+    if (per_trade) {
+      if (ns.num_trades > kMaxTradeGroups)
+        fail(CCR_ERR_UNSUPPORTED, "per-trade statistics: at most 16 trades per netting set");
+      if (!berms.empty()) fail(CCR_ERR_UNSUPPORTED, "per-trade statistics: Bermudan swaptions are not supported");
+      sd.num_groups = G;
+      sd.tdate_off = static_cast<int32_t>(plan.tdates.size());
+      plan.max_groups = std::max(plan.max_groups, G);
+    }
+
+    // The value terms of one collector on date t into rec's term ranges and the date reals
+    // (constants, deterministic-carry forwards, state bounds).
+    auto emitTerms = [&](const Collector& col, double t, bool isValued, DateRec& rec, double* reals) {
       for (int r = 0; r < kMaxRates; ++r) {
         rec.bond_beg[r] = static_cast<int32_t>(plan.bonds.size() / 2) - sd.bond_off;
         if (r < g.nrates) {
@@ -566,7 +568,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       // series zero-padded on top); none unless every rate has one.
       rec.poly_beg = static_cast<int32_t>(plan.poly.size()) - sd.poly_off;
       rec.poly_rows = 0;
-      if (valued[static_cast<std::size_t>(j)]) {
+      if (isValued) {
         std::vector<std::vector<double>> coef(static_cast<std::size_t>(g.nrates));
         bool all = true;
         std::size_t rows = 0;
@@ -621,14 +623,46 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
         plan.opts.insert(plan.opts.end(), o, o + kOptStride);
       }
       rec.opt_end = static_cast<int32_t>(plan.opts.size() / kOptStride) - sd.opt_off;
+      for (int r = 0; r < kMaxRates; ++r) reals[kDateC0 + r] = col.c0[r];
+      for (int a = 0; a < kMaxAssets; ++a) reals[kDateLin + a] = col.lin[a];
+    };
+
+    // Emit dates.
+    int64_t setTerms = 0;
+    for (int j = 0; j < T; ++j) {
+      const double t = sim[static_cast<std::size_t>(j)];
+      DateRec rec{};
+      rec.rep = repOf[static_cast<std::size_t>(j)];
+      rec.flags = (valued[static_cast<std::size_t>(j)] ? kValued : 0) | (rec.rep >= 0 ? kReporting : 0) |
+                  (isCall[static_cast<std::size_t>(j)] ? kStoreValue : 0);
+      rec.call_pos = call[static_cast<std::size_t>(j)] % ring;
+      rec.store_pos = isCall[static_cast<std::size_t>(j)] ? j % ring : -1;
+      double reals[kDateRealStride] = {0};
+      const int bondBase = static_cast<int>(plan.bonds.size() / 2), fixBase = static_cast<int>(plan.fixed.size() / 4);
+      if (per_trade) {
+        Collector none;
+        none.t = t;
+        emitTerms(none, t, false, rec, reals);
+        for (int i = 0; i < G; ++i) {
+          Collector one;
+          one.t = t;
+          if (valued[static_cast<std::size_t>(j)]) collect(trades[static_cast<std::size_t>(i)], m, sim, one);
+          DateRec trec{};
+          trec.flags = rec.flags;
+          double treals[kDateRealStride] = {0};
+          emitTerms(one, t, valued[static_cast<std::size_t>(j)] != 0, trec, treals);
+          plan.tdates.push_back(trec);
+          plan.tdreals.insert(plan.tdreals.end(), treals, treals + kDateRealStride);
+        }
+      } else {
+        emitTerms(cols[static_cast<std::size_t>(j)], t, valued[static_cast<std::size_t>(j)] != 0, rec, reals);
+      }
       rec.fixstore_beg = static_cast<int32_t>(plan.fixstore.size() / 2) - sd.fixstore_off;
       for (const auto& rs : storeAt[static_cast<std::size_t>(j)]) {
         plan.fixstore.push_back(rs.first);
         plan.fixstore.push_back(rs.second);
       }
       rec.fixstore_end = static_cast<int32_t>(plan.fixstore.size() / 2) - sd.fixstore_off;
-      for (int r = 0; r < kMaxRates; ++r) reals[kDateC0 + r] = col.c0[r];
-      for (int a = 0; a < kMaxAssets; ++a) reals[kDateLin + a] = col.lin[a];
       if (ns.has_collateral && ns.collateral.type == CCR_COLLATERAL_BOND) {
         const double T_ = ns.collateral.bond_maturity;
         reals[kDateBond] = T_ <= t ? 1.0 : m.rates[0].A(t, T_);

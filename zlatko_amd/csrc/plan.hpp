@@ -53,6 +53,11 @@ struct HostPlan {
   std::vector<GemmTile> tiles;      // continuation GEMM tiles
   std::vector<GemmTile> ntiles;     // narrow continuation GEMM tiles (sets with few Bermudans)
   std::vector<int32_t> gcols;       // strategy column per (tile, swaption), -1: none
+  // Per-trade plans: one valuation record (terms of that trade alone) per (set, date, trade),
+  // [tdate_off + j G + i], and its reals; the set's own records then carry no terms.
+  std::vector<DateRec> tdates;
+  std::vector<double> tdreals;
+  int32_t max_groups = 0;
 };
 
 // ExposureEngine::reportingGrid / simulationGrid (src/exposure/exposure_engine.cpp:35-48).
@@ -61,7 +66,7 @@ TimeGrid simulationGrid(const TimeGrid& base, const ccr_netting_set& set);
 
 HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
-                   int32_t precision, int32_t block);
+                   int32_t precision, int32_t block, bool per_trade = false);
 
 // Workgroup size of the path kernel: the workgroups of every set, their waves and the offsets of
 // the per-wave partial sums.

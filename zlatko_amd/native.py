@@ -45,6 +45,8 @@ def lib():
         L.ccr_hip_set_shared_workspace.argtypes = [P, I32]
         L.ccr_hip_set_initial_margin.argtypes = [P, D, D]
         L.ccr_hip_initial_margin.argtypes = [P, P, I32, P, I32, P]
+        L.ccr_hip_set_trade_statistics.argtypes = [P, I32]
+        L.ccr_hip_trade_statistics.argtypes = [P, P, I32, I32, P]
         L.ccr_xva_defaults.argtypes = [P]
         L.ccr_xva_defaults.restype = None
         L.ccr_exposure_xva.argtypes = [P, P, I32, P, D, P, P, P]
@@ -136,6 +138,15 @@ class Plan:
 
     def execute_async(self):
         self.engine._check(lib().ccr_hip_execute_async(self.engine.ctx, self.handle))
+
+    def trade_statistics(self, s: int, trade: int) -> dict:
+        """Per-trade statistics of trade `trade` of netting set `s` from the last execute: one array
+        per name of abi.TRADE_STATS, one value per reporting date."""
+        info = self.info()
+        arrays = {n: np.zeros(max(1, info.max_reporting_dates)) for n in abi.TRADE_STATS}
+        out = abi.TradeStatisticsC(info.max_reporting_dates, 0, *[abi.dptr(arrays[n]) for n in abi.TRADE_STATS])
+        self.engine._check(lib().ccr_hip_trade_statistics(self.engine.ctx, self.handle, s, trade, C.byref(out)))
+        return {n: a[:out.num_dates].copy() for n, a in arrays.items()}
 
     def initial_margin(self, s: int = 0) -> np.ndarray:
         """initialMarginProfile (src/xva/xva.cpp:59-76) of netting set `s` from the last execute,
@@ -237,6 +248,11 @@ class Engine:
         """Plans compiled from now on keep their per-execute buffers in one workspace of the
         context (valid until the next execute of any plan), so waves of netting sets fit in HBM."""
         self._check(lib().ccr_hip_set_shared_workspace(self.ctx, 1 if enable else 0))
+
+    def set_trade_statistics(self, enable: bool):
+        """Plans compiled from now on also produce per-trade allocation statistics (at most 16
+        trades per netting set, no Bermudan swaptions)."""
+        self._check(lib().ccr_hip_set_trade_statistics(self.ctx, 1 if enable else 0))
 
     def set_initial_margin(self, quantile: float, margin_period_of_risk: float = 10.0 / 250.0):
         """Plans compiled from now on also produce the initial-margin profile of every netting set

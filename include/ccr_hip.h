@@ -502,6 +502,12 @@ typedef struct ccr_xva_result {
 int32_t ccr_exposure_xva(const double* times, const double* discounted_ee, int32_t size,
                          const ccr_credit_curve* counterparty, double recovery_counterparty,
                          const ccr_credit_curve* own, const ccr_xva_parameters* parameters, double* cva_fca_kva);
+/* allocateExposureXva (xva.cpp:97-106): CVA, FCA and KVA of each trade's discounted Euler
+ * contribution (row i of contributions[num_trades][size], e.g. ccr_trade_statistics'
+ * marginal_discounted) into out[num_trades][3]; the rows add up to the netting set's. */
+int32_t ccr_allocate_exposure_xva(const double* times, const double* contributions, int32_t num_trades, int32_t size,
+                                  const ccr_credit_curve* counterparty, double recovery_counterparty,
+                                  const ccr_credit_curve* own, const ccr_xva_parameters* parameters, double* out);
 /* fundingBenefit (xva.cpp:45-51) from the discounted ENE (<= 0). */
 double ccr_funding_benefit(const double* times, const double* discounted_ene, int32_t size,
                            const ccr_credit_curve* counterparty, const ccr_credit_curve* own,

@@ -443,3 +443,34 @@ def test_trade_statistics_errors():
     for p in (fresh, off):
         p.close()
     e.close()
+
+
+def test_xva_allocation_of_per_trade_contributions_adds_up():
+    """xva_allocation_adds_up (tests/test_xva.cpp:90-106): CVA, FCA and KVA of each trade's
+    discounted Euler contribution from the GPU add up to those of the netting set's EE* (no CSA)
+    within 1e-10 of the total."""
+    import ctypes as C
+    from zlatko_amd import native
+    spec = W.small_reference_case(num_paths=2000, fx=True, credits=True, options=4)
+    e = native.Engine()
+    e.set_trade_statistics(True)
+    plan = e.compile(spec)
+    prof = plan.execute()[0]
+    n = spec.sets_c[0].num_trades
+    contrib = np.ascontiguousarray([plan.trade_statistics(0, i)["marginal_discounted"] for i in range(n)])
+    t = prof.get("times")
+    piv, ch, oh = np.array([1.0, 3.0, 5.0]), np.array([0.01, 0.015, 0.02]), np.array([0.006, 0.006, 0.008])
+    cpty, own = abi.CreditCurveC(abi.dptr(piv), abi.dptr(ch), 3), abi.CreditCurveC(abi.dptr(piv), abi.dptr(oh), 3)
+    p = abi.XvaParametersC()
+    native.lib().ccr_xva_defaults(C.byref(p))
+    parts = np.zeros((n, 3))
+    assert native.lib().ccr_allocate_exposure_xva(abi.dptr(t), abi.dptr(contrib), n, len(t), C.byref(cpty),
+                                                  C.c_double(0.4), C.byref(own), C.byref(p), abi.dptr(parts)) == 0
+    whole = np.zeros(3)
+    assert native.lib().ccr_exposure_xva(abi.dptr(t), abi.dptr(prof.get("discounted_expected_exposure")), len(t),
+                                         C.byref(cpty), C.c_double(0.4), C.byref(own), C.byref(p),
+                                         abi.dptr(whole)) == 0
+    assert (whole > 0).all()
+    assert np.all(np.abs(parts.sum(axis=0) - whole) <= 1e-10 * whole), (parts.sum(axis=0), whole)
+    plan.close()
+    e.close()

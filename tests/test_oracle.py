@@ -364,6 +364,29 @@ def test_xva_matches_live_reference_build():
         assert np.allclose(mine, w, rtol=1e-13, atol=1e-9), (mine, w)
 
 
+@pytest.mark.skipif(not _ref_has("ccr_ref_xva"), reason="reference build (oracle/_ref) not present")
+def test_xva_allocation_matches_live_reference_build():
+    """allocateExposureXva (src/xva/xva.cpp:97-106) is exposureXva of each trade's contribution:
+    the C ABI's rows against the reference's xva() CVA / FCA / KVA of the same rows."""
+    from zlatko_amd import native
+    t = np.array([0.0, 0.25, 0.5, 1.0, 2.0, 3.5, 5.0])
+    contrib = np.ascontiguousarray([1e5 * np.sin(t + k) * (5.0 - t) for k in range(3)])
+    zeros = np.zeros_like(t)
+    piv, ch, oh = np.array([1.0, 4.0]), np.array([0.015, 0.03]), np.array([0.005, 0.007])
+    cpty, own = abi.CreditCurveC(abi.dptr(piv), abi.dptr(ch), 2), abi.CreditCurveC(abi.dptr(piv), abi.dptr(oh), 2)
+    p = abi.XvaParametersC()
+    native.lib().ccr_xva_defaults(C.byref(p))
+    got = np.zeros((3, 3))
+    assert native.lib().ccr_allocate_exposure_xva(abi.dptr(t), abi.dptr(contrib), 3, len(t), C.byref(cpty),
+                                                  C.c_double(0.4), C.byref(own), C.byref(p), abi.dptr(got)) == 0
+    for k in range(3):
+        row = np.ascontiguousarray(contrib[k])
+        want = abi.XvaResultC()
+        assert oracle_lib.ref().ccr_ref_xva(abi.dptr(t), abi.dptr(row), abi.dptr(zeros), len(t), None, C.byref(cpty),
+                                            0.4, C.byref(own), 0.4, C.byref(p), C.byref(want)) == 0
+        assert np.allclose(got[k], [want.cva, want.fca, want.kva], rtol=1e-13, atol=1e-9), (got[k], want.cva)
+
+
 # ------------------------------------------------------------ invariants (oracle-only extensions included)
 
 def test_threads_do_not_change_bits():

@@ -1161,6 +1161,18 @@ int32_t ccr_exposure_xva(const double* t, const double* dee, int32_t n, const cc
   return CCR_OK;
 }
 
+int32_t ccr_allocate_exposure_xva(const double* t, const double* contributions, int32_t num_trades, int32_t n,
+                                  const ccr_credit_curve* counterparty, double rc, const ccr_credit_curve* own,
+                                  const ccr_xva_parameters* p, double* out) {
+  if (!contributions || num_trades < 0) return CCR_ERR_INVALID_ARGUMENT;
+  for (int32_t i = 0; i < num_trades; ++i) {
+    const int32_t rcode = ccr_exposure_xva(t, contributions + static_cast<int64_t>(i) * n, n, counterparty, rc, own, p,
+                                           out + 3 * static_cast<int64_t>(i));
+    if (rcode != CCR_OK) return rcode;
+  }
+  return CCR_OK;
+}
+
 double ccr_funding_benefit(const double* t, const double* dene, int32_t n, const ccr_credit_curve* counterparty,
                            const ccr_credit_curve* own, double spread) {
   if (!t || !dene || !counterparty || !own) return 0.0;

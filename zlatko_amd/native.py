@@ -46,6 +46,7 @@ def lib():
         L.ccr_hip_set_initial_margin.argtypes = [P, D, D]
         L.ccr_hip_initial_margin.argtypes = [P, P, I32, P, I32, P]
         L.ccr_hip_set_trade_statistics.argtypes = [P, I32]
+        L.ccr_allocate_exposure_xva.argtypes = [P, P, I32, I32, P, D, P, P, P]
         L.ccr_hip_trade_statistics.argtypes = [P, P, I32, I32, P]
         L.ccr_xva_defaults.argtypes = [P]
         L.ccr_xva_defaults.restype = None

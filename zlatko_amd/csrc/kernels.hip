@@ -174,20 +174,26 @@ __device__ __noinline__ double lerpUnfused(double lo, double t, double hi) {
   return lo + m;
 }
 
-constexpr int kPfeThreads = 256;
-constexpr int kPfeWaves = kPfeThreads / 64;
+// Threads per workgroup of the select: one per eight keys of the buffer (512 for 4,096 keys,
+// 1,024 for 8,192), so the buffer's sorts take four compare-exchanges per thread and pass, and the
+// row pass keeps that many loads in flight while the buffer limits workgroups per CU.
+template <int kPfeKeys>
+constexpr int pfeThreads() {
+  return kPfeKeys / 8;
+}
+constexpr int kPfeMaxWaves = 16;
 constexpr int kPfeSample = 2048; // smallest sample of a row longer than the key buffer
 constexpr int kPfeBins = 4096;   // 12-bit digits of the radix fallback (histogram aliases the keys)
 constexpr int kPfeUnroll = 4;    // row elements in flight per thread
 
 // Ascending bitonic sort of keys[0, n2), n2 a power of two, by the whole workgroup: each step
 // visits the n2 / 2 compare-exchange pairs once.
-template <typename K>
+template <int kThreads, typename K>
 __device__ inline void bitonicSort(K* keys, int n2) {
   const int tid = static_cast<int>(threadIdx.x);
   for (int k = 2; k <= n2; k <<= 1)
     for (int jj = k >> 1; jj > 0; jj >>= 1) {
-      for (int p = tid; p < (n2 >> 1); p += kPfeThreads) {
+      for (int p = tid; p < (n2 >> 1); p += kThreads) {
         const int i = ((p & ~(jj - 1)) << 1) | (p & (jj - 1));
         const int ixj = i + jj;
         const bool asc = (i & k) == 0;
@@ -235,12 +241,13 @@ __device__ inline void appendKeys(bool take, K k, K* keys, unsigned int* count) 
 }
 
 // Finds the bin of hist[0, kPfeBins) holding rank `rank` (0-based, below the histogram's total):
-// each thread owns 16 consecutive bins, a workgroup prefix sum of the per-thread counts names the
+// each thread owns kPfeBins / kThreads consecutive bins, a workgroup prefix sum of the per-thread counts names the
 // owner, which scans its bins. Leaves the bin and the rank within it in *bin / *binRank; ends with
 // a barrier.
+template <int kThreads>
 __device__ inline void pfeFindBin(const unsigned int* hist, long long rank, unsigned int* waveTot, int* bin,
                                   long long* binRank) {
-  constexpr int per = kPfeBins / kPfeThreads;
+  constexpr int per = kPfeBins / kThreads;
   const int tid = static_cast<int>(threadIdx.x), lane = tid & 63, w = tid >> 6;
   unsigned int c[per], tot = 0;
 #pragma unroll
@@ -281,14 +288,16 @@ __device__ inline void pfeFindBin(const unsigned int* hist, long long rank, unsi
 // Otherwise (a sample miss, or too many keys between the splitters: large P with a central
 // quantile) an MSB radix select over 12-bit digits narrows the prefix until its keys fit in LDS.
 template <typename Real, int kPfeKeys>
-__global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets, const Real* rows, int max_rep,
+__global__ void __launch_bounds__(pfeThreads<kPfeKeys>()) ccr_pfe_kernel(const SetDev* sets, const Real* rows, int max_rep,
                                                               int num_paths, double level, double* out,
                                                               int which) {
   using K = typename KeyOf<Real>::T;
   constexpr int kBits = KeyOf<Real>::kBits;
   __shared__ K keys[kPfeKeys];
   unsigned int* hist = reinterpret_cast<unsigned int*>(keys);
-  __shared__ unsigned int sBelow, sEqA, sEqB, sOpen, sWave[kPfeWaves];
+  constexpr int kPfeThreads = pfeThreads<kPfeKeys>();
+  static_assert(kPfeThreads / 64 <= kPfeMaxWaves && kPfeBins % kPfeThreads == 0, "select workgroup shape");
+  __shared__ unsigned int sBelow, sEqA, sEqB, sOpen, sWave[kPfeMaxWaves];
   __shared__ K sMinAbove;
   __shared__ long long sRank;
   __shared__ int sBin;
@@ -319,7 +328,7 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
   while (n2 < ns) n2 <<= 1;
   for (int i = tid; i < n2; i += kPfeThreads) keys[i] = i < ns ? keyOf<Real>(row[i]) : ~K(0);
   __syncthreads();
-  bitonicSort(keys, n2);
+  bitonicSort<kPfeThreads>(keys, n2);
   if (ns == num_paths) {
     if (tid == 0) finish(keys[lo], keys[hi]);
     return;
@@ -377,7 +386,7 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
       while (m2 < static_cast<int>(nOpen)) m2 <<= 1;
       for (int i = static_cast<int>(nOpen) + tid; i < m2; i += kPfeThreads) keys[i] = ~K(0);
       __syncthreads();
-      bitonicSort(keys, m2);
+      bitonicSort<kPfeThreads>(keys, m2);
       if (tid == 0) {
         auto at = [&](long long r, K* v) {
           long long i = r - nBelow;
@@ -422,7 +431,7 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
       if ((k & mask) == prefix) atomicAdd(&hist[static_cast<int>((k >> shift) & digit)], 1u);
     }
     __syncthreads();
-    pfeFindBin(hist, rank, sWave, &sBin, &sRank);
+    pfeFindBin<kPfeThreads>(hist, rank, sWave, &sBin, &sRank);
     const int bin = sBin;
     const unsigned int cnt = hist[bin];
     rank = sRank;
@@ -455,7 +464,7 @@ __global__ void __launch_bounds__(kPfeThreads) ccr_pfe_kernel(const SetDev* sets
     while (m2 < static_cast<int>(cnt)) m2 <<= 1;
     for (int i = static_cast<int>(cnt) + tid; i < m2; i += kPfeThreads) keys[i] = ~K(0);
     __syncthreads();
-    bitonicSort(keys, m2);
+    bitonicSort<kPfeThreads>(keys, m2);
     if (tid == 0) {
       const K kLo = keys[rank];
       const K kHi = hi == lo ? kLo : (rank + 1 < static_cast<long long>(cnt) ? keys[rank + 1] : sMinAbove);
@@ -602,11 +611,11 @@ hipError_t launchPfe(const SetDev* sets, const Real* rows, int nsets, int max_re
   // Rows beyond 100,000 paths take an 8,192-key buffer, so the keys between the splitters of a
   // 4,096-key sample still fit.
   if (num_paths > 100000)
-    hipLaunchKernelGGL((ccr_pfe_kernel<Real, 8192>), dim3(nsets * max_rep), dim3(256), 0, stream, sets, rows, max_rep,
-                       num_paths, level, out, which);
+    hipLaunchKernelGGL((ccr_pfe_kernel<Real, 8192>), dim3(nsets * max_rep), dim3(pfeThreads<8192>()), 0, stream, sets,
+                       rows, max_rep, num_paths, level, out, which);
   else
-    hipLaunchKernelGGL((ccr_pfe_kernel<Real, 4096>), dim3(nsets * max_rep), dim3(256), 0, stream, sets, rows, max_rep,
-                       num_paths, level, out, which);
+    hipLaunchKernelGGL((ccr_pfe_kernel<Real, 4096>), dim3(nsets * max_rep), dim3(pfeThreads<4096>()), 0, stream, sets,
+                       rows, max_rep, num_paths, level, out, which);
   return hipGetLastError();
 }
 

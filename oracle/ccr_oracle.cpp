@@ -914,14 +914,15 @@ double secondsSince(std::chrono::steady_clock::time_point t0) {
 constexpr int kTradeStatCount = 5;
 void runSet(const Generator& g, const ccr_simulation_config& cfg, const Grid& base, const ccr_netting_set& ns,
             double q, int threads, ccr_exposure_out* out, Vec* im = nullptr, double imQ = 0.99,
-            double imMpor = 10.0 / 250.0, std::vector<Vec>* tstats = nullptr) {
+            double imMpor = 10.0 / 250.0, std::vector<Vec>* tstats = nullptr, const Grid* repGrid = nullptr,
+            const Grid* simGrid = nullptr) {
   if (ns.num_trades <= 0 || !ns.trades) invalid("ExposureEngine::run: empty netting set");
   std::vector<Trade> trades;
   for (int i = 0; i < ns.num_trades; ++i) trades.push_back(makeTrade(ns.trades[i], g));
   const int nC = static_cast<int>(g.credits.size());
   if (ns.counterparty_credit >= nC || ns.own_credit >= nC) throw Failure(CCR_ERR_OUT_OF_RANGE, "credit index out of range");
-  const Grid rep = reportingGrid(base, trades);
-  const Grid sim = simulationGrid(base, trades, ns);
+  const Grid rep = repGrid ? *repGrid : reportingGrid(base, trades);
+  const Grid sim = simGrid ? *simGrid : simulationGrid(base, trades, ns);
   auto t0 = std::chrono::steady_clock::now();
   double drawS = 0.0;
   const Scenarios s = g.generate(sim, static_cast<std::size_t>(cfg.num_paths), cfg.seed, cfg.antithetic != 0, &drawS);
@@ -1115,6 +1116,35 @@ int32_t ccr_oracle_run_exposure(const ccr_scenario_generator* generator, const c
     const Grid base(Vec(base_grid, base_grid + std::max(0, base_size)));
     for (int i = 0; i < num_sets; ++i) runSet(g, *config, base, sets[i], pfe_quantile, threads, &outs[i]);
     gTiming.total_s = secondsSince(t0);
+  });
+}
+
+// Portfolio book (src/app/analyses.cpp:1690-1709, 1727-1766): every set reports on the union of
+// the sets' reporting grids and simulates the union of their simulation grids built on it.
+int32_t ccr_oracle_run_book(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                            const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
+                            double pfe_quantile, int32_t threads, ccr_exposure_out* outs, char* err,
+                            int32_t err_capacity) {
+  return guard(err, err_capacity, [&] {
+    if (!generator || !config || !sets || !outs || num_sets <= 0) invalid("null argument");
+    const Generator g(*generator);
+    const Grid base(Vec(base_grid, base_grid + std::max(0, base_size)));
+    std::vector<std::vector<Trade>> trades(static_cast<std::size_t>(num_sets));
+    Vec r, v;
+    for (int i = 0; i < num_sets; ++i) {
+      for (int k = 0; k < sets[i].num_trades; ++k) trades[static_cast<std::size_t>(i)].push_back(makeTrade(sets[i].trades[k], g));
+      const Grid gr = reportingGrid(base, trades[static_cast<std::size_t>(i)]);
+      r.insert(r.end(), gr.t.begin(), gr.t.end());
+    }
+    const Grid rep(r);
+    for (int i = 0; i < num_sets; ++i) {
+      const Grid gs = simulationGrid(rep, trades[static_cast<std::size_t>(i)], sets[i]);
+      v.insert(v.end(), gs.t.begin(), gs.t.end());
+    }
+    const Grid sim(v);
+    for (int i = 0; i < num_sets; ++i)
+      runSet(g, *config, base, sets[i], pfe_quantile, threads, &outs[i], nullptr, 0.99, 10.0 / 250.0, nullptr, &rep,
+             &sim);
   });
 }
 

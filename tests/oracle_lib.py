@@ -34,6 +34,8 @@ def oracle():
         lib.ccr_oracle_run_exposure.argtypes = [P, P, P, I32, P, I32, D, I32, P, C.c_char_p, I32]
         lib.ccr_oracle_initial_margin.restype = I32
         lib.ccr_oracle_trade_statistics.restype = I32
+        lib.ccr_oracle_run_book.restype = I32
+        lib.ccr_oracle_run_book.argtypes = [P, P, P, I32, P, I32, D, I32, P, C.c_char_p, I32]
         lib.ccr_oracle_trade_statistics.argtypes = [P, P, P, I32, P, I32, P, P, I32, C.c_char_p, I32]
         lib.ccr_oracle_initial_margin.argtypes = [P, P, P, I32, P, D, D, I32, P, P, I32, P, C.c_char_p, I32]
         for n in ("ccr_oracle_discount", "ccr_oracle_instantaneous_forward", "ccr_oracle_survival",
@@ -225,6 +227,32 @@ def ref_initial_margin(spec: abi.Spec, i: int, quantile: float, mpor: float = 10
     if rc != 0:
         raise RuntimeError(f"reference failed ({rc}): {err.value.decode()}")
     return margin[:n.value]
+
+
+def book_grid_size(spec: abi.Spec) -> int:
+    """Reporting dates of the book: the union of the sets' reporting grids."""
+    t = set()
+    for i in range(len(spec.sets)):
+        cap = grid_size(spec, i)
+        out = np.zeros(cap)
+        n = C.c_int32()
+        oracle().ccr_oracle_reporting_grid(abi.dptr(spec.base_grid), len(spec.base_grid), C.byref(spec.sets_c[i]),
+                                           abi.dptr(out), cap, C.byref(n))
+        t.update(out[:n.value].tolist())
+    return len(t) + 8  # capacity with room; the union de-duplicates within 1e-9
+
+
+def run_oracle_book(spec: abi.Spec, threads: int = 4, rows: bool = False):
+    cap = book_grid_size(spec)
+    outs, arr = abi.make_outs(len(spec.sets), cap, spec.num_paths, rows)
+    err = C.create_string_buffer(512)
+    rc = oracle().ccr_oracle_run_book(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                      len(spec.base_grid), spec.sets_c, len(spec.sets), spec.pfe_quantile, threads,
+                                      arr, err, 512)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
+    abi.sync_outs(outs, arr)
+    return outs
 
 
 def oracle_timing() -> OracleTiming:

@@ -387,6 +387,57 @@ def test_xva_allocation_matches_live_reference_build():
         assert np.allclose(got[k], [want.cva, want.fca, want.kva], rtol=1e-13, atol=1e-9), (got[k], want.cva)
 
 
+def _sim_grid(base, ns_c):
+    cap = 4096
+    out = np.zeros(cap)
+    n = C.c_int32()
+    O().ccr_oracle_simulation_grid(abi.dptr(base), len(base), C.byref(ns_c), abi.dptr(out), cap, C.byref(n))
+    return out[:n.value]
+
+
+def test_oracle_book_shares_scenarios_and_grids():
+    """Book mode: every set reports on the union of the reporting grids; the deflator is pathwise
+    the same for all sets (E[D] bit-identical) and so is the survival of a shared counterparty."""
+    spec = W.book_case(num_paths=300)
+    outs = oracle_lib.run_oracle_book(spec, threads=3)
+    t0 = outs[0].get("times")
+    for o in outs[1:]:
+        assert np.array_equal(o.get("times"), t0)
+        assert np.array_equal(o.get("mean_deflator"), outs[0].get("mean_deflator"))
+    assert np.array_equal(outs[0].get("mean_survival"), outs[1].get("mean_survival"))
+    assert not np.array_equal(outs[0].get("mean_survival"), outs[2].get("mean_survival"))
+    own = [oracle_lib.grid_size(spec, i) for i in range(3)]
+    assert len(t0) > max(own)
+
+
+@pytest.mark.skipif(oracle_lib.ref() is None, reason="reference build (oracle/_ref) not present")
+def test_oracle_book_matches_reference_engine_on_the_union_grid():
+    """An uncollateralised set of the book is the reference engine run on the book's simulation
+    grid as its base grid (then its simulation grid is that union and its draws are the book's):
+    the oracle's book profile equals the reference's on the book's reporting dates, bit for bit."""
+    spec = W.book_case(num_paths=200)
+    outs = oracle_lib.run_oracle_book(spec, threads=3)
+    rep = outs[0].get("times")
+    sim = np.unique(np.concatenate([_sim_grid(np.ascontiguousarray(rep), spec.sets_c[i]) for i in range(3)]))
+    sim = np.ascontiguousarray(sim)
+    for i in (1, 2):
+        cap = len(sim) + 8
+        ref_outs, arr = abi.make_outs(1, cap, spec.num_paths, False)
+        err = C.create_string_buffer(256)
+        cva, dva = C.c_double(), C.c_double()
+        rc = oracle_lib.ref().ccr_ref_run_exposure(C.byref(spec.generator), C.byref(spec.config), abi.dptr(sim),
+                                                   len(sim), C.byref(spec.sets_c[i]), spec.pfe_quantile, arr,
+                                                   C.byref(cva), C.byref(dva), err, 256)
+        assert rc == 0, err.value
+        abi.sync_outs(ref_outs, arr)
+        r = ref_outs[0]
+        idx = [int(np.argmin(np.abs(r.get("times") - x))) for x in rep]
+        assert np.abs(r.get("times")[idx] - rep).max() <= 1e-9
+        for f in ("expected_value", "expected_exposure", "expected_negative_exposure", "potential_future_exposure",
+                  "discounted_expected_exposure", "discounted_expected_negative_exposure", "mean_deflator"):
+            assert np.array_equal(r.get(f)[idx], outs[i].get(f)), f
+
+
 # ------------------------------------------------------------ invariants (oracle-only extensions included)
 
 def test_threads_do_not_change_bits():

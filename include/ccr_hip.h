@@ -391,6 +391,20 @@ int32_t ccr_hip_set_initial_margin(ccr_hip_ctx* ctx, double quantile, double mar
 int32_t ccr_hip_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set, double* margin,
                                int32_t capacity, int32_t* size);
 
+/* Portfolio book (src/app/analyses.cpp:1604-1709, 1727-1766): plans compiled after
+ * ccr_hip_set_book(ctx, 1) simulate every netting set on the union of the sets' simulation grids
+ * and report on the union of their reporting grids, from one seed, so rates, FX, intensities and
+ * deflators are pathwise identical across the book and EE, ENE, EE*, ENE* and the pathwise CVA /
+ * DVA increments of different netting sets add up exactly. */
+int32_t ccr_hip_set_book(ccr_hip_ctx* ctx, int32_t enable);
+/* sumProfiles (src/app/analyses.cpp:1769-1784): the additive fields (expected value, EE, ENE, EE*,
+ * ENE*, CVA and DVA increments) summed over the netting sets of counterparty credit
+ * `counterparty` (-1: every set) of one book execute; times, mean deflator and survival are the
+ * first set's; PFE is 0 (quantiles do not add). CCR_ERR_INVALID_ARGUMENT if the sets' grids differ,
+ * CCR_ERR_OUT_OF_RANGE if no set has that counterparty. Needs no device. */
+int32_t ccr_book_totals(const ccr_exposure_out* outs, const ccr_netting_set* sets, int32_t num_sets,
+                        int32_t counterparty, ccr_exposure_out* total);
+
 /* Per-trade statistics (gpu::FusedOptions::perTrade and FusedResult::TradeStats,
  * include/ccr/gpu/fused_exposure.hpp:251-256, 299; the estimators of marginalExposureContributions
  * and incrementalExposure, src/exposure/allocation.cpp:8-45): plans compiled after

@@ -132,13 +132,32 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
   a.bermv = bermv.data();
 }
 
+int32_t runExposure(const ccr_scenario_generator* gen, const ccr_simulation_config* cfg, const double* base,
+                    int32_t nbase, const ccr_netting_set* sets, int32_t nsets, double q, ccr_exposure_out* outs,
+                    char* err, int32_t cap, bool book);
+
 }  // namespace
 
 extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr_simulation_config* cfg,
                                     const double* base, int32_t nbase, const ccr_netting_set* sets, int32_t nsets,
                                     double q, ccr_exposure_out* outs, char* err, int32_t cap) {
+  return runExposure(gen, cfg, base, nbase, sets, nsets, q, outs, err, cap, false);
+}
+
+// The same with the plan built in book mode (common grids for every set).
+extern "C" int32_t emu_run_book(const ccr_scenario_generator* gen, const ccr_simulation_config* cfg,
+                                const double* base, int32_t nbase, const ccr_netting_set* sets, int32_t nsets,
+                                double q, ccr_exposure_out* outs, char* err, int32_t cap) {
+  return runExposure(gen, cfg, base, nbase, sets, nsets, q, outs, err, cap, true);
+}
+
+namespace {
+
+int32_t runExposure(const ccr_scenario_generator* gen, const ccr_simulation_config* cfg, const double* base,
+                    int32_t nbase, const ccr_netting_set* sets, int32_t nsets, double q, ccr_exposure_out* outs,
+                    char* err, int32_t cap, bool book) {
   try {
-    HostPlan h = buildPlan(*gen, *cfg, base, nbase, sets, nsets, q, CCR_FP64, 256);
+    HostPlan h = buildPlan(*gen, *cfg, base, nbase, sets, nsets, q, CCR_FP64, 256, false, book);
     std::vector<double> stream(static_cast<size_t>(h.stream_len));
     generateStream(cfg->seed, h.stream_len, stream.data());
     std::vector<double> steps = h.steps, dreal = h.date_reals, bonds = h.bonds, fixed = h.fixed;
@@ -218,6 +237,8 @@ extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr
     return CCR_ERR_RUNTIME;
   }
 }
+
+}  // namespace
 
 // Per-trade plan of one netting set, run through the kernel body's per-trade path (the generic
 // shape with per-trade values, as the GPU launch picks): stats[(i * kTradeStats + m) * capacity + r]

@@ -474,3 +474,35 @@ def test_xva_allocation_of_per_trade_contributions_adds_up():
     assert np.all(np.abs(parts.sum(axis=0) - whole) <= 1e-10 * whole), (parts.sum(axis=0), whole)
     plan.close()
     e.close()
+
+
+def test_book_matches_oracle_and_totals_add_up_fp64():
+    """Portfolio book: every netting set on the union grids with pathwise identical scenarios
+    (E[D] bit-identical across sets on the GPU), profiles within 1e-10 of the oracle's book, and
+    ccr_book_totals per counterparty equal to the sums of the sets' additive fields."""
+    import ctypes as C
+    from zlatko_amd import native
+    spec = W.book_case(num_paths=3000)
+    e = native.Engine()
+    e.set_book(True)
+    plan = e.compile(spec)
+    outs = plan.execute()
+    _check(spec, outs, oracle_lib.run_oracle_book(spec, threads=4))
+    for o in outs[1:]:
+        assert np.array_equal(o.get("mean_deflator"), outs[0].get("mean_deflator"))
+    arr = (abi.ExposureOutC * len(outs))(*[o.c for o in outs])
+    for cp, members in ((0, [0, 1]), (1, [2]), (-1, [0, 1, 2])):
+        tot, tarr = abi.make_outs(1, outs[0].c.num_dates, 0, False)
+        assert native.lib().ccr_book_totals(arr, spec.sets_c, len(outs), cp, tarr) == 0
+        abi.sync_outs(tot, tarr)
+        for f in ("expected_exposure", "discounted_expected_exposure", "cva_increments", "dva_increments"):
+            want = sum(outs[i].get(f) for i in members)
+            assert np.abs(tot[0].get(f) - want).max() <= 1e-12 * max(np.abs(want).max(), 1.0), (cp, f)
+    tot, tarr = abi.make_outs(1, outs[0].c.num_dates, 0, False)
+    assert native.lib().ccr_book_totals(arr, spec.sets_c, len(outs), 5, tarr) == abi.CCR_ERR_OUT_OF_RANGE
+    plan.close()
+    e.set_book(False)
+    single = e.compile(spec)
+    assert single.info().max_reporting_dates < outs[0].c.num_dates  # own grids again
+    single.close()
+    e.close()

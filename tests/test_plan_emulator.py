@@ -23,6 +23,7 @@ def emu():
     lib.emu_run_exposure.argtypes = [C.c_void_p] * 3 + [C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.c_void_p,
                                                          C.c_char_p, C.c_int32]
     lib.emu_plan_stats.argtypes = [C.c_void_p] * 3 + [C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]
+    lib.emu_run_book.argtypes = lib.emu_run_exposure.argtypes
     lib.emu_run_trade_statistics.argtypes = [C.c_void_p] * 3 + [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
                                                                  C.c_char_p, C.c_int32]
     return lib
@@ -120,3 +121,20 @@ def test_emulated_trade_statistics_match_oracle(emu, kw):
     for k, m in enumerate(abi.TRADE_STATS):
         e = np.abs(got[:, k, :] - want[m]).max() / scale
         assert e <= 1e-12, (m, e)
+
+
+def test_emulated_book_matches_oracle(emu):
+    """Book plans (union grids, shared scenarios) through the plan builder and kernel body on the
+    CPU against the oracle's book restatement."""
+    spec = W.book_case(num_paths=96)
+    want = oracle_lib.run_oracle_book(spec)
+    cap = oracle_lib.book_grid_size(spec)
+    outs, arr = abi.make_outs(len(spec.sets), cap, spec.num_paths, False)
+    err = C.create_string_buffer(512)
+    rc = emu.emu_run_book(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                          len(spec.base_grid), spec.sets_c, len(spec.sets), spec.pfe_quantile, arr, err, 512)
+    assert rc == 0, err.value
+    abi.sync_outs(outs, arr)
+    for o, w in zip(outs, want):
+        errs = compare.field_errors(o, w)
+        assert errs["times"] == 0.0 and compare.worst(errs) <= 1e-12, errs

@@ -59,6 +59,8 @@ struct ccr_hip_ctx {
   double im_quantile = 0.0, im_mpor = 10.0 / 250.0;
   // Per-trade statistics for plans compiled while set.
   bool per_trade = false;
+  // Book mode (common grids, pathwise identical scenarios) for plans compiled while set.
+  bool book = false;
 };
 
 namespace {
@@ -331,7 +333,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   auto plan = std::make_unique<ccr_hip_plan>();
   plan->device = ctx->device;
   plan->host = buildPlan(*generator, *config, base_grid, base_size, sets, num_sets, pfe_quantile, precision,
-                         ctx->block > 0 ? ctx->block : kMaxBlock, ctx->per_trade);
+                         ctx->block > 0 ? ctx->block : kMaxBlock, ctx->per_trade, ctx->book);
   HostPlan& h = plan->host;
   if (ctx->block == 0) assignBlocks(h, occupancyBlock(h));
   const bool fp32 = precision == CCR_FP32;
@@ -846,6 +848,58 @@ int32_t ccr_hip_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32
       margin[k] = plan->im_q[qBase + static_cast<size_t>(k - 1)] * scale * plan->im_dsum[dBase + static_cast<size_t>(k)] / paths;
     }
   });
+}
+
+int32_t ccr_hip_set_book(ccr_hip_ctx* ctx, int32_t enable) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  ctx->book = enable != 0;
+  return CCR_OK;
+}
+
+int32_t ccr_book_totals(const ccr_exposure_out* outs, const ccr_netting_set* sets, int32_t num_sets,
+                        int32_t counterparty, ccr_exposure_out* total) {
+  if (!outs || !sets || !total || num_sets <= 0) return CCR_ERR_INVALID_ARGUMENT;
+  // sumProfiles (src/app/analyses.cpp:1769-1784) plus the pathwise CVA / DVA increments, which are
+  // linear in the exposure on common scenarios; PFE does not add across netting sets.
+  int32_t n = -1;
+  for (int32_t s = 0; s < num_sets; ++s) {
+    if (counterparty >= 0 && sets[s].counterparty_credit != counterparty) continue;
+    if (n < 0) n = outs[s].num_dates;
+    if (outs[s].num_dates != n) return CCR_ERR_INVALID_ARGUMENT;  // not one book: grids differ
+    if (outs[s].times && outs[0].times)
+      for (int32_t k = 0; k < n; ++k)
+        if (outs[s].times[k] != outs[0].times[k]) return CCR_ERR_INVALID_ARGUMENT;
+  }
+  if (n < 0) return CCR_ERR_OUT_OF_RANGE;  // no netting set of this counterparty
+  if (total->capacity < n) return CCR_ERR_INVALID_ARGUMENT;
+  total->num_dates = n;
+  double* const dst[] = {total->expected_value, total->expected_exposure, total->expected_negative_exposure,
+                         total->discounted_expected_exposure, total->discounted_expected_negative_exposure,
+                         total->cva_increments, total->dva_increments};
+  for (int m = 0; m < 7; ++m)
+    if (dst[m])
+      for (int32_t k = 0; k < n; ++k) dst[m][k] = 0.0;
+  bool first = true;
+  for (int32_t s = 0; s < num_sets; ++s) {
+    if (counterparty >= 0 && sets[s].counterparty_credit != counterparty) continue;
+    const ccr_exposure_out& o = outs[s];
+    const double* const src[] = {o.expected_value, o.expected_exposure, o.expected_negative_exposure,
+                                 o.discounted_expected_exposure, o.discounted_expected_negative_exposure,
+                                 o.cva_increments, o.dva_increments};
+    for (int m = 0; m < 7; ++m)
+      if (dst[m] && src[m])
+        for (int32_t k = 0; k < n; ++k) dst[m][k] += src[m][k];
+    if (first) {
+      for (int32_t k = 0; k < n; ++k) {
+        if (total->times && o.times) total->times[k] = o.times[k];
+        if (total->mean_deflator && o.mean_deflator) total->mean_deflator[k] = o.mean_deflator[k];
+        if (total->mean_survival && o.mean_survival) total->mean_survival[k] = o.mean_survival[k];
+        if (total->potential_future_exposure) total->potential_future_exposure[k] = 0.0;
+      }
+      first = false;
+    }
+  }
+  return CCR_OK;
 }
 
 int32_t ccr_hip_set_trade_statistics(ccr_hip_ctx* ctx, int32_t enable) {

@@ -346,7 +346,7 @@ void generateStream(uint64_t seed, int64_t n, double* out) {
 
 HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
-                   int32_t precision, int32_t block, bool per_trade) {
+                   int32_t precision, int32_t block, bool per_trade, bool book) {
   if (!sets || num_sets <= 0) invalid("no netting sets");
   if (cfg.num_paths <= 0) invalid("ScenarioGenerator::generate: numPaths must be positive");
   if (cfg.num_paths > (int64_t(1) << 31) - 1) fail(CCR_ERR_UNSUPPORTED, "at most 2^31 - 1 paths");
@@ -393,6 +393,24 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       g.chol[slot * kMaxCorr + k] = m.chol[static_cast<std::size_t>(i * m.ncorr + k)];
   }
 
+  // Book: every netting set reports on the union of the sets' reporting grids and simulates the
+  // union of their simulation grids (each built on that reporting union), so the scenarios are
+  // pathwise identical across the sets and additive statistics add up exactly
+  // (src/app/analyses.cpp:1690-1709, 1727-1766).
+  TimeGrid bookRep, bookSim;
+  if (book) {
+    std::vector<double> r, v;
+    for (int s = 0; s < num_sets; ++s) {
+      const TimeGrid g = reportingGrid(base, sets[s]);
+      r.insert(r.end(), g.t.begin(), g.t.end());
+    }
+    bookRep = TimeGrid(r);
+    for (int s = 0; s < num_sets; ++s) {
+      const TimeGrid g = simulationGrid(bookRep, sets[s]);
+      v.insert(v.end(), g.t.begin(), g.t.end());
+    }
+    bookSim = TimeGrid(v);
+  }
   int64_t maxStride = 0;
   for (int s = 0; s < num_sets; ++s) {
     const ccr_netting_set& ns = sets[s];
@@ -404,8 +422,8 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       Trade t = makeTrade(ns.trades[i], m);
       (t.kind == CCR_TRADE_BERMUDAN_SWAPTION ? berms : trades).push_back(std::move(t));
     }
-    const TimeGrid rep = reportingGrid(base, ns);
-    const TimeGrid sim = simulationGrid(base, ns);
+    const TimeGrid rep = book ? bookRep : reportingGrid(base, ns);
+    const TimeGrid sim = book ? bookSim : simulationGrid(base, ns);
     const int T = static_cast<int>(sim.size()), R = static_cast<int>(rep.size());
     plan.rep_times.push_back(rep.t);
     plan.sim_times.push_back(sim.t);

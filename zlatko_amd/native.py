@@ -46,6 +46,8 @@ def lib():
         L.ccr_hip_set_initial_margin.argtypes = [P, D, D]
         L.ccr_hip_initial_margin.argtypes = [P, P, I32, P, I32, P]
         L.ccr_hip_set_trade_statistics.argtypes = [P, I32]
+        L.ccr_hip_set_book.argtypes = [P, I32]
+        L.ccr_book_totals.argtypes = [P, P, I32, I32, P]
         L.ccr_allocate_exposure_xva.argtypes = [P, P, I32, I32, P, D, P, P, P]
         L.ccr_hip_trade_statistics.argtypes = [P, P, I32, I32, P]
         L.ccr_xva_defaults.argtypes = [P]
@@ -249,6 +251,11 @@ class Engine:
         """Plans compiled from now on keep their per-execute buffers in one workspace of the
         context (valid until the next execute of any plan), so waves of netting sets fit in HBM."""
         self._check(lib().ccr_hip_set_shared_workspace(self.ctx, 1 if enable else 0))
+
+    def set_book(self, enable: bool):
+        """Plans compiled from now on put every netting set on the book's common grids (pathwise
+        identical scenarios, additive statistics)."""
+        self._check(lib().ccr_hip_set_book(self.ctx, 1 if enable else 0))
 
     def set_trade_statistics(self, enable: bool):
         """Plans compiled from now on also produce per-trade allocation statistics (at most 16

@@ -249,3 +249,31 @@ def small_reference_case(num_paths: int = 64, dates: int = 12, csa: bool = False
     if credits:
         ns.counterparty_credit, ns.own_credit = 0, 1
     return Spec(model, [ns], uniform_grid(5.0, dates), num_paths, seed, True, 0.95, "small")
+
+
+def book_case(num_paths: int = 256, dates: int = 10, seed: int = 7) -> Spec:
+    """A small portfolio book: three netting sets on different date sets (a CSA set of swaps, an
+    uncollateralised set of swaps and FX forwards, an uncollateralised set of FX options), the
+    first two facing counterparty credit 0 and the third credit 1."""
+    rng = SplitMix64(23)
+    curve = Curve([0.5, 2.0, 5.0, 10.0], [0.020, 0.025, 0.030, 0.033])
+    model = Model(HullWhite(curve, 0.05, 0.01))
+    model.assets.append(Asset(1.1, 0.12, Curve.flat(0.01)))
+    model.credits.append(Cir(CreditCurve([1.0, 3.0, 5.0], [0.01, 0.015, 0.02]), 0.5, 0.012, 0.08, 0.008, 4))
+    model.credits.append(Cir(CreditCurve.flat(0.01), 0.4, 0.01, 0.05, 0.01, 2))
+    model.correlation = random_correlation(SplitMix64(5), 4, 0.5)
+    swaps = [Trade(CCR_TRADE_SWAP, rng.uniform(1e5, 1e6), rng.uniform(0.02, 0.035), rng.bernoulli(),
+                   schedule=swap_schedule(0.0, rng.uniform(1.0, 5.0), 2)) for _ in range(3)]
+    a = NettingSet(swaps)
+    a.collateral = Collateral(threshold_counterparty=2e4, threshold_own=5e4, mta=1e4, mpr=10.0 / 250.0)
+    a.counterparty_credit, a.own_credit = 0, 1
+    b = NettingSet([Trade(CCR_TRADE_SWAP, rng.uniform(1e5, 1e6), rng.uniform(0.02, 0.035), rng.bernoulli(),
+                          schedule=swap_schedule(0.0, rng.uniform(1.0, 4.0), 4)),
+                    Trade(CCR_TRADE_ASSET_FORWARD, 5e5, 1.05, asset=0, maturity=2.7),
+                    Trade(CCR_TRADE_ASSET_FORWARD, -3e5, 1.12, asset=0, maturity=1.3)])
+    b.counterparty_credit, b.own_credit = 0, 1
+    c = NettingSet([Trade(CCR_TRADE_EUROPEAN_OPTION, rng.uniform(1e5, 1e6) * (-1.0 if i == 2 else 1.0),
+                          rng.uniform(0.95, 1.25), asset=0, maturity=rng.uniform(0.5, 3.0),
+                          option_type=CCR_OPTION_PUT if i % 2 else CCR_OPTION_CALL) for i in range(3)])
+    c.counterparty_credit, c.own_credit = 1, 0
+    return Spec(model, [a, b, c], uniform_grid(5.0, dates), num_paths, seed, True, 0.95, "book")

@@ -32,10 +32,11 @@ from zlatko_amd import abi, native  # noqa: E402
 from zlatko_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+VALU_SIMDS = 1024          # 256 CUs x 4 SIMDs
+VALU_CLOCK_HZ = 2.4e9      # peak engine clock
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() else \
-    "trade·path·date revals/sec"
+METRIC = "trade·path·date revals/sec; CVA wall time at ≤0.5% error, plain vs surrogate"
 # Workload name -> (spec builder taking a path count, default path count).
 WORKLOADS = {
     "c1": (lambda p: W.c1(num_paths=p), 5000),
@@ -60,17 +61,22 @@ WORKLOAD_TEXT = {
 }
 
 
-def measured_traffic(workload: str, precision: str):
-    """HBM bytes per path-kernel launch from the committed counter summary of this workload and
-    precision (FETCH_SIZE and WRITE_SIZE passes of rocprofv3, each scaled by the copy-kernel
-    calibration stated there), or None when no summary holds the line 'traffic_bytes: N'."""
+def profile_figure(workload: str, precision: str, key: str):
+    """A per-launch figure of the path kernel from the committed counter summary of this
+    workload and precision (the line '<key>: N'), or None. traffic_bytes: HBM bytes (FETCH_SIZE
+    and WRITE_SIZE passes of rocprofv3, each scaled by the copy-kernel calibration stated there);
+    valu_instructions: SQ_INSTS_VALU."""
     f = ROOT / "profiles" / f"path_kernel_{workload}_{precision}.md"
     if not f.exists():
         return None, None
     for line in f.read_text().splitlines():
-        if line.strip().startswith("traffic_bytes:"):
+        if line.strip().startswith(key + ":"):
             return float(line.split(":", 1)[1].split()[0]), str(f.relative_to(ROOT))
     return None, None
+
+
+def measured_traffic(workload: str, precision: str):
+    return profile_figure(workload, precision, "traffic_bytes")
 
 
 def describe(spec: abi.Spec, infos, waves: int) -> dict:
@@ -125,6 +131,50 @@ def time_gpu(engine, plans, steps: int, warmup: int):
     engine.synchronize()
     wall = time.perf_counter() - t0
     return wall, {k: float(np.mean(v)) for k, v in sums.items()}, engine.timing()
+
+
+def cva_wall_time(engine, steps: int) -> dict:
+    """The metric's second figure, on the credit + Bermudan workload (c4, 100,000 paths): GPU time
+    of one CVA -- one execute of the plan (simulation, American Monte Carlo or the surrogate,
+    exposures, pathwise CVA increments) -- on the plain FP64 path and on the FP32 path with the
+    MLP surrogate valuing the Bermudans, with the surrogate run's CVA error against the plain one.
+    The surrogate is trained and gated on two independent 20,000-path plans (training time
+    reported apart)."""
+    tr = engine.compile(W.c4(num_paths=20000, seed=101))
+    ho = engine.compile(W.c4(num_paths=20000, seed=202))
+    t0 = time.perf_counter()
+    sur = engine.surrogate_train(tr, ho, train_paths=2000, epochs=80, lr=5e-4, batch=4096, gate=0.02)
+    train_s = time.perf_counter() - t0
+    tr.close()
+    ho.close()
+    spec = W.c4()
+    p64, p32 = engine.compile(spec, abi.CCR_FP64), engine.compile(spec, abi.CCR_FP32)
+
+    def timed(run):
+        run()
+        engine.synchronize()
+        ms = []
+        for _ in range(steps):
+            run()
+            ms.append(engine.timing().total_ms)
+        return float(np.mean(ms))
+
+    plain_ms = timed(p64.execute_async)
+    fp32_ms = timed(p32.execute_async)
+    sur_ms = timed(lambda: p32.execute_surrogate_async(sur))
+    ref = p64.execute()[0]
+    out = p32.execute_surrogate(sur)[0]
+    cva_ref = float(ref.get("cva_increments")[1:].sum())
+    cva = float(out.get("cva_increments")[1:].sum())
+    rep = sur.report
+    for p in (p64, p32):
+        p.close()
+    sur.close()
+    return {"workload": WORKLOAD_TEXT["c4"], "paths": int(spec.num_paths), "plain_fp64_ms": plain_ms,
+            "fp32_regression_ms": fp32_ms, "fp32_surrogate_ms": sur_ms, "speedup_surrogate_vs_plain": plain_ms / sur_ms,
+            "cva_plain_fp64": cva_ref, "cva_fp32_surrogate": cva, "cva_rel_err": abs(cva - cva_ref) / abs(cva_ref),
+            "surrogate_sets": f"{rep.enabled_sets}/{rep.candidate_sets}", "holdout_ee_err": rep.max_ee_error,
+            "train_s": train_s}
 
 
 def host_cpu() -> str:
@@ -182,6 +232,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cva", action="store_true", help="skip the plain vs surrogate CVA wall time (c4)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 2^27 doubles on the GPU (1 GiB read, 1 GiB written): the reference launch "
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--wave-sets", type=int, default=0,
                     help="netting sets per plan (0: all in one plan unless sets x paths exceeds 1e7)")
     args = ap.parse_args()
@@ -189,6 +243,15 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path-sharded multi-GPU design is described, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8}))
+        return
 
     make, default_paths = WORKLOADS[args.workload]
     spec_fn = lambda p: make(p)  # noqa: E731
@@ -248,6 +311,13 @@ def main():
     if traffic is not None and len(specs) == 1 and not args.paths:
         result["roofline"]["traffic"] = traffic
         result["roofline"]["traffic_source"] = source
+    # The ceiling that binds this kernel: VALU issue. Every wave64 VALU instruction occupies its
+    # SIMD for 4 cycles (FP64 FMA: 16 lanes per cycle); 1,024 SIMDs at 2.4 GHz.
+    valu, vsource = profile_figure(args.workload, args.precision, "valu_instructions")
+    if valu is not None and len(specs) == 1 and not args.paths:
+        issue = valu * 4.0 / (VALU_SIMDS * VALU_CLOCK_HZ * kms["path"] * 1e-3)
+        result["compute"] = {"bound": "valu", "valu_instructions": valu, "issue_frac": issue,
+                             "source": vsource}
     for p in plans:
         p.close()
     if not args.no_fp32 and prec == abi.CCR_FP64:
@@ -260,6 +330,8 @@ def main():
                            / np.abs(b.get("expected_exposure")).max()) for a, b in zip(o32, outs))
         result["fp32"] = {"value": revals / (w32 / args.steps), "ms_per_step": 1000.0 * w32 / args.steps,
                           "path_kernel_ms": k32["path"], "ee_rel_err_vs_fp64": ee_err}
+    if not args.no_cva and args.workload == "c2" and not args.paths:
+        result["cva_wall_time"] = cva_wall_time(engine, args.steps)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(spec_fn, args.cpu_seconds, threads)

@@ -312,7 +312,7 @@ int32_t ccr_hip_mlp_predict(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, int32_t 
  * For trades without a closed form (Bermudan swaptions): per netting set, an MLP (BF16 matrix
  * cores, SiLU) maps the per-date state (t, x, running max and min of x along the path) to the
  * set's Bermudan value. It is trained on the pathwise values the American Monte Carlo method
- * produced on a training plan, then judged on an independent hold-out plan: a set keeps the
+ * produced on a training plan, then checked on an independent hold-out plan: a set keeps the
  * surrogate only if its hold-out EE stays within gate_threshold (relative to the peak plain EE);
  * otherwise that set falls back to the regression method. No counterpart in the reference. */
 typedef struct ccr_hip_surrogate ccr_hip_surrogate;

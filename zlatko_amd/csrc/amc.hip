@@ -444,7 +444,8 @@ __global__ void __launch_bounds__(kAmcChunk) amc_value_kernel(AmcArgs a, const R
   const int64_t p = p0 < a.paths ? p0 : a.paths - 1;
   const Real v = bermudanSum<Real>(S, j, p, a.paths, toConst(a.berms), toConst(a.bdates), toConst(a.cont), a.exidx,
                                    a.states, toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off),
-                                   toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(a.poly) + S.poly_off);
+                                   toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(a.poly) + S.poly_off,
+                                   toConst(a.clive), toConst(a.clive_rng)[2 * d], toConst(a.clive_rng)[2 * d + 1]);
   if (p0 < a.paths) a.bermv[static_cast<int64_t>(d) * a.paths + p0] = static_cast<double>(v);
 }
 

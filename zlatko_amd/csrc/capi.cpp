@@ -408,6 +408,8 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.cdates = ncd > 0 ? upload(ctx, plan.get(), h.cdates.data(), h.cdates.size()) : nullptr;
     a.bdate_berm = upload(ctx, plan.get(), h.bdate_berm.data(), h.bdate_berm.size());
     a.bdate_cd = upload(ctx, plan.get(), h.bdate_cd.data(), h.bdate_cd.size());
+    a.clive = upload(ctx, plan.get(), h.clive.data(), h.clive.size());
+    a.clive_rng = upload(ctx, plan.get(), h.clive_rng.data(), h.clive_rng.size());
     a.poly = plan->d_poly;
     a.states = plan->d_states;
     execAlloc(ctx, plan.get(), a.ev, static_cast<size_t>(h.ev_len) * sizeof(double));

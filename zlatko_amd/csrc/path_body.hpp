@@ -162,12 +162,15 @@ __host__ __device__ inline Real evalRange(const CCR_CONST TermRange& tr, Real x,
 // Sum over the Bermudan swaptions of set S of position x value on date j of path p
 // (bermudan_swaption.cpp:104-140): the underlying swap once exercised before the next exercise
 // date (physical; 0 for cash), else max(continuation, 0). states holds x then D ([date][path]
-// from S.state_off, D after state_len); bonds / fixed / poly are the set's tables.
+// from S.state_off, D after state_len); bonds / fixed / poly are the set's tables. Only the
+// swaptions live[lbeg, lend) are visited: the plan lists, per continuation date, those that can
+// be nonzero on some path (the others add an exact 0).
 template <typename Real>
 __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, int64_t p, int64_t paths,
                                             cptr<BermDev> berms, cptr<BermDate> bdates, cptr<double> cont,
                                             const int16_t* exidx, const double* states, cptr<Real> bonds,
-                                            cptr<Real> fixed, cptr<double> poly) {
+                                            cptr<Real> fixed, cptr<double> poly, cptr<int32_t> live, int lbeg,
+                                            int lend) {
   const double* xrow = states + S.state_off;
   const double xd = xrow[static_cast<int64_t>(j) * paths + p];
   const Real x = static_cast<Real>(xd);
@@ -177,7 +180,8 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
   // all paths): z is computed once and again only if a record holds other constants.
   double zCenter = 0.0, zScale = 0.0, z = 0.0;
   bool haveZ = false;
-  for (int b = S.berm_beg; b < S.berm_end; ++b) {
+  for (int l = lbeg; l < lend; ++l) {
+    const int b = live[l];
     const CCR_CONST BermDev& bm = berms[b];
     const CCR_CONST BermDate& bd = bdates[bm.date_rec + j];
     const bool exercised = exidx[static_cast<int64_t>(b) * paths + p] < bd.next_k;

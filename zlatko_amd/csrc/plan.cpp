@@ -799,6 +799,19 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
           plan.cdates.push_back(s);
           plan.cdates.push_back(j);
           plan.ctimes.push_back(sim[static_cast<std::size_t>(j)]);
+          // The swaptions with a value on this date: an exercise still ahead (continuation), or
+          // an underlying with cash flows left that exercised paths hold. The others are 0 on
+          // every path and the valuation skips them.
+          plan.clive_rng.push_back(static_cast<int32_t>(plan.clive.size()));
+          for (int b = 0; b < nB; ++b) {
+            const Collector& c = bCols[static_cast<std::size_t>(b)][static_cast<std::size_t>(j)];
+            const bool ahead = bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(j)] <
+                               static_cast<int>(berms[static_cast<std::size_t>(b)].exercise.size());
+            const bool held = bHas[static_cast<std::size_t>(b)][static_cast<std::size_t>(j)] &&
+                              (!c.bond.empty() || !c.fix.empty() || c.c0[0] != 0.0);
+            if (ahead || held) plan.clive.push_back(static_cast<int32_t>(plan.berms.size()) + b);
+          }
+          plan.clive_rng.push_back(static_cast<int32_t>(plan.clive.size()));
         }
     for (int b = 0; b < nB; ++b) {
       const Trade& tr = berms[static_cast<std::size_t>(b)];
@@ -905,6 +918,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
   plan.stream_len = blocksPaths * maxStride;
   for (int s = 0; s < num_sets; ++s)
     plan.sets[static_cast<std::size_t>(s)].rows_off = static_cast<int64_t>(s) * plan.max_rep * P;
+  if (plan.clive.empty()) plan.clive.push_back(0);  // a valid table even when no swaption is ever live
   // One zero coupon record past the end: the path kernel reads coupons two at a time.
   plan.fixed.insert(plan.fixed.end(), 4, 0.0);
   assignBlocks(plan, block);

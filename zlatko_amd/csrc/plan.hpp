@@ -50,6 +50,8 @@ struct HostPlan {
   std::vector<double> ctimes;       // time of each continuation date
   std::vector<int32_t> bdate_berm;  // per BermDate record: its Bermudan
   std::vector<int32_t> bdate_cd;    // per BermDate record: its continuation date (-1: none)
+  std::vector<int32_t> clive;       // per continuation date: the swaptions with a value (global indices)
+  std::vector<int32_t> clive_rng;   // [begin, end) into clive per continuation date
   std::vector<GemmTile> tiles;      // continuation GEMM tiles
   std::vector<GemmTile> ntiles;     // narrow continuation GEMM tiles (sets with few Bermudans)
   std::vector<int32_t> gcols;       // strategy column per (tile, swaption), -1: none

@@ -74,7 +74,7 @@ struct AmcArgs {
   const int32_t* eidx;        // exercise date index per (Bermudan, exercise)
   const int32_t* exlist;      // Bermudan of each global exercise
   const int32_t* cdates;      // (set, date) per continuation date
-  const int32_t* clive;       // swaptions with a value, per continuation date ...
+  const LiveRec* clive;       // swaptions with a value, per continuation date ...
   const int32_t* clive_rng;   // ... [begin, end) pairs
   const int32_t* bdate_berm;  // Bermudan of each date record
   const int32_t* bdate_cd;    // continuation date of each date record (-1: none)

@@ -128,6 +128,14 @@ struct BermDev {
   int32_t ev_off;     // first of ne x P exercise values / strategy values (x P)
 };
 
+// One swaption that can be nonzero on a continuation date: its global index, its date record
+// (BermDate / continuation fit, date_rec + j), its exercise count and its position, so the
+// valuation reads them in one load instead of a chain through the swaption's record.
+struct LiveRec {
+  int32_t b, rec, ne, pad;
+  double position;
+};
+
 struct BermDate {
   int32_t next_k;     // first exercise strictly after t_j (ne: none left)
   int32_t has_swap;   // the underlying's value terms are present (physical settlement, exercisable)

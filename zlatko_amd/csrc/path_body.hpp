@@ -169,7 +169,7 @@ template <typename Real>
 __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, int64_t p, int64_t paths,
                                             cptr<BermDev> berms, cptr<BermDate> bdates, cptr<double> cont,
                                             const int16_t* exidx, const double* states, cptr<Real> bonds,
-                                            cptr<Real> fixed, cptr<double> poly, cptr<int32_t> live, int lbeg,
+                                            cptr<Real> fixed, cptr<double> poly, cptr<LiveRec> live, int lbeg,
                                             int lend) {
   const double* xrow = states + S.state_off;
   const double xd = xrow[static_cast<int64_t>(j) * paths + p];
@@ -180,14 +180,15 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
   // all paths): z is computed once and again only if a record holds other constants.
   double zCenter = 0.0, zScale = 0.0, z = 0.0;
   bool haveZ = false;
+  (void)berms;
+  (void)j;
   for (int l = lbeg; l < lend; ++l) {
-    const int b = live[l];
-    const CCR_CONST BermDev& bm = berms[b];
-    const CCR_CONST BermDate& bd = bdates[bm.date_rec + j];
-    const bool exercised = exidx[static_cast<int64_t>(b) * paths + p] < bd.next_k;
+    const LiveRec lr = live[l];
+    const CCR_CONST BermDate& bd = bdates[lr.rec];
+    const bool exercised = exidx[static_cast<int64_t>(lr.b) * paths + p] < bd.next_k;
     Real v = 0;
-    if (bd.next_k < bm.ne) {
-      cptr<double> c = cont + static_cast<int64_t>(bm.date_rec + j) * kContStride;
+    if (bd.next_k < lr.ne) {
+      cptr<double> c = cont + static_cast<int64_t>(lr.rec) * kContStride;
       const double coef[4] = {c[2], c[3], c[4], c[5]};
       if (!haveZ || c[0] != zCenter || c[1] != zScale) {
         zCenter = c[0];
@@ -204,7 +205,7 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
     } else if (exercised) {
       v = 0;
     }
-    V += static_cast<Real>(bm.position) * v;
+    V += static_cast<Real>(lr.position) * v;
   }
   return V;
 }

@@ -809,7 +809,15 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
                                static_cast<int>(berms[static_cast<std::size_t>(b)].exercise.size());
             const bool held = bHas[static_cast<std::size_t>(b)][static_cast<std::size_t>(j)] &&
                               (!c.bond.empty() || !c.fix.empty() || c.c0[0] != 0.0);
-            if (ahead || held) plan.clive.push_back(static_cast<int32_t>(plan.berms.size()) + b);
+            if (ahead || held) {
+              // The swaption's date records start after those of the set's earlier swaptions.
+              LiveRec lr{};
+              lr.b = static_cast<int32_t>(plan.berms.size()) + b;
+              lr.rec = static_cast<int32_t>(plan.bdates.size()) + b * T + j;
+              lr.ne = static_cast<int32_t>(berms[static_cast<std::size_t>(b)].exercise.size());
+              lr.position = berms[static_cast<std::size_t>(b)].position;
+              plan.clive.push_back(lr);
+            }
           }
           plan.clive_rng.push_back(static_cast<int32_t>(plan.clive.size()));
         }
@@ -918,7 +926,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
   plan.stream_len = blocksPaths * maxStride;
   for (int s = 0; s < num_sets; ++s)
     plan.sets[static_cast<std::size_t>(s)].rows_off = static_cast<int64_t>(s) * plan.max_rep * P;
-  if (plan.clive.empty()) plan.clive.push_back(0);  // a valid table even when no swaption is ever live
+  if (plan.clive.empty()) plan.clive.push_back(LiveRec{});  // a valid table even when no swaption is ever live
   // One zero coupon record past the end: the path kernel reads coupons two at a time.
   plan.fixed.insert(plan.fixed.end(), 4, 0.0);
   assignBlocks(plan, block);

@@ -50,7 +50,7 @@ struct HostPlan {
   std::vector<double> ctimes;       // time of each continuation date
   std::vector<int32_t> bdate_berm;  // per BermDate record: its Bermudan
   std::vector<int32_t> bdate_cd;    // per BermDate record: its continuation date (-1: none)
-  std::vector<int32_t> clive;       // per continuation date: the swaptions with a value (global indices)
+  std::vector<LiveRec> clive;       // per continuation date: the swaptions with a value
   std::vector<int32_t> clive_rng;   // [begin, end) into clive per continuation date
   std::vector<GemmTile> tiles;      // continuation GEMM tiles
   std::vector<GemmTile> ntiles;     // narrow continuation GEMM tiles (sets with few Bermudans)

@@ -1148,6 +1148,37 @@ int32_t ccr_oracle_run_book(const ccr_scenario_generator* generator, const ccr_s
   });
 }
 
+// initialMarginProfile of netting set `index` of a book (its values on the book's simulation grid).
+int32_t ccr_oracle_book_initial_margin(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                                       const double* base_grid, int32_t base_size, const ccr_netting_set* sets,
+                                       int32_t num_sets, int32_t index, double quantile_level, double mpor,
+                                       int32_t threads, ccr_exposure_out* out, double* margin, int32_t capacity,
+                                       int32_t* size, char* err, int32_t err_capacity) {
+  return guard(err, err_capacity, [&] {
+    if (!generator || !config || !sets || !out || !margin || index < 0 || index >= num_sets) invalid("bad argument");
+    const Generator g(*generator);
+    const Grid base(Vec(base_grid, base_grid + std::max(0, base_size)));
+    std::vector<std::vector<Trade>> trades(static_cast<std::size_t>(num_sets));
+    Vec r, v;
+    for (int i = 0; i < num_sets; ++i) {
+      for (int k = 0; k < sets[i].num_trades; ++k) trades[static_cast<std::size_t>(i)].push_back(makeTrade(sets[i].trades[k], g));
+      const Grid gr = reportingGrid(base, trades[static_cast<std::size_t>(i)]);
+      r.insert(r.end(), gr.t.begin(), gr.t.end());
+    }
+    const Grid rep(r);
+    for (int i = 0; i < num_sets; ++i) {
+      const Grid gs = simulationGrid(rep, trades[static_cast<std::size_t>(i)], sets[i]);
+      v.insert(v.end(), gs.t.begin(), gs.t.end());
+    }
+    const Grid sim(v);
+    Vec im;
+    runSet(g, *config, base, sets[index], 0.95, threads, out, &im, quantile_level, mpor, nullptr, &rep, &sim);
+    if (size) *size = static_cast<int32_t>(im.size());
+    if (capacity < static_cast<int32_t>(im.size())) invalid("capacity smaller than the simulation grid");
+    std::copy(im.begin(), im.end(), margin);
+  });
+}
+
 // Per-trade statistics of one netting set next to its exposure profile: stats[(i * 5 + m) *
 // capacity + r] for trade i, statistic m, reporting date r.
 int32_t ccr_oracle_trade_statistics(const ccr_scenario_generator* generator, const ccr_simulation_config* config,

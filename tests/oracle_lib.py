@@ -35,6 +35,9 @@ def oracle():
         lib.ccr_oracle_initial_margin.restype = I32
         lib.ccr_oracle_trade_statistics.restype = I32
         lib.ccr_oracle_run_book.restype = I32
+        lib.ccr_oracle_book_initial_margin.restype = I32
+        lib.ccr_oracle_book_initial_margin.argtypes = [P, P, P, I32, P, I32, I32, D, D, I32, P, P, I32, P,
+                                                       C.c_char_p, I32]
         lib.ccr_oracle_run_book.argtypes = [P, P, P, I32, P, I32, D, I32, P, C.c_char_p, I32]
         lib.ccr_oracle_trade_statistics.argtypes = [P, P, P, I32, P, I32, P, P, I32, C.c_char_p, I32]
         lib.ccr_oracle_initial_margin.argtypes = [P, P, P, I32, P, D, D, I32, P, P, I32, P, C.c_char_p, I32]
@@ -253,6 +256,21 @@ def run_oracle_book(spec: abi.Spec, threads: int = 4, rows: bool = False):
         raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
     abi.sync_outs(outs, arr)
     return outs
+
+
+def oracle_book_initial_margin(spec: abi.Spec, i: int, quantile: float, mpor: float = 10.0 / 250.0) -> np.ndarray:
+    cap = book_grid_size(spec) * 2 + 16
+    outs, arr = abi.make_outs(1, cap, spec.num_paths, False)
+    margin = np.zeros(cap)
+    n = C.c_int32()
+    err = C.create_string_buffer(512)
+    rc = oracle().ccr_oracle_book_initial_margin(C.byref(spec.generator), C.byref(spec.config),
+                                                 abi.dptr(spec.base_grid), len(spec.base_grid), spec.sets_c,
+                                                 len(spec.sets), i, quantile, mpor, 4, arr, abi.dptr(margin), cap,
+                                                 C.byref(n), err, 512)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
+    return margin[:n.value]
 
 
 def oracle_timing() -> OracleTiming:

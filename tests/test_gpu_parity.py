@@ -506,3 +506,24 @@ def test_book_matches_oracle_and_totals_add_up_fp64():
     assert single.info().max_reporting_dates < outs[0].c.num_dates  # own grids again
     single.close()
     e.close()
+
+
+def test_book_initial_margin_matches_oracle_fp64():
+    """The IM proxy of every netting set of a book (each valued on every date of the book's
+    simulation grid) against the oracle's book restatement."""
+    from zlatko_amd import native
+    spec = W.book_case(num_paths=900)
+    e = native.Engine()
+    e.set_book(True)
+    e.set_initial_margin(0.99, 10.0 / 250.0)
+    plan = e.compile(spec)
+    plan.execute()
+    for s in range(len(spec.sets)):
+        got = plan.initial_margin(s)
+        want = oracle_lib.oracle_book_initial_margin(spec, s, 0.99)
+        assert len(got) == len(want) and got[0] == 0.0
+        err = np.abs(got - want).max() / max(np.abs(want).max(), 1.0)
+        print("book IM", s, f"{err:.2e}", flush=True)
+        assert err <= FP64_TOL
+    plan.close()
+    e.close()

@@ -438,6 +438,25 @@ def test_oracle_book_matches_reference_engine_on_the_union_grid():
             assert np.array_equal(r.get(f)[idx], outs[i].get(f)), f
 
 
+@pytest.mark.skipif(not _ref_has("ccr_ref_initial_margin"), reason="reference build (oracle/_ref) not present")
+def test_oracle_book_initial_margin_matches_reference_on_the_union_grid():
+    """The IM proxy of an uncollateralised book set equals the reference's initialMarginProfile
+    of its engine run with the book's simulation grid as the base grid, bit for bit."""
+    spec = W.book_case(num_paths=200)
+    rep = oracle_lib.run_oracle_book(spec, threads=3)[0].get("times")
+    sim = np.ascontiguousarray(np.unique(np.concatenate(
+        [_sim_grid(np.ascontiguousarray(rep), spec.sets_c[i]) for i in range(3)])))
+    for i in (1, 2):
+        want = np.zeros(len(sim) + 8)
+        n = C.c_int32()
+        err = C.create_string_buffer(256)
+        assert oracle_lib.ref().ccr_ref_initial_margin(C.byref(spec.generator), C.byref(spec.config), abi.dptr(sim),
+                                                      len(sim), C.byref(spec.sets_c[i]), 0.99, 10.0 / 250.0,
+                                                      abi.dptr(want), len(want), C.byref(n), err, 256) == 0, err.value
+        got = oracle_lib.oracle_book_initial_margin(spec, i, 0.99)
+        assert n.value == len(got) and np.array_equal(got, want[:n.value])
+
+
 # ------------------------------------------------------------ invariants (oracle-only extensions included)
 
 def test_threads_do_not_change_bits():

@@ -333,7 +333,8 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   auto plan = std::make_unique<ccr_hip_plan>();
   plan->device = ctx->device;
   plan->host = buildPlan(*generator, *config, base_grid, base_size, sets, num_sets, pfe_quantile, precision,
-                         ctx->block > 0 ? ctx->block : kMaxBlock, ctx->per_trade, ctx->book);
+                         ctx->block > 0 ? ctx->block : kMaxBlock, ctx->per_trade, ctx->book,
+                         ctx->im_quantile > 0.0);
   HostPlan& h = plan->host;
   if (ctx->block == 0) assignBlocks(h, occupancyBlock(h));
   const bool fp32 = precision == CCR_FP32;
@@ -354,8 +355,8 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   execAlloc(ctx, plan.get(), plan->d_rows, static_cast<size_t>(num_sets) * h.max_rep * P * elem);
   execAlloc(ctx, plan.get(), plan->d_partials, static_cast<size_t>(h.total_waves) * h.max_rep * kStats * sizeof(double));
   if (ctx->im_quantile > 0.0) {
-    // The proxy needs the netted value on every simulation date (true of every plan: without a
-    // CSA the simulation grid is the reporting grid, with one every date is valued).
+    // The proxy needs the netted value on every simulation date (the plan values every date
+    // when it is on: book plans simulate dates of other sets that their own valuation skips).
     for (const DateRec& d : h.dates)
       if (!(d.flags & kValued)) invalid("initial margin: a simulation date is not valued");
     plan->im_quantile = ctx->im_quantile;

@@ -346,7 +346,7 @@ void generateStream(uint64_t seed, int64_t n, double* out) {
 
 HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
-                   int32_t precision, int32_t block, bool per_trade, bool book) {
+                   int32_t precision, int32_t block, bool per_trade, bool book, bool value_all) {
   if (!sets || num_sets <= 0) invalid("no netting sets");
   if (cfg.num_paths <= 0) invalid("ScenarioGenerator::generate: numPaths must be positive");
   if (cfg.num_paths > (int64_t(1) << 31) - 1) fail(CCR_ERR_UNSUPPORTED, "at most 2^31 - 1 paths");
@@ -483,7 +483,8 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     std::vector<char> valued(static_cast<std::size_t>(T), 0);
     std::map<std::pair<int, int>, int> lastUse;  // (rate, fixing date) -> last date using it
     for (int j = 0; j < T; ++j) {
-      valued[static_cast<std::size_t>(j)] = repOf[static_cast<std::size_t>(j)] >= 0 || isCall[static_cast<std::size_t>(j)];
+      valued[static_cast<std::size_t>(j)] =
+          value_all || repOf[static_cast<std::size_t>(j)] >= 0 || isCall[static_cast<std::size_t>(j)];
       if (!valued[static_cast<std::size_t>(j)]) continue;
       Collector& col = cols[static_cast<std::size_t>(j)];
       col.t = sim[static_cast<std::size_t>(j)];

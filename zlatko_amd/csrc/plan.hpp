@@ -68,7 +68,8 @@ TimeGrid simulationGrid(const TimeGrid& base, const ccr_netting_set& set);
 
 HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
-                   int32_t precision, int32_t block, bool per_trade = false, bool book = false);
+                   int32_t precision, int32_t block, bool per_trade = false, bool book = false,
+                   bool value_all = false);
 
 // Workgroup size of the path kernel: the workgroups of every set, their waves and the offsets of
 // the per-wave partial sums.

@@ -483,9 +483,12 @@ __global__ void __launch_bounds__(256) amc_features_kernel(const double* states,
   }
 }
 
-__global__ void amc_scatter_kernel(const float* y, int64_t n, double mean, double sd, double* out) {
+// Network outputs of the first n_live values, exact zeros for the rest (dates on which no
+// swaption of the set can be nonzero).
+__global__ void amc_scatter_kernel(const float* y, int64_t n_live, int64_t n, double mean, double sd, double* out) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = static_cast<double>(y[i]) * sd + mean;
+  if (i < n_live) out[i] = static_cast<double>(y[i]) * sd + mean;
+  else if (i < n) out[i] = 0.0;
 }
 
 hipError_t last() { return hipGetLastError(); }
@@ -501,9 +504,11 @@ hipError_t launchSurrogateFeatures(const double* states, int64_t state_off, int6
   return last();
 }
 
-hipError_t launchSurrogateScatter(const float* y, int64_t n, double mean, double sd, double* out, hipStream_t stream) {
-  hipLaunchKernelGGL(amc_scatter_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, y, n, mean,
-                     sd, out);
+hipError_t launchSurrogateScatter(const float* y, int64_t n_live, int64_t n, double mean, double sd, double* out,
+                                  hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(amc_scatter_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, y, n_live,
+                     n, mean, sd, out);
   return last();
 }
 

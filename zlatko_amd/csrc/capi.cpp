@@ -506,6 +506,18 @@ int setRows(const HostPlan& h, int s) {
   return rows;
 }
 
+// Rows (continuation dates, in order) of set s up to its last one on which some swaption can be
+// nonzero; past it every value is an exact 0.
+int liveRows(const HostPlan& h, int s) {
+  int row = 0, live = 0;
+  for (size_t i = 0; i < h.cdates.size() / 2; ++i) {
+    if (h.cdates[2 * i] != s) continue;
+    ++row;
+    if (h.clive_rng[2 * i + 1] > h.clive_rng[2 * i]) live = row;
+  }
+  return live;
+}
+
 // The distinct exercise dates of set s's Bermudans (grid indices, ascending, at most
 // kSurrogateMaxExercises).
 SurrogateExercises exercisesOf(const HostPlan& h, int s) {
@@ -563,11 +575,14 @@ void surrogateRows(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate
     check(launchSurrogateFeatures(plan->d_states, S.state_off, P, plan->d_dates + S.date_off, plan->d_ctimes, S.num_dates,
                                   S.cd_base, ex, plan->d_feats, ctx->stream),
           "surrogate features launch");
-    check(mlpInfer(net.run->device(), plan->d_feats, static_cast<int64_t>(rows) * P, net.xmean, net.xsd, plan->d_ysur,
-                   ctx->stream),
-          "surrogate inference launch");
-    check(launchSurrogateScatter(plan->d_ysur, static_cast<int64_t>(rows) * P, net.ymean, net.ysd,
-                                 plan->d_bermv + static_cast<int64_t>(S.cd_base) * P, ctx->stream),
+    // The network runs on the rows up to the set's last date with a live swaption; the rest are 0.
+    const int live = liveRows(h, net.set);
+    if (live > 0)
+      check(mlpInfer(net.run->device(), plan->d_feats, static_cast<int64_t>(live) * P, net.xmean, net.xsd,
+                     plan->d_ysur, ctx->stream),
+            "surrogate inference launch");
+    check(launchSurrogateScatter(plan->d_ysur, static_cast<int64_t>(live) * P, static_cast<int64_t>(rows) * P,
+                                 net.ymean, net.ysd, plan->d_bermv + static_cast<int64_t>(S.cd_base) * P, ctx->stream),
           "surrogate scatter launch");
   }
 }

@@ -117,7 +117,8 @@ hipError_t launchSurrogateFeatures(const double* states, int64_t state_off, int6
                                    const double* ctimes, int num_dates, int cd_base, const SurrogateExercises& ex,
                                    float* feats, hipStream_t stream);
 // bermv rows [cd_base, cd_base + rows) x paths = y * sd + mean.
-hipError_t launchSurrogateScatter(const float* y, int64_t n, double mean, double sd, double* out, hipStream_t stream);
+hipError_t launchSurrogateScatter(const float* y, int64_t n_live, int64_t n, double mean, double sd, double* out,
+                                  hipStream_t stream);
 
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);

@@ -125,7 +125,7 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
     const int j = h.cdates[2 * static_cast<size_t>(d) + 1];
     for (int p = 0; p < P; ++p)
       bermv[static_cast<size_t>(d) * P + p] =
-          bermudanSum<double>(S, j, p, P, h.berms.data(), h.bdates.data(), cont.data(), exidx.data(), states.data(),
+          bermudanSum<double>(S, j, p, P, h.bdates.data(), cont.data(), exidx.data(), states.data(),
                               a.bonds + 2 * static_cast<int64_t>(S.bond_off), a.fixed + 4 * static_cast<int64_t>(S.fixed_off),
                               a.poly + S.poly_off, h.clive.data(), h.clive_rng[2 * static_cast<size_t>(d)],
                               h.clive_rng[2 * static_cast<size_t>(d) + 1]);

@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(kAmcChunk) amc_value_kernel(AmcArgs a, const R
   const int j = toConst(a.cdates)[2 * d + 1];
   const int64_t p0 = static_cast<int64_t>(chunk) * kAmcChunk + threadIdx.x;
   const int64_t p = p0 < a.paths ? p0 : a.paths - 1;
-  const Real v = bermudanSum<Real>(S, j, p, a.paths, toConst(a.berms), toConst(a.bdates), toConst(a.cont), a.exidx,
+  const Real v = bermudanSum<Real>(S, j, p, a.paths, toConst(a.bdates), toConst(a.cont), a.exidx,
                                    a.states, toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off),
                                    toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(a.poly) + S.poly_off,
                                    toConst(a.clive), toConst(a.clive_rng)[2 * d], toConst(a.clive_rng)[2 * d + 1]);

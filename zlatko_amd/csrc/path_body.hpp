@@ -167,7 +167,7 @@ __host__ __device__ inline Real evalRange(const CCR_CONST TermRange& tr, Real x,
 // be nonzero on some path (the others add an exact 0).
 template <typename Real>
 __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, int64_t p, int64_t paths,
-                                            cptr<BermDev> berms, cptr<BermDate> bdates, cptr<double> cont,
+                                            cptr<BermDate> bdates, cptr<double> cont,
                                             const int16_t* exidx, const double* states, cptr<Real> bonds,
                                             cptr<Real> fixed, cptr<double> poly, cptr<LiveRec> live, int lbeg,
                                             int lend) {
@@ -180,8 +180,6 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
   // all paths): z is computed once and again only if a record holds other constants.
   double zCenter = 0.0, zScale = 0.0, z = 0.0;
   bool haveZ = false;
-  (void)berms;
-  (void)j;
   for (int l = lbeg; l < lend; ++l) {
     const LiveRec lr = live[l];
     const CCR_CONST BermDate& bd = bdates[lr.rec];

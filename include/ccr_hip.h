@@ -391,6 +391,14 @@ int32_t ccr_hip_set_initial_margin(ccr_hip_ctx* ctx, double quantile, double mar
 int32_t ccr_hip_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set, double* margin,
                                int32_t capacity, int32_t* size);
 
+/* Graph mode (no counterpart in the reference): while enabled, ccr_hip_execute and
+ * ccr_hip_execute_async replay the plan's whole execute -- states pass, regressions, path kernel,
+ * reductions, selects -- as one HIP graph, captured on the plan's first execute in this mode (and
+ * again if its shared workspace has moved). Results are those of the eager launches; surrogate
+ * executes stay eager. ccr_hip_last_timing then reports the whole replay in total_ms (the stage
+ * fields are 0). */
+int32_t ccr_hip_set_graph(ccr_hip_ctx* ctx, int32_t enable);
+
 /* Portfolio book (src/app/analyses.cpp:1604-1709, 1727-1766): plans compiled after
  * ccr_hip_set_book(ctx, 1) simulate every netting set on the union of the sets' simulation grids
  * and report on the union of their reporting grids, from one seed, so rates, FX, intensities and

@@ -527,3 +527,41 @@ def test_book_initial_margin_matches_oracle_fp64():
         assert err <= FP64_TOL
     plan.close()
     e.close()
+
+
+def test_graph_mode_replays_match_eager_executes():
+    """Graph mode: each plan's execute captured once as a HIP graph and replayed. Three replays of
+    each plan (a Bermudan/AMC plan, an FP32 CSA plan, per-trade and IM outputs), with eager
+    executes of the other plans and synchronisations between them, give the eager profiles bit
+    for bit; a plan whose shared workspace grew is captured again."""
+    from zlatko_amd import native
+    specs = [W.small_reference_case(num_paths=600, csa=True, fx=True, credits=True, bermudans=3),
+             W.c2(num_paths=700, num_sets=3, num_trades=30, weeks=52, horizon=1.0)]
+    eager = native.Engine()
+    eager.set_initial_margin(0.99)
+    want = [eager.compile(specs[0]).execute(), eager.compile(specs[1], abi.CCR_FP32).execute()]
+    e = native.Engine()
+    e.set_initial_margin(0.99)
+    e.set_shared_workspace(True)
+    plans = [e.compile(specs[0]), e.compile(specs[1], abi.CCR_FP32)]
+    e.set_graph(True)
+    for rep in range(3):
+        for i, p in enumerate(plans):
+            got = p.execute()
+            for o, w in zip(got, want[i]):
+                for k in abi.PROFILE_FIELDS:
+                    assert np.array_equal(o.get(k), w.get(k)), (rep, i, k)
+            assert p.initial_margin(0)[0] == 0.0
+        e.set_graph(False)
+        plans[rep % 2].execute()  # eager work between replays
+        e.synchronize()
+        e.set_graph(True)
+    bigger = e.compile(W.small_reference_case(num_paths=2000, csa=True, bermudans=2))
+    bigger.execute()  # grows the shared workspace: the next replay recaptures
+    for o, w in zip(plans[0].execute(), want[0]):
+        for k in abi.PROFILE_FIELDS:
+            assert np.array_equal(o.get(k), w.get(k)), k
+    for p in plans + [bigger]:
+        p.close()
+    e.close()
+    eager.close()

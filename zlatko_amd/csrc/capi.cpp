@@ -61,6 +61,11 @@ struct ccr_hip_ctx {
   bool per_trade = false;
   // Book mode (common grids, pathwise identical scenarios) for plans compiled while set.
   bool book = false;
+  // Executes replay one captured HIP graph per plan while set.
+  bool use_graph = false;
+  // Launches are being captured into a graph (no timing events recorded), and whether the last
+  // launch recorded the per-stage events (graph replays record the total only).
+  bool capturing = false, stage_events = true;
 };
 
 namespace {
@@ -127,7 +132,11 @@ struct ccr_hip_plan {
   double* d_tstats = nullptr;
   std::vector<double> tstats;
   bool tstats_ready = false;
+  // The plan's execute captured as a HIP graph (graph mode), with the workspace it was bound to.
+  hipGraphExec_t graph_exec = nullptr;
+  void* graph_ws = nullptr;
   ~ccr_hip_plan() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     for (auto& b : bufs)
       if (b.p) (void)hipFree(b.p);
   }
@@ -318,6 +327,12 @@ int occupancyBlock(const HostPlan& h) {
 
 template <typename Real>
 PathArgs<Real> pathArgs(const ccr_hip_plan* plan);
+
+// Timing event i of an execute (0 start, 1 after the AMC stage, 2 after the path kernel, 3 after
+// the reductions, 4 end); not recorded while the launches are captured into a graph.
+void markEvent(ccr_hip_ctx* ctx, int i) {
+  if (!ctx->capturing) check(hipEventRecord(ctx->ev[i], ctx->stream), "hipEventRecord");
+}
 
 void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
                  const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
@@ -608,9 +623,9 @@ void launchPathStages(ccr_hip_ctx* ctx, ccr_hip_plan* plan, int blocks, const cc
       check(launchAmc<Real>(plan->amc, a.bonds, a.fixed, h.max_exercises, ctx->stream), "regression kernel launch");
     if (sur) surrogateRows(ctx, plan, sur);
   }
-  check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
+  markEvent(ctx, 1);
   check(launchPath<Real>(a, blocks, h.block, ctx->stream), "path kernel launch");
-  check(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
+  markEvent(ctx, 2);
 }
 
 void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* sur = nullptr) {
@@ -623,7 +638,7 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* su
   const HostPlan& h = plan->host;
   const int nsets = static_cast<int>(h.sets.size());
   const int blocks = static_cast<int>(h.block_set.size());
-  check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
+  markEvent(ctx, 0);
   if (h.precision == CCR_FP32) launchPathStages<float>(ctx, plan, blocks, sur);
   else launchPathStages<double>(ctx, plan, blocks, sur);
   check(launchReduce(plan->d_sets, plan->d_partials, nsets, h.max_rep, h.total_waves / nsets,
@@ -635,7 +650,7 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* su
           "per-trade reduce kernel launch");
     plan->tstats_ready = false;
   }
-  check(hipEventRecord(ctx->ev[3], ctx->stream), "hipEventRecord");
+  markEvent(ctx, 3);
   if (h.precision == CCR_FP32)
     check(launchPfe<float>(plan->d_sets, static_cast<const float*>(plan->d_rows), nsets, h.max_rep, h.g.num_paths,
                            h.pfe_q, plan->d_pfe, ctx->stream),
@@ -661,10 +676,56 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* su
           "initial-margin reduce launch");
     plan->im_ready = false;
   }
-  check(hipEventRecord(ctx->ev[4], ctx->stream), "hipEventRecord");
-  ctx->timing_pending = true;
+  markEvent(ctx, 4);
+  ctx->timing_pending = !ctx->capturing;
+  ctx->stage_events = true;
   ctx->upload_ms = plan->upload_ms;
   ctx->draws_ms = plan->draws_ms;
+}
+
+// Graph mode: the plan's whole execute (every launch of launchAll and its timing events) captured
+// once into a HIP graph and replayed; captured again when the shared workspace it was bound to
+// has moved. The workspace is bound (and grown) before the capture, outside it.
+void launchGraph(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
+  check(hipSetDevice(ctx->device), "hipSetDevice");
+  bindWorkspace(ctx, plan);
+  if (!plan->graph_exec || plan->graph_ws != ctx->ws) {
+    if (plan->graph_exec) {
+      check(hipGraphExecDestroy(plan->graph_exec), "hipGraphExecDestroy");
+      plan->graph_exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    ctx->capturing = true;
+    try {
+      launchAll(ctx, plan);
+    } catch (...) {
+      ctx->capturing = false;
+      (void)hipStreamEndCapture(ctx->stream, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    ctx->capturing = false;
+    check(hipStreamEndCapture(ctx->stream, &graph), "hipStreamEndCapture");
+    const hipError_t e = hipGraphInstantiate(&plan->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    check(e, "hipGraphInstantiate");
+    plan->graph_ws = ctx->ws;
+  }
+  markEvent(ctx, 0);
+  check(hipGraphLaunch(plan->graph_exec, ctx->stream), "hipGraphLaunch");
+  markEvent(ctx, 4);
+  if (plan->d_im_q) plan->im_ready = false;
+  if (plan->tstride > 0) plan->tstats_ready = false;
+  ctx->timing_pending = true;
+  ctx->stage_events = false;
+  ctx->upload_ms = plan->upload_ms;
+  ctx->draws_ms = plan->draws_ms;
+}
+
+void launchExecute(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
+  if (ctx->use_graph && ctx->fault == CCR_FAULT_NONE) launchGraph(ctx, plan);
+  else launchAll(ctx, plan);
 }
 
 // Reads the event timings of the last launch (waits for it to finish).
@@ -673,10 +734,12 @@ void readTimings(ccr_hip_ctx* ctx) {
   ctx->timing_pending = false;
   check(hipEventSynchronize(ctx->ev[4]), "hipEventSynchronize");
   float m = 0, a = 0, b = 0, c = 0, t = 0;
-  check(hipEventElapsedTime(&m, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
-  check(hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
-  check(hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
-  check(hipEventElapsedTime(&c, ctx->ev[3], ctx->ev[4]), "hipEventElapsedTime");
+  if (ctx->stage_events) {
+    check(hipEventElapsedTime(&m, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
+    check(hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
+    check(hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]), "hipEventElapsedTime");
+    check(hipEventElapsedTime(&c, ctx->ev[3], ctx->ev[4]), "hipEventElapsedTime");
+  }
   check(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[4]), "hipEventElapsedTime");
   ctx->timing.amc_ms = m;
   ctx->timing.path_ms = a;
@@ -868,6 +931,12 @@ int32_t ccr_hip_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32
   });
 }
 
+int32_t ccr_hip_set_graph(ccr_hip_ctx* ctx, int32_t enable) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  ctx->use_graph = enable != 0;
+  return CCR_OK;
+}
+
 int32_t ccr_hip_set_book(ccr_hip_ctx* ctx, int32_t enable) {
   if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
   ctx->book = enable != 0;
@@ -1026,7 +1095,7 @@ int32_t ccr_hip_execute_async(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
   if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
   return guarded(ctx, [&] {
     if (!plan) invalid("null plan");
-    launchAll(ctx, plan);
+    launchExecute(ctx, plan);
   });
 }
 
@@ -1042,7 +1111,7 @@ int32_t ccr_hip_execute(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* 
   if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
   return guarded(ctx, [&] {
     if (!plan || !outs) invalid("null argument");
-    launchAll(ctx, plan);
+    launchExecute(ctx, plan);
     readBack(ctx, plan, outs);
   });
 }

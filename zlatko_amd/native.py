@@ -47,6 +47,7 @@ def lib():
         L.ccr_hip_initial_margin.argtypes = [P, P, I32, P, I32, P]
         L.ccr_hip_set_trade_statistics.argtypes = [P, I32]
         L.ccr_hip_set_book.argtypes = [P, I32]
+        L.ccr_hip_set_graph.argtypes = [P, I32]
         L.ccr_book_totals.argtypes = [P, P, I32, I32, P]
         L.ccr_allocate_exposure_xva.argtypes = [P, P, I32, I32, P, D, P, P, P]
         L.ccr_hip_trade_statistics.argtypes = [P, P, I32, I32, P]
@@ -251,6 +252,10 @@ class Engine:
         """Plans compiled from now on keep their per-execute buffers in one workspace of the
         context (valid until the next execute of any plan), so waves of netting sets fit in HBM."""
         self._check(lib().ccr_hip_set_shared_workspace(self.ctx, 1 if enable else 0))
+
+    def set_graph(self, enable: bool):
+        """Executes replay one captured HIP graph per plan from now on."""
+        self._check(lib().ccr_hip_set_graph(self.ctx, 1 if enable else 0))
 
     def set_book(self, enable: bool):
         """Plans compiled from now on put every netting set on the book's common grids (pathwise

@@ -30,6 +30,11 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 __device__ inline float actF(int kind, float z) {
   return kind == CCR_ACT_TANH ? tanhf(z) : z / (1.0f + __expf(-z));
 }
+// The activation for the fused BF16 inference, whose outputs are rounded to BF16 (8-bit
+// mantissa): SiLU with the hardware reciprocal (1 ulp) instead of an IEEE division.
+__device__ inline float actInfer(int kind, float z) {
+  return kind == CCR_ACT_TANH ? tanhf(z) : z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 // Derivative of the activation at pre-activation z.
 __device__ inline float actPrimeF(int kind, float z) {
   if (kind == CCR_ACT_TANH) {
@@ -364,7 +369,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) bf16_infer_kernel(InferArgs 
           // This wavefront's tile rows 4 g + i, column o (all lanes have read the previous
           // layer's rows into afr before any lane writes: wavefront-synchronous).
 #pragma unroll
-          for (int i = 0; i < 4; ++i) act[(4 * g + i) * kActStride + o] = static_cast<__bf16>(actF(a.act, acc[i] + bo));
+          for (int i = 0; i < 4; ++i) act[(4 * g + i) * kActStride + o] = static_cast<__bf16>(actInfer(a.act, acc[i] + bo));
         }
       }
     }

@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(64) bf16_wgrad_kernel(const __bf16* dfm, const
 // in LDS, each layer's output turned into the next layer's A operand through a per-wavefront LDS
 // tile. Inputs x [n][inputs] (raw) are standardised on load with (mean, sd); writes the
 // standardised outputs y [n][outputs]. Grid-stride over tiles.
-constexpr int kInferWaves = 8;
+constexpr int kInferWaves = 16;
 constexpr int kActStride = 128 + 8;  // BF16 elements per activation row in LDS (padding vs bank conflicts)
 struct InferArgs {
   const __bf16* w[kMlpMaxLayers];

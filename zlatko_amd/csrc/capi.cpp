@@ -709,6 +709,7 @@ void launchGraph(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
     check(hipStreamEndCapture(ctx->stream, &graph), "hipStreamEndCapture");
     const hipError_t e = hipGraphInstantiate(&plan->graph_exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) plan->graph_exec = nullptr;
     check(e, "hipGraphInstantiate");
     plan->graph_ws = ctx->ws;
   }

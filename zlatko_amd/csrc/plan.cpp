@@ -451,6 +451,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     }
     std::vector<int> call(static_cast<std::size_t>(T), 0);
     std::vector<char> isCall(static_cast<std::size_t>(T), 0);
+    std::vector<int> ringPos(static_cast<std::size_t>(T), 0);  // margin ring slot of each call date
     int ring = 1;
     if (ns.has_collateral) {
       const ccr_collateral& c = ns.collateral;
@@ -465,8 +466,29 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
         call[static_cast<std::size_t>(j)] =
             static_cast<int>(sim.indexAtOrBefore(std::max(sim[static_cast<std::size_t>(j)] - c.margin_period_of_risk, 0.0)));
         isCall[static_cast<std::size_t>(call[static_cast<std::size_t>(j)])] = 1;
-        ring = std::max(ring, j - call[static_cast<std::size_t>(j)] + 1);
       }
+      // Ring slots by interval colouring: call date c holds its value from c to its last reader
+      // (the last j with call[j] = c); a slot is reused by a later call date once its interval
+      // has ended (strictly before: a date stores its own value before it reads its call value).
+      std::vector<int> lastRead(static_cast<std::size_t>(T), -1);
+      for (int j = 0; j < T; ++j) lastRead[static_cast<std::size_t>(call[static_cast<std::size_t>(j)])] = j;
+      std::vector<int> freeAfter;  // last reader of the interval currently in each slot
+      for (int cd = 0; cd < T; ++cd) {
+        if (!isCall[static_cast<std::size_t>(cd)]) continue;
+        int slot = -1;
+        for (std::size_t q = 0; q < freeAfter.size(); ++q)
+          if (freeAfter[q] < cd) {
+            slot = static_cast<int>(q);
+            break;
+          }
+        if (slot < 0) {
+          slot = static_cast<int>(freeAfter.size());
+          freeAfter.push_back(0);
+        }
+        freeAfter[static_cast<std::size_t>(slot)] = lastRead[static_cast<std::size_t>(cd)];
+        ringPos[static_cast<std::size_t>(cd)] = slot;
+      }
+      ring = std::max(1, static_cast<int>(freeAfter.size()));
       sd.has_csa = 1;
       sd.sec_type = c.type;
       sd.csa_asset = c.asset_index;
@@ -654,8 +676,8 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       rec.rep = repOf[static_cast<std::size_t>(j)];
       rec.flags = (valued[static_cast<std::size_t>(j)] ? kValued : 0) | (rec.rep >= 0 ? kReporting : 0) |
                   (isCall[static_cast<std::size_t>(j)] ? kStoreValue : 0);
-      rec.call_pos = call[static_cast<std::size_t>(j)] % ring;
-      rec.store_pos = isCall[static_cast<std::size_t>(j)] ? j % ring : -1;
+      rec.call_pos = ringPos[static_cast<std::size_t>(call[static_cast<std::size_t>(j)])];
+      rec.store_pos = isCall[static_cast<std::size_t>(j)] ? ringPos[static_cast<std::size_t>(j)] : -1;
       double reals[kDateRealStride] = {0};
       const int bondBase = static_cast<int>(plan.bonds.size() / 2), fixBase = static_cast<int>(plan.fixed.size() / 4);
       if (per_trade) {

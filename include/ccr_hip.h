@@ -5,7 +5,7 @@
  * correlated risk-factor simulation, per-scenario trade revaluation (including
  * American Monte Carlo), CSA netting and collateral, and the fused EE/PFE/EPE/CVA/DVA
  * reductions. Nothing C++ crosses this boundary: plain structs, pointers, sizes and
- * int status codes. The C++17 `ccr::` mirror in zlatko_amd/host/ccr_hip.hpp builds these
+ * int status codes. The C++17 `ccr::` mirror in include/ccr_hip.hpp builds these
  * structs from model and trade objects and rethrows the status codes as the exception
  * types the reference uses.
  *
@@ -213,8 +213,10 @@ typedef struct ccr_exposure_out {
 
 enum ccr_precision {
   CCR_FP64 = 0,  /* FP64 everywhere: the parity mode (1e-10 relative to the FP64 oracle) */
-  CCR_FP32 = 1,  /* FP32 path evolution and revaluation, compensated sums for discount
-                    integrals and exposure means; FP64 final statistics */
+  CCR_FP32 = 1,  /* FP32 path evolution (Kahan-compensated integrals of the short rate and the
+                    log-spots), FP32 exponential terms, fixed coupons and collateral; the merged-bond
+                    power series (Horner), the American Monte Carlo regressions and the
+                    statistics' wavefront sums stay FP64 */
   CCR_BF16 = 2   /* MLP surrogate only: BF16 matrix-core products, FP32 accumulation */
 };
 

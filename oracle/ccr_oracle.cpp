@@ -5,7 +5,8 @@
 // A restatement of the reference CPU engine's semantics (paths relative to the reference root):
 // every function names the file:line it follows. Floating-point expressions keep the reference's
 // operation order so that, on reference-representable inputs, results agree with a build of the
-// reference bit for bit or to a few ulps (checked in tests/test_oracle_ref.py).
+// reference bit for bit or to a few ulps (checked in tests/test_oracle.py against oracle/_ref,
+// the reference's own sources compiled by oracle/Makefile).
 //
 // Extensions beyond the reference model (foreign Hull-White rates, two-leg swaps, pathwise DVA)
 // are the oracle's own and are pinned by reductions (sigma_f = 0, equal schedules) and invariants.
@@ -418,6 +419,11 @@ struct Cir {
 };
 
 // ---------------------------------------------------------------- scenario generator
+// Replayed normals (tests only): when set, ScenarioGenerator::generate consumes these in its
+// draw order instead of the mt19937_64 stream.
+thread_local const double* gReplay = nullptr;
+thread_local std::size_t gReplayLen = 0;
+
 // ScenarioGenerator: src/models/scenario_generator.cpp:22-143 (plus foreign rates).
 struct Scenarios {
   Grid grid;
@@ -460,7 +466,15 @@ struct Generator {
     for (std::size_t f = 0; f < foreign.size(); ++f)
       if (fxOf[f] >= 0) {
         const std::size_t a = static_cast<std::size_t>(fxOf[f]);
+        // Under the domestic measure the foreign state drifts by -rho(x_f, S_a) sigma_f sigma_S,
+        // S_a the price of one unit of currency f in domestic units. The mutant build (test
+        // infrastructure: a planted error the multi-currency martingale tests must catch)
+        // flips the sign.
+#ifdef CCR_ORACLE_MUTANT_QUANTO_SIGN
+        quanto[f] = corr(1 + f, 1 + foreign.size() + a) * foreign[f].s * assets[a].vol;
+#else
         quanto[f] = -corr(1 + f, 1 + foreign.size() + a) * foreign[f].s * assets[a].vol;
+#endif
       }
   }
   std::size_t numFactors() const { return 1 + foreign.size() + assets.size() + credits.size(); }
@@ -495,6 +509,14 @@ struct Generator {
 
     std::mt19937_64 engine(seed);
     std::normal_distribution<double> normal(0.0, 1.0);
+    // Draws come from the seeded stream, or from the replayed normals of this thread when a test
+    // has installed some (each generate() starts at their beginning, as each one re-seeds).
+    std::size_t replayPos = 0;
+    auto draw = [&]() -> double {
+      if (!gReplay) return normal(engine);
+      if (replayPos >= gReplayLen) invalid("replayed normals exhausted");
+      return gReplay[replayPos++];
+    };
     const std::size_t steps = nT > 0 ? nT - 1 : 0;
     Vec z(steps * nz), w(nCorr), spot(nA), xf(nF), y(nC), iy(nC), irf(nF);
     double drawS = 0.0;
@@ -504,7 +526,7 @@ struct Generator {
         for (double& v : z) v = -v;
       } else {
         const auto t0 = std::chrono::steady_clock::now();
-        for (double& v : z) v = normal(engine);
+        for (double& v : z) v = draw();
         drawS += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       }
       double x = 0.0, defl = 1.0;
@@ -1116,6 +1138,62 @@ int32_t ccr_oracle_run_exposure(const ccr_scenario_generator* generator, const c
     const Grid base(Vec(base_grid, base_grid + std::max(0, base_size)));
     for (int i = 0; i < num_sets; ++i) runSet(g, *config, base, sets[i], pfe_quantile, threads, &outs[i]);
     gTiming.total_s = secondsSince(t0);
+  });
+}
+
+// ccr_oracle_run_exposure on replayed normals instead of the seeded stream (tests of model
+// reductions that need identical draws under two draw layouts).
+int32_t ccr_oracle_run_exposure_replay(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                                       const double* base_grid, int32_t base_size, const ccr_netting_set* sets,
+                                       int32_t num_sets, double pfe_quantile, int32_t threads, const double* normals,
+                                       int64_t num_normals, ccr_exposure_out* outs, char* err, int32_t err_capacity) {
+  if (!normals || num_normals <= 0) {
+    if (err && err_capacity > 0) std::snprintf(err, static_cast<std::size_t>(err_capacity), "no normals");
+    return CCR_ERR_INVALID_ARGUMENT;
+  }
+  gReplay = normals;
+  gReplayLen = static_cast<std::size_t>(num_normals);
+  const int32_t rc = ccr_oracle_run_exposure(generator, config, base_grid, base_size, sets, num_sets, pfe_quantile,
+                                             threads, outs, err, err_capacity);
+  gReplay = nullptr;
+  gReplayLen = 0;
+  return rc;
+}
+
+// The scenario set of ScenarioGenerator::generate (src/models/scenario_generator.cpp:61-143) on
+// `grid`, from the seeded stream or from replayed normals (normals NULL: the stream). Every
+// output is [path][date] row-major and may be NULL; foreign states, spots and survivals are
+// stacked per factor ([factor][path][date]).
+int32_t ccr_oracle_scenarios(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                             const double* grid, int32_t grid_size, const double* normals, int64_t num_normals,
+                             double* x, double* deflator, double* foreign_x, double* spots, double* survival,
+                             char* err, int32_t err_capacity) {
+  return guard(err, err_capacity, [&] {
+    if (!generator || !config || !grid || grid_size <= 0) invalid("null argument");
+    const Generator g(*generator);
+    const Grid gr(Vec(grid, grid + grid_size));
+    gReplay = normals;
+    gReplayLen = normals ? static_cast<std::size_t>(num_normals) : 0;
+    Scenarios sc;
+    try {
+      sc = g.generate(gr, static_cast<std::size_t>(config->num_paths), config->seed, config->antithetic != 0, nullptr);
+    } catch (...) {
+      gReplay = nullptr;
+      gReplayLen = 0;
+      throw;
+    }
+    gReplay = nullptr;
+    gReplayLen = 0;
+    const std::size_t n = sc.P * gr.size();
+    auto put = [&](double* dst, const std::vector<Mat>& ms) {
+      if (!dst) return;
+      for (std::size_t f = 0; f < ms.size(); ++f) std::copy(ms[f].d.begin(), ms[f].d.end(), dst + f * n);
+    };
+    if (x) std::copy(sc.x.d.begin(), sc.x.d.end(), x);
+    if (deflator) std::copy(sc.D.d.begin(), sc.D.d.end(), deflator);
+    put(foreign_x, sc.xf);
+    put(spots, sc.S);
+    put(survival, sc.Q);
   });
 }
 

@@ -37,6 +37,19 @@ int32_t ccr_oracle_run_exposure(const ccr_scenario_generator* generator, const c
                                 int32_t num_sets, double pfe_quantile, int32_t threads, ccr_exposure_out* outs,
                                 char* err, int32_t err_capacity);
 int32_t ccr_oracle_last_timing(ccr_oracle_timing* out);
+/* ccr_oracle_run_exposure with the normals replayed from `normals` (num_normals of them, in the
+ * generator's draw order; every netting set's generate() starts at their beginning) instead of
+ * the seeded stream. */
+int32_t ccr_oracle_run_exposure_replay(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                                       const double* base_grid, int32_t base_size, const ccr_netting_set* sets,
+                                       int32_t num_sets, double pfe_quantile, int32_t threads, const double* normals,
+                                       int64_t num_normals, ccr_exposure_out* outs, char* err, int32_t err_capacity);
+/* ScenarioGenerator::generate on `grid` (seeded stream, or replayed normals when normals is not
+ * NULL). Outputs [path][date] (foreign states, spots, survivals stacked per factor); NULL skips. */
+int32_t ccr_oracle_scenarios(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                             const double* grid, int32_t grid_size, const double* normals, int64_t num_normals,
+                             double* x, double* deflator, double* foreign_x, double* spots, double* survival,
+                             char* err, int32_t err_capacity);
 
 /* Grids (src/core/time_grid.cpp, src/exposure/exposure_engine.cpp:35-48). */
 int32_t ccr_oracle_time_grid(const double* times, int32_t n, double* out, int32_t capacity, int32_t* size);

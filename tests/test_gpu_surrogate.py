@@ -12,16 +12,19 @@ from zlatko_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def test_bf16_fused_inference_matches_fp64_predict(engine):
-    """The fused inference kernel (weights in LDS) against the FP64 restatement of Mlp::predict."""
-    desc = abi.MlpDescC(7, 128, 3, 1, abi.CCR_ACT_SILU)
+@pytest.mark.parametrize("hidden_layers", [3, 4, 5])
+def test_bf16_fused_inference_matches_fp64_predict(engine, hidden_layers):
+    """The fused inference kernel (weights in LDS) against the FP64 restatement of Mlp::predict, at
+    every depth the BF16 path trains (4 and 5 hidden layers leave room in LDS for fewer wavefronts
+    per workgroup than 3 do)."""
+    desc = abi.MlpDescC(7, 128, hidden_layers, 1, abi.CCR_ACT_SILU)
     X, Y = OL.mlp_dataset(3001, 7, 1, seed=4)
     p, s = OL.oracle_fit_mlp(desc, X, Y, 20, lr=0.003)
     ref = np.zeros_like(Y)
     OL.oracle().ccr_oracle_mlp_predict(C.byref(desc), abi.dptr(p), abi.dptr(s), abi.dptr(X), X.shape[0], abi.dptr(ref))
     got = engine.mlp_predict(desc, abi.CCR_BF16, p, s, X)
     err = np.abs(got - ref).max() / Y.std()
-    print("fused bf16 inference", f"{err:.2e}", flush=True)
+    print("fused bf16 inference", hidden_layers, f"{err:.2e}", flush=True)
     assert err < 3e-2
 
 
@@ -38,6 +41,28 @@ def test_gate_rejects_and_falls_back_bit_exact(engine):
     b = prod.execute_surrogate(sur)[0].as_dict()
     for k in abi.PROFILE_FIELDS:
         assert np.array_equal(a[k], b[k]), k
+    for p in (tr, ho, prod):
+        p.close()
+    sur.close()
+
+
+@pytest.mark.parametrize("hidden_layers", [4, 5])
+def test_deeper_surrogates_train_and_execute(engine, hidden_layers):
+    """Every depth the BF16 path accepts for training also runs the surrogate's fused inference:
+    with the gate open the surrogate execute succeeds and the set keeps the surrogate; its EE stays
+    within the gate's hold-out error plus Monte Carlo noise of the plain execute."""
+    tr = engine.compile(W.c4(num_paths=3000, dates=40, seed=101))
+    ho = engine.compile(W.c4(num_paths=3000, dates=40, seed=202))
+    sur = engine.surrogate_train(tr, ho, train_paths=1000, hidden_layers=hidden_layers, epochs=20, lr=0.001,
+                                 batch=4096, gate=1.0)
+    assert sur.report.enabled_sets == 1
+    prod = engine.compile(W.c4(num_paths=3000, dates=40, seed=42))
+    a = prod.execute()[0]
+    b = prod.execute_surrogate(sur)[0]
+    ee_a, ee_b = a.get("expected_exposure"), b.get("expected_exposure")
+    err = np.abs(ee_b - ee_a).max() / ee_a.max()
+    print("surrogate depth", hidden_layers, f"hold-out {sur.report.max_ee_error:.3e} production {err:.3e}", flush=True)
+    assert np.all(np.isfinite(ee_b)) and err < sur.report.max_ee_error + 0.05
     for p in (tr, ho, prod):
         p.close()
     sur.close()

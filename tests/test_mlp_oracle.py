@@ -23,7 +23,7 @@ def test_fit_mlp_matches_reference_golden_bit_for_bit():
     assert np.array_equal(s, np.array(g["stdz"]))
 
 
-@pytest.mark.skipif(OL.ref() is None, reason="reference build (oracle/_ref) not present")
+@pytest.mark.skipif("OL.ref() is None", reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("hidden,epochs,seed", [(8, 1, 3), (16, 40, 7), (12, 120, 11)])
 def test_fit_mlp_matches_live_reference(hidden, epochs, seed):
     X, Y = OL.mlp_dataset(70, 4, 3, seed=seed)

@@ -247,7 +247,7 @@ def test_oracle_bermudan_matches_reference_golden_vectors():
         assert np.array_equal(o.get(f), np.array(ref)), f
 
 
-@pytest.mark.skipif(oracle_lib.ref() is None, reason="reference build (oracle/_ref) not present")
+@pytest.mark.skipif("oracle_lib.ref() is None", reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("kw", [dict(num_paths=200, csa=True, fx=True, credits=True),
                                 dict(num_paths=150, two_leg=True, csa=True),
                                 dict(num_paths=120, csa=True, fx=True, credits=True, bermudans=4),
@@ -268,7 +268,7 @@ def _ref_has(name):
     return r is not None and hasattr(r, name)
 
 
-@pytest.mark.skipif(not _ref_has("ccr_ref_initial_margin"), reason="reference build (oracle/_ref) not present")
+@pytest.mark.skipif("not _ref_has('ccr_ref_initial_margin')", reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("kw", [dict(num_paths=200, fx=True, credits=True),
                                 dict(num_paths=150, csa=True, fx=True, credits=True),
                                 dict(num_paths=120, csa=True, bermudans=3),
@@ -299,7 +299,7 @@ def test_oracle_initial_margin_scaling_and_extremes():
     assert (top >= a).all() and (a >= low).all() and (top[live] > low[live]).all()
 
 
-@pytest.mark.skipif(not _ref_has("ccr_ref_trade_statistics"), reason="reference build (oracle/_ref) not present")
+@pytest.mark.skipif("not _ref_has('ccr_ref_trade_statistics')", reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("kw", [dict(num_paths=200),
                                 dict(num_paths=150, csa=True, fx=True, credits=True),
                                 dict(num_paths=150, fx=True, options=6)])
@@ -331,7 +331,7 @@ def test_oracle_trade_statistics_invariants():
     assert (o["standalone"] >= 0.0).all() and o["standalone"].max() > 0.0
 
 
-@pytest.mark.skipif(not _ref_has("ccr_ref_xva"), reason="reference build (oracle/_ref) not present")
+@pytest.mark.skipif("not _ref_has('ccr_ref_xva')", reason="reference build (oracle/_ref) not present")
 def test_xva_matches_live_reference_build():
     """xva (src/xva/xva.cpp:78-96) of the reference against the oracle's restatement and the C ABI's
     ccr_xva, with and without an initial-margin profile."""
@@ -364,7 +364,7 @@ def test_xva_matches_live_reference_build():
         assert np.allclose(mine, w, rtol=1e-13, atol=1e-9), (mine, w)
 
 
-@pytest.mark.skipif(not _ref_has("ccr_ref_xva"), reason="reference build (oracle/_ref) not present")
+@pytest.mark.skipif("not _ref_has('ccr_ref_xva')", reason="reference build (oracle/_ref) not present")
 def test_xva_allocation_matches_live_reference_build():
     """allocateExposureXva (src/xva/xva.cpp:97-106) is exposureXva of each trade's contribution:
     the C ABI's rows against the reference's xva() CVA / FCA / KVA of the same rows."""
@@ -410,7 +410,7 @@ def test_oracle_book_shares_scenarios_and_grids():
     assert len(t0) > max(own)
 
 
-@pytest.mark.skipif(oracle_lib.ref() is None, reason="reference build (oracle/_ref) not present")
+@pytest.mark.skipif("oracle_lib.ref() is None", reason="reference build (oracle/_ref) not present")
 def test_oracle_book_matches_reference_engine_on_the_union_grid():
     """An uncollateralised set of the book is the reference engine run on the book's simulation
     grid as its base grid (then its simulation grid is that union and its draws are the book's):
@@ -438,7 +438,7 @@ def test_oracle_book_matches_reference_engine_on_the_union_grid():
             assert np.array_equal(r.get(f)[idx], outs[i].get(f)), f
 
 
-@pytest.mark.skipif(not _ref_has("ccr_ref_initial_margin"), reason="reference build (oracle/_ref) not present")
+@pytest.mark.skipif("not _ref_has('ccr_ref_initial_margin')", reason="reference build (oracle/_ref) not present")
 def test_oracle_book_initial_margin_matches_reference_on_the_union_grid():
     """The IM proxy of an uncollateralised book set equals the reference's initialMarginProfile
     of its engine run with the book's simulation grid as the base grid, bit for bit."""

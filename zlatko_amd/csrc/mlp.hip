@@ -288,9 +288,12 @@ __global__ void __launch_bounds__(64) bf16_wgrad_kernel(const __bf16* dfm, const
 // Fused inference (hidden = 128): every layer of a 16-sample tile per wavefront, weights staged
 // in LDS, each layer's output turned into the next layer's A operand through a per-wavefront LDS
 // tile. Inputs x [n][inputs] (raw) are standardised on load with (mean, sd); writes the
-// standardised outputs y [n][outputs]. Grid-stride over tiles.
+// standardised outputs y [n][outputs]. Grid-stride over tiles. The wavefronts per workgroup
+// (blockDim.x / 64, at most kInferWaves) are chosen at launch so that the weights and one
+// activation tile per wavefront fit the workgroup's LDS.
 constexpr int kInferWaves = 16;
 constexpr int kActStride = 128 + 8;  // BF16 elements per activation row in LDS (padding vs bank conflicts)
+constexpr size_t kInferLdsBytes = 160 * 1024;  // LDS of one gfx950 workgroup
 struct InferArgs {
   const __bf16* w[kMlpMaxLayers];
   const float* b[kMlpMaxLayers];
@@ -319,9 +322,10 @@ __global__ void __launch_bounds__(64 * kInferWaves) bf16_infer_kernel(InferArgs 
   __bf16* act = wl + wtotal + (static_cast<int>(threadIdx.x) >> 6) * 16 * kActStride;
   __syncthreads();
   const int lane = static_cast<int>(threadIdx.x) & 63, r = lane & 15, g = lane >> 4;
+  const int waves = static_cast<int>(blockDim.x) >> 6;
   const int64_t tiles = (a.n + 15) / 16;
-  for (int64_t tile = static_cast<int64_t>(blockIdx.x) * kInferWaves + (threadIdx.x >> 6); tile < tiles;
-       tile += static_cast<int64_t>(gridDim.x) * kInferWaves) {
+  for (int64_t tile = static_cast<int64_t>(blockIdx.x) * waves + (threadIdx.x >> 6); tile < tiles;
+       tile += static_cast<int64_t>(gridDim.x) * waves) {
     const int64_t s0 = tile * 16;
     // Layer-0 A operand: sample s0 + r, features 8 g .. 8 g + 7.
     bf16x8 a0;
@@ -406,13 +410,18 @@ hipError_t mlpInfer(const MlpDevice& m, const float* x, int64_t n, const double*
   a.act = sh.act;
   int welems = 0;
   for (int l = 0; l < sh.layers; ++l) welems += sh.pad[l] * sh.pad[l + 1];
-  const size_t lds = static_cast<size_t>(welems) * 2 + static_cast<size_t>(kInferWaves) * 16 * kActStride * 2;
+  // As many wavefronts (16 at most) as leave room for one activation tile each beside the weights:
+  // 16 up to 3 hidden layers, 3 at 5 hidden layers.
+  const size_t wbytes = static_cast<size_t>(welems) * 2, tile = static_cast<size_t>(16) * kActStride * 2;
+  if (wbytes + tile > kInferLdsBytes) return hipErrorInvalidValue;
+  const int waves = static_cast<int>(std::min<size_t>(kInferWaves, (kInferLdsBytes - wbytes) / tile));
+  const size_t lds = wbytes + static_cast<size_t>(waves) * tile;
   const int64_t tiles = (n + 15) / 16;
-  const unsigned grid = static_cast<unsigned>(std::min<int64_t>((tiles + kInferWaves - 1) / kInferWaves, 2048));
+  const unsigned grid = static_cast<unsigned>(std::min<int64_t>((tiles + waves - 1) / waves, 2048));
   if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&bf16_infer_kernel),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)))
     return e;
-  hipLaunchKernelGGL(bf16_infer_kernel, dim3(grid), dim3(64 * kInferWaves), lds, stream, a);
+  hipLaunchKernelGGL(bf16_infer_kernel, dim3(grid), dim3(64 * waves), lds, stream, a);
   return last();
 }
 

@@ -23,13 +23,36 @@ def build_oracle():
     subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "oracle"], check=True)
 
 
+MUTANT_SO = ROOT / "oracle" / "_mutants" / "liboracle_quanto.so"
+_mutant = None
+
+
 def oracle():
     global _oracle
     if _oracle is None:
         if not ORACLE_SO.exists():
             build_oracle()
-        lib = C.CDLL(str(ORACLE_SO))
+        _oracle = _bind(C.CDLL(str(ORACLE_SO)))
+    return _oracle
+
+
+def mutant_quanto():
+    """The oracle built with a planted error (the quanto drift's sign flipped): checks of the
+    multi-currency model must reject it. Built on first use (CPU only)."""
+    global _mutant
+    if _mutant is None:
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "mutants"], check=True)
+        _mutant = _bind(C.CDLL(str(MUTANT_SO)))
+    return _mutant
+
+
+def _bind(lib):
+    if True:
         D, I32, I64, P = C.c_double, C.c_int32, C.c_int64, C.c_void_p
+        lib.ccr_oracle_run_exposure_replay.restype = I32
+        lib.ccr_oracle_run_exposure_replay.argtypes = [P, P, P, I32, P, I32, D, I32, P, I64, P, C.c_char_p, I32]
+        lib.ccr_oracle_scenarios.restype = I32
+        lib.ccr_oracle_scenarios.argtypes = [P, P, P, I32, P, I64, P, P, P, P, P, C.c_char_p, I32]
         lib.ccr_oracle_run_exposure.restype = I32
         lib.ccr_oracle_run_exposure.argtypes = [P, P, P, I32, P, I32, D, I32, P, C.c_char_p, I32]
         lib.ccr_oracle_initial_margin.restype = I32
@@ -81,8 +104,7 @@ def oracle():
         lib.ccr_oracle_mlp_gradient.argtypes = [P, P, P, P, I64, P]
         lib.ccr_oracle_fit_mlp.argtypes = [P, P, P, I64, P, P, P]
         lib.ccr_oracle_mlp_predict.argtypes = [P, P, P, P, I64, P]
-        _oracle = lib
-    return _oracle
+    return lib
 
 
 def ref():
@@ -298,3 +320,81 @@ def normals(seed: int, n: int) -> np.ndarray:
     out = np.empty(n)
     oracle().ccr_oracle_normals(seed, n, out.ctypes.data_as(C.c_void_p))
     return out
+
+
+def run_oracle_replay(spec: abi.Spec, z: np.ndarray, threads: int = 1, rows: bool = False, lib=None):
+    """run_oracle on replayed normals z (the generator's draw order) instead of the seeded stream."""
+    lib = lib or oracle()
+    z = np.ascontiguousarray(z, dtype=np.float64).reshape(-1)
+    cap = max(grid_size(spec, i) for i in range(len(spec.sets)))
+    outs, arr = abi.make_outs(len(spec.sets), cap, spec.num_paths, rows)
+    err = C.create_string_buffer(512)
+    rc = lib.ccr_oracle_run_exposure_replay(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                            len(spec.base_grid), spec.sets_c, len(spec.sets), spec.pfe_quantile,
+                                            threads, abi.dptr(z), z.size, arr, err, 512)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
+    abi.sync_outs(outs, arr)
+    return outs
+
+
+def run_oracle_with(spec: abi.Spec, lib, threads: int = 1, rows: bool = False):
+    """run_oracle through a given oracle build (e.g. the mutant)."""
+    cap = max(grid_size(spec, i) for i in range(len(spec.sets)))
+    outs, arr = abi.make_outs(len(spec.sets), cap, spec.num_paths, rows)
+    err = C.create_string_buffer(512)
+    rc = lib.ccr_oracle_run_exposure(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                     len(spec.base_grid), spec.sets_c, len(spec.sets), spec.pfe_quantile, threads, arr,
+                                     err, 512)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
+    abi.sync_outs(outs, arr)
+    return outs
+
+
+def scenarios(spec: abi.Spec, grid: np.ndarray, z: np.ndarray = None, lib=None) -> dict:
+    """ScenarioGenerator::generate on `grid`: x, D [P][T]; xf [F][P][T]; S [A][P][T]; Q [C][P][T]."""
+    lib = lib or oracle()
+    grid = np.ascontiguousarray(grid, dtype=np.float64)
+    m = spec.model
+    P = spec.num_paths
+    T = len(np.unique(grid))  # the grid is already a TimeGrid (sorted, de-duplicated, starting at 0)
+    out = {"x": np.zeros((P, T)), "D": np.zeros((P, T)), "xf": np.zeros((max(1, len(m.foreign)), P, T)),
+           "S": np.zeros((max(1, len(m.assets)), P, T)), "Q": np.zeros((max(1, len(m.credits)), P, T))}
+    zp = None
+    if z is not None:
+        z = np.ascontiguousarray(z, dtype=np.float64).reshape(-1)
+        zp = abi.dptr(z)
+    err = C.create_string_buffer(512)
+    rc = lib.ccr_oracle_scenarios(C.byref(spec.generator), C.byref(spec.config), abi.dptr(grid), len(grid), zp,
+                                  0 if z is None else z.size, abi.dptr(out["x"]), abi.dptr(out["D"]),
+                                  abi.dptr(out["xf"]), abi.dptr(out["S"]), abi.dptr(out["Q"]), err, 512)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
+    return out
+
+
+def simulation_grid(spec: abi.Spec, i: int) -> np.ndarray:
+    cap = sim_grid_size(spec, i)
+    out = np.zeros(cap)
+    n = C.c_int32()
+    oracle().ccr_oracle_simulation_grid(abi.dptr(spec.base_grid), len(spec.base_grid), C.byref(spec.sets_c[i]),
+                                        abi.dptr(out), cap, C.byref(n))
+    return out[:n.value]
+
+
+def zero_bond_factors(hw: abi.HullWhite, t: float, T: float):
+    """(A, B) with HullWhite1F::zeroBond(t, T, x) = A exp(-B x), from the oracle's zeroBond."""
+    holder = abi.Spec(abi.Model(hw), [abi.NettingSet([abi.Trade(abi.CCR_TRADE_ASSET_FORWARD, 1.0, maturity=1.0)])],
+                      np.array([0.0, 1.0]), 2)
+    h = holder.generator.rates
+    a = oracle().ccr_oracle_zero_bond(C.byref(h), t, T, 0.0)
+    b = -np.log(oracle().ccr_oracle_zero_bond(C.byref(h), t, T, 1.0) / a)
+    return a, b
+
+
+def discount(curve: abi.Curve, t: float) -> float:
+    holder = abi.Spec(abi.Model(abi.HullWhite(curve, 0.05, 0.01)),
+                      [abi.NettingSet([abi.Trade(abi.CCR_TRADE_ASSET_FORWARD, 1.0, maturity=1.0)])],
+                      np.array([0.0, 1.0]), 2)
+    return oracle().ccr_oracle_discount(C.byref(holder.generator.rates.curve), t)

@@ -468,27 +468,6 @@ def test_threads_do_not_change_bits():
             assert np.array_equal(x.get(f), y.get(f)), f
 
 
-def test_foreign_rate_with_zero_vol_reduces_to_deterministic_carry():
-    """A foreign Hull-White with sigma = 0 is the reference's deterministic carry curve: FX
-    martingale and forwards must then agree with the carry-curve model to rounding on the
-    same draws of the FX driver (the extra normal feeds a zero-variance integral)."""
-    base = W.small_reference_case(num_paths=200, fx=True)
-    m = base.model
-    foreign = abi.HullWhite(m.assets[0].carry, 0.05, 0.0)
-    m2 = abi.Model(m.rates, [foreign], [abi.Asset(m.assets[0].spot, m.assets[0].vol, m.assets[0].carry, 0)],
-                   [], None)
-    # Correlation in the extended order [dom, foreign, asset]; the foreign driver is independent.
-    corr = np.eye(3)
-    corr[0, 2] = corr[2, 0] = m.correlation[0, 1]
-    m2.correlation = corr
-    s1 = abi.Spec(abi.Model(m.rates, [], m.assets, [], m.correlation), base.sets, base.base_grid, 200, 7, True, 0.95)
-    s2 = abi.Spec(m2, base.sets, base.base_grid, 200, 7, True, 0.95)
-    # Different draw layouts: compare the deterministic-forward part through E[D S] = S0 Q(0,t).
-    for spec in (s1, s2):
-        o = oracle_lib.run_oracle(spec)[0]
-        assert np.all(np.isfinite(o.get("expected_exposure")))
-
-
 def test_monte_carlo_martingales_of_the_oracle():
     """E[D(t)] = P(0,t) within MC error on the simulated domestic curve (test_models.cpp:18-37)."""
     spec = W.small_reference_case(num_paths=4000, fx=True)

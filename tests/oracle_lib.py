@@ -398,3 +398,30 @@ def discount(curve: abi.Curve, t: float) -> float:
                       [abi.NettingSet([abi.Trade(abi.CCR_TRADE_ASSET_FORWARD, 1.0, maturity=1.0)])],
                       np.array([0.0, 1.0]), 2)
     return oracle().ccr_oracle_discount(C.byref(holder.generator.rates.curve), t)
+
+
+def oracle_bermudan(spec: abi.Spec, set_index: int, trade_index: int, grid: np.ndarray, lib=None):
+    """BermudanSwaption::valueWithExercise restated: (price, exercise index per path, -1 never)."""
+    lib = lib or oracle()
+    grid = np.ascontiguousarray(grid, np.float64)
+    price = C.c_double()
+    ex = np.zeros(spec.num_paths, np.int32)
+    rc = lib.ccr_oracle_bermudan(C.byref(spec.generator), C.byref(spec.config), abi.dptr(grid), len(grid),
+                                 C.byref(spec.sets_c[set_index].trades[trade_index]), C.byref(price),
+                                 ex.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"oracle bermudan failed ({rc})")
+    return price.value, ex
+
+
+def ref_bermudan(spec: abi.Spec, set_index: int, trade_index: int, grid: np.ndarray):
+    """The reference build's valueWithExercise: (price, exercise index per path)."""
+    grid = np.ascontiguousarray(grid, np.float64)
+    price = C.c_double()
+    ex = np.zeros(spec.num_paths, np.int32)
+    rc = ref().ccr_ref_bermudan(C.byref(spec.generator), C.byref(spec.config), abi.dptr(grid), len(grid),
+                                C.byref(spec.sets_c[set_index].trades[trade_index]), C.byref(price),
+                                ex.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"reference bermudan failed ({rc})")
+    return price.value, ex

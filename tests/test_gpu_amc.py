@@ -29,6 +29,8 @@ def _check(spec, outs, ref, tol=FP64_TOL):
         print(spec.name, s, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
         assert errs["times"] == 0.0
         assert compare.worst(errs) <= tol, errs
+        pe = compare.pointwise_errors(outs[s], ref[s])
+        assert compare.worst(pe) <= tol, pe
 
 
 def test_bermudan_golden_vectors_fp64(engine):
@@ -68,6 +70,30 @@ def test_c3_reduced_fp64(engine):
     plan.close()
     assert info.num_bermudans == 40 and info.num_exercises > 40
     _check(spec, out, oracle_lib.run_oracle(spec, threads=8))
+
+
+def test_c3_full_portfolio_fp64(engine):
+    """Config 3's whole portfolio -- 200 Bermudan swaptions (1-10y into 5-10y, annual exercise,
+    degree 3), 120 dates -- at 1,000 paths: the netting set's profile against the oracle, and every
+    swaption's price and per-path exercise indices from ccr_hip_bermudan against the oracle's
+    valueWithExercise (which tests/test_oracle.py pins to the reference build on this case)."""
+    spec = W.c3(num_paths=1000)
+    plan = engine.compile(spec)
+    info = plan.info()
+    out = plan.execute()
+    plan.close()
+    assert info.num_bermudans == 200
+    _check(spec, out, oracle_lib.run_oracle(spec, threads=16))
+    grid = np.ascontiguousarray(spec.base_grid)
+    flips = 0
+    worst = 0.0
+    for t in range(200):
+        price, ex = engine.bermudan(spec, 0, t, grid)
+        rp, rex = oracle_lib.oracle_bermudan(spec, 0, t, grid)
+        worst = max(worst, abs(price - rp) / abs(rp))
+        flips += int(np.sum(ex != rex))
+    print("c3 full: 200 swaptions, worst price error", f"{worst:.2e}", "exercise flips", flips, flush=True)
+    assert worst <= FP64_TOL and flips == 0
 
 
 def test_c4_with_bermudans_fp64(engine):

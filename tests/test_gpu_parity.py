@@ -28,12 +28,15 @@ SMALL = {
 }
 
 
-def _check(spec, outs, ref, tol=FP64_TOL):
+def _check(spec, outs, ref, tol=FP64_TOL, pointwise=True):
     for s in range(len(spec.sets)):
         errs = compare.field_errors(outs[s], ref[s])
         print(spec.name, s, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
         assert errs["times"] == 0.0
         assert compare.worst(errs) <= tol, errs
+        if pointwise:
+            pe = compare.pointwise_errors(outs[s], ref[s])
+            assert compare.worst(pe) <= tol, pe
 
 
 @pytest.mark.parametrize("case", sorted(SMALL))
@@ -72,6 +75,21 @@ def test_c2_reduced_fp64(engine):
     """3-currency HW + 2 FX, CSA netting sets (config 2 shape, fewer paths/sets)."""
     spec = W.c2(num_paths=1000, num_sets=4, num_trades=80, weeks=104, horizon=2.0)
     _check(spec, engine.run(spec), oracle_lib.run_oracle(spec, threads=8))
+
+
+def test_c2_full_portfolio_fp64(engine):
+    """Config 2 in full -- 20 CSA netting sets, 1,000 trades (700 two-leg IRS in 3 currencies, 300
+    FX forwards), 260 weekly reporting dates (up to 619 simulation dates with the margin-call and
+    event dates), the plan shape the path kernel is tuned for -- at 384 paths: every set and date
+    against the oracle, scale-relative and pointwise."""
+    spec = W.c2(num_paths=384)
+    plan = engine.compile(spec)
+    info = plan.info()
+    outs = plan.execute()
+    plan.close()
+    assert info.num_sets == 20 and info.max_sim_dates > 600
+    ref = oracle_lib.run_oracle(spec, threads=16)
+    _check(spec, outs, ref)
 
 
 def test_c4_plain_credit_fp64(engine):

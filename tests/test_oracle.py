@@ -248,6 +248,25 @@ def test_oracle_bermudan_matches_reference_golden_vectors():
 
 
 @pytest.mark.skipif("oracle_lib.ref() is None", reason="reference build (oracle/_ref) not present")
+def test_oracle_matches_reference_on_the_full_c3_portfolio():
+    """Config 3's 200 Bermudan swaptions on 120 dates (1,000 paths) are reference-representable:
+    the oracle's netting-set profile equals the reference engine's bit for bit, and so do every
+    swaption's price and per-path exercise indices from valueWithExercise."""
+    spec = W.c3(num_paths=1000)
+    o = oracle_lib.run_oracle(spec, threads=8)[0]
+    r, _, _ = oracle_lib.run_ref(spec)
+    for f in ("times", "expected_value", "expected_exposure", "expected_negative_exposure",
+              "potential_future_exposure", "discounted_expected_exposure", "discounted_expected_negative_exposure",
+              "mean_deflator"):
+        assert np.array_equal(o.get(f), r.get(f)), f
+    grid = np.ascontiguousarray(spec.base_grid)
+    for t in range(200):
+        po, eo = oracle_lib.oracle_bermudan(spec, 0, t, grid)
+        pr, er = oracle_lib.ref_bermudan(spec, 0, t, grid)
+        assert po == pr and np.array_equal(eo, er), t
+
+
+@pytest.mark.skipif("oracle_lib.ref() is None", reason="reference build (oracle/_ref) not present")
 @pytest.mark.parametrize("kw", [dict(num_paths=200, csa=True, fx=True, credits=True),
                                 dict(num_paths=150, two_leg=True, csa=True),
                                 dict(num_paths=120, csa=True, fx=True, credits=True, bermudans=4),

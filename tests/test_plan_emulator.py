@@ -47,6 +47,8 @@ CASES = {
     "all": lambda: W.small_reference_case(96, csa=True, fx=True, credits=True, two_leg=True),
     "c1": lambda: W.c1(num_paths=100),
     "c2_shape": lambda: W.c2(num_paths=48, num_sets=4, num_trades=60, weeks=52, horizon=1.0),
+    "c2_full": lambda: W.c2(num_paths=24),
+    "c3_full": lambda: W.c3(num_paths=48),
     "c4_plain": lambda: W.c4(num_paths=48, dates=24, with_bermudans=False),
     "bermudan": lambda: bermudan_case(200),
     "c3_small": lambda: W.c3(num_paths=96, dates=30, num_swaptions=12),
@@ -76,6 +78,8 @@ def test_emulated_plan_matches_oracle(emu, case):
         errs = compare.field_errors(e[s], o[s])
         assert errs["times"] == 0.0
         assert compare.worst(errs) <= 1e-12, errs
+        pe = compare.pointwise_errors(e[s], o[s])
+        assert compare.worst(pe) <= 1e-10, pe
         rows_e, rows_o = e[s].exposure_rows(spec.num_paths), o[s].exposure_rows(spec.num_paths)
         scale = max(rows_o.max(), 1.0)
         assert np.abs(rows_e - rows_o).max() <= 1e-12 * scale

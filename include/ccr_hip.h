@@ -458,6 +458,21 @@ int32_t ccr_hip_plan_info_get(const ccr_hip_plan* plan, ccr_hip_plan_info* info)
 int32_t ccr_hip_execute(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs);
 /* Same, without reading results back (for timing loops). */
 int32_t ccr_hip_execute_async(ccr_hip_ctx* ctx, ccr_hip_plan* plan);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation). ccr_hip_execute for a plan
+ * compiled after ccr_hip_set_shared_workspace(ctx, 1), with its per-execute buffers (exposure rows,
+ * wave partials, American Monte Carlo states, exercise values, regressions and valuation rows)
+ * placed in a library-owned static device array of 64 MiB per GPU (held until the process ends)
+ * instead of any allocation: the call makes no hipMalloc / hipFree and no host allocation except
+ * the message of a failure. The statistics are copied straight into the caller's arrays.
+ * Errors: CCR_ERR_INVALID_ARGUMENT for a plan compiled without the shared workspace,
+ * CCR_ERR_OUT_OF_RANGE when its buffers exceed 64 MiB (e.g. ~50,000 paths x 120 dates for one
+ * FP64 netting set), CCR_ERR_UNSUPPORTED for plans with the initial-margin proxy or per-trade
+ * statistics and for FP32 exposure rows. Every call on one GPU shares that GPU's array, which keeps
+ * the last call's contents (every kernel writes what it later reads, so nothing is cleared): calls
+ * that overlap on one GPU -- two contexts, two host threads -- race. On a device error the output
+ * arrays may be partly written (num_dates is set only on success). */
+int32_t ccr_hip_execute_noalloc(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs);
 /* Waits for the context's stream, then for the whole device. */
 int32_t ccr_hip_synchronize(ccr_hip_ctx* ctx);
 int32_t ccr_hip_last_timing(const ccr_hip_ctx* ctx, ccr_hip_timing* timing);

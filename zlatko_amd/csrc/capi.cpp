@@ -8,6 +8,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -184,6 +185,16 @@ void execAlloc(ccr_hip_ctx* ctx, ccr_hip_plan* plan, T*& field, size_t bytes) {
   field = nullptr;
 }
 
+// Points the plan's per-execute buffers (workspace slots) into the memory at `base`.
+void bindSlots(ccr_hip_plan* plan, void* base) {
+  for (auto& s : plan->ws_slots) *s.first = static_cast<char*>(base) + s.second;
+  AmcArgs& a = plan->amc;
+  a.states = plan->d_states;
+  a.cont = plan->d_cont;
+  a.exidx = plan->d_exidx;
+  plan->d_bermv = a.bermv;
+}
+
 void bindWorkspace(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
   if (plan->ws_slots.empty()) return;
   if (ctx->ws_bytes < plan->ws_need) {
@@ -194,12 +205,7 @@ void bindWorkspace(ccr_hip_ctx* ctx, ccr_hip_plan* plan) {
     check(hipMalloc(&ctx->ws, plan->ws_need), "hipMalloc (shared workspace)");
     ctx->ws_bytes = plan->ws_need;
   }
-  for (auto& s : plan->ws_slots) *s.first = static_cast<char*>(ctx->ws) + s.second;
-  AmcArgs& a = plan->amc;
-  a.states = plan->d_states;
-  a.cont = plan->d_cont;
-  a.exidx = plan->d_exidx;
-  plan->d_bermv = a.bermv;
+  bindSlots(plan, ctx->ws);
 }
 
 void* devAlloc(ccr_hip_ctx* ctx, ccr_hip_plan* plan, size_t bytes) {
@@ -628,13 +634,16 @@ void launchPathStages(ccr_hip_ctx* ctx, ccr_hip_plan* plan, int blocks, const cc
   markEvent(ctx, 2);
 }
 
-void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* sur = nullptr) {
+// wsBase: the memory the plan's per-execute buffers are bound to for this execute (null: the
+// context's shared workspace, grown if needed).
+void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* sur = nullptr, void* wsBase = nullptr) {
   if (ctx->fault == CCR_FAULT_LAUNCH) {
     ctx->fault = CCR_FAULT_NONE;
     fail(CCR_ERR_DEVICE, "kernel launch failed (forced by ccr_hip_set_fault)");
   }
   check(hipSetDevice(ctx->device), "hipSetDevice");
-  bindWorkspace(ctx, plan);
+  if (wsBase) bindSlots(plan, wsBase);
+  else bindWorkspace(ctx, plan);
   const HostPlan& h = plan->host;
   const int nsets = static_cast<int>(h.sets.size());
   const int blocks = static_cast<int>(h.block_set.size());
@@ -840,6 +849,43 @@ int32_t writeGrid(const TimeGrid& g, double* out, int32_t cap, int32_t* size) {
   return CCR_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free execution
+// ccr_hip_execute_noalloc binds a plan's per-execute buffers (exposure rows, wave partials,
+// American Monte Carlo states, exercise values, regressions and valuation rows) to this array
+// instead of allocating. Its name is a host-side stand-in: host code uses the device address
+// hipGetSymbolAddress gives for it, taken once per device in ccr_hip_create (with that device
+// current) and kept in gScratchPtr. Every kernel writes each element of these buffers before
+// reading it in the same execute, so nothing needs clearing between calls.
+constexpr size_t kScratchBytes = static_cast<size_t>(64) << 20;
+constexpr int kScratchDevices = 64;
+std::mutex gScratchMutex;
+void* gScratchPtr[kScratchDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// The device address of `scratch` on `device` (current), looked up on the first call per device.
+void scratchInit(int device) {
+  if (device < 0 || device >= kScratchDevices) fail(CCR_ERR_UNSUPPORTED, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (gScratchPtr[device]) return;
+  void* p = nullptr;
+  check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress (scratch)");
+  gScratchPtr[device] = p;
+}
+
+// Strided copy of one statistic of one set (element k of every kStats-wide date record) into
+// the caller's array: a 2-D copy, no staging buffer.
+void copyStat(double* dst, const double* stats, int k, int R, hipStream_t stream) {
+  if (!dst || R <= 0) return;
+  check(hipMemcpy2DAsync(dst, sizeof(double), stats + k, kStats * sizeof(double), sizeof(double),
+                         static_cast<size_t>(R), hipMemcpyDeviceToHost, stream),
+        "hipMemcpy2DAsync");
+}
+
 }  // namespace
 
 extern "C" {
@@ -874,6 +920,13 @@ int32_t ccr_hip_create(const ccr_hip_options* options, ccr_hip_ctx** out) {
   for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&ctx->ev[i]);
   if (e != hipSuccess) {
     gCreateError = std::string("HIP initialisation failed: ") + hipGetErrorString(e);
+    return CCR_ERR_DEVICE;
+  }
+  // The allocation-free entry point's scratch on this device (its address, once per device).
+  try {
+    scratchInit(ctx->device);
+  } catch (const std::exception& x) {
+    gCreateError = x.what();
     return CCR_ERR_DEVICE;
   }
   gCreateError.clear();
@@ -1115,6 +1168,70 @@ int32_t ccr_hip_execute(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* 
     launchExecute(ctx, plan);
     readBack(ctx, plan, outs);
   });
+}
+
+int32_t ccr_hip_execute_noalloc(ccr_hip_ctx* ctx, ccr_hip_plan* plan, ccr_exposure_out* outs) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  // No std::function wrapper here: the call allocates nothing on the device and nothing on the
+  // host unless it fails (the error message).
+  try {
+    if (!plan || !outs) invalid("null argument");
+    if (plan->ws_slots.empty())
+      invalid("ccr_hip_execute_noalloc: compile the plan after ccr_hip_set_shared_workspace(ctx, 1)");
+    if (plan->ws_need > kScratchBytes)
+      fail(CCR_ERR_OUT_OF_RANGE, "ccr_hip_execute_noalloc: the plan needs " + std::to_string(plan->ws_need) +
+                                     " bytes of per-execute buffers; the scratch holds at most " +
+                                     std::to_string(kScratchBytes) + " (64 MiB)");
+    if (plan->d_im_q || plan->tstride > 0)
+      fail(CCR_ERR_UNSUPPORTED, "ccr_hip_execute_noalloc: plans with the initial-margin proxy or per-trade "
+                                "statistics need ccr_hip_execute");
+    const HostPlan& h = plan->host;
+    const size_t nsets = h.sets.size();
+    const int64_t P = h.g.num_paths;
+    for (size_t s = 0; s < nsets; ++s) {
+      if (outs[s].capacity < h.sets[s].num_rep) invalid("output capacity smaller than the reporting grid");
+      if (outs[s].exposure_rows && h.precision == CCR_FP32)
+        fail(CCR_ERR_UNSUPPORTED, "ccr_hip_execute_noalloc: FP32 exposure rows need ccr_hip_execute (widening)");
+    }
+    void* base = ctx->device >= 0 && ctx->device < kScratchDevices ? gScratchPtr[ctx->device] : nullptr;
+    if (!base) fail(CCR_ERR_DEVICE, "ccr_hip_execute_noalloc: no scratch address for this device");
+    launchAll(ctx, plan, nullptr, base);
+    for (size_t s = 0; s < nsets; ++s) {
+      ccr_exposure_out& o = outs[s];
+      const int R = h.sets[s].num_rep;
+      const double* st = plan->d_stats + s * static_cast<size_t>(h.max_rep) * kStats;
+      if (o.times) std::copy(h.rep_times[s].begin(), h.rep_times[s].begin() + R, o.times);
+      copyStat(o.expected_value, st, 0, R, ctx->stream);
+      copyStat(o.expected_exposure, st, 1, R, ctx->stream);
+      copyStat(o.expected_negative_exposure, st, 2, R, ctx->stream);
+      copyStat(o.discounted_expected_exposure, st, 3, R, ctx->stream);
+      copyStat(o.discounted_expected_negative_exposure, st, 4, R, ctx->stream);
+      copyStat(o.mean_deflator, st, 5, R, ctx->stream);
+      copyStat(o.mean_survival, st, 6, R, ctx->stream);
+      copyStat(o.cva_increments, st, 7, R, ctx->stream);
+      copyStat(o.dva_increments, st, 8, R, ctx->stream);
+      if (o.potential_future_exposure)
+        check(hipMemcpyAsync(o.potential_future_exposure, plan->d_pfe + s * static_cast<size_t>(h.max_rep),
+                             static_cast<size_t>(R) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+              "hipMemcpyAsync");
+      if (o.exposure_rows)
+        check(hipMemcpyAsync(o.exposure_rows, static_cast<const double*>(plan->d_rows) + h.sets[s].rows_off,
+                             static_cast<size_t>(R) * static_cast<size_t>(P) * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream),
+              "hipMemcpyAsync");
+    }
+    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    for (size_t s = 0; s < nsets; ++s) outs[s].num_dates = h.sets[s].num_rep;
+    readTimings(ctx);
+    ctx->err.clear();
+    return CCR_OK;
+  } catch (const Error& e) {
+    ctx->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return CCR_ERR_RUNTIME;
+  }
 }
 
 int32_t ccr_hip_last_timing(const ccr_hip_ctx* ctx, ccr_hip_timing* timing) {

@@ -70,6 +70,8 @@ def lib():
         L.ccr_hip_execute.restype = I32
         L.ccr_hip_execute_async.argtypes = [P, P]
         L.ccr_hip_execute_async.restype = I32
+        L.ccr_hip_execute_noalloc.argtypes = [P, P, P]
+        L.ccr_hip_execute_noalloc.restype = I32
         L.ccr_hip_synchronize.argtypes = [P]
         L.ccr_hip_synchronize.restype = I32
         L.ccr_hip_last_timing.argtypes = [P, P]
@@ -142,6 +144,14 @@ class Plan:
 
     def execute_async(self):
         self.engine._check(lib().ccr_hip_execute_async(self.engine.ctx, self.handle))
+
+    def execute_noalloc(self, rows: bool = False):
+        """ccr_hip_execute_noalloc: the per-execute buffers in the library's static scratch."""
+        info = self.info()
+        outs, arr = abi.make_outs(len(self.spec.sets), info.max_reporting_dates, self.spec.num_paths, rows)
+        self.engine._check(lib().ccr_hip_execute_noalloc(self.engine.ctx, self.handle, arr))
+        abi.sync_outs(outs, arr)
+        return outs
 
     def trade_statistics(self, s: int, trade: int) -> dict:
         """Per-trade statistics of trade `trade` of netting set `s` from the last execute: one array

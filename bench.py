@@ -1,13 +1,16 @@
 #!/usr/bin/env python3
 """Exposure-engine hot path on one MI355X: trade x path x date revaluations per second.
 
-Workload (default): 3 currencies on Hull-White + 2 lognormal FX (random SPD correlation,
-|rho| <= 0.6), 1,000 trades (700 two-leg IRS in 3 currencies, 300 FX forwards) over 20
-netting sets under CSA (threshold U[0, 1e7], MTA 5e5, 10-day MPoR), 50,000 paths x 260 weekly
-reporting dates (plus trade event and margin-call dates), PFE 97.5%. A step is one execution of
-the compiled plan over all netting sets: path simulation, revaluation, collateral, EE/ENE/EE*/
-ENE*/CVA increments and the exact PFE select, with the replayed normal stream and the plan
-tables already resident in HBM.
+Workload (default, the largest configuration that fits one GPU): the batched uneven portfolio --
+500 netting sets with Zipf(1.2) trade counts clipped to [1, 5,000] (20,744 trades: two-leg IRS in
+3 currencies, FX forwards, 1,009 Bermudan swaptions), every other set under CSA, 3-currency
+Hull-White + 2 lognormal FX + one CIR++ counterparty intensity, 200,000 paths x 120 dates, PFE
+97.5%. The sets form one portfolio book (the reference's book semantics: every set on the book's
+union grids, pathwise identical scenarios), compiled as waves of 50 netting sets (one plan each,
+one shared workspace, the whole book's grids). A step is one execution of every wave: scenario
+simulation, revaluation (American Monte Carlo for the Bermudans), collateral,
+EE/ENE/EE*/ENE*/CVA increments and the exact PFE select, with the replayed normal stream and the
+plan tables already resident in HBM.
 
 Prints one JSON line. `roofline` is that of the dominant kernel (the path kernel), timed with
 HIP events on the stream it runs on; `cpu_baseline` is the FP64 oracle (the reference engine's
@@ -56,8 +59,8 @@ WORKLOAD_TEXT = {
     "c4": "c4: 20 two-leg IRS + 10 FX forwards + 10 Bermudans, HW + FX + CIR++ counterparty / own intensities "
           "(0.3 wrong-way), 100,000 paths x 120 dates",
     "c5": "c5: 500 netting sets, Zipf(1.2) trade counts 1..5000 (20,744 trades: 3-ccy IRS, FX forwards, 1,009 "
-          "Bermudans), half under CSA, 3-ccy HW + 2 FX + shared CIR++, 120 dates; netting-set waves (one plan "
-          "each, one shared workspace) once sets x paths exceeds 1e7",
+          "Bermudans), half under CSA, 3-ccy HW + 2 FX + shared CIR++, 200,000 paths x 120 dates; one portfolio book "
+          "(union grids, shared scenarios) in netting-set waves (one plan each, one shared workspace)",
 }
 
 
@@ -94,6 +97,15 @@ def describe(spec: abi.Spec, infos, waves: int) -> dict:
     if waves > 1:
         d["netting_set_waves"] = waves
     return d
+
+
+def path_kernel_bytes(plans, outs, infos, elem: int) -> int:
+    """Algorithmic HBM bytes of the path kernel per step: the replayed normal stream read once per
+    plan (a book plan's sets replay one slice; independent sets each their own, counted once by
+    the longest) plus max(V - C, 0) written once per (set, reporting date, path)."""
+    stream = sum(i.stream_normals for i in infos) * elem
+    rows = sum(o.c.num_dates for o in outs) * infos[0].num_paths * elem
+    return stream + rows
 
 
 def split_waves(spec: abi.Spec, wave_sets: int):
@@ -187,32 +199,35 @@ def host_cpu() -> str:
     return f"{os.cpu_count()} logical CPUs"
 
 
-def cpu_baseline(spec_fn, budget_s: float, threads: int):
-    """FP64 oracle on a bounded path sample of the same workload (all netting sets): all host
-    threads (the reported value), then one thread on a smaller sample."""
+def cpu_baseline(spec_fn, budget_s: float, threads: int, book: bool):
+    """FP64 oracle on a bounded path sample of the same workload (all netting sets; the book
+    restatement for book runs): all host threads (the reported value), then one thread on a
+    smaller sample."""
     from tests import oracle_lib
+    run = oracle_lib.run_oracle_book if book else oracle_lib.run_oracle
 
     def timed(nthreads: int, budget: float, probe_paths: int):
         probe = spec_fn(probe_paths)
         t0 = time.perf_counter()
-        oracle_lib.run_oracle(probe, threads=nthreads)
+        run(probe, threads=nthreads)
         per_path = (time.perf_counter() - t0) / probe_paths
         paths = int(max(probe_paths, min(2000, budget / max(per_path, 1e-9))))
         paths -= paths % 2
         spec = spec_fn(paths)
         t0 = time.perf_counter()
-        oracle_lib.run_oracle(spec, threads=nthreads)
+        run(spec, threads=nthreads)
         wall = time.perf_counter() - t0
         return spec, paths, wall, oracle_lib.oracle_timing()
 
     spec, paths, wall, tm = timed(threads, budget_s, 8)
     _, paths1, wall1, tm1 = timed(1, budget_s / 3, 2)
+    what = "book restatement" if book else "ExposureEngine restated"
     return {
         "value": tm.revaluations / wall,
         "unit": "revals/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP64 oracle (ExposureEngine restated), same {len(spec.sets)} netting sets and grids, {paths} paths, "
+        "sample": f"FP64 oracle ({what}), same {len(spec.sets)} netting sets and grids, {paths} paths, "
                   f"{wall:.1f} s wall (draws {tm.draws_s:.2f} s, pricing {tm.pricing_s:.1f} s, "
                   f"collateral {tm.collateral_s:.2f} s, statistics {tm.statistics_s:.2f} s); host {host_cpu()}",
         "value_1thread": tm1.revaluations / wall1,
@@ -220,15 +235,33 @@ def cpu_baseline(spec_fn, budget_s: float, threads: int):
     }
 
 
+def compile_waves(engine, spec: abi.Spec, prec: int, wave_sets: int, book: bool):
+    """One plan per wave of netting sets; book runs use the whole book's grids in every wave."""
+    specs = split_waves(spec, wave_sets)
+    engine.set_book(book)
+    if book and len(specs) > 1:
+        rep, sim = native.book_grids(spec)
+        engine.set_book_grids(rep, sim)
+    if len(specs) > 1:
+        engine.set_shared_workspace(True)
+    plans = [engine.compile(s, prec) for s in specs]
+    return specs, plans
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--paths", type=int, default=0, help="0: the workload's own path count")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2",
-                    help="c2 (default, the metric's workload); c1, c3 (Bermudan AMC), c4 (credit + Bermudans), c5 (500 uneven sets)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c5",
+                    help="c5 (default: the largest single-GPU configuration); c1, c2 (3-ccy CSA sets), c3 "
+                         "(Bermudan AMC), c4 (credit + Bermudans)")
+    ap.add_argument("--book-group", type=int, default=0,
+                    help="netting sets per workgroup of the book kernel (0: the library's default)")
+    ap.add_argument("--no-book", action="store_true",
+                    help="c5: independent netting sets (one engine run per set) instead of one book")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -256,17 +289,17 @@ def main():
     make, default_paths = WORKLOADS[args.workload]
     spec_fn = lambda p: make(p)  # noqa: E731
     spec = spec_fn(args.paths or default_paths)
+    book = args.workload == "c5" and not args.no_book
     # Netting-set waves: portfolios whose per-execute buffers would not fit in HBM at once run as
     # one plan per wave, the waves sharing one context workspace (sized to the largest wave).
     wave_sets = args.wave_sets
     if wave_sets == 0 and len(spec.sets) * spec.num_paths > 10_000_000:
         wave_sets = max(1, 10_000_000 // spec.num_paths)
-    specs = split_waves(spec, wave_sets)
     prec = abi.CCR_FP32 if args.precision == "fp32" else abi.CCR_FP64
     engine = native.Engine()
-    if len(specs) > 1:
-        engine.set_shared_workspace(True)
-    plans = [engine.compile(s, prec) for s in specs]
+    if args.book_group:
+        engine.set_book_group(args.book_group)
+    specs, plans = compile_waves(engine, spec, prec, wave_sets, book)
     infos = [p.info() for p in plans]
     revals = sum(i.revaluations for i in infos)
     wall, kms, last = time_gpu(engine, plans, args.steps, args.warmup)
@@ -274,12 +307,10 @@ def main():
     value = revals / (wall / args.steps)
 
     elem = 4 if prec == abi.CCR_FP32 else 8
-    # Path kernel algorithmic bytes: every netting set's slice of the replayed normal stream read
-    # once (P/2 x (T_s - 1) x nz per set) + max(V - C, 0) rows written once.
     outs = [o for p in plans for o in p.execute()]
-    rows_elems = sum(o.c.num_dates for o in outs) * infos[0].num_paths
-    path_bytes = sum(i.normals_read for i in infos) * elem + rows_elems * elem
+    path_bytes = path_kernel_bytes(plans, outs, infos, elem)
     achieved_gbs = path_bytes / (kms["path"] * 1e-3) / 1e9
+    variant = f"{args.workload}{'' if book or args.workload != 'c5' else '_sets'}"
 
     result = {
         "metric": METRIC,
@@ -305,36 +336,45 @@ def main():
             "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": None,
             "algorithmic_bytes": path_bytes,
+            "kernel": "ccr_path_kernel (all waves of one step)",
         },
     }
-    traffic, source = measured_traffic(args.workload, args.precision)
-    if traffic is not None and len(specs) == 1 and not args.paths:
+    if book:
+        result["config"]["book"] = True
+    traffic, source = measured_traffic(variant, args.precision)
+    if traffic is not None and not args.paths:
         result["roofline"]["traffic"] = traffic
         result["roofline"]["traffic_source"] = source
+        result["roofline"]["reread_factor"] = traffic / path_bytes
     # The ceiling that binds this kernel: VALU issue. Every wave64 VALU instruction occupies its
-    # SIMD for 4 cycles (FP64 FMA: 16 lanes per cycle); 1,024 SIMDs at 2.4 GHz.
-    valu, vsource = profile_figure(args.workload, args.precision, "valu_instructions")
-    if valu is not None and len(specs) == 1 and not args.paths:
-        issue = valu * 4.0 / (VALU_SIMDS * VALU_CLOCK_HZ * kms["path"] * 1e-3)
+    # SIMD for 4 cycles in FP64 (16 lanes per cycle) and 2 in FP32; 1,024 SIMDs at 2.4 GHz.
+    valu, vsource = profile_figure(variant, args.precision, "valu_instructions")
+    if valu is not None and not args.paths:
+        cycles = 4.0 if prec == abi.CCR_FP64 else 2.0
+        issue = valu * cycles / (VALU_SIMDS * VALU_CLOCK_HZ * kms["path"] * 1e-3)
         result["compute"] = {"bound": "valu", "valu_instructions": valu, "issue_frac": issue,
                              "source": vsource}
     for p in plans:
         p.close()
     if not args.no_fp32 and prec == abi.CCR_FP64:
-        p32 = [engine.compile(s, abi.CCR_FP32) for s in specs]
+        _, p32 = compile_waves(engine, spec, abi.CCR_FP32, wave_sets, book)
         w32, k32, _ = time_gpu(engine, p32, args.steps, args.warmup)
         o32 = [o for p in p32 for o in p.execute()]
         for p in p32:
             p.close()
         ee_err = max(float(np.abs(a.get("expected_exposure") - b.get("expected_exposure")).max()
                            / np.abs(b.get("expected_exposure")).max()) for a, b in zip(o32, outs))
+        cva64 = sum(float(o.get("cva_increments")[1:].sum()) for o in outs)
+        cva32 = sum(float(o.get("cva_increments")[1:].sum()) for o in o32)
         result["fp32"] = {"value": revals / (w32 / args.steps), "ms_per_step": 1000.0 * w32 / args.steps,
-                          "path_kernel_ms": k32["path"], "ee_rel_err_vs_fp64": ee_err}
-    if not args.no_cva and args.workload == "c2" and not args.paths:
+                          "path_kernel_ms": k32["path"], "ee_rel_err_vs_fp64": ee_err,
+                          "cva_fp64": cva64, "cva_fp32": cva32,
+                          "cva_rel_err_vs_fp64": abs(cva32 - cva64) / abs(cva64) if cva64 else None}
+    if not args.no_cva and args.workload == "c5" and not args.paths:
         result["cva_wall_time"] = cva_wall_time(engine, args.steps)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        result["cpu_baseline"] = cpu_baseline(spec_fn, args.cpu_seconds, threads)
+        result["cpu_baseline"] = cpu_baseline(spec_fn, args.cpu_seconds, threads, book)
     engine.close()
     print(json.dumps(result), flush=True)
 

@@ -407,6 +407,26 @@ int32_t ccr_hip_set_graph(ccr_hip_ctx* ctx, int32_t enable);
  * deflators are pathwise identical across the book and EE, ENE, EE*, ENE* and the pathwise CVA /
  * DVA increments of different netting sets add up exactly. */
 int32_t ccr_hip_set_book(ccr_hip_ctx* ctx, int32_t enable);
+/* The book's grids (src/app/analyses.cpp:1690-1709): the union of the sets' reporting grids and
+ * the union of their simulation grids built on it. Writes up to the capacities and the full
+ * sizes. Needs no device. */
+int32_t ccr_hip_book_grids(const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
+                           double* reporting, int32_t reporting_capacity, int32_t* reporting_size, double* simulation,
+                           int32_t simulation_capacity, int32_t* simulation_size);
+/* A book compiled as several plans (waves of its netting sets): book plans compiled afterwards
+ * report on `reporting` and simulate on `simulation` (the whole book's grids from
+ * ccr_hip_book_grids) instead of the union of their own sets' grids, so every wave replays the
+ * same scenarios. Compile fails with CCR_ERR_INVALID_ARGUMENT if a set needs a date missing from
+ * them. NULL or size 0 clears. Sets with Bermudan swaptions share one states pass and one
+ * regression design per date within a book plan. */
+int32_t ccr_hip_set_book_grids(ccr_hip_ctx* ctx, const double* reporting, int32_t reporting_size,
+                               const double* simulation, int32_t simulation_size);
+/* Book plans compiled afterwards run the book kernel with this many netting sets per workgroup
+ * (1, 2, 4 or 8; default 4; 1 = one set per workgroup): a lane steps its path's market once per
+ * date and values every set of its group, sharing the fixing states. Results are bit-identical
+ * for every choice. Plans with European options, per-trade statistics or the initial-margin
+ * proxy run one set per workgroup. */
+int32_t ccr_hip_set_book_group(ccr_hip_ctx* ctx, int32_t sets_per_workgroup);
 /* sumProfiles (src/app/analyses.cpp:1769-1784): the additive fields (expected value, EE, ENE, EE*,
  * ENE*, CVA and DVA increments) summed over the netting sets of counterparty credit
  * `counterparty` (-1: every set) of one book execute; times, mean deflator and survival are the

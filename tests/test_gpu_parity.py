@@ -526,6 +526,86 @@ def test_book_matches_oracle_and_totals_add_up_fp64():
     e.close()
 
 
+def _book_spec(num_paths):
+    """A config-5-shaped book small enough for the oracle: 24 uneven netting sets (Zipf sizes
+    up to 60 trades), IRS in 3 currencies, FX forwards and Bermudans (10% of trades), every other
+    set under CSA, one shared CIR++ counterparty credit."""
+    return W.c5(num_paths=num_paths, num_sets=24, largest=60, bermudan_share=0.1)
+
+
+def test_book_with_bermudans_matches_oracle_fp64():
+    """Book plans share one states pass and one regression design per date among every set with
+    Bermudans, and tile the continuation GEMM across sets valued on the same dates: profiles of
+    every set within 1e-10 of the oracle's book (scale-relative and pointwise)."""
+    from zlatko_amd import native
+    spec = _book_spec(512)
+    e = native.Engine()
+    e.set_book(True)
+    plan = e.compile(spec)
+    info = plan.info()
+    outs = plan.execute()
+    plan.close()
+    e.close()
+    holders = sum(1 for ns in spec.sets if any(t.kind == abi.CCR_TRADE_BERMUDAN_SWAPTION for t in ns.trades))
+    print("book with Bermudans:", holders, "sets hold", info.num_bermudans, "swaptions", flush=True)
+    assert holders >= 4
+    _check(spec, outs, oracle_lib.run_oracle_book(spec, threads=16))
+
+
+@pytest.mark.parametrize("precision", [abi.CCR_FP64, abi.CCR_FP32])
+def test_book_kernel_group_sizes_give_identical_results(precision):
+    """The book kernel values 1, 2, 4 or 8 netting sets per workgroup from one market step per
+    date: every choice gives the same profiles bit for bit (FP64 and FP32), with 24 sets so the
+    last group is partial for 8 ... and exact for the others."""
+    from zlatko_amd import native
+    spec = _book_spec(777)
+    got = {}
+    for group in (1, 2, 4, 8):
+        e = native.Engine()
+        e.set_book(True)
+        e.set_book_group(group)
+        p = e.compile(spec, precision)
+        got[group] = p.execute(rows=True)
+        p.close()
+        e.close()
+    for group in (2, 4, 8):
+        for a, b in zip(got[group], got[1]):
+            for f in abi.PROFILE_FIELDS:
+                assert np.array_equal(a.get(f), b.get(f)), (group, f)
+            assert np.array_equal(a.exposure_rows(spec.num_paths), b.exposure_rows(spec.num_paths)), group
+
+
+def test_book_in_waves_matches_one_plan_fp64():
+    """A book compiled as waves of netting sets on the whole book's grids (ccr_hip_book_grids,
+    ccr_hip_set_book_grids; one shared workspace) reproduces the single book plan: same dates,
+    identical deflators and survivals, profiles within 1e-13 (the continuation GEMM's tiles
+    group the swaptions differently)."""
+    from zlatko_amd import native
+    spec = _book_spec(1024)
+    e = native.Engine()
+    e.set_book(True)
+    whole = e.compile(spec)
+    want = whole.execute()
+    whole.close()
+    rep, sim = native.book_grids(spec)
+    e.set_book_grids(rep, sim)
+    e.set_shared_workspace(True)
+    got = []
+    for w0 in range(0, len(spec.sets), 8):
+        sub = abi.Spec(spec.model, spec.sets[w0:w0 + 8], spec.base_grid, spec.num_paths, int(spec.config.seed),
+                       bool(spec.config.antithetic), spec.pfe_quantile, spec.name)
+        p = e.compile(sub)
+        got += p.execute()
+        p.close()
+    e.close()
+    for a, b in zip(got, want):
+        assert np.array_equal(a.get("times"), b.get("times"))
+        assert np.array_equal(a.get("mean_deflator"), b.get("mean_deflator"))
+        assert np.array_equal(a.get("mean_survival"), b.get("mean_survival"))
+        errs = compare.field_errors(a, b)
+        assert compare.worst(errs) <= 1e-13, errs
+
+
 def test_book_initial_margin_matches_oracle_fp64():
     """The IM proxy of every netting set of a book (each valued on every date of the book's
     simulation grid) against the oracle's book restatement."""

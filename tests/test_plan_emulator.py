@@ -127,10 +127,12 @@ def test_emulated_trade_statistics_match_oracle(emu, kw):
         assert e <= 1e-12, (m, e)
 
 
-def test_emulated_book_matches_oracle(emu):
-    """Book plans (union grids, shared scenarios) through the plan builder and kernel body on the
-    CPU against the oracle's book restatement."""
-    spec = W.book_case(num_paths=96)
+@pytest.mark.parametrize("case", ["book", "bermudans"])
+def test_emulated_book_matches_oracle(emu, case):
+    """Book plans (union grids, shared scenarios; sets with Bermudans sharing one states column)
+    through the plan builder and kernel body on the CPU against the oracle's book restatement."""
+    spec = W.book_case(num_paths=96) if case == "book" else W.c5(num_paths=64, num_sets=16, largest=40,
+                                                                   bermudan_share=0.15)
     want = oracle_lib.run_oracle_book(spec)
     cap = oracle_lib.book_grid_size(spec)
     outs, arr = abi.make_outs(len(spec.sets), cap, spec.num_paths, False)

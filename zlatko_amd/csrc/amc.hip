@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(kAmcChunk) amc_update_kernel(AmcArgs a, int k)
 // every chunk in order), so the grid stays small for plans with many continuation dates.
 __global__ void __launch_bounds__(kAmcChunk) amc_cmom1_kernel(AmcArgs a) {
   __shared__ double lds[kWaves * 3];
-  const int d = static_cast<int>(blockIdx.y);
+  const int d = a.cuniq[blockIdx.y];
   const CCR_CONST SetDev& S = toConst(a.sets)[a.cdates[2 * d]];
   const double* xs = a.states + S.state_off + static_cast<int64_t>(a.cdates[2 * d + 1]) * a.paths;
   double v[3] = {0, 0, 0};
@@ -226,8 +226,9 @@ __global__ void __launch_bounds__(kAmcChunk) amc_cmom1_kernel(AmcArgs a) {
 }
 
 __global__ void __launch_bounds__(256) amc_cfin1_kernel(AmcArgs a) {
-  const int d = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (d >= a.ncdates) return;
+  const int i = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= a.ncuniq) return;
+  const int d = a.cuniq[i];
   double s[3];
   chunkSum<3>(a.part, d, a.ncchunks, s);
   if ((threadIdx.x & 63) != 0) return;
@@ -238,7 +239,7 @@ __global__ void __launch_bounds__(256) amc_cfin1_kernel(AmcArgs a) {
 // itself, W = phi(z) / D, is formed where the continuation GEMM loads it).
 __global__ void __launch_bounds__(kAmcChunk) amc_design_kernel(AmcArgs a) {
   __shared__ double lds[kWaves * 10];
-  const int d = static_cast<int>(blockIdx.y);
+  const int d = a.cuniq[blockIdx.y];
   const CCR_CONST SetDev& S = toConst(a.sets)[a.cdates[2 * d]];
   const int j = a.cdates[2 * d + 1];
   const RegParams r = regParams(a.csum[3 * d], a.csum[3 * d + 1], a.csum[3 * d + 2], 3);
@@ -263,8 +264,9 @@ __global__ void __launch_bounds__(kAmcChunk) amc_design_kernel(AmcArgs a) {
 }
 
 __global__ void __launch_bounds__(256) amc_gram_kernel(AmcArgs a) {
-  const int d = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (d >= a.ncdates) return;
+  const int i = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= a.ncuniq) return;
+  const int d = a.cuniq[i];
   double s[10];
   chunkSum<10>(a.part, d, a.ncchunks, s);
   if ((threadIdx.x & 63) != 0) return;
@@ -291,9 +293,9 @@ __global__ void __launch_bounds__(256) amc_cgemm_kernel(AmcArgs a) {
   const double* xsrc = nullptr;
   RegParams rp;
   if (wd < t.nd) {
-    const int d = t.d0 + wd;
+    const int d = a.tcd[t.d_off + wd], m = a.cmap[d];
     xsrc = a.states + toConst(a.sets)[a.cdates[2 * d]].state_off + static_cast<int64_t>(a.cdates[2 * d + 1]) * a.paths;
-    rp = regParams(a.csum[3 * d], a.csum[3 * d + 1], a.csum[3 * d + 2], 3);
+    rp = regParams(a.csum[3 * m], a.csum[3 * m + 1], a.csum[3 * m + 2], 3);
   }
   const int col = sc < t.nb ? a.gcols[t.col_off + sc] : -1;
   const double* ssrc = col >= 0 ? a.strategy + static_cast<int64_t>(col) * a.paths : nullptr;
@@ -330,12 +332,12 @@ __global__ void __launch_bounds__(256) amc_cgemm_kernel(AmcArgs a) {
   if (c >= t.nb) return;
   const int cc = a.gcols[t.col_off + c];
   if (cc < 0) return;
-  const int date_rec = a.berms[t.b0 + c].date_rec;
+  const int date_rec = a.berms[a.gberm[t.col_off + c]].date_rec;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int row = 8 * w + i, dd = row >> 2, basis = row & 3;
     if (dd >= t.nd) continue;
-    const int r = date_rec + a.cdates[2 * (t.d0 + dd) + 1];
+    const int r = date_rec + a.cdates[2 * a.tcd[t.d_off + dd] + 1];
     a.gpart[(static_cast<int64_t>(kc) * a.nbdates + r) * 4 + basis] = acc[i];
   }
 }
@@ -360,9 +362,9 @@ __global__ void __launch_bounds__(256) amc_cgemm_narrow_kernel(AmcArgs a) {
   const double* xsrc = nullptr;
   RegParams rp;
   if (wd < t.nd) {
-    const int d = t.d0 + wd;
+    const int d = a.tcd[t.d_off + wd], m = a.cmap[d];
     xsrc = a.states + toConst(a.sets)[a.cdates[2 * d]].state_off + static_cast<int64_t>(a.cdates[2 * d + 1]) * a.paths;
-    rp = regParams(a.csum[3 * d], a.csum[3 * d + 1], a.csum[3 * d + 2], 3);
+    rp = regParams(a.csum[3 * m], a.csum[3 * m + 1], a.csum[3 * m + 2], 3);
   }
   const int col = sc < t.nb ? a.gcols[t.col_off + sc] : -1;
   const double* ssrc = col >= 0 ? a.strategy + static_cast<int64_t>(col) * a.paths : nullptr;
@@ -400,7 +402,7 @@ __global__ void __launch_bounds__(256) amc_cgemm_narrow_kernel(AmcArgs a) {
   __syncthreads();
   if (half == 1 || dd >= t.nd || b >= t.nb) return;
   if (a.gcols[t.col_off + b] < 0) return;
-  const int r = a.berms[t.b0 + b].date_rec + a.cdates[2 * (t.d0 + dd) + 1];
+  const int r = a.berms[a.gberm[t.col_off + b]].date_rec + a.cdates[2 * a.tcd[t.d_off + dd] + 1];
   for (int c = 0; c < 4; ++c) a.gpart[(static_cast<int64_t>(kc) * a.nbdates + r) * 4 + c] = acc[c] + red[tid][c];
 }
 
@@ -421,9 +423,10 @@ __global__ void __launch_bounds__(256) amc_csolve_kernel(AmcArgs a) {
   double xty[4] = {0, 0, 0, 0};
   for (int kc = 0; kc < a.nkc; ++kc)
     for (int i = 0; i < 4; ++i) xty[i] += a.gpart[(static_cast<int64_t>(kc) * a.nbdates + r) * 4 + i];
-  const RegParams rp = regParams(a.csum[3 * d], a.csum[3 * d + 1], a.csum[3 * d + 2], bm.degree);
+  const int m = a.cmap[d];  // the continuation date holding this date's moments and Gram
+  const RegParams rp = regParams(a.csum[3 * m], a.csum[3 * m + 1], a.csum[3 * m + 2], bm.degree);
   double coef[4];
-  regSolve(rp, a.gram + 10 * d, xty, xty[0], coef);
+  regSolve(rp, a.gram + 10 * m, xty, xty[0], coef);
   out[0] = rp.center;
   out[1] = rp.scale;
   for (int i = 0; i < 4; ++i) out[2 + i] = coef[i];
@@ -532,7 +535,7 @@ hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int
     if (hipError_t e = last()) return e;
   }
   if (a.ncdates > 0) {
-    const dim3 gd(a.ncchunks, a.ncdates), gds((a.ncdates + 3) / 4);
+    const dim3 gd(a.ncchunks, a.ncuniq), gds((a.ncuniq + 3) / 4);
     hipLaunchKernelGGL(amc_cmom1_kernel, gd, blk, 0, stream, a);
     hipLaunchKernelGGL(amc_cfin1_kernel, gds, dim3(256), 0, stream, a);
     hipLaunchKernelGGL(amc_design_kernel, gd, blk, 0, stream, a);

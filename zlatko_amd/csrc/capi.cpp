@@ -60,8 +60,12 @@ struct ccr_hip_ctx {
   double im_quantile = 0.0, im_mpor = 10.0 / 250.0;
   // Per-trade statistics for plans compiled while set.
   bool per_trade = false;
-  // Book mode (common grids, pathwise identical scenarios) for plans compiled while set.
+  // Book mode (common grids, pathwise identical scenarios) for plans compiled while set, and the
+  // grids of the whole book when it is compiled as several plans (empty: each plan's own union).
   bool book = false;
+  std::vector<double> book_rep, book_sim;
+  // Book plans: netting sets valued per workgroup by the book kernel (1: one set per workgroup).
+  int book_group = 4;
   // Executes replay one captured HIP graph per plan while set.
   bool use_graph = false;
   // Launches are being captured into a graph (no timing events recorded), and whether the last
@@ -311,16 +315,17 @@ void uploadStream(ccr_hip_ctx* ctx, ccr_hip_plan* plan, uint64_t seed, bool fp32
 // Workgroup size of the path kernel with the most waves resident per CU for this plan (its LDS
 // rows per path and its kernel specialisation's registers); ties go to the larger workgroup.
 int occupancyBlock(const HostPlan& h) {
-  const int ldsRows = h.g.max_ring + h.price_rows + h.g.max_slots + h.max_groups;
+  const int ldsRows = h.group > 1 ? h.g.max_slots + h.group * (h.g.max_ring + h.price_rows)
+                                  : h.g.max_ring + h.price_rows + h.g.max_slots + h.max_groups;
   const bool trades = h.max_groups > 0;
   int best = kMaxBlock, bestWaves = -1;
   for (int block = kMaxBlock; block >= 64; block -= 64) {
     int wg = 0;
     const hipError_t e = h.precision == CCR_FP32
                              ? pathOccupancy<float>(h.g.nrates, h.g.nassets, h.g.ncredits, !h.opts.empty(), ldsRows,
-                                                    block, &wg, trades)
+                                                    block, &wg, trades, h.group)
                              : pathOccupancy<double>(h.g.nrates, h.g.nassets, h.g.ncredits, !h.opts.empty(), ldsRows,
-                                                     block, &wg, trades);
+                                                     block, &wg, trades, h.group);
     check(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     const int waves = wg * (block / 64);
     if (waves > bestWaves) {
@@ -353,10 +358,15 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   check(hipSetDevice(ctx->device), "hipSetDevice");
   auto plan = std::make_unique<ccr_hip_plan>();
   plan->device = ctx->device;
+  const bool bookGrids = ctx->book && !ctx->book_rep.empty();
   plan->host = buildPlan(*generator, *config, base_grid, base_size, sets, num_sets, pfe_quantile, precision,
                          ctx->block > 0 ? ctx->block : kMaxBlock, ctx->per_trade, ctx->book,
-                         ctx->im_quantile > 0.0);
+                         ctx->im_quantile > 0.0, bookGrids ? &ctx->book_rep : nullptr,
+                         bookGrids ? &ctx->book_sim : nullptr);
   HostPlan& h = plan->host;
+  if (ctx->book && ctx->book_group > 1 && !ctx->per_trade && ctx->im_quantile <= 0.0 && h.opts.empty() &&
+      h.sets.size() > 1)
+    h.group = ctx->book_group;
   if (ctx->block == 0) assignBlocks(h, occupancyBlock(h));
   const bool fp32 = precision == CCR_FP32;
   const auto t0 = std::chrono::steady_clock::now();
@@ -446,6 +456,11 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.tiles = h.tiles.empty() ? nullptr : upload(ctx, plan.get(), h.tiles.data(), h.tiles.size());
     a.narrow = h.ntiles.empty() ? nullptr : upload(ctx, plan.get(), h.ntiles.data(), h.ntiles.size());
     a.gcols = h.gcols.empty() ? nullptr : upload(ctx, plan.get(), h.gcols.data(), h.gcols.size());
+    a.gberm = h.gberm.empty() ? nullptr : upload(ctx, plan.get(), h.gberm.data(), h.gberm.size());
+    a.tcd = h.tcd.empty() ? nullptr : upload(ctx, plan.get(), h.tcd.data(), h.tcd.size());
+    a.cmap = ncd > 0 ? upload(ctx, plan.get(), h.cmap.data(), h.cmap.size()) : nullptr;
+    a.cuniq = ncd > 0 ? upload(ctx, plan.get(), h.cuniq.data(), h.cuniq.size()) : nullptr;
+    a.ncuniq = static_cast<int32_t>(h.cuniq.size());
     execAlloc(ctx, plan.get(), a.gpart, static_cast<size_t>(nkc) * h.bdates.size() * 4 * sizeof(double));
     a.cont = plan->d_cont;
     a.exidx = plan->d_exidx;
@@ -499,6 +514,7 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   a.opts = static_cast<const Real*>(plan->d_opts);
   a.bermv = plan->d_bermv;
   a.states = nullptr;
+  a.states_set = -1;
   a.rows = static_cast<Real*>(plan->d_rows);
   a.partials = plan->d_partials;
   a.max_rep = h.max_rep;
@@ -509,6 +525,9 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   a.nassets = h.g.nassets;
   a.ncredits = h.g.ncredits;
   a.has_options = h.opts.empty() ? 0 : 1;
+  a.group = h.group;
+  a.blocks_per_set = h.sets.empty() ? 0 : h.sets[0].num_blocks;
+  a.nsets = static_cast<int32_t>(h.sets.size());
   a.im_rows = static_cast<Real*>(plan->d_im_rows);
   a.im_part = plan->d_im_part;
   a.im_dates = h.max_dates;
@@ -617,7 +636,9 @@ void launchPathStages(ccr_hip_ctx* ctx, ccr_hip_plan* plan, int blocks, const cc
   if (!h.berms.empty()) {
     PathArgs<Real> s = a;
     s.states = plan->d_states;
-    check(launchStates<Real>(s, blocks, h.block, ctx->stream), "states kernel launch");
+    s.states_set = h.states_set;
+    const int sblocks = h.states_set >= 0 ? h.sets[static_cast<size_t>(h.states_set)].num_blocks : blocks;
+    check(launchStates<Real>(s, sblocks, h.block, ctx->stream), "states kernel launch");
     bool allSurrogate = sur != nullptr;
     for (int si = 0; sur && si < static_cast<int>(h.sets.size()); ++si) {
       if (h.sets[static_cast<size_t>(si)].berm_end == h.sets[static_cast<size_t>(si)].berm_beg) continue;
@@ -995,6 +1016,41 @@ int32_t ccr_hip_set_book(ccr_hip_ctx* ctx, int32_t enable) {
   if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
   ctx->book = enable != 0;
   return CCR_OK;
+}
+
+int32_t ccr_hip_set_book_group(ccr_hip_ctx* ctx, int32_t sets_per_workgroup) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (sets_per_workgroup > 1 && !groupSupported(sets_per_workgroup))
+      fail(CCR_ERR_UNSUPPORTED, "book kernel: 1, 2, 4 or 8 netting sets per workgroup");
+    ctx->book_group = sets_per_workgroup;
+  });
+}
+
+int32_t ccr_hip_set_book_grids(ccr_hip_ctx* ctx, const double* reporting, int32_t reporting_size,
+                               const double* simulation, int32_t simulation_size) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (reporting_size <= 0 || simulation_size <= 0 || !reporting || !simulation) {
+      ctx->book_rep.clear();
+      ctx->book_sim.clear();
+      return;
+    }
+    ctx->book_rep.assign(reporting, reporting + reporting_size);
+    ctx->book_sim.assign(simulation, simulation + simulation_size);
+  });
+}
+
+int32_t ccr_hip_book_grids(const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
+                           double* reporting, int32_t reporting_capacity, int32_t* reporting_size, double* simulation,
+                           int32_t simulation_capacity, int32_t* simulation_size) {
+  return guarded(nullptr, [&] {
+    if (!sets || num_sets <= 0 || (!base_grid && base_size > 0)) invalid("null argument");
+    const TimeGrid base(std::vector<double>(base_grid, base_grid + std::max(0, base_size)));
+    const TimeGrid rep = bookReportingGrid(base, sets, num_sets);
+    writeGrid(rep, reporting, reporting_capacity, reporting_size);
+    writeGrid(bookSimulationGrid(rep, sets, num_sets), simulation, simulation_capacity, simulation_size);
+  });
 }
 
 int32_t ccr_book_totals(const ccr_exposure_out* outs, const ccr_netting_set* sets, int32_t num_sets,

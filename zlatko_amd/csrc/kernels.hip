@@ -106,12 +106,75 @@ ccr_path_kernel(PathArgs<Real> a) {
   }, onValue, onTrades);
 }
 
+// Book plans: G netting sets per workgroup (simulateGroup), one path per lane. Workgroup b (XCD
+// remapped) covers path block b % blocks_per_set of the sets [G (b / blocks_per_set), +G); each
+// set's statistics go to its own wave partials and exposure rows as in ccr_path_kernel. LDS per
+// path: the shared fixing slots, then each set's margin-call ring and price rows.
+// Waves per SIMD the book kernel's registers are held to: 5 up to four sets per workgroup (the
+// LDS of four sets' rings and the shared fixing slots allows five at 256 threads; measured on the
+// 500-set portfolio: path kernel 615 ms unconstrained at 3 waves, 499 ms at 4, 457 ms at 5), 4 for
+// eight (its LDS allows three).
+template <int G>
+constexpr int groupWaveHint() {
+  return G <= 4 ? 5 : 4;
+}
+
+template <typename Real, typename Sh, int G>
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(groupWaveHint<G>())))
+ccr_group_kernel(PathArgs<Real> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const CCR_CONST GlobalDev& g = *toConst(a.g);
+  const int rb = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  const int grp = rb / a.blocks_per_set, lb = rb % a.blocks_per_set;
+  const int s0 = grp * G, ns = a.nsets - s0 < G ? a.nsets - s0 : G;
+  const int bd = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
+  const int p = lb * bd + tid;
+  const bool active = p < g.num_paths;
+  const int pe = active ? p : g.num_paths - 1;  // inactive lanes replay a valid path, contribute nothing
+  const int waveLocal = lb * (bd >> 6) + (tid >> 6);
+  cptr<SetDev> sets = toConst(a.sets);
+  const int64_t q = g.antithetic ? (pe >> 1) : pe;
+  const Real* zbase = a.stream + q * sets[s0].stream_stride;
+  Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
+  simulateGroup<Real, Sh, G>(toConst(a.g), sets, s0, ns, pe, zbase, toConst(a.steps), toConst(a.dates),
+                             toConst(a.date_reals), toConst(a.bonds), toConst(a.fixed), toConst(a.fixstore),
+                             toConst(a.poly), a.bermv, a.ring, a.price_rows, a.slots, col, bd,
+                             [&](int k, int rr, const double* st9, double vp) {
+    const CCR_CONST SetDev& S = sets[s0 + k];
+    if (active) a.rows[S.rows_off + static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
+    double vals[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) vals[i] = active ? st9[i] : 0.0;
+    int idx;
+    const double s = waveSumMany<8>(vals, &idx);
+    double dva = 0.0;
+    if constexpr (!(Sh::kExact && Sh::NC == 0)) dva = waveSum(active ? st9[8] : 0.0);
+    const int lane = tid & 63;
+    double* part = a.partials + S.partial_off + static_cast<int64_t>(waveLocal) * a.max_rep * kStats;
+    if (lane < 9) part[rr * kStats + (lane < 8 ? idx : 8)] = lane < 8 ? s : dva;
+  });
+}
+
+bool groupSupported(int group) {
+  for (int gsz : kGroupSizes)
+    if (gsz == group) return true;
+  return false;
+}
+
 // States pass for American Monte Carlo: the domestic state and the deflator of every date of the
 // sets that hold Bermudan swaptions (simulateStates: the domestic rate only).
+// Book plans (states_set >= 0) run the blocks of one set: every set with Bermudans shares its
+// grid, its draws and so its states.
 template <typename Real>
 __global__ void __launch_bounds__(kMaxBlock) ccr_states_kernel(PathArgs<Real> a) {
-  const int b = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
-  cptr<SetDev> sp = toConst(a.sets) + toConst(a.block_set)[b];
+  int b = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  cptr<SetDev> sp;
+  if (a.states_set >= 0) {
+    sp = toConst(a.sets) + a.states_set;
+    b += sp->block_begin;
+  } else {
+    sp = toConst(a.sets) + toConst(a.block_set)[b];
+  }
   if (sp->state_off < 0) return;
   const int bd = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
   const int p = (b - sp->block_begin) * bd + tid;
@@ -493,8 +556,27 @@ hipError_t launchShape(const PathArgs<Real>& args, int blocks, int block, size_t
 // presence of European options.
 // Per-trade plans run the generic shape with options and per-trade values compiled in; their
 // LDS column adds one row per trade.
+template <typename Real, typename Sh>
+hipError_t launchGroupShape(const PathArgs<Real>& args, int block, size_t lds, hipStream_t stream) {
+  const int ngroups = (args.nsets + args.group - 1) / args.group;
+  const dim3 grid(static_cast<unsigned>(ngroups * args.blocks_per_set));
+  switch (args.group) {
+    case 2: hipLaunchKernelGGL((ccr_group_kernel<Real, Sh, 2>), grid, dim3(block), lds, stream, args); break;
+    case 4: hipLaunchKernelGGL((ccr_group_kernel<Real, Sh, 4>), grid, dim3(block), lds, stream, args); break;
+    case 8: hipLaunchKernelGGL((ccr_group_kernel<Real, Sh, 8>), grid, dim3(block), lds, stream, args); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
+  if (args.group > 1) {
+    const size_t lds = static_cast<size_t>(args.slots + args.group * (args.ring + args.price_rows)) * block * sizeof(Real);
+    return withShape(args.nrates, args.nassets, args.ncredits, [&](auto shape) {
+      return launchGroupShape<Real, decltype(shape)>(args, block, lds, stream);
+    });
+  }
   if (args.tstride > 0) {
     const size_t lds = static_cast<size_t>(args.ring + args.price_rows + args.slots + args.tstride / kTradeStats) *
                        block * sizeof(Real);
@@ -509,8 +591,18 @@ hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStre
 
 template <typename Real>
 hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu,
-                         bool trades) {
+                         bool trades, int group) {
   const size_t lds = static_cast<size_t>(ldsRows) * block * sizeof(Real);
+  if (group > 1)
+    return withShape(nrates, nassets, ncredits, [&](auto shape) -> hipError_t {
+      using Sh = decltype(shape);
+      switch (group) {
+        case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(wgPerCu, ccr_group_kernel<Real, Sh, 2>, block, lds);
+        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(wgPerCu, ccr_group_kernel<Real, Sh, 4>, block, lds);
+        case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(wgPerCu, ccr_group_kernel<Real, Sh, 8>, block, lds);
+        default: return hipErrorInvalidValue;
+      }
+    });
   if (trades)
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(wgPerCu, ccr_path_kernel<Real, GenericShape, true, true>, block,
                                                         lds);
@@ -523,8 +615,8 @@ hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, in
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(wgPerCu, ccr_path_kernel<Real, Sh, false>, block, lds);
   });
 }
-template hipError_t pathOccupancy<double>(int, int, int, bool, int, int, int*, bool);
-template hipError_t pathOccupancy<float>(int, int, int, bool, int, int, int*, bool);
+template hipError_t pathOccupancy<double>(int, int, int, bool, int, int, int*, bool, int);
+template hipError_t pathOccupancy<float>(int, int, int, bool, int, int, int*, bool, int);
 
 __global__ void __launch_bounds__(256) ccr_narrow_kernel(const double* in, float* out, int64_t n) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
@@ -560,10 +652,16 @@ __global__ void __launch_bounds__(256) ccr_domestic_normals_kernel(const SetDev*
 template <typename Real>
 hipError_t launchDomesticNormals(const PathArgs<Real>& args, const SetDev* hostSets, int nsets, Real* dom,
                                  hipStream_t stream) {
+  // One gather per distinct slice (the sets of a book share one).
   std::vector<int32_t> which;
+  std::vector<int64_t> seen;
   int maxDates = 0;
   for (int s = 0; s < nsets; ++s)
     if (hostSets[s].dom_off >= 0) {
+      bool dup = false;
+      for (int64_t o : seen) dup = dup || o == hostSets[s].dom_off;
+      if (dup) continue;
+      seen.push_back(hostSets[s].dom_off);
       which.push_back(s);
       maxDates = hostSets[s].num_dates > maxDates ? hostSets[s].num_dates : maxDates;
     }

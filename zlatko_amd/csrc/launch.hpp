@@ -40,6 +40,7 @@ struct PathArgs {
   const Real* opts;    // European option terms
   const double* bermv;      // American Monte Carlo: [continuation date][path] sum of the set's swaptions
   double* states;           // states-only pass: [set dates][path] x then D (null otherwise)
+  int32_t states_set;       // states pass of a book plan: the one set it runs (-1: every set's blocks)
   Real* rows;
   double* partials;
   int32_t max_rep;
@@ -48,6 +49,9 @@ struct PathArgs {
   int32_t slots;       // LDS rows of fixing states
   int32_t nrates, nassets, ncredits;  // model shape: selects the kernel specialisation
   int32_t has_options;                // European options present: selects the kernel variant
+  int32_t group;                      // book plans: netting sets per workgroup (0: one set per workgroup)
+  int32_t blocks_per_set;             // path blocks of every set (group launches)
+  int32_t nsets;
   int64_t state_len;                  // elements of x in `states` (D follows)
   // Initial-margin proxy (null when off): |V_j - V_j-1| rows per set from SetDev::im_off and
   // per-wave deflator sums from SetDev::im_part_off, im_dates dates per row block.
@@ -90,6 +94,10 @@ struct AmcArgs {
   const GemmTile* tiles;
   const GemmTile* narrow;     // narrow tiles
   const int32_t* gcols;       // strategy column per (tile, swaption), -1: none
+  const int32_t* gberm;       // global Bermudan per (tile, swaption)
+  const int32_t* tcd;         // continuation date per (tile, date)
+  const int32_t* cmap;        // continuation date holding the moments and Gram of each one
+  const int32_t* cuniq;       // the continuation dates that hold them (ncuniq)
   double* gpart;
   double* cont;
   int16_t* exidx;             // [Bermudan][path]
@@ -98,6 +106,7 @@ struct AmcArgs {
   int32_t nberms, ncols, ncdates, nbdates, nchunks, ntiles, nkc;
   int32_t nnarrow;
   int32_t ncchunks;           // continuation-date partials per date: ceil(nchunks / kContGroup)
+  int32_t ncuniq;
 };
 
 template <typename Real>
@@ -130,10 +139,14 @@ template <typename Real>
 hipError_t launchDomesticNormals(const PathArgs<Real>& args, const SetDev* hostSets, int nsets, Real* dom,
                                  hipStream_t stream);
 // Workgroups of the path kernel resident per CU (registers, LDS, workgroup slots) for a model
-// shape, the option variant, ldsRows LDS rows per thread and a workgroup size.
+// shape, the option variant, ldsRows LDS rows per thread and a workgroup size (group > 1: the
+// book kernel with that many netting sets per workgroup).
 template <typename Real>
 hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu,
-                         bool trades = false);
+                         bool trades = false, int group = 0);
+// Netting sets per workgroup the book kernel is compiled for (1 = the per-set kernel).
+constexpr int kGroupSizes[] = {2, 4, 8};
+bool groupSupported(int group);
 // The states-only pass of the path kernel (AMC): writes x and D of every date into args.states.
 template <typename Real>
 hipError_t launchStates(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);

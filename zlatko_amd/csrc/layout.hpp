@@ -142,13 +142,14 @@ struct BermDate {
   TermRange swap;     // value of the underlying swap at t_j
 };
 
-// One tile of the continuation GEMM: dates [d0, d0 + nd) of one epoch (continuation-date
-// indices), swaptions [b0, b0 + nb), and the strategy column of each swaption in the epoch
-// (cols[i] = exercise-value row, -1: no exercise left).
+// One tile of the continuation GEMM: nd dates of one epoch (their continuation-date indices are
+// tcd[d_off ..]) x nb swaptions (global indices gberm[col_off ..]) with the strategy column of
+// each in the epoch (gcols[col_off ..]: exercise-value row, -1: no exercise left). The swaptions
+// of a tile belong to netting sets valued on the same dates: one set, or in a book every set
+// with the same valued dates (the design rows of a date are then shared by all of them).
 struct GemmTile {
-  int32_t d0, nd, b0, nb;
-  int32_t col_off;    // first of nb entries in the plan's column table
-  int32_t pad[3];
+  int32_t d_off, nd, nb, col_off;
+  int32_t pad[4];
 };
 
 struct SetDev {

@@ -235,6 +235,395 @@ inline auto withShape(int nrates, int nassets, int ncredits, F&& f) {
   }
 }
 
+// ---------------------------------------------------------------- one path's market
+// Risk-factor state of one path (rates, deflator, assets, credits) and the normals of the next
+// step, loaded one step ahead: correlated drivers, then the integral normal of each rate
+// (domestic right after the drivers, foreign ones at foreign_off).
+template <typename Real, typename Sh>
+struct Market {
+  Real x[Sh::NR];
+  Real S[Sh::NA > 0 ? Sh::NA : 1], lnS[Sh::NA > 0 ? Sh::NA : 1], lnSc[Sh::NA > 0 ? Sh::NA : 1];
+  Real y[Sh::NC > 0 ? Sh::NC : 1], iy[Sh::NC > 0 ? Sh::NC : 1], Qs[Sh::NC > 0 ? Sh::NC : 1];
+  Real lnD, lnDc, D;  // FP32: Kahan sum of int r; FP64: running product
+  Real zn[Sh::kPre];
+  double prevQ[Sh::NC > 0 ? Sh::NC : 1];  // each credit's survival on the previous reporting date
+};
+
+template <typename Real, typename Sh>
+__host__ __device__ __forceinline__ void marketLoad(Market<Real, Sh>& mk, const CCR_CONST GlobalDev& g, const Real* zbase,
+                                                    int s) {
+  const int nR = Sh::kExact ? Sh::NR : g.nrates, nCorr = Sh::kExact ? Sh::kCorr : g.ncorr;
+  const Real* z = zbase + static_cast<int64_t>(s) * g.nz;
+#pragma unroll
+  for (int i = 0; i < Sh::kCorr; ++i) mk.zn[i] = i < nCorr ? z[i] : Real(0);
+#pragma unroll
+  for (int r = 0; r < Sh::NR; ++r) mk.zn[Sh::kCorr + r] = r < nR ? z[r == 0 ? nCorr : g.foreign_off + r - 1] : Real(0);
+}
+
+// kPrefetch: the normals of step j are loaded during step j - 1 (latency hidden behind one date's
+// valuation); without it they are loaded at the step itself (no registers held across dates).
+template <typename Real, typename Sh, bool kPrefetch = true>
+__host__ __device__ __forceinline__ void marketInit(Market<Real, Sh>& mk, const CCR_CONST GlobalDev& g, const Real* zbase,
+                                                    int numDates) {
+#pragma unroll
+  for (int r = 0; r < Sh::NR; ++r) mk.x[r] = 0;
+#pragma unroll
+  for (int i = 0; i < Sh::NA; ++i) {
+    mk.S[i] = static_cast<Real>(g.spot[i]);
+    mk.lnS[i] = 0;
+    mk.lnSc[i] = 0;
+  }
+#pragma unroll
+  for (int c = 0; c < Sh::NC; ++c) {
+    mk.y[c] = static_cast<Real>(g.cir_y0[c]);
+    mk.iy[c] = 0;
+    mk.Qs[c] = 1;
+    mk.prevQ[c] = 1.0;
+  }
+  mk.lnD = 0;
+  mk.lnDc = 0;
+  mk.D = 1;
+  if (kPrefetch && numDates > 1) marketLoad(mk, g, zbase, 0);
+}
+
+// The step t_{j-1} -> t_j (j >= 1) of every risk factor, from the prefetched normals (then the
+// next step's are loaded): w = L z (scenario_generator.cpp:118-124), exact Hull-White steps
+// (hull_white.cpp:40-68; foreign rates with the quanto drift), deflator D *= exp(-int r)
+// (scenario_generator.cpp:126-128), lognormal assets (lognormal_asset.cpp:17-22) with the carry of
+// a curve or a foreign rate, CIR++ bridge sub-steps and pathwise survival (cir_intensity.cpp:45-66).
+template <typename Real, typename Sh, bool kPrefetch = true>
+__host__ __device__ __forceinline__ void marketStep(Market<Real, Sh>& mk, const CCR_CONST GlobalDev& g, int j, int numDates,
+                                                    cptr<Real> steps, const Real* zbase, Real sgn) {
+  constexpr bool kDouble = std::is_same<Real, double>::value;
+  constexpr int NR = Sh::NR, NA = Sh::NA, NC = Sh::NC;
+  const int nR = Sh::kExact ? NR : g.nrates, nA = Sh::kExact ? NA : g.nassets, nC = Sh::kExact ? NC : g.ncredits;
+  cptr<Real> st = steps + static_cast<int64_t>(j) * kStepStride;
+  const Real* z = zbase + static_cast<int64_t>(j - 1) * g.nz;
+  Real zc[Sh::kPre];
+  if (!kPrefetch) marketLoad(mk, g, zbase, j - 1);
+#pragma unroll
+  for (int i = 0; i < Sh::kPre; ++i) zc[i] = sgn * mk.zn[i];
+  if (kPrefetch && j + 1 < numDates) marketLoad(mk, g, zbase, j);
+  // Correlated drivers at fixed factor slots: rates [0, kMaxRates), assets, credits. Row `slot`
+  // of the factor holds compact row `lim` (lower triangular: columns k <= lim).
+  Real w[Sh::kCorr];
+  {
+    cptr<double> chol = g.chol;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // Opaque per step: the factors are re-read through the scalar cache each step instead of
+    // being hoisted out of the date loop into registers.
+    asm volatile("" : "+s"(chol));
+#endif
+#pragma unroll
+    for (int i = 0; i < Sh::kCorr; ++i) {
+      const int slot = i < NR ? i : i < NR + NA ? kMaxRates + (i - NR) : kMaxRates + kMaxAssets + (i - NR - NA);
+      const int lim = i < NR ? i : i < NR + NA ? nR + (i - NR) : nR + nA + (i - NR - NA);
+      const bool used = i < NR ? i < nR : i < NR + NA ? i - NR < nA : i - NR - NA < nC;
+      Real sum = 0;
+      if (used) {
+#pragma unroll
+        for (int k = 0; k < Sh::kCorr; ++k)
+          if (k <= lim) sum += static_cast<Real>(chol[slot * kMaxCorr + k]) * zc[k];
+      }
+      w[i] = sum;
+    }
+  }
+  Real ir[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r < nR) {
+      cptr<Real> o = st + kStepRate + 8 * r;
+      const Real z2 = zc[Sh::kCorr + r];
+      const Real x0 = mk.x[r];
+      mk.x[r] = x0 * o[0] + o[1] * w[r] + o[5];
+      const Real ix = x0 * o[4] + o[2] * w[r] + o[3] * z2 + o[6];
+      ir[r] = ix + o[7];
+    } else {
+      ir[r] = 0;
+    }
+  }
+  if constexpr (kDouble) {
+    mk.D *= exp(-ir[0]);
+  } else {
+    const Real yk = ir[0] - mk.lnDc;  // Kahan-compensated integral of the short rate
+    const Real tk = mk.lnD + yk;
+    mk.lnDc = (tk - mk.lnD) - yk;
+    mk.lnD = tk;
+    mk.D = expr(-mk.lnD);
+  }
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    if (i < nA) {
+      cptr<Real> o = st + kStepAsset + 3 * i;
+      const int rr = g.asset_rate[i];
+      Real icarry = o[0];
+#pragma unroll
+      for (int r = 1; r < NR; ++r)
+        if (r == rr) icarry = ir[r];
+      const Real drift = ir[0] - icarry - o[2] + o[1] * w[NR + i];
+      if constexpr (kDouble) {
+        mk.S[i] = mk.S[i] * exp(drift);
+      } else {
+        const Real yk = drift - mk.lnSc[i];
+        const Real tk = mk.lnS[i] + yk;
+        mk.lnSc[i] = (tk - mk.lnS[i]) - yk;
+        mk.lnS[i] = tk;
+        mk.S[i] = static_cast<Real>(g.spot[i]) * expr(mk.lnS[i]);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (c < nC) {
+      cptr<Real> o = st + kStepCredit + kStepCreditStride * c;
+      const int sub = g.credit_sub[c];
+      const Real* ex = z + g.credit_off[c];
+      const Real kappa = static_cast<Real>(g.cir_kappa[c]), theta = static_cast<Real>(g.cir_theta[c]);
+      const Real xi = static_cast<Real>(g.cir_xi[c]), h = o[1];
+      Real remaining = o[0] * w[NR + NA + c];
+      Real yc = mk.y[c], iyc = mk.iy[c];
+      for (int k = 0; k < sub; ++k) {
+        Real dW = remaining;
+        if (k + 1 < sub) dW = o[3 + k] * remaining + o[3 + kMaxSub + k] * (sgn * ex[k]);
+        remaining -= dW;
+        const Real yp = yc > Real(0) ? yc : Real(0);
+        const Real yn = yc + kappa * (theta - yp) * h + xi * sqrtr(yp) * dW;
+        iyc += Real(0.5) * (yp + (yn > Real(0) ? yn : Real(0))) * h;
+        yc = yn;
+      }
+      mk.y[c] = yc;
+      mk.iy[c] = iyc;
+      mk.Qs[c] = expr(-iyc - o[2]);
+    }
+  }
+}
+
+// Fixing states recorded on date j: x of each listed rate into its slot row.
+template <typename Real, typename Sh>
+__host__ __device__ __forceinline__ void storeFixings(const Market<Real, Sh>& mk, int beg, int end, cptr<int32_t> fstore,
+                                                      Real* xs, int stride) {
+  for (int i = beg; i < end; ++i) {
+    const int r = fstore[2 * i], slot = fstore[2 * i + 1];
+    Real xv = mk.x[0];
+#pragma unroll
+    for (int k = 1; k < Sh::NR; ++k)
+      if (k == r) xv = mk.x[k];
+    xs[slot * stride] = xv;
+  }
+}
+
+// Value of the terms of one valuation record on its date: V = sum over rate groups of FX x (bond
+// polynomial or merged bond terms + fixed coupons) + deterministic-carry forwards (+ options).
+template <typename Real, typename Sh, bool kOptions>
+__host__ __device__ __forceinline__ Real valueRecord(const Market<Real, Sh>& mk, const CCR_CONST GlobalDev& g,
+                                                     const DateRec& vr, cptr<Real> vdr, cptr<double> poly,
+                                                     cptr<Real> bonds, cptr<Real> fixed, cptr<Real> opts,
+                                                     const Real* xs, int stride) {
+  constexpr int NR = Sh::NR, NA = Sh::NA;
+  const int nR = Sh::kExact ? NR : g.nrates, nA = Sh::kExact ? NA : g.nassets;
+  Real Vt = 0;
+  // Merged bonds (and their constants) of every rate as power series in its state: Horner
+  // in FP64 over the interleaved coefficient rows (highest power first, one coefficient per
+  // rate per row), the rates' chains side by side and kPolyBatch rows per batch of scalar loads.
+  // The coefficients are uniform: the date loop is uniform across the wavefront.
+  const int rows = vr.poly_rows;
+  bool usePoly = rows > 0;
+  if (usePoly) {
+    bool inside = true;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < nR) inside = inside && static_cast<double>(mk.x[r] < Real(0) ? -mk.x[r] : mk.x[r]) <= vdr[kDateXmax + r];
+    usePoly = allLanes(inside);
+  }
+  double hp[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) hp[r] = 0.0;
+  if (usePoly) {
+    cptr<double> pc = poly + vr.poly_beg;
+    for (int k = 0; k < rows; k += kPolyBatch) {
+      double c[kPolyBatch][NR];
+#pragma unroll
+      for (int u = 0; u < kPolyBatch; ++u)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) c[u][r] = r < nR ? pc[(k + u) * nR + r] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kPolyBatch; ++u)
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          if (r < nR) hp[r] = fmaU3(hp[r], static_cast<double>(mk.x[r]), c[u][r]);
+    }
+  }
+  // Fixed floating coupons of every rate (contiguous over the rates), two per batch so that
+  // their record loads, fixing-state reads and exponentials overlap; the plan pads the coupon
+  // table with one zero record, so the second record of a batch always exists.
+  Real fsum[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) fsum[r] = 0;
+  {
+    const int fe = vr.fix_beg[kMaxRates];
+    for (int i = vr.fix_beg[0]; i < fe; i += 2) {
+      cptr<Real> f = fixed + 4 * static_cast<int64_t>(i);
+      const Real a0 = f[0], b0 = f[1], c0 = f[2], a1 = f[4], b1 = f[5], c1 = f[6];
+      const int s0 = static_cast<int>(f[3]), s1 = static_cast<int>(f[7]);
+      const bool two = i + 1 < fe;
+      int r0 = 0, r1 = 0;
+#pragma unroll
+      for (int r = 1; r < NR; ++r) {
+        if (i >= vr.fix_beg[r]) r0 = r;
+        if (i + 1 >= vr.fix_beg[r]) r1 = r;
+      }
+      Real x0 = mk.x[0], x1 = mk.x[0];
+#pragma unroll
+      for (int r = 1; r < NR; ++r) {
+        if (r == r0) x0 = mk.x[r];
+        if (r == r1) x1 = mk.x[r];
+      }
+      const Real xk0 = xs[s0 * stride], xk1 = xs[s1 * stride];
+      const Real t0 = a0 * expTerm(-b0 * x0 - c0 * xk0);
+      const Real t1 = a1 * expTerm(-b1 * x1 - c1 * xk1);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        if (r == r0) fsum[r] += t0;
+        if (two && r == r1) fsum[r] += t1;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r < nR) {
+      const Real nx = -mk.x[r];
+      Real acc;
+      if (usePoly) {
+        acc = static_cast<Real>(hp[r]);
+      } else {
+        acc = vdr[kDateC0 + r];
+        for (int i = vr.bond_beg[r]; i < vr.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expTerm(bonds[2 * i + 1] * nx);
+      }
+      acc += fsum[r];
+      Real mult = 1;
+      if (r > 0) {
+        const int fa = g.fx_of[r];
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+          if (i == fa) mult = mk.S[i];
+      }
+      Vt += mult * acc;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+    if (i < nA) Vt += mk.S[i] * vdr[kDateLin + i];
+  if constexpr (NA > 0 && kOptions) {
+    for (int i = vr.opt_beg; i < vr.opt_end; ++i) {
+      cptr<Real> o = opts + kOptStride * static_cast<int64_t>(i);
+      const int oa = static_cast<int>(o[0]);
+      Real Sa = mk.S[0];
+#pragma unroll
+      for (int k = 1; k < NA; ++k)
+        if (k == oa) Sa = mk.S[k];
+      const Real df = o[4] * expr(-o[5] * mk.x[0]);
+      Vt += o[2] * blackFormula(o[1], Sa * o[6] / df, o[3], o[7], df);
+    }
+  }
+  return Vt;
+}
+
+// Per-path state a netting set carries from date to date: CSA collateral units and the previous
+// reporting date's discounted exposures (pathwise CVA / DVA trapezoids; the survivals of that
+// date are the market's, Market::prevQ).
+template <typename Real>
+struct SetPath {
+  Real units;
+  double prevDvp, prevDvn;
+};
+template <typename Real>
+__host__ __device__ __forceinline__ void setPathInit(SetPath<Real>& st) {
+  st.units = 0;
+  st.prevDvp = 0;
+  st.prevDvn = 0;
+}
+
+// After the sets' reports of a reporting date: its survivals become the previous ones.
+template <typename Real, typename Sh>
+__host__ __device__ __forceinline__ void marketReported(Market<Real, Sh>& mk) {
+#pragma unroll
+  for (int c = 0; c < Sh::NC; ++c) mk.prevQ[c] = static_cast<double>(mk.Qs[c]);
+}
+
+// Collateral (collateral.cpp:10-47, the margin-call value read from the set's ring) and, on a
+// reporting date, the 9 statistics of ExposureProfile::compute and pathwiseCva for a set whose
+// value before collateral on date j is V; emit(rep, stats[kStats], positive_exposure).
+template <typename Real, typename Sh, typename Emit>
+__host__ __device__ __forceinline__ Real collateralAndReport(const Market<Real, Sh>& mk, const CCR_CONST SetDev& S,
+                                                            const DateRec& rec, cptr<Real> dr, int j, Real V,
+                                                            SetPath<Real>& st, Real* ringV, Real* ringP, int stride,
+                                                            Emit&& emit) {
+  constexpr int NA = Sh::NA, NC = Sh::NC;
+  Real C = 0;
+  if (S.has_csa) {
+    Real price = 1;
+    if (S.sec_type == CCR_COLLATERAL_BOND) {
+      price = dr[kDateBond] * expr(-dr[kDateBond + 1] * mk.x[0]);
+    } else if (S.sec_type == CCR_COLLATERAL_ASSET) {
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+        if (i == S.csa_asset) price = mk.S[i];
+    }
+    const bool sec = S.sec_type != CCR_COLLATERAL_CASH;
+    if (rec.store_pos >= 0) {
+      ringV[rec.store_pos * stride] = V;
+      if (sec) ringP[rec.store_pos * stride] = price;
+    }
+    const Real vc = ringV[rec.call_pos * stride];
+    const Real pc = sec ? ringP[rec.call_pos * stride] : Real(1);
+    const Real keep = static_cast<Real>(S.keep), thc = static_cast<Real>(S.thr_cpty);
+    const Real tho = static_cast<Real>(S.thr_own);
+    Real target = 0;
+    if (vc > thc) target += vc - thc;
+    if (S.thr_own >= 0.0 && -vc > tho) target -= -vc - tho;
+    if (j == 0) st.units = target / (keep * pc);
+    Real diff = target - st.units * keep * pc;
+    diff = diff < Real(0) ? -diff : diff;
+    if (diff >= static_cast<Real>(S.mta)) st.units = target / (keep * pc);
+    C = st.units * price + static_cast<Real>(S.ia);
+  }
+  if (!(rec.flags & kReporting)) return C;
+  const double v = static_cast<double>(V) - static_cast<double>(C);
+  const double d = static_cast<double>(mk.D);
+  const double vp = v > 0.0 ? v : 0.0, vn = v < 0.0 ? v : 0.0;
+  const double dvp = d * vp, dvn = d * (-vn);
+  double qc = 1.0, qo = 1.0, pqc = 1.0, pqo = 1.0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (c == S.cpty) {
+      qc = static_cast<double>(mk.Qs[c]);
+      pqc = mk.prevQ[c];
+    }
+    if (c == S.own) {
+      qo = static_cast<double>(mk.Qs[c]);
+      pqo = mk.prevQ[c];
+    }
+  }
+  const int rr = rec.rep;
+  double st9[kStats];
+  st9[0] = v;
+  st9[1] = vp;
+  st9[2] = vn;
+  st9[3] = dvp;
+  st9[4] = d * vn;
+  st9[5] = d;
+  st9[6] = qc;
+  st9[7] = rr > 0 ? 0.5 * (st.prevDvp + dvp) * (pqc - qc) : 0.0;
+  st9[8] = rr > 0 ? 0.5 * (st.prevDvn + dvn) * (pqo - qo) : 0.0;
+  st.prevDvp = dvp;
+  st.prevDvn = dvn;
+  emit(rr, st9, vp);
+  return C;
+}
+
+// The per-set kernel's body keeps its own copy of the market step and valuation above (inlined
+// by hand): its register allocation is tuned to five waves per SIMD and the shared helpers, used
+// by the book kernel below, change it.
 // col: this path's column of per-path scratch (LDS on the GPU), stride: distance between rows;
 // rows [0, ring) hold the margin-call values, [ring, ring + priceRows) the collateral prices,
 // then the fixing states. emit(rep, stats[kStats], positive_exposure) is called on every
@@ -598,6 +987,64 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     prevQo = qo;
     emit(rr, st9, vp);
     if constexpr (kTrades) onTrades(rr, V, D, tv, stride);
+  }
+}
+
+// Book plans, G netting sets per lane: the sets of a book share their grid and their draws, so
+// one lane steps its path's market once per date and values every set of its group on it. Each
+// set keeps its own collateral units, CVA / DVA trapezoid state, margin-call ring (LDS rows
+// [slots + g (ring + priceRows), ...)) and Bermudan rows; the fixing states (the first `slots`
+// rows) are shared, recorded from the group's first set's lists (a book's lists are the same).
+// sp[g], g < ns: the group's sets; emit(g, rep, stats, positive_exposure).
+template <typename Real, typename Sh, int G, typename Emit>
+__host__ __device__ inline void simulateGroup(cptr<GlobalDev> gp, cptr<SetDev> sets, int s0, int ns, int pe,
+                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
+                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
+                                              cptr<int32_t> fstore, cptr<double> poly, const double* bermv, int ring,
+                                              int priceRows, int slots, Real* col, int stride, Emit&& emit) {
+  const CCR_CONST GlobalDev& g = *gp;
+  const CCR_CONST SetDev& S0 = sets[s0];
+  Real* xs = col;
+  const Real sgn = (g.antithetic && (pe & 1)) ? Real(-1) : Real(1);
+  // The book's shared steps (every set of a book has the same step rows).
+  cptr<Real> st0 = steps + static_cast<int64_t>(S0.date_off) * kStepStride;
+  Market<Real, Sh> mk;
+  marketInit<Real, Sh, false>(mk, g, zbase, S0.num_dates);
+  SetPath<Real> stp[G];
+  int bcount[G];
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    setPathInit(stp[k]);
+    bcount[k] = 0;
+  }
+  for (int j = 0; j < S0.num_dates; ++j) {
+    if (j > 0) marketStep<Real, Sh, false>(mk, g, j, S0.num_dates, st0, zbase, sgn);
+    {
+      const DateRec r0 = dates[S0.date_off + j];
+      storeFixings(mk, r0.fixstore_beg, r0.fixstore_end, fstore + 2 * S0.fixstore_off, xs, stride);
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      if (k >= ns) continue;
+      const CCR_CONST SetDev& S = sets[s0 + k];
+      const DateRec rec = dates[S.date_off + j];
+      if (!(rec.flags & kValued) && !S.has_csa) continue;
+      cptr<Real> dr = dreal + (static_cast<int64_t>(S.date_off) + j) * kDateRealStride;
+      Real V = 0;
+      if (rec.flags & kValued) {
+        V = valueRecord<Real, Sh, false>(mk, g, rec, dr, poly + S.poly_off, bonds + 2 * static_cast<int64_t>(S.bond_off),
+                                         fixed + 4 * static_cast<int64_t>(S.fixed_off), nullptr, xs, stride);
+        if (bermv && S.berm_end > S.berm_beg) {
+          V += static_cast<Real>(bermv[(static_cast<int64_t>(S.cd_base) + bcount[k]) * g.num_paths + pe]);
+          ++bcount[k];
+        }
+      }
+      Real* ringV = col + (slots + k * (ring + priceRows)) * stride;
+      collateralAndReport(mk, S, rec, dr, j, V, stp[k], ringV, ringV + ring * stride, stride,
+                          [&](int rr, const double* st9, double vp) { emit(k, rr, st9, vp); });
+    }
+    // A book's sets report on the same dates (the book's reporting grid).
+    if (dates[S0.date_off + j].flags & kReporting) marketReported(mk);
   }
 }
 

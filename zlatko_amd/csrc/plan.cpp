@@ -276,6 +276,63 @@ void collect(const Trade& tr, const Models& m, const TimeGrid& grid, Collector& 
   }
 }
 
+// The valued dates of a netting set on `sim` (reporting dates of `rep`, margin-call dates of a
+// CSA, or every date) and, per (rate, fixing date), the last valued date whose fixed floating
+// coupons read that fixing (the condition of collectSwapPeriods' fixed branch).
+void fixingUses(const std::vector<Trade>& trades, const ccr_netting_set& ns, const TimeGrid& rep, const TimeGrid& sim,
+                bool value_all, std::map<std::pair<int, int>, int>& lastUse) {
+  const int T = static_cast<int>(sim.size());
+  std::vector<char> valued(static_cast<std::size_t>(T), value_all ? 1 : 0);
+  for (double t : rep.t)
+    if (const auto j = sim.find(t)) valued[*j] = 1;
+  if (ns.has_collateral)
+    for (int j = 0; j < T; ++j)
+      valued[sim.indexAtOrBefore(std::max(sim[static_cast<std::size_t>(j)] - ns.collateral.margin_period_of_risk, 0.0))] = 1;
+  for (int j = 0; j < T; ++j) {
+    if (!valued[static_cast<std::size_t>(j)]) continue;
+    const double t = sim[static_cast<std::size_t>(j)];
+    for (const Trade& tr : trades) {
+      if (tr.kind != CCR_TRADE_SWAP || t >= tr.maturity - kPaidTolerance) continue;
+      const std::vector<double>& fl = tr.fsched.empty() ? tr.sched : tr.fsched;
+      for (std::size_t i = 1; i < fl.size(); ++i) {
+        const double ts = fl[i - 1], te = fl[i];
+        if (te <= t + kPaidTolerance || ts >= t - kPaidTolerance) continue;
+        const auto key = std::make_pair(tr.rate, static_cast<int>(sim.indexAtOrBefore(ts)));
+        auto it = lastUse.find(key);
+        if (it == lastUse.end() || it->second < j) lastUse[key] = j;
+      }
+    }
+  }
+}
+
+// Fixing slots by interval colouring of [fixing date, last use]: slotOf (rate, k) and, per date
+// k, the (rate, slot) pairs stored there; returns the slot count.
+int assignSlots(const std::map<std::pair<int, int>, int>& lastUse, int T, std::map<std::pair<int, int>, int>& slotOf,
+                std::vector<std::vector<std::pair<int, int>>>& storeAt) {
+  std::vector<int> slotFreeAfter;  // last use of the interval currently in each slot
+  storeAt.assign(static_cast<std::size_t>(T), {});
+  std::vector<std::tuple<int, int, int>> intervals;  // (k, rate, last)
+  for (const auto& kv : lastUse) intervals.emplace_back(kv.first.second, kv.first.first, kv.second);
+  std::sort(intervals.begin(), intervals.end());
+  for (const auto& iv : intervals) {
+    const int k = std::get<0>(iv), r = std::get<1>(iv), last = std::get<2>(iv);
+    int slot = -1;
+    for (std::size_t q = 0; q < slotFreeAfter.size(); ++q)
+      if (slotFreeAfter[q] < k) {
+        slot = static_cast<int>(q);
+        break;
+      }
+    if (slot < 0) {
+      slot = static_cast<int>(slotFreeAfter.size());
+      slotFreeAfter.push_back(0);
+    }
+    slotFreeAfter[static_cast<std::size_t>(slot)] = last;
+    slotOf[{r, k}] = slot;
+    storeAt[static_cast<std::size_t>(k)].emplace_back(r, slot);
+  }
+  return static_cast<int>(slotFreeAfter.size());
+}
+
 // Bound on |x_r(t)| used by the bond polynomial: |mean| + kPolySigmas standard deviations of the
 // Gaussian Hull-White state (constant drift `mu` under the simulation measure).
 double stateBound(const HullWhite& hw, double mu, double t) {
@@ -344,9 +401,72 @@ void generateStream(uint64_t seed, int64_t n, double* out) {
   for (int64_t i = 0; i < n; ++i) out[i] = normal(engine);
 }
 
+TimeGrid bookReportingGrid(const TimeGrid& base, const ccr_netting_set* sets, int32_t num_sets) {
+  std::vector<double> r;
+  for (int s = 0; s < num_sets; ++s) {
+    const TimeGrid g = reportingGrid(base, sets[s]);
+    r.insert(r.end(), g.t.begin(), g.t.end());
+  }
+  return TimeGrid(r);
+}
+
+TimeGrid bookSimulationGrid(const TimeGrid& rep, const ccr_netting_set* sets, int32_t num_sets) {
+  std::vector<double> v;
+  for (int s = 0; s < num_sets; ++s) {
+    const TimeGrid g = simulationGrid(rep, sets[s]);
+    v.insert(v.end(), g.t.begin(), g.t.end());
+  }
+  return TimeGrid(v);
+}
+
+namespace {
+
+// Continuation GEMM tiles of a group of swaptions (global indices, all valued on the dates vj
+// with continuation dates vcd): runs of dates on which every swaption of the group has the same
+// next exercise (an epoch), kGemmDates dates (kNarrowDates in narrow tiles, for groups of at most
+// kNarrowCols) x up to kGemmCols of the swaptions that still have an exercise ahead.
+void buildTiles(HostPlan& plan, const std::vector<int>& group, const std::vector<int>& vj,
+                const std::vector<int>& vcd) {
+  if (group.empty() || vj.empty()) return;
+  auto next = [&](int b, int j) { return plan.bdates[static_cast<std::size_t>(plan.berms[static_cast<std::size_t>(b)].date_rec + j)].next_k; };
+  auto sameEpoch = [&](int ja, int jb) {
+    for (int b : group)
+      if (next(b, ja) != next(b, jb)) return false;
+    return true;
+  };
+  const bool narrow = static_cast<int>(group.size()) <= kNarrowCols;
+  const std::size_t maxDates = static_cast<std::size_t>(narrow ? kNarrowDates : kGemmDates);
+  std::size_t i0 = 0;
+  while (i0 < vj.size()) {
+    std::size_t i1 = i0 + 1;
+    while (i1 < vj.size() && i1 - i0 < maxDates && sameEpoch(vj[i0], vj[i1])) ++i1;
+    std::vector<int> live;
+    for (int b : group)
+      if (next(b, vj[i0]) < plan.berms[static_cast<std::size_t>(b)].ne) live.push_back(b);
+    for (std::size_t c0 = 0; c0 < live.size(); c0 += kGemmCols) {
+      GemmTile tile{};
+      tile.d_off = static_cast<int32_t>(plan.tcd.size());
+      tile.nd = static_cast<int32_t>(i1 - i0);
+      for (std::size_t i = i0; i < i1; ++i) plan.tcd.push_back(vcd[i]);
+      tile.nb = static_cast<int32_t>(std::min<std::size_t>(kGemmCols, live.size() - c0));
+      tile.col_off = static_cast<int32_t>(plan.gcols.size());
+      for (std::size_t c = c0; c < c0 + static_cast<std::size_t>(tile.nb); ++c) {
+        const BermDev& bm = plan.berms[static_cast<std::size_t>(live[c])];
+        plan.gcols.push_back(bm.ev_off + next(live[c], vj[i0]));
+        plan.gberm.push_back(live[c]);
+      }
+      (narrow ? plan.ntiles : plan.tiles).push_back(tile);
+    }
+    i0 = i1;
+  }
+}
+
+}  // namespace
+
 HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
-                   int32_t precision, int32_t block, bool per_trade, bool book, bool value_all) {
+                   int32_t precision, int32_t block, bool per_trade, bool book, bool value_all,
+                   const std::vector<double>* book_rep, const std::vector<double>* book_sim) {
   if (!sets || num_sets <= 0) invalid("no netting sets");
   if (cfg.num_paths <= 0) invalid("ScenarioGenerator::generate: numPaths must be positive");
   if (cfg.num_paths > (int64_t(1) << 31) - 1) fail(CCR_ERR_UNSUPPORTED, "at most 2^31 - 1 paths");
@@ -399,19 +519,38 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
   // (src/app/analyses.cpp:1690-1709, 1727-1766).
   TimeGrid bookRep, bookSim;
   if (book) {
-    std::vector<double> r, v;
-    for (int s = 0; s < num_sets; ++s) {
-      const TimeGrid g = reportingGrid(base, sets[s]);
-      r.insert(r.end(), g.t.begin(), g.t.end());
+    bookRep = book_rep ? TimeGrid(*book_rep) : bookReportingGrid(base, sets, num_sets);
+    bookSim = book_sim ? TimeGrid(*book_sim) : bookSimulationGrid(bookRep, sets, num_sets);
+    // Grids of a larger book: every date this plan's sets need must be on them.
+    if (book_rep || book_sim) {
+      const TimeGrid ownRep = bookReportingGrid(base, sets, num_sets);
+      for (double t : ownRep.t)
+        if (!bookRep.find(t)) invalid("book grids: a reporting date of the netting sets is missing");
+      for (double t : bookSimulationGrid(bookRep, sets, num_sets).t)
+        if (!bookSim.find(t)) invalid("book grids: a simulation date of the netting sets is missing");
     }
-    bookRep = TimeGrid(r);
+  }
+  // Book plans: one states pass and one moment / Gram per date serve every set with Bermudans.
+  int64_t bookStateOff = -1, bookDomOff = -1;
+  std::map<int, int32_t> bookCd;  // simulation date -> continuation date holding its moments
+  std::map<std::pair<int, int>, int> bookSlotOf;
+  std::vector<std::vector<std::pair<int, int>>> bookStoreAt;
+  int bookSlots = 0;
+  if (book) {
+    std::map<std::pair<int, int>, int> uses;
     for (int s = 0; s < num_sets; ++s) {
-      const TimeGrid g = simulationGrid(bookRep, sets[s]);
-      v.insert(v.end(), g.t.begin(), g.t.end());
+      std::vector<Trade> trades;
+      for (int i = 0; i < sets[s].num_trades && sets[s].trades; ++i) {
+        Trade t = makeTrade(sets[s].trades[i], m);
+        if (t.kind != CCR_TRADE_BERMUDAN_SWAPTION) trades.push_back(std::move(t));
+      }
+      fixingUses(trades, sets[s], bookRep, bookSim, value_all, uses);
     }
-    bookSim = TimeGrid(v);
+    bookSlots = assignSlots(uses, static_cast<int>(bookSim.size()), bookSlotOf, bookStoreAt);
   }
   int64_t maxStride = 0;
+  // Book plans: swaptions of sets valued on the same dates (their valued-date flags), and those dates.
+  std::map<std::vector<char>, std::pair<std::vector<int>, std::vector<int>>> bookGroups;
   for (int s = 0; s < num_sets; ++s) {
     const ccr_netting_set& ns = sets[s];
     if (ns.num_trades <= 0 || !ns.trades) invalid("ExposureEngine::run: empty netting set");
@@ -550,33 +689,17 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
         has[static_cast<std::size_t>(j)] = 1;
       }
     }
-    // Slots: interval colouring of [fixing date, last use].
+    // Fixing slots: the set's own, or in a book the book's (the fixing states are those of the
+    // shared scenarios, one LDS column per path serving every set of a workgroup).
     std::map<std::pair<int, int>, int> slotOf;
-    std::vector<int> slotFreeAfter;  // last use of the interval currently in each slot
-    std::vector<std::vector<std::pair<int, int>>> storeAt(static_cast<std::size_t>(T));
-    for (const auto& kv : lastUse) {  // ordered by (rate, k); re-sort by k
-      (void)kv;
+    std::vector<std::vector<std::pair<int, int>>> storeAt;
+    if (book) {
+      slotOf = bookSlotOf;
+      storeAt = bookStoreAt;
+      sd.num_slots = bookSlots;
+    } else {
+      sd.num_slots = assignSlots(lastUse, T, slotOf, storeAt);
     }
-    std::vector<std::tuple<int, int, int>> intervals;  // (k, rate, last)
-    for (const auto& kv : lastUse) intervals.emplace_back(kv.first.second, kv.first.first, kv.second);
-    std::sort(intervals.begin(), intervals.end());
-    for (const auto& iv : intervals) {
-      const int k = std::get<0>(iv), r = std::get<1>(iv), last = std::get<2>(iv);
-      int slot = -1;
-      for (std::size_t q = 0; q < slotFreeAfter.size(); ++q)
-        if (slotFreeAfter[q] < k) {
-          slot = static_cast<int>(q);
-          break;
-        }
-      if (slot < 0) {
-        slot = static_cast<int>(slotFreeAfter.size());
-        slotFreeAfter.push_back(0);
-      }
-      slotFreeAfter[static_cast<std::size_t>(slot)] = last;
-      slotOf[{r, k}] = slot;
-      storeAt[static_cast<std::size_t>(k)].emplace_back(r, slot);
-    }
-    sd.num_slots = static_cast<int32_t>(slotFreeAfter.size());
 
     // Per-trade plans: every trade of the set gets its own valuation records (its terms alone);
     // the set's records then carry none, and the kernel nets the trades' values in trade order.
@@ -808,20 +931,41 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       return tr;
     };
     sd.berm_beg = static_cast<int32_t>(plan.berms.size());
-    sd.state_off = nB > 0 ? plan.state_len : -1;
-    sd.dom_off = nB > 0 ? plan.dom_len : -1;
-    if (nB > 0) plan.dom_len += static_cast<int64_t>(T - 1) * 2 * (cfg.antithetic ? (static_cast<int64_t>(P) + 1) / 2 : P);
-    if (nB > 0) plan.state_len += static_cast<int64_t>(T) * P;
+    sd.state_off = -1;
+    sd.dom_off = -1;
+    if (nB > 0 && book && bookStateOff >= 0) {
+      sd.state_off = bookStateOff;  // same grid, same draws: the states of the book's first such set
+      sd.dom_off = bookDomOff;
+    } else if (nB > 0) {
+      sd.state_off = plan.state_len;
+      sd.dom_off = plan.dom_len;
+      plan.dom_len += static_cast<int64_t>(T - 1) * 2 * (cfg.antithetic ? (static_cast<int64_t>(P) + 1) / 2 : P);
+      plan.state_len += static_cast<int64_t>(T) * P;
+      if (book) {
+        bookStateOff = sd.state_off;
+        bookDomOff = sd.dom_off;
+        plan.states_set = s;
+      }
+    }
     // Continuation regressions run on every valued date of a set with Bermudans.
     std::vector<int32_t> cdOf(static_cast<std::size_t>(T), -1);
     sd.cd_base = static_cast<int32_t>(plan.cdates.size() / 2);
     if (nB > 0)
       for (int j = 0; j < T; ++j)
         if (valued[static_cast<std::size_t>(j)]) {
-          cdOf[static_cast<std::size_t>(j)] = static_cast<int32_t>(plan.cdates.size() / 2);
+          const int32_t cd = static_cast<int32_t>(plan.cdates.size() / 2);
+          cdOf[static_cast<std::size_t>(j)] = cd;
           plan.cdates.push_back(s);
           plan.cdates.push_back(j);
           plan.ctimes.push_back(sim[static_cast<std::size_t>(j)]);
+          int32_t canon = cd;
+          if (book) {
+            const auto it = bookCd.find(j);
+            if (it == bookCd.end()) bookCd[j] = cd;
+            else canon = it->second;
+          }
+          plan.cmap.push_back(canon);
+          if (canon == cd) plan.cuniq.push_back(cd);
           // The swaptions with a value on this date: an exercise still ahead (continuation), or
           // an underlying with cash flows left that exercised paths hold. The others are 0 on
           // every path and the valuation skips them.
@@ -893,47 +1037,20 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       plan.berms.push_back(bd);
     }
     sd.berm_end = static_cast<int32_t>(plan.berms.size());
-    // Continuation GEMM tiles: runs of valued dates with the same next exercise for every swaption
-    // (an epoch), cut into kGemmDates dates x kGemmCols swaptions (kNarrowDates dates x all
-    // swaptions in narrow tiles for sets with at most kNarrowCols); epochs past every swaption's
-    // last exercise need no regression.
+    // Continuation GEMM tiles of the set's swaptions (book plans tile all sets together below).
     if (nB > 0) {
-      std::vector<int> vdates;
+      std::vector<int> group, vj, vcd;
+      for (int b = 0; b < nB; ++b) group.push_back(sd.berm_beg + b);
       for (int j = 0; j < T; ++j)
-        if (valued[static_cast<std::size_t>(j)]) vdates.push_back(j);
-      auto sameEpoch = [&](int ja, int jb) {
-        for (int b = 0; b < nB; ++b)
-          if (bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(ja)] !=
-              bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(jb)])
-            return false;
-        return true;
-      };
-      std::size_t i0 = 0;
-      while (i0 < vdates.size()) {
-        std::size_t i1 = i0 + 1;
-        const bool narrow = nB <= kNarrowCols;
-        const std::size_t maxDates = static_cast<std::size_t>(narrow ? kNarrowDates : kGemmDates);
-        while (i1 < vdates.size() && i1 - i0 < maxDates && sameEpoch(vdates[i0], vdates[i1])) ++i1;
-        const int j0 = vdates[i0];
-        bool any = false;
-        for (int b = 0; b < nB; ++b)
-          any = any || bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(j0)] <
-                           static_cast<int>(berms[static_cast<std::size_t>(b)].exercise.size());
-        for (int b0 = 0; any && b0 < nB; b0 += kGemmCols) {
-          GemmTile tile{};
-          tile.d0 = sd.cd_base + static_cast<int32_t>(i0);
-          tile.nd = static_cast<int32_t>(i1 - i0);
-          tile.b0 = sd.berm_beg + b0;
-          tile.nb = std::min(kGemmCols, nB - b0);
-          tile.col_off = static_cast<int32_t>(plan.gcols.size());
-          for (int b = b0; b < b0 + tile.nb; ++b) {
-            const BermDev& bm = plan.berms[static_cast<std::size_t>(sd.berm_beg + b)];
-            const int k = bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(j0)];
-            plan.gcols.push_back(k < bm.ne ? bm.ev_off + k : -1);
-          }
-          (narrow ? plan.ntiles : plan.tiles).push_back(tile);
+        if (valued[static_cast<std::size_t>(j)]) {
+          vj.push_back(j);
+          vcd.push_back(cdOf[static_cast<std::size_t>(j)]);
         }
-        i0 = i1;
+      if (book) {
+        bookGroups[valued].first.insert(bookGroups[valued].first.end(), group.begin(), group.end());
+        bookGroups[valued].second = vj;
+      } else {
+        buildTiles(plan, group, vj, vcd);
       }
     }
     plan.revaluations += static_cast<int64_t>(ns.num_trades) * P * R;
@@ -944,6 +1061,12 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     g.max_ring = std::max(g.max_ring, ring);
     if (ns.has_collateral && ns.collateral.type != CCR_COLLATERAL_CASH) plan.price_rows = std::max(plan.price_rows, ring);
     g.max_slots = std::max(g.max_slots, sd.num_slots);
+  }
+  // Book tiles: one group per valued-date pattern, its dates' rows from the shared moments.
+  for (const auto& kv : bookGroups) {
+    std::vector<int> vcd;
+    for (int j : kv.second.second) vcd.push_back(bookCd.at(j));
+    buildTiles(plan, kv.second.first, kv.second.second, vcd);
   }
   const int64_t blocksPaths = cfg.antithetic ? (static_cast<int64_t>(P) + 1) / 2 : P;
   plan.stream_len = blocksPaths * maxStride;

@@ -53,8 +53,16 @@ struct HostPlan {
   std::vector<LiveRec> clive;       // per continuation date: the swaptions with a value
   std::vector<int32_t> clive_rng;   // [begin, end) into clive per continuation date
   std::vector<GemmTile> tiles;      // continuation GEMM tiles
-  std::vector<GemmTile> ntiles;     // narrow continuation GEMM tiles (sets with few Bermudans)
+  std::vector<GemmTile> ntiles;     // narrow continuation GEMM tiles (groups of few Bermudans)
   std::vector<int32_t> gcols;       // strategy column per (tile, swaption), -1: none
+  std::vector<int32_t> gberm;       // global Bermudan index per (tile, swaption)
+  std::vector<int32_t> tcd;         // continuation date per (tile, date)
+  // Continuation dates whose state column is another's (book plans: every set's date j has the
+  // same x and D): cmap[cd] is the continuation date holding the moments and Gram of cd; cuniq
+  // lists those (cmap[cd] == cd).
+  std::vector<int32_t> cmap, cuniq;
+  int32_t states_set = -1;          // book plans: the one set whose blocks run the states pass
+  int32_t group = 0;                // book plans: netting sets per workgroup of the book kernel (0: one)
   // Per-trade plans: one valuation record (terms of that trade alone) per (set, date, trade),
   // [tdate_off + j G + i], and its reals; the set's own records then carry no terms.
   std::vector<DateRec> tdates;
@@ -66,10 +74,19 @@ struct HostPlan {
 TimeGrid reportingGrid(const TimeGrid& base, const ccr_netting_set& set);
 TimeGrid simulationGrid(const TimeGrid& base, const ccr_netting_set& set);
 
+// book: every set on the book's union grids (bookReportingGrid / bookSimulationGrid of the plan's
+// sets, or book_rep / book_sim when given -- the grids of a whole book compiled as several plans,
+// which must hold every set's own dates).
 HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
                    int32_t precision, int32_t block, bool per_trade = false, bool book = false,
-                   bool value_all = false);
+                   bool value_all = false, const std::vector<double>* book_rep = nullptr,
+                   const std::vector<double>* book_sim = nullptr);
+
+// The portfolio book's grids (src/app/analyses.cpp:1690-1709): the union of the sets' reporting
+// grids, and the union of their simulation grids built on that reporting union.
+TimeGrid bookReportingGrid(const TimeGrid& base, const ccr_netting_set* sets, int32_t num_sets);
+TimeGrid bookSimulationGrid(const TimeGrid& rep, const ccr_netting_set* sets, int32_t num_sets);
 
 // Workgroup size of the path kernel: the workgroups of every set, their waves and the offsets of
 // the per-wave partial sums.

@@ -47,6 +47,9 @@ def lib():
         L.ccr_hip_initial_margin.argtypes = [P, P, I32, P, I32, P]
         L.ccr_hip_set_trade_statistics.argtypes = [P, I32]
         L.ccr_hip_set_book.argtypes = [P, I32]
+        L.ccr_hip_set_book_grids.argtypes = [P, P, I32, P, I32]
+        L.ccr_hip_set_book_group.argtypes = [P, I32]
+        L.ccr_hip_book_grids.argtypes = [P, I32, P, I32, P, I32, P, P, I32, P]
         L.ccr_hip_set_graph.argtypes = [P, I32]
         L.ccr_book_totals.argtypes = [P, P, I32, I32, P]
         L.ccr_allocate_exposure_xva.argtypes = [P, P, I32, I32, P, D, P, P, P]
@@ -124,6 +127,18 @@ def grid(spec: abi.Spec, i: int = 0, simulation: bool = False) -> np.ndarray:
     out = np.zeros(n.value)
     fn(abi.dptr(spec.base_grid), len(spec.base_grid), C.byref(spec.sets_c[i]), abi.dptr(out), n.value, C.byref(n))
     return out
+
+
+def book_grids(spec: abi.Spec):
+    """ccr_hip_book_grids: (reporting, simulation) union grids of all of spec's netting sets."""
+    nr, ns = C.c_int32(), C.c_int32()
+    args = (abi.dptr(spec.base_grid), len(spec.base_grid), spec.sets_c, len(spec.sets))
+    rc = lib().ccr_hip_book_grids(*args, None, 0, C.byref(nr), None, 0, C.byref(ns))
+    if rc:
+        raise CcrError(rc, "book grids")
+    r, v = np.zeros(nr.value), np.zeros(ns.value)
+    lib().ccr_hip_book_grids(*args, abi.dptr(r), len(r), C.byref(nr), abi.dptr(v), len(v), C.byref(ns))
+    return r, v
 
 
 class Plan:
@@ -271,6 +286,18 @@ class Engine:
         """Plans compiled from now on put every netting set on the book's common grids (pathwise
         identical scenarios, additive statistics)."""
         self._check(lib().ccr_hip_set_book(self.ctx, 1 if enable else 0))
+
+    def set_book_group(self, sets_per_workgroup: int):
+        self._check(lib().ccr_hip_set_book_group(self.ctx, sets_per_workgroup))
+
+    def set_book_grids(self, reporting, simulation):
+        """Book plans compiled afterwards use these grids (None: their own union grids)."""
+        if reporting is None:
+            self._check(lib().ccr_hip_set_book_grids(self.ctx, None, 0, None, 0))
+            return
+        self._book_grids = (np.ascontiguousarray(reporting, np.float64), np.ascontiguousarray(simulation, np.float64))
+        r, v = self._book_grids
+        self._check(lib().ccr_hip_set_book_grids(self.ctx, abi.dptr(r), len(r), abi.dptr(v), len(v)))
 
     def set_trade_statistics(self, enable: bool):
         """Plans compiled from now on also produce per-trade allocation statistics (at most 16

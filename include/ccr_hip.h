@@ -422,7 +422,7 @@ int32_t ccr_hip_book_grids(const double* base_grid, int32_t base_size, const ccr
 int32_t ccr_hip_set_book_grids(ccr_hip_ctx* ctx, const double* reporting, int32_t reporting_size,
                                const double* simulation, int32_t simulation_size);
 /* Book plans compiled afterwards run the book kernel with this many netting sets per workgroup
- * (1, 2, 4 or 8; default 4; 1 = one set per workgroup): a lane steps its path's market once per
+ * (1, 2, 4 or 8; default 2; 1 = one set per workgroup): a lane steps its path's market once per
  * date and values every set of its group, sharing the fixing states. Results are bit-identical
  * for every choice. Plans with European options, per-trade statistics or the initial-margin
  * proxy run one set per workgroup. */

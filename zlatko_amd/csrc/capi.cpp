@@ -65,7 +65,9 @@ struct ccr_hip_ctx {
   bool book = false;
   std::vector<double> book_rep, book_sim;
   // Book plans: netting sets valued per workgroup by the book kernel (1: one set per workgroup).
-  int book_group = 4;
+  // Two: the per-set LDS rows (ring, collateral units, previous exposures) of two sets leave five
+  // waves per SIMD (500-set portfolio: path kernel 459 ms at 2, 607 ms at 4, 964 ms at 8).
+  int book_group = 2;
   // Executes replay one captured HIP graph per plan while set.
   bool use_graph = false;
   // Launches are being captured into a graph (no timing events recorded), and whether the last

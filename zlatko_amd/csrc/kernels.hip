@@ -110,13 +110,13 @@ ccr_path_kernel(PathArgs<Real> a) {
 // remapped) covers path block b % blocks_per_set of the sets [G (b / blocks_per_set), +G); each
 // set's statistics go to its own wave partials and exposure rows as in ccr_path_kernel. LDS per
 // path: the shared fixing slots, then each set's margin-call ring and price rows.
-// Waves per SIMD the book kernel's registers are held to: 5 up to four sets per workgroup (the
-// LDS of four sets' rings and the shared fixing slots allows five at 256 threads; measured on the
-// 500-set portfolio: path kernel 615 ms unconstrained at 3 waves, 499 ms at 4, 457 ms at 5), 4 for
-// eight (its LDS allows three).
+// Waves per SIMD the book kernel's registers are held to (96 VGPRs; the per-set state is in LDS,
+// so the group size does not change the registers). LDS then decides the occupancy: for the
+// 3-currency CSA portfolio (6 fixing rows + per set 2 ring rows and 3 state rows of 8 bytes) two
+// sets per 256-thread workgroup allow five waves per SIMD, four sets three, eight sets one.
 template <int G>
 constexpr int groupWaveHint() {
-  return G <= 4 ? 5 : 4;
+  return 5;
 }
 
 template <typename Real, typename Sh, int G>
@@ -136,9 +136,12 @@ ccr_group_kernel(PathArgs<Real> a) {
   const int64_t q = g.antithetic ? (pe >> 1) : pe;
   const Real* zbase = a.stream + q * sets[s0].stream_stride;
   Real* col = reinterpret_cast<Real*>(lds_raw) + tid;
+  // Per-set state rows (doubles) after the Real rows; bd is a multiple of 64, so they stay aligned.
+  const int realRows = a.slots + G * (a.ring + a.price_rows);
+  double* state = reinterpret_cast<double*>(lds_raw + static_cast<size_t>(realRows) * bd * sizeof(Real)) + tid;
   simulateGroup<Real, Sh, G>(toConst(a.g), sets, s0, ns, pe, zbase, toConst(a.steps), toConst(a.dates),
                              toConst(a.date_reals), toConst(a.bonds), toConst(a.fixed), toConst(a.fixstore),
-                             toConst(a.poly), a.bermv, a.ring, a.price_rows, a.slots, col, bd,
+                             toConst(a.poly), a.bermv, a.ring, a.price_rows, a.slots, col, state, bd,
                              [&](int k, int rr, const double* st9, double vp) {
     const CCR_CONST SetDev& S = sets[s0 + k];
     if (active) a.rows[S.rows_off + static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
@@ -572,7 +575,8 @@ hipError_t launchGroupShape(const PathArgs<Real>& args, int block, size_t lds, h
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
   if (args.group > 1) {
-    const size_t lds = static_cast<size_t>(args.slots + args.group * (args.ring + args.price_rows)) * block * sizeof(Real);
+    const size_t lds = static_cast<size_t>(args.slots + args.group * (args.ring + args.price_rows)) * block * sizeof(Real) +
+                       static_cast<size_t>(3 * args.group) * block * sizeof(double);
     return withShape(args.nrates, args.nassets, args.ncredits, [&](auto shape) {
       return launchGroupShape<Real, decltype(shape)>(args, block, lds, stream);
     });
@@ -592,7 +596,9 @@ hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStre
 template <typename Real>
 hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu,
                          bool trades, int group) {
-  const size_t lds = static_cast<size_t>(ldsRows) * block * sizeof(Real);
+  // Group kernels: ldsRows Real rows plus three double rows per set.
+  const size_t lds = static_cast<size_t>(ldsRows) * block * sizeof(Real) +
+                     (group > 1 ? static_cast<size_t>(3 * group) * block * sizeof(double) : 0);
   if (group > 1)
     return withShape(nrates, nassets, ncredits, [&](auto shape) -> hipError_t {
       using Sh = decltype(shape);

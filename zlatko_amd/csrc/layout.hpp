@@ -57,7 +57,8 @@ struct DateRec {
   int32_t poly_beg, poly_rows;
   int32_t pad_poly[2 * kMaxRates - 2];
   int32_t opt_beg, opt_end;          // European options alive at this date: [opt_beg, opt_end)
-  int32_t pad[2];
+  int32_t vidx;                      // index of this date among the set's valued dates (-1: not valued)
+  int32_t pad;
 };
 static_assert(sizeof(DateRec) == 112, "DateRec must stay 112 bytes");
 

@@ -399,24 +399,25 @@ __host__ __device__ __forceinline__ void marketStep(Market<Real, Sh>& mk, const 
 }
 
 // Fixing states recorded on date j: x of each listed rate into its slot row.
+// One predicated store per rate with a constant index into the state: a select among the rates'
+// states would be folded into a dynamically indexed load, which keeps the whole market state in
+// scratch memory.
 template <typename Real, typename Sh>
 __host__ __device__ __forceinline__ void storeFixings(const Market<Real, Sh>& mk, int beg, int end, cptr<int32_t> fstore,
                                                       Real* xs, int stride) {
   for (int i = beg; i < end; ++i) {
     const int r = fstore[2 * i], slot = fstore[2 * i + 1];
-    Real xv = mk.x[0];
 #pragma unroll
-    for (int k = 1; k < Sh::NR; ++k)
-      if (k == r) xv = mk.x[k];
-    xs[slot * stride] = xv;
+    for (int k = 0; k < Sh::NR; ++k)
+      if (k == r) xs[slot * stride] = mk.x[k];
   }
 }
 
 // Value of the terms of one valuation record on its date: V = sum over rate groups of FX x (bond
 // polynomial or merged bond terms + fixed coupons) + deterministic-carry forwards (+ options).
-template <typename Real, typename Sh, bool kOptions>
+template <typename Real, typename Sh, bool kOptions, typename Rec>
 __host__ __device__ __forceinline__ Real valueRecord(const Market<Real, Sh>& mk, const CCR_CONST GlobalDev& g,
-                                                     const DateRec& vr, cptr<Real> vdr, cptr<double> poly,
+                                                     const Rec& vr, cptr<Real> vdr, cptr<double> poly,
                                                      cptr<Real> bonds, cptr<Real> fixed, cptr<Real> opts,
                                                      const Real* xs, int stride) {
   constexpr int NR = Sh::NR, NA = Sh::NA;
@@ -553,9 +554,9 @@ __host__ __device__ __forceinline__ void marketReported(Market<Real, Sh>& mk) {
 // Collateral (collateral.cpp:10-47, the margin-call value read from the set's ring) and, on a
 // reporting date, the 9 statistics of ExposureProfile::compute and pathwiseCva for a set whose
 // value before collateral on date j is V; emit(rep, stats[kStats], positive_exposure).
-template <typename Real, typename Sh, typename Emit>
+template <typename Real, typename Sh, typename Rec, typename Emit>
 __host__ __device__ __forceinline__ Real collateralAndReport(const Market<Real, Sh>& mk, const CCR_CONST SetDev& S,
-                                                            const DateRec& rec, cptr<Real> dr, int j, Real V,
+                                                            const Rec& rec, cptr<Real> dr, int j, Real V,
                                                             SetPath<Real>& st, Real* ringV, Real* ringP, int stride,
                                                             Emit&& emit) {
   constexpr int NA = Sh::NA, NC = Sh::NC;
@@ -793,11 +794,9 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     const DateRec rec = dates[j];
     for (int i = rec.fixstore_beg; i < rec.fixstore_end; ++i) {
       const int r = fstore[2 * i], slot = fstore[2 * i + 1];
-      Real xv = x[0];
 #pragma unroll
-      for (int k = 1; k < NR; ++k)
-        if (k == r) xv = x[k];
-      xs[slot * stride] = xv;
+      for (int k = 0; k < NR; ++k)
+        if (k == r) xs[slot * stride] = x[k];  // constant-index predicated stores (see storeFixings)
     }
     if (!(rec.flags & kValued) && !S.has_csa) continue;
     cptr<Real> dr = dreal + static_cast<int64_t>(j) * kDateRealStride;
@@ -991,17 +990,20 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 }
 
 // Book plans, G netting sets per lane: the sets of a book share their grid and their draws, so
-// one lane steps its path's market once per date and values every set of its group on it. Each
-// set keeps its own collateral units, CVA / DVA trapezoid state, margin-call ring (LDS rows
-// [slots + g (ring + priceRows), ...)) and Bermudan rows; the fixing states (the first `slots`
-// rows) are shared, recorded from the group's first set's lists (a book's lists are the same).
+// one lane steps its path's market once per date and values every set of its group on it. Per
+// path, LDS holds the shared fixing states (rows [0, slots), recorded from the group's first
+// set's lists -- a book's lists are the same), each set's margin-call ring and price rows (rows
+// slots + k (ring + priceRows) ...) and, in `state` (a double column), each set's collateral
+// units and previous discounted exposures (rows 3k .. 3k + 2): the set loop is not unrolled, so
+// per-set state kept in registers would sit in scratch memory instead.
 // sp[g], g < ns: the group's sets; emit(g, rep, stats, positive_exposure).
 template <typename Real, typename Sh, int G, typename Emit>
-__host__ __device__ inline void simulateGroup(cptr<GlobalDev> gp, cptr<SetDev> sets, int s0, int ns, int pe,
+__host__ __device__ __forceinline__ void simulateGroup(cptr<GlobalDev> gp, cptr<SetDev> sets, int s0, int ns, int pe,
                                               const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                               cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
                                               cptr<int32_t> fstore, cptr<double> poly, const double* bermv, int ring,
-                                              int priceRows, int slots, Real* col, int stride, Emit&& emit) {
+                                              int priceRows, int slots, Real* col, double* state, int stride,
+                                              Emit&& emit) {
   const CCR_CONST GlobalDev& g = *gp;
   const CCR_CONST SetDev& S0 = sets[s0];
   Real* xs = col;
@@ -1010,41 +1012,41 @@ __host__ __device__ inline void simulateGroup(cptr<GlobalDev> gp, cptr<SetDev> s
   cptr<Real> st0 = steps + static_cast<int64_t>(S0.date_off) * kStepStride;
   Market<Real, Sh> mk;
   marketInit<Real, Sh, false>(mk, g, zbase, S0.num_dates);
-  SetPath<Real> stp[G];
-  int bcount[G];
-#pragma unroll
-  for (int k = 0; k < G; ++k) {
-    setPathInit(stp[k]);
-    bcount[k] = 0;
+  for (int k = 0; k < ns; ++k) {
+    state[(3 * k) * stride] = 0.0;
+    state[(3 * k + 1) * stride] = 0.0;
+    state[(3 * k + 2) * stride] = 0.0;
   }
   for (int j = 0; j < S0.num_dates; ++j) {
     if (j > 0) marketStep<Real, Sh, false>(mk, g, j, S0.num_dates, st0, zbase, sgn);
-    {
-      const DateRec r0 = dates[S0.date_off + j];
-      storeFixings(mk, r0.fixstore_beg, r0.fixstore_end, fstore + 2 * S0.fixstore_off, xs, stride);
-    }
-#pragma unroll
-    for (int k = 0; k < G; ++k) {
-      if (k >= ns) continue;
+    const CCR_CONST DateRec& r0 = dates[S0.date_off + j];
+    storeFixings(mk, r0.fixstore_beg, r0.fixstore_end, fstore + 2 * S0.fixstore_off, xs, stride);
+    for (int k = 0; k < ns; ++k) {
       const CCR_CONST SetDev& S = sets[s0 + k];
-      const DateRec rec = dates[S.date_off + j];
+      const CCR_CONST DateRec& rec = dates[S.date_off + j];
       if (!(rec.flags & kValued) && !S.has_csa) continue;
       cptr<Real> dr = dreal + (static_cast<int64_t>(S.date_off) + j) * kDateRealStride;
       Real V = 0;
       if (rec.flags & kValued) {
         V = valueRecord<Real, Sh, false>(mk, g, rec, dr, poly + S.poly_off, bonds + 2 * static_cast<int64_t>(S.bond_off),
                                          fixed + 4 * static_cast<int64_t>(S.fixed_off), nullptr, xs, stride);
-        if (bermv && S.berm_end > S.berm_beg) {
-          V += static_cast<Real>(bermv[(static_cast<int64_t>(S.cd_base) + bcount[k]) * g.num_paths + pe]);
-          ++bcount[k];
-        }
+        // Bermudan swaptions: the set's row of this valued date from the valuation kernel.
+        if (bermv && S.berm_end > S.berm_beg)
+          V += static_cast<Real>(bermv[(static_cast<int64_t>(S.cd_base) + rec.vidx) * g.num_paths + pe]);
       }
+      SetPath<Real> st;
+      st.units = static_cast<Real>(state[(3 * k) * stride]);
+      st.prevDvp = state[(3 * k + 1) * stride];
+      st.prevDvn = state[(3 * k + 2) * stride];
       Real* ringV = col + (slots + k * (ring + priceRows)) * stride;
-      collateralAndReport(mk, S, rec, dr, j, V, stp[k], ringV, ringV + ring * stride, stride,
+      collateralAndReport(mk, S, rec, dr, j, V, st, ringV, ringV + ring * stride, stride,
                           [&](int rr, const double* st9, double vp) { emit(k, rr, st9, vp); });
+      state[(3 * k) * stride] = static_cast<double>(st.units);
+      state[(3 * k + 1) * stride] = st.prevDvp;
+      state[(3 * k + 2) * stride] = st.prevDvn;
     }
     // A book's sets report on the same dates (the book's reporting grid).
-    if (dates[S0.date_off + j].flags & kReporting) marketReported(mk);
+    if (r0.flags & kReporting) marketReported(mk);
   }
 }
 

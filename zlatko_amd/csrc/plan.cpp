@@ -793,6 +793,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
 
     // Emit dates.
     int64_t setTerms = 0;
+    int32_t valuedSoFar = 0;
     for (int j = 0; j < T; ++j) {
       const double t = sim[static_cast<std::size_t>(j)];
       DateRec rec{};
@@ -801,6 +802,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
                   (isCall[static_cast<std::size_t>(j)] ? kStoreValue : 0);
       rec.call_pos = ringPos[static_cast<std::size_t>(call[static_cast<std::size_t>(j)])];
       rec.store_pos = isCall[static_cast<std::size_t>(j)] ? ringPos[static_cast<std::size_t>(j)] : -1;
+      rec.vidx = valued[static_cast<std::size_t>(j)] ? valuedSoFar++ : -1;
       double reals[kDateRealStride] = {0};
       const int bondBase = static_cast<int>(plan.bonds.size() / 2), fixBase = static_cast<int>(plan.fixed.size() / 4);
       if (per_trade) {

@@ -107,9 +107,11 @@ ccr_path_kernel(PathArgs<Real> a) {
 }
 
 // Book plans: G netting sets per workgroup (simulateGroup), one path per lane. Workgroup b (XCD
-// remapped) covers path block b % blocks_per_set of the sets [G (b / blocks_per_set), +G); each
-// set's statistics go to its own wave partials and exposure rows as in ccr_path_kernel. LDS per
-// path: the shared fixing slots, then each set's margin-call ring and price rows.
+// remapped) covers path block b / ngroups of the sets [G (b % ngroups), +G): path-block-major, so
+// the groups reading one path block's replayed normals (every set of a book replays the same)
+// run together on one XCD and share its L2. Each set's statistics go to its own wave partials and
+// exposure rows as in ccr_path_kernel. LDS per path: the shared fixing slots, then each set's
+// margin-call ring and price rows, then its state rows.
 // Waves per SIMD the book kernel's registers are held to (96 VGPRs; the per-set state is in LDS,
 // so the group size does not change the registers). LDS then decides the occupancy: for the
 // 3-currency CSA portfolio (6 fixing rows + per set 2 ring rows and 3 state rows of 8 bytes) two
@@ -125,7 +127,8 @@ ccr_group_kernel(PathArgs<Real> a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const CCR_CONST GlobalDev& g = *toConst(a.g);
   const int rb = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
-  const int grp = rb / a.blocks_per_set, lb = rb % a.blocks_per_set;
+  const int ngroups = (a.nsets + G - 1) / G;
+  const int grp = rb % ngroups, lb = rb / ngroups;
   const int s0 = grp * G, ns = a.nsets - s0 < G ? a.nsets - s0 : G;
   const int bd = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
   const int p = lb * bd + tid;

@@ -214,9 +214,10 @@ typedef struct ccr_exposure_out {
 enum ccr_precision {
   CCR_FP64 = 0,  /* FP64 everywhere: the parity mode (1e-10 relative to the FP64 oracle) */
   CCR_FP32 = 1,  /* FP32 path evolution (Kahan-compensated integrals of the short rate and the
-                    log-spots), FP32 exponential terms, fixed coupons and collateral; the merged-bond
-                    power series (Horner), the American Monte Carlo regressions and the
-                    statistics' wavefront sums stay FP64 */
+                    log-spots) and FP32 revaluation: merged-bond power series (FP32 coefficients,
+                    Horner in FP32), exponential terms, fixed coupons, options, collateral and the
+                    Bermudans' exercise values; the American Monte Carlo regressions, the
+                    statistics' wavefront sums and the final statistics stay FP64 */
   CCR_BF16 = 2   /* MLP surrogate only: BF16 matrix-core products, FP32 accumulation */
 };
 

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "launch.hpp"
 #include "path_body.hpp"
@@ -28,6 +29,13 @@ namespace ccrhip {
 namespace {
 
 constexpr int kWaves = kAmcChunk / 64;
+
+// The bond-polynomial table of the plan's precision.
+template <typename Real>
+__device__ inline const Real* polyOf(const AmcArgs& a) {
+  if constexpr (std::is_same<Real, double>::value) return a.poly;
+  else return a.polyf;
+}
 
 __global__ void __launch_bounds__(256) amc_init_kernel(const BermDev* berms, int nB, int64_t paths, int16_t* exidx) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -49,7 +57,7 @@ __global__ void __launch_bounds__(kAmcChunk) amc_ev_kernel(AmcArgs a, const Real
   // The tail swap starts at the exercise date: no coupon is fixed yet.
   const Real v = evalRange<Real>(toConst(a.evterms)[bm.ev_rec + k], static_cast<Real>(x),
                                  toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off),
-                                 toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(a.poly) + S.poly_off,
+                                 toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(polyOf<Real>(a)) + S.poly_off,
                                  [](int) { return Real(0); });
   if (p0 < a.paths) a.ev[static_cast<int64_t>(e) * a.paths + p0] = static_cast<double>(v);
 }
@@ -447,7 +455,7 @@ __global__ void __launch_bounds__(kAmcChunk) amc_value_kernel(AmcArgs a, const R
   const int64_t p = p0 < a.paths ? p0 : a.paths - 1;
   const Real v = bermudanSum<Real>(S, j, p, a.paths, toConst(a.bdates), toConst(a.cont), a.exidx,
                                    a.states, toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off),
-                                   toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(a.poly) + S.poly_off,
+                                   toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(polyOf<Real>(a)) + S.poly_off,
                                    toConst(a.clive), toConst(a.clive_rng)[2 * d], toConst(a.clive_rng)[2 * d + 1]);
   if (p0 < a.paths) a.bermv[static_cast<int64_t>(d) * a.paths + p0] = static_cast<double>(v);
 }

@@ -100,7 +100,7 @@ struct ccr_hip_plan {
   void* d_bonds = nullptr;
   void* d_fixed = nullptr;
   int32_t* d_fixstore = nullptr;
-  double* d_poly = nullptr;
+  void* d_poly = nullptr;     // bond-polynomial coefficients in the plan's precision
   void* d_opts = nullptr;
   void* d_rows = nullptr;
   double* d_partials = nullptr;
@@ -381,7 +381,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   plan->d_bonds = uploadReals(ctx, plan.get(), h.bonds, fp32);
   plan->d_fixed = uploadReals(ctx, plan.get(), h.fixed, fp32);
   plan->d_fixstore = upload(ctx, plan.get(), h.fixstore.data(), h.fixstore.size());
-  plan->d_poly = upload(ctx, plan.get(), h.poly.data(), h.poly.size());
+  plan->d_poly = uploadReals(ctx, plan.get(), h.poly, fp32);
   plan->d_opts = h.opts.empty() ? nullptr : uploadReals(ctx, plan.get(), h.opts, fp32);
   const size_t elem = fp32 ? sizeof(float) : sizeof(double);
   const int64_t P = h.g.num_paths;
@@ -444,7 +444,8 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.bdate_cd = upload(ctx, plan.get(), h.bdate_cd.data(), h.bdate_cd.size());
     a.clive = upload(ctx, plan.get(), h.clive.data(), h.clive.size());
     a.clive_rng = upload(ctx, plan.get(), h.clive_rng.data(), h.clive_rng.size());
-    a.poly = plan->d_poly;
+    a.poly = fp32 ? nullptr : static_cast<const double*>(plan->d_poly);
+    a.polyf = fp32 ? static_cast<const float*>(plan->d_poly) : nullptr;
     a.states = plan->d_states;
     execAlloc(ctx, plan.get(), a.ev, static_cast<size_t>(h.ev_len) * sizeof(double));
     execAlloc(ctx, plan.get(), a.strategy, static_cast<size_t>(h.ev_len) * sizeof(double));
@@ -512,7 +513,7 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   a.bonds = static_cast<const Real*>(plan->d_bonds);
   a.fixed = static_cast<const Real*>(plan->d_fixed);
   a.fixstore = plan->d_fixstore;
-  a.poly = plan->d_poly;
+  a.poly = static_cast<const Real*>(plan->d_poly);
   a.opts = static_cast<const Real*>(plan->d_opts);
   a.bermv = plan->d_bermv;
   a.states = nullptr;

@@ -36,7 +36,7 @@ struct PathArgs {
   const Real* bonds;
   const Real* fixed;
   const int32_t* fixstore;
-  const double* poly;  // bond-polynomial coefficients (FP64 in both precisions)
+  const Real* poly;    // bond-polynomial coefficients in the plan's precision
   const Real* opts;    // European option terms
   const double* bermv;      // American Monte Carlo: [continuation date][path] sum of the set's swaptions
   double* states;           // states-only pass: [set dates][path] x then D (null otherwise)
@@ -82,7 +82,8 @@ struct AmcArgs {
   const int32_t* clive_rng;   // ... [begin, end) pairs
   const int32_t* bdate_berm;  // Bermudan of each date record
   const int32_t* bdate_cd;    // continuation date of each date record (-1: none)
-  const double* poly;
+  const double* poly;         // bond-polynomial coefficients of FP64 plans ...
+  const float* polyf;         // ... and of FP32 plans
   const double* states;
   double* ev;
   double* strategy;

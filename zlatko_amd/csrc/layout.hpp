@@ -81,6 +81,8 @@ constexpr int kDateRealStride = 16;
 // below kPolyTolerance of sum_T |c_T A_T| for |x| <= xmax (a bound of 9 standard deviations plus
 // the drift); a wavefront with any lane beyond xmax evaluates the exponential terms instead.
 constexpr double kPolyTolerance = 1e-18;
+// FP32 plans: a tail below 1e-9 of sum_T |c_T A_T| is under the rounding of their FP32 coefficients.
+constexpr double kPolyToleranceFp32 = 1e-9;
 constexpr int kPolyBatch = 8;  // coefficient rows per batch of scalar loads in the path kernel
 constexpr double kPolySigmas = 9.0;
 constexpr int kPolyMaxDegree = 48;

@@ -89,6 +89,14 @@ __host__ __device__ inline Real expTerm(Real v) {
   if constexpr (std::is_same<Real, double>::value) return expBond(v);
   else return __expf(v);
 }
+// One Horner step h x + c of a merged-bond power series in the plan's precision (c uniform): the
+// FP64 form reads c from SGPRs (fmaU3); FP32 plans hold FP32 coefficients.
+template <typename Real>
+__host__ __device__ inline Real hornerStep(Real h, Real x, Real c) {
+  if constexpr (std::is_same<Real, double>::value) return fmaU3(h, x, c);
+  else return fmaf(h, x, c);
+}
+
 // True when the predicate holds on every lane of the wavefront (the path itself on the host).
 __host__ __device__ inline bool allLanes(bool p) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -131,22 +139,21 @@ __host__ __device__ inline Real blackFormula(Real sign, Real forward, Real strik
 // row k named in each coupon's fourth element.
 template <typename Real, typename Fixing>
 __host__ __device__ inline Real evalRange(const CCR_CONST TermRange& tr, Real x, cptr<Real> bonds, cptr<Real> fixed,
-                                          cptr<double> poly, Fixing&& fixing) {
+                                          cptr<Real> poly, Fixing&& fixing) {
   Real acc;
   if (tr.poly_len > 0 && allLanes(static_cast<double>(x < Real(0) ? -x : x) <= tr.xmax)) {
     // Horner over kPolyBatch coefficients per batch of scalar loads (the range is uniform
     // across the wavefront wherever this runs: one exercise or one valued date per workgroup).
-    cptr<double> pc = poly + tr.poly_beg;
-    const double xd = static_cast<double>(x);
-    double h = 0.0;
+    cptr<Real> pc = poly + tr.poly_beg;
+    Real h = 0;
     for (int k = 0; k < tr.poly_len; k += kPolyBatch) {
-      double c[kPolyBatch];
+      Real c[kPolyBatch];
 #pragma unroll
       for (int u = 0; u < kPolyBatch; ++u) c[u] = pc[k + u];
 #pragma unroll
-      for (int u = 0; u < kPolyBatch; ++u) h = fmaU3(h, xd, c[u]);
+      for (int u = 0; u < kPolyBatch; ++u) h = hornerStep<Real>(h, x, c[u]);
     }
-    acc = static_cast<Real>(h);
+    acc = h;
   } else {
     acc = static_cast<Real>(tr.c0);
     for (int i = tr.bond_beg; i < tr.bond_end; ++i) acc += bonds[2 * i] * expTerm(-bonds[2 * i + 1] * x);
@@ -169,7 +176,7 @@ template <typename Real>
 __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, int64_t p, int64_t paths,
                                             cptr<BermDate> bdates, cptr<double> cont,
                                             const int16_t* exidx, const double* states, cptr<Real> bonds,
-                                            cptr<Real> fixed, cptr<double> poly, cptr<LiveRec> live, int lbeg,
+                                            cptr<Real> fixed, cptr<Real> poly, cptr<LiveRec> live, int lbeg,
                                             int lend) {
   const double* xrow = states + S.state_off;
   const double xd = xrow[static_cast<int64_t>(j) * paths + p];
@@ -417,7 +424,7 @@ __host__ __device__ __forceinline__ void storeFixings(const Market<Real, Sh>& mk
 // polynomial or merged bond terms + fixed coupons) + deterministic-carry forwards (+ options).
 template <typename Real, typename Sh, bool kOptions, typename Rec>
 __host__ __device__ __forceinline__ Real valueRecord(const Market<Real, Sh>& mk, const CCR_CONST GlobalDev& g,
-                                                     const Rec& vr, cptr<Real> vdr, cptr<double> poly,
+                                                     const Rec& vr, cptr<Real> vdr, cptr<Real> poly,
                                                      cptr<Real> bonds, cptr<Real> fixed, cptr<Real> opts,
                                                      const Real* xs, int stride) {
   constexpr int NR = Sh::NR, NA = Sh::NA;
@@ -436,22 +443,22 @@ __host__ __device__ __forceinline__ Real valueRecord(const Market<Real, Sh>& mk,
       if (r < nR) inside = inside && static_cast<double>(mk.x[r] < Real(0) ? -mk.x[r] : mk.x[r]) <= vdr[kDateXmax + r];
     usePoly = allLanes(inside);
   }
-  double hp[NR];
+  Real hp[NR];
 #pragma unroll
-  for (int r = 0; r < NR; ++r) hp[r] = 0.0;
+  for (int r = 0; r < NR; ++r) hp[r] = 0;
   if (usePoly) {
-    cptr<double> pc = poly + vr.poly_beg;
+    cptr<Real> pc = poly + vr.poly_beg;
     for (int k = 0; k < rows; k += kPolyBatch) {
-      double c[kPolyBatch][NR];
+      Real c[kPolyBatch][NR];
 #pragma unroll
       for (int u = 0; u < kPolyBatch; ++u)
 #pragma unroll
-        for (int r = 0; r < NR; ++r) c[u][r] = r < nR ? pc[(k + u) * nR + r] : 0.0;
+        for (int r = 0; r < NR; ++r) c[u][r] = r < nR ? pc[(k + u) * nR + r] : Real(0);
 #pragma unroll
       for (int u = 0; u < kPolyBatch; ++u)
 #pragma unroll
         for (int r = 0; r < NR; ++r)
-          if (r < nR) hp[r] = fmaU3(hp[r], static_cast<double>(mk.x[r]), c[u][r]);
+          if (r < nR) hp[r] = hornerStep<Real>(hp[r], mk.x[r], c[u][r]);
     }
   }
   // Fixed floating coupons of every rate (contiguous over the rates), two per batch so that
@@ -495,7 +502,7 @@ __host__ __device__ __forceinline__ Real valueRecord(const Market<Real, Sh>& mk,
       const Real nx = -mk.x[r];
       Real acc;
       if (usePoly) {
-        acc = static_cast<Real>(hp[r]);
+        acc = hp[r];
       } else {
         acc = vdr[kDateC0 + r];
         for (int i = vr.bond_beg[r]; i < vr.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expTerm(bonds[2 * i + 1] * nx);
@@ -640,7 +647,7 @@ template <typename Real, typename Sh, bool kOptions, bool kTrades, typename Emit
 __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp, int pe,
                                              const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                              cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
-                                             cptr<int32_t> fstore, cptr<double> poly, cptr<Real> opts,
+                                             cptr<int32_t> fstore, cptr<Real> poly, cptr<Real> opts,
                                              const double* bermv, int ring,
                                              int priceRows, Real* col, int stride, Emit&& emit,
                                              OnValue&& onValue, cptr<DateRec> tdates, cptr<Real> tdreals,
@@ -816,22 +823,22 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
           if (r < nR) inside = inside && static_cast<double>(x[r] < Real(0) ? -x[r] : x[r]) <= vdr[kDateXmax + r];
         usePoly = allLanes(inside);
       }
-      double hp[NR];
+      Real hp[NR];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) hp[r] = 0.0;
+      for (int r = 0; r < NR; ++r) hp[r] = 0;
       if (usePoly) {
-        cptr<double> pc = poly + vr.poly_beg;
+        cptr<Real> pc = poly + vr.poly_beg;
         for (int k = 0; k < rows; k += kPolyBatch) {
-          double c[kPolyBatch][NR];
+          Real c[kPolyBatch][NR];
 #pragma unroll
           for (int u = 0; u < kPolyBatch; ++u)
 #pragma unroll
-            for (int r = 0; r < NR; ++r) c[u][r] = r < nR ? pc[(k + u) * nR + r] : 0.0;
+            for (int r = 0; r < NR; ++r) c[u][r] = r < nR ? pc[(k + u) * nR + r] : Real(0);
 #pragma unroll
           for (int u = 0; u < kPolyBatch; ++u)
 #pragma unroll
             for (int r = 0; r < NR; ++r)
-              if (r < nR) hp[r] = fmaU3(hp[r], static_cast<double>(x[r]), c[u][r]);
+              if (r < nR) hp[r] = hornerStep<Real>(hp[r], x[r], c[u][r]);
         }
       }
       // Fixed floating coupons of every rate (contiguous over the rates), two per batch so that
@@ -875,7 +882,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
           const Real nx = -x[r];
           Real acc;
           if (usePoly) {
-            acc = static_cast<Real>(hp[r]);
+            acc = hp[r];
           } else {
             acc = vdr[kDateC0 + r];
             for (int i = vr.bond_beg[r]; i < vr.bond_beg[r + 1]; ++i) acc += bonds[2 * i] * expTerm(bonds[2 * i + 1] * nx);
@@ -1001,7 +1008,7 @@ template <typename Real, typename Sh, int G, typename Emit>
 __host__ __device__ __forceinline__ void simulateGroup(cptr<GlobalDev> gp, cptr<SetDev> sets, int s0, int ns, int pe,
                                               const Real* __restrict__ zbase, cptr<Real> steps, cptr<DateRec> dates,
                                               cptr<Real> dreal, cptr<Real> bonds, cptr<Real> fixed,
-                                              cptr<int32_t> fstore, cptr<double> poly, const double* bermv, int ring,
+                                              cptr<int32_t> fstore, cptr<Real> poly, const double* bermv, int ring,
                                               int priceRows, int slots, Real* col, double* state, int stride,
                                               Emit&& emit) {
   const CCR_CONST GlobalDev& g = *gp;

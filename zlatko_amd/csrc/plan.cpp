@@ -347,7 +347,7 @@ double stateBound(const HullWhite& hw, double mu, double t) {
 // remainder bound sum_i |cA_i| (B_i X)^{K+1} / (K+1)! e^{B_i X} is below kPolyTolerance times
 // sum_i |cA_i| for |x| <= X. Returns false (no polynomial) when no degree up to kPolyMaxDegree
 // reaches the tolerance. Accumulated in long double.
-bool bondPolynomial(const double* terms, int n, double c0, double xmax, std::vector<double>& coef) {
+bool bondPolynomial(const double* terms, int n, double c0, double xmax, std::vector<double>& coef, double tolerance) {
   coef.clear();
   long double gross = 0.0L, bmax = 0.0L;
   for (int i = 0; i < n; ++i) {
@@ -374,7 +374,7 @@ bool bondPolynomial(const double* terms, int n, double c0, double xmax, std::vec
       tail[static_cast<std::size_t>(i)] *= std::fabs(static_cast<long double>(terms[2 * i + 1])) * xmax / (k + 2);
     }
     coef.push_back(static_cast<double>(ck));
-    if (rest <= static_cast<long double>(kPolyTolerance) * gross) return true;
+    if (rest <= static_cast<long double>(tolerance) * gross) return true;
   }
   coef.clear();
   return false;
@@ -473,6 +473,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
   if (pfe_quantile < 0.0 || pfe_quantile > 1.0) invalid("quantile: level must be in [0, 1]");
   if (block <= 0 || block % 64 != 0 || block > kMaxBlock) invalid("block size must be a multiple of 64 up to 256");
   const Models m = makeModels(gen);
+  const double polyTol = precision == CCR_FP32 ? kPolyToleranceFp32 : kPolyTolerance;
   const TimeGrid base(std::vector<double>(base_grid, base_grid + std::max(0, base_size)));
   HostPlan plan;
   plan.precision = precision;
@@ -742,7 +743,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
               stateBound(m.rates[static_cast<std::size_t>(r)], m.quanto[static_cast<std::size_t>(r)], t);
           reals[kDateXmax + r] = xmax;
           auto& c = coef[static_cast<std::size_t>(r)];
-          all = all && bondPolynomial(plan.bonds.data() + 2 * b0, b1 - b0, col.c0[r], xmax, c);
+          all = all && bondPolynomial(plan.bonds.data() + 2 * b0, b1 - b0, col.c0[r], xmax, c, polyTol);
           rows = std::max(rows, c.size());
         }
         if (all) {
@@ -918,7 +919,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       tr.xmax = xmax;
       tr.poly_beg = static_cast<int32_t>(plan.poly.size());
       std::vector<double> coef;
-      if (bondPolynomial(plan.bonds.data() + 2 * tr.bond_beg, tr.bond_end - tr.bond_beg, tr.c0, xmax, coef)) {
+      if (bondPolynomial(plan.bonds.data() + 2 * tr.bond_beg, tr.bond_end - tr.bond_beg, tr.c0, xmax, coef, polyTol)) {
         const std::size_t len = (coef.size() + kPolyBatch - 1) / kPolyBatch * kPolyBatch;
         plan.poly.insert(plan.poly.end(), len - coef.size(), 0.0);
         plan.poly.insert(plan.poly.end(), coef.rbegin(), coef.rend());

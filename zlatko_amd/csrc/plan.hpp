@@ -92,8 +92,10 @@ TimeGrid bookSimulationGrid(const TimeGrid& rep, const ccr_netting_set* sets, in
 // the per-wave partial sums.
 void assignBlocks(HostPlan& plan, int block);
 
-// Power-series coefficients of one date's merged bond terms (see layout.hpp, kPolyTolerance).
-bool bondPolynomial(const double* terms, int n, double c0, double xmax, std::vector<double>& coef);
+// Power-series coefficients of one date's merged bond terms, truncated at `tolerance` (see
+// layout.hpp, kPolyTolerance).
+bool bondPolynomial(const double* terms, int n, double c0, double xmax, std::vector<double>& coef,
+                    double tolerance = kPolyTolerance);
 
 // The reference normal stream: std::mt19937_64(seed) + std::normal_distribution<double>, as
 // ccr::NormalGenerator (include/ccr/core/random.hpp:13-26), n draws.

@@ -428,6 +428,22 @@ int32_t ccr_hip_set_book_grids(ccr_hip_ctx* ctx, const double* reporting, int32_
  * for every choice. Plans with European options, per-trade statistics or the initial-margin
  * proxy run one set per workgroup. */
 int32_t ccr_hip_set_book_group(ccr_hip_ctx* ctx, int32_t sets_per_workgroup);
+
+/* Bermudan regressions (PolynomialRegression as BermudanSwaption::valueWithExercise uses it,
+ * src/instruments/bermudan_swaption.cpp:77-93, 110-116; src/core/regression.cpp:29-90) for plans
+ * compiled after the call. basis -1 (the default): the state basis 1, z, z^2, z^3 on per-date
+ * designs shared by the swaptions of a date. Otherwise every exercise decision and every
+ * continuation (swaption, valued date) is its own regression record:
+ *   CCR_BASIS_STATE              the same state basis (reproduces the default's fits);
+ *   CCR_BASIS_SWAP_RATE_ANNUITY  the 10 monomials of total degree <= 3 in the standardised swap
+ *                                rate and annuity of the tail swap entered at the (next) exercise,
+ *                                the swaption's degree capped at 3 as the total degree; no
+ *                                counterpart in the reference, whose basis is the state alone.
+ * gram_precision CCR_FP64 sums the Gram of [basis | target] in FP64; CCR_FP32 on FP32 matrix cores
+ * (FP32 products and accumulation per 4,096 paths, FP64 across path chunks). The normal equations
+ * (ridge 1e-12 n) are solved in FP64 by elimination with partial pivoting (matrix.cpp:70-96). */
+enum ccr_regression_basis { CCR_BASIS_STATE = 0, CCR_BASIS_SWAP_RATE_ANNUITY = 1 };
+int32_t ccr_hip_set_regression(ccr_hip_ctx* ctx, int32_t basis, int32_t gram_precision);
 /* sumProfiles (src/app/analyses.cpp:1769-1784): the additive fields (expected value, EE, ENE, EE*,
  * ENE*, CVA and DVA increments) summed over the netting sets of counterparty credit
  * `counterparty` (-1: every set) of one book execute; times, mean deflator and survival are the

@@ -747,6 +747,137 @@ struct BermudanValuation {
   double price = 0.0;
 };
 
+// Basis of the Bermudan regressions (ccr_oracle_set_regression_basis): -1 or 0 the reference's
+// state basis (PolyReg); 1 the swap-rate / annuity basis below. Process-wide, set before a run.
+int gBasis = -1;
+
+// Swap-rate / annuity regression -- an extension with no reference counterpart, written on the
+// rules of PolynomialRegression (src/core/regression.cpp:29-90) applied to two regressors:
+// standardise each (mean, population sd over the sample); a degenerate one (sd at most 1e-6 of
+// its root mean square) drops the fit to the sample mean; the total degree (the trade's degree, at most 3)
+// falls while n < 2 x terms; design = the monomials of total degree <= d in (s, a) in the order
+// 1, s, a, s^2, s a, a^2, s^3, s^2 a, s a^2, a^3; normal equations with the ridge 1e-12 n plus
+// 1e-4 x their diagonal, solved by Cholesky.
+struct SaReg {
+  std::size_t degree;
+  double m1 = 0.0, s1 = 1.0, m2 = 0.0, s2 = 1.0;
+  std::size_t deg = 0;
+  Vec coef;
+  explicit SaReg(std::size_t d) : degree(d) {}
+  static std::size_t terms(std::size_t d) { return (d + 1) * (d + 2) / 2; }
+  static void row(double s, double a, double* phi) {
+    const double s2 = s * s, a2 = a * a;
+    phi[0] = 1.0;
+    phi[1] = s;
+    phi[2] = a;
+    phi[3] = s2;
+    phi[4] = s * a;
+    phi[5] = a2;
+    phi[6] = s2 * s;
+    phi[7] = s2 * a;
+    phi[8] = s * a2;
+    phi[9] = a2 * a;
+  }
+  void fit(const Vec& u1, const Vec& u2, const Vec& y, const std::vector<char>* mask) {
+    std::size_t n = 0;
+    double a1 = 0.0, q1 = 0.0, a2 = 0.0, q2 = 0.0, sy = 0.0;
+    for (std::size_t i = 0; i < u1.size(); ++i) {
+      if (mask && !(*mask)[i]) continue;
+      ++n;
+      a1 += u1[i];
+      q1 += u1[i] * u1[i];
+      a2 += u2[i];
+      q2 += u2[i] * u2[i];
+      sy += y[i];
+    }
+    m1 = m2 = 0.0;
+    s1 = s2 = 1.0;
+    deg = 0;
+    if (n == 0) {
+      coef = {0.0};
+      return;
+    }
+    const double nn = static_cast<double>(n);
+    m1 = a1 / nn;
+    m2 = a2 / nn;
+    const double sd1 = std::sqrt(std::max(q1 / nn - m1 * m1, 0.0));
+    const double sd2 = std::sqrt(std::max(q2 / nn - m2 * m2, 0.0));
+    deg = std::min<std::size_t>(degree, 3);
+    // Constant regressor: the one-pass variance of a constant carries rounding of ~1e-16 E[u^2];
+    // a deviation of at most 1e-6 of the root mean square counts as none.
+    if (!(sd1 > 1e-6 * std::sqrt(q1 / nn)) || !(sd2 > 1e-6 * std::sqrt(q2 / nn))) deg = 0;
+    while (deg > 0 && n < 2 * terms(deg)) --deg;
+    if (deg == 0) {
+      coef = {sy / nn};
+      return;
+    }
+    s1 = sd1;
+    s2 = sd2;
+    // Normal equations summed as leastSquares does (regression.cpp:8-27), the ridge 1e-12 n plus
+    // 1e-4 x the diagonal (the monomials are numerically dependent when both regressors are
+    // functions of one state), solved by Cholesky (matrix.cpp:49-68) and two triangular solves.
+    const std::size_t k = terms(deg);
+    Mat gm(k, k, 0.0);
+    Vec rhs(k, 0.0);
+    double phi[10];
+    for (std::size_t i = 0; i < u1.size(); ++i) {
+      if (mask && !(*mask)[i]) continue;
+      row((u1[i] - m1) / s1, (u2[i] - m2) / s2, phi);
+      for (std::size_t a = 0; a < k; ++a) {
+        rhs[a] += phi[a] * y[i];
+        for (std::size_t b = 0; b <= a; ++b) gm(a, b) += phi[a] * phi[b];
+      }
+    }
+    for (std::size_t a = 0; a < k; ++a) {
+      for (std::size_t b = 0; b < a; ++b) gm(b, a) = gm(a, b);
+      gm(a, a) += 1e-12 * nn + 1e-4 * gm(a, a);
+    }
+    Mat l;
+    try {
+      l = cholesky(gm);
+    } catch (const std::exception&) {
+      deg = 0;
+      s1 = s2 = 1.0;
+      coef = {sy / nn};
+      return;
+    }
+    Vec z(k);
+    for (std::size_t i = 0; i < k; ++i) {
+      double v = rhs[i];
+      for (std::size_t m = 0; m < i; ++m) v -= l(i, m) * z[m];
+      z[i] = v / l(i, i);
+    }
+    coef.assign(k, 0.0);
+    for (std::size_t i = k; i-- > 0;) {
+      double v = z[i];
+      for (std::size_t m = i + 1; m < k; ++m) v -= l(m, i) * coef[m];
+      coef[i] = v / l(i, i);
+    }
+  }
+  double predict(double u1, double u2) const {
+    if (deg == 0) return coef[0];
+    double phi[10];
+    row((u1 - m1) / s1, (u2 - m2) / s2, phi);
+    double r = 0.0;
+    for (std::size_t c = 0; c < coef.size(); ++c) r += coef[c] * phi[c];
+    return r;
+  }
+};
+
+// Swap rate F / A and annuity A at date j of path p of a tail swap: A = sum_i tau_i P(t, T_i) over
+// its fixed periods, F = P(t, T_0) - P(t, T_n).
+void swapRateAnnuityAt(const Vec& sched, const HW& hw, double t, double x, double& u1, double& u2) {
+  double A = 0.0;
+  for (std::size_t i = 1; i < sched.size(); ++i) A += (sched[i] - sched[i - 1]) * hw.zeroBond(t, sched[i], x);
+  const double F = hw.zeroBond(t, sched.front(), x) - hw.zeroBond(t, sched.back(), x);
+  u1 = F / A;
+  u2 = A;
+}
+void swapRateAnnuity(const Swap& tail, const HW& hw, const Scenarios& s, std::size_t p, std::size_t j, double& u1,
+                     double& u2) {
+  swapRateAnnuityAt(tail.sched, hw, s.grid[j], s.x(p, j), u1, u2);
+}
+
 BermudanValuation bermudan(const Trade& tr, const Generator& g, const Scenarios& s) {
   const std::size_t nP = s.P, nT = s.grid.size(), nE = tr.exercise.size();
   std::vector<std::size_t> eIdx(nE);
@@ -766,6 +897,13 @@ BermudanValuation bermudan(const Trade& tr, const Generator& g, const Scenarios&
   Vec realised(nP, 0.0), target(nP), xcol(nP);
   std::vector<char> itm(nP);
   PolyReg reg(tr.degree);
+  // Swap-rate / annuity basis: the regressors of the tail entered at exercise k, at date j.
+  const bool sa = gBasis == 1;
+  SaReg sreg(tr.degree);
+  Vec u1(nP), u2(nP);
+  auto features = [&](std::size_t k, std::size_t j) {
+    for (std::size_t p = 0; p < nP; ++p) swapRateAnnuity(tails[k], g.rates, s, p, j, u1[p], u2[p]);
+  };
   for (std::size_t k = nE; k-- > 0;) {
     const std::size_t j = eIdx[k];
     for (std::size_t p = 0; p < nP; ++p) {
@@ -773,10 +911,15 @@ BermudanValuation bermudan(const Trade& tr, const Generator& g, const Scenarios&
       target[p] = realised[p] / s.D(p, j);
       itm[p] = ev[k][p] > 0.0;
     }
-    reg.fit(xcol, target, &itm);
+    if (sa) {
+      features(k, j);
+      sreg.fit(u1, u2, target, &itm);
+    } else {
+      reg.fit(xcol, target, &itm);
+    }
     for (std::size_t p = 0; p < nP; ++p) {
       const double e = ev[k][p];
-      if (e > 0.0 && e > reg.predict(xcol[p])) {
+      if (e > 0.0 && e > (sa ? sreg.predict(u1[p], u2[p]) : reg.predict(xcol[p]))) {
         exercise[k][p] = 1;
         realised[p] = s.D(p, j) * e;
       }
@@ -802,8 +945,14 @@ BermudanValuation bermudan(const Trade& tr, const Generator& g, const Scenarios&
         xcol[p] = s.x(p, j);
         target[p] = strategy[nextK][p] / s.D(p, j);
       }
-      reg.fit(xcol, target, nullptr);
-      for (std::size_t p = 0; p < nP; ++p) cont[p] = std::max(reg.predict(xcol[p]), 0.0);
+      if (sa) {
+        features(nextK, j);
+        sreg.fit(u1, u2, target, nullptr);
+        for (std::size_t p = 0; p < nP; ++p) cont[p] = std::max(sreg.predict(u1[p], u2[p]), 0.0);
+      } else {
+        reg.fit(xcol, target, nullptr);
+        for (std::size_t p = 0; p < nP; ++p) cont[p] = std::max(reg.predict(xcol[p]), 0.0);
+      }
     } else {
       std::fill(cont.begin(), cont.end(), 0.0);
     }
@@ -1374,6 +1523,36 @@ int32_t ccr_oracle_solve(const double* a, const double* b, int32_t n, double* x)
     std::copy(r.begin(), r.end(), x);
   });
 }
+int32_t ccr_oracle_swap_rate_annuity(const ccr_hull_white* hw, const double* sched, int32_t n, double t, double x,
+                                     double* rate, double* annuity) {
+  return guard(nullptr, 0, [&] {
+    if (n < 2) invalid("swap rate: the schedule needs two dates");
+    swapRateAnnuityAt(Vec(sched, sched + n), HW(*hw), t, x, *rate, *annuity);
+  });
+}
+
+int32_t ccr_oracle_sa_regression(const double* u1, const double* u2, const double* y, const uint8_t* mask, int64_t n,
+                                 int32_t degree, double* coefficients, double* params) {
+  return guard(nullptr, 0, [&] {
+    SaReg reg(static_cast<std::size_t>(std::max(0, degree)));
+    std::vector<char> m;
+    if (mask) m.assign(mask, mask + n);
+    reg.fit(Vec(u1, u1 + n), Vec(u2, u2 + n), Vec(y, y + n), mask ? &m : nullptr);
+    for (int i = 0; i < 10; ++i) coefficients[i] = i < static_cast<int>(reg.coef.size()) ? reg.coef[static_cast<std::size_t>(i)] : 0.0;
+    params[0] = reg.m1;
+    params[1] = reg.s1;
+    params[2] = reg.m2;
+    params[3] = reg.s2;
+    params[4] = static_cast<double>(reg.deg);
+  });
+}
+
+int32_t ccr_oracle_set_regression_basis(int32_t basis) {
+  if (basis < -1 || basis > 1) return CCR_ERR_INVALID_ARGUMENT;
+  gBasis = basis;
+  return CCR_OK;
+}
+
 int32_t ccr_oracle_regression(const double* x, const double* y, const uint8_t* mask, int64_t n, int32_t degree,
                               double* coefficients, double* center, double* scale, int32_t* effective_degree) {
   return guard(nullptr, 0, [&] {

@@ -80,6 +80,17 @@ int32_t ccr_oracle_cholesky(const double* a, int32_t n, double* l);
 int32_t ccr_oracle_solve(const double* a, const double* b, int32_t n, double* x);
 /* PolynomialRegression::fit on masked samples (mask NULL = all); coefficients (degree+1
  * entries, the unused ones 0), center, scale and the effective degree. */
+/* Basis of the Bermudan regressions of later runs: -1 / 0 the reference's state basis
+ * (bermudan_swaption.cpp:77-93), 1 the swap-rate / annuity basis (total degree <= 3 in the
+ * standardised swap rate and annuity of the tail swap; no reference counterpart). */
+int32_t ccr_oracle_set_regression_basis(int32_t basis);
+/* Swap rate (P(t,T_0) - P(t,T_n)) / A and annuity A = sum_i tau_i P(t,T_i) of a schedule at (t, x). */
+int32_t ccr_oracle_swap_rate_annuity(const ccr_hull_white* hw, const double* sched, int32_t n, double t, double x,
+                                     double* rate, double* annuity);
+/* The swap-rate / annuity regression on given regressors: coefficients[10] (zeros past the
+ * effective terms) and params = mean / sd of each regressor, effective total degree. */
+int32_t ccr_oracle_sa_regression(const double* u1, const double* u2, const double* y, const uint8_t* mask, int64_t n,
+                                 int32_t degree, double* coefficients, double* params);
 int32_t ccr_oracle_regression(const double* x, const double* y, const uint8_t* mask, int64_t n, int32_t degree,
                               double* coefficients, double* center, double* scale, int32_t* effective_degree);
 /* simulateCollateral on a [paths][dates] value matrix (unit_price NULL for cash). */

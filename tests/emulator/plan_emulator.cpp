@@ -50,6 +50,124 @@ void fitRecord(const std::vector<double>& x, const std::vector<double>& y, const
   rec[7] = 0.0;
 }
 
+// Regression basis of the plans built by the emulator (emu_set_regression_basis).
+int gBasis = -1;
+
+// The regression engine on the CPU: every record of the plan (exercise records per swaption from
+// the last exercise down, then the continuation records) fitted with the product's
+// basisParams / basisRow / basisSolve arithmetic, moments and Gram summed in path order.
+struct EngineState {
+  std::vector<double> rfit;
+};
+
+void runEngine(HostPlan& h, PathArgs<double>& a, const std::vector<std::vector<std::vector<double>>>& ev,
+               std::vector<double>& cont, std::vector<int16_t>& exidx, EngineState& eng) {
+  const int P = h.g.num_paths;
+  const double* X = a.states;
+  const double* Dv = a.states + h.state_len;
+  eng.rfit.assign(h.rrecs.size() * kBasisStride, 0.0);
+  std::vector<std::vector<std::vector<double>>> strategy(h.berms.size());
+  for (size_t b = 0; b < h.berms.size(); ++b)
+    strategy[b].assign(static_cast<size_t>(h.berms[b].ne), std::vector<double>(static_cast<size_t>(P), 0.0));
+  auto features = [&](const RegRec& rr, double x, double& u1, double& u2) {
+    u1 = x;
+    u2 = 0.0;
+    if (h.basis != kBasisSwapRateAnnuity) return;
+    const SetDev& S = h.sets[static_cast<size_t>(h.berms[static_cast<size_t>(rr.b)].set)];
+    auto noFix = [](int) { return 0.0; };
+    const double* bb = a.bonds + 2 * static_cast<int64_t>(S.bond_off);
+    const double* ff = a.fixed + 4 * static_cast<int64_t>(S.fixed_off);
+    const double* pp = a.poly + S.poly_off;
+    const double A = evalRange<double>(h.saterms[static_cast<size_t>(rr.ann)], x, bb, ff, pp, noFix);
+    const double F = evalRange<double>(h.saterms[static_cast<size_t>(rr.flt)], x, bb, ff, pp, noFix);
+    u1 = F / A;
+    u2 = A;
+  };
+  // Sample of record r on path p: in / features / target numerator.
+  auto sample = [&](int r, int p, bool& in, double& u1, double& u2, double& y, double& e, double& D, double& num) {
+    const RegRec& rr = h.rrecs[static_cast<size_t>(r)];
+    const BermDev& bm = h.berms[static_cast<size_t>(rr.b)];
+    const SetDev& S = h.sets[static_cast<size_t>(bm.set)];
+    const int64_t at = S.state_off + static_cast<int64_t>(rr.j) * P + p;
+    D = Dv[at];
+    const auto& st = strategy[static_cast<size_t>(rr.b)];
+    if (rr.kind == 0) {
+      e = ev[static_cast<size_t>(rr.b)][static_cast<size_t>(rr.k)][static_cast<size_t>(p)];
+      in = e > 0.0;
+      num = rr.k + 1 < bm.ne ? st[static_cast<size_t>(rr.k + 1)][static_cast<size_t>(p)] : 0.0;
+    } else {
+      e = 0.0;
+      in = true;
+      num = st[static_cast<size_t>(rr.k)][static_cast<size_t>(p)];
+    }
+    y = num / D;
+    features(rr, X[at], u1, u2);
+  };
+  auto fit = [&](int r) {
+    const RegRec& rr = h.rrecs[static_cast<size_t>(r)];
+    double m[kBasisMom] = {0, 0, 0, 0, 0, 0};
+    bool in;
+    double u1, u2, y, e, D, num;
+    for (int p = 0; p < P; ++p) {
+      sample(r, p, in, u1, u2, y, e, D, num);
+      if (!in) continue;
+      m[0] += 1.0;
+      m[1] += u1;
+      m[2] += u1 * u1;
+      m[3] += u2;
+      m[4] += u2 * u2;
+      m[5] += y;
+    }
+    double* rp = eng.rfit.data() + static_cast<size_t>(r) * kBasisStride;
+    basisParams(h.basis, h.berms[static_cast<size_t>(rr.b)].degree, m, rp);
+    if (rp[5] < 1.0) return;
+    double g[kBasisGram] = {0};
+    for (int p = 0; p < P; ++p) {
+      sample(r, p, in, u1, u2, y, e, D, num);
+      if (!in) continue;
+      double phi[kBasisMax + 1];
+      basisRow(h.basis, rp, u1, u2, phi);
+      phi[kBasisMax] = y;
+      for (int i = 0; i <= kBasisMax; ++i)
+        for (int c = i; c <= kBasisMax; ++c) g[gramIndex(i, c)] += phi[i] * phi[c];
+    }
+    basisSolve(h.basis, g, rp);
+  };
+  for (int k = h.max_exercises - 1; k >= 0; --k)
+    for (size_t b = 0; b < h.berms.size(); ++b) {
+      const int r = h.rex[b * static_cast<size_t>(h.max_exercises) + static_cast<size_t>(k)];
+      if (r < 0) continue;
+      fit(r);
+      const double* rp = eng.rfit.data() + static_cast<size_t>(r) * kBasisStride;
+      int16_t* ex = exidx.data() + b * static_cast<size_t>(P);
+      for (int p = 0; p < P; ++p) {
+        bool in;
+        double u1, u2, y, e, D, num;
+        sample(r, p, in, u1, u2, y, e, D, num);
+        double realised = num;
+        if (e > 0.0 && e > basisPredict(h.basis, rp, u1, u2)) {
+          ex[p] = static_cast<int16_t>(k);
+          realised = D * e;
+        }
+        strategy[b][static_cast<size_t>(k)][static_cast<size_t>(p)] = realised;
+      }
+    }
+  for (size_t r = static_cast<size_t>(h.nex_recs); r < h.rrecs.size(); ++r) fit(static_cast<int>(r));
+  for (size_t r = 0; r < h.bdates.size(); ++r) {
+    double* out = cont.data() + r * kContStride;
+    for (int i = 0; i < kContStride; ++i) out[i] = 0.0;
+    out[1] = 1.0;
+    out[6] = -1.0;
+    const int rec = h.rcont[r];
+    if (rec < 0 || h.basis != kBasisState) continue;
+    const double* rp = eng.rfit.data() + static_cast<size_t>(rec) * kBasisStride;
+    out[0] = rp[1];
+    out[1] = rp[2];
+    for (int i = 0; i < 4; ++i) out[2 + i] = rp[6 + i];
+    out[6] = rp[5];
+  }
+}
+
 // The American Monte Carlo stage on the CPU: states pass, exercise values, backward induction
 // and continuation fits, filling cont and exidx for the valuation pass.
 void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::vector<double>& cont,
@@ -67,6 +185,8 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
   }
   const double* X = states.data();
   const double* Dv = states.data() + h.state_len;
+  EngineState eng;
+  std::vector<std::vector<std::vector<double>>> evAll(h.basis >= 0 ? h.berms.size() : 0);
   for (size_t b = 0; b < h.berms.size(); ++b) {
     const BermDev& bm = h.berms[b];
     const SetDev& S = h.sets[static_cast<size_t>(bm.set)];
@@ -79,6 +199,12 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
             evalRange<double>(h.evterms[static_cast<size_t>(bm.ev_rec + k)], X[S.state_off + static_cast<int64_t>(j) * P + p],
                               a.bonds + 2 * static_cast<int64_t>(S.bond_off), a.fixed + 4 * static_cast<int64_t>(S.fixed_off),
                               a.poly + S.poly_off, [](int) { return 0.0; });
+    }
+    std::fill(exidx.data() + b * static_cast<size_t>(P), exidx.data() + (b + 1) * static_cast<size_t>(P),
+              static_cast<int16_t>(ne));
+    if (h.basis >= 0) {
+      evAll[b] = ev;
+      continue;
     }
     std::vector<std::vector<double>> strategy(static_cast<size_t>(ne) + 1, std::vector<double>(static_cast<size_t>(P), 0.0));
     std::vector<double> realised(static_cast<size_t>(P), 0.0), x(static_cast<size_t>(P)), y(static_cast<size_t>(P));
@@ -118,6 +244,19 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
       fitRecord(x, y, all, bm.degree, rec);
     }
   }
+  if (h.basis >= 0) {
+    PathArgs<double> e = a;
+    e.states = states.data();
+    runEngine(h, e, evAll, cont, exidx, eng);
+  }
+  BasisView view;
+  view.basis = h.basis;
+  if (h.basis == kBasisSwapRateAnnuity) {
+    view.rrecs = h.rrecs.data();
+    view.rcont = h.rcont.data();
+    view.rfit = eng.rfit.data();
+    view.saterms = h.saterms.data();
+  }
   const int ncd = static_cast<int>(h.cdates.size() / 2);
   bermv.assign(static_cast<size_t>(ncd) * P, 0.0);
   for (int d = 0; d < ncd; ++d) {
@@ -128,7 +267,7 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
           bermudanSum<double>(S, j, p, P, h.bdates.data(), cont.data(), exidx.data(), states.data(),
                               a.bonds + 2 * static_cast<int64_t>(S.bond_off), a.fixed + 4 * static_cast<int64_t>(S.fixed_off),
                               a.poly + S.poly_off, h.clive.data(), h.clive_rng[2 * static_cast<size_t>(d)],
-                              h.clive_rng[2 * static_cast<size_t>(d) + 1]);
+                              h.clive_rng[2 * static_cast<size_t>(d) + 1], view);
   }
   a.bermv = bermv.data();
 }
@@ -138,6 +277,10 @@ int32_t runExposure(const ccr_scenario_generator* gen, const ccr_simulation_conf
                     char* err, int32_t cap, bool book);
 
 }  // namespace
+
+// Bermudan regressions of later runs: -1 the shared-design state basis, 0 / 1 the regression
+// engine with the state / swap-rate-annuity basis.
+extern "C" void emu_set_regression_basis(int32_t basis) { gBasis = basis; }
 
 extern "C" int32_t emu_run_exposure(const ccr_scenario_generator* gen, const ccr_simulation_config* cfg,
                                     const double* base, int32_t nbase, const ccr_netting_set* sets, int32_t nsets,
@@ -158,7 +301,8 @@ int32_t runExposure(const ccr_scenario_generator* gen, const ccr_simulation_conf
                     int32_t nbase, const ccr_netting_set* sets, int32_t nsets, double q, ccr_exposure_out* outs,
                     char* err, int32_t cap, bool book) {
   try {
-    HostPlan h = buildPlan(*gen, *cfg, base, nbase, sets, nsets, q, CCR_FP64, 256, false, book);
+    HostPlan h = buildPlan(*gen, *cfg, base, nbase, sets, nsets, q, CCR_FP64, 256, false, book, false, nullptr,
+                           nullptr, gBasis);
     std::vector<double> stream(static_cast<size_t>(h.stream_len));
     generateStream(cfg->seed, h.stream_len, stream.data());
     std::vector<double> steps = h.steps, dreal = h.date_reals, bonds = h.bonds, fixed = h.fixed;

@@ -92,6 +92,9 @@ def _bind(lib):
         lib.ccr_oracle_cholesky.argtypes = [P, I32, P]
         lib.ccr_oracle_solve.argtypes = [P, P, I32, P]
         lib.ccr_oracle_regression.argtypes = [P, P, P, I64, I32, P, P, P, P]
+        lib.ccr_oracle_set_regression_basis.argtypes = [I32]
+        lib.ccr_oracle_swap_rate_annuity.argtypes = [P, P, I32, D, D, P, P]
+        lib.ccr_oracle_sa_regression.argtypes = [P, P, P, P, I64, I32, P, P]
         lib.ccr_oracle_collateral.argtypes = [P, P, P, I32, I32, P, P]
         lib.ccr_oracle_profile.argtypes = [P, P, I32, I32, D, P, P, P, P, P, P]
         lib.ccr_oracle_bermudan.argtypes = [P, P, P, I32, P, P, P]

@@ -36,6 +36,9 @@ CCR_COLLATERAL_ASSET = 2
 CCR_FP64 = 0
 CCR_FP32 = 1
 
+CCR_BASIS_STATE = 0
+CCR_BASIS_SWAP_RATE_ANNUITY = 1
+
 CCR_FAULT_NONE = 0
 CCR_FAULT_ALLOC = 1
 CCR_FAULT_LAUNCH = 2

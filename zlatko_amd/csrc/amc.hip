@@ -453,11 +453,300 @@ __global__ void __launch_bounds__(kAmcChunk) amc_value_kernel(AmcArgs a, const R
   const int j = toConst(a.cdates)[2 * d + 1];
   const int64_t p0 = static_cast<int64_t>(chunk) * kAmcChunk + threadIdx.x;
   const int64_t p = p0 < a.paths ? p0 : a.paths - 1;
+  BasisView eng;
+  eng.basis = a.basis;
+  if (a.basis == kBasisSwapRateAnnuity) {
+    eng.rrecs = toConst(a.rrecs);
+    eng.rcont = toConst(a.rcont);
+    eng.rfit = toConst(static_cast<const double*>(a.rfit));
+    eng.saterms = toConst(a.saterms);
+  }
   const Real v = bermudanSum<Real>(S, j, p, a.paths, toConst(a.bdates), toConst(a.cont), a.exidx,
                                    a.states, toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off),
                                    toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(polyOf<Real>(a)) + S.poly_off,
-                                   toConst(a.clive), toConst(a.clive_rng)[2 * d], toConst(a.clive_rng)[2 * d + 1]);
+                                   toConst(a.clive), toConst(a.clive_rng)[2 * d], toConst(a.clive_rng)[2 * d + 1], eng);
   if (p0 < a.paths) a.bermv[static_cast<int64_t>(d) * a.paths + p0] = static_cast<double>(v);
+}
+
+// ---------------------------------------------------------------- regression engine (basis choice)
+// Records (layout.hpp RegRec) are regressed in the reference's order: exercise k of every
+// swaption at once, k from the last exercise down, then every continuation record at once. Per
+// batch: moments per kBasisChunk paths -> standardisation and degree per record -> Gram of
+// [phi | y] per kBasisChunk paths (FP64 VALU, or FP32 matrix cores) -> chunk sums in chunk order
+// and the FP64 solve per record -> the exercise decision (exercise batches).
+
+// Record of grid row `y` of a batch: exercise k of swaption y (k >= 0), or continuation record y.
+__device__ inline int basisRecord(const AmcArgs& a, int k, int y) {
+  return k >= 0 ? a.rex[static_cast<int64_t>(y) * a.max_exercises + k] : a.nex_recs + y;
+}
+
+// Target of record rr on path p: deflator, exercise value (exercise records), numerator of the
+// target (the value realised after this exercise, or the strategy of the next exercise) and
+// whether the path enters the fit (exercise records: in the money; continuation: every path).
+struct BasisTarget {
+  double x, D, e, num;
+  bool in;
+};
+__device__ inline BasisTarget basisTarget(const AmcArgs& a, const CCR_CONST RegRec& rr, const CCR_CONST BermDev& bm,
+                                          int64_t p) {
+  const CCR_CONST SetDev& S = toConst(a.sets)[bm.set];
+  const int64_t at = S.state_off + static_cast<int64_t>(rr.j) * a.paths + p;
+  BasisTarget t;
+  t.x = a.states[at];
+  t.D = a.states[a.state_len + at];
+  if (rr.kind == 0) {
+    t.e = a.ev[static_cast<int64_t>(bm.ev_off + rr.k) * a.paths + p];
+    t.in = t.e > 0.0;
+    t.num = rr.k + 1 < bm.ne ? a.strategy[static_cast<int64_t>(bm.ev_off + rr.k + 1) * a.paths + p] : 0.0;
+  } else {
+    t.e = 0.0;
+    t.in = true;
+    t.num = a.strategy[static_cast<int64_t>(bm.ev_off + rr.k) * a.paths + p];
+  }
+  return t;
+}
+
+// Regressors of record rr at state x: the state itself, or the swap rate F / A and annuity A of
+// the tail (the annuity and float-leg terms valued in the plan's precision).
+template <typename Real>
+__device__ inline void basisFeatures(const AmcArgs& a, const CCR_CONST RegRec& rr, const CCR_CONST BermDev& bm,
+                                     double x, const Real* bonds, const Real* fixed, double& u1, double& u2) {
+  if (a.basis != kBasisSwapRateAnnuity) {
+    u1 = x;
+    u2 = 0.0;
+    return;
+  }
+  const CCR_CONST SetDev& S = toConst(a.sets)[bm.set];
+  cptr<Real> bb = toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off);
+  cptr<Real> ff = toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off);
+  cptr<Real> pp = toConst(polyOf<Real>(a)) + S.poly_off;
+  auto noFix = [](int) { return Real(0); };
+  const Real xr = static_cast<Real>(x);
+  const double A = static_cast<double>(evalRange<Real>(toConst(a.saterms)[rr.ann], xr, bb, ff, pp, noFix));
+  const double F = static_cast<double>(evalRange<Real>(toConst(a.saterms)[rr.flt], xr, bb, ff, pp, noFix));
+  u1 = F / A;
+  u2 = A;
+}
+
+// Sums N partials laid out [chunk][N] from q0 over the chunks in a fixed order (lane l takes
+// chunks l, l + 64, ...), then a DPP wavefront sum. Valid in every lane.
+template <int N>
+__device__ inline void chunkSumAt(const double* q0, int nchunks, double (&s)[N]) {
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+#pragma unroll
+  for (int i = 0; i < N; ++i) s[i] = 0.0;
+  for (int c = lane; c < nchunks; c += 64) {
+    const double* q = q0 + static_cast<int64_t>(c) * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) s[i] += q[i];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) s[i] = waveSum(s[i]);
+}
+
+__device__ inline double* recordPart(const AmcArgs& a, int rec) {
+  return a.rpart + static_cast<int64_t>(rec) * a.nbc * kBasisGram;
+}
+
+constexpr int kBasisRounds = kBasisChunk / 256;  // paths per thread of a moment / Gram workgroup
+
+// Moments n, sum u1, sum u1^2, sum u2, sum u2^2, sum y over the record's sample in one path chunk.
+template <typename Real>
+__global__ void __launch_bounds__(256) amc_rmom_kernel(AmcArgs a, const Real* bonds, const Real* fixed, int k) {
+  __shared__ double lds[4 * kBasisMom];
+  const int rec = basisRecord(a, k, static_cast<int>(blockIdx.y));
+  if (rec < 0) return;
+  const CCR_CONST RegRec& rr = toConst(a.rrecs)[rec];
+  const CCR_CONST BermDev& bm = toConst(a.berms)[rr.b];
+  double v[kBasisMom] = {0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < kBasisRounds; ++r) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * kBasisChunk + r * 256 + threadIdx.x;
+    if (p >= a.paths) break;
+    const BasisTarget t = basisTarget(a, rr, bm, p);
+    if (!t.in) continue;
+    double u1, u2;
+    basisFeatures<Real>(a, rr, bm, t.x, bonds, fixed, u1, u2);
+    v[0] += 1.0;
+    v[1] += u1;
+    v[2] += u1 * u1;
+    v[3] += u2;
+    v[4] += u2 * u2;
+    v[5] += t.num / t.D;
+  }
+  blockSum<kBasisMom>(v, lds, 4);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBasisMom; ++i) recordPart(a, rec)[static_cast<int64_t>(blockIdx.x) * kBasisMom + i] = v[i];
+}
+
+// One wavefront per record: standardisation and effective degree (basisParams).
+__global__ void __launch_bounds__(256) amc_rfin_kernel(AmcArgs a, int k, int ny) {
+  const int y = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (y >= ny) return;
+  const int rec = basisRecord(a, k, y);
+  if (rec < 0) return;
+  double s[kBasisMom];
+  chunkSumAt<kBasisMom>(recordPart(a, rec), a.nbc, s);
+  if ((threadIdx.x & 63) != 0) return;
+  double rp[kBasisStride];
+  basisParams(a.basis, a.berms[a.rrecs[rec].b].degree, s, rp);
+  for (int i = 0; i < kBasisStride; ++i) a.rfit[static_cast<int64_t>(rec) * kBasisStride + i] = rp[i];
+}
+
+// Design row of a sample: the basis terms and the target in column 10.
+template <typename Real>
+__device__ inline bool basisSampleRow(const AmcArgs& a, const CCR_CONST RegRec& rr, const CCR_CONST BermDev& bm,
+                                      const double* rp, int64_t p, const Real* bonds, const Real* fixed,
+                                      double (&phi)[kBasisMax + 1]) {
+  const BasisTarget t = basisTarget(a, rr, bm, p);
+  if (!t.in) return false;
+  double u1, u2;
+  basisFeatures<Real>(a, rr, bm, t.x, bonds, fixed, u1, u2);
+  basisRow(a.basis, rp, u1, u2, phi);
+  phi[kBasisMax] = t.num / t.D;
+  return true;
+}
+
+// Gram of [phi | y] over one path chunk in FP64: each thread sums its 16 paths, the block
+// combines the wavefront sums in wavefront order.
+template <typename Real>
+__global__ void __launch_bounds__(256) amc_rgram_kernel(AmcArgs a, const Real* bonds, const Real* fixed, int k) {
+  __shared__ double lds[4 * kBasisGram];
+  const int rec = basisRecord(a, k, static_cast<int>(blockIdx.y));
+  if (rec < 0) return;
+  double rp[6];
+  for (int i = 0; i < 6; ++i) rp[i] = toConst(static_cast<const double*>(a.rfit))[static_cast<int64_t>(rec) * kBasisStride + i];
+  if (rp[5] < 1.0) return;  // degree 0 or no sample: no Gram
+  const CCR_CONST RegRec& rr = toConst(a.rrecs)[rec];
+  const CCR_CONST BermDev& bm = toConst(a.berms)[rr.b];
+  double v[kBasisGram];
+#pragma unroll
+  for (int i = 0; i < kBasisGram; ++i) v[i] = 0.0;
+  for (int r = 0; r < kBasisRounds; ++r) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * kBasisChunk + r * 256 + threadIdx.x;
+    if (p >= a.paths) break;
+    double phi[kBasisMax + 1];
+    if (!basisSampleRow<Real>(a, rr, bm, rp, p, bonds, fixed, phi)) continue;
+#pragma unroll
+    for (int i = 0; i <= kBasisMax; ++i)
+#pragma unroll
+      for (int c = i; c <= kBasisMax; ++c) v[gramIndex(i, c)] += phi[i] * phi[c];
+  }
+  blockSum<kBasisGram>(v, lds, 4);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBasisGram; ++i) recordPart(a, rec)[static_cast<int64_t>(blockIdx.x) * kBasisGram + i] = v[i];
+}
+
+// The same Gram on FP32 matrix cores (v_mfma_f32_16x16x4_f32): per round every wavefront stages
+// the FP32 design rows [phi | y | 0...] of 64 paths in LDS (row = path, 16 columns) and takes
+// 16 rank-4 updates G += R^T R, R the 4 x 16 block of rows 4s..4s+3: lane l supplies R[l / 16][l % 16]
+// as both the A (16 x 4) and the B (4 x 16) operand. The accumulator holds G[m][n] at lane
+// n + 16 (m / 4), element m % 4 (G is symmetric, so either orientation reads alike); the four
+// wavefront Grams are added in FP64 in wavefront order.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <typename Real>
+__global__ void __launch_bounds__(256) amc_rgram32_kernel(AmcArgs a, const Real* bonds, const Real* fixed, int k) {
+  __shared__ float rows[4][64][17];
+  __shared__ float gw[4][16][17];
+  const int rec = basisRecord(a, k, static_cast<int>(blockIdx.y));
+  if (rec < 0) return;
+  double rp[6];
+  for (int i = 0; i < 6; ++i) rp[i] = toConst(static_cast<const double*>(a.rfit))[static_cast<int64_t>(rec) * kBasisStride + i];
+  if (rp[5] < 1.0) return;
+  const CCR_CONST RegRec& rr = toConst(a.rrecs)[rec];
+  const CCR_CONST BermDev& bm = toConst(a.berms)[rr.b];
+  const int tid = static_cast<int>(threadIdx.x), w = tid >> 6, lane = tid & 63;
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int r = 0; r < kBasisRounds; ++r) {
+    // Wavefront w takes paths [w * kBasisChunk / 4 + 64 r, + 64) of the chunk.
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * kBasisChunk + w * (kBasisChunk / 4) + r * 64 + lane;
+    double phi[kBasisMax + 1];
+    const bool in = p < a.paths && basisSampleRow<Real>(a, rr, bm, rp, p, bonds, fixed, phi);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) rows[w][lane][c] = (in && c <= kBasisMax) ? static_cast<float>(phi[c]) : 0.0f;
+    __syncthreads();
+#pragma unroll
+    for (int s4 = 0; s4 < 16; ++s4) {
+      const float v = rows[w][4 * s4 + (lane >> 4)][lane & 15];
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(v, v, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) gw[w][4 * (lane >> 4) + i][lane & 15] = acc[i];
+  __syncthreads();
+  if (tid < kBasisGram) {
+    int i = 0, rem = tid;
+    while (rem >= kBasisMax + 1 - i) {
+      rem -= kBasisMax + 1 - i;
+      ++i;
+    }
+    const int c = i + rem;
+    double s = 0.0;
+    for (int q = 0; q < 4; ++q) s += static_cast<double>(gw[q][i][c]);
+    recordPart(a, rec)[static_cast<int64_t>(blockIdx.x) * kBasisGram + tid] = s;
+  }
+}
+
+// One wavefront per record: the chunk sums of the Gram and the FP64 solve (basisSolve).
+__global__ void __launch_bounds__(256) amc_rsolve_kernel(AmcArgs a, int k, int ny) {
+  const int y = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (y >= ny) return;
+  const int rec = basisRecord(a, k, y);
+  if (rec < 0) return;
+  double* fit = a.rfit + static_cast<int64_t>(rec) * kBasisStride;
+  if (fit[5] < 1.0) return;
+  double g[kBasisGram];
+  chunkSumAt<kBasisGram>(recordPart(a, rec), a.nbc, g);
+  if ((threadIdx.x & 63) != 0) return;
+  double rp[kBasisStride];
+  for (int i = 0; i < kBasisStride; ++i) rp[i] = fit[i];
+  basisSolve(a.basis, g, rp);
+  for (int i = 6; i < kBasisStride; ++i) fit[i] = rp[i];
+}
+
+// Exercise where e > 0 and e > prediction (bermudan_swaption.cpp:84-92); record the strategy.
+template <typename Real>
+__global__ void __launch_bounds__(kAmcChunk) amc_rupdate_kernel(AmcArgs a, const Real* bonds, const Real* fixed, int k) {
+  const int b = static_cast<int>(blockIdx.y);
+  const int rec = basisRecord(a, k, b);
+  if (rec < 0) return;
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * kAmcChunk + threadIdx.x;
+  if (p >= a.paths) return;
+  const CCR_CONST RegRec& rr = toConst(a.rrecs)[rec];
+  const CCR_CONST BermDev& bm = toConst(a.berms)[b];
+  const BasisTarget t = basisTarget(a, rr, bm, p);
+  double realised = t.num;
+  if (t.e > 0.0) {
+    double rp[kBasisStride];
+    for (int i = 0; i < kBasisStride; ++i)
+      rp[i] = toConst(static_cast<const double*>(a.rfit))[static_cast<int64_t>(rec) * kBasisStride + i];
+    double u1, u2;
+    basisFeatures<Real>(a, rr, bm, t.x, bonds, fixed, u1, u2);
+    if (t.e > basisPredict(a.basis, rp, u1, u2)) {
+      a.exidx[static_cast<int64_t>(b) * a.paths + p] = static_cast<int16_t>(k);
+      realised = t.D * t.e;
+    }
+  }
+  a.strategy[static_cast<int64_t>(bm.ev_off + k) * a.paths + p] = realised;
+}
+
+// Per (swaption, date) record the continuation fit in the valuation's layout: the state basis as
+// center, scale, c0..c3, degree (the swap-rate / annuity basis is read from its fit records);
+// records without a valued continuation get degree -1.
+__global__ void __launch_bounds__(256) amc_rcont_kernel(AmcArgs a) {
+  const int r = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= a.nbdates) return;
+  double* out = a.cont + static_cast<int64_t>(r) * kContStride;
+  for (int i = 0; i < kContStride; ++i) out[i] = 0.0;
+  out[1] = 1.0;
+  out[6] = -1.0;
+  const int rec = a.rcont[r];
+  if (rec < 0 || a.basis != kBasisState) return;
+  const double* rp = a.rfit + static_cast<int64_t>(rec) * kBasisStride;
+  out[0] = rp[1];
+  out[1] = rp[2];
+  for (int i = 0; i < 4; ++i) out[2 + i] = rp[6 + i];
+  out[6] = rp[5];
 }
 
 // Surrogate inputs per path: the date loop of one set; running extremes of the domestic state and
@@ -534,6 +823,32 @@ hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int
   if (hipError_t e = last()) return e;
   // One wavefront per item in the finishing kernels.
   const dim3 gb(a.nchunks, a.nberms), gs((a.nberms + 3) / 4);
+  if (a.basis >= 0) {
+    // The regression engine: exercise batches, then every continuation record.
+    auto batch = [&](int k, int ny) {
+      const dim3 gr(a.nbc, ny), gf((ny + 3) / 4);
+      hipLaunchKernelGGL((amc_rmom_kernel<Real>), gr, dim3(256), 0, stream, a, bonds, fixed, k);
+      hipLaunchKernelGGL(amc_rfin_kernel, gf, dim3(256), 0, stream, a, k, ny);
+      if (a.gram32)
+        hipLaunchKernelGGL((amc_rgram32_kernel<Real>), gr, dim3(256), 0, stream, a, bonds, fixed, k);
+      else
+        hipLaunchKernelGGL((amc_rgram_kernel<Real>), gr, dim3(256), 0, stream, a, bonds, fixed, k);
+      hipLaunchKernelGGL(amc_rsolve_kernel, gf, dim3(256), 0, stream, a, k, ny);
+    };
+    for (int k = max_exercises - 1; k >= 0; --k) {
+      batch(k, a.nberms);
+      hipLaunchKernelGGL((amc_rupdate_kernel<Real>), gb, blk, 0, stream, a, bonds, fixed, k);
+      if (hipError_t e = last()) return e;
+    }
+    const int ncont = a.nrecs - a.nex_recs;
+    if (ncont > 0) batch(-1, ncont);
+    hipLaunchKernelGGL(amc_rcont_kernel, dim3((a.nbdates + 255) / 256), dim3(256), 0, stream, a);
+    if (hipError_t e = last()) return e;
+    if (a.ncdates > 0)
+      hipLaunchKernelGGL((amc_value_kernel<Real>), dim3((a.nchunks + 7) / 8 * 8, a.ncdates), blk, 0, stream, a,
+                         bonds, fixed);
+    return last();
+  }
   for (int k = max_exercises - 1; k >= 0; --k) {
     hipLaunchKernelGGL(amc_mom1_kernel, gb, blk, 0, stream, a, k);
     hipLaunchKernelGGL(amc_fin1_kernel, gs, dim3(256), 0, stream, a, k);

@@ -68,6 +68,11 @@ struct ccr_hip_ctx {
   // Two: the per-set LDS rows (ring, collateral units, previous exposures) of two sets leave five
   // waves per SIMD (500-set portfolio: path kernel 459 ms at 2, 607 ms at 4, 964 ms at 8).
   int book_group = 2;
+  // Bermudan regressions of plans compiled while set: -1 the reference's state basis on the
+  // shared per-date designs; CCR_BASIS_STATE / CCR_BASIS_SWAP_RATE_ANNUITY on the regression
+  // engine, its Grams in FP64 or on FP32 matrix cores.
+  int basis = -1;
+  int gram32 = 0;
   // Executes replay one captured HIP graph per plan while set.
   bool use_graph = false;
   // Launches are being captured into a graph (no timing events recorded), and whether the last
@@ -364,7 +369,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   plan->host = buildPlan(*generator, *config, base_grid, base_size, sets, num_sets, pfe_quantile, precision,
                          ctx->block > 0 ? ctx->block : kMaxBlock, ctx->per_trade, ctx->book,
                          ctx->im_quantile > 0.0, bookGrids ? &ctx->book_rep : nullptr,
-                         bookGrids ? &ctx->book_sim : nullptr);
+                         bookGrids ? &ctx->book_sim : nullptr, ctx->basis);
   HostPlan& h = plan->host;
   if (ctx->book && ctx->book_group > 1 && !ctx->per_trade && ctx->im_quantile <= 0.0 && h.opts.empty() &&
       h.sets.size() > 1)
@@ -481,6 +486,24 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
     a.ntiles = static_cast<int>(h.tiles.size());
     a.nnarrow = static_cast<int>(h.ntiles.size());
     a.nkc = nkc;
+    a.basis = -1;
+    if (h.basis >= 0) {
+      const int nrecs = static_cast<int>(h.rrecs.size());
+      const int nbc = static_cast<int>((P + kBasisChunk - 1) / kBasisChunk);
+      a.basis = h.basis;
+      a.gram32 = ctx->gram32;
+      a.nrecs = nrecs;
+      a.nex_recs = h.nex_recs;
+      a.nbc = nbc;
+      a.max_exercises = std::max(1, h.max_exercises);
+      a.rrecs = upload(ctx, plan.get(), h.rrecs.data(), h.rrecs.size());
+      a.rex = upload(ctx, plan.get(), h.rex.data(), h.rex.size());
+      a.rcont = upload(ctx, plan.get(), h.rcont.data(), h.rcont.size());
+      a.saterms = upload(ctx, plan.get(), h.saterms.data(), h.saterms.size());
+      execAlloc(ctx, plan.get(), a.rpart,
+                static_cast<size_t>(std::max(1, nrecs)) * nbc * kBasisGram * sizeof(double));
+      execAlloc(ctx, plan.get(), a.rfit, static_cast<size_t>(std::max(1, nrecs)) * kBasisStride * sizeof(double));
+    }
   }
   check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   plan->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1027,6 +1050,18 @@ int32_t ccr_hip_set_book_group(ccr_hip_ctx* ctx, int32_t sets_per_workgroup) {
     if (sets_per_workgroup > 1 && !groupSupported(sets_per_workgroup))
       fail(CCR_ERR_UNSUPPORTED, "book kernel: 1, 2, 4 or 8 netting sets per workgroup");
     ctx->book_group = sets_per_workgroup;
+  });
+}
+
+int32_t ccr_hip_set_regression(ccr_hip_ctx* ctx, int32_t basis, int32_t gram_precision) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (basis != -1 && basis != CCR_BASIS_STATE && basis != CCR_BASIS_SWAP_RATE_ANNUITY)
+      invalid("regression basis must be -1, CCR_BASIS_STATE or CCR_BASIS_SWAP_RATE_ANNUITY");
+    if (gram_precision != CCR_FP64 && gram_precision != CCR_FP32)
+      invalid("Gram precision must be CCR_FP64 or CCR_FP32");
+    ctx->basis = basis;
+    ctx->gram32 = gram_precision == CCR_FP32 ? 1 : 0;
   });
 }
 

@@ -23,6 +23,18 @@ __host__ __device__ inline cptr<T> toConst(const T* p) {
   return (cptr<T>)p;  // address-space cast (a C-style cast is the form clang accepts)
 }
 
+// The regression engine's continuation fits as the valuation reads them: with the swap-rate /
+// annuity basis, per (swaption, date) record its fit record rcont[record] (rfit, kBasisStride
+// reals) and the tail's annuity / float-leg terms; otherwise (basis < 0 or the state basis) the
+// per-record fits of `cont`.
+struct BasisView {
+  int basis = -1;
+  cptr<RegRec> rrecs = nullptr;
+  cptr<int32_t> rcont = nullptr;
+  cptr<double> rfit = nullptr;
+  cptr<TermRange> saterms = nullptr;
+};
+
 template <typename Real>
 struct PathArgs {
   const GlobalDev* g;
@@ -108,6 +120,16 @@ struct AmcArgs {
   int32_t nnarrow;
   int32_t ncchunks;           // continuation-date partials per date: ceil(nchunks / kContGroup)
   int32_t ncuniq;
+  // Regression engine with a basis choice (basis >= 0): records, exercise and continuation
+  // look-ups, the tails' annuity / float-leg terms, per-record moments / Gram partials
+  // [record][chunk][kBasisGram] and fits [record][kBasisStride].
+  int32_t basis, gram32, nrecs, nex_recs, nbc, max_exercises;
+  const RegRec* rrecs;
+  const int32_t* rex;
+  const int32_t* rcont;
+  const TermRange* saterms;
+  double* rpart;
+  double* rfit;
 };
 
 template <typename Real>

@@ -122,6 +122,36 @@ struct TermRange {
   double c0, xmax;
 };
 
+// Regression engine with a choice of basis (config 3's swap-rate / annuity regressions): one
+// record per regression -- the exercise decision at exercise k of a swaption (in-the-money paths,
+// target realised / D) or its continuation on a valued date (all paths, target strategy[k] / D,
+// k the next exercise) -- regressed on
+//   kBasisState:            1, z, z^2, z^3 of the standardised domestic state (the reference's
+//                           PolynomialRegression, bermudan_swaption.cpp:77-93);
+//   kBasisSwapRateAnnuity:  the 10 monomials of total degree <= 3 in the standardised swap rate S
+//                           and annuity A of the swaption's tail from exercise k:
+//                           1, s, a, s^2, s a, a^2, s^3, s^2 a, s a^2, a^3.
+// The Gram of [phi | y] (11 x 11) is summed per kBasisChunk paths on FP32 matrix cores or in FP64,
+// the chunk sums in FP64 in chunk order; the normal equations (ridge 1e-12 n, plus kSaRidge x the
+// diagonal for the swap-rate / annuity basis) are solved in FP64: elimination with partial
+// pivoting for the state basis, Cholesky for the swap-rate / annuity basis.
+constexpr int32_t kBasisState = CCR_BASIS_STATE, kBasisSwapRateAnnuity = CCR_BASIS_SWAP_RATE_ANNUITY;
+constexpr int kBasisMax = 10;      // regression terms
+constexpr int kBasisChunk = 4096;  // paths per workgroup of the moment and Gram passes (256 threads x 16)
+constexpr int kBasisMom = 6;       // n, sum u1, sum u1^2, sum u2, sum u2^2, sum y
+constexpr int kBasisGram = 66;     // upper triangle of the 11 x 11 Gram of [phi | y]
+constexpr int kBasisStride = 16;   // per record: n, m1, sd1, m2, sd2, degree, coefficients[10]
+constexpr double kSaRidge = 1e-4;  // swap-rate / annuity basis: ridge relative to the Gram diagonal
+constexpr double kSaDegenerate = 1e-6;  // ... and a regressor with sd <= this x its rms is constant
+struct RegRec {
+  int32_t b;      // global Bermudan
+  int32_t j;      // simulation date index on its set's grid
+  int32_t k;      // exercise index (kind 0) or the next exercise (kind 1)
+  int32_t kind;   // 0: exercise decision, 1: continuation
+  int32_t ann, flt;  // swap-rate / annuity basis: TermRanges of the tail's annuity and float leg at t_j
+  int32_t pad[2];
+};
+
 struct BermDev {
   int32_t set, ne, degree, cash;
   double position;

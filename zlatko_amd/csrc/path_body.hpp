@@ -177,7 +177,7 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
                                             cptr<BermDate> bdates, cptr<double> cont,
                                             const int16_t* exidx, const double* states, cptr<Real> bonds,
                                             cptr<Real> fixed, cptr<Real> poly, cptr<LiveRec> live, int lbeg,
-                                            int lend) {
+                                            int lend, const BasisView& eng) {
   const double* xrow = states + S.state_off;
   const double xd = xrow[static_cast<int64_t>(j) * paths + p];
   const Real x = static_cast<Real>(xd);
@@ -192,7 +192,19 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
     const CCR_CONST BermDate& bd = bdates[lr.rec];
     const bool exercised = exidx[static_cast<int64_t>(lr.b) * paths + p] < bd.next_k;
     Real v = 0;
-    if (bd.next_k < lr.ne) {
+    if (bd.next_k < lr.ne && eng.basis == kBasisSwapRateAnnuity && eng.rcont[lr.rec] >= 0) {
+      // The continuation fit of this (swaption, date) on the swap rate and annuity of the tail
+      // entered at the next exercise.
+      const int rid = eng.rcont[lr.rec];
+      const CCR_CONST RegRec& rr = eng.rrecs[rid];
+      auto noFix = [](int) { return Real(0); };
+      const double A = static_cast<double>(evalRange<Real>(eng.saterms[rr.ann], x, bonds, fixed, poly, noFix));
+      const double F = static_cast<double>(evalRange<Real>(eng.saterms[rr.flt], x, bonds, fixed, poly, noFix));
+      double rp[kBasisStride];
+      for (int i = 0; i < kBasisStride; ++i) rp[i] = eng.rfit[static_cast<int64_t>(rid) * kBasisStride + i];
+      const double pred = basisPredict(kBasisSwapRateAnnuity, rp, F / A, A);
+      v = static_cast<Real>(pred > 0.0 ? pred : 0.0);
+    } else if (bd.next_k < lr.ne && eng.basis != kBasisSwapRateAnnuity) {
       cptr<double> c = cont + static_cast<int64_t>(lr.rec) * kContStride;
       const double coef[4] = {c[2], c[3], c[4], c[5]};
       if (!haveZ || c[0] != zCenter || c[1] != zScale) {

@@ -466,7 +466,7 @@ void buildTiles(HostPlan& plan, const std::vector<int>& group, const std::vector
 HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
                    int32_t precision, int32_t block, bool per_trade, bool book, bool value_all,
-                   const std::vector<double>* book_rep, const std::vector<double>* book_sim) {
+                   const std::vector<double>* book_rep, const std::vector<double>* book_sim, int32_t basis) {
   if (!sets || num_sets <= 0) invalid("no netting sets");
   if (cfg.num_paths <= 0) invalid("ScenarioGenerator::generate: numPaths must be positive");
   if (cfg.num_paths > (int64_t(1) << 31) - 1) fail(CCR_ERR_UNSUPPORTED, "at most 2^31 - 1 paths");
@@ -476,6 +476,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
   const double polyTol = precision == CCR_FP32 ? kPolyToleranceFp32 : kPolyTolerance;
   const TimeGrid base(std::vector<double>(base_grid, base_grid + std::max(0, base_size)));
   HostPlan plan;
+  plan.basis = basis;
   plan.precision = precision;
   plan.pfe_q = pfe_quantile;
   plan.block = block;
@@ -550,6 +551,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     bookSlots = assignSlots(uses, static_cast<int>(bookSim.size()), bookSlotOf, bookStoreAt);
   }
   int64_t maxStride = 0;
+  std::vector<RegRec> exRecs, contRecs;  // regression-engine records (basis >= 0)
   // Book plans: swaptions of sets valued on the same dates (their valued-date flags), and those dates.
   std::map<std::vector<char>, std::pair<std::vector<int>, std::vector<int>>> bookGroups;
   for (int s = 0; s < num_sets; ++s) {
@@ -1034,6 +1036,52 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
         plan.bdate_berm.push_back(static_cast<int32_t>(plan.berms.size()));
         plan.bdate_cd.push_back(cdOf[static_cast<std::size_t>(j)]);
       }
+      // Regression-engine records: exercise k at its date (tail k), and the continuation of every
+      // valued date with an exercise ahead (tail of the next exercise). For the swap-rate /
+      // annuity basis each carries the annuity sum_i tau_i P(t, T_i) and the float leg
+      // P(t, T_start) - P(t, T_n) of its tail as value terms on its date.
+      if (basis >= 0) {
+        auto tailTerms = [&](int k, int j, RegRec& rr) {
+          rr.ann = rr.flt = -1;
+          if (basis != kBasisSwapRateAnnuity) return;
+          const double t = sim[static_cast<std::size_t>(j)];
+          std::vector<double> sched;
+          for (double x : tr.sched)
+            if (x >= tr.exercise[static_cast<std::size_t>(k)] - kPaidTolerance) sched.push_back(x);
+          Collector ann, flt;
+          ann.t = flt.t = t;
+          for (std::size_t i = 1; i < sched.size(); ++i) ann.addBond(0, sched[i], sched[i] - sched[i - 1]);
+          flt.addBond(0, sched.front(), 1.0);
+          flt.addBond(0, sched.back(), -1.0);
+          const double xmax = stateBound(m.rates[0], 0.0, t);
+          rr.ann = static_cast<int32_t>(plan.saterms.size());
+          plan.saterms.push_back(emitRange(ann, t, xmax, false));
+          rr.flt = static_cast<int32_t>(plan.saterms.size());
+          plan.saterms.push_back(emitRange(flt, t, xmax, false));
+        };
+        const int gb = static_cast<int>(plan.berms.size());
+        for (int k = 0; k < ne; ++k) {
+          RegRec rr{};
+          rr.b = gb;
+          rr.j = bEidx[static_cast<std::size_t>(b)][static_cast<std::size_t>(k)];
+          rr.k = k;
+          rr.kind = 0;
+          tailTerms(k, rr.j, rr);
+          exRecs.push_back(rr);
+        }
+        for (int j = 0; j < T; ++j) {
+          const int k = bNext[static_cast<std::size_t>(b)][static_cast<std::size_t>(j)];
+          if (cdOf[static_cast<std::size_t>(j)] < 0 || k >= ne) continue;
+          RegRec rr{};
+          rr.b = gb;
+          rr.j = j;
+          rr.k = k;
+          rr.kind = 1;
+          rr.pad[0] = bd.date_rec + j;  // its BermDate record
+          tailTerms(k, j, rr);
+          contRecs.push_back(rr);
+        }
+      }
       plan.ev_len += static_cast<int64_t>(ne) * P;
       plan.cont_len += static_cast<int64_t>(T) * kContStride;
       plan.max_exercises = std::max(plan.max_exercises, ne);
@@ -1070,6 +1118,20 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     std::vector<int> vcd;
     for (int j : kv.second.second) vcd.push_back(bookCd.at(j));
     buildTiles(plan, kv.second.first, kv.second.second, vcd);
+  }
+  if (basis >= 0 && !plan.berms.empty()) {
+    plan.nex_recs = static_cast<int32_t>(exRecs.size());
+    plan.rrecs = exRecs;
+    plan.rrecs.insert(plan.rrecs.end(), contRecs.begin(), contRecs.end());
+    plan.rex.assign(plan.berms.size() * static_cast<std::size_t>(std::max(1, plan.max_exercises)), -1);
+    for (int i = 0; i < plan.nex_recs; ++i) {
+      const RegRec& rr = plan.rrecs[static_cast<std::size_t>(i)];
+      plan.rex[static_cast<std::size_t>(rr.b) * static_cast<std::size_t>(plan.max_exercises) + static_cast<std::size_t>(rr.k)] = i;
+    }
+    plan.rcont.assign(plan.bdates.size(), -1);
+    for (std::size_t i = static_cast<std::size_t>(plan.nex_recs); i < plan.rrecs.size(); ++i)
+      plan.rcont[static_cast<std::size_t>(plan.rrecs[i].pad[0])] = static_cast<int32_t>(i);
+    if (plan.saterms.empty()) plan.saterms.push_back(TermRange{});
   }
   const int64_t blocksPaths = cfg.antithetic ? (static_cast<int64_t>(P) + 1) / 2 : P;
   plan.stream_len = blocksPaths * maxStride;

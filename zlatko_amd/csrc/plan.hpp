@@ -62,6 +62,13 @@ struct HostPlan {
   // lists those (cmap[cd] == cd).
   std::vector<int32_t> cmap, cuniq;
   int32_t states_set = -1;          // book plans: the one set whose blocks run the states pass
+  // Regression engine with a basis choice (basis >= 0; -1: the shared-design state-basis path).
+  int32_t basis = -1;
+  std::vector<RegRec> rrecs;        // exercise records first (per Bermudan, exercise 0..ne-1), then continuation
+  std::vector<int32_t> rex;         // [Bermudan][max_exercises]: record of exercise k (-1: none)
+  std::vector<int32_t> rcont;       // per BermDate record: its continuation record (-1: none)
+  std::vector<TermRange> saterms;   // annuity / float-leg terms of the swap-rate / annuity records
+  int32_t nex_recs = 0;             // exercise records (the continuation records follow)
   int32_t group = 0;                // book plans: netting sets per workgroup of the book kernel (0: one)
   // Per-trade plans: one valuation record (terms of that trade alone) per (set, date, trade),
   // [tdate_off + j G + i], and its reals; the set's own records then carry no terms.
@@ -81,7 +88,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
                    int32_t precision, int32_t block, bool per_trade = false, bool book = false,
                    bool value_all = false, const std::vector<double>* book_rep = nullptr,
-                   const std::vector<double>* book_sim = nullptr);
+                   const std::vector<double>* book_sim = nullptr, int32_t basis = -1);
 
 // The portfolio book's grids (src/app/analyses.cpp:1690-1709): the union of the sets' reporting
 // grids, and the union of their simulation grids built on that reporting union.

@@ -49,6 +49,7 @@ def lib():
         L.ccr_hip_set_book.argtypes = [P, I32]
         L.ccr_hip_set_book_grids.argtypes = [P, P, I32, P, I32]
         L.ccr_hip_set_book_group.argtypes = [P, I32]
+        L.ccr_hip_set_regression.argtypes = [P, I32, I32]
         L.ccr_hip_book_grids.argtypes = [P, I32, P, I32, P, I32, P, P, I32, P]
         L.ccr_hip_set_graph.argtypes = [P, I32]
         L.ccr_book_totals.argtypes = [P, P, I32, I32, P]
@@ -289,6 +290,12 @@ class Engine:
 
     def set_book_group(self, sets_per_workgroup: int):
         self._check(lib().ccr_hip_set_book_group(self.ctx, sets_per_workgroup))
+
+    def set_regression(self, basis: int, gram_precision: int = 0):
+        """Bermudan regressions of plans compiled from now on: basis -1 (the state basis on shared
+        per-date designs), BASIS_STATE or BASIS_SWAP_RATE_ANNUITY on the regression engine, its
+        Grams in FP64 (gram_precision FP64) or on FP32 matrix cores (FP32)."""
+        self._check(lib().ccr_hip_set_regression(self.ctx, basis, gram_precision))
 
     def set_book_grids(self, reporting, simulation):
         """Book plans compiled afterwards use these grids (None: their own union grids)."""

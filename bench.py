@@ -269,6 +269,12 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 2^27 doubles on the GPU (1 GiB read, 1 GiB written): the reference launch "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
+    ap.add_argument("--basis", choices=["default", "shared", "state", "sa"], default="default",
+                    help="Bermudan regressions: shared (the state basis on shared per-date designs), state (the "
+                         "same basis per record on the regression engine), sa (swap rate / annuity, total degree "
+                         "3); default: sa for c3 (the configuration's basis), shared otherwise")
+    ap.add_argument("--gram", choices=["fp64", "fp32"], default="fp32",
+                    help="regression-engine Grams in FP64 or on FP32 matrix cores")
     ap.add_argument("--wave-sets", type=int, default=0,
                     help="netting sets per plan (0: all in one plan unless sets x paths exceeds 1e7)")
     args = ap.parse_args()
@@ -299,6 +305,10 @@ def main():
     engine = native.Engine()
     if args.book_group:
         engine.set_book_group(args.book_group)
+    basis = args.basis if args.basis != "default" else ("sa" if args.workload == "c3" else "shared")
+    gram = abi.CCR_FP32 if args.gram == "fp32" else abi.CCR_FP64
+    engine.set_regression({"shared": -1, "state": abi.CCR_BASIS_STATE, "sa": abi.CCR_BASIS_SWAP_RATE_ANNUITY}[basis],
+                          gram)
     specs, plans = compile_waves(engine, spec, prec, wave_sets, book)
     infos = [p.info() for p in plans]
     revals = sum(i.revaluations for i in infos)
@@ -341,6 +351,13 @@ def main():
     }
     if book:
         result["config"]["book"] = True
+    if any(i.num_bermudans for i in infos):
+        result["config"]["regression"] = {
+            "basis": {"shared": "state (1, z, z^2, z^3) on shared per-date designs",
+                      "state": "state (1, z, z^2, z^3) per record",
+                      "sa": "swap rate / annuity, total degree 3 (k = 10) per record"}[basis]}
+        if basis != "shared":
+            result["config"]["regression"]["gram"] = "fp32 matrix cores" if gram == abi.CCR_FP32 else "fp64"
     traffic, source = measured_traffic(variant, args.precision)
     if traffic is not None and not args.paths:
         result["roofline"]["traffic"] = traffic

@@ -262,12 +262,17 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
   for (int d = 0; d < ncd; ++d) {
     const SetDev& S = h.sets[static_cast<size_t>(h.cdates[2 * static_cast<size_t>(d)])];
     const int j = h.cdates[2 * static_cast<size_t>(d) + 1];
-    for (int p = 0; p < P; ++p)
+    for (int p = 0; p < P; ++p) {
+      const double* bb = a.bonds + 2 * static_cast<int64_t>(S.bond_off);
+      const double* ff = a.fixed + 4 * static_cast<int64_t>(S.fixed_off);
+      const int lb = h.clive_rng[2 * static_cast<size_t>(d)], le = h.clive_rng[2 * static_cast<size_t>(d) + 1];
       bermv[static_cast<size_t>(d) * P + p] =
-          bermudanSum<double>(S, j, p, P, h.bdates.data(), cont.data(), exidx.data(), states.data(),
-                              a.bonds + 2 * static_cast<int64_t>(S.bond_off), a.fixed + 4 * static_cast<int64_t>(S.fixed_off),
-                              a.poly + S.poly_off, h.clive.data(), h.clive_rng[2 * static_cast<size_t>(d)],
-                              h.clive_rng[2 * static_cast<size_t>(d) + 1], view);
+          h.basis == kBasisSwapRateAnnuity
+              ? bermudanSum<double, true>(S, j, p, P, h.bdates.data(), cont.data(), exidx.data(), states.data(), bb,
+                                          ff, a.poly + S.poly_off, h.clive.data(), lb, le, view)
+              : bermudanSum<double, false>(S, j, p, P, h.bdates.data(), cont.data(), exidx.data(), states.data(), bb,
+                                           ff, a.poly + S.poly_off, h.clive.data(), lb, le, view);
+    }
   }
   a.bermv = bermv.data();
 }

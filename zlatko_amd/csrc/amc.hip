@@ -444,7 +444,7 @@ __global__ void __launch_bounds__(256) amc_csolve_kernel(AmcArgs a) {
 // Per (valued date of a set with Bermudans, path): the sum over the set's swaptions of
 // position x value. Workgroups of one path chunk land on one XCD (the grid's x extent is a
 // multiple of 8), so the chunk's exercise indices are read from that XCD's L2.
-template <typename Real>
+template <typename Real, bool kSa>
 __global__ void __launch_bounds__(kAmcChunk) amc_value_kernel(AmcArgs a, const Real* bonds, const Real* fixed) {
   const int chunk = static_cast<int>(blockIdx.x);
   if (chunk >= a.nchunks) return;
@@ -455,13 +455,13 @@ __global__ void __launch_bounds__(kAmcChunk) amc_value_kernel(AmcArgs a, const R
   const int64_t p = p0 < a.paths ? p0 : a.paths - 1;
   BasisView eng;
   eng.basis = a.basis;
-  if (a.basis == kBasisSwapRateAnnuity) {
+  if constexpr (kSa) {
     eng.rrecs = toConst(a.rrecs);
     eng.rcont = toConst(a.rcont);
     eng.rfit = toConst(static_cast<const double*>(a.rfit));
     eng.saterms = toConst(a.saterms);
   }
-  const Real v = bermudanSum<Real>(S, j, p, a.paths, toConst(a.bdates), toConst(a.cont), a.exidx,
+  const Real v = bermudanSum<Real, kSa>(S, j, p, a.paths, toConst(a.bdates), toConst(a.cont), a.exidx,
                                    a.states, toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off),
                                    toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(polyOf<Real>(a)) + S.poly_off,
                                    toConst(a.clive), toConst(a.clive_rng)[2 * d], toConst(a.clive_rng)[2 * d + 1], eng);
@@ -844,9 +844,12 @@ hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int
     if (ncont > 0) batch(-1, ncont);
     hipLaunchKernelGGL(amc_rcont_kernel, dim3((a.nbdates + 255) / 256), dim3(256), 0, stream, a);
     if (hipError_t e = last()) return e;
-    if (a.ncdates > 0)
-      hipLaunchKernelGGL((amc_value_kernel<Real>), dim3((a.nchunks + 7) / 8 * 8, a.ncdates), blk, 0, stream, a,
-                         bonds, fixed);
+    if (a.ncdates > 0 && a.basis == kBasisSwapRateAnnuity)
+      hipLaunchKernelGGL((amc_value_kernel<Real, true>), dim3((a.nchunks + 7) / 8 * 8, a.ncdates), blk, 0, stream,
+                         a, bonds, fixed);
+    else if (a.ncdates > 0)
+      hipLaunchKernelGGL((amc_value_kernel<Real, false>), dim3((a.nchunks + 7) / 8 * 8, a.ncdates), blk, 0, stream,
+                         a, bonds, fixed);
     return last();
   }
   for (int k = max_exercises - 1; k >= 0; --k) {
@@ -871,8 +874,8 @@ hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int
   }
   hipLaunchKernelGGL(amc_csolve_kernel, dim3((a.nbdates + 255) / 256), dim3(256), 0, stream, a);
   if (a.ncdates > 0)
-    hipLaunchKernelGGL((amc_value_kernel<Real>), dim3((a.nchunks + 7) / 8 * 8, a.ncdates), blk, 0, stream, a, bonds,
-                       fixed);
+    hipLaunchKernelGGL((amc_value_kernel<Real, false>), dim3((a.nchunks + 7) / 8 * 8, a.ncdates), blk, 0, stream, a,
+                       bonds, fixed);
   return last();
 }
 

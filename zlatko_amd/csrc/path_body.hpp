@@ -172,7 +172,7 @@ __host__ __device__ inline Real evalRange(const CCR_CONST TermRange& tr, Real x,
 // from S.state_off, D after state_len); bonds / fixed / poly are the set's tables. Only the
 // swaptions live[lbeg, lend) are visited: the plan lists, per continuation date, those that can
 // be nonzero on some path (the others add an exact 0).
-template <typename Real>
+template <typename Real, bool kSa = false>
 __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, int64_t p, int64_t paths,
                                             cptr<BermDate> bdates, cptr<double> cont,
                                             const int16_t* exidx, const double* states, cptr<Real> bonds,
@@ -192,19 +192,21 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
     const CCR_CONST BermDate& bd = bdates[lr.rec];
     const bool exercised = exidx[static_cast<int64_t>(lr.b) * paths + p] < bd.next_k;
     Real v = 0;
-    if (bd.next_k < lr.ne && eng.basis == kBasisSwapRateAnnuity && eng.rcont[lr.rec] >= 0) {
+    if constexpr (kSa) {
       // The continuation fit of this (swaption, date) on the swap rate and annuity of the tail
       // entered at the next exercise.
-      const int rid = eng.rcont[lr.rec];
-      const CCR_CONST RegRec& rr = eng.rrecs[rid];
-      auto noFix = [](int) { return Real(0); };
-      const double A = static_cast<double>(evalRange<Real>(eng.saterms[rr.ann], x, bonds, fixed, poly, noFix));
-      const double F = static_cast<double>(evalRange<Real>(eng.saterms[rr.flt], x, bonds, fixed, poly, noFix));
-      double rp[kBasisStride];
-      for (int i = 0; i < kBasisStride; ++i) rp[i] = eng.rfit[static_cast<int64_t>(rid) * kBasisStride + i];
-      const double pred = basisPredict(kBasisSwapRateAnnuity, rp, F / A, A);
-      v = static_cast<Real>(pred > 0.0 ? pred : 0.0);
-    } else if (bd.next_k < lr.ne && eng.basis != kBasisSwapRateAnnuity) {
+      const int rid = bd.next_k < lr.ne ? eng.rcont[lr.rec] : -1;
+      if (rid >= 0) {
+        const CCR_CONST RegRec& rr = eng.rrecs[rid];
+        auto noFix = [](int) { return Real(0); };
+        const double A = static_cast<double>(evalRange<Real>(eng.saterms[rr.ann], x, bonds, fixed, poly, noFix));
+        const double F = static_cast<double>(evalRange<Real>(eng.saterms[rr.flt], x, bonds, fixed, poly, noFix));
+        double rp[kBasisStride];
+        for (int i = 0; i < kBasisStride; ++i) rp[i] = eng.rfit[static_cast<int64_t>(rid) * kBasisStride + i];
+        const double pred = basisPredict(kBasisSwapRateAnnuity, rp, F / A, A);
+        v = static_cast<Real>(pred > 0.0 ? pred : 0.0);
+      }
+    } else if (bd.next_k < lr.ne) {
       cptr<double> c = cont + static_cast<int64_t>(lr.rec) * kContStride;
       const double coef[4] = {c[2], c[3], c[4], c[5]};
       if (!haveZ || c[0] != zCenter || c[1] != zScale) {

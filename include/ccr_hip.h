@@ -474,6 +474,12 @@ int32_t ccr_hip_set_trade_statistics(ccr_hip_ctx* ctx, int32_t enable);
 int32_t ccr_hip_trade_statistics(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set, int32_t trade,
                                  ccr_trade_statistics* out);
 
+/* ccr::NormalGenerator(seed).fill(out, n) (include/ccr/core/random.hpp:13-26): std::mt19937_64 +
+ * std::normal_distribution<double>, the stream every plan replays. Generated on up to 16 host
+ * threads (the engine run sequentially, the polar attempts in parallel chunks), bit-identical to
+ * the sequential stream. Needs no device. */
+int32_t ccr_hip_normals(uint64_t seed, int64_t n, double* out);
+
 /* ExposureEngine::reportingGrid / simulationGrid of one netting set (exposure_engine.cpp:35-48).
  * Writes up to `capacity` times and the full size into *size. Needs no device. */
 int32_t ccr_hip_reporting_grid(const double* base_grid, int32_t base_size, const ccr_netting_set* set,

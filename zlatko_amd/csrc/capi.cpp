@@ -16,6 +16,7 @@
 #include "../../include/ccr_hip.h"
 #include "launch.hpp"
 #include "mlp_host.hpp"
+#include "draws.hpp"
 #include "plan.hpp"
 
 
@@ -245,8 +246,9 @@ void* uploadReals(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const std::vector<double
   return d;
 }
 
-// Generates the first n normals of the stream for `seed` in chunks into two pinned staging
-// buffers, overlapping the generation of one chunk with the upload of the other.
+// Generates the first n normals of the stream for `seed` (on the host's cores, bit-identical to
+// the sequential stream) in chunks into two pinned staging buffers, overlapping the generation of
+// one chunk with the upload of the other.
 std::shared_ptr<StreamBuf> generateStream64(ccr_hip_ctx* ctx, uint64_t seed, int64_t n) {
   auto buf = std::make_shared<StreamBuf>();
   buf->seed = seed;
@@ -264,8 +266,7 @@ std::shared_ptr<StreamBuf> generateStream64(ccr_hip_ctx* ctx, uint64_t seed, int
   check(hipHostMalloc(&stage[1], static_cast<size_t>(chunk) * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
   check(hipEventCreate(&done[0]), "hipEventCreate");
   check(hipEventCreate(&done[1]), "hipEventCreate");
-  std::mt19937_64 engine(seed);
-  std::normal_distribution<double> normal(0.0, 1.0);
+  NormalStream normals(seed);
   double genMs = 0.0;
   const auto t0 = std::chrono::steady_clock::now();
   int which = 0;
@@ -274,8 +275,7 @@ std::shared_ptr<StreamBuf> generateStream64(ccr_hip_ctx* ctx, uint64_t seed, int
     const int64_t m = std::min(chunk, n - off);
     if (pending[which]) check(hipEventSynchronize(done[which]), "hipEventSynchronize");
     const auto g0 = std::chrono::steady_clock::now();
-    double* d = static_cast<double*>(stage[which]);
-    for (int64_t i = 0; i < m; ++i) d[i] = normal(engine);
+    normals.fill(static_cast<double*>(stage[which]), m);
     genMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - g0).count();
     check(hipMemcpyAsync(buf->d + off, stage[which], static_cast<size_t>(m) * sizeof(double), hipMemcpyHostToDevice,
                          ctx->stream),
@@ -1179,6 +1179,13 @@ int32_t ccr_hip_set_fault(ccr_hip_ctx* ctx, int32_t fault) {
   if (fault < CCR_FAULT_NONE || fault > CCR_FAULT_PLAN) return CCR_ERR_INVALID_ARGUMENT;
   ctx->fault = fault;
   return CCR_OK;
+}
+
+int32_t ccr_hip_normals(uint64_t seed, int64_t n, double* out) {
+  return guarded(nullptr, [&] {
+    if (n < 0 || (n > 0 && !out)) invalid("ccr_hip_normals: bad arguments");
+    parallelNormals(seed, n, out);
+  });
 }
 
 int32_t ccr_hip_reporting_grid(const double* base_grid, int32_t base_size, const ccr_netting_set* set, double* times,

@@ -66,6 +66,7 @@ def lib():
         L.ccr_xva.argtypes = [P, P, P, I32, P, P, D, P, D, P, P]
         L.ccr_xva.restype = I32
         L.ccr_hip_reporting_grid.argtypes = [P, I32, P, P, I32, P]
+        L.ccr_hip_normals.argtypes = [C.c_uint64, C.c_int64, P]
         L.ccr_hip_simulation_grid.argtypes = [P, I32, P, P, I32, P]
         L.ccr_hip_compile.argtypes = [P, P, P, P, I32, P, I32, D, I32, P]
         L.ccr_hip_compile.restype = I32
@@ -127,6 +128,15 @@ def grid(spec: abi.Spec, i: int = 0, simulation: bool = False) -> np.ndarray:
         raise CcrError(rc, "grid")
     out = np.zeros(n.value)
     fn(abi.dptr(spec.base_grid), len(spec.base_grid), C.byref(spec.sets_c[i]), abi.dptr(out), n.value, C.byref(n))
+    return out
+
+
+def normals(seed: int, n: int) -> np.ndarray:
+    """The replayed normal stream (ccr::NormalGenerator(seed), n draws), on the host's cores."""
+    out = np.empty(n, np.float64)
+    rc = lib().ccr_hip_normals(seed, n, out.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"ccr_hip_normals failed ({rc})")
     return out
 
 

@@ -152,10 +152,15 @@ def cva_wall_time(engine, steps: int) -> dict:
     MLP surrogate valuing the Bermudans, with the surrogate run's CVA error against the plain one.
     The surrogate is trained and gated on two independent 20,000-path plans (training time
     reported apart)."""
+    # The portfolio runs before leave the context in book mode with the book's grids and a shared
+    # workspace; this workload is one stand-alone netting set.
+    engine.set_book(False)
+    engine.set_book_grids(None, None)
+    engine.set_shared_workspace(False)
     tr = engine.compile(W.c4(num_paths=20000, seed=101))
     ho = engine.compile(W.c4(num_paths=20000, seed=202))
     t0 = time.perf_counter()
-    sur = engine.surrogate_train(tr, ho, train_paths=2000, epochs=80, lr=5e-4, batch=4096, gate=0.02)
+    sur = engine.surrogate_train(tr, ho, train_paths=2000, epochs=80, lr=5e-4, batch=4096, gate=0.005)
     train_s = time.perf_counter() - t0
     tr.close()
     ho.close()

@@ -1354,6 +1354,8 @@ int32_t ccr_oracle_run_book(const ccr_scenario_generator* generator, const ccr_s
                             int32_t err_capacity) {
   return guard(err, err_capacity, [&] {
     if (!generator || !config || !sets || !outs || num_sets <= 0) invalid("null argument");
+    gTiming = ccr_oracle_timing{};
+    const auto t0 = std::chrono::steady_clock::now();
     const Generator g(*generator);
     const Grid base(Vec(base_grid, base_grid + std::max(0, base_size)));
     std::vector<std::vector<Trade>> trades(static_cast<std::size_t>(num_sets));
@@ -1372,6 +1374,7 @@ int32_t ccr_oracle_run_book(const ccr_scenario_generator* generator, const ccr_s
     for (int i = 0; i < num_sets; ++i)
       runSet(g, *config, base, sets[i], pfe_quantile, threads, &outs[i], nullptr, 0.99, 10.0 / 250.0, nullptr, &rep,
              &sim);
+    gTiming.total_s = secondsSince(t0);
   });
 }
 

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define CCR_HIP_ABI_VERSION 1
+#define CCR_HIP_ABI_VERSION 2
 
 /* ------------------------------------------------------------------ status codes */
 /* The C++ mirror maps them to the reference's exceptions:
@@ -312,12 +312,16 @@ int32_t ccr_hip_mlp_predict(ccr_hip_ctx* ctx, const ccr_mlp_desc* desc, int32_t 
                             const double* stdz, const double* X, int64_t n, double* Y);
 
 /* ------------------------------------------------------------------ exposure surrogate
- * For trades without a closed form (Bermudan swaptions): per netting set, an MLP (BF16 matrix
- * cores, SiLU) maps the per-date state (t, x, running max and min of x along the path) to the
- * set's Bermudan value. It is trained on the pathwise values the American Monte Carlo method
- * produced on a training plan, then checked on an independent hold-out plan: a set keeps the
- * surrogate only if its hold-out EE stays within gate_threshold (relative to the peak plain EE);
- * otherwise that set falls back to the regression method. No counterpart in the reference. */
+ * For the trade type without a closed form (Bermudan swaptions): an MLP (BF16 matrix cores, SiLU)
+ * maps the per-date state -- (t, x, running max and min of x along the path, x at every exercise
+ * date already passed) -- to the Bermudan value of netting sets, one output per set. The netting
+ * sets that share one simulated state (every set with Bermudans of a portfolio-book plan; a set
+ * alone otherwise) share one network (at most 128 outputs each), so a book's networks are
+ * evaluated once per (date, path) for all its sets. Trained on the pathwise values the American
+ * Monte Carlo method produced on a training plan, then checked on an independent hold-out plan:
+ * a set keeps the surrogate only if its hold-out EE stays within gate_threshold (relative to the
+ * peak plain EE); otherwise that set falls back to the regression method (which then runs for
+ * the plan). No counterpart in the reference. */
 typedef struct ccr_hip_surrogate ccr_hip_surrogate;
 
 typedef struct ccr_surrogate_options {
@@ -339,6 +343,7 @@ typedef struct ccr_surrogate_report {
   double train_ms;         /* GPU training time (all sets) */
   double loss;             /* largest final training loss (standardised MSE) */
   double max_ee_error;     /* worst hold-out EE error (relative to the peak plain EE) */
+  int32_t networks;        /* networks trained (one per group of sets sharing a simulated state) */
 } ccr_surrogate_report;
 
 /* Trains on train_plan (executed plain) and gates on holdout_plan (executed plain and with the

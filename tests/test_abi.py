@@ -20,7 +20,7 @@ def test_library_exports_every_declared_symbol():
     assert len(declared) >= 17
     missing = [s for s in declared if s not in exported]
     assert not missing, missing
-    assert lib.ccr_hip_abi_version() == 1
+    assert lib.ccr_hip_abi_version() == 2
 
 
 def test_grids_match_oracle_bit_for_bit():

@@ -165,7 +165,8 @@ class SurrogateOptionsC(C.Structure):
 
 class SurrogateReportC(C.Structure):
     _fields_ = [("candidate_sets", C.c_int32), ("enabled_sets", C.c_int32), ("samples", C.c_int64),
-                ("train_ms", C.c_double), ("loss", C.c_double), ("max_ee_error", C.c_double)]
+                ("train_ms", C.c_double), ("loss", C.c_double), ("max_ee_error", C.c_double),
+                ("networks", C.c_int32)]
 
 
 def mlp_param_count(d: MlpDescC) -> int:

@@ -749,12 +749,13 @@ __global__ void __launch_bounds__(256) amc_rcont_kernel(AmcArgs a) {
   out[6] = rp[5];
 }
 
-// Surrogate inputs per path: the date loop of one set; running extremes of the domestic state and
-// its value at each exercise date already passed (0 before it). The exercise-date states are
-// kept in registers (compile-time indexed).
+// Surrogate inputs per path: the date loop over the recorded states of a group of netting sets;
+// running extremes of the domestic state and its value at each exercise date already passed (0
+// before it), written on the dates of the group's rows (urow[j] >= 0). The exercise-date states
+// are kept in registers (compile-time indexed).
 __global__ void __launch_bounds__(256) amc_features_kernel(const double* states, int64_t state_off, int64_t paths,
-                                                           const DateRec* dates, const double* ctimes, int num_dates,
-                                                           int cd_base, SurrogateExercises ex, float* feats) {
+                                                           const int32_t* urow, const double* utimes, int num_dates,
+                                                           SurrogateExercises ex, float* feats) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= paths) return;
   const int inputs = kSurrogateBase + ex.count;
@@ -762,7 +763,6 @@ __global__ void __launch_bounds__(256) amc_features_kernel(const double* states,
   float xe[kSurrogateMaxExercises];
 #pragma unroll
   for (int m = 0; m < kSurrogateMaxExercises; ++m) xe[m] = 0.0f;
-  int64_t row = 0;
   for (int j = 0; j < num_dates; ++j) {
     const double x = states[state_off + static_cast<int64_t>(j) * paths + p];
     hi = x > hi ? x : hi;
@@ -770,45 +770,40 @@ __global__ void __launch_bounds__(256) amc_features_kernel(const double* states,
 #pragma unroll
     for (int m = 0; m < kSurrogateMaxExercises; ++m)
       if (m < ex.count && ex.date[m] == j) xe[m] = static_cast<float>(x);
-    if (!(dates[j].flags & kValued)) continue;
-    float* f = feats + (row * paths + p) * inputs;
-    f[0] = static_cast<float>(ctimes[cd_base + row]);
+    const int32_t u = urow[j];
+    if (u < 0) continue;
+    float* f = feats + (static_cast<int64_t>(u) * paths + p) * inputs;
+    f[0] = static_cast<float>(utimes[u]);
     f[1] = static_cast<float>(x);
     f[2] = static_cast<float>(hi);
     f[3] = static_cast<float>(lo);
 #pragma unroll
     for (int m = 0; m < kSurrogateMaxExercises; ++m)
       if (m < ex.count) f[kSurrogateBase + m] = xe[m];
-    ++row;
   }
 }
 
-// Network outputs of the first n_live values, exact zeros for the rest (dates on which no
-// swaption of the set can be nonzero).
-__global__ void amc_scatter_kernel(const float* y, int64_t n_live, int64_t n, double mean, double sd, double* out) {
+__global__ void zero_kernel(double* out, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n_live) out[i] = static_cast<double>(y[i]) * sd + mean;
-  else if (i < n) out[i] = 0.0;
+  if (i < n) out[i] = 0.0;
 }
 
 hipError_t last() { return hipGetLastError(); }
 
 }  // namespace
 
-hipError_t launchSurrogateFeatures(const double* states, int64_t state_off, int64_t paths, const DateRec* dates,
-                                   const double* ctimes, int num_dates, int cd_base, const SurrogateExercises& ex,
-                                   float* feats, hipStream_t stream) {
+hipError_t launchSurrogateFeatures(const double* states, int64_t state_off, int64_t paths, const int32_t* urow,
+                                   const double* utimes, int num_dates, const SurrogateExercises& ex, float* feats,
+                                   hipStream_t stream) {
   if (state_off < 0 || ex.count < 0 || ex.count > kSurrogateMaxExercises) return hipErrorInvalidValue;
   hipLaunchKernelGGL(amc_features_kernel, dim3(static_cast<unsigned>((paths + 255) / 256)), dim3(256), 0, stream, states,
-                     state_off, paths, dates, ctimes, num_dates, cd_base, ex, feats);
+                     state_off, paths, urow, utimes, num_dates, ex, feats);
   return last();
 }
 
-hipError_t launchSurrogateScatter(const float* y, int64_t n_live, int64_t n, double mean, double sd, double* out,
-                                  hipStream_t stream) {
+hipError_t launchZeroRows(double* out, int64_t n, hipStream_t stream) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(amc_scatter_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, y, n_live,
-                     n, mean, sd, out);
+  hipLaunchKernelGGL(zero_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, out, n);
   return last();
 }
 

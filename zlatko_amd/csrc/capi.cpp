@@ -74,6 +74,9 @@ struct ccr_hip_ctx {
   // engine, its Grams in FP64 or on FP32 matrix cores.
   int basis = -1;
   int gram32 = 0;
+  // Surrogate inputs of the last surrogate execute (grown as needed).
+  float* sur_feats = nullptr;
+  size_t sur_feats_bytes = 0;
   // Executes replay one captured HIP graph per plan while set.
   bool use_graph = false;
   // Launches are being captured into a graph (no timing events recorded), and whether the last
@@ -120,8 +123,6 @@ struct ccr_hip_plan {
   double* d_states = nullptr;
   double* d_bermv = nullptr;
   double* d_ctimes = nullptr;
-  float* d_feats = nullptr;   // surrogate inputs (allocated on first surrogate execute)
-  float* d_ysur = nullptr;
   AmcArgs amc{};
   double draws_ms = 0.0, upload_ms = 0.0;
   // Per-execute buffers placed in the context's shared workspace: (field, byte offset).
@@ -584,14 +585,16 @@ int liveRows(const HostPlan& h, int s) {
   return live;
 }
 
-// The distinct exercise dates of set s's Bermudans (grid indices, ascending, at most
-// kSurrogateMaxExercises).
-SurrogateExercises exercisesOf(const HostPlan& h, int s) {
-  const SetDev& S = h.sets[static_cast<size_t>(s)];
+// The distinct exercise dates (grid indices, ascending, at most kSurrogateMaxExercises) of the
+// Bermudans of the given sets.
+SurrogateExercises exercisesOf(const HostPlan& h, const std::vector<int>& sets) {
   std::vector<int32_t> d;
-  for (int b = S.berm_beg; b < S.berm_end; ++b) {
-    const BermDev& bm = h.berms[static_cast<size_t>(b)];
-    for (int k = 0; k < bm.ne; ++k) d.push_back(h.eidx[static_cast<size_t>(bm.eidx_off + k)]);
+  for (int s : sets) {
+    const SetDev& S = h.sets[static_cast<size_t>(s)];
+    for (int b = S.berm_beg; b < S.berm_end; ++b) {
+      const BermDev& bm = h.berms[static_cast<size_t>(b)];
+      for (int k = 0; k < bm.ne; ++k) d.push_back(h.eidx[static_cast<size_t>(bm.eidx_off + k)]);
+    }
   }
   std::sort(d.begin(), d.end());
   d.erase(std::unique(d.begin(), d.end()), d.end());
@@ -601,56 +604,183 @@ SurrogateExercises exercisesOf(const HostPlan& h, int s) {
   return ex;
 }
 
+// The valued dates (grid indices, in row order) of set s's continuation rows.
+std::vector<int32_t> valuedDates(const HostPlan& h, int s) {
+  std::vector<int32_t> d;
+  for (size_t i = 0; i < h.cdates.size() / 2; ++i)
+    if (h.cdates[2 * i] == s) d.push_back(h.cdates[2 * i + 1]);
+  return d;
+}
+
+// Outputs per surrogate network: the netting sets of one group (sets with Bermudans that share
+// recorded states) are split into networks of at most this many outputs.
+constexpr int kSurrogateMaxOutputs = 128;
+
 }  // namespace
 
 struct ccr_hip_surrogate {
-  struct Net {
+  // One network output: the Bermudan value of netting set `set` on its valued dates.
+  struct Output {
     int set = -1;
     int nberms = 0;
     bool enabled = true;
     double ee_error = 0.0;
+    double ymean = 0.0, ysd = 1.0;
+    int32_t cd_base = 0, rows = 0, live = 0;  // its continuation rows in the plan; rows >= live are 0
+    std::vector<int32_t> dates;               // its valued dates (grid indices), checked against a plan
+  };
+  // One network over the rows of a group of sets: the union of their valued dates up to the last
+  // one with a live swaption (urows rows), every output a set.
+  struct Net {
+    std::vector<Output> outs;
     std::unique_ptr<MlpRun> run;  // trained weights (inference only)
     int inputs = kSurrogateBase;
     double xmean[kSurrogateBase + kSurrogateMaxExercises], xsd[kSurrogateBase + kSurrogateMaxExercises];
-    double ymean = 0.0, ysd = 1.0;
+    SurrogateExercises ex{};
+    int num_dates = 0, urows = 0;
+    std::vector<int32_t> urow, map;  // urow [num_dates]: row of each date (-1: none); map [output][urows]
+    std::vector<double> utimes;      // [urows]
+    int32_t* d_urow = nullptr;
+    double* d_utimes = nullptr;
+    int32_t* d_map = nullptr;        // the enabled outputs' entries (the others -1)
+    InferOut* d_outs = nullptr;
   };
   int device = 0;
   int num_sets = 0;
   std::vector<Net> nets;
+  std::vector<void*> bufs;
+  ~ccr_hip_surrogate() {
+    for (void* p : bufs) (void)hipFree(p);
+  }
+  template <typename T>
+  T* upload(const std::vector<T>& v, hipStream_t stream) {
+    void* p = nullptr;
+    check(hipMalloc(&p, std::max<size_t>(v.size() * sizeof(T), 256)), "hipMalloc");
+    bufs.push_back(p);
+    if (!v.empty()) check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return static_cast<T*>(p);
+  }
 };
 
 namespace {
 
-// Surrogate inputs, inference and Bermudan rows of every enabled set of `plan`.
+// The networks of a plan (no weights yet): groups of sets with Bermudans sharing recorded states,
+// split into at most kSurrogateMaxOutputs outputs each, with their row tables.
+std::vector<ccr_hip_surrogate::Net> surrogateNets(const HostPlan& h) {
+  std::vector<std::vector<int>> groups;
+  std::vector<int64_t> offs;
+  for (int s = 0; s < static_cast<int>(h.sets.size()); ++s) {
+    const SetDev& S = h.sets[static_cast<size_t>(s)];
+    if (S.berm_end == S.berm_beg) continue;
+    const auto it = std::find(offs.begin(), offs.end(), S.state_off);
+    if (it == offs.end()) {
+      offs.push_back(S.state_off);
+      groups.push_back({s});
+    } else {
+      groups[static_cast<size_t>(it - offs.begin())].push_back(s);
+    }
+  }
+  std::vector<ccr_hip_surrogate::Net> nets;
+  for (const auto& g : groups) {
+    const size_t parts = (g.size() + kSurrogateMaxOutputs - 1) / kSurrogateMaxOutputs;
+    for (size_t q = 0; q < parts; ++q) {
+      const size_t b0 = g.size() * q / parts, b1 = g.size() * (q + 1) / parts;
+      ccr_hip_surrogate::Net net;
+      std::vector<int> sets(g.begin() + static_cast<std::ptrdiff_t>(b0), g.begin() + static_cast<std::ptrdiff_t>(b1));
+      net.num_dates = h.sets[static_cast<size_t>(sets[0])].num_dates;
+      std::vector<int32_t> uni;
+      std::vector<double> tOf(static_cast<size_t>(net.num_dates), 0.0);
+      for (int s : sets) {
+        const SetDev& S = h.sets[static_cast<size_t>(s)];
+        ccr_hip_surrogate::Output o;
+        o.set = s;
+        o.nberms = S.berm_end - S.berm_beg;
+        o.cd_base = S.cd_base;
+        o.dates = valuedDates(h, s);
+        o.rows = static_cast<int32_t>(o.dates.size());
+        o.live = liveRows(h, s);
+        for (int r = 0; r < o.live; ++r) {
+          uni.push_back(o.dates[static_cast<size_t>(r)]);
+          tOf[static_cast<size_t>(o.dates[static_cast<size_t>(r)])] = h.ctimes[static_cast<size_t>(o.cd_base + r)];
+        }
+        net.outs.push_back(std::move(o));
+      }
+      std::sort(uni.begin(), uni.end());
+      uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+      net.urows = static_cast<int>(uni.size());
+      net.urow.assign(static_cast<size_t>(net.num_dates), -1);
+      for (int u = 0; u < net.urows; ++u) {
+        net.urow[static_cast<size_t>(uni[static_cast<size_t>(u)])] = u;
+        net.utimes.push_back(tOf[static_cast<size_t>(uni[static_cast<size_t>(u)])]);
+      }
+      net.map.assign(net.outs.size() * static_cast<size_t>(net.urows), -1);
+      for (size_t o = 0; o < net.outs.size(); ++o) {
+        const auto& out = net.outs[o];
+        for (int r = 0; r < out.live; ++r)
+          net.map[o * static_cast<size_t>(net.urows) + static_cast<size_t>(net.urow[static_cast<size_t>(out.dates[static_cast<size_t>(r)])])] = r;
+      }
+      net.ex = exercisesOf(h, sets);
+      net.inputs = kSurrogateBase + net.ex.count;
+      nets.push_back(std::move(net));
+    }
+  }
+  return nets;
+}
+
+// Whether plan h has the sets, valued dates and continuation rows the surrogate was trained on.
+bool surrogateFits(const HostPlan& h, const ccr_hip_surrogate* sur) {
+  if (static_cast<int>(h.sets.size()) != sur->num_sets) return false;
+  for (const auto& net : sur->nets)
+    for (const auto& o : net.outs) {
+      const SetDev& S = h.sets[static_cast<size_t>(o.set)];
+      if (S.berm_end - S.berm_beg != o.nberms || S.cd_base != o.cd_base || S.num_dates != net.num_dates ||
+          valuedDates(h, o.set) != o.dates || liveRows(h, o.set) != o.live)
+        return false;
+    }
+  return true;
+}
+
+// Surrogate inputs (the context's feature buffer) of every network with an enabled output,
+// inference straight into the Bermudan value rows of its enabled sets, exact zeros past each
+// enabled set's last live row.
 void surrogateRows(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* sur) {
   const HostPlan& h = plan->host;
   const int64_t P = h.g.num_paths;
-  if (!plan->d_feats) {
-    int maxRows = 0;
-    for (int s = 0; s < static_cast<int>(h.sets.size()); ++s) maxRows = std::max(maxRows, setRows(h, s));
-    plan->d_feats = static_cast<float*>(
-        devAlloc(ctx, plan, static_cast<size_t>(maxRows) * P * (kSurrogateBase + kSurrogateMaxExercises) * sizeof(float)));
-    plan->d_ysur = static_cast<float*>(devAlloc(ctx, plan, static_cast<size_t>(maxRows) * P * sizeof(float)));
-  }
   for (const auto& net : sur->nets) {
-    if (!net.enabled) continue;
-    const SetDev& S = h.sets[static_cast<size_t>(net.set)];
-    const int rows = setRows(h, net.set);
-    const SurrogateExercises ex = exercisesOf(h, net.set);
-    if (kSurrogateBase + ex.count != net.inputs) invalid("surrogate: trained for other exercise dates");
-    check(launchSurrogateFeatures(plan->d_states, S.state_off, P, plan->d_dates + S.date_off, plan->d_ctimes, S.num_dates,
-                                  S.cd_base, ex, plan->d_feats, ctx->stream),
+    bool any = false;
+    for (const auto& o : net.outs) any = any || o.enabled;
+    if (!any || net.urows == 0) continue;
+    if (static_cast<size_t>(net.urows) * P * net.inputs * sizeof(float) > ctx->sur_feats_bytes)
+      invalid("surrogate: feature buffer not sized for this plan");
+    const SetDev& S = h.sets[static_cast<size_t>(net.outs[0].set)];
+    check(launchSurrogateFeatures(plan->d_states, S.state_off, P, net.d_urow, net.d_utimes, net.num_dates, net.ex,
+                                  ctx->sur_feats, ctx->stream),
           "surrogate features launch");
-    // The network runs on the rows up to the set's last date with a live swaption; the rest are 0.
-    const int live = liveRows(h, net.set);
-    if (live > 0)
-      check(mlpInfer(net.run->device(), plan->d_feats, static_cast<int64_t>(live) * P, net.xmean, net.xsd,
-                     plan->d_ysur, ctx->stream),
-            "surrogate inference launch");
-    check(launchSurrogateScatter(plan->d_ysur, static_cast<int64_t>(live) * P, static_cast<int64_t>(rows) * P,
-                                 net.ymean, net.ysd, plan->d_bermv + static_cast<int64_t>(S.cd_base) * P, ctx->stream),
-          "surrogate scatter launch");
+    check(mlpInferRows(net.run->device(), ctx->sur_feats, static_cast<int64_t>(net.urows) * P, net.xmean, net.xsd, P,
+                       net.urows, net.d_map, net.d_outs, plan->d_bermv, ctx->stream),
+          "surrogate inference launch");
+    for (const auto& o : net.outs)
+      if (o.enabled && o.rows > o.live)
+        check(launchZeroRows(plan->d_bermv + (static_cast<int64_t>(o.cd_base) + o.live) * P,
+                             static_cast<int64_t>(o.rows - o.live) * P, ctx->stream),
+              "surrogate zero rows launch");
   }
+}
+
+// Grows the context's surrogate feature buffer to what the networks need on `plan`.
+void reserveSurrogateFeatures(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, const ccr_hip_surrogate* sur) {
+  size_t need = 0;
+  for (const auto& net : sur->nets)
+    need = std::max(need, static_cast<size_t>(net.urows) * static_cast<size_t>(plan->host.g.num_paths) *
+                              static_cast<size_t>(net.inputs) * sizeof(float));
+  if (need <= ctx->sur_feats_bytes) return;
+  check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  if (ctx->sur_feats) (void)hipFree(ctx->sur_feats);
+  ctx->sur_feats = nullptr;
+  ctx->sur_feats_bytes = 0;
+  check(hipMalloc(reinterpret_cast<void**>(&ctx->sur_feats), need), "hipMalloc");
+  ctx->sur_feats_bytes = need;
 }
 
 // States pass and regressions (sets with Bermudan swaptions) or the surrogate, then the path
@@ -669,7 +799,8 @@ void launchPathStages(ccr_hip_ctx* ctx, ccr_hip_plan* plan, int blocks, const cc
     for (int si = 0; sur && si < static_cast<int>(h.sets.size()); ++si) {
       if (h.sets[static_cast<size_t>(si)].berm_end == h.sets[static_cast<size_t>(si)].berm_beg) continue;
       bool on = false;
-      for (const auto& net : sur->nets) on = on || (net.set == si && net.enabled);
+      for (const auto& net : sur->nets)
+        for (const auto& o : net.outs) on = on || (o.set == si && o.enabled);
       allSurrogate = allSurrogate && on;
     }
     if (!allSurrogate)
@@ -988,6 +1119,7 @@ void ccr_hip_destroy(ccr_hip_ctx* ctx) {
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->sur_feats) (void)hipFree(ctx->sur_feats);
   delete ctx;
 }
 
@@ -1621,54 +1753,76 @@ int32_t ccr_hip_surrogate_train(ccr_hip_ctx* ctx, ccr_hip_plan* train, ccr_hip_p
     auto sur = std::make_unique<ccr_hip_surrogate>();
     sur->device = ctx->device;
     sur->num_sets = static_cast<int>(ht.sets.size());
+    sur->nets = surrogateNets(ht);
+    if (!surrogateFits(hh, sur.get())) invalid("surrogate: the plans hold different netting sets");
     ccr_surrogate_report rep{};
     // Method-generated pathwise values: the training plan executed with the regressions.
     launchAll(ctx, train);
     check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     const int64_t P = ht.g.num_paths;
     const int64_t nTrain = o->train_paths > 0 ? std::min<int64_t>(o->train_paths, P) : P;
-    int maxRows = 0;
-    for (int s = 0; s < static_cast<int>(ht.sets.size()); ++s) maxRows = std::max(maxRows, setRows(ht, s));
-    constexpr int kMaxIn = kSurrogateBase + kSurrogateMaxExercises;
-    float* feats = nullptr;
-    check(hipMalloc(reinterpret_cast<void**>(&feats), static_cast<size_t>(maxRows) * P * kMaxIn * sizeof(float)),
-          "hipMalloc");
-    std::unique_ptr<float, hipError_t (*)(void*)> featsOwner(feats, hipFree);
-    for (int s = 0; s < static_cast<int>(ht.sets.size()); ++s) {
-      const SetDev& S = ht.sets[static_cast<size_t>(s)];
-      if (S.berm_end == S.berm_beg) continue;
-      const int rows = setRows(ht, s);
-      const SurrogateExercises ex = exercisesOf(ht, s);
-      const int in = kSurrogateBase + ex.count;
-      check(launchSurrogateFeatures(train->d_states, S.state_off, P, train->d_dates + S.date_off, train->d_ctimes,
-                                    S.num_dates, S.cd_base, ex, feats, ctx->stream),
-            "surrogate features launch");
-      std::vector<float> f(static_cast<size_t>(rows) * P * in);
-      std::vector<double> v(static_cast<size_t>(rows) * P);
-      check(hipMemcpyAsync(f.data(), feats, f.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
-      check(hipMemcpyAsync(v.data(), train->d_bermv + static_cast<int64_t>(S.cd_base) * P, v.size() * sizeof(double),
-                           hipMemcpyDeviceToHost, ctx->stream),
-            "hipMemcpyAsync");
-      check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-      const int64_t n = static_cast<int64_t>(rows) * nTrain;
-      std::vector<double> X(static_cast<size_t>(n * in)), Y(static_cast<size_t>(n));
-      int64_t k = 0;
-      for (int r = 0; r < rows; ++r)
-        for (int64_t p = 0; p < nTrain; ++p, ++k) {
-          for (int i = 0; i < in; ++i) X[static_cast<size_t>(k * in + i)] = f[static_cast<size_t>((r * P + p) * in + i)];
-          Y[static_cast<size_t>(k)] = v[static_cast<size_t>(r * P + p)];
+    for (auto& net : sur->nets) {
+      net.d_urow = sur->upload(net.urow, ctx->stream);
+      net.d_utimes = sur->upload(net.utimes, ctx->stream);
+      const int in = net.inputs, nout = static_cast<int>(net.outs.size());
+      const int U = net.urows;
+      // Inputs of the training rows (u, p < nTrain) and targets: each output's value on the dates
+      // its set values (NaN elsewhere: a missing target the loss skips).
+      std::vector<float> f(static_cast<size_t>(U) * P * in);
+      if (U > 0) {
+        float* feats = nullptr;
+        check(hipMalloc(reinterpret_cast<void**>(&feats), f.size() * sizeof(float)), "hipMalloc");
+        std::unique_ptr<float, hipError_t (*)(void*)> featsOwner(feats, hipFree);
+        const SetDev& S = ht.sets[static_cast<size_t>(net.outs[0].set)];
+        check(launchSurrogateFeatures(train->d_states, S.state_off, P, net.d_urow, net.d_utimes, net.num_dates, net.ex,
+                                      feats, ctx->stream),
+              "surrogate features launch");
+        check(hipMemcpyAsync(f.data(), feats, f.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+      }
+      const int64_t n = static_cast<int64_t>(U) * nTrain;
+      if (n == 0) invalid("surrogate: no live Bermudan rows to train on");
+      std::vector<double> X(static_cast<size_t>(n * in)), Y(static_cast<size_t>(n * nout), std::nan(""));
+      for (int u = 0; u < U; ++u)
+        for (int64_t p = 0; p < nTrain; ++p)
+          for (int i = 0; i < in; ++i)
+            X[static_cast<size_t>((u * nTrain + p) * in + i)] = f[static_cast<size_t>((u * P + p) * in + i)];
+      for (int k = 0; k < nout; ++k) {
+        auto& out = net.outs[static_cast<size_t>(k)];
+        std::vector<double> v(static_cast<size_t>(out.rows) * P);
+        check(hipMemcpyAsync(v.data(), train->d_bermv + static_cast<int64_t>(out.cd_base) * P, v.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, ctx->stream),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        double sum = 0.0, cnt = 0.0;
+        for (int u = 0; u < U; ++u) {
+          const int32_t r = net.map[static_cast<size_t>(k) * U + u];
+          if (r < 0) continue;
+          for (int64_t p = 0; p < nTrain; ++p) {
+            const double y = v[static_cast<size_t>(r * P + p)];
+            Y[static_cast<size_t>((u * nTrain + p) * nout + k)] = y;
+            sum += y;
+            cnt += 1.0;
+          }
         }
-      ccr_hip_surrogate::Net net;
-      net.set = s;
-      net.nberms = S.berm_end - S.berm_beg;
-      net.inputs = in;
+        // Standardiser of this output over its present targets (mean, population sd; 0 -> 1).
+        out.ymean = cnt > 0.0 ? sum / cnt : 0.0;
+        double ss = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+          const double y = Y[static_cast<size_t>(i * nout + k)];
+          if (y == y) ss += (y - out.ymean) * (y - out.ymean);
+        }
+        out.ysd = cnt > 0.0 && ss > 0.0 ? std::sqrt(ss / cnt) : 1.0;
+        for (int64_t i = 0; i < n; ++i) {
+          double& y = Y[static_cast<size_t>(i * nout + k)];
+          if (y == y) y = (y - out.ymean) / out.ysd;
+        }
+      }
       standardiserFit(X.data(), n, in, net.xmean, net.xsd);
-      standardiserFit(Y.data(), n, 1, &net.ymean, &net.ysd);
-      for (int64_t i = 0; i < n; ++i) {
+      for (int64_t i = 0; i < n; ++i)
         for (int j = 0; j < in; ++j)
           X[static_cast<size_t>(i * in + j)] = (X[static_cast<size_t>(i * in + j)] - net.xmean[j]) / net.xsd[j];
-        Y[static_cast<size_t>(i)] = (Y[static_cast<size_t>(i)] - net.ymean) / net.ysd;
-      }
       if (o->batch > 0) {  // mini-batches of consecutive rows: shuffle the rows once
         std::vector<int64_t> perm(static_cast<size_t>(n));
         for (int64_t i = 0; i < n; ++i) perm[static_cast<size_t>(i)] = i;
@@ -1678,12 +1832,12 @@ int32_t ccr_hip_surrogate_train(ccr_hip_ctx* ctx, ccr_hip_plan* train, ccr_hip_p
         for (int64_t i = 0; i < n; ++i) {
           const int64_t q = perm[static_cast<size_t>(i)];
           for (int j = 0; j < in; ++j) X2[static_cast<size_t>(i * in + j)] = X[static_cast<size_t>(q * in + j)];
-          Y2[static_cast<size_t>(i)] = Y[static_cast<size_t>(q)];
+          for (int k = 0; k < nout; ++k) Y2[static_cast<size_t>(i * nout + k)] = Y[static_cast<size_t>(q * nout + k)];
         }
         X.swap(X2);
         Y.swap(Y2);
       }
-      const ccr_mlp_desc desc{in, o->hidden, o->hidden_layers, 1, CCR_ACT_SILU};
+      const ccr_mlp_desc desc{in, o->hidden, o->hidden_layers, nout, CCR_ACT_SILU};
       std::vector<double> params(static_cast<size_t>(mlpShape(desc).count));
       {
         MlpRun trainer(desc, CCR_BF16, n, ctx->stream);
@@ -1699,27 +1853,41 @@ int32_t ccr_hip_surrogate_train(ccr_hip_ctx* ctx, ccr_hip_plan* train, ccr_hip_p
       }
       net.run = std::make_unique<MlpRun>(desc, CCR_BF16, kMlpRows, ctx->stream);
       net.run->setParams(params.data());
+      std::vector<InferOut> io;
+      for (const auto& out : net.outs) io.push_back(InferOut{out.cd_base, out.ymean, out.ysd});
+      net.d_outs = sur->upload(io, ctx->stream);
+      net.d_map = sur->upload(net.map, ctx->stream);
       rep.samples += n;
-      rep.candidate_sets += 1;
-      sur->nets.push_back(std::move(net));
+      rep.candidate_sets += nout;
+      rep.networks += 1;
     }
-    // The gate: hold-out EE with the surrogate against the regression method.
+    // The gate: hold-out EE with the surrogate against the regression method, per netting set.
     launchAll(ctx, holdout);
     const auto plain = expectedExposure(ctx, holdout);
+    reserveSurrogateFeatures(ctx, holdout, sur.get());
     launchAll(ctx, holdout, sur.get());
     const auto approx = expectedExposure(ctx, holdout);
     for (auto& net : sur->nets) {
-      const auto& a = plain[static_cast<size_t>(net.set)];
-      const auto& b = approx[static_cast<size_t>(net.set)];
-      double peak = 0.0, err = 0.0;
-      for (size_t r = 0; r < a.size(); ++r) {
-        peak = std::max(peak, std::fabs(a[r]));
-        err = std::max(err, std::fabs(b[r] - a[r]));
+      std::vector<int32_t> map = net.map;
+      for (size_t k = 0; k < net.outs.size(); ++k) {
+        auto& out = net.outs[k];
+        const auto& a = plain[static_cast<size_t>(out.set)];
+        const auto& b = approx[static_cast<size_t>(out.set)];
+        double peak = 0.0, err = 0.0;
+        for (size_t r = 0; r < a.size(); ++r) {
+          peak = std::max(peak, std::fabs(a[r]));
+          err = std::max(err, std::fabs(b[r] - a[r]));
+        }
+        out.ee_error = peak > 0.0 ? err / peak : err;
+        out.enabled = out.ee_error <= o->gate_threshold;
+        rep.max_ee_error = std::max(rep.max_ee_error, out.ee_error);
+        rep.enabled_sets += out.enabled ? 1 : 0;
+        if (!out.enabled)
+          for (int u = 0; u < net.urows; ++u) map[k * static_cast<size_t>(net.urows) + static_cast<size_t>(u)] = -1;
       }
-      net.ee_error = peak > 0.0 ? err / peak : err;
-      net.enabled = net.ee_error <= o->gate_threshold;
-      rep.max_ee_error = std::max(rep.max_ee_error, net.ee_error);
-      rep.enabled_sets += net.enabled ? 1 : 0;
+      check(hipMemcpyAsync(net.d_map, map.data(), map.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
+            "hipMemcpyAsync");
+      check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     }
     if (report) *report = rep;
     *out = sur.release();
@@ -1731,10 +1899,11 @@ int32_t ccr_hip_surrogate_set_info(const ccr_hip_surrogate* sur, int32_t set, in
   if (enabled) *enabled = -1;
   if (ee_error) *ee_error = 0.0;
   for (const auto& net : sur->nets)
-    if (net.set == set) {
-      if (enabled) *enabled = net.enabled ? 1 : 0;
-      if (ee_error) *ee_error = net.ee_error;
-    }
+    for (const auto& o : net.outs)
+      if (o.set == set) {
+        if (enabled) *enabled = o.enabled ? 1 : 0;
+        if (ee_error) *ee_error = o.ee_error;
+      }
   return CCR_OK;
 }
 
@@ -1742,12 +1911,9 @@ int32_t ccr_hip_execute_surrogate_async(ccr_hip_ctx* ctx, ccr_hip_plan* plan, co
   if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
   return guarded(ctx, [&] {
     if (!plan || !sur) invalid("null argument");
-    if (static_cast<int>(plan->host.sets.size()) != sur->num_sets || sur->device != ctx->device)
-      invalid("surrogate: trained for other netting sets or another device");
-    for (const auto& net : sur->nets) {
-      const SetDev& S = plan->host.sets[static_cast<size_t>(net.set)];
-      if (S.berm_end - S.berm_beg != net.nberms) invalid("surrogate: trained for other netting sets");
-    }
+    if (sur->device != ctx->device) invalid("surrogate: trained on another device");
+    if (!surrogateFits(plan->host, sur)) invalid("surrogate: trained for other netting sets");
+    reserveSurrogateFeatures(ctx, plan, sur);
     launchAll(ctx, plan, sur);
   });
 }

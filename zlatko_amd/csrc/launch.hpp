@@ -135,22 +135,21 @@ struct AmcArgs {
 template <typename Real>
 hipError_t launchAmc(const AmcArgs& args, const Real* bonds, const Real* fixed, int max_exercises, hipStream_t stream);
 
-// Surrogate features of one set with Bermudans on every valued date and path, from the recorded
-// states: rows [valued date][path][kSurrogateBase + count] = (t, x, running max x, running min x,
-// x at each of the set's exercise dates once passed, else 0) (FP32).
+// Surrogate features of a group of netting sets with Bermudans that share one set of recorded
+// states (one set, or every such set of a book plan): rows [u][path][kSurrogateBase + count] =
+// (t, x, running max x, running min x, x at each exercise date once passed, else 0) (FP32) for
+// the group's rows u: date j of the states is row urow[j] (-1: no row), at time utimes[u].
 constexpr int kSurrogateBase = 4;
 constexpr int kSurrogateMaxExercises = 28;  // BF16 network inputs are at most 32
 struct SurrogateExercises {
   int32_t count;                           // distinct exercise dates used (the earliest ones)
-  int32_t date[kSurrogateMaxExercises];    // their indices on the set's simulation grid
+  int32_t date[kSurrogateMaxExercises];    // their indices on the simulation grid of the states
 };
-// `dates` are the set's date records (num_dates of them); ctimes the plan's continuation-date times.
-hipError_t launchSurrogateFeatures(const double* states, int64_t state_off, int64_t paths, const DateRec* dates,
-                                   const double* ctimes, int num_dates, int cd_base, const SurrogateExercises& ex,
-                                   float* feats, hipStream_t stream);
-// bermv rows [cd_base, cd_base + rows) x paths = y * sd + mean.
-hipError_t launchSurrogateScatter(const float* y, int64_t n_live, int64_t n, double mean, double sd, double* out,
-                                  hipStream_t stream);
+hipError_t launchSurrogateFeatures(const double* states, int64_t state_off, int64_t paths, const int32_t* urow,
+                                   const double* utimes, int num_dates, const SurrogateExercises& ex, float* feats,
+                                   hipStream_t stream);
+// out[0, n) = 0.
+hipError_t launchZeroRows(double* out, int64_t n, hipStream_t stream);
 
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);

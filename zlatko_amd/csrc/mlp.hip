@@ -198,7 +198,8 @@ __global__ void __launch_bounds__(256) bf16_fwd_kernel(const __bf16* ain, const 
   }
 }
 
-// Output deltas 2 (y - t) / (n out) in BF16 (sample- and feature-major) and squared-error sums.
+// Output deltas 2 (y - t) / (n out) in BF16 (sample- and feature-major) and squared-error sums. A
+// NaN target is a missing one (a netting set that does not value that date): no error, no delta.
 __global__ void __launch_bounds__(256) bf16_loss_kernel(const float* out, int po, const float* ys, int nout, int64_t n,
                                                         int64_t n_pad, __bf16* d_sm, __bf16* d_fm, float* loss_part) {
   __shared__ double lds[4];
@@ -207,7 +208,8 @@ __global__ void __launch_bounds__(256) bf16_loss_kernel(const float* out, int po
   if (idx < n * nout) {
     const int64_t s = idx / nout;
     const int k = static_cast<int>(idx % nout);
-    const float e = out[s * po + k] - ys[idx];
+    const float t = ys[idx];
+    const float e = t == t ? out[s * po + k] - t : 0.0f;
     const __bf16 d = static_cast<__bf16>(2.0f * e / static_cast<float>(n * nout));
     d_sm[s * po + k] = d;
     d_fm[static_cast<int64_t>(k) * n_pad + s] = d;
@@ -285,13 +287,21 @@ __global__ void __launch_bounds__(64) bf16_wgrad_kernel(const __bf16* dfm, const
   }
 }
 
-// Fused inference (hidden = 128): every layer of a 16-sample tile per wavefront, weights staged
-// in LDS, each layer's output turned into the next layer's A operand through a per-wavefront LDS
-// tile. Inputs x [n][inputs] (raw) are standardised on load with (mean, sd); writes the
-// standardised outputs y [n][outputs]. Grid-stride over tiles. The wavefronts per workgroup
-// (blockDim.x / 64, at most kInferWaves) are chosen at launch so that the weights and one
-// activation tile per wavefront fit the workgroup's LDS.
-constexpr int kInferWaves = 16;
+// Fused inference (hidden = 128): every layer of kTiles 16-sample tiles per wavefront, the weights
+// and biases staged once per workgroup in LDS. Each product is computed transposed, Z^T = W A^T: a
+// weight fragment (A operand, 16 outputs x 32 inputs) is read from LDS once and multiplied into
+// every sample tile of the wavefront (B operands, 32 inputs x 16 samples), and the accumulators
+// start at the bias. A layer's 8 x kTiles accumulator tiles are all computed before any is
+// activated, so the MFMA chains and the activations of different tiles interleave. The result
+// lands as four consecutive outputs of one sample per lane, so a layer's activations go back to the
+// wavefront's LDS tile in one 8-byte store per lane and tile, sample-major, where the next layer
+// reads its B operands (16 bytes per lane). Weight rows in LDS are padded by 8 elements so that the
+// 16 lanes reading the rows of one fragment hit distinct banks (unless the padding would leave no
+// room for four wavefronts: five hidden layers). Inputs x [n][inputs] (raw) are
+// standardised on load with (mean, sd). The last layer's outputs (standardised) go to the sink:
+// y [n][outputs], or Bermudan value rows (below). Grid-stride over groups of 16 kTiles samples. The
+// wavefronts per workgroup are chosen at launch so that the weights and one activation tile per
+// wavefront fit the workgroup's LDS.
 constexpr int kActStride = 128 + 8;  // BF16 elements per activation row in LDS (padding vs bank conflicts)
 constexpr size_t kInferLdsBytes = 160 * 1024;  // LDS of one gfx950 workgroup
 struct InferArgs {
@@ -299,85 +309,250 @@ struct InferArgs {
   const float* b[kMlpMaxLayers];
   const float* x;
   float* y;
-  float mean[32], sd[32];
+  float mean[32], rsd[32];  // input standardiser: (x - mean) * rsd, rsd = 1 / sd
   int64_t n;
-  int layers, inputs, outputs, act;
+  int layers, inputs, outputs;
+  int w0stride, wstride;  // LDS rows of the first layer's weights and of the others' (elements)
+  // Value-row sink (y == nullptr): sample s = u * paths + p; output o goes to
+  // rows[(outs[o].base + row_map[o * urows + u]) * paths + p] = y_o * outs[o].sd + outs[o].mean where
+  // the map is >= 0 (dates output o's netting set values), nowhere otherwise.
+  double* rows;
+  const int32_t* row_map;
+  const InferOut* outs;
+  int64_t paths;
+  int32_t urows;
 };
 
-__device__ inline void stageWeights(__bf16* dst, const __bf16* src, int elems) {
-  for (int i = static_cast<int>(threadIdx.x) * 8; i < elems; i += static_cast<int>(blockDim.x) * 8)
-    *reinterpret_cast<bf16x8*>(dst + i) = load8(src + i);
+// LDS elements (BF16) of the staged weights and floats of the staged biases.
+__host__ __device__ inline int inferWeightElems(int layers, int po, int w0stride, int wstride) {
+  return 128 * w0stride + (layers - 2) * 128 * wstride + po * wstride;
+}
+__host__ __device__ inline int inferBiasFloats(int layers, int po) { return (layers - 1) * 128 + po; }
+
+// Rows x fi weights (row-major, BF16) into LDS rows of `stride` elements.
+__device__ inline void stageRows(__bf16* dst, const __bf16* src, int rows, int fi, int stride) {
+  const int chunks = rows * (fi / 8);
+  for (int c = static_cast<int>(threadIdx.x); c < chunks; c += static_cast<int>(blockDim.x)) {
+    const int r = c / (fi / 8), k = (c % (fi / 8)) * 8;
+    *reinterpret_cast<bf16x8*>(dst + r * stride + k) = load8(src + r * fi + k);
+  }
 }
 
-__global__ void __launch_bounds__(64 * kInferWaves) bf16_infer_kernel(InferArgs a) {
+template <int kAct>
+__device__ inline float activate(float z) {
+  if constexpr (kAct == CCR_ACT_TANH) return tanhf(z);
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+
+// Activations of a layer's 8 x kTiles tiles into the wavefront's LDS tile.
+template <int kTiles, int kAct>
+__device__ inline void activateStore(const f32x4 (&acc)[8][kTiles], __bf16* act, int n, int g) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+#pragma unroll
+    for (int tl = 0; tl < kTiles; ++tl) {
+      bf16x4 v;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = static_cast<__bf16>(activate<kAct>(acc[t][tl][i]));
+      *reinterpret_cast<bf16x4*>(act + (16 * tl + n) * kActStride + t * 16 + 4 * g) = v;
+    }
+}
+
+// Wavefronts per workgroup at most: 12 with one tile (168 VGPRs), 8 with two (256 VGPRs).
+constexpr int inferMaxWaves(int tiles) { return tiles == 1 ? 12 : 8; }
+
+template <int kTiles, int kAct>
+__global__ void __launch_bounds__(64 * inferMaxWaves(kTiles)) bf16_infer_kernel(InferArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __bf16* wl = reinterpret_cast<__bf16*>(lds_raw);
-  // Layers in order: first 128 x 32, middle 128 x 128, last 32 x 128 (32 x 32 alone).
-  int wtotal = 0;
-  for (int l = 0; l < a.layers; ++l) {
-    const int fi = l == 0 ? 32 : 128, fo = l + 1 == a.layers ? 32 : 128;
-    stageWeights(wl + wtotal, a.w[l], fi * fo);
-    wtotal += fi * fo;
+  const int L = a.layers, po = (a.outputs + 31) / 32 * 32, ntl = po / 16;
+  // Weights: first 128 x 32, middle 128 x 128 (L - 2 of them), last po x 128; then the biases.
+  const int w0s = a.w0stride, ws = a.wstride;
+  stageRows(wl, a.w[0], 128, 32, w0s);
+  for (int l = 1; l + 1 < L; ++l) stageRows(wl + 128 * w0s + (l - 1) * 128 * ws, a.w[l], 128, 128, ws);
+  __bf16* wlast = wl + 128 * w0s + (L - 2) * 128 * ws;
+  stageRows(wlast, a.w[L - 1], po, 128, ws);
+  const int welems = inferWeightElems(L, po, w0s, ws);
+  float* bl = reinterpret_cast<float*>(wl + welems);
+  for (int l = 0; l < L; ++l) {
+    const int fo = l + 1 == L ? po : 128;
+    for (int i = static_cast<int>(threadIdx.x); i < fo; i += static_cast<int>(blockDim.x)) bl[l * 128 + i] = a.b[l][i];
   }
-  __bf16* act = wl + wtotal + (static_cast<int>(threadIdx.x) >> 6) * 16 * kActStride;
+  __bf16* act = wl + welems + 2 * inferBiasFloats(L, po) + (static_cast<int>(threadIdx.x) >> 6) * (16 * kTiles) * kActStride;
   __syncthreads();
-  const int lane = static_cast<int>(threadIdx.x) & 63, r = lane & 15, g = lane >> 4;
+  const int lane = static_cast<int>(threadIdx.x) & 63, n = lane & 15, g = lane >> 4;
   const int waves = static_cast<int>(blockDim.x) >> 6;
-  const int64_t tiles = (a.n + 15) / 16;
-  for (int64_t tile = static_cast<int64_t>(blockIdx.x) * waves + (threadIdx.x >> 6); tile < tiles;
-       tile += static_cast<int64_t>(gridDim.x) * waves) {
-    const int64_t s0 = tile * 16;
-    // Layer-0 A operand: sample s0 + r, features 8 g .. 8 g + 7.
-    bf16x8 a0;
+  const int64_t groups = (a.n + 16 * kTiles - 1) / (16 * kTiles);
+  for (int64_t grp = static_cast<int64_t>(blockIdx.x) * waves + (threadIdx.x >> 6); grp < groups;
+       grp += static_cast<int64_t>(gridDim.x) * waves) {
+    const int64_t s0 = grp * 16 * kTiles;
+    f32x4 acc[8][kTiles];
+    bf16x8 bfr[kTiles][4];
+    // Layer 0: B operands from the inputs (sample s0 + 16 tile + n, features 8 g .. 8 g + 7).
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int j = 8 * g + i;
-      const int64_t s = s0 + r;
-      float v = 0.0f;
-      if (j < a.inputs && s < a.n) v = (a.x[s * a.inputs + j] - a.mean[j]) / a.sd[j];
-      a0[i] = static_cast<__bf16>(v);
-    }
-    int wo = 0;
-    for (int l = 0; l < a.layers; ++l) {
-      const bool lastL = l + 1 == a.layers;
-      const int fi = l == 0 ? 32 : 128, nt = lastL ? 2 : 8;
-      const __bf16* w = wl + wo;
-      wo += fi * nt * 16;
-      bf16x8 afr[4];
-      if (l == 0) {
-        afr[0] = a0;
-      } else {
+    for (int tl = 0; tl < kTiles; ++tl) {
+      const int64_t s = s0 + 16 * tl + n;
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) afr[ks] = *reinterpret_cast<const bf16x8*>(act + r * kActStride + ks * 32 + 8 * g);
+      for (int i = 0; i < 8; ++i) {
+        const int j = 8 * g + i;
+        float v = 0.0f;
+        if (j < a.inputs && s < a.n) v = (a.x[s * a.inputs + j] - a.mean[j]) * a.rsd[j];
+        bfr[tl][0][i] = static_cast<__bf16>(v);
       }
-      for (int t = 0; t < nt; ++t) {
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (l == 0) {
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[0], *reinterpret_cast<const bf16x8*>(w + (t * 16 + r) * fi + 8 * g),
-                                                        acc, 0, 0, 0);
-        } else {
+    }
 #pragma unroll
-          for (int ks = 0; ks < 4; ++ks)
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                afr[ks], *reinterpret_cast<const bf16x8*>(w + (t * 16 + r) * fi + ks * 32 + 8 * g), acc, 0, 0, 0);
+    for (int t = 0; t < 8; ++t) {
+      const f32x4 bias = *reinterpret_cast<const f32x4*>(bl + t * 16 + 4 * g);
+      const bf16x8 wf = *reinterpret_cast<const bf16x8*>(wl + (t * 16 + n) * w0s + 8 * g);
+#pragma unroll
+      for (int tl = 0; tl < kTiles; ++tl) acc[t][tl] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, bfr[tl][0], bias, 0, 0, 0);
+    }
+    activateStore<kTiles, kAct>(acc, act, n, g);
+    // Hidden 128 x 128 layers (every lane reads its B operands before any lane of the wavefront
+    // overwrites the tile: wavefront-synchronous, LDS operations in order).
+    for (int l = 1; l + 1 < L; ++l) {
+      const __bf16* w = wl + 128 * w0s + (l - 1) * 128 * ws;
+#pragma unroll
+      for (int tl = 0; tl < kTiles; ++tl)
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+          bfr[tl][ks] = *reinterpret_cast<const bf16x8*>(act + (16 * tl + n) * kActStride + ks * 32 + 8 * g);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const f32x4 bias = *reinterpret_cast<const f32x4*>(bl + l * 128 + t * 16 + 4 * g);
+#pragma unroll
+        for (int tl = 0; tl < kTiles; ++tl) acc[t][tl] = bias;
+      }
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const bf16x8 wf = *reinterpret_cast<const bf16x8*>(w + (t * 16 + n) * ws + ks * 32 + 8 * g);
+#pragma unroll
+          for (int tl = 0; tl < kTiles; ++tl)
+            acc[t][tl] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, bfr[tl][ks], acc[t][tl], 0, 0, 0);
         }
-        const int o = t * 16 + r;
-        const float bo = a.b[l][o];
-        if (lastL) {
+      activateStore<kTiles, kAct>(acc, act, n, g);
+    }
+    // Last layer: po outputs (ntl tiles of 16).
+#pragma unroll
+    for (int tl = 0; tl < kTiles; ++tl)
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+        bfr[tl][ks] = *reinterpret_cast<const bf16x8*>(act + (16 * tl + n) * kActStride + ks * 32 + 8 * g);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (t >= ntl) break;
+      const f32x4 bias = *reinterpret_cast<const f32x4*>(bl + (L - 1) * 128 + t * 16 + 4 * g);
+#pragma unroll
+      for (int tl = 0; tl < kTiles; ++tl) acc[t][tl] = bias;
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const bf16x8 wf = *reinterpret_cast<const bf16x8*>(wlast + (t * 16 + n) * ws + ks * 32 + 8 * g);
+#pragma unroll
+        for (int tl = 0; tl < kTiles; ++tl)
+          acc[t][tl] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, bfr[tl][ks], acc[t][tl], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (t >= ntl) break;
+      const int o0 = t * 16 + 4 * g;  // this lane's four outputs o0 .. o0 + 3
+#pragma unroll
+      for (int tl = 0; tl < kTiles; ++tl) {
+        const int64_t s = s0 + 16 * tl + n;
+        if (s >= a.n) continue;
+        if (a.y) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (o0 + i < a.outputs) a.y[s * a.outputs + o0 + i] = acc[t][tl][i];
+        } else {
+          const int64_t u = s / a.paths, p = s - u * a.paths;
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const int64_t s = s0 + 4 * g + i;
-            if (o < a.outputs && s < a.n) a.y[s * a.outputs + o] = acc[i] + bo;
+            const int o = o0 + i;
+            if (o >= a.outputs) continue;
+            const int32_t r = a.row_map[static_cast<int64_t>(o) * a.urows + u];
+            if (r < 0) continue;
+            const InferOut oc = a.outs[o];
+            a.rows[(oc.base + static_cast<int64_t>(r)) * a.paths + p] = static_cast<double>(acc[t][tl][i]) * oc.sd + oc.mean;
           }
-        } else {
-          // This wavefront's tile rows 4 g + i, column o (all lanes have read the previous
-          // layer's rows into afr before any lane writes: wavefront-synchronous).
-#pragma unroll
-          for (int i = 0; i < 4; ++i) act[(4 * g + i) * kActStride + o] = static_cast<__bf16>(actInfer(a.act, acc[i] + bo));
         }
       }
     }
   }
+}
+
+// Launch shape of the fused inference: sample tiles per wavefront and wavefronts per workgroup
+// (two tiles while at least four wavefronts fit beside the weights, else one), and the LDS bytes.
+struct InferShape {
+  int tiles = 0, waves = 0, w0stride = 0, wstride = 0;
+  size_t lds = 0;
+};
+// Padded weight rows (conflict-free fragment reads) with two tiles per wavefront while at least
+// four wavefronts fit, else unpadded rows, two tiles and four wavefronts, else one tile.
+InferShape inferShape(const MlpShape& sh) {
+  const int po = sh.pad[sh.layers];
+  InferShape r;
+  const int strides[2][2] = {{32 + 8, 128 + 8}, {32, 128}};
+  for (int tiles : {1, 2})
+    for (const auto& st : strides) {
+      const size_t fixed = static_cast<size_t>(inferWeightElems(sh.layers, po, st[0], st[1])) * 2 +
+                           static_cast<size_t>(inferBiasFloats(sh.layers, po)) * 4;
+      const size_t tile = static_cast<size_t>(16 * tiles) * kActStride * 2;
+      if (fixed + tile > kInferLdsBytes) continue;
+      const int waves = static_cast<int>(std::min<size_t>(inferMaxWaves(tiles), (kInferLdsBytes - fixed) / tile));
+      if (tiles == 2 && waves < 4) continue;
+      r.tiles = tiles;
+      r.waves = waves;
+      r.w0stride = st[0];
+      r.wstride = st[1];
+      r.lds = fixed + static_cast<size_t>(waves) * tile;
+      return r;
+    }
+  return r;
+}
+
+template <int kTiles>
+void inferLaunch(const InferArgs& a, int act, const InferShape& is, unsigned grid, hipStream_t stream) {
+  if (act == CCR_ACT_TANH)
+    hipLaunchKernelGGL((bf16_infer_kernel<kTiles, CCR_ACT_TANH>), dim3(grid), dim3(64 * is.waves), is.lds, stream, a);
+  else
+    hipLaunchKernelGGL((bf16_infer_kernel<kTiles, CCR_ACT_SILU>), dim3(grid), dim3(64 * is.waves), is.lds, stream, a);
+}
+
+hipError_t launchInfer(const MlpDevice& m, InferArgs& a, const double* mean, const double* sd, hipStream_t stream) {
+  const MlpShape& sh = m.sh;
+  if (m.precision != CCR_BF16 || sh.pad[0] != 32 || sh.layers < 2) return hipErrorInvalidValue;
+  for (int l = 1; l < sh.layers; ++l)
+    if (sh.size[l] != 128) return hipErrorInvalidValue;
+  for (int l = 0; l < sh.layers; ++l) {
+    a.w[l] = static_cast<const __bf16*>(m.w16[l]);
+    a.b[l] = m.bpad[l];
+  }
+  for (int j = 0; j < sh.size[0]; ++j) {
+    a.mean[j] = static_cast<float>(mean[j]);
+    a.rsd[j] = static_cast<float>(1.0 / sd[j]);
+  }
+  a.layers = sh.layers;
+  a.inputs = sh.size[0];
+  a.outputs = sh.size[sh.layers];
+  const InferShape is = inferShape(sh);
+  if (is.tiles == 0) return hipErrorInvalidValue;
+  a.w0stride = is.w0stride;
+  a.wstride = is.wstride;
+  const int64_t groups = (a.n + 16 * is.tiles - 1) / (16 * is.tiles);
+  const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((groups + is.waves - 1) / is.waves, 4096)));
+  const void* fns[4] = {reinterpret_cast<const void*>(&bf16_infer_kernel<1, CCR_ACT_TANH>),
+                        reinterpret_cast<const void*>(&bf16_infer_kernel<1, CCR_ACT_SILU>),
+                        reinterpret_cast<const void*>(&bf16_infer_kernel<2, CCR_ACT_TANH>),
+                        reinterpret_cast<const void*>(&bf16_infer_kernel<2, CCR_ACT_SILU>)};
+  for (const void* fn : fns)
+    if (hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(is.lds)))
+      return e;
+  if (is.tiles == 2) inferLaunch<2>(a, sh.act, is, grid, stream);
+  else inferLaunch<1>(a, sh.act, is, grid, stream);
+  return hipGetLastError();
 }
 
 hipError_t last() { return hipGetLastError(); }
@@ -388,41 +563,29 @@ unsigned blocksFor(int64_t n, int per) { return static_cast<unsigned>((n + per -
 
 hipError_t mlpInfer(const MlpDevice& m, const float* x, int64_t n, const double* mean, const double* sd, float* y,
                     hipStream_t stream) {
-  const MlpShape& sh = m.sh;
-  if (m.precision != CCR_BF16 || sh.pad[0] != 32 || sh.pad[sh.layers] != 32) return hipErrorInvalidValue;
-  for (int l = 1; l < sh.layers; ++l)
-    if (sh.size[l] != 128) return hipErrorInvalidValue;
+  if (n <= 0) return hipSuccess;
+  if (!y) return hipErrorInvalidValue;
   InferArgs a{};
-  for (int l = 0; l < sh.layers; ++l) {
-    a.w[l] = static_cast<const __bf16*>(m.w16[l]);
-    a.b[l] = m.bpad[l];
-  }
-  for (int j = 0; j < sh.size[0]; ++j) {
-    a.mean[j] = static_cast<float>(mean[j]);
-    a.sd[j] = static_cast<float>(sd[j]);
-  }
   a.x = x;
   a.y = y;
   a.n = n;
-  a.layers = sh.layers;
-  a.inputs = sh.size[0];
-  a.outputs = sh.size[sh.layers];
-  a.act = sh.act;
-  int welems = 0;
-  for (int l = 0; l < sh.layers; ++l) welems += sh.pad[l] * sh.pad[l + 1];
-  // As many wavefronts (16 at most) as leave room for one activation tile each beside the weights:
-  // 16 up to 3 hidden layers, 3 at 5 hidden layers.
-  const size_t wbytes = static_cast<size_t>(welems) * 2, tile = static_cast<size_t>(16) * kActStride * 2;
-  if (wbytes + tile > kInferLdsBytes) return hipErrorInvalidValue;
-  const int waves = static_cast<int>(std::min<size_t>(kInferWaves, (kInferLdsBytes - wbytes) / tile));
-  const size_t lds = wbytes + static_cast<size_t>(waves) * tile;
-  const int64_t tiles = (n + 15) / 16;
-  const unsigned grid = static_cast<unsigned>(std::min<int64_t>((tiles + waves - 1) / waves, 2048));
-  if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&bf16_infer_kernel),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)))
-    return e;
-  hipLaunchKernelGGL(bf16_infer_kernel, dim3(grid), dim3(64 * waves), lds, stream, a);
-  return last();
+  return launchInfer(m, a, mean, sd, stream);
+}
+
+hipError_t mlpInferRows(const MlpDevice& m, const float* x, int64_t n, const double* mean, const double* sd,
+                        int64_t paths, int32_t urows, const int32_t* row_map, const InferOut* outs, double* rows,
+                        hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  if (!rows || !row_map || !outs || paths <= 0 || urows <= 0 || n > paths * urows) return hipErrorInvalidValue;
+  InferArgs a{};
+  a.x = x;
+  a.n = n;
+  a.rows = rows;
+  a.row_map = row_map;
+  a.outs = outs;
+  a.paths = paths;
+  a.urows = urows;
+  return launchInfer(m, a, mean, sd, stream);
 }
 
 hipError_t mlpLoadInputs(const MlpDevice& m, const float* xs, hipStream_t stream) {
@@ -467,6 +630,12 @@ hipError_t mlpForward(const MlpDevice& m, hipStream_t stream) {
       hipLaunchKernelGGL((bf16_fwd_kernel<128, 128, false>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.ld, sh.act);
     else if (pi == 128 && po == 32 && lastL)
       hipLaunchKernelGGL((bf16_fwd_kernel<128, 32, true>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.ld, sh.act);
+    else if (pi == 128 && po == 64 && lastL)
+      hipLaunchKernelGGL((bf16_fwd_kernel<128, 64, true>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.ld, sh.act);
+    else if (pi == 128 && po == 96 && lastL)
+      hipLaunchKernelGGL((bf16_fwd_kernel<128, 96, true>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.ld, sh.act);
+    else if (pi == 128 && po == 128 && lastL)
+      hipLaunchKernelGGL((bf16_fwd_kernel<128, 128, true>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.ld, sh.act);
     else if (pi == 32 && po == 32 && lastL)
       hipLaunchKernelGGL((bf16_fwd_kernel<32, 32, true>), grid, blk, 0, stream, ain, w, m.bpad[l], m.z[l + 1], asm_, afm, m.ld, sh.act);
     else
@@ -511,6 +680,10 @@ hipError_t mlpBackward(const MlpDevice& m, hipStream_t stream) {
       const int pi = sh.pad[l], po = sh.pad[l + 1];
       if (pi == 128 && po == 32)
         hipLaunchKernelGGL((bf16_bwd_kernel<128, 32>), grid, blk, 0, stream, dn, wt, m.z[l], dsm, dfm, m.ld, sh.act);
+      else if (pi == 128 && po == 64)
+        hipLaunchKernelGGL((bf16_bwd_kernel<128, 64>), grid, blk, 0, stream, dn, wt, m.z[l], dsm, dfm, m.ld, sh.act);
+      else if (pi == 128 && po == 96)
+        hipLaunchKernelGGL((bf16_bwd_kernel<128, 96>), grid, blk, 0, stream, dn, wt, m.z[l], dsm, dfm, m.ld, sh.act);
       else if (pi == 128 && po == 128)
         hipLaunchKernelGGL((bf16_bwd_kernel<128, 128>), grid, blk, 0, stream, dn, wt, m.z[l], dsm, dfm, m.ld, sh.act);
       else

@@ -74,9 +74,21 @@ hipError_t mlpAdam(const MlpDevice& m, float lr, float wd, float c1, float c2, h
 hipError_t mlpRefreshBf16(const MlpDevice& m, hipStream_t stream);      // BF16 weight copies from params
 // BF16 inputs from FP32 standardised rows (the BF16 path's layer-0 activations).
 hipError_t mlpLoadInputs(const MlpDevice& m, const float* xs, hipStream_t stream);
-// Fused BF16 inference with m's weights (hidden = 128): raw rows x [n][inputs], standardised on
-// load with (mean, sd); standardised outputs y [n][outputs]. Needs no per-sample buffers.
+// Fused BF16 inference with m's weights (hidden = 128, inputs <= 32, outputs <= 128): raw rows
+// x [n][inputs], standardised on load with (mean, sd); standardised outputs y [n][outputs]. Needs no
+// per-sample buffers.
 hipError_t mlpInfer(const MlpDevice& m, const float* x, int64_t n, const double* mean, const double* sd, float* y,
                     hipStream_t stream);
+// Destination of one network output in mlpInferRows: rows base + r of a [row][path] FP64 array,
+// de-standardised with (mean, sd).
+struct InferOut {
+  int64_t base;
+  double mean, sd;
+};
+// mlpInfer with the outputs written straight into value rows: sample s = u * paths + p (u < urows);
+// output o goes to rows[(outs[o].base + row_map[o * urows + u]) * paths + p] where row_map >= 0.
+hipError_t mlpInferRows(const MlpDevice& m, const float* x, int64_t n, const double* mean, const double* sd,
+                        int64_t paths, int32_t urows, const int32_t* row_map, const InferOut* outs, double* rows,
+                        hipStream_t stream);
 
 }  // namespace ccrhip

@@ -98,7 +98,8 @@ MlpRun::MlpRun(const ccr_mlp_desc& d, int precision, int64_t n, hipStream_t stre
   const int L = sh.layers;
   if (precision == CCR_BF16) {
     if (d.hidden_layers > 0 && d.hidden != 128) fail(CCR_ERR_UNSUPPORTED, "Mlp BF16: hidden width must be 128");
-    if (d.inputs > 32 || d.outputs > 32) fail(CCR_ERR_UNSUPPORTED, "Mlp BF16: at most 32 inputs and 32 outputs");
+    if (d.inputs > 32) fail(CCR_ERR_UNSUPPORTED, "Mlp BF16: at most 32 inputs");
+    if (d.hidden_layers == 0 && d.outputs > 32) fail(CCR_ERR_UNSUPPORTED, "Mlp BF16: at most 32 outputs without hidden layers");
   }
   m_.precision = precision;
   m_.n = n;

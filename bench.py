@@ -272,6 +272,51 @@ def cva_wall_time_c5(engine, specs, steps: int, outs64, ms64: float, ms32: float
     return res
 
 
+def surrogate_stage(paths: int, epochs: int, steps: int) -> dict:
+    """The Bermudan stage (states pass plus regressions, or plus the network) of one FP32 execute of
+    the portfolio book's first wave -- the 50 largest netting sets on the whole book's grids -- with
+    the regression method and with the surrogate network (gate open), HIP-event timed."""
+    engine = native.Engine()
+    spec = W.c5(num_paths=paths)
+    engine.set_book(True)
+    rep, sim = native.book_grids(spec)
+    engine.set_book_grids(rep, sim)
+    wave = split_waves(spec, 50)[0]
+    mk = lambda p, seed: abi.Spec(wave.model, wave.sets, wave.base_grid, p, seed, True, wave.pfe_quantile,  # noqa: E731
+                                  wave.name)
+    tr, ho = engine.compile(mk(8000, 101)), engine.compile(mk(8000, 202))
+    print("surrogate stage: training plans compiled", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    sur = engine.surrogate_train(tr, ho, train_paths=4000, epochs=epochs, lr=5e-4, batch=4096, gate=1.0)
+    train_s = time.perf_counter() - t0
+    print(f"surrogate stage: trained in {train_s:.1f} s", file=sys.stderr, flush=True)
+    tr.close()
+    ho.close()
+    plan = engine.compile(mk(paths, 42), abi.CCR_FP32)
+    print("surrogate stage: production plan compiled", file=sys.stderr, flush=True)
+
+    def stage(run):
+        run()
+        engine.synchronize()
+        t = []
+        for _ in range(max(1, steps)):
+            run()
+            t.append(engine.timing())
+        return float(np.median([x.amc_ms for x in t])), float(np.median([x.total_ms for x in t]))
+
+    reg, reg_total = stage(plan.execute_async)
+    net, net_total = stage(lambda: plan.execute_surrogate_async(sur))
+    r = sur.report
+    out = {"workload": "first wave of the portfolio book (50 largest netting sets), FP32 plan", "paths": paths,
+           "bermudan_stage_regressions_ms": reg, "bermudan_stage_surrogate_ms": net, "stage_speedup": reg / net,
+           "execute_regressions_ms": reg_total, "execute_surrogate_ms": net_total, "networks": r.networks,
+           "outputs": r.candidate_sets, "holdout_ee_err_max": r.max_ee_error, "train_s": train_s, "epochs": epochs}
+    plan.close()
+    sur.close()
+    engine.close()
+    return out
+
+
 def host_cpu() -> str:
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -358,6 +403,11 @@ def main():
                          "3); default: sa for c3 (the configuration's basis), shared otherwise")
     ap.add_argument("--gram", choices=["fp64", "fp32"], default="fp32",
                     help="regression-engine Grams in FP64 or on FP32 matrix cores")
+    ap.add_argument("--surrogate-stage", action="store_true",
+                    help="only the Bermudan stage of the portfolio book's first wave (the 50 largest netting sets) "
+                         "at --paths (default 200,000), FP32 plan: regressions against the surrogate network "
+                         "(trained for --sur-epochs on 8,000-path plans)")
+    ap.add_argument("--sur-epochs", type=int, default=60)
     ap.add_argument("--wave-sets", type=int, default=0,
                     help="netting sets per plan (0: all in one plan unless sets x paths exceeds 1e7)")
     args = ap.parse_args()
@@ -373,6 +423,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8}))
+        return
+
+    if args.surrogate_stage:
+        print(json.dumps(surrogate_stage(args.paths or 200000, args.sur_epochs, args.steps)), flush=True)
         return
 
     make, default_paths = WORKLOADS[args.workload]

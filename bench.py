@@ -317,6 +317,53 @@ def surrogate_stage(paths: int, epochs: int, steps: int) -> dict:
     return out
 
 
+def nested_im(workload: str, paths: int, inner: int, steps: int) -> dict:
+    """GPU time of the nested conditional initial margin on one workload (the execute with it minus
+    the plain execute, both HIP-event timed), and the FP64 oracle's time on a path sample."""
+    if workload not in ("c1", "c2"):
+        raise SystemExit("--nested-im runs on c1 or c2 (no Bermudans)")
+    make, default_paths = WORKLOADS[workload]
+    spec = make(paths or default_paths)
+    q, mpor = 0.99, 10.0 / 250.0
+    engine = native.Engine()
+
+    def timed(plan):
+        plan.execute_async()
+        engine.synchronize()
+        t = []
+        for _ in range(max(1, steps)):
+            plan.execute_async()
+            t.append(engine.timing().total_ms)
+        return float(np.median(t))
+
+    plain = engine.compile(spec)
+    plain_ms = timed(plain)
+    plain.close()
+    engine.set_nested_initial_margin(inner, q, mpor)
+    plan = engine.compile(spec)
+    nested_ms = timed(plan)
+    info = plan.info()
+    plan.execute()
+    im = plan.nested_initial_margin(0)
+    plan.close()
+    engine.close()
+    from tests import oracle_lib
+    sample = make(64)
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    ref = oracle_lib.oracle_nested_im(sample, 0, inner, q, mpor, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    reps = int(info.max_reporting_dates)
+    evals = int(info.num_paths) * reps * inner * len(spec.sets)
+    return {"workload": WORKLOAD_TEXT[workload], "paths": int(info.num_paths), "inner_paths": inner, "quantile": q,
+            "execute_plain_ms": plain_ms, "execute_with_nested_im_ms": nested_ms,
+            "nested_im_ms": nested_ms - plain_ms, "inner_revaluations": evals,
+            "inner_revaluations_per_s": evals / ((nested_ms - plain_ms) * 1e-3),
+            "set0_mean_im_first": float(im["mean"][0]),
+            "cpu_oracle": {"sample": "netting set 0, 64 paths", "seconds": cpu_s, "threads": threads,
+                           "inner_revaluations_per_s": 64 * len(ref["mean"]) * inner / cpu_s}}
+
+
 def host_cpu() -> str:
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -408,6 +455,10 @@ def main():
                          "at --paths (default 200,000), FP32 plan: regressions against the surrogate network "
                          "(trained for --sur-epochs on 8,000-path plans)")
     ap.add_argument("--sur-epochs", type=int, default=60)
+    ap.add_argument("--nested-im", type=int, default=0, metavar="INNER",
+                    help="only the nested conditional initial margin (INNER inner paths, 99%%, 10-day margin period) "
+                         "on --workload (c1 or c2) at --paths: the execute with and without it, and the FP64 oracle "
+                         "on a path sample")
     ap.add_argument("--wave-sets", type=int, default=0,
                     help="netting sets per plan (0: all in one plan unless sets x paths exceeds 1e7)")
     args = ap.parse_args()
@@ -425,6 +476,9 @@ def main():
         print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8}))
         return
 
+    if args.nested_im:
+        print(json.dumps(nested_im(args.workload, args.paths, args.nested_im, args.steps)), flush=True)
+        return
     if args.surrogate_stage:
         print(json.dumps(surrogate_stage(args.paths or 200000, args.sur_epochs, args.steps)), flush=True)
         return

@@ -399,6 +399,36 @@ int32_t ccr_hip_set_initial_margin(ccr_hip_ctx* ctx, double quantile, double mar
 int32_t ccr_hip_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set, double* margin,
                                int32_t capacity, int32_t* size);
 
+/* Nested conditional initial margin (gpu::FusedOptions::imInnerPaths / imQuantile /
+ * imMarginPeriod, include/ccr/gpu/fused_exposure.hpp:293-304; the kernel's inner Monte Carlo,
+ * src/gpu/fused_kernels.cpp:445-479; the grid extension, src/gpu/fused_exposure.cpp:199-212).
+ * Plans compiled after ccr_hip_set_nested_initial_margin(ctx, inner_paths, quantile, mpor) with
+ * inner_paths >= 2 add t + mpor of every reporting date t (up to the horizon) to each netting
+ * set's simulation grid and, on every execute, run from every path and reporting date t
+ * inner_paths inner paths (antithetic pairs, from the path's state at t, drawn from a
+ * counter-based generator keyed by the seed, netting set, path, date and pair) to t + mpor, and
+ * take IM(t, path) = the quantile of |V(t + mpor) - V(t)| of the uncollateralised netted value
+ * over them (0 where t + mpor is past the horizon). At most 1,024 inner paths, and the
+ * quantile's order statistic must lie among the 16 largest samples (the reference's rule).
+ * Netting sets with Bermudan swaptions, book plans and per-trade plans are not supported.
+ * inner_paths = 0 switches it off for later compiles. After an execute,
+ * ccr_hip_nested_initial_margin gives per reporting date of netting set `set`: E[D IM(t)]
+ * (FusedResult::conditionalImDiscounted), E[IM(t)] (conditionalImMean) and the 5 / 50 / 95%
+ * quantiles of IM(t) over the paths (conditionalImBands). */
+typedef struct ccr_nested_im_out {
+  int32_t capacity;        /* entries available in each array below */
+  int32_t num_dates;       /* written: reporting dates */
+  double* discounted;      /* E[D IM] */
+  double* mean;            /* E[IM] */
+  double* band05;          /* 5% quantile of IM over the paths */
+  double* band50;
+  double* band95;
+} ccr_nested_im_out;
+int32_t ccr_hip_set_nested_initial_margin(ccr_hip_ctx* ctx, int32_t inner_paths, double quantile,
+                                          double margin_period_of_risk);
+int32_t ccr_hip_nested_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set,
+                                      ccr_nested_im_out* out);
+
 /* Graph mode (no counterpart in the reference): while enabled, ccr_hip_execute and
  * ccr_hip_execute_async replay the plan's whole execute -- states pass, regressions, path kernel,
  * reductions, selects -- as one HIP graph, captured on the plan's first execute in this mode (and

@@ -22,6 +22,7 @@
 #include <limits>
 #include <memory>
 #include <optional>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -1269,6 +1270,180 @@ std::vector<Trade> tradesOf(const ccr_netting_set* set, const Generator* g) {
   return trades;
 }
 
+// ---------------------------------------------------------------- nested initial margin
+// The nested conditional initial margin of the fused pipeline (gpu::FusedOptions::imInnerPaths,
+// src/gpu/fused_kernels.cpp:445-479, the CPU mirror src/gpu/fused_reference.cpp:371-402; the grid
+// extension src/gpu/fused_exposure.cpp:199-212) restated in FP64 on the engine's scenarios: from
+// every path and reporting date t whose t + mpor lies on the simulation grid, `inner` inner paths
+// (antithetic pairs) continue the path's rates and assets over (t, t + mpor] and revalue the
+// netting set there (fixings inside the window from the inner path, earlier ones from the outer
+// path); IM = the interpolated order statistic of |V_inner(t + mpor) - V(t)| at level q, read from
+// the 16 largest samples as the kernel does (the top-16 rule of gpu::compile guarantees the
+// statistic is among them). The inner normals are this build's own counter-based generator (the
+// reference's fused pipeline keys PCG draws by its own stream layout; an FP64 restatement of the
+// engine cannot share them): SplitMix64's finaliser chained over (seed, set, path, date, pair),
+// normal k of inner step j by the Box-Muller cosine branch of two uniforms.
+std::uint64_t nestedMix(std::uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double nestedNormal(std::uint64_t key, std::size_t step, std::size_t k) {
+  const std::uint64_t c = key ^ ((static_cast<std::uint64_t>(step) << 16) | static_cast<std::uint64_t>(k));
+  const std::uint64_t a = nestedMix(c), b = nestedMix(c ^ 0xD1B54A32D192ED03ull);
+  const double u1 = (static_cast<double>(a >> 11) + 0.5) * 0x1.0p-53;
+  const double u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+// The simulation grid of a netting set with the nested initial margin: the engine's grid merged
+// with t + mpor for every reporting date t up to its horizon (fused_exposure.cpp:199-212).
+Grid nestedSimulationGrid(const Grid& rep, const Grid& sim, double mpor) {
+  Vec ahead;
+  for (double t : rep.t)
+    if (t + mpor <= sim.t.back() + 1e-10) ahead.push_back(t + mpor);
+  Vec v = sim.t;
+  v.insert(v.end(), ahead.begin(), ahead.end());
+  return Grid(v);
+}
+
+struct NestedResult {
+  Vec discounted, mean, b05, b50, b95;
+  Vec samples;  // the inner samples |V_inner - V| of (samplePath, sampleRep), when asked
+  double sampleIm = 0.0;
+};
+
+NestedResult nestedIm(const Generator& g, const ccr_simulation_config& cfg, const Grid& base, const ccr_netting_set& ns,
+                      int setIndex, int inner, double q, double mpor, int threads, long samplePath, int sampleRep) {
+  if (ns.num_trades <= 0 || !ns.trades) invalid("ExposureEngine::run: empty netting set");
+  std::vector<Trade> trades;
+  for (int i = 0; i < ns.num_trades; ++i) trades.push_back(makeTrade(ns.trades[i], g));
+  for (const auto& t : trades)
+    if (t.kind == CCR_TRADE_BERMUDAN_SWAPTION) throw Failure(CCR_ERR_UNSUPPORTED, "nested initial margin: Bermudan swaptions");
+  const Grid rep = reportingGrid(base, trades);
+  const Grid sim = nestedSimulationGrid(rep, simulationGrid(base, trades, ns), mpor);
+  const Scenarios s = g.generate(sim, static_cast<std::size_t>(cfg.num_paths), cfg.seed, cfg.antithetic != 0, nullptr);
+  const std::size_t nP = s.P, nT = s.grid.size(), nR = rep.size();
+  const std::size_t nF = g.foreign.size(), nA = g.assets.size(), nCorr = g.numFactors();
+  std::size_t nz = nCorr + 1;
+  for (const auto& c : g.credits) nz += c.sub - 1;
+  const std::size_t foreignOff = nz;
+  nz += nF;
+  std::vector<std::size_t> ri(nR), target(nR, nT);
+  for (std::size_t r = 0; r < nR; ++r) {
+    ri[r] = *s.grid.find(rep[r]);
+    if (const auto k = s.grid.find(s.grid[ri[r]] + mpor, 1e-8)) target[r] = *k;
+  }
+  auto value = [&](const Scenarios& sc, std::size_t p, std::size_t j) {
+    double total = 0.0;
+    for (std::size_t i = 0; i < trades.size(); ++i) {
+      const double v = pathwiseValue(trades[i], g, sc, p, j);
+      total = i == 0 ? v : total + v;
+    }
+    return total;
+  };
+  Mat im(nP, nR, 0.0);
+  NestedResult res;
+  const double rank = q * static_cast<double>(inner - 1);
+  const auto kq = static_cast<std::size_t>(std::floor(rank));
+  const std::size_t hiIdx = std::min<std::size_t>(static_cast<std::size_t>(inner) - 1 - kq, 15);
+  std::mutex sampleLock;
+  parallelFor(nP, threads, [&](std::size_t p) {
+    // One path's scenario (P = 1) holding the outer path; an inner path overwrites (t, t + mpor].
+    Scenarios w;
+    w.grid = s.grid;
+    w.P = 1;
+    w.x = Mat(1, nT, 0.0);
+    w.D = Mat(1, nT, 1.0);
+    w.xf.assign(nF, Mat(1, nT, 0.0));
+    w.S.assign(nA, Mat(1, nT, 0.0));
+    for (std::size_t j = 0; j < nT; ++j) {
+      w.x(0, j) = s.x(p, j);
+      for (std::size_t f = 0; f < nF; ++f) w.xf[f](0, j) = s.xf[f](p, j);
+      for (std::size_t a = 0; a < nA; ++a) w.S[a](0, j) = s.S[a](p, j);
+    }
+    const std::uint64_t pathKey = nestedMix(nestedMix(nestedMix(cfg.seed) ^ static_cast<std::uint64_t>(setIndex)) ^
+                                            static_cast<std::uint64_t>(p));
+    Vec zz(nz, 0.0), wv(nCorr), irf(nF), samples;
+    for (std::size_t r = 0; r < nR; ++r) {
+      const std::size_t j = ri[r], tg = target[r];
+      if (tg >= nT) continue;  // t + mpor past the horizon: IM 0
+      const double V = value(s, p, j);
+      double top[16] = {};
+      samples.clear();
+      const std::uint64_t dateKey = nestedMix(pathKey ^ (static_cast<std::uint64_t>(j) << 32));
+      for (int i = 0; i < inner; ++i) {
+        const std::uint64_t key = nestedMix(dateKey ^ static_cast<std::uint64_t>(i >> 1));
+        const double sign = (i & 1) ? -1.0 : 1.0;
+        double x = s.x(p, j);
+        Vec xf(nF), spot(nA);
+        for (std::size_t f = 0; f < nF; ++f) xf[f] = s.xf[f](p, j);
+        for (std::size_t a = 0; a < nA; ++a) spot[a] = s.S[a](p, j);
+        for (std::size_t jj = j + 1; jj <= tg; ++jj) {
+          // The step of ScenarioGenerator::generate (scenario_generator.cpp:118-134) for the rates
+          // and assets on the inner normals (drivers, then the integral normals).
+          for (std::size_t k = 0; k < nCorr; ++k) zz[k] = sign * nestedNormal(key, jj, k);
+          zz[nCorr] = sign * nestedNormal(key, jj, nCorr);
+          for (std::size_t f = 0; f < nF; ++f) zz[foreignOff + f] = sign * nestedNormal(key, jj, foreignOff + f);
+          const double t0 = s.grid[jj - 1], t1 = s.grid[jj];
+          for (std::size_t a = 0; a < nCorr; ++a) {
+            double sum = 0.0;
+            for (std::size_t k = 0; k <= a; ++k) sum += g.L(a, k) * zz[k];
+            wv[a] = sum;
+          }
+          double ir = 0.0;
+          g.rates.evolve(t0, t1, x, wv[0], zz[nCorr], 0.0, x, ir);
+          w.x(0, jj) = x;
+          for (std::size_t f = 0; f < nF; ++f) {
+            g.foreign[f].evolve(t0, t1, xf[f], wv[1 + f], zz[foreignOff + f], g.quanto[f], xf[f], irf[f]);
+            w.xf[f](0, jj) = xf[f];
+          }
+          for (std::size_t a = 0; a < nA; ++a) {
+            const Asset& as = g.assets[a];
+            spot[a] = as.foreign < 0 ? as.evolve(t0, t1, spot[a], ir, wv[1 + nF + a])
+                                     : as.evolveWithCarry(t0, t1, spot[a], ir, irf[as.foreign], wv[1 + nF + a]);
+            w.S[a](0, jj) = spot[a];
+          }
+        }
+        double cur = std::fabs(value(w, 0, tg) - V);
+        if (static_cast<long>(p) == samplePath && static_cast<int>(r) == sampleRep) samples.push_back(cur);
+        for (int k = 0; k < 16; ++k)
+          if (cur > top[k]) std::swap(cur, top[k]);
+      }
+      // Restore the outer path over the window.
+      for (std::size_t jj = j + 1; jj <= tg; ++jj) {
+        w.x(0, jj) = s.x(p, jj);
+        for (std::size_t f = 0; f < nF; ++f) w.xf[f](0, jj) = s.xf[f](p, jj);
+        for (std::size_t a = 0; a < nA; ++a) w.S[a](0, jj) = s.S[a](p, jj);
+      }
+      const double ok = top[hiIdx], ok1 = hiIdx > 0 ? top[hiIdx - 1] : ok;
+      im(p, r) = ok + (rank - static_cast<double>(kq)) * (ok1 - ok);
+      if (static_cast<long>(p) == samplePath && static_cast<int>(r) == sampleRep) {
+        std::lock_guard<std::mutex> lk(sampleLock);
+        res.samples = samples;
+        res.sampleIm = im(p, r);
+      }
+    }
+  });
+  const double n = static_cast<double>(nP);
+  for (Vec* v : {&res.discounted, &res.mean, &res.b05, &res.b50, &res.b95}) v->assign(nR, 0.0);
+  for (std::size_t r = 0; r < nR; ++r) {
+    double sd = 0.0, sm = 0.0;
+    Vec col(nP);
+    for (std::size_t p = 0; p < nP; ++p) {
+      sd += s.D(p, ri[r]) * im(p, r) / n;
+      sm += im(p, r) / n;
+      col[p] = im(p, r);
+    }
+    res.discounted[r] = sd;
+    res.mean[r] = sm;
+    res.b05[r] = quantile(col, 0.05);
+    res.b50[r] = quantile(col, 0.5);
+    res.b95[r] = quantile(col, 0.95);
+  }
+  return res;
+}
+
 }  // namespace oracle
 
 using namespace oracle;
@@ -1375,6 +1550,41 @@ int32_t ccr_oracle_run_book(const ccr_scenario_generator* generator, const ccr_s
       runSet(g, *config, base, sets[i], pfe_quantile, threads, &outs[i], nullptr, 0.99, 10.0 / 250.0, nullptr, &rep,
              &sim);
     gTiming.total_s = secondsSince(t0);
+  });
+}
+
+// Nested conditional initial margin of netting set `set_index` of `sets` (see nestedIm): per
+// reporting date E[D IM], E[IM] and the 5 / 50 / 95% quantiles of IM over the paths. sample_path
+// >= 0: also the inner samples of (sample_path, sample_rep) into samples (inner values) and that
+// IM into sample_im.
+int32_t ccr_oracle_nested_im(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                             const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t set_index,
+                             int32_t inner_paths, double quantile, double mpor, int32_t threads, double* discounted,
+                             double* mean, double* band05, double* band50, double* band95, int32_t capacity,
+                             int32_t* num_dates, int64_t sample_path, int32_t sample_rep, double* samples,
+                             double* sample_im, char* err, int32_t err_capacity) {
+  return guard(err, err_capacity, [&] {
+    if (!generator || !config || !sets || set_index < 0) invalid("null argument");
+    if (inner_paths < 2 || inner_paths > 1024) invalid("nested initial margin: 2 to 1024 inner paths");
+    if (static_cast<double>(inner_paths - 1) - std::floor(quantile * (inner_paths - 1)) >= 15.0)
+      invalid("nested initial margin: the quantile needs more inner paths");
+    const Generator g(*generator);
+    const Grid base(Vec(base_grid, base_grid + std::max(0, base_size)));
+    const NestedResult r = nestedIm(g, *config, base, sets[set_index], set_index, inner_paths, quantile, mpor, threads,
+                                    sample_path, sample_rep);
+    const int32_t n = static_cast<int32_t>(r.mean.size());
+    if (num_dates) *num_dates = n;
+    if (n > capacity) throw Failure(CCR_ERR_OUT_OF_RANGE, "capacity too small");
+    for (int32_t i = 0; i < n; ++i) {
+      if (discounted) discounted[i] = r.discounted[static_cast<std::size_t>(i)];
+      if (mean) mean[i] = r.mean[static_cast<std::size_t>(i)];
+      if (band05) band05[i] = r.b05[static_cast<std::size_t>(i)];
+      if (band50) band50[i] = r.b50[static_cast<std::size_t>(i)];
+      if (band95) band95[i] = r.b95[static_cast<std::size_t>(i)];
+    }
+    if (samples)
+      for (std::size_t i = 0; i < r.samples.size(); ++i) samples[i] = r.samples[i];
+    if (sample_im) *sample_im = r.sampleIm;
   });
 }
 

@@ -83,6 +83,14 @@ int32_t ccr_oracle_solve(const double* a, const double* b, int32_t n, double* x)
 /* Basis of the Bermudan regressions of later runs: -1 / 0 the reference's state basis
  * (bermudan_swaption.cpp:77-93), 1 the swap-rate / annuity basis (total degree <= 3 in the
  * standardised swap rate and annuity of the tail swap; no reference counterpart). */
+/* Nested conditional initial margin of netting set set_index (per reporting date: E[D IM], E[IM],
+ * 5 / 50 / 95% quantiles); sample_path >= 0: the inner samples of (sample_path, sample_rep). */
+int32_t ccr_oracle_nested_im(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                             const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t set_index,
+                             int32_t inner_paths, double quantile, double mpor, int32_t threads, double* discounted,
+                             double* mean, double* band05, double* band50, double* band95, int32_t capacity,
+                             int32_t* num_dates, int64_t sample_path, int32_t sample_rep, double* samples,
+                             double* sample_im, char* err, int32_t err_capacity);
 int32_t ccr_oracle_set_regression_basis(int32_t basis);
 /* Swap rate (P(t,T_0) - P(t,T_n)) / A and annuity A = sum_i tau_i P(t,T_i) of a schedule at (t, x). */
 int32_t ccr_oracle_swap_rate_annuity(const ccr_hull_white* hw, const double* sched, int32_t n, double t, double x,

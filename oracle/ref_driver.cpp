@@ -6,7 +6,8 @@
 // It builds reference objects from the C structs of include/ccr_hip.h and calls
 // ccr::ExposureEngine::run (src/exposure/exposure_engine.cpp:50-81), ccr::pathwiseCva
 // (src/cva/cva.cpp:55-77), ccr::initialMarginProfile and ccr::xva (src/xva/xva.cpp:59-96) and
-// ccr::BermudanSwaption::valueWithExercise. Only reference-
+// ccr::BermudanSwaption::valueWithExercise, and the fused pipeline's CPU mirror
+// (gpu::compile / runFusedReference / summarise) for its nested conditional initial margin. Only reference-
 // representable inputs are accepted: no foreign rates; a two-leg swap is expressed as
 // IRS(fixed schedule, K) + IRS(float schedule, 0) - IRS(fixed schedule, 0), all reference swaps.
 
@@ -25,6 +26,7 @@
 #include "ccr/cva/cva.hpp"
 #include "ccr/exposure/allocation.hpp"
 #include "ccr/exposure/exposure_engine.hpp"
+#include "ccr/gpu/fused_exposure.hpp"
 #include "ccr/instruments/asset_forward.hpp"
 #include "ccr/instruments/bermudan_swaption.hpp"
 #include "ccr/instruments/european_option.hpp"
@@ -282,6 +284,37 @@ int32_t ccr_ref_run_exposure(const ccr_scenario_generator* generator, const ccr_
         *pathwise_dva =
             ccr::pathwiseCva(r.scenarios, neg, static_cast<std::size_t>(set->own_credit), 0.0, r.reportingIndices);
       }
+    }
+    return CCR_OK;
+  } catch (const std::exception& e) {
+    return fail(err, err_capacity, e.what(), CCR_ERR_RUNTIME);
+  }
+}
+
+// The nested conditional initial margin of the reference's fused pipeline on its CPU mirror
+// (gpu::FusedOptions::imInnerPaths, src/gpu/fused_reference.cpp:371-402): per reporting date
+// FusedResult::conditionalImDiscounted and conditionalImMean (f32 kernels, its own counter-based
+// draws).
+int32_t ccr_ref_fused_nested_im(const ccr_scenario_generator* generator, const ccr_simulation_config* config,
+                                const double* base_grid, int32_t base_size, const ccr_netting_set* set,
+                                int32_t inner_paths, double quantile, double mpor, double* discounted, double* mean,
+                                int32_t capacity, int32_t* num_dates, char* err, int32_t err_capacity) {
+  try {
+    const ccr::ScenarioGenerator gen = generatorOf(*generator);
+    const ccr::SimulationConfig cfg{static_cast<std::size_t>(config->num_paths), config->seed, config->antithetic != 0};
+    const ccr::ExposureEngine engine(gen, cfg, ccr::TimeGrid(std::vector<double>(base_grid, base_grid + base_size)));
+    ccr::gpu::FusedOptions o;
+    o.imInnerPaths = static_cast<std::size_t>(inner_paths);
+    o.imQuantile = quantile;
+    o.imMarginPeriod = mpor;
+    const ccr::gpu::FusedPlan plan = ccr::gpu::compile(engine, nettingSetOf(*set), o);
+    const ccr::gpu::FusedResult r = ccr::gpu::summarise(plan, ccr::gpu::runFusedReference(plan));
+    const std::size_t n = r.conditionalImMean.size();
+    if (num_dates) *num_dates = static_cast<int32_t>(n);
+    if (static_cast<int32_t>(n) > capacity) return fail(err, err_capacity, "capacity", CCR_ERR_INVALID_ARGUMENT);
+    for (std::size_t k = 0; k < n; ++k) {
+      if (discounted) discounted[k] = r.conditionalImDiscounted[k];
+      if (mean) mean[k] = r.conditionalImMean[k];
     }
     return CCR_OK;
   } catch (const std::exception& e) {

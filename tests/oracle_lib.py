@@ -49,6 +49,9 @@ def mutant_quanto():
 def _bind(lib):
     if True:
         D, I32, I64, P = C.c_double, C.c_int32, C.c_int64, C.c_void_p
+        lib.ccr_oracle_nested_im.restype = I32
+        lib.ccr_oracle_nested_im.argtypes = [P, P, P, I32, P, I32, I32, D, D, I32, P, P, P, P, P, I32, P, I64, I32, P, P,
+                                             C.c_char_p, I32]
         lib.ccr_oracle_run_exposure_replay.restype = I32
         lib.ccr_oracle_run_exposure_replay.argtypes = [P, P, P, I32, P, I32, D, I32, P, I64, P, C.c_char_p, I32]
         lib.ccr_oracle_scenarios.restype = I32
@@ -132,6 +135,9 @@ def ref():
             lib.ccr_ref_trade_statistics.restype = I32
             lib.ccr_ref_trade_statistics.argtypes = [P, P, P, I32, P, P, I32, C.c_char_p, I32]
             lib.ccr_ref_xva.argtypes = [P, P, P, I32, P, P, D, P, D, P, P]
+        if hasattr(lib, "ccr_ref_fused_nested_im"):
+            lib.ccr_ref_fused_nested_im.restype = I32
+            lib.ccr_ref_fused_nested_im.argtypes = [P, P, P, I32, P, I32, D, D, P, P, I32, P, C.c_char_p, I32]
         _ref = lib
     return _ref
 
@@ -268,6 +274,46 @@ def book_grid_size(spec: abi.Spec) -> int:
                                            abi.dptr(out), cap, C.byref(n))
         t.update(out[:n.value].tolist())
     return len(t) + 8  # capacity with room; the union de-duplicates within 1e-9
+
+
+def oracle_nested_im(spec: abi.Spec, i: int, inner: int, quantile: float, mpor: float = 10.0 / 250.0,
+                     threads: int = 8, sample=None, lib=None):
+    """Nested conditional initial margin of netting set i: dict of per-reporting-date arrays
+    (discounted E[D IM], mean E[IM], band05 / band50 / band95); sample=(path, reporting index):
+    also that pair's inner samples and IM ("samples", "sample_im")."""
+    cap = grid_size(spec, i)
+    arrs = {k: np.zeros(cap) for k in ("discounted", "mean", "band05", "band50", "band95")}
+    n = C.c_int32()
+    err = C.create_string_buffer(512)
+    samples = np.zeros(max(inner, 1))
+    sim = C.c_double()
+    sp, sr = sample if sample is not None else (-1, -1)
+    rc = (lib or oracle()).ccr_oracle_nested_im(
+        C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid), len(spec.base_grid), spec.sets_c, i,
+        inner, quantile, mpor, threads, *(abi.dptr(arrs[k]) for k in ("discounted", "mean", "band05", "band50", "band95")),
+        cap, C.byref(n), sp, sr, abi.dptr(samples), C.byref(sim), err, 512)
+    if rc != 0:
+        raise RuntimeError(f"oracle failed ({rc}): {err.value.decode()}")
+    out = {k: v[:n.value].copy() for k, v in arrs.items()}
+    if sample is not None:
+        out["samples"] = samples[:inner].copy()
+        out["sample_im"] = sim.value
+    return out
+
+
+def ref_fused_nested_im(spec: abi.Spec, i: int, inner: int, quantile: float, mpor: float = 10.0 / 250.0):
+    """The reference fused pipeline's nested conditional initial margin (its CPU mirror, f32, its
+    own draws): (E[D IM], E[IM]) per reporting date."""
+    cap = grid_size(spec, i) + 8
+    d, m = np.zeros(cap), np.zeros(cap)
+    n = C.c_int32()
+    err = C.create_string_buffer(512)
+    rc = ref().ccr_ref_fused_nested_im(C.byref(spec.generator), C.byref(spec.config), abi.dptr(spec.base_grid),
+                                       len(spec.base_grid), C.byref(spec.sets_c[i]), inner, quantile, mpor,
+                                       abi.dptr(d), abi.dptr(m), cap, C.byref(n), err, 512)
+    if rc != 0:
+        raise RuntimeError(f"reference failed ({rc}): {err.value.decode()}")
+    return d[:n.value].copy(), m[:n.value].copy()
 
 
 def run_oracle_book(spec: abi.Spec, threads: int = 4, rows: bool = False):

@@ -163,6 +163,11 @@ class SurrogateOptionsC(C.Structure):
                 ("gate_threshold", C.c_double), ("batch", C.c_int32)]
 
 
+class NestedImOutC(C.Structure):
+    _fields_ = [("capacity", C.c_int32), ("num_dates", C.c_int32)] + [
+        (k, C.POINTER(C.c_double)) for k in ("discounted", "mean", "band05", "band50", "band95")]
+
+
 class SurrogateReportC(C.Structure):
     _fields_ = [("candidate_sets", C.c_int32), ("enabled_sets", C.c_int32), ("samples", C.c_int64),
                 ("train_ms", C.c_double), ("loss", C.c_double), ("max_ee_error", C.c_double),

@@ -61,6 +61,9 @@ struct ccr_hip_ctx {
   double im_quantile = 0.0, im_mpor = 10.0 / 250.0;
   // Per-trade statistics for plans compiled while set.
   bool per_trade = false;
+  // Nested conditional initial margin for plans compiled while nested_inner > 0.
+  int nested_inner = 0;
+  double nested_q = 0.99, nested_mpor = 10.0 / 250.0;
   // Book mode (common grids, pathwise identical scenarios) for plans compiled while set, and the
   // grids of the whole book when it is compiled as several plans (empty: each plan's own union).
   bool book = false;
@@ -146,6 +149,17 @@ struct ccr_hip_plan {
   double* d_tstats = nullptr;
   std::vector<double> tstats;
   bool tstats_ready = false;
+  // Nested conditional initial margin (null when off): per set date the target date, IM rows and
+  // per-wave partials (laid out as the exposure rows and statistics), their reductions
+  // [set][R_max][kStats] (slots 0: E[D IM], 1: E[IM]) and the 5 / 50 / 95% quantiles [3][set][R_max].
+  int32_t* d_ntarget = nullptr;
+  void* d_nrows = nullptr;
+  double* d_npart = nullptr;
+  double* d_nstats = nullptr;
+  double* d_nbands = nullptr;
+  int32_t nested_inner = 0;
+  double nested_q = 0.0;
+  uint64_t seed = 0;
   // The plan's execute captured as a HIP graph (graph mode), with the workspace it was bound to.
   hipGraphExec_t graph_exec = nullptr;
   void* graph_ws = nullptr;
@@ -370,7 +384,7 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   plan->host = buildPlan(*generator, *config, base_grid, base_size, sets, num_sets, pfe_quantile, precision,
                          ctx->block > 0 ? ctx->block : kMaxBlock, ctx->per_trade, ctx->book,
                          ctx->im_quantile > 0.0, bookGrids ? &ctx->book_rep : nullptr,
-                         bookGrids ? &ctx->book_sim : nullptr, ctx->basis);
+                         bookGrids ? &ctx->book_sim : nullptr, ctx->basis, ctx->nested_inner, ctx->nested_mpor);
   HostPlan& h = plan->host;
   if (ctx->book && ctx->book_group > 1 && !ctx->per_trade && ctx->im_quantile <= 0.0 && h.opts.empty() &&
       h.sets.size() > 1)
@@ -393,6 +407,17 @@ void compilePlan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, cons
   const int64_t P = h.g.num_paths;
   execAlloc(ctx, plan.get(), plan->d_rows, static_cast<size_t>(num_sets) * h.max_rep * P * elem);
   execAlloc(ctx, plan.get(), plan->d_partials, static_cast<size_t>(h.total_waves) * h.max_rep * kStats * sizeof(double));
+  plan->seed = config->seed;
+  if (h.nested_inner > 0) {
+    plan->nested_inner = h.nested_inner;
+    plan->nested_q = ctx->nested_q;
+    plan->d_ntarget = upload(ctx, plan.get(), h.nested_target.data(), h.nested_target.size());
+    execAlloc(ctx, plan.get(), plan->d_nrows, static_cast<size_t>(num_sets) * h.max_rep * P * elem);
+    execAlloc(ctx, plan.get(), plan->d_npart, static_cast<size_t>(h.total_waves) * h.max_rep * kStats * sizeof(double));
+    plan->d_nstats = static_cast<double*>(
+        devAlloc(ctx, plan.get(), static_cast<size_t>(num_sets) * h.max_rep * kStats * sizeof(double)));
+    plan->d_nbands = static_cast<double*>(devAlloc(ctx, plan.get(), static_cast<size_t>(3) * num_sets * h.max_rep * sizeof(double)));
+  }
   if (ctx->im_quantile > 0.0) {
     // The proxy needs the netted value on every simulation date (the plan values every date
     // when it is on: book plans simulate dates of other sets that their own valuation skips).
@@ -565,6 +590,9 @@ PathArgs<Real> pathArgs(const ccr_hip_plan* plan) {
   a.state_len = h.state_len;
   return a;
 }
+
+// Levels of the nested initial margin's bands (FusedOptions::bandLevels, fused_exposure.hpp:323).
+constexpr double kNestedBands[3] = {0.05, 0.5, 0.95};
 
 int setRows(const HostPlan& h, int s) {
   int rows = 0;
@@ -863,6 +891,32 @@ void launchAll(ccr_hip_ctx* ctx, ccr_hip_plan* plan, const ccr_hip_surrogate* su
           "initial-margin reduce launch");
     plan->im_ready = false;
   }
+  if (plan->d_ntarget) {
+    NestedArgs n{};
+    n.target = plan->d_ntarget;
+    n.inner = plan->nested_inner;
+    n.level = plan->nested_q;
+    n.seed = plan->seed;
+    n.rows_raw = plan->d_nrows;
+    n.partials = plan->d_npart;
+    const int64_t per = static_cast<int64_t>(nsets) * h.max_rep;
+    if (h.precision == CCR_FP32) {
+      check(launchNested<float>(pathArgs<float>(plan), n, blocks, h.block, ctx->stream), "nested initial margin launch");
+      for (int k = 0; k < 3; ++k)
+        check(launchPfe<float>(plan->d_sets, static_cast<const float*>(plan->d_nrows), nsets, h.max_rep, h.g.num_paths,
+                               kNestedBands[k], plan->d_nbands + k * per, ctx->stream),
+              "nested initial margin select launch");
+    } else {
+      check(launchNested<double>(pathArgs<double>(plan), n, blocks, h.block, ctx->stream), "nested initial margin launch");
+      for (int k = 0; k < 3; ++k)
+        check(launchPfe<double>(plan->d_sets, static_cast<const double*>(plan->d_nrows), nsets, h.max_rep,
+                                h.g.num_paths, kNestedBands[k], plan->d_nbands + k * per, ctx->stream),
+              "nested initial margin select launch");
+    }
+    check(launchReduce(plan->d_sets, plan->d_npart, nsets, h.max_rep, h.total_waves / nsets,
+                       1.0 / static_cast<double>(h.g.num_paths), plan->d_nstats, ctx->stream),
+          "nested initial margin reduce launch");
+  }
   markEvent(ctx, 4);
   ctx->timing_pending = !ctx->capturing;
   ctx->stage_events = true;
@@ -1124,6 +1178,56 @@ void ccr_hip_destroy(ccr_hip_ctx* ctx) {
 }
 
 const char* ccr_hip_last_error(const ccr_hip_ctx* ctx) { return ctx ? ctx->err.c_str() : gCreateError.c_str(); }
+
+int32_t ccr_hip_set_nested_initial_margin(ccr_hip_ctx* ctx, int32_t inner_paths, double quantile,
+                                          double margin_period_of_risk) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (inner_paths == 0) {
+      ctx->nested_inner = 0;
+      return;
+    }
+    // The checks of gpu::compile (src/gpu/fused_exposure.cpp:220-246).
+    if (inner_paths < 2 || inner_paths > 1024) invalid("nested initial margin: 2 to 1024 inner paths");
+    if (!(quantile >= 0.0 && quantile <= 1.0)) invalid("imQuantile must be in [0, 1]");
+    const double rank = quantile * static_cast<double>(inner_paths - 1);
+    if (static_cast<double>(inner_paths - 1) - std::floor(rank) >= static_cast<double>(kNestedTop - 1))
+      invalid("nested initial margin: the quantile needs more inner paths (keeps the 16 largest)");
+    if (!(margin_period_of_risk > 0.0)) invalid("nested initial margin: the margin period must be positive");
+    ctx->nested_inner = inner_paths;
+    ctx->nested_q = quantile;
+    ctx->nested_mpor = margin_period_of_risk;
+  });
+}
+
+int32_t ccr_hip_nested_initial_margin(ccr_hip_ctx* ctx, const ccr_hip_plan* plan, int32_t set, ccr_nested_im_out* out) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (!plan || !out) invalid("null argument");
+    if (!plan->d_ntarget) invalid("nested initial margin: the plan was compiled without ccr_hip_set_nested_initial_margin");
+    const HostPlan& h = plan->host;
+    const int nsets = static_cast<int>(h.sets.size());
+    if (set < 0 || set >= nsets) fail(CCR_ERR_OUT_OF_RANGE, "netting set index out of range");
+    const int R = h.sets[static_cast<size_t>(set)].num_rep;
+    if (out->capacity < R) fail(CCR_ERR_OUT_OF_RANGE, "nested initial margin: output capacity too small");
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t per = static_cast<size_t>(nsets) * h.max_rep;
+    std::vector<double> st(per * kStats), bands(3 * per);
+    check(hipMemcpyAsync(st.data(), plan->d_nstats, st.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+          "hipMemcpyAsync");
+    check(hipMemcpyAsync(bands.data(), plan->d_nbands, bands.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+          "hipMemcpyAsync");
+    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    double* dst[5] = {out->discounted, out->mean, out->band05, out->band50, out->band95};
+    for (int r = 0; r < R; ++r) {
+      const size_t i = static_cast<size_t>(set) * h.max_rep + static_cast<size_t>(r);
+      const double v[5] = {st[i * kStats], st[i * kStats + 1], bands[i], bands[per + i], bands[2 * per + i]};
+      for (int k = 0; k < 5; ++k)
+        if (dst[k]) dst[k][r] = v[k];
+    }
+    out->num_dates = R;
+  });
+}
 
 int32_t ccr_hip_set_initial_margin(ccr_hip_ctx* ctx, double quantile, double margin_period_of_risk) {
   if (!ctx) return CCR_ERR_INVALID_ARGUMENT;

@@ -161,6 +161,129 @@ ccr_group_kernel(PathArgs<Real> a) {
   });
 }
 
+// Nested conditional initial margin (src/gpu/fused_kernels.cpp:445-479), one outer path per lane:
+// the path's market and fixings as in ccr_path_kernel (replayed normals), and on every reporting
+// date t with a target t + mpor, inner paths from the path's state over (t, t + mpor] --
+// innerStep on the inner generator's normals, the fixings of the window's dates into the path's
+// fixing rows (those rows saved before and restored after every inner path) -- valued at the
+// target with the same valuation records. IM(t) is the interpolated order statistic of
+// |V_inner(t + mpor) - V(t)| at the plan's level, read from the 16 largest samples (kept in
+// registers, sorted by insertion). Writes IM per (reporting date, path) into rows laid out as the
+// exposure rows and, per wavefront, the sums of D IM and IM into partials laid out as the
+// statistics' (slots 0 and 1).
+template <typename Real, typename Sh, bool kOptions>
+__global__ void __launch_bounds__(kMaxBlock) ccr_nested_im_kernel(PathArgs<Real> a, NestedArgs n) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const CCR_CONST GlobalDev& g = *toConst(a.g);
+  const int b = xcdRemap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  const int set = toConst(a.block_set)[b];
+  const CCR_CONST SetDev& S = toConst(a.sets)[set];
+  const int bd = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
+  const int lb = b - S.block_begin;
+  const int p = lb * bd + tid;
+  const bool active = p < g.num_paths;
+  const int pe = active ? p : g.num_paths - 1;  // inactive lanes replay a valid path, contribute nothing
+  const int waveLocal = lb * (bd >> 6) + (tid >> 6);
+  double* part = n.partials + S.partial_off + static_cast<int64_t>(waveLocal) * a.max_rep * kStats;
+  Real* rows = static_cast<Real*>(n.rows_raw) + S.rows_off;
+  Real* xs = reinterpret_cast<Real*>(lds_raw) + tid;
+  const int stride = bd;
+  const int64_t q = g.antithetic ? (pe >> 1) : pe;
+  const Real* zbase = a.stream + q * S.stream_stride;
+  cptr<Real> steps = toConst(a.steps) + static_cast<int64_t>(S.date_off) * kStepStride;
+  cptr<DateRec> dates = toConst(a.dates) + S.date_off;
+  cptr<Real> dreal = toConst(a.date_reals) + static_cast<int64_t>(S.date_off) * kDateRealStride;
+  cptr<Real> bonds = toConst(a.bonds) + 2 * static_cast<int64_t>(S.bond_off);
+  cptr<Real> fixed = toConst(a.fixed) + 4 * static_cast<int64_t>(S.fixed_off);
+  cptr<int32_t> fstore = toConst(a.fixstore) + 2 * S.fixstore_off;
+  cptr<Real> poly = toConst(a.poly) + S.poly_off;
+  cptr<Real> opts = kOptions ? toConst(a.opts) + kOptStride * static_cast<int64_t>(S.opt_off) : nullptr;
+  cptr<int32_t> target = toConst(n.target) + S.date_off;
+  const Real sgn = (g.antithetic && (pe & 1)) ? Real(-1) : Real(1);
+  const uint64_t pathKey = nestedPathKey(n.seed, set, pe);
+  const double rank = n.level * static_cast<double>(n.inner - 1);
+  const int kq = static_cast<int>(floor(rank));
+  const int hiIdx = n.inner - 1 - kq < kNestedTop - 1 ? n.inner - 1 - kq : kNestedTop - 1;
+  Market<Real, Sh> mk;
+  marketInit<Real, Sh, false>(mk, g, zbase, S.num_dates);
+  for (int j = 0; j < S.num_dates; ++j) {
+    if (j > 0) marketStep<Real, Sh, false>(mk, g, j, S.num_dates, steps, zbase, sgn);
+    const CCR_CONST DateRec& rec = dates[j];
+    storeFixings(mk, rec.fixstore_beg, rec.fixstore_end, fstore, xs, stride);
+    if (!(rec.flags & kReporting)) continue;
+    const int tgt = target[j];
+    double im = 0.0;
+    if (tgt >= 0) {
+      const Real V = valueRecord<Real, Sh, kOptions>(mk, g, rec, dreal + static_cast<int64_t>(j) * kDateRealStride, poly,
+                                                     bonds, fixed, opts, xs, stride);
+      // The fixing rows the window's dates write: saved, restored after every inner path.
+      const int wb = dates[j + 1].fixstore_beg, nsave = dates[tgt].fixstore_end - wb;
+      Real saved[kNestedMaxSave];
+#pragma unroll
+      for (int k = 0; k < kNestedMaxSave; ++k)
+        if (k < nsave) saved[k] = xs[fstore[2 * (wb + k) + 1] * stride];
+      double top[kNestedTop];
+#pragma unroll
+      for (int k = 0; k < kNestedTop; ++k) top[k] = 0.0;
+      for (int i = 0; i < n.inner; ++i) {
+        Market<Real, Sh> mi = mk;
+        const uint64_t key = nestedPairKey(pathKey, j, i >> 1);
+        const double isg = (i & 1) ? -1.0 : 1.0;
+        for (int jj = j + 1; jj <= tgt; ++jj) {
+          innerStep<Real, Sh>(mi, g, jj, steps, key, isg);
+          const CCR_CONST DateRec& rj = dates[jj];
+          storeFixings(mi, rj.fixstore_beg, rj.fixstore_end, fstore, xs, stride);
+        }
+        const Real vi = valueRecord<Real, Sh, kOptions>(mi, g, dates[tgt], dreal + static_cast<int64_t>(tgt) * kDateRealStride,
+                                                        poly, bonds, fixed, opts, xs, stride);
+#pragma unroll
+        for (int k = 0; k < kNestedMaxSave; ++k)
+          if (k < nsave) xs[fstore[2 * (wb + k) + 1] * stride] = saved[k];
+        double cur = fabs(static_cast<double>(vi) - static_cast<double>(V));
+#pragma unroll
+        for (int k = 0; k < kNestedTop; ++k)
+          if (cur > top[k]) {
+            const double t = top[k];
+            top[k] = cur;
+            cur = t;
+          }
+      }
+      double ok = 0.0, ok1 = 0.0;
+#pragma unroll
+      for (int k = 0; k < kNestedTop; ++k) {
+        if (k == hiIdx) ok = top[k];
+        if (k == hiIdx - 1) ok1 = top[k];
+      }
+      if (hiIdx == 0) ok1 = ok;
+      im = ok + (rank - static_cast<double>(kq)) * (ok1 - ok);
+    }
+    const int rr = rec.rep;
+    if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(im);
+    const double d = static_cast<double>(mk.D);
+    const double sDim = waveSum(active ? d * im : 0.0), sIm = waveSum(active ? im : 0.0);
+    const int lane = tid & 63;
+    if (lane < 2) part[rr * kStats + lane] = lane == 0 ? sDim : sIm;
+  }
+}
+
+template <typename Real>
+hipError_t launchNested(const PathArgs<Real>& args, const NestedArgs& n, int blocks, int block, hipStream_t stream) {
+  const size_t lds = static_cast<size_t>(args.slots > 0 ? args.slots : 1) * block * sizeof(Real);
+  return withShape(args.nrates, args.nassets, args.ncredits, [&](auto shape) -> hipError_t {
+    using Sh = decltype(shape);
+    if constexpr (Sh::NA > 0) {
+      if (args.has_options) {
+        hipLaunchKernelGGL((ccr_nested_im_kernel<Real, Sh, true>), dim3(blocks), dim3(block), lds, stream, args, n);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((ccr_nested_im_kernel<Real, Sh, false>), dim3(blocks), dim3(block), lds, stream, args, n);
+    return hipGetLastError();
+  });
+}
+template hipError_t launchNested<double>(const PathArgs<double>&, const NestedArgs&, int, int, hipStream_t);
+template hipError_t launchNested<float>(const PathArgs<float>&, const NestedArgs&, int, int, hipStream_t);
+
 bool groupSupported(int group) {
   for (int gsz : kGroupSizes)
     if (gsz == group) return true;

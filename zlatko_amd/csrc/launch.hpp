@@ -151,6 +151,21 @@ hipError_t launchSurrogateFeatures(const double* states, int64_t state_off, int6
 // out[0, n) = 0.
 hipError_t launchZeroRows(double* out, int64_t n, hipStream_t stream);
 
+// Nested conditional initial margin (ccr_nested_im_kernel): per set date [date_off + j] the date
+// of t_j + mpor (-1: none), inner paths, the quantile level, the simulation seed; IM rows laid out
+// as the exposure rows and per-wave partials laid out as the statistics' (slots 0: D IM, 1: IM).
+constexpr int kNestedTop = 16;  // largest inner samples kept per (path, date)
+struct NestedArgs {
+  const int32_t* target;
+  int32_t inner;
+  double level;
+  uint64_t seed;
+  void* rows_raw;  // IM rows in the plan's precision
+  double* partials;
+};
+template <typename Real>
+hipError_t launchNested(const PathArgs<Real>& args, const NestedArgs& n, int blocks, int block, hipStream_t stream);
+
 template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream);
 // out[i] = (float)in[i], i < n (round to nearest even, as a host static_cast).

@@ -23,6 +23,9 @@ constexpr int kMaxRates = 4;      // domestic + up to 3 foreign Hull-White rates
 constexpr int kMaxAssets = 4;     // lognormal FX / equity factors
 constexpr int kMaxCredits = 2;    // CIR++ intensities (counterparty, own)
 constexpr int kMaxCorr = kMaxRates + kMaxAssets + kMaxCredits;
+// Fixing states a nested initial-margin window (t, t + mpor] may overwrite (the nested kernel
+// saves and restores them around every inner path), at most.
+constexpr int kNestedMaxSave = 16;
 constexpr int kMaxSub = 8;        // CIR++ sub-steps per grid step
 constexpr int kStats = 9;         // E[V-C], EE, ENE, EE*, ENE*, E[D], E[Q], CVA incr., DVA incr.
 constexpr int kMaxBlock = 256;    // launch bound of the path kernel (threads per workgroup)

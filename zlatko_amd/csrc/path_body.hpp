@@ -419,6 +419,109 @@ __host__ __device__ __forceinline__ void marketStep(Market<Real, Sh>& mk, const 
   }
 }
 
+// ---------------------------------------------------------------- nested initial margin
+// Inner paths of the nested conditional initial margin draw their normals from a counter-based
+// generator: SplitMix64's finaliser (Steele, Lea and Flood, 2014) as the mixing function, keys
+// chained over (seed, netting set, outer path, reporting date, antithetic pair), and normal k of
+// inner step j (k the index in a step's normal layout: correlated drivers, then the domestic
+// integral normal, then the foreign ones) by the Box-Muller cosine branch of two uniforms from
+// mix64(c) and mix64(c ^ constant), c = key ^ (j << 16 | k).
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t nestedPathKey(uint64_t seed, int set, int path) {
+  return mix64(mix64(mix64(seed) ^ static_cast<uint64_t>(set)) ^ static_cast<uint64_t>(path));
+}
+__host__ __device__ inline uint64_t nestedPairKey(uint64_t pathKey, int date, int pair) {
+  return mix64(mix64(pathKey ^ (static_cast<uint64_t>(date) << 32)) ^ static_cast<uint64_t>(pair));
+}
+__host__ __device__ inline double nestedNormal(uint64_t key, int step, int k) {
+  const uint64_t c = key ^ ((static_cast<uint64_t>(step) << 16) | static_cast<uint64_t>(k));
+  const uint64_t a = mix64(c), b = mix64(c ^ 0xD1B54A32D192ED03ull);
+  const double u1 = (static_cast<double>(a >> 11) + 0.5) * 0x1.0p-53;
+  const double u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// One inner step t_{j-1} -> t_j of the rates and assets (the credit factors do not enter a
+// netting set's value and are not stepped): marketStep's arithmetic on normals from the inner
+// generator (key, sign: the antithetic partner flips every normal).
+template <typename Real, typename Sh>
+__host__ __device__ __forceinline__ void innerStep(Market<Real, Sh>& mk, const CCR_CONST GlobalDev& g, int j,
+                                                   cptr<Real> steps, uint64_t key, double sign) {
+  constexpr bool kDouble = std::is_same<Real, double>::value;
+  constexpr int NR = Sh::NR, NA = Sh::NA;
+  const int nR = Sh::kExact ? NR : g.nrates, nA = Sh::kExact ? NA : g.nassets;
+  const int nCorr = Sh::kExact ? Sh::kCorr : g.ncorr;
+  cptr<Real> st = steps + static_cast<int64_t>(j) * kStepStride;
+  Real zc[NR + NA > 0 ? NR + NA : 1];
+#pragma unroll
+  for (int k = 0; k < NR + NA; ++k) {
+    const bool used = k < NR ? k < nR : k - NR < nA;
+    const int idx = k < NR ? k : nR + (k - NR);  // compact driver index
+    zc[k] = used ? static_cast<Real>(sign * nestedNormal(key, j, idx)) : Real(0);
+  }
+  Real z2[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+    z2[r] = r < nR ? static_cast<Real>(sign * nestedNormal(key, j, r == 0 ? nCorr : g.foreign_off + r - 1)) : Real(0);
+  // Correlated drivers of the rates and assets (rows of the factor with columns k <= lim, compact).
+  Real w[NR + NA > 0 ? NR + NA : 1];
+  cptr<double> chol = g.chol;
+#pragma unroll
+  for (int i = 0; i < NR + NA; ++i) {
+    const int slot = i < NR ? i : kMaxRates + (i - NR);
+    const int lim = i < NR ? i : nR + (i - NR);
+    const bool used = i < NR ? i < nR : i - NR < nA;
+    Real sum = 0;
+    if (used) {
+#pragma unroll
+      for (int k = 0; k < NR + NA; ++k) {
+        const int ck = k < NR ? k : nR + (k - NR);  // compact index of slot k
+        const bool kused = k < NR ? k < nR : k - NR < nA;
+        if (kused && ck <= lim) sum += static_cast<Real>(chol[slot * kMaxCorr + ck]) * zc[k];
+      }
+    }
+    w[i] = sum;
+  }
+  Real ir[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r < nR) {
+      cptr<Real> o = st + kStepRate + 8 * r;
+      const Real x0 = mk.x[r];
+      mk.x[r] = x0 * o[0] + o[1] * w[r] + o[5];
+      const Real ix = x0 * o[4] + o[2] * w[r] + o[3] * z2[r] + o[6];
+      ir[r] = ix + o[7];
+    } else {
+      ir[r] = 0;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    if (i < nA) {
+      cptr<Real> o = st + kStepAsset + 3 * i;
+      const int rr = g.asset_rate[i];
+      Real icarry = o[0];
+#pragma unroll
+      for (int r = 1; r < NR; ++r)
+        if (r == rr) icarry = ir[r];
+      const Real drift = ir[0] - icarry - o[2] + o[1] * w[NR + i];
+      if constexpr (kDouble) {
+        mk.S[i] = mk.S[i] * exp(drift);
+      } else {
+        const Real yk = drift - mk.lnSc[i];
+        const Real tk = mk.lnS[i] + yk;
+        mk.lnSc[i] = (tk - mk.lnS[i]) - yk;
+        mk.lnS[i] = tk;
+        mk.S[i] = static_cast<Real>(g.spot[i]) * expr(mk.lnS[i]);
+      }
+    }
+  }
+}
+
 // Fixing states recorded on date j: x of each listed rate into its slot row.
 // One predicated store per rate with a constant index into the state: a select among the rates'
 // states would be folded into a dynamically indexed load, which keeps the whole market state in

@@ -466,8 +466,12 @@ void buildTiles(HostPlan& plan, const std::vector<int>& group, const std::vector
 HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_config& cfg, const double* base_grid,
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
                    int32_t precision, int32_t block, bool per_trade, bool book, bool value_all,
-                   const std::vector<double>* book_rep, const std::vector<double>* book_sim, int32_t basis) {
+                   const std::vector<double>* book_rep, const std::vector<double>* book_sim, int32_t basis,
+                   int32_t nested_inner, double nested_mpor) {
   if (!sets || num_sets <= 0) invalid("no netting sets");
+  if (nested_inner > 0 && (book || per_trade))
+    fail(CCR_ERR_UNSUPPORTED, "nested initial margin: not with book or per-trade plans");
+  if (nested_inner > 0 && !(nested_mpor > 0.0)) invalid("nested initial margin: the margin period must be positive");
   if (cfg.num_paths <= 0) invalid("ScenarioGenerator::generate: numPaths must be positive");
   if (cfg.num_paths > (int64_t(1) << 31) - 1) fail(CCR_ERR_UNSUPPORTED, "at most 2^31 - 1 paths");
   if (pfe_quantile < 0.0 || pfe_quantile > 1.0) invalid("quantile: level must be in [0, 1]");
@@ -476,6 +480,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
   const double polyTol = precision == CCR_FP32 ? kPolyToleranceFp32 : kPolyTolerance;
   const TimeGrid base(std::vector<double>(base_grid, base_grid + std::max(0, base_size)));
   HostPlan plan;
+  plan.nested_inner = nested_inner > 0 ? nested_inner : 0;
   plan.basis = basis;
   plan.precision = precision;
   plan.pfe_q = pfe_quantile;
@@ -565,7 +570,16 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       (t.kind == CCR_TRADE_BERMUDAN_SWAPTION ? berms : trades).push_back(std::move(t));
     }
     const TimeGrid rep = book ? bookRep : reportingGrid(base, ns);
-    const TimeGrid sim = book ? bookSim : simulationGrid(base, ns);
+    TimeGrid sim = book ? bookSim : simulationGrid(base, ns);
+    if (nested_inner > 0) {
+      // The inner paths run from every reporting date t to t + mpor, so t + mpor joins the
+      // simulation grid up to its horizon (src/gpu/fused_exposure.cpp:199-212).
+      if (!berms.empty()) fail(CCR_ERR_UNSUPPORTED, "nested initial margin: Bermudan swaptions are not supported");
+      std::vector<double> ahead;
+      for (double t : rep.t)
+        if (t + nested_mpor <= sim.t.back() + 1e-10) ahead.push_back(t + nested_mpor);
+      sim = sim.merged(ahead);
+    }
     const int T = static_cast<int>(sim.size()), R = static_cast<int>(rep.size());
     plan.rep_times.push_back(rep.t);
     plan.sim_times.push_back(sim.t);
@@ -646,9 +660,23 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     std::vector<Collector> cols(static_cast<std::size_t>(T));
     std::vector<char> valued(static_cast<std::size_t>(T), 0);
     std::map<std::pair<int, int>, int> lastUse;  // (rate, fixing date) -> last date using it
+    // Nested initial margin: the date t + mpor of each reporting date (found within 1e-8, as
+    // fused_exposure.cpp:612-620), valued for the inner paths.
+    std::vector<int32_t> target(static_cast<std::size_t>(T), -1);
+    if (nested_inner > 0)
+      for (int j = 0; j < T; ++j) {
+        if (repOf[static_cast<std::size_t>(j)] < 0) continue;
+        if (const auto k = sim.find(sim[static_cast<std::size_t>(j)] + nested_mpor, 1e-8)) {
+          if (static_cast<int>(*k) <= j) invalid("nested initial margin: the margin period must be positive");
+          target[static_cast<std::size_t>(j)] = static_cast<int32_t>(*k);
+        }
+      }
+    std::vector<char> isTarget(static_cast<std::size_t>(T), 0);
+    for (int32_t k : target)
+      if (k >= 0) isTarget[static_cast<std::size_t>(k)] = 1;
     for (int j = 0; j < T; ++j) {
-      valued[static_cast<std::size_t>(j)] =
-          value_all || repOf[static_cast<std::size_t>(j)] >= 0 || isCall[static_cast<std::size_t>(j)];
+      valued[static_cast<std::size_t>(j)] = value_all || repOf[static_cast<std::size_t>(j)] >= 0 ||
+                                            isCall[static_cast<std::size_t>(j)] || isTarget[static_cast<std::size_t>(j)];
       if (!valued[static_cast<std::size_t>(j)]) continue;
       Collector& col = cols[static_cast<std::size_t>(j)];
       col.t = sim[static_cast<std::size_t>(j)];
@@ -841,6 +869,7 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
       setTerms += nTerms;
       plan.dates.push_back(rec);
       plan.date_reals.insert(plan.date_reals.end(), reals, reals + kDateRealStride);
+      plan.nested_target.push_back(target[static_cast<std::size_t>(j)]);
 
       // Step coefficients t_{j-1} -> t_j.
       double st[kStepStride] = {0};
@@ -1106,6 +1135,15 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
     }
     plan.revaluations += static_cast<int64_t>(ns.num_trades) * P * R;
     sd.rows_off = static_cast<int64_t>(s) * 0;  // filled below once max_rep is known
+    if (nested_inner > 0)
+      for (int j = 0; j < T; ++j) {
+        const int32_t k = target[static_cast<std::size_t>(j)];
+        if (k < 0) continue;
+        const int save = plan.dates[static_cast<std::size_t>(sd.date_off + k)].fixstore_end -
+                         plan.dates[static_cast<std::size_t>(sd.date_off + j + 1)].fixstore_beg;
+        if (save > kNestedMaxSave)
+          fail(CCR_ERR_UNSUPPORTED, "nested initial margin: more fixings inside a margin period than supported");
+      }
     plan.sets.push_back(sd);
     plan.max_dates = std::max(plan.max_dates, T);
     plan.max_rep = std::max(plan.max_rep, R);

@@ -75,7 +75,13 @@ struct HostPlan {
   std::vector<DateRec> tdates;
   std::vector<double> tdreals;
   int32_t max_groups = 0;
+  // Nested conditional initial margin (nested_inner >= 2): per simulation date of every set
+  // ([date_off + j]), the date index of t_j + mpor for a reporting date j (-1: past the horizon,
+  // or not a reporting date).
+  int32_t nested_inner = 0;
+  std::vector<int32_t> nested_target;
 };
+
 
 // ExposureEngine::reportingGrid / simulationGrid (src/exposure/exposure_engine.cpp:35-48).
 TimeGrid reportingGrid(const TimeGrid& base, const ccr_netting_set& set);
@@ -88,7 +94,8 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
                    int32_t base_size, const ccr_netting_set* sets, int32_t num_sets, double pfe_quantile,
                    int32_t precision, int32_t block, bool per_trade = false, bool book = false,
                    bool value_all = false, const std::vector<double>* book_rep = nullptr,
-                   const std::vector<double>* book_sim = nullptr, int32_t basis = -1);
+                   const std::vector<double>* book_sim = nullptr, int32_t basis = -1, int32_t nested_inner = 0,
+                   double nested_mpor = 0.0);
 
 // The portfolio book's grids (src/app/analyses.cpp:1690-1709): the union of the sets' reporting
 // grids, and the union of their simulation grids built on that reporting union.

@@ -44,6 +44,8 @@ def lib():
         L.ccr_hip_set_fault.argtypes = [P, I32]
         L.ccr_hip_set_shared_workspace.argtypes = [P, I32]
         L.ccr_hip_set_initial_margin.argtypes = [P, D, D]
+        L.ccr_hip_set_nested_initial_margin.argtypes = [P, I32, D, D]
+        L.ccr_hip_nested_initial_margin.argtypes = [P, P, I32, P]
         L.ccr_hip_initial_margin.argtypes = [P, P, I32, P, I32, P]
         L.ccr_hip_set_trade_statistics.argtypes = [P, I32]
         L.ccr_hip_set_book.argtypes = [P, I32]
@@ -198,6 +200,16 @@ class Plan:
                                                         C.byref(n)))
         return out[:n.value].copy()
 
+    def nested_initial_margin(self, s: int = 0) -> dict:
+        """Nested conditional initial margin of netting set `s` from the last execute, per reporting
+        date: discounted (E[D IM]), mean (E[IM]), band05 / band50 / band95 (quantiles of IM)."""
+        cap = max(1, self.info().max_reporting_dates)
+        arrs = {k: np.zeros(cap) for k in ("discounted", "mean", "band05", "band50", "band95")}
+        out = abi.NestedImOutC(cap, 0, *(abi.dptr(arrs[k]) for k in ("discounted", "mean", "band05", "band50",
+                                                                         "band95")))
+        self.engine._check(lib().ccr_hip_nested_initial_margin(self.engine.ctx, self.handle, s, C.byref(out)))
+        return {k: v[:out.num_dates].copy() for k, v in arrs.items()}
+
     def execute_surrogate(self, surrogate: "Surrogate", rows: bool = False):
         info = self.info()
         outs, arr = abi.make_outs(len(self.spec.sets), info.max_reporting_dates, self.spec.num_paths, rows)
@@ -320,6 +332,12 @@ class Engine:
         """Plans compiled from now on also produce per-trade allocation statistics (at most 16
         trades per netting set, no Bermudan swaptions)."""
         self._check(lib().ccr_hip_set_trade_statistics(self.ctx, 1 if enable else 0))
+
+    def set_nested_initial_margin(self, inner_paths: int, quantile: float = 0.99,
+                                  margin_period_of_risk: float = 10.0 / 250.0):
+        """Plans compiled from now on also produce the nested conditional initial margin of every
+        netting set (inner_paths 0 switches it off)."""
+        self._check(lib().ccr_hip_set_nested_initial_margin(self.ctx, inner_paths, quantile, margin_period_of_risk))
 
     def set_initial_margin(self, quantile: float, margin_period_of_risk: float = 10.0 / 250.0):
         """Plans compiled from now on also produce the initial-margin profile of every netting set

@@ -435,6 +435,8 @@ def main():
                          "(Bermudan AMC), c4 (credit + Bermudans)")
     ap.add_argument("--book-group", type=int, default=0,
                     help="netting sets per workgroup of the book kernel (0: the library's default)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="c5: keep the library's default book-kernel shape instead of autotuning it on the first wave")
     ap.add_argument("--no-book", action="store_true",
                     help="c5: independent netting sets (one engine run per set) instead of one book")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -500,6 +502,20 @@ def main():
     gram = abi.CCR_FP32 if args.gram == "fp32" else abi.CCR_FP64
     engine.set_regression({"shared": -1, "state": abi.CCR_BASIS_STATE, "sa": abi.CCR_BASIS_SWAP_RATE_ANNUITY}[basis],
                           gram)
+    tuned = None
+    if book and not args.no_autotune and not args.book_group:
+        # The project's autotuner: (workgroup size x netting sets per workgroup) of the book kernel,
+        # timed on the first wave (the largest sets) at 16,384 paths in this precision.
+        engine.set_book(True)
+        rep_grid, sim_grid = native.book_grids(spec)
+        engine.set_book_grids(rep_grid, sim_grid)
+        first = split_waves(spec, wave_sets)[0]
+        t0 = time.perf_counter()
+        blk, grp, tt = engine.autotune_book(first, prec, 16384)
+        tuned = {"block": blk, "sets_per_workgroup": grp, "seconds": time.perf_counter() - t0,
+                 "path_kernel_ms_16384_paths": {f"{b}x{g}": round(float(tt[i][k]), 3)
+                                                for i, b in enumerate((64, 128, 192, 256))
+                                                for k, g in enumerate((1, 2, 4, 8)) if tt[i][k] > 0}}
     specs, plans = compile_waves(engine, spec, prec, wave_sets, book)
     infos = [p.info() for p in plans]
     revals = sum(i.revaluations for i in infos)
@@ -542,6 +558,8 @@ def main():
     }
     if book:
         result["config"]["book"] = True
+    if tuned:
+        result["autotune"] = tuned
     if any(i.num_bermudans for i in infos):
         result["config"]["regression"] = {
             "basis": {"shared": "state (1, z, z^2, z^3) on shared per-date designs",

@@ -570,6 +570,18 @@ int32_t ccr_hip_autotune(ccr_hip_ctx* ctx, const ccr_scenario_generator* generat
                          const double* base_grid, int32_t base_size, const ccr_netting_set* sets, int32_t num_sets,
                          int32_t precision, int64_t sample_paths, int32_t* block_size, double* times_ms);
 
+/* Autotuner of the book kernel's shape (no counterpart in the reference): for a book context
+ * (ccr_hip_set_book), compiles the netting sets with min(config->num_paths, sample_paths) paths
+ * for every (workgroup size 64 / 128 / 192 / 256) x (netting sets per workgroup 1 / 2 / 4 / 8)
+ * and times the path kernel of each (best of 3 executes, HIP events); the fastest pair becomes
+ * the context's workgroup size and book group for later compiles. times_ms (16 entries, workgroup
+ * size major; may be NULL) receives every candidate's time (a shape the kernel's LDS rules out
+ * reads 0). Results do not depend on the choice. */
+int32_t ccr_hip_autotune_book(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator,
+                              const ccr_simulation_config* config, const double* base_grid, int32_t base_size,
+                              const ccr_netting_set* sets, int32_t num_sets, int32_t precision, int64_t sample_paths,
+                              int32_t* block_size, int32_t* sets_per_workgroup, double* times_ms);
+
 /* BermudanSwaption::valueWithExercise (src/instruments/bermudan_swaption.cpp:49-146) for one
  * Bermudan on its own scenario set: the simulation grid is `grid` merged with the trade's event
  * dates (as ExposureEngine::simulationGrid does), the regressions run on the GPU in `precision`.

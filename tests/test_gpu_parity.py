@@ -127,6 +127,35 @@ def test_autotuner_picks_a_size_and_keeps_results(engine):
         assert np.array_equal(before[k], after[k]), k
 
 
+def test_book_autotuner_picks_a_shape_and_keeps_results():
+    """The book kernel's autotuner over (workgroup size x sets per workgroup): a measured choice,
+    and the book's results are bit-identical before and after (every shape sums in one order)."""
+    from zlatko_amd import native
+    spec = W.c5(num_paths=2048, num_sets=12, largest=60, dates=24)
+    e = native.Engine()
+    try:
+        e.set_book(True)
+
+        def run():
+            plan = e.compile(spec)
+            outs = [o.as_dict() for o in plan.execute()]
+            plan.close()
+            return outs
+
+        before = run()
+        block, group, times = e.autotune_book(spec, sample_paths=2048)
+        print("book autotune", block, group, times.tolist(), flush=True)
+        assert block in (64, 128, 192, 256) and group in (1, 2, 4, 8)
+        assert times[(64, 128, 192, 256).index(block)][(1, 2, 4, 8).index(group)] == times[times > 0].min()
+        assert (times > 0).sum() >= 8
+        after = run()
+        for a, b in zip(before, after):
+            for k in abi.PROFILE_FIELDS:
+                assert np.array_equal(a[k], b[k]), k
+    finally:
+        e.close()
+
+
 def test_fp32_error_vs_fp64_oracle(engine):
     spec = W.c2(num_paths=1000, num_sets=2, num_trades=40, weeks=52, horizon=1.0)
     ref = oracle_lib.run_oracle(spec, threads=8)

@@ -1635,6 +1635,68 @@ int32_t ccr_hip_autotune(ccr_hip_ctx* ctx, const ccr_scenario_generator* generat
   return rc;
 }
 
+int32_t ccr_hip_autotune_book(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator,
+                              const ccr_simulation_config* config, const double* base_grid, int32_t base_size,
+                              const ccr_netting_set* sets, int32_t num_sets, int32_t precision, int64_t sample_paths,
+                              int32_t* block_size, int32_t* sets_per_workgroup, double* times_ms) {
+  if (!ctx) return CCR_ERR_INVALID_ARGUMENT;
+  const int keepBlock = ctx->block, keepGroup = ctx->book_group;
+  const int32_t rc = guarded(ctx, [&] {
+    if (!config || sample_paths <= 0) invalid("ccr_hip_autotune_book: invalid arguments");
+    if (!ctx->book) invalid("ccr_hip_autotune_book: the context is not in book mode");
+    ccr_simulation_config sample = *config;
+    sample.num_paths = std::min<int64_t>(config->num_paths, sample_paths);
+    const int blocks[4] = {64, 128, 192, 256}, groups[4] = {1, 2, 4, 8};
+    int bestBlock = keepBlock, bestGroup = keepGroup;
+    double bestMs = 1e300;
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 4; ++k) {
+        if (times_ms) times_ms[4 * i + k] = 0.0;
+        ctx->block = blocks[i];
+        ctx->book_group = groups[k];
+        ccr_hip_plan* raw = nullptr;
+        compilePlan(ctx, generator, &sample, base_grid, base_size, sets, num_sets, 0.95, precision, &raw);
+        std::unique_ptr<ccr_hip_plan, void (*)(ccr_hip_plan*)> plan(raw, ccr_hip_plan_destroy);
+        // A group the plan cannot take (one set, options, the IM proxy) runs as group 1: only
+        // the first such candidate is timed.
+        const int effective = plan->host.group > 1 ? plan->host.group : 1;
+        if (effective != groups[k]) continue;
+        // A shape whose LDS does not fit a workgroup is skipped.
+        int wg = 0;
+        const int ldsRows = plan->host.g.max_slots + effective * (plan->host.g.max_ring + plan->host.price_rows);
+        const hipError_t occ = precision == CCR_FP32
+                                   ? pathOccupancy<float>(plan->host.g.nrates, plan->host.g.nassets, plan->host.g.ncredits,
+                                                          !plan->host.opts.empty(), ldsRows, blocks[i], &wg, false,
+                                                          effective)
+                                   : pathOccupancy<double>(plan->host.g.nrates, plan->host.g.nassets, plan->host.g.ncredits,
+                                                           !plan->host.opts.empty(), ldsRows, blocks[i], &wg, false,
+                                                           effective);
+        if (occ != hipSuccess || wg <= 0) continue;
+        double ms = 1e300;
+        for (int rep = 0; rep < 4; ++rep) {
+          launchAll(ctx, plan.get());
+          readTimings(ctx);
+          if (rep > 0) ms = std::min(ms, ctx->timing.path_ms);  // the first execute warms up
+        }
+        if (times_ms) times_ms[4 * i + k] = ms;
+        if (ms < bestMs) {
+          bestMs = ms;
+          bestBlock = blocks[i];
+          bestGroup = groups[k];
+        }
+      }
+    ctx->block = bestBlock;
+    ctx->book_group = bestGroup;
+    if (block_size) *block_size = bestBlock;
+    if (sets_per_workgroup) *sets_per_workgroup = bestGroup;
+  });
+  if (rc != CCR_OK) {
+    ctx->block = keepBlock;
+    ctx->book_group = keepGroup;
+  }
+  return rc;
+}
+
 int32_t ccr_hip_bermudan(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator, const ccr_simulation_config* config,
                          const double* grid, int32_t grid_size, const ccr_trade* trade, int32_t precision, double* price,
                          int32_t* exercise_index) {

@@ -344,6 +344,17 @@ class Engine:
         (quantile <= 0 switches it off)."""
         self._check(lib().ccr_hip_set_initial_margin(self.ctx, quantile, margin_period_of_risk))
 
+    def autotune_book(self, spec: abi.Spec, precision: int = abi.CCR_FP64, sample_paths: int = 16384):
+        """Picks the book kernel's workgroup size and netting sets per workgroup for this workload
+        (book mode); returns (size, sets per workgroup, times in ms [4 sizes][4 groups], 0 = not run)."""
+        block, group = C.c_int32(), C.c_int32()
+        times = np.zeros(16)
+        self._check(lib().ccr_hip_autotune_book(self.ctx, C.byref(spec.generator), C.byref(spec.config),
+                                                abi.dptr(spec.base_grid), len(spec.base_grid), spec.sets_c,
+                                                len(spec.sets), precision, sample_paths, C.byref(block),
+                                                C.byref(group), abi.dptr(times)))
+        return block.value, group.value, times.reshape(4, 4)
+
     def autotune(self, spec: abi.Spec, precision: int = abi.CCR_FP64, sample_paths: int = 16384):
         """Picks the path kernel's workgroup size for this workload; returns (size, times in ms)."""
         block = C.c_int32()

@@ -776,7 +776,12 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
           all = all && bondPolynomial(plan.bonds.data() + 2 * b0, b1 - b0, col.c0[r], xmax, c, polyTol);
           rows = std::max(rows, c.size());
         }
-        if (all) {
+        // A record with no bond term and no constant on any rate (no cash flow left on this date)
+        // keeps no rows: the kernel's sum of the (empty) terms is the same exact zero.
+        bool anyTerm = false;
+        for (int r = 0; r < g.nrates; ++r)
+          anyTerm = anyTerm || rec.bond_beg[r + 1] > rec.bond_beg[r] || col.c0[r] != 0.0;
+        if (all && anyTerm) {
           rows = (rows + kPolyBatch - 1) / kPolyBatch * kPolyBatch;
           for (std::size_t k = 0; k < rows; ++k) {
             const std::size_t power = rows - 1 - k;

@@ -458,7 +458,7 @@ int32_t ccr_hip_book_grids(const double* base_grid, int32_t base_size, const ccr
 int32_t ccr_hip_set_book_grids(ccr_hip_ctx* ctx, const double* reporting, int32_t reporting_size,
                                const double* simulation, int32_t simulation_size);
 /* Book plans compiled afterwards run the book kernel with this many netting sets per workgroup
- * (1, 2, 4 or 8; default 2; 1 = one set per workgroup): a lane steps its path's market once per
+ * (1, 2, 4 or 8; default 4; 1 = one set per workgroup): a lane steps its path's market once per
  * date and values every set of its group, sharing the fixing states. Results are bit-identical
  * for every choice. Plans with European options, per-trade statistics or the initial-margin
  * proxy run one set per workgroup. */
@@ -576,7 +576,8 @@ int32_t ccr_hip_autotune(ccr_hip_ctx* ctx, const ccr_scenario_generator* generat
  * and times the path kernel of each (best of 3 executes, HIP events); the fastest pair becomes
  * the context's workgroup size and book group for later compiles. times_ms (16 entries, workgroup
  * size major; may be NULL) receives every candidate's time (a shape the kernel's LDS rules out
- * reads 0). Results do not depend on the choice. */
+ * reads 0). Results do not depend on the choice. The sample should give every shape several
+ * workgroups per compute unit (tens of thousands of paths): a small one favours small groups. */
 int32_t ccr_hip_autotune_book(ccr_hip_ctx* ctx, const ccr_scenario_generator* generator,
                               const ccr_simulation_config* config, const double* base_grid, int32_t base_size,
                               const ccr_netting_set* sets, int32_t num_sets, int32_t precision, int64_t sample_paths,

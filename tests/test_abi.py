@@ -23,6 +23,15 @@ def test_library_exports_every_declared_symbol():
     assert lib.ccr_hip_abi_version() == 2
 
 
+def test_every_declared_function_has_a_ctypes_signature():
+    """ctypes passes a Python int as a C int unless argtypes say otherwise: an int64_t argument
+    past the sixth (a stack slot) would read garbage in its upper half. Every function of the
+    header has its signature declared on the bound library."""
+    lib = native.lib()
+    unbound = [s for s in native.header_symbols() if getattr(lib, s).argtypes is None and s != "ccr_hip_abi_version"]
+    assert not unbound, unbound
+
+
 def test_grids_match_oracle_bit_for_bit():
     for spec in (W.small_reference_case(8, csa=True, fx=True), W.c2(num_paths=8, num_sets=3, num_trades=30)):
         for i in range(len(spec.sets)):

@@ -69,9 +69,9 @@ struct ccr_hip_ctx {
   bool book = false;
   std::vector<double> book_rep, book_sim;
   // Book plans: netting sets valued per workgroup by the book kernel (1: one set per workgroup).
-  // Two: the per-set LDS rows (ring, collateral units, previous exposures) of two sets leave five
-  // waves per SIMD (500-set portfolio: path kernel 459 ms at 2, 607 ms at 4, 964 ms at 8).
-  int book_group = 2;
+  // Four, the unrolled set loop with its state in registers: the market step is shared by four
+  // sets (500-set portfolio, 200,000 paths, 256 threads: path kernel 362 ms at 4, 400 ms at 2).
+  int book_group = 4;
   // Bermudan regressions of plans compiled while set: -1 the reference's state basis on the
   // shared per-date designs; CCR_BASIS_STATE / CCR_BASIS_SWAP_RATE_ANNUITY on the regression
   // engine, its Grams in FP64 or on FP32 matrix cores.

@@ -41,6 +41,23 @@ constexpr int pathWaveHint() {
              ? 5 : 1;
 }
 
+// The wavefront sums of a reporting date's kStats statistics into this wave's partials (out: the
+// date's kStats slots): the first eight in one transposed wavefront sum (lanes 0..7 store one
+// each), the DVA increment -- 0 without intensities (Q = 1) -- in a sum of its own (stored by
+// lane 8 either way). Inactive lanes (past the last path) add zeros.
+template <typename Sh>
+__device__ inline void storeStats(const double* st9, bool active, double* out) {
+  double vals[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) vals[k] = active ? st9[k] : 0.0;
+  int idx;
+  const double s = waveSumMany<8>(vals, &idx);
+  double dva = 0.0;
+  if constexpr (!(Sh::kExact && Sh::NC == 0)) dva = waveSum(active ? st9[8] : 0.0);
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  if (lane < 9) out[lane < 8 ? idx : 8] = lane < 8 ? s : dva;
+}
+
 template <typename Real, typename Sh, bool kOptions, bool kTrades = false>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(pathWaveHint<Real, Sh, kOptions>())))
 ccr_path_kernel(PathArgs<Real> a) {
@@ -91,18 +108,7 @@ ccr_path_kernel(PathArgs<Real> a) {
   };
   simulatePath<Real, Sh, kOptions, kTrades>(a, sp, pe, col, bd, [&](int rr, const double* st9, double vp) {
     if (active) rows[static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
-    // The first eight statistics in one transposed wavefront sum (lanes 0..7 store one each);
-    // the DVA increment is 0 without intensities (Q = 1), else a wavefront sum of its own
-    // (stored by lane 8 either way).
-    double vals[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) vals[k] = active ? st9[k] : 0.0;
-    int idx;
-    const double s = waveSumMany<8>(vals, &idx);
-    double dva = 0.0;
-    if constexpr (!(Sh::kExact && Sh::NC == 0)) dva = waveSum(active ? st9[8] : 0.0);
-    const int lane = tid & 63;
-    if (lane < 9) part[rr * kStats + (lane < 8 ? idx : 8)] = lane < 8 ? s : dva;
+    storeStats<Sh>(st9, active, part + rr * kStats);
   }, onValue, onTrades);
 }
 
@@ -111,14 +117,15 @@ ccr_path_kernel(PathArgs<Real> a) {
 // the groups reading one path block's replayed normals (every set of a book replays the same)
 // run together on one XCD and share its L2. Each set's statistics go to its own wave partials and
 // exposure rows as in ccr_path_kernel. LDS per path: the shared fixing slots, then each set's
-// margin-call ring and price rows, then its state rows.
-// Waves per SIMD the book kernel's registers are held to (96 VGPRs; the per-set state is in LDS,
-// so the group size does not change the registers). LDS then decides the occupancy: for the
-// 3-currency CSA portfolio (6 fixing rows + per set 2 ring rows and 3 state rows of 8 bytes) two
-// sets per 256-thread workgroup allow five waves per SIMD, four sets three, eight sets one.
+// margin-call ring and price rows, then (rolled set loops) its state rows.
+// Waves per SIMD the book kernel's registers are held to: five (96 VGPRs) with the per-set state
+// in LDS, four (128 VGPRs) for the unrolled four-set loop, whose state and code need more
+// registers (it spills at 96) but no state rows in LDS: with the LDS state, four sets of the
+// 3-currency CSA book (3 fixing rows + per set 2 ring rows and 3 state rows of 8 bytes per path)
+// allowed only three waves per SIMD at 256 threads.
 template <int G>
 constexpr int groupWaveHint() {
-  return 5;
+  return groupUnrolled(G) ? 4 : 5;
 }
 
 template <typename Real, typename Sh, int G>
@@ -148,16 +155,8 @@ ccr_group_kernel(PathArgs<Real> a) {
                              [&](int k, int rr, const double* st9, double vp) {
     const CCR_CONST SetDev& S = sets[s0 + k];
     if (active) a.rows[S.rows_off + static_cast<int64_t>(rr) * g.num_paths + p] = static_cast<Real>(vp);
-    double vals[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) vals[i] = active ? st9[i] : 0.0;
-    int idx;
-    const double s = waveSumMany<8>(vals, &idx);
-    double dva = 0.0;
-    if constexpr (!(Sh::kExact && Sh::NC == 0)) dva = waveSum(active ? st9[8] : 0.0);
-    const int lane = tid & 63;
     double* part = a.partials + S.partial_off + static_cast<int64_t>(waveLocal) * a.max_rep * kStats;
-    if (lane < 9) part[rr * kStats + (lane < 8 ? idx : 8)] = lane < 8 ? s : dva;
+    storeStats<Sh>(st9, active, part + rr * kStats);
   });
 }
 
@@ -702,7 +701,7 @@ template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
   if (args.group > 1) {
     const size_t lds = static_cast<size_t>(args.slots + args.group * (args.ring + args.price_rows)) * block * sizeof(Real) +
-                       static_cast<size_t>(3 * args.group) * block * sizeof(double);
+                       (!groupUnrolled(args.group) ? static_cast<size_t>(3 * args.group) * block * sizeof(double) : 0);
     return withShape(args.nrates, args.nassets, args.ncredits, [&](auto shape) {
       return launchGroupShape<Real, decltype(shape)>(args, block, lds, stream);
     });
@@ -722,9 +721,10 @@ hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStre
 template <typename Real>
 hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu,
                          bool trades, int group) {
-  // Group kernels: ldsRows Real rows plus three double rows per set.
+  // Group kernels: ldsRows Real rows plus, where the set loop is not unrolled, three double
+  // rows per set.
   const size_t lds = static_cast<size_t>(ldsRows) * block * sizeof(Real) +
-                     (group > 1 ? static_cast<size_t>(3 * group) * block * sizeof(double) : 0);
+                     (!groupUnrolled(group) ? static_cast<size_t>(3 * group) * block * sizeof(double) : 0);
   if (group > 1)
     return withShape(nrates, nassets, ncredits, [&](auto shape) -> hipError_t {
       using Sh = decltype(shape);

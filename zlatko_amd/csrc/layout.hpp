@@ -26,6 +26,11 @@ constexpr int kMaxCorr = kMaxRates + kMaxAssets + kMaxCredits;
 // Fixing states a nested initial-margin window (t, t + mpor] may overwrite (the nested kernel
 // saves and restores them around every inner path), at most.
 constexpr int kNestedMaxSave = 16;
+
+// Book kernels whose loop over a workgroup's netting sets is unrolled, the per-set collateral
+// and discounted-exposure state in registers instead of LDS rows: four sets per workgroup (two
+// sets fit five waves per SIMD with the state in LDS; eight would hold too much code).
+constexpr bool groupUnrolled(int group) { return group == 4; }
 constexpr int kMaxSub = 8;        // CIR++ sub-steps per grid step
 constexpr int kStats = 9;         // E[V-C], EE, ENE, EE*, ENE*, E[D], E[Q], CVA incr., DVA incr.
 constexpr int kMaxBlock = 256;    // launch bound of the path kernel (threads per workgroup)

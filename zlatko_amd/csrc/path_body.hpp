@@ -683,6 +683,8 @@ __host__ __device__ __forceinline__ Real collateralAndReport(const Market<Real, 
                                                             const Rec& rec, cptr<Real> dr, int j, Real V,
                                                             SetPath<Real>& st, Real* ringV, Real* ringP, int stride,
                                                             Emit&& emit) {
+  // Not contracted into FMAs: the per-set kernel's copy of this arithmetic rounds alike.
+#pragma clang fp contract(off)
   constexpr int NA = Sh::NA, NC = Sh::NC;
   Real C = 0;
   if (S.has_csa) {
@@ -1055,6 +1057,9 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     }
     Real C = 0;
     if (S.has_csa) {
+      // The collateral and statistics arithmetic is not contracted into FMAs (here and in
+      // collateralAndReport): every kernel that reports a set rounds them alike.
+#pragma clang fp contract(off)
       Real price = 1;
       if (S.sec_type == CCR_COLLATERAL_BOND) {
         price = dr[kDateBond] * expr(-dr[kDateBond + 1] * x[0]);
@@ -1083,43 +1088,47 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
     }
     if (rec.flags & kValued) onValue(j, V, D);
     if (!(rec.flags & kReporting)) continue;
-    const double v = static_cast<double>(V) - static_cast<double>(C);
-    const double d = static_cast<double>(D);
-    const double vp = v > 0.0 ? v : 0.0, vn = v < 0.0 ? v : 0.0;
-    const double dvp = d * vp, dvn = d * (-vn);
-    double qc = 1.0, qo = 1.0;
+    {
+#pragma clang fp contract(off)
+      const double v = static_cast<double>(V) - static_cast<double>(C);
+      const double d = static_cast<double>(D);
+      const double vp = v > 0.0 ? v : 0.0, vn = v < 0.0 ? v : 0.0;
+      const double dvp = d * vp, dvn = d * (-vn);
+      double qc = 1.0, qo = 1.0;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      if (c == S.cpty) qc = static_cast<double>(Qs[c]);
-      if (c == S.own) qo = static_cast<double>(Qs[c]);
+      for (int c = 0; c < NC; ++c) {
+        if (c == S.cpty) qc = static_cast<double>(Qs[c]);
+        if (c == S.own) qo = static_cast<double>(Qs[c]);
+      }
+      const int rr = rec.rep;
+      double st9[kStats];
+      st9[0] = v;
+      st9[1] = vp;
+      st9[2] = vn;
+      st9[3] = dvp;
+      st9[4] = d * vn;
+      st9[5] = d;
+      st9[6] = qc;
+      st9[7] = rr > 0 ? 0.5 * (prevDvp + dvp) * (prevQc - qc) : 0.0;
+      st9[8] = rr > 0 ? 0.5 * (prevDvn + dvn) * (prevQo - qo) : 0.0;
+      prevDvp = dvp;
+      prevDvn = dvn;
+      prevQc = qc;
+      prevQo = qo;
+      emit(rr, st9, vp);
+      if constexpr (kTrades) onTrades(rr, V, D, tv, stride);
     }
-    const int rr = rec.rep;
-    double st9[kStats];
-    st9[0] = v;
-    st9[1] = vp;
-    st9[2] = vn;
-    st9[3] = dvp;
-    st9[4] = d * vn;
-    st9[5] = d;
-    st9[6] = qc;
-    st9[7] = rr > 0 ? 0.5 * (prevDvp + dvp) * (prevQc - qc) : 0.0;
-    st9[8] = rr > 0 ? 0.5 * (prevDvn + dvn) * (prevQo - qo) : 0.0;
-    prevDvp = dvp;
-    prevDvn = dvn;
-    prevQc = qc;
-    prevQo = qo;
-    emit(rr, st9, vp);
-    if constexpr (kTrades) onTrades(rr, V, D, tv, stride);
   }
 }
 
 // Book plans, G netting sets per lane: the sets of a book share their grid and their draws, so
 // one lane steps its path's market once per date and values every set of its group on it. Per
 // path, LDS holds the shared fixing states (rows [0, slots), recorded from the group's first
-// set's lists -- a book's lists are the same), each set's margin-call ring and price rows (rows
-// slots + k (ring + priceRows) ...) and, in `state` (a double column), each set's collateral
-// units and previous discounted exposures (rows 3k .. 3k + 2): the set loop is not unrolled, so
-// per-set state kept in registers would sit in scratch memory instead.
+// set's lists -- a book's lists are the same) and each set's margin-call ring and price rows (rows
+// slots + k (ring + priceRows) ...). Each set's collateral units and previous discounted
+// exposures stay in registers when the set loop is unrolled (groupUnrolled(G)); otherwise the
+// loop stays rolled and they live in `state` (a double column, rows 3k .. 3k + 2), since
+// registers indexed by a loop counter would sit in scratch memory.
 // sp[g], g < ns: the group's sets; emit(g, rep, stats, positive_exposure).
 template <typename Real, typename Sh, int G, typename Emit>
 __host__ __device__ __forceinline__ void simulateGroup(cptr<GlobalDev> gp, cptr<SetDev> sets, int s0, int ns, int pe,
@@ -1128,6 +1137,7 @@ __host__ __device__ __forceinline__ void simulateGroup(cptr<GlobalDev> gp, cptr<
                                               cptr<int32_t> fstore, cptr<Real> poly, const double* bermv, int ring,
                                               int priceRows, int slots, Real* col, double* state, int stride,
                                               Emit&& emit) {
+  constexpr bool kRegs = groupUnrolled(G);
   const CCR_CONST GlobalDev& g = *gp;
   const CCR_CONST SetDev& S0 = sets[s0];
   Real* xs = col;
@@ -1136,19 +1146,25 @@ __host__ __device__ __forceinline__ void simulateGroup(cptr<GlobalDev> gp, cptr<
   cptr<Real> st0 = steps + static_cast<int64_t>(S0.date_off) * kStepStride;
   Market<Real, Sh> mk;
   marketInit<Real, Sh, false>(mk, g, zbase, S0.num_dates);
-  for (int k = 0; k < ns; ++k) {
-    state[(3 * k) * stride] = 0.0;
-    state[(3 * k + 1) * stride] = 0.0;
-    state[(3 * k + 2) * stride] = 0.0;
+  SetPath<Real> regs[kRegs ? G : 1];
+#pragma unroll
+  for (int k = 0; k < (kRegs ? G : 1); ++k) setPathInit(regs[k]);
+  if constexpr (!kRegs) {
+    for (int k = 0; k < ns; ++k) {
+      state[(3 * k) * stride] = 0.0;
+      state[(3 * k + 1) * stride] = 0.0;
+      state[(3 * k + 2) * stride] = 0.0;
+    }
   }
   for (int j = 0; j < S0.num_dates; ++j) {
     if (j > 0) marketStep<Real, Sh, false>(mk, g, j, S0.num_dates, st0, zbase, sgn);
     const CCR_CONST DateRec& r0 = dates[S0.date_off + j];
     storeFixings(mk, r0.fixstore_beg, r0.fixstore_end, fstore + 2 * S0.fixstore_off, xs, stride);
-    for (int k = 0; k < ns; ++k) {
+    // Set k of the group on date j: value, collateral and (reporting dates) statistics.
+    auto setStep = [&](int k, SetPath<Real>& st) {
       const CCR_CONST SetDev& S = sets[s0 + k];
       const CCR_CONST DateRec& rec = dates[S.date_off + j];
-      if (!(rec.flags & kValued) && !S.has_csa) continue;
+      if (!(rec.flags & kValued) && !S.has_csa) return;
       cptr<Real> dr = dreal + (static_cast<int64_t>(S.date_off) + j) * kDateRealStride;
       Real V = 0;
       if (rec.flags & kValued) {
@@ -1158,16 +1174,25 @@ __host__ __device__ __forceinline__ void simulateGroup(cptr<GlobalDev> gp, cptr<
         if (bermv && S.berm_end > S.berm_beg)
           V += static_cast<Real>(bermv[(static_cast<int64_t>(S.cd_base) + rec.vidx) * g.num_paths + pe]);
       }
-      SetPath<Real> st;
-      st.units = static_cast<Real>(state[(3 * k) * stride]);
-      st.prevDvp = state[(3 * k + 1) * stride];
-      st.prevDvn = state[(3 * k + 2) * stride];
       Real* ringV = col + (slots + k * (ring + priceRows)) * stride;
       collateralAndReport(mk, S, rec, dr, j, V, st, ringV, ringV + ring * stride, stride,
                           [&](int rr, const double* st9, double vp) { emit(k, rr, st9, vp); });
-      state[(3 * k) * stride] = static_cast<double>(st.units);
-      state[(3 * k + 1) * stride] = st.prevDvp;
-      state[(3 * k + 2) * stride] = st.prevDvn;
+    };
+    if constexpr (kRegs) {
+#pragma unroll
+      for (int k = 0; k < G; ++k)
+        if (k < ns) setStep(k, regs[k]);
+    } else {
+      for (int k = 0; k < ns; ++k) {
+        SetPath<Real> st;
+        st.units = static_cast<Real>(state[(3 * k) * stride]);
+        st.prevDvp = state[(3 * k + 1) * stride];
+        st.prevDvn = state[(3 * k + 2) * stride];
+        setStep(k, st);
+        state[(3 * k) * stride] = static_cast<double>(st.units);
+        state[(3 * k + 1) * stride] = st.prevDvp;
+        state[(3 * k + 2) * stride] = st.prevDvn;
+      }
     }
     // A book's sets report on the same dates (the book's reporting grid).
     if (r0.flags & kReporting) marketReported(mk);

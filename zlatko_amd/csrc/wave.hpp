@@ -15,15 +15,24 @@ __device__ inline double dppMove(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// DPP move of a double with all rows and banks enabled, for patterns whose source lanes all
+// exist (quad permutations, row rotations and mirrors): no previous value is needed.
+template <int Ctrl>
+__device__ inline double dppFull(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), Ctrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), Ctrl, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
 // Sum over the 64 lanes of a wavefront in a fixed order (bit-reproducible), with DPP lane moves
 // only: pairs (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows (row_half_mirror),
 // rows (row_mirror), then row_bcast:15 and row_bcast:31 carry the row sums up; lane 63 ends
 // with the total, which is broadcast.
 __device__ inline double waveSum(double v) {
-  v += dppMove<0xb1, 0xf>(v);
-  v += dppMove<0x4e, 0xf>(v);
-  v += dppMove<0x141, 0xf>(v);
-  v += dppMove<0x140, 0xf>(v);
+  v += dppFull<0xb1>(v);
+  v += dppFull<0x4e>(v);
+  v += dppFull<0x141>(v);
+  v += dppFull<0x140>(v);
   v += dppMove<0x142, 0xa>(v);
   v += dppMove<0x143, 0xc>(v);
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);

@@ -105,6 +105,8 @@ def lib():
         L.ccr_hip_surrogate_destroy.argtypes = [P]
         L.ccr_hip_autotune.restype = I32
         L.ccr_hip_autotune.argtypes = [P, P, P, P, I32, P, I32, I32, I64, P, P]
+        L.ccr_hip_autotune_book.restype = I32
+        L.ccr_hip_autotune_book.argtypes = [P, P, P, P, I32, P, I32, I32, I64, P, P, P]
         L.ccr_hip_bermudan.restype = I32
         L.ccr_hip_bermudan.argtypes = [P, P, P, P, I32, P, I32, P, P]
         L.ccr_expected_positive_exposure.restype = D

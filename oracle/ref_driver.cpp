@@ -29,6 +29,7 @@
 #include "ccr/gpu/fused_exposure.hpp"
 #include "ccr/instruments/asset_forward.hpp"
 #include "ccr/instruments/bermudan_swaption.hpp"
+#include "ccr/instruments/credit_default_swap.hpp"
 #include "ccr/instruments/european_option.hpp"
 #include "ccr/instruments/interest_rate_swap.hpp"
 #include "ccr/xva/xva.hpp"
@@ -349,6 +350,39 @@ void ccr_ref_normals(uint64_t seed, int64_t n, double* out) {
 }
 
 double ccr_ref_quantile(const double* v, int64_t n, double p) { return ccr::quantile(std::vector<double>(v, v + n), p); }
+
+// The market parsing of the front door, on the reference's own functions: bootstrapCreditCurve
+// (quarterly premiums) of CDS quotes on a yield curve into hazards[n] (pillars = sorted
+// maturities), InterestRateSwap::vanilla(...).parRate, TimeGrid::standardExposureGrid and
+// cvaRunningSpread's risky annuity.
+int32_t ccr_ref_bootstrap_credit(const ccr_yield_curve* curve, const double* maturities, const double* spreads,
+                                 int32_t n, double recovery, double* pillars, double* hazards) {
+  try {
+    std::vector<ccr::CdsQuote> q;
+    for (int32_t i = 0; i < n; ++i) q.push_back({maturities[i], spreads[i]});
+    const ccr::CreditCurve cc = ccr::bootstrapCreditCurve(*curveOf(*curve), q, recovery);
+    for (std::size_t i = 0; i < cc.pillars().size(); ++i) {
+      pillars[i] = cc.pillars()[i];
+      hazards[i] = cc.hazards()[i];
+    }
+    return CCR_OK;
+  } catch (const std::exception&) {
+    return CCR_ERR_INVALID_ARGUMENT;
+  }
+}
+double ccr_ref_par_rate(const ccr_yield_curve* curve, double start, double tenor, int32_t freq) {
+  const auto swap = ccr::InterestRateSwap::vanilla("s", 1.0, 0.0, start, tenor, freq,
+                                                   ccr::InterestRateSwap::Direction::PayFixed);
+  return swap.parRate(*curveOf(*curve));
+}
+int32_t ccr_ref_standard_grid(double horizon, double* out, int32_t capacity) {
+  const auto g = ccr::TimeGrid::standardExposureGrid(horizon).times();
+  for (std::size_t i = 0; i < g.size() && static_cast<int32_t>(i) < capacity; ++i) out[i] = g[i];
+  return static_cast<int32_t>(g.size());
+}
+double ccr_ref_risky_annuity(const ccr_yield_curve* curve, const ccr_credit_curve* credit, double maturity) {
+  return ccr::riskyAnnuity(*curveOf(*curve), *creditOf(*credit), maturity);
+}
 
 // ccr::ai::fitMlp on rows X [n][din], Y [n][dout]; writes W0, B0, W1, B1, W2, B2 (row-major) and
 // the standardisers x_mean, x_sd, y_mean, y_sd.

@@ -138,6 +138,15 @@ def ref():
         if hasattr(lib, "ccr_ref_fused_nested_im"):
             lib.ccr_ref_fused_nested_im.restype = I32
             lib.ccr_ref_fused_nested_im.argtypes = [P, P, P, I32, P, I32, D, D, P, P, I32, P, C.c_char_p, I32]
+        if hasattr(lib, "ccr_ref_bootstrap_credit"):
+            lib.ccr_ref_bootstrap_credit.restype = I32
+            lib.ccr_ref_bootstrap_credit.argtypes = [P, P, P, I32, D, P, P]
+            lib.ccr_ref_par_rate.restype = D
+            lib.ccr_ref_par_rate.argtypes = [P, D, D, I32]
+            lib.ccr_ref_standard_grid.restype = I32
+            lib.ccr_ref_standard_grid.argtypes = [D, P, I32]
+            lib.ccr_ref_risky_annuity.restype = D
+            lib.ccr_ref_risky_annuity.argtypes = [P, P, D]
         _ref = lib
     return _ref
 
@@ -474,3 +483,41 @@ def ref_bermudan(spec: abi.Spec, set_index: int, trade_index: int, grid: np.ndar
     if rc != 0:
         raise RuntimeError(f"reference bermudan failed ({rc})")
     return price.value, ex
+
+
+def _curve_c(times, rates):
+    t = np.ascontiguousarray(times, np.float64)
+    r = np.ascontiguousarray(rates, np.float64)
+    return abi.YieldCurveC(abi.dptr(t), abi.dptr(r), len(t)), (t, r)
+
+
+def ref_bootstrap_credit(curve_times, curve_rates, maturities, spreads, recovery):
+    """The reference's bootstrapCreditCurve (quarterly premiums): (pillars, hazards)."""
+    c, keep = _curve_c(curve_times, curve_rates)
+    m = np.ascontiguousarray(maturities, np.float64)
+    sp = np.ascontiguousarray(spreads, np.float64)
+    pillars, hazards = np.zeros(len(m)), np.zeros(len(m))
+    rc = ref().ccr_ref_bootstrap_credit(C.byref(c), abi.dptr(m), abi.dptr(sp), len(m), recovery, abi.dptr(pillars),
+                                        abi.dptr(hazards))
+    if rc:
+        raise RuntimeError("reference bootstrap failed")
+    return pillars, hazards
+
+
+def ref_par_rate(curve_times, curve_rates, start, tenor, freq):
+    c, keep = _curve_c(curve_times, curve_rates)
+    return ref().ccr_ref_par_rate(C.byref(c), start, tenor, freq)
+
+
+def ref_standard_grid(horizon):
+    out = np.zeros(512)
+    n = ref().ccr_ref_standard_grid(horizon, abi.dptr(out), len(out))
+    return out[:n]
+
+
+def ref_risky_annuity(curve_times, curve_rates, pillars, hazards, maturity):
+    c, keep = _curve_c(curve_times, curve_rates)
+    p = np.ascontiguousarray(pillars, np.float64)
+    h = np.ascontiguousarray(hazards, np.float64)
+    cc = abi.CreditCurveC(abi.dptr(p), abi.dptr(h), len(p))
+    return ref().ccr_ref_risky_annuity(C.byref(c), C.byref(cc), maturity)

@@ -20,7 +20,7 @@ LIB = HERE / "libccr_hip.so"
 ARCH = os.environ.get("CCR_HIP_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels.hip", "amc.hip", "mlp.hip", "mlp_host.cpp", "capi.cpp"]
-HOST_SOURCES = ["plan.cpp", "draws.cpp"]
+HOST_SOURCES = ["plan.cpp", "draws.cpp", "app.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-parameter"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

@@ -1,0 +1,1618 @@
+// app.cpp -- the Spec/Value front door (ccr_hip_app_call): the reference's JSON-like analysis
+// specifications run on this library's engine. A spec is parsed with the reference's keys and
+// defaults (parseCurve / parseQuotes / parseSwap / parseTrade / parseCsa / parseEnv,
+// src/app/analyses.cpp:85-290), credit curves are bootstrapped from CDS quotes
+// (bootstrapCreditCurve, src/instruments/credit_default_swap.cpp:76-94), the analysis is split
+// into engine jobs (analysisJobs, analyses.cpp:682-780; the book, :1604-1766), each job is
+// compiled and executed through the C-ABI (ccr_hip_compile / ccr_hip_execute), and the results
+// are shaped as the reference's entry points return them (profileToJs :294-311, combine
+// :1304-1447, cvaAnalytics :801-852, xvaResult :950-1057, allocationResult :1059-1106,
+// bookResult :1817-2023). Host code only.
+#include "../../include/ccr_hip_app.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ccr_hip.h"
+#include "app_value.hpp"
+#include "model.hpp"
+
+namespace ccrhip {
+namespace app {
+namespace {
+
+constexpr double kNaN = std::numeric_limits<double>::quiet_NaN();
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+// ------------------------------------------------------------------ spec access (analyses.cpp:29-41)
+
+bool has(const Value& o, const char* key) { return o[key].present(); }
+double num(const Value& o, const char* key, double fallback) { return has(o, key) ? o[key].asNumber(key) : fallback; }
+std::string str(const Value& o, const char* key, const std::string& fallback) {
+  return has(o, key) ? o[key].asString(key) : fallback;
+}
+Value arr(const std::vector<double>& v) { return Value::numbers(v); }
+
+// ------------------------------------------------------------------ market data
+
+struct CurveData {
+  std::vector<double> t, r;
+  ccr_yield_curve c() const { return ccr_yield_curve{t.data(), r.data(), static_cast<int32_t>(t.size())}; }
+  Curve curve() const { return Curve(c()); }
+};
+
+// parseCurve (analyses.cpp:85-89): absent -> flat at the fallback rate; {flat: r}; {times, rates}.
+// A flat curve is one pillar at 1y (YieldCurve::flat, src/market/yield_curve.cpp:18).
+CurveData parseCurve(const Value& c, double fallback) {
+  if (!c.present()) return {{1.0}, {fallback}};
+  if (has(c, "flat")) return {{1.0}, {c["flat"].asNumber("flat")}};
+  CurveData d{c["times"].asNumbers("curve times"), c["rates"].asNumbers("curve rates")};
+  if (d.t.empty() || d.t.size() != d.r.size()) throw std::invalid_argument("YieldCurve: times and rates must match");
+  return d;
+}
+
+Credit creditOf(const std::vector<double>& pillars, const std::vector<double>& hazards) {
+  Credit c;
+  c.p = pillars;
+  c.h = hazards;
+  return c;
+}
+
+// CDS premium dates: from the maturity backwards in steps of 1 / frequency while above 1e-9, in
+// ascending order (credit_default_swap.cpp:13-19).
+std::vector<double> premiumTimes(double maturity, int frequency) {
+  std::vector<double> t;
+  const double step = 1.0 / frequency;
+  for (double x = maturity; x > 1e-9; x -= step) t.push_back(x);
+  std::reverse(t.begin(), t.end());
+  return t;
+}
+
+// Risky annuity of a CDS per unit spread: premium tau P(0, t) Q(t) plus the premium accrued on
+// default, half a period on average, discounted at the period's mid-point (:38-49).
+double cdsAnnuity(const Curve& yc, const Credit& cc, double maturity, int frequency) {
+  double a = 0.0, prev = 0.0;
+  for (double t : premiumTimes(maturity, frequency)) {
+    const double tau = t - prev;
+    a += tau * yc.discount(t) * cc.survival(t);
+    a += 0.5 * tau * yc.discount(0.5 * (prev + t)) * (cc.survival(prev) - cc.survival(t));
+    prev = t;
+  }
+  return a;
+}
+
+// Protection leg: (1 - R) sum of P(0, mid) x default probability over 12 sub-periods of every
+// premium period (:51-63).
+double cdsProtection(const Curve& yc, const Credit& cc, double maturity, int frequency, double recovery) {
+  constexpr int kSub = 12;
+  double leg = 0.0, prev = 0.0;
+  for (double t : premiumTimes(maturity, frequency)) {
+    const double h = (t - prev) / kSub;
+    for (int s = 0; s < kSub; ++s) {
+      const double a = prev + s * h, b = a + h;
+      leg += yc.discount(0.5 * (a + b)) * (cc.survival(a) - cc.survival(b));
+    }
+    prev = t;
+  }
+  return (1.0 - recovery) * leg;
+}
+
+// Brent's method (R. P. Brent, "Algorithms for Minimization without Derivatives", 1973, ch. 4):
+// a bracketing root finder that takes inverse quadratic or secant steps when they stay inside
+// the bracket and shrink it fast enough, and bisection otherwise; stops when the bracket is
+// within 2 eps |b| + tol / 2 (the reference's solveBrent, include/ccr/core/math.hpp:50-109, runs
+// the same method).
+template <typename F>
+double brentRoot(F&& f, double a, double b, double tol, int maxIter = 200) {
+  double fa = f(a), fb = f(b);
+  if (fa == 0.0) return a;
+  if (fb == 0.0) return b;
+  if ((fa > 0.0) == (fb > 0.0)) throw std::invalid_argument("bootstrap: the root is not bracketed");
+  double c = b, fc = fb, step = b - a, lastStep = step;
+  for (int it = 0; it < maxIter; ++it) {
+    if ((fb > 0.0) == (fc > 0.0)) {  // keep the root between b and c
+      c = a;
+      fc = fa;
+      step = lastStep = b - a;
+    }
+    if (std::fabs(fc) < std::fabs(fb)) {  // b is the best estimate
+      a = b;
+      b = c;
+      c = a;
+      fa = fb;
+      fb = fc;
+      fc = fa;
+    }
+    const double tol1 = 2.0 * std::numeric_limits<double>::epsilon() * std::fabs(b) + 0.5 * tol;
+    const double half = 0.5 * (c - b);
+    if (std::fabs(half) <= tol1 || fb == 0.0) return b;
+    if (std::fabs(lastStep) >= tol1 && std::fabs(fa) > std::fabs(fb)) {
+      double p, q;
+      const double s = fb / fa;
+      if (a == c) {  // secant
+        p = 2.0 * half * s;
+        q = 1.0 - s;
+      } else {  // inverse quadratic interpolation through a, b, c
+        const double qa = fa / fc, rb = fb / fc;
+        p = s * (2.0 * half * qa * (qa - rb) - (b - a) * (rb - 1.0));
+        q = (qa - 1.0) * (rb - 1.0) * (s - 1.0);
+      }
+      if (p > 0.0) q = -q;
+      p = std::fabs(p);
+      if (2.0 * p < std::min(3.0 * half * q - std::fabs(tol1 * q), std::fabs(lastStep * q))) {
+        lastStep = step;
+        step = p / q;
+      } else {
+        step = half;
+        lastStep = step;
+      }
+    } else {
+      step = half;
+      lastStep = step;
+    }
+    a = b;
+    fa = fb;
+    b += std::fabs(step) > tol1 ? step : (half > 0.0 ? tol1 : -tol1);
+    fb = f(b);
+  }
+  throw std::runtime_error("bootstrap: the root finder did not converge");
+}
+
+struct Quote {
+  double maturity, spread;
+};
+
+// bootstrapCreditCurve (credit_default_swap.cpp:76-94): quotes by maturity; for each, the flat
+// hazard on (previous pillar, maturity] that reprices the CDS at its par spread (quarterly
+// premiums), bracketed by doubling from 1 up to 100, solved to 1e-14.
+Credit bootstrapCredit(const Curve& yc, std::vector<Quote> quotes, double recovery, int frequency = 4) {
+  if (quotes.empty()) throw std::invalid_argument("bootstrapCreditCurve: no quotes");
+  if (recovery < 0.0 || recovery >= 1.0) throw std::invalid_argument("CreditDefaultSwap: recovery in [0, 1)");
+  std::sort(quotes.begin(), quotes.end(), [](const Quote& x, const Quote& y) { return x.maturity < y.maturity; });
+  Credit cc;
+  for (const Quote& q : quotes) {
+    if (q.maturity <= 0.0) throw std::invalid_argument("CreditDefaultSwap: invalid arguments");
+    cc.p.push_back(q.maturity);
+    cc.h.push_back(0.0);
+    auto excess = [&](double h) {
+      cc.h.back() = h;
+      return cdsProtection(yc, cc, q.maturity, frequency, recovery) / cdsAnnuity(yc, cc, q.maturity, frequency) -
+             q.spread;
+    };
+    double hi = 1.0;
+    while (excess(hi) < 0.0 && hi < 100.0) hi *= 2.0;
+    cc.h.back() = brentRoot(excess, 0.0, hi, 1e-14);
+  }
+  return cc;
+}
+
+// The free riskyAnnuity (credit_default_swap.cpp:96-104): sum tau P(0, t) Q(t), no accrual.
+double premiumAnnuity(const Curve& yc, const Credit& cc, double maturity, int frequency = 4) {
+  double a = 0.0, prev = 0.0;
+  for (double t : premiumTimes(maturity, frequency)) {
+    a += (t - prev) * yc.discount(t) * cc.survival(t);
+    prev = t;
+  }
+  return a;
+}
+
+std::vector<Quote> parseQuotes(const Value& q) {  // analyses.cpp:91-96
+  std::vector<Quote> out;
+  for (std::size_t i = 0; i < q.size(); ++i)
+    out.push_back({q.at(i)["maturity"].asNumber("quote maturity"), q.at(i)["spread"].asNumber("quote spread")});
+  return out;
+}
+
+// ------------------------------------------------------------------ environment
+
+struct AssetData {
+  std::string name;
+  double spot, vol;
+  CurveData carry;
+};
+
+struct CreditData {
+  std::string name;
+  double recovery;
+  std::vector<Quote> quotes;
+  Credit curve;
+  double kappa, theta, xi, y0;
+  int substeps;
+};
+
+struct TradeData {
+  std::string id, type;
+  int kind = CCR_TRADE_SWAP;
+  double notional = 0.0, fixedRate = 0.0, maturity = 0.0, position = 1.0;
+  bool payFixed = true, cash = false;
+  int asset = 0, optionType = CCR_OPTION_CALL, degree = 3;
+  std::vector<double> schedule, exercises;
+  double exposureMaturity() const {  // Trade::maturity() of the reference's trade types
+    if (kind == CCR_TRADE_SWAP) return schedule.back();
+    if (kind == CCR_TRADE_BERMUDAN_SWAPTION) return cash ? exercises.back() : schedule.back();
+    return maturity;
+  }
+  ccr_trade c() const {
+    ccr_trade t{};
+    t.kind = kind;
+    t.pay_fixed = payFixed ? 1 : 0;
+    t.asset = asset;
+    t.notional = notional;
+    t.fixed_rate = fixedRate;
+    t.maturity = maturity;
+    if (!schedule.empty()) {
+      t.schedule = schedule.data();
+      t.schedule_size = static_cast<int32_t>(schedule.size());
+    }
+    if (!exercises.empty()) {
+      t.exercise_times = exercises.data();
+      t.num_exercises = static_cast<int32_t>(exercises.size());
+    }
+    t.cash_settled = cash ? 1 : 0;
+    t.position = position;
+    t.regression_degree = degree;
+    t.option_type = optionType;
+    return t;
+  }
+};
+
+struct Env {
+  CurveData domestic;
+  double a = 0.05, sigma = 0.01;
+  std::vector<AssetData> assets;
+  std::vector<CreditData> credits;
+  std::vector<double> correlation;  // nF x nF row-major, nF = 1 + assets + credits
+  ccr_simulation_config sim{2000, 42, 1};
+  std::vector<double> baseGrid;
+  bool hasCsa = false;
+  ccr_collateral csa{};
+  double pfeQuantile = 0.95;
+  int counterparty = 0;
+  bool hasOwn = false;
+  double ownRecovery = 0.4;
+  std::vector<Quote> ownQuotes;
+  Credit ownCurve;
+  std::vector<TradeData> trades;
+
+  int factors() const { return 1 + static_cast<int>(assets.size() + credits.size()); }
+  const CreditData& cpty() const {
+    if (credits.empty()) throw std::invalid_argument("spec has no counterparty credit");
+    if (counterparty < 0 || counterparty >= static_cast<int>(credits.size()))
+      throw std::invalid_argument("counterparty index out of range");
+    return credits[static_cast<std::size_t>(counterparty)];
+  }
+  // Our own curve for XVA: without spec.own we are default-free (analyses.cpp:527-530).
+  Credit ownOrRiskless() const { return hasOwn ? ownCurve : creditOf({1.0}, {0.0}); }
+};
+
+// InterestRateSwap::vanilla (interest_rate_swap.cpp:27-35): round(tenor x frequency) periods from
+// `start`, the last date exactly start + tenor.
+std::vector<double> vanillaSchedule(double start, double tenor, int frequency) {
+  if (tenor <= 0.0 || frequency <= 0) throw std::invalid_argument("InterestRateSwap::vanilla: bad arguments");
+  const auto periods = static_cast<std::size_t>(std::lround(tenor * frequency));
+  std::vector<double> s(periods + 1);
+  for (std::size_t i = 0; i <= periods; ++i) s[i] = start + static_cast<double>(i) / frequency;
+  s.back() = start + tenor;
+  if (s.size() < 2 || s.front() < 0.0) throw std::invalid_argument("InterestRateSwap: invalid schedule");
+  for (std::size_t i = 1; i < s.size(); ++i)
+    if (s[i] <= s[i - 1]) throw std::invalid_argument("InterestRateSwap: schedule must be strictly increasing");
+  return s;
+}
+
+// InterestRateSwap::parRate (:89-94): (P(s_0) - P(s_n)) / sum tau_i P(s_i).
+double parRate(const std::vector<double>& s, const Curve& yc) {
+  double annuity = 0.0;
+  for (std::size_t i = 1; i < s.size(); ++i) annuity += (s[i] - s[i - 1]) * yc.discount(s[i]);
+  return (yc.discount(s.front()) - yc.discount(s.back())) / annuity;
+}
+
+// parseSwap (analyses.cpp:142-149): the fixed rate given, or the par rate on the domestic curve.
+void parseSwap(const Value& t, const Curve& yc, TradeData& d) {
+  d.payFixed = str(t, "direction", "payer") == "payer";
+  d.notional = num(t, "notional", 1e6);
+  d.schedule = vanillaSchedule(num(t, "start", 0.0), num(t, "tenor", 5.0), static_cast<int>(num(t, "freq", 2)));
+  d.fixedRate = has(t, "fixedRate") ? t["fixedRate"].asNumber("fixedRate") : parRate(d.schedule, yc);
+  d.maturity = d.schedule.back();
+}
+
+int assetIndex(const Env& env, const std::string& name) {
+  for (std::size_t i = 0; i < env.assets.size(); ++i)
+    if (env.assets[i].name == name) return static_cast<int>(i);
+  throw std::invalid_argument("unknown asset '" + name + "'");
+}
+
+// parseTrade (analyses.cpp:151-200): swap, forward, option, bermudan.
+TradeData parseTrade(const Value& t, const Env& env) {
+  TradeData d;
+  d.type = str(t, "type", "swap");
+  d.id = str(t, "id", d.type);
+  const Curve yc = env.domestic.curve();
+  if (d.type == "swap") {
+    d.kind = CCR_TRADE_SWAP;
+    parseSwap(t, yc, d);
+  } else if (d.type == "forward") {
+    d.kind = CCR_TRADE_ASSET_FORWARD;
+    d.asset = assetIndex(env, str(t, "asset", "FX"));
+    d.notional = num(t, "notional", 1e6);
+    d.fixedRate = num(t, "strike", 1.0);
+    d.maturity = num(t, "maturity", 1.0);
+  } else if (d.type == "option") {
+    d.kind = CCR_TRADE_EUROPEAN_OPTION;
+    d.asset = assetIndex(env, str(t, "asset", "FX"));
+    d.optionType = str(t, "optionType", "call") == "call" ? CCR_OPTION_CALL : CCR_OPTION_PUT;
+    d.notional = num(t, "notional", 1e6);
+    d.fixedRate = num(t, "strike", 1.0);
+    d.maturity = num(t, "expiry", 1.0);
+  } else if (d.type == "bermudan") {
+    d.kind = CCR_TRADE_BERMUDAN_SWAPTION;
+    parseSwap(t, yc, d);
+    if (has(t, "exercises")) {
+      d.exercises = t["exercises"].asNumbers("exercises");
+    } else {
+      d.exercises.assign(d.schedule.begin(), d.schedule.end() - 1);
+    }
+    d.cash = str(t, "settlement", "physical") != "physical";
+    d.position = num(t, "position", 1.0);
+    d.degree = static_cast<int>(num(t, "degree", 3));
+  } else {
+    throw std::invalid_argument("unknown trade type '" + d.type + "'");
+  }
+  return d;
+}
+
+// parseCsa (analyses.cpp:202-224).
+ccr_collateral parseCsa(const Value& c) {
+  ccr_collateral k{};
+  k.threshold_counterparty = num(c, "thresholdCounterparty", 0.0);
+  k.threshold_own = has(c, "thresholdOwn") ? c["thresholdOwn"].asNumber("thresholdOwn") : kInf;
+  if (k.threshold_own < 0.0) k.threshold_own = kInf;
+  k.minimum_transfer_amount = num(c, "mta", 0.0);
+  k.independent_amount = num(c, "independentAmount", 0.0);
+  k.margin_period_of_risk = num(c, "mpr", 10.0 / 250.0);
+  const std::string type = str(c, "collateral", "cash");
+  k.haircut = num(c, "haircut", 0.0);
+  k.type = CCR_COLLATERAL_CASH;
+  if (type == "bond") {
+    k.type = CCR_COLLATERAL_BOND;
+    k.bond_maturity = num(c, "bondMaturity", 10.0);
+  } else if (type == "asset") {
+    k.type = CCR_COLLATERAL_ASSET;
+    k.asset_index = static_cast<int32_t>(num(c, "collateralAssetIndex", 0));
+  } else if (type != "cash") {
+    throw std::invalid_argument("unknown collateral '" + type + "' (cash, bond or asset)");
+  }
+  return k;
+}
+
+// TimeGrid::standardExposureGrid (src/core/time_grid.cpp:30-39): weekly to 13 weeks, monthly
+// from 4 to 12 months, then quarterly from 1.25y, capped at and ending on the horizon.
+std::vector<double> standardGrid(double horizon) {
+  if (horizon <= 0.0) throw std::invalid_argument("TimeGrid::standardExposureGrid: horizon must be positive");
+  std::vector<double> t;
+  for (int w = 1; w <= 13; ++w) t.push_back(w * 7.0 / 365.0);
+  for (int m = 4; m <= 12; ++m) t.push_back(m / 12.0);
+  for (double q = 1.25; q < horizon + kGridTolerance; q += 0.25) t.push_back(q);
+  t.erase(std::remove_if(t.begin(), t.end(), [horizon](double x) { return x > horizon; }), t.end());
+  t.push_back(horizon);
+  return TimeGrid(t).t;
+}
+
+// parseEnv (analyses.cpp:226-290).
+Env parseEnv(const Value& spec) {
+  if (!spec.isObject()) throw std::invalid_argument("the spec must be an object");
+  Env env;
+  env.domestic = parseCurve(spec["domestic"], 0.03);
+  env.domestic.curve();  // validates
+  const Value& hw = spec["hw"];
+  env.a = num(hw, "a", 0.05);
+  env.sigma = num(hw, "sigma", 0.01);
+  const Curve yc = env.domestic.curve();
+  const Value& assets = spec["assets"];
+  for (std::size_t i = 0; i < assets.size(); ++i) {
+    const Value& a = assets.at(i);
+    env.assets.push_back({str(a, "name", "FX"), num(a, "spot", 1.0), num(a, "vol", 0.1), parseCurve(a["carry"], 0.01)});
+  }
+  const Value& credits = spec["credits"];
+  for (std::size_t i = 0; i < credits.size(); ++i) {
+    const Value& c = credits.at(i);
+    CreditData d;
+    d.recovery = num(c, "recovery", 0.4);
+    d.quotes = parseQuotes(c["quotes"]);
+    d.curve = bootstrapCredit(yc, d.quotes, d.recovery);
+    const Value& cir = c["cir"];
+    d.name = str(c, "name", "CPTY");
+    d.kappa = num(cir, "kappa", 0.4);
+    d.theta = num(cir, "theta", 0.02);
+    d.xi = num(cir, "xi", 0.08);
+    d.y0 = num(cir, "y0", 0.01);
+    d.substeps = static_cast<int>(num(cir, "substeps", 4));
+    env.credits.push_back(std::move(d));
+  }
+  const int nF = env.factors();
+  env.correlation.assign(static_cast<std::size_t>(nF * nF), 0.0);
+  for (int i = 0; i < nF; ++i) env.correlation[static_cast<std::size_t>(i * nF + i)] = 1.0;
+  if (has(spec, "correlation")) {
+    const Value& c = spec["correlation"];
+    if (static_cast<int>(c.size()) != nF) throw std::invalid_argument("correlation matrix has wrong dimension");
+    for (int i = 0; i < nF; ++i) {
+      if (static_cast<int>(c.at(static_cast<std::size_t>(i)).size()) != nF)
+        throw std::invalid_argument("correlation matrix has wrong dimension");
+      for (int j = 0; j < nF; ++j)
+        env.correlation[static_cast<std::size_t>(i * nF + j)] =
+            c.at(static_cast<std::size_t>(i)).at(static_cast<std::size_t>(j)).asNumber("correlation");
+    }
+  }
+  const Value& sim = spec["sim"];
+  const double paths = num(sim, "numPaths", 2000);
+  if (!(paths >= 1.0)) throw std::invalid_argument("SimulationConfig: numPaths must be positive");
+  env.sim.num_paths = static_cast<int64_t>(paths);
+  env.sim.seed = static_cast<uint64_t>(num(sim, "seed", 42));
+  env.sim.antithetic = has(sim, "antithetic") ? (sim["antithetic"].asBool("antithetic") ? 1 : 0) : 1;
+
+  const Value& grid = spec["grid"];
+  if (has(grid, "times")) {
+    env.baseGrid = TimeGrid(grid["times"].asNumbers("grid times")).t;
+  } else if (str(grid, "type", "standard") == "uniform") {
+    const double horizon = num(grid, "horizon", 5.0);
+    const double steps = num(grid, "steps", 20);
+    if (horizon <= 0.0 || steps < 1.0) throw std::invalid_argument("TimeGrid::uniform: invalid arguments");
+    std::vector<double> t;
+    const auto n = static_cast<std::size_t>(steps);
+    for (std::size_t i = 0; i <= n; ++i) t.push_back(horizon * static_cast<double>(i) / static_cast<double>(n));
+    env.baseGrid = TimeGrid(t).t;
+  } else {
+    env.baseGrid = standardGrid(num(grid, "horizon", 5.0));
+  }
+  if (has(spec, "csa") && !(has(spec["csa"], "enabled") && !spec["csa"]["enabled"].asBool("csa.enabled"))) {
+    env.hasCsa = true;
+    env.csa = parseCsa(spec["csa"]);
+  }
+  env.pfeQuantile = num(spec, "pfeQuantile", 0.95);
+  env.counterparty = static_cast<int>(num(spec, "counterparty", 0));
+  if (has(spec, "own")) {
+    env.hasOwn = true;
+    env.ownRecovery = num(spec["own"], "recovery", 0.4);
+    env.ownQuotes = parseQuotes(spec["own"]["quotes"]);
+    env.ownCurve = bootstrapCredit(yc, env.ownQuotes, env.ownRecovery);
+  }
+  const Value& trades = spec["trades"];
+  for (std::size_t i = 0; i < trades.size(); ++i) env.trades.push_back(parseTrade(trades.at(i), env));
+  return env;
+}
+
+// ------------------------------------------------------------------ engine jobs
+
+// The C structs of one environment's scenario generator (pointers into the Env).
+struct Model {
+  std::vector<ccr_lognormal_asset> assets;
+  std::vector<ccr_cir_intensity> credits;
+  std::vector<double> correlation;
+  ccr_scenario_generator gen{};
+  Model(const Env& env, const std::vector<double>& corr) : correlation(corr) {
+    for (const auto& a : env.assets) assets.push_back({a.spot, a.vol, a.carry.c(), -1});
+    for (const auto& c : env.credits)
+      credits.push_back({ccr_credit_curve{c.curve.p.data(), c.curve.h.data(), static_cast<int32_t>(c.curve.p.size())},
+                         c.kappa, c.theta, c.xi, c.y0, c.substeps});
+    gen.rates = ccr_hull_white{env.domestic.c(), env.a, env.sigma};
+    gen.num_assets = static_cast<int32_t>(assets.size());
+    gen.assets = assets.empty() ? nullptr : assets.data();
+    gen.num_credits = static_cast<int32_t>(credits.size());
+    gen.credits = credits.empty() ? nullptr : credits.data();
+    gen.correlation = correlation.data();
+  }
+  Model(const Model&) = delete;
+};
+
+// One netting set of a job: its trades (C structs into the Env's trade data) and its CSA.
+struct SetData {
+  std::string label;
+  std::vector<ccr_trade> trades;
+  bool hasCsa = false;
+  ccr_collateral csa{};
+  int counterparty = -1;
+  ccr_netting_set c() const {
+    ccr_netting_set s{};
+    s.num_trades = static_cast<int32_t>(trades.size());
+    s.trades = trades.empty() ? nullptr : trades.data();
+    s.has_collateral = hasCsa ? 1 : 0;
+    s.collateral = csa;
+    s.counterparty_credit = counterparty;
+    s.own_credit = -1;
+    return s;
+  }
+};
+
+SetData nettingSet(const Env& env, const std::string& label, bool withCsa) {
+  SetData s;
+  s.label = label;
+  for (const auto& t : env.trades) s.trades.push_back(t.c());
+  if (withCsa && env.hasCsa) {
+    s.hasCsa = true;
+    s.csa = env.csa;
+  }
+  s.counterparty = env.credits.empty() ? -1 : env.counterparty;
+  return s;
+}
+
+// What one netting set of an executed plan returns.
+struct Profile {
+  std::vector<double> times, ev, ee, ene, pfe, dee, dene, deflator, survival, cvaInc;
+  double pfeQuantile = 0.95;
+};
+struct TradeStat {
+  std::vector<double> marginalDiscounted, marginal, incrementalDiscounted, standaloneDiscounted, standalone;
+};
+struct NestedIm {
+  std::vector<double> discounted, mean, b05, b50, b95;
+};
+struct SetResult {
+  Profile profile;
+  int64_t numPaths = 0;
+  int simulationDates = 0;
+  std::vector<TradeStat> trades;
+  bool nested = false;
+  NestedIm im;
+  std::vector<double> imProxy;  // per reporting date (IM proxy plans)
+};
+
+// Options of the plans of one run.
+struct RunOptions {
+  bool book = false;
+  bool perTrade = false;
+  double imProxyQuantile = 0.0, imMpr = 10.0 / 250.0;  // IM proxy (quantile > 0)
+  int nestedInner = 0;
+  double nestedQuantile = 0.99;
+};
+
+void check(int32_t rc, ccr_hip_ctx* ctx, const char* what) {
+  if (rc == CCR_OK) return;
+  const char* m = ctx ? ccr_hip_last_error(ctx) : nullptr;
+  throw std::runtime_error(std::string(what) + ": " + (m && *m ? m : "error " + std::to_string(rc)));
+}
+
+// Resets the context's per-compile switches to the plain exposure setup.
+void resetContext(ccr_hip_ctx* ctx) {
+  ccr_hip_set_book(ctx, 0);
+  ccr_hip_set_book_grids(ctx, nullptr, 0, nullptr, 0);
+  ccr_hip_set_trade_statistics(ctx, 0);
+  ccr_hip_set_initial_margin(ctx, 0.0, 10.0 / 250.0);
+  ccr_hip_set_nested_initial_margin(ctx, 0, 0.99, 10.0 / 250.0);
+}
+
+// Compiles the netting sets of one generator into one plan, executes it once and reads every
+// set's profile (and the per-trade statistics / initial margin the options ask for).
+std::vector<SetResult> runSets(ccr_hip_ctx* ctx, const Env& env, const Model& model, const std::vector<SetData>& sets,
+                               const RunOptions& o) {
+  resetContext(ctx);
+  check(ccr_hip_set_book(ctx, o.book ? 1 : 0), ctx, "ccr_hip_set_book");
+  if (o.perTrade) check(ccr_hip_set_trade_statistics(ctx, 1), ctx, "ccr_hip_set_trade_statistics");
+  if (o.imProxyQuantile > 0.0)
+    check(ccr_hip_set_initial_margin(ctx, o.imProxyQuantile, o.imMpr), ctx, "ccr_hip_set_initial_margin");
+  if (o.nestedInner > 0)
+    check(ccr_hip_set_nested_initial_margin(ctx, o.nestedInner, o.nestedQuantile, o.imMpr), ctx,
+          "ccr_hip_set_nested_initial_margin");
+  std::vector<ccr_netting_set> cs;
+  for (const auto& s : sets) cs.push_back(s.c());
+  ccr_hip_plan* plan = nullptr;
+  const int32_t rc = ccr_hip_compile(ctx, &model.gen, &env.sim, env.baseGrid.data(),
+                                     static_cast<int32_t>(env.baseGrid.size()), cs.data(),
+                                     static_cast<int32_t>(cs.size()), env.pfeQuantile, CCR_FP64, &plan);
+  if (rc == CCR_ERR_UNSUPPORTED) {
+    resetContext(ctx);
+    throw std::domain_error(ccr_hip_last_error(ctx));
+  }
+  check(rc, ctx, "compile");
+  std::unique_ptr<ccr_hip_plan, void (*)(ccr_hip_plan*)> guard(plan, ccr_hip_plan_destroy);
+  ccr_hip_plan_info info{};
+  check(ccr_hip_plan_info_get(plan, &info), ctx, "plan info");
+  const int cap = info.max_reporting_dates;
+  constexpr int kFields = 11;
+  std::vector<std::vector<double>> store(sets.size(), std::vector<double>(static_cast<std::size_t>(cap * kFields)));
+  std::vector<ccr_exposure_out> outs(sets.size());
+  for (std::size_t s = 0; s < sets.size(); ++s) {
+    double* b = store[s].data();
+    ccr_exposure_out& x = outs[s];
+    x = ccr_exposure_out{};
+    x.capacity = cap;
+    double** fields[kFields] = {&x.times, &x.expected_value, &x.expected_exposure, &x.expected_negative_exposure,
+                                &x.potential_future_exposure, &x.discounted_expected_exposure,
+                                &x.discounted_expected_negative_exposure, &x.mean_deflator, &x.mean_survival,
+                                &x.cva_increments, &x.dva_increments};
+    for (int f = 0; f < kFields; ++f) *fields[f] = b + f * cap;
+  }
+  check(ccr_hip_execute(ctx, plan, outs.data()), ctx, "execute");
+  const int32_t nbase = static_cast<int32_t>(env.baseGrid.size());
+  auto gridOf = [&](const ccr_netting_set& set) {
+    int32_t size = 0;
+    check(ccr_hip_simulation_grid(env.baseGrid.data(), nbase, &set, nullptr, 0, &size), nullptr, "simulation grid");
+    std::vector<double> g(static_cast<std::size_t>(size));
+    check(ccr_hip_simulation_grid(env.baseGrid.data(), nbase, &set, g.data(), size, &size), nullptr, "simulation grid");
+    return g;
+  };
+  std::vector<double> bookSim;
+  if (o.book) {
+    int32_t nr = 0, ns = 0;
+    check(ccr_hip_book_grids(env.baseGrid.data(), nbase, cs.data(), static_cast<int32_t>(cs.size()), nullptr, 0, &nr,
+                             nullptr, 0, &ns), nullptr, "book grids");
+    bookSim.resize(static_cast<std::size_t>(ns));
+    check(ccr_hip_book_grids(env.baseGrid.data(), nbase, cs.data(), static_cast<int32_t>(cs.size()), nullptr, 0, &nr,
+                             bookSim.data(), ns, &ns), nullptr, "book grids");
+  }
+  std::vector<SetResult> res(sets.size());
+  for (std::size_t s = 0; s < sets.size(); ++s) {
+    const ccr_exposure_out& x = outs[s];
+    const auto n = static_cast<std::size_t>(x.num_dates);
+    auto take = [n](const double* p) { return std::vector<double>(p, p + n); };
+    Profile& pr = res[s].profile;
+    pr.times = take(x.times);
+    pr.ev = take(x.expected_value);
+    pr.ee = take(x.expected_exposure);
+    pr.ene = take(x.expected_negative_exposure);
+    pr.pfe = take(x.potential_future_exposure);
+    pr.dee = take(x.discounted_expected_exposure);
+    pr.dene = take(x.discounted_expected_negative_exposure);
+    pr.deflator = take(x.mean_deflator);
+    pr.survival = take(x.mean_survival);
+    pr.cvaInc = take(x.cva_increments);
+    pr.pfeQuantile = env.pfeQuantile;
+    res[s].numPaths = info.num_paths;
+    res[s].simulationDates = static_cast<int>(o.book ? bookSim.size() : gridOf(cs[s]).size());
+    if (o.perTrade) {
+      for (std::size_t i = 0; i < sets[s].trades.size(); ++i) {
+        TradeStat t;
+        for (auto* v : {&t.marginalDiscounted, &t.marginal, &t.incrementalDiscounted, &t.standaloneDiscounted,
+                        &t.standalone})
+          v->assign(n, 0.0);
+        ccr_trade_statistics ts{};
+        ts.capacity = static_cast<int32_t>(n);
+        ts.marginal_discounted = t.marginalDiscounted.data();
+        ts.marginal = t.marginal.data();
+        ts.incremental_discounted = t.incrementalDiscounted.data();
+        ts.standalone_discounted = t.standaloneDiscounted.data();
+        ts.standalone = t.standalone.data();
+        check(ccr_hip_trade_statistics(ctx, plan, static_cast<int32_t>(s), static_cast<int32_t>(i), &ts), ctx,
+              "trade statistics");
+        res[s].trades.push_back(std::move(t));
+      }
+    }
+    if (o.nestedInner > 0) {
+      NestedIm& im = res[s].im;
+      for (auto* v : {&im.discounted, &im.mean, &im.b05, &im.b50, &im.b95}) v->assign(n, 0.0);
+      ccr_nested_im_out no{};
+      no.capacity = static_cast<int32_t>(n);
+      no.discounted = im.discounted.data();
+      no.mean = im.mean.data();
+      no.band05 = im.b05.data();
+      no.band50 = im.b50.data();
+      no.band95 = im.b95.data();
+      check(ccr_hip_nested_initial_margin(ctx, plan, static_cast<int32_t>(s), &no), ctx, "nested initial margin");
+      res[s].nested = true;
+    }
+    if (o.imProxyQuantile > 0.0) {
+      // The proxy on the set's simulation grid (quantile x sqrt(mpor / dt) x E[D]; a book's
+      // union grid in book plans), read on the reporting dates (0 at t = 0).
+      const std::vector<double>& sim = o.book ? bookSim : gridOf(cs[s]);
+      std::vector<double> margin(sim.size() + 1);
+      int32_t size = 0;
+      check(ccr_hip_initial_margin(ctx, plan, static_cast<int32_t>(s), margin.data(),
+                                   static_cast<int32_t>(margin.size()), &size), ctx, "initial margin");
+      if (static_cast<std::size_t>(size) != sim.size())
+        throw std::runtime_error("initial margin: the profile does not match the simulation grid");
+      const TimeGrid g(sim);
+      res[s].imProxy.assign(n, 0.0);
+      for (std::size_t k = 1; k < n; ++k) {
+        const auto j = g.find(pr.times[k]);
+        if (j) res[s].imProxy[k] = margin[*j];
+      }
+    }
+  }
+  resetContext(ctx);
+  return res;
+}
+
+// ------------------------------------------------------------------ result builders
+
+double epe(const Profile& p, double horizon) {
+  return ccr_expected_positive_exposure(p.times.data(), p.ee.data(), static_cast<int32_t>(p.times.size()), horizon);
+}
+double eepe(const Profile& p, double horizon) {
+  return ccr_effective_expected_positive_exposure(p.times.data(), p.ee.data(), static_cast<int32_t>(p.times.size()),
+                                                  horizon);
+}
+std::vector<double> effectiveEe(const Profile& p) {
+  std::vector<double> e(p.ee.size());
+  if (!e.empty()) ccr_effective_expected_exposure(p.ee.data(), static_cast<int32_t>(e.size()), e.data());
+  return e;
+}
+
+// profileToJs (analyses.cpp:294-311).
+Value profileToJs(const Profile& p) {
+  Value o = Value::object();
+  o.setNumbers("times", p.times);
+  o.setNumbers("expectedValue", p.ev);
+  o.setNumbers("ee", p.ee);
+  o.setNumbers("ene", p.ene);
+  o.setNumbers("pfe", p.pfe);
+  o.setNumbers("discountedEe", p.dee);
+  o.setNumbers("discountedEne", p.dene);
+  o.setNumbers("effectiveEe", effectiveEe(p));
+  o.set("pfeQuantile", p.pfeQuantile);
+  o.set("epe1y", epe(p, 1.0));
+  o.set("eepe1y", eepe(p, 1.0));
+  o.set("epeLife", epe(p, p.times.empty() ? 0.0 : p.times.back()));
+  o.set("maxPfe", p.pfe.empty() ? 0.0 : *std::max_element(p.pfe.begin(), p.pfe.end()));
+  return o;
+}
+
+// totalProfileToJs (analyses.cpp:1786-1799): no quantiles.
+Value totalProfileToJs(const Profile& p) {
+  Value o = Value::object();
+  o.setNumbers("times", p.times);
+  o.setNumbers("expectedValue", p.ev);
+  o.setNumbers("ee", p.ee);
+  o.setNumbers("ene", p.ene);
+  o.setNumbers("discountedEe", p.dee);
+  o.setNumbers("discountedEne", p.dene);
+  o.setNumbers("effectiveEe", effectiveEe(p));
+  o.set("epe1y", epe(p, 1.0));
+  o.set("eepe1y", eepe(p, 1.0));
+  o.set("epeLife", epe(p, p.times.empty() ? 0.0 : p.times.back()));
+  return o;
+}
+
+ccr_credit_curve curveC(const Credit& c) {
+  return ccr_credit_curve{c.p.data(), c.h.data(), static_cast<int32_t>(c.p.size())};
+}
+
+// cvaTermStructure (src/cva/cva.cpp:11-20): (1 - R) 1/2 (EE*_{i-1} + EE*_i) (Q(t_{i-1}) - Q(t_i)).
+std::vector<double> cvaTerms(const std::vector<double>& times, const std::vector<double>& dee, const Credit& cc,
+                             double recovery) {
+  std::vector<double> terms(times.size(), 0.0);
+  for (std::size_t i = 1; i < times.size(); ++i)
+    terms[i] = (1.0 - recovery) * (0.5 * (dee[i - 1] + dee[i])) * (cc.survival(times[i - 1]) - cc.survival(times[i]));
+  return terms;
+}
+double unilateralCva(const std::vector<double>& times, const std::vector<double>& dee, const Credit& cc,
+                     double recovery) {
+  const ccr_credit_curve c = curveC(cc);
+  return ccr_unilateral_cva(times.data(), dee.data(), static_cast<int32_t>(times.size()), &c, recovery);
+}
+
+// FusedResult::pathwiseCva (fused_exposure.cpp:888-892): (1 - R) x the sum of the increments.
+double pathwiseCva(const Profile& p, double recovery) {
+  double s = 0.0;
+  for (std::size_t k = 1; k < p.cvaInc.size(); ++k) s += p.cvaInc[k];
+  return (1.0 - recovery) * s;
+}
+
+// E[D V+ | default in period k] (fused_exposure.cpp:851-854): the increment over the simulated
+// default probability E[Q_{k-1}] - E[Q_k]; NaN at k = 0 and where that probability is 0.
+std::vector<double> conditionalDee(const Profile& p) {
+  std::vector<double> out;
+  for (std::size_t k = 0; k < p.cvaInc.size(); ++k) {
+    const double pd = k ? p.survival[k - 1] - p.survival[k] : 0.0;
+    out.push_back(k && pd > 0 ? p.cvaInc[k] / pd : kNaN);
+  }
+  return out;
+}
+
+Value exposureResult(const Env& env, const SetResult& r) {  // combine, "exposure" (:1309-1342)
+  Value out = Value::object();
+  out.set("profile", profileToJs(r.profile));
+  out.set("numPaths", static_cast<double>(r.numPaths));
+  out.set("simulationDates", static_cast<double>(r.simulationDates));
+  out.set("reportingDates", static_cast<double>(r.profile.times.size()));
+  out.setBool("collateralised", env.hasCsa);
+  out.set("trades", Value::array());
+  return out;
+}
+
+// cvaAnalytics (analyses.cpp:801-852) plus the pathwise CVA of the "cva" analysis (:1343-1350).
+Value cvaResult(const Value& spec, const Env& env, const SetResult& r) {
+  const CreditData& cp = env.cpty();
+  const Credit& cc = cp.curve;
+  const double rec = cp.recovery;
+  const Profile& p = r.profile;
+  Value out = Value::object();
+  out.set("profile", profileToJs(p));
+  const double cva = unilateralCva(p.times, p.dee, cc, rec);
+  out.set("cva", cva);
+  const std::vector<double> terms = cvaTerms(p.times, p.dee, cc, rec);
+  std::vector<double> cumulative;
+  double running = 0.0;
+  for (double t : terms) cumulative.push_back(running += t);
+  out.setNumbers("termStructure", terms);
+  out.setNumbers("cumulativeCva", cumulative);
+  std::vector<double> surv, own;
+  for (double t : p.times) {
+    surv.push_back(cc.survival(t));
+    if (env.hasOwn) own.push_back(env.ownCurve.survival(t));
+  }
+  out.setNumbers("counterpartySurvival", surv);
+  if (env.hasOwn) {
+    const ccr_credit_curve c = curveC(cc), o = curveC(env.ownCurve);
+    double bc = 0.0, bd = 0.0;
+    ccr_bilateral_cva(p.times.data(), p.dee.data(), p.dene.data(), static_cast<int32_t>(p.times.size()), &c, rec, &o,
+                      env.ownRecovery, &bc, &bd);
+    Value b = Value::object();
+    b.set("cva", bc);
+    b.set("dva", bd);
+    b.set("total", bc - bd);
+    out.set("bilateral", b);
+    out.setNumbers("ownSurvival", own);
+  }
+  double maturity = 0.0, notional = 0.0;
+  for (const auto& t : env.trades) {
+    maturity = std::max(maturity, t.exposureMaturity());
+    notional = std::max(notional, std::fabs(t.notional));
+  }
+  notional = num(spec, "spreadNotional", notional);
+  if (maturity > 0.0 && notional > 0.0) {
+    const double annuity = premiumAnnuity(env.domestic.curve(), cc, maturity);
+    if (annuity <= 0.0) throw std::invalid_argument("cvaRunningSpread: degenerate annuity");
+    out.set("runningSpread", cva / (std::fabs(notional) * annuity));  // cvaRunningSpread (cva.cpp:50-55)
+    out.set("spreadNotional", notional);
+    out.set("spreadMaturity", maturity);
+  }
+  std::vector<double> recs, cvas;
+  for (int i = 0; i <= 10; ++i) {
+    recs.push_back(i / 10.0);
+    cvas.push_back(unilateralCva(p.times, p.dee, cc, recs.back()));
+  }
+  out.setNumbers("recoveryGrid", recs);
+  out.setNumbers("cvaByRecovery", cvas);
+  const double pw = pathwiseCva(p, rec);
+  out.set("pathwiseCva", pw);
+  out.set("pathwiseExcess", pw / cva - 1.0);
+  return out;
+}
+
+// parseXva (analyses.cpp:512-525).
+ccr_xva_parameters parseXva(const Value& spec) {
+  ccr_xva_parameters p;
+  ccr_xva_defaults(&p);
+  if (!has(spec, "xva")) return p;
+  const Value& x = spec["xva"];
+  p.funding_spread_borrow = num(x, "fundingSpreadBorrow", p.funding_spread_borrow);
+  p.funding_spread_lend = num(x, "fundingSpreadLend", p.funding_spread_lend);
+  p.margin_funding_spread = num(x, "marginFundingSpread", p.margin_funding_spread);
+  p.hurdle_rate = num(x, "hurdleRate", p.hurdle_rate);
+  p.capital_ratio = num(x, "capitalRatio", p.capital_ratio);
+  p.risk_weight = num(x, "riskWeight", p.risk_weight);
+  p.alpha = num(x, "alpha", p.alpha);
+  return p;
+}
+
+ccr_xva_result xvaOf(const Profile& p, std::size_t k, const std::vector<double>* im, const Credit& cc, double rec,
+                     const Credit& own, double ownRec, const ccr_xva_parameters& params) {
+  const ccr_credit_curve c = curveC(cc), o = curveC(own);
+  ccr_xva_result x{};
+  check(ccr_xva(p.times.data(), p.dee.data(), p.dene.data(), static_cast<int32_t>(k), im ? im->data() : nullptr, &c,
+                rec, &o, ownRec, &params, &x),
+        nullptr, "xva");
+  return x;
+}
+
+Value xvaToJs(const ccr_xva_result& x) {  // analyses.cpp:532-545
+  Value o = Value::object();
+  o.set("cva", x.cva);
+  o.set("dva", x.dva);
+  o.set("fca", x.fca);
+  o.set("fba", x.fba);
+  o.set("fva", x.fca - x.fba);
+  o.set("mva", x.mva);
+  o.set("kva", x.kva);
+  o.set("total", x.cva - x.dva + x.fca - x.fba + x.mva + x.kva);
+  return o;
+}
+
+// xvaResult (analyses.cpp:950-1057), without the CS01 sensitivities. The initial margin is the
+// nested conditional IM where the job ran it, else the proxy.
+Value xvaResult(const Value& spec, const Env& env, const SetResult& r) {
+  const CreditData& cp = env.cpty();
+  const Credit& cc = cp.curve;
+  const double rec = cp.recovery;
+  const Credit own = env.ownOrRiskless();
+  const ccr_xva_parameters params = parseXva(spec);
+  const Profile& p = r.profile;
+  const std::size_t n = p.times.size();
+  const std::vector<double> proxy = r.imProxy.empty() ? std::vector<double>(n, 0.0) : r.imProxy;
+  const std::vector<double> im = r.nested ? r.im.discounted : proxy;
+  const ccr_xva_result total = xvaOf(p, n, &im, cc, rec, own, env.ownRecovery, params);
+  Value out = Value::object();
+  out.set("profile", profileToJs(p));
+  out.setNumbers("initialMargin", im);
+  out.set("imMethod", r.nested ? "nested" : "proxy");
+  out.setNumbers("initialMarginProxy", proxy);
+  out.set("mvaProxy", xvaOf(p, n, &proxy, cc, rec, own, env.ownRecovery, params).mva);
+  if (r.nested) {
+    out.set("imInnerPaths", num(spec, "imInnerPaths", 128));
+    out.setNumbers("initialMarginMean", r.im.mean);
+    Value bands = Value::array();
+    bands.push(arr(r.im.b05));
+    bands.push(arr(r.im.b50));
+    bands.push(arr(r.im.b95));
+    out.set("initialMarginBands", bands);
+  }
+  out.set("xva", xvaToJs(total));
+  const double t = total.cva - total.dva + total.fca - total.fba + total.mva + total.kva;
+  out.setBool("identityHolds", std::fabs(total.cva - total.dva + total.fca - total.fba + total.mva + total.kva - t) <=
+                                   1e-6 * std::max(1.0, std::fabs(t)));
+  out.set("numPaths", static_cast<double>(r.numPaths));
+  auto part = [&](double ccr_xva_result::*field) {
+    std::vector<double> c(n, 0.0);
+    for (std::size_t k = 1; k < n; ++k) c[k] = xvaOf(p, k + 1, &im, cc, rec, own, env.ownRecovery, params).*field;
+    return c;
+  };
+  out.setNumbers("cumulativeCva", part(&ccr_xva_result::cva));
+  out.setNumbers("cumulativeFca", part(&ccr_xva_result::fca));
+  out.setNumbers("cumulativeMva", part(&ccr_xva_result::mva));
+  out.setNumbers("cumulativeKva", part(&ccr_xva_result::kva));
+  std::vector<double> spreads, fva, hurdles, kva;
+  for (int i = 0; i <= 10; ++i) {
+    ccr_xva_parameters q = params;
+    q.funding_spread_borrow = q.funding_spread_lend = 0.0005 * i;
+    spreads.push_back(q.funding_spread_borrow);
+    const ccr_xva_result a = xvaOf(p, n, nullptr, cc, rec, own, env.ownRecovery, q);
+    fva.push_back(a.fca - a.fba);
+    q = params;
+    q.hurdle_rate = 0.02 * i;
+    hurdles.push_back(q.hurdle_rate);
+    kva.push_back(xvaOf(p, n, nullptr, cc, rec, own, env.ownRecovery, q).kva);
+  }
+  out.setNumbers("spreadGrid", spreads);
+  out.setNumbers("fvaBySpread", fva);
+  out.setNumbers("hurdleGrid", hurdles);
+  out.setNumbers("kvaByHurdle", kva);
+  Value trades = Value::array();
+  if (!r.trades.empty()) {
+    std::vector<double> contrib;
+    for (const auto& t : r.trades) contrib.insert(contrib.end(), t.marginalDiscounted.begin(), t.marginalDiscounted.end());
+    std::vector<double> parts(3 * r.trades.size());
+    const ccr_credit_curve c = curveC(cc), o = curveC(own);
+    check(ccr_allocate_exposure_xva(p.times.data(), contrib.data(), static_cast<int32_t>(r.trades.size()),
+                                    static_cast<int32_t>(n), &c, rec, &o, &params, parts.data()),
+          nullptr, "allocateExposureXva");
+    for (std::size_t i = 0; i < r.trades.size(); ++i) {
+      Value t = Value::object();
+      t.set("id", env.trades[i].id);
+      t.set("cva", parts[3 * i]);
+      t.set("fca", parts[3 * i + 1]);
+      t.set("kva", parts[3 * i + 2]);
+      trades.push(t);
+    }
+  }
+  out.set("trades", trades);
+  return out;
+}
+
+// allocationResult (analyses.cpp:1059-1106).
+Value allocationResult(const Value& spec, const Env& env, const SetResult& r) {
+  const CreditData& cp = env.cpty();
+  const Credit& cc = cp.curve;
+  const double rec = cp.recovery;
+  const Credit own = env.ownOrRiskless();
+  const ccr_xva_parameters params = parseXva(spec);
+  const Profile& p = r.profile;
+  const std::size_t n = p.times.size();
+  const ccr_credit_curve c = curveC(cc), o = curveC(own);
+  std::vector<double> contrib, parts(3 * r.trades.size());
+  for (const auto& t : r.trades) contrib.insert(contrib.end(), t.marginalDiscounted.begin(), t.marginalDiscounted.end());
+  if (!r.trades.empty())
+    check(ccr_allocate_exposure_xva(p.times.data(), contrib.data(), static_cast<int32_t>(r.trades.size()),
+                                    static_cast<int32_t>(n), &c, rec, &o, &params, parts.data()),
+          nullptr, "allocateExposureXva");
+  double cfk[3] = {0, 0, 0};
+  check(ccr_exposure_xva(p.times.data(), p.dee.data(), static_cast<int32_t>(n), &c, rec, &o, &params, cfk), nullptr,
+        "exposureXva");
+  const double totalCva = unilateralCva(p.times, p.dee, cc, rec);
+  Value out = Value::object();
+  out.setNumbers("times", p.times);
+  out.set("totalCva", totalCva);
+  out.set("totalFca", cfk[1]);
+  out.set("totalKva", cfk[2]);
+  out.setNumbers("totalEe", p.ee);
+  out.set("numPaths", static_cast<double>(r.numPaths));
+  double marginalSum = 0.0, incrementalSum = 0.0, standaloneSum = 0.0;
+  Value trades = Value::array();
+  for (std::size_t i = 0; i < r.trades.size(); ++i) {
+    const TradeStat& t = r.trades[i];
+    const double m = unilateralCva(p.times, t.marginalDiscounted, cc, rec);
+    const double inc = unilateralCva(p.times, t.incrementalDiscounted, cc, rec);
+    const double sa = unilateralCva(p.times, t.standaloneDiscounted, cc, rec);
+    marginalSum += m;
+    incrementalSum += inc;
+    standaloneSum += sa;
+    Value x = Value::object();
+    x.set("id", env.trades[i].id);
+    x.setNumbers("marginalEe", t.marginal);
+    x.set("marginalCva", m);
+    x.set("marginalShare", totalCva != 0.0 ? m / totalCva : kNaN);
+    x.set("marginalFca", parts[3 * i + 1]);
+    x.set("marginalKva", parts[3 * i + 2]);
+    x.set("incrementalCva", inc);
+    x.set("standaloneCva", sa);
+    x.setNumbers("standaloneEe", t.standalone);
+    trades.push(x);
+  }
+  out.set("trades", trades);
+  out.set("marginalCvaSum", marginalSum);
+  out.set("incrementalCvaSum", incrementalSum);
+  out.set("standaloneCvaSum", standaloneSum);
+  out.set("nettingSaving", standaloneSum != 0.0 ? 1.0 - totalCva / standaloneSum : kNaN);
+  out.setBool("addsUp", std::fabs(marginalSum - totalCva) < 1e-4 * std::fabs(totalCva));
+  return out;
+}
+
+Value csaToJs(bool enabled, const ccr_collateral& c) {  // analyses.cpp:924-939
+  Value o = Value::object();
+  o.setBool("enabled", enabled);
+  if (enabled) {
+    o.set("thresholdCounterparty", c.threshold_counterparty);
+    o.set("thresholdOwn", std::isfinite(c.threshold_own) ? c.threshold_own : -1.0);
+    o.setBool("oneWay", !std::isfinite(c.threshold_own));
+    o.set("mta", c.minimum_transfer_amount);
+    o.set("mprDays", c.margin_period_of_risk * 250.0);
+    o.set("collateral", c.type == CCR_COLLATERAL_CASH   ? std::string("cash")
+                        : c.type == CCR_COLLATERAL_BOND ? "bond " + std::to_string(static_cast<int>(c.bond_maturity)) + "y"
+                                                        : std::string("asset"));
+    o.set("haircut", c.haircut);
+  }
+  return o;
+}
+
+// ------------------------------------------------------------------ analyses
+
+std::string analysisOf(const Value& spec) { return str(spec, "analysis", "exposure"); }
+
+// The engine's per-trade statistics cover up to 16 trades without Bermudans.
+bool perTradeFits(const Env& env) {
+  if (env.trades.size() > 16) return false;
+  for (const auto& t : env.trades)
+    if (t.kind == CCR_TRADE_BERMUDAN_SWAPTION) return false;
+  return true;
+}
+
+Value analyseSingle(ccr_hip_ctx* ctx, const Value& spec, const std::string& analysis) {
+  const Env env = parseEnv(spec);
+  if (env.trades.empty()) throw std::invalid_argument("ExposureEngine::run: empty netting set");
+  const Model model(env, env.correlation);
+  if (analysis == "exposure") {
+    RunOptions o;
+    return exposureResult(env, runSets(ctx, env, model, {nettingSet(env, "base", true)}, o)[0]);
+  }
+  if (analysis == "cva") {
+    RunOptions o;
+    return cvaResult(spec, env, runSets(ctx, env, model, {nettingSet(env, "base", true)}, o)[0]);
+  }
+  if (analysis == "xva") {
+    // xvaOptions (analyses.cpp:647-668): per-trade statistics for the allocation (where the
+    // engine has them) and the nested conditional IM unless spec.imMethod is "proxy" (or the
+    // set holds Bermudans). The nested IM extends the simulation grid, so it runs in a plan of
+    // its own; the profile and the allocation come from one plan.
+    RunOptions base;
+    base.perTrade = perTradeFits(env);
+    SetResult r = runSets(ctx, env, model, {nettingSet(env, "base", true)}, base)[0];
+    RunOptions im;
+    im.imMpr = num(spec, "imMprDays", 10.0) / 250.0;
+    const double q = num(spec, "imQuantile", 0.99);
+    bool nestable = str(spec, "imMethod", "nested") == "nested";
+    for (const auto& t : env.trades) nestable = nestable && t.kind != CCR_TRADE_BERMUDAN_SWAPTION;
+    if (nestable) {
+      im.nestedInner = static_cast<int>(num(spec, "imInnerPaths", 128));
+      im.nestedQuantile = q;
+      r.im = runSets(ctx, env, model, {nettingSet(env, "base", true)}, im)[0].im;
+      r.nested = true;
+    } else {
+      im.imProxyQuantile = q;
+      r.imProxy = runSets(ctx, env, model, {nettingSet(env, "base", true)}, im)[0].imProxy;
+    }
+    return xvaResult(spec, env, r);
+  }
+  if (analysis == "allocation") {
+    // The allocation of the uncollateralised set (analyses.cpp:758-760).
+    if (!perTradeFits(env))
+      throw std::domain_error("per-trade statistics cover at most 16 trades without Bermudan swaptions");
+    RunOptions o;
+    o.perTrade = true;
+    return allocationResult(spec, env, runSets(ctx, env, model, {nettingSet(env, "base", false)}, o)[0]);
+  }
+  if (analysis == "collateral") {
+    // csaVariants (analyses.cpp:598-620), all variants in one plan (same scenarios).
+    ccr_collateral c = has(spec, "csa") ? parseCsa(spec["csa"]) : ccr_collateral{};
+    if (!has(spec, "csa")) {
+      c.threshold_own = kInf;
+      c.margin_period_of_risk = 10.0 / 250.0;
+    }
+    ccr_collateral zero = c, mpr20 = c, oneWay = c, bond = c, asset = c;
+    zero.threshold_counterparty = zero.threshold_own = zero.minimum_transfer_amount = 0.0;
+    mpr20.margin_period_of_risk = 20.0 / 250.0;
+    oneWay.threshold_own = kInf;
+    bond.type = CCR_COLLATERAL_BOND;
+    bond.haircut = 0.02;
+    bond.bond_maturity = std::max(10.0, num(spec["grid"], "horizon", 10.0));
+    std::vector<std::pair<std::string, std::pair<bool, ccr_collateral>>> variants = {
+        {"no CSA", {false, c}},         {"specified CSA", {true, c}}, {"zero threshold & MTA", {true, zero}},
+        {"MPR 20 days", {true, mpr20}}, {"one-way (we never post)", {true, oneWay}},
+        {"bond collateral (2% haircut)", {true, bond}}};
+    if (!env.assets.empty()) {
+      asset.type = CCR_COLLATERAL_ASSET;
+      asset.haircut = 0.15;
+      asset.asset_index = 0;
+      variants.push_back({env.assets[0].name + " collateral (15% haircut)", {true, asset}});
+    }
+    std::vector<SetData> sets;
+    for (const auto& v : variants) {
+      SetData s = nettingSet(env, v.first, false);
+      s.hasCsa = v.second.first;
+      s.csa = v.second.second;
+      sets.push_back(std::move(s));
+    }
+    RunOptions o;
+    const auto res = runSets(ctx, env, model, sets, o);
+    const double noCsa = eepe(res[0].profile, 1.0);
+    Value list = Value::array();
+    for (std::size_t i = 0; i < res.size(); ++i) {
+      Value x = Value::object();
+      x.set("name", variants[i].first);
+      x.set("csa", csaToJs(variants[i].second.first, variants[i].second.second));
+      x.set("profile", profileToJs(res[i].profile));
+      x.set("eepeVsNoCsa", noCsa > 0.0 ? eepe(res[i].profile, 1.0) / noCsa : kNaN);
+      list.push(x);
+    }
+    Value out = Value::object();
+    out.set("variants", list);
+    return out;
+  }
+  if (analysis == "wrongWayRisk") {
+    // analyses.cpp:761-769 and :1360-1383: the counterparty intensity's correlation with one
+    // driver (spec.wwrFactor, default the first asset, else the rate) swept over spec.rhos.
+    if (env.credits.empty()) throw std::invalid_argument("wrong-way risk needs a stochastic counterparty intensity");
+    const int nF = env.factors();
+    const int driver = static_cast<int>(num(spec, "wwrFactor", env.assets.empty() ? 0 : 1));
+    const int creditFactor = 1 + static_cast<int>(env.assets.size()) + env.counterparty;
+    if (driver < 0 || driver >= nF || creditFactor >= nF) throw std::invalid_argument("wrong-way risk: factor out of range");
+    const std::vector<double> rhos = has(spec, "rhos") ? spec["rhos"].asNumbers("rhos")
+                                                       : std::vector<double>{-0.6, -0.3, 0.0, 0.3, 0.6};
+    const CreditData& cp = env.cpty();
+    Value list = Value::array();
+    for (double rho : rhos) {
+      std::vector<double> corr = env.correlation;
+      corr[static_cast<std::size_t>(driver * nF + creditFactor)] = rho;
+      corr[static_cast<std::size_t>(creditFactor * nF + driver)] = rho;
+      Value x = Value::object();
+      x.set("rho", rho);
+      try {
+        const Model m(env, corr);
+        RunOptions o;
+        const SetResult r = runSets(ctx, env, m, {nettingSet(env, "base", true)}, o)[0];
+        const double independent = unilateralCva(r.profile.times, r.profile.dee, cp.curve, cp.recovery);
+        const double pw = pathwiseCva(r.profile, cp.recovery);
+        x.set("pathwiseCva", pw);
+        x.set("independentCva", independent);
+        x.set("multiplier", pw / independent);
+        x.setNumbers("times", r.profile.times);
+        x.setNumbers("discountedEe", r.profile.dee);
+        x.setNumbers("conditionalDiscountedEe", conditionalDee(r.profile));
+      } catch (const std::exception& e) {
+        x.set("error", std::string(e.what()));
+      }
+      list.push(x);
+    }
+    Value out = Value::object();
+    out.set("results", list);
+    out.set("driver", static_cast<double>(driver));
+    return out;
+  }
+  throw std::invalid_argument("unknown analysis '" + analysis + "'");
+}
+
+// ------------------------------------------------------------------ portfolio book
+
+struct BookPart {
+  std::string id, counterparty;
+  int credit = -1;
+  std::size_t trades = 0;
+  bool collateralised = false;
+};
+
+// A book (spec.scope = "book", spec.book.nettingSets = [{id, counterparty, csa, trades}]) of
+// "exposure", "cva", "xva" or "allocation", compiled as one book plan: every netting set on the
+// union grids and the same scenarios (bookParts / compileBook, analyses.cpp:1671-1766), each
+// with its own counterparty's simulated intensity (the generator holds every credit that is
+// some set's counterparty, each its own factor; the reference's book gives each set a
+// generator with only its own credit, as the last factor), totals by sumProfiles and by
+// counterparty (bookResult, :1817-2023).
+Value analyseBook(ccr_hip_ctx* ctx, const Value& spec) {
+  const std::string analysis = analysisOf(spec);
+  static const std::vector<std::string> portfolio = {"exposure", "collateral", "allocation", "cva",
+                                                     "xva",      "wrongWayRisk", "hedging", "instruments"};
+  if (std::find(portfolio.begin(), portfolio.end(), analysis) == portfolio.end())
+    throw std::invalid_argument("the " + analysis + " analysis does not run on a portfolio");
+  if (analysis != "exposure" && analysis != "cva" && analysis != "xva" && analysis != "allocation")
+    throw std::domain_error("the " + analysis + " analysis of a portfolio book is not provided by this front door");
+  const Value& nets = spec["book"]["nettingSets"];
+  if (nets.size() == 0) throw std::invalid_argument("the portfolio has no netting sets");
+  const Value& credits = spec["credits"];
+  std::vector<BookPart> parts;
+  for (std::size_t i = 0; i < nets.size(); ++i) {
+    const Value& ns = nets.at(i);
+    BookPart p;
+    p.id = str(ns, "id", "netting set " + std::to_string(i + 1));
+    p.trades = ns["trades"].size();
+    if (p.trades == 0) throw std::invalid_argument("netting set '" + p.id + "' has no trades");
+    // creditOf (analyses.cpp:1629-1641): the counterparty by credit name or index.
+    if (credits.size() > 0) {
+      const Value& c = ns["counterparty"];
+      if (c.isString()) {
+        for (std::size_t k = 0; k < credits.size(); ++k)
+          if (str(credits.at(k), "name", "") == c.asString()) p.credit = static_cast<int>(k);
+        if (p.credit < 0)
+          throw std::invalid_argument("netting set '" + p.id + "': unknown counterparty '" + c.asString() + "'");
+      } else {
+        const double k = num(ns, "counterparty", 0);
+        if (k < 0 || k >= static_cast<double>(credits.size()))
+          throw std::invalid_argument("netting set '" + p.id + "': counterparty index out of range");
+        p.credit = static_cast<int>(k);
+      }
+      p.counterparty = str(credits.at(static_cast<std::size_t>(p.credit)), "name", "CPTY");
+    }
+    parts.push_back(p);
+  }
+  // The market of the book: the spec without its trades and CSA, simulating the credits some
+  // netting set has as its counterparty (in the spec's order) with their block of the
+  // correlation matrix (as singleCredit, analyses.cpp:1644-1669, does for one credit).
+  std::vector<int> used;
+  for (const auto& p : parts)
+    if (p.credit >= 0 && std::find(used.begin(), used.end(), p.credit) == used.end()) used.push_back(p.credit);
+  std::sort(used.begin(), used.end());
+  Value base = Value::object();
+  for (const auto& kv : spec.members())
+    if (kv.first != "book" && kv.first != "scope" && kv.first != "trades" && kv.first != "csa" &&
+        kv.first != "credits" && kv.first != "correlation" && kv.first != "counterparty")
+      base.set(kv.first, kv.second);
+  if (credits.size() > 0) {
+    Value cs = Value::array();
+    for (int k : used) cs.push(credits.at(static_cast<std::size_t>(k)));
+    base.set("credits", cs);
+    if (has(spec, "correlation")) {
+      const std::size_t nA = spec["assets"].size();
+      const Value& corr = spec["correlation"];
+      if (corr.size() != 1 + nA + credits.size()) throw std::invalid_argument("correlation matrix has wrong dimension");
+      std::vector<std::size_t> keep;
+      for (std::size_t i = 0; i <= nA; ++i) keep.push_back(i);
+      for (int k : used) keep.push_back(1 + nA + static_cast<std::size_t>(k));
+      Value m = Value::array();
+      for (std::size_t i : keep) {
+        Value row = Value::array();
+        for (std::size_t j : keep) row.push(Value::number(corr.at(i).at(j).asNumber("correlation")));
+        m.push(row);
+      }
+      base.set("correlation", m);
+    }
+  }
+  const Env market = parseEnv(base);
+  std::vector<Env> envs;
+  for (std::size_t i = 0; i < parts.size(); ++i) {
+    const Value& ns = nets.at(i);
+    Value s = base;
+    s.set("trades", ns["trades"]);
+    Value csa = has(ns, "csa") ? ns["csa"] : Value::object();
+    if (!has(ns, "csa")) csa.setBool("enabled", false);
+    parts[i].collateralised = !(has(csa, "enabled") && !csa["enabled"].asBool("csa.enabled"));
+    s.set("csa", csa);
+    const auto at = std::find(used.begin(), used.end(), parts[i].credit);
+    s.set("counterparty", static_cast<double>(at == used.end() ? 0 : at - used.begin()));
+    envs.push_back(parseEnv(s));
+  }
+  const Model model(market, market.correlation);
+  std::vector<SetData> sets;
+  for (std::size_t i = 0; i < parts.size(); ++i) sets.push_back(nettingSet(envs[i], parts[i].id, true));
+  RunOptions o;
+  o.book = true;
+  bool perTrade = analysis == "allocation" || analysis == "xva";
+  for (const auto& e : envs) perTrade = perTrade && perTradeFits(e);
+  if (analysis == "allocation" && !perTrade)
+    throw std::domain_error("per-trade statistics cover at most 16 trades without Bermudan swaptions per netting set");
+  if (analysis == "allocation")
+    for (auto& s : sets) s.hasCsa = false;  // the uncollateralised sets
+  RunOptions first = o;
+  first.perTrade = perTrade;
+  std::vector<SetResult> res = runSets(ctx, market, model, sets, first);
+  if (analysis == "xva") {
+    // Book plans carry the IM proxy, not the nested IM.
+    RunOptions m = o;
+    m.imProxyQuantile = num(spec, "imQuantile", 0.99);
+    m.imMpr = num(spec, "imMprDays", 10.0) / 250.0;
+    const auto im = runSets(ctx, market, model, sets, m);
+    for (std::size_t i = 0; i < res.size(); ++i) res[i].imProxy = im[i].imProxy;
+  }
+  std::vector<Value> rs;
+  for (std::size_t i = 0; i < parts.size(); ++i) {
+    try {
+      const Env& e = envs[i];
+      if (analysis == "exposure") rs.push_back(exposureResult(e, res[i]));
+      else if (analysis == "cva") rs.push_back(cvaResult(spec, e, res[i]));
+      else if (analysis == "xva") rs.push_back(xvaResult(spec, e, res[i]));
+      else rs.push_back(allocationResult(spec, e, res[i]));
+    } catch (const std::exception& ex) {
+      throw std::invalid_argument("netting set '" + parts[i].id + "': " + ex.what());
+    }
+  }
+  // sumProfiles (analyses.cpp:1769-1784): the additive fields; PFE does not add.
+  Profile total = res[0].profile;
+  for (std::size_t i = 1; i < res.size(); ++i) {
+    const Profile& p = res[i].profile;
+    if (p.times.size() != total.times.size()) throw std::runtime_error("book: the netting sets' grids differ");
+    for (std::size_t k = 0; k < p.times.size(); ++k) {
+      total.ev[k] += p.ev[k];
+      total.ee[k] += p.ee[k];
+      total.ene[k] += p.ene[k];
+      total.dee[k] += p.dee[k];
+      total.dene[k] += p.dene[k];
+    }
+  }
+  auto numAt = [](const Value& o, const char* a, const char* b = nullptr) {
+    const Value& v = b ? o[a][b] : o[a];
+    return v.present() ? v.asNumber(a) : 0.0;
+  };
+  auto sumArrays = [&](const char* key) {
+    std::vector<double> s;
+    for (const auto& r : rs) {
+      if (!r[key].present()) continue;
+      const auto v = r[key].asNumbers(key);
+      if (s.empty()) s.assign(v.size(), 0.0);
+      for (std::size_t i = 0; i < s.size() && i < v.size(); ++i) s[i] += v[i];
+    }
+    return s;
+  };
+  std::vector<std::string> cps;
+  for (const auto& p : parts)
+    if (std::find(cps.begin(), cps.end(), p.counterparty) == cps.end()) cps.push_back(p.counterparty);
+  auto byCounterparty = [&](const std::vector<std::pair<std::string, std::function<double(const Value&)>>>& fields) {
+    Value list = Value::array();
+    for (const auto& c : cps) {
+      Value o2 = Value::object();
+      o2.set("name", c);
+      Value ids = Value::array();
+      std::vector<double> acc(fields.size(), 0.0);
+      for (std::size_t i = 0; i < parts.size(); ++i)
+        if (parts[i].counterparty == c) {
+          ids.push(Value::string(parts[i].id));
+          for (std::size_t f = 0; f < fields.size(); ++f) acc[f] += fields[f].second(rs[i]);
+        }
+      for (std::size_t f = 0; f < fields.size(); ++f) o2.set(fields[f].first, acc[f]);
+      o2.set("nettingSets", ids);
+      list.push(o2);
+    }
+    return list;
+  };
+  auto concatTrades = [&](const std::vector<const char*>& fields) {
+    Value list = Value::array();
+    for (std::size_t i = 0; i < parts.size(); ++i) {
+      const Value& t = rs[i]["trades"];
+      for (std::size_t k = 0; k < t.size(); ++k) {
+        Value x = Value::object();
+        x.set("id", t.at(k)["id"]);
+        x.set("nettingSet", parts[i].id);
+        x.set("counterparty", parts[i].counterparty);
+        for (const char* f : fields)
+          if (t.at(k)[f].present()) x.set(f, t.at(k)[f]);
+        list.push(x);
+      }
+    }
+    return list;
+  };
+
+  Value setsOut = Value::array();
+  for (std::size_t i = 0; i < parts.size(); ++i) {
+    const Value& r = rs[i];
+    const Profile& pr = res[i].profile;
+    Value s = Value::object();
+    if (analysis == "exposure") {
+      s.set("value0", pr.ev.empty() ? 0.0 : pr.ev[0]);
+      s.set("eepe1y", eepe(pr, 1.0));
+      s.set("epeLife", epe(pr, pr.times.back()));
+      s.set("maxPfe", pr.pfe.empty() ? 0.0 : *std::max_element(pr.pfe.begin(), pr.pfe.end()));
+    } else if (analysis == "cva") {
+      s.set("cva", numAt(r, "cva"));
+      s.set("dva", numAt(r, "bilateral", "dva"));
+      s.set("bilateral", numAt(r, "bilateral", "total"));
+      s.set("pathwiseCva", numAt(r, "pathwiseCva"));
+    } else if (analysis == "xva") {
+      for (const char* k : {"cva", "dva", "fca", "fba", "fva", "mva", "kva", "total"}) s.set(k, numAt(r["xva"], k));
+    } else {
+      for (const char* k : {"totalCva", "standaloneCvaSum", "incrementalCvaSum", "nettingSaving"}) s.set(k, numAt(r, k));
+    }
+    Value x = Value::object();
+    x.set("id", parts[i].id);
+    x.set("counterparty", parts[i].counterparty);
+    x.set("trades", static_cast<double>(parts[i].trades));
+    x.setBool("collateralised", parts[i].collateralised);
+    x.set("summary", s);
+    x.set("result", r);
+    setsOut.push(x);
+  }
+  Value tot = Value::object();
+  tot.set("profile", totalProfileToJs(total));
+  if (analysis == "cva") {
+    double cva = 0, dva = 0, bil = 0, pw = 0;
+    for (const auto& r : rs) {
+      cva += numAt(r, "cva");
+      dva += numAt(r, "bilateral", "dva");
+      bil += numAt(r, "bilateral", "total");
+      pw += numAt(r, "pathwiseCva");
+    }
+    tot.set("cva", cva);
+    tot.set("dva", dva);
+    tot.set("bilateral", bil);
+    tot.set("pathwiseCva", pw);
+    tot.setNumbers("cumulativeCva", sumArrays("cumulativeCva"));
+    tot.setNumbers("termStructure", sumArrays("termStructure"));
+    tot.set("byCounterparty", byCounterparty({{"cva", [&](const Value& r) { return numAt(r, "cva"); }},
+                                              {"dva", [&](const Value& r) { return numAt(r, "bilateral", "dva"); }},
+                                              {"pathwiseCva", [&](const Value& r) { return numAt(r, "pathwiseCva"); }}}));
+  } else if (analysis == "xva") {
+    Value x = Value::object();
+    std::vector<std::pair<std::string, std::function<double(const Value&)>>> fields;
+    for (const char* k : {"cva", "dva", "fca", "fba", "fva", "mva", "kva", "total"}) {
+      double v = 0;
+      for (const auto& r : rs) v += numAt(r["xva"], k);
+      x.set(k, v);
+      if (std::string(k) != "fva") fields.push_back({k, [&, k](const Value& r) { return numAt(r["xva"], k); }});
+    }
+    tot.set("xva", x);
+    for (const char* k : {"cumulativeCva", "cumulativeFca", "cumulativeMva", "cumulativeKva", "initialMargin"})
+      tot.setNumbers(k, sumArrays(k));
+    tot.set("byCounterparty", byCounterparty(fields));
+    tot.set("trades", concatTrades({"cva", "fca", "kva"}));
+  } else if (analysis == "allocation") {
+    double totalCva = 0, standalone = 0, marginal = 0;
+    for (const auto& r : rs) {
+      totalCva += numAt(r, "totalCva");
+      standalone += numAt(r, "standaloneCvaSum");
+      marginal += numAt(r, "marginalCvaSum");
+    }
+    tot.set("totalCva", totalCva);
+    tot.set("standaloneCvaSum", standalone);
+    tot.set("marginalCvaSum", marginal);
+    tot.set("nettingSaving", standalone != 0.0 ? 1.0 - totalCva / standalone : kNaN);
+    tot.setNumbers("totalEe", sumArrays("totalEe"));
+    tot.set("trades", concatTrades({"marginalCva", "marginalFca", "marginalKva", "incrementalCva", "standaloneCva"}));
+  }
+  Value out = Value::object();
+  out.set("scope", "book");
+  out.set("analysis", analysis);
+  out.set("name", str(spec["book"], "name", "book"));
+  out.set("nettingSets", setsOut);
+  out.set("total", tot);
+  return out;
+}
+
+// ------------------------------------------------------------------ entry points
+
+Value guarded(const std::function<Value()>& body) {  // analyses.cpp:68-81
+  try {
+    return body();
+  } catch (const std::domain_error& e) {
+    Value o = Value::object();
+    o.set("unsupported", std::string(e.what()));
+    return o;
+  } catch (const std::exception& e) {
+    Value o = Value::object();
+    o.set("error", std::string(e.what()));
+    return o;
+  }
+}
+
+Value version() {
+  Value o = Value::object();
+  o.set("library", "zlatko_amd ccr_hip");
+  o.set("abi", static_cast<double>(ccr_hip_abi_version()));
+  o.set("backend", "hip");
+  return o;
+}
+
+// The jobs of an analysis as the engine would compile them (no device): parsed trades,
+// bootstrapped curves and grid sizes.
+Value jobs(const Value& spec) {
+  const Env env = parseEnv(spec);
+  Value out = Value::object();
+  out.set("analysis", analysisOf(spec));
+  out.setNumbers("baseGrid", env.baseGrid);
+  out.set("numPaths", static_cast<double>(env.sim.num_paths));
+  out.set("seed", static_cast<double>(env.sim.seed));
+  out.setBool("antithetic", env.sim.antithetic != 0);
+  Value credits = Value::array();
+  for (const auto& c : env.credits) {
+    Value x = Value::object();
+    x.set("name", c.name);
+    x.set("recovery", c.recovery);
+    x.setNumbers("pillars", c.curve.p);
+    x.setNumbers("hazards", c.curve.h);
+    credits.push(x);
+  }
+  out.set("credits", credits);
+  if (env.hasOwn) {
+    Value x = Value::object();
+    x.setNumbers("pillars", env.ownCurve.p);
+    x.setNumbers("hazards", env.ownCurve.h);
+    out.set("own", x);
+  }
+  Value trades = Value::array();
+  for (const auto& t : env.trades) {
+    Value x = Value::object();
+    x.set("id", t.id);
+    x.set("type", t.type);
+    x.set("notional", t.notional);
+    x.set("fixedRate", t.fixedRate);
+    x.set("maturity", t.exposureMaturity());
+    if (!t.schedule.empty()) x.setNumbers("schedule", t.schedule);
+    if (!t.exercises.empty()) x.setNumbers("exercises", t.exercises);
+    if (t.type == "swap" || t.type == "bermudan") x.set("direction", t.payFixed ? "payer" : "receiver");
+    trades.push(x);
+  }
+  out.set("trades", trades);
+  if (!env.trades.empty()) {
+    const SetData s = nettingSet(env, "base", true);
+    const ccr_netting_set c = s.c();
+    int32_t rep = 0, sim = 0;
+    double none[1];
+    check(ccr_hip_reporting_grid(env.baseGrid.data(), static_cast<int32_t>(env.baseGrid.size()), &c, none, 0, &rep),
+          nullptr, "reporting grid");
+    check(ccr_hip_simulation_grid(env.baseGrid.data(), static_cast<int32_t>(env.baseGrid.size()), &c, none, 0, &sim),
+          nullptr, "simulation grid");
+    out.set("reportingDates", static_cast<double>(rep));
+    out.set("simulationDates", static_cast<double>(sim));
+  }
+  out.setBool("collateralised", env.hasCsa);
+  return out;
+}
+
+Value call(ccr_hip_ctx* ctx, const std::string& function, const std::string& specText) {
+  return guarded([&]() -> Value {
+    if (function == "version") return version();
+    const Value spec = Value::parse(specText);
+    if (function == "hipJobs") return jobs(spec);
+    if (function == "hipAnalyse") {
+      if (str(spec, "scope", "") == "book") return analyseBook(ctx, spec);
+      return analyseSingle(ctx, spec, analysisOf(spec));
+    }
+    Value o = Value::object();
+    o.set("error", "unknown entry point '" + function + "'");
+    return o;
+  });
+}
+
+}  // namespace
+}  // namespace app
+}  // namespace ccrhip
+
+extern "C" int32_t ccr_hip_app_call(ccr_hip_ctx* ctx, const char* function, const char* spec_json, char** result_json) {
+  if (!function || !spec_json || !result_json) return CCR_ERR_INVALID_ARGUMENT;
+  *result_json = nullptr;
+  if (!ctx && std::strcmp(function, "hipAnalyse") == 0) return CCR_ERR_INVALID_ARGUMENT;
+  std::string text;
+  try {
+    text = ccrhip::app::call(ctx, function, spec_json).dump();
+  } catch (const std::bad_alloc&) {
+    return CCR_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception&) {
+    return CCR_ERR_RUNTIME;
+  }
+  char* out = static_cast<char*>(std::malloc(text.size() + 1));
+  if (!out) return CCR_ERR_OUT_OF_MEMORY;
+  std::memcpy(out, text.c_str(), text.size() + 1);
+  *result_json = out;
+  return CCR_OK;
+}
+
+extern "C" void ccr_hip_app_free(char* result_json) { std::free(result_json); }
+
+extern "C" const char* ccr_hip_app_functions(void) { return "version,hipJobs,hipAnalyse"; }

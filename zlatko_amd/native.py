@@ -6,6 +6,7 @@ entry points raises instead of falling back to anything else.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import re
 from pathlib import Path
 from typing import List, Optional
@@ -17,6 +18,7 @@ from . import abi
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libccr_hip.so"
 HEADER = HERE.parent / "include" / "ccr_hip.h"
+APP_HEADER = HERE.parent / "include" / "ccr_hip_app.h"
 
 _lib = None
 
@@ -114,14 +116,34 @@ def lib():
         L.ccr_effective_expected_exposure.argtypes = [P, I32, P]
         L.ccr_effective_expected_positive_exposure.restype = D
         L.ccr_effective_expected_positive_exposure.argtypes = [P, P, I32, D]
+        L.ccr_hip_app_call.restype = I32
+        L.ccr_hip_app_call.argtypes = [P, C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.ccr_hip_app_free.restype = None
+        L.ccr_hip_app_free.argtypes = [C.c_void_p]
+        L.ccr_hip_app_functions.restype = C.c_char_p
+        L.ccr_hip_app_functions.argtypes = []
         _lib = L
     return _lib
 
 
 def header_symbols() -> List[str]:
-    """Function names declared in include/ccr_hip.h."""
-    text = HEADER.read_text()
+    """Function names declared in include/ccr_hip.h and include/ccr_hip_app.h."""
+    text = HEADER.read_text() + APP_HEADER.read_text()
     return sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(ccr_\w+)\s*\(", text, re.M)))
+
+
+def app_call(function: str, spec, engine: "Engine" = None) -> dict:
+    """ccr_hip_app_call: the Spec/Value front door. spec is a JSON-serialisable dict (or JSON
+    text); returns the result document as a dict ({"error": ...} for a failed analysis)."""
+    text = spec if isinstance(spec, str) else json.dumps(spec)
+    out = C.c_void_p()
+    rc = lib().ccr_hip_app_call(engine.ctx if engine else None, function.encode(), text.encode(), C.byref(out))
+    if rc:
+        raise CcrError(rc, "ccr_hip_app_call")
+    try:
+        return json.loads(C.string_at(out.value).decode())
+    finally:
+        lib().ccr_hip_app_free(out)
 
 
 def grid(spec: abi.Spec, i: int = 0, simulation: bool = False) -> np.ndarray:

@@ -28,9 +28,9 @@ extern "C" {
  *                 bootstrapped credit curves, grid sizes); needs no device (cf. gpuJobs,
  *                 analyses.cpp:2058-2095);
  *   "hipAnalyse"  runs spec.analysis -- "exposure", "cva", "xva", "collateral", "wrongWayRisk"
- *                 or "allocation", or a portfolio book (spec.scope = "book") of "exposure",
- *                 "cva", "xva" or "allocation" -- on the HIP engine of `ctx` (cf. cpuKernels /
- *                 gpuAnalyse, analyses.cpp:2179-2214).
+ *                 or "allocation", or a portfolio book (spec.scope = "book") of any of them --
+ *                 on the HIP engine of `ctx` (cf. cpuKernels / gpuAnalyse,
+ *                 analyses.cpp:2179-2214).
  * A failure of the analysis itself (a bad spec, an unknown analysis or entry point, a setup the
  * kernels do not implement) is a result { "error": "..." } (or { "unsupported": "..." }) with
  * CCR_OK, as the reference returns it. CCR_ERR_INVALID_ARGUMENT: a null function, spec or

@@ -203,5 +203,15 @@ def test_book_xva_and_cva_add_up_by_netting_set_and_counterparty(engine):
     assert "error" in native.app_call("hipAnalyse", A.book(A.base_spec(), sets, "models"), engine)
     bad = A.book(A.base_spec(), [A.netting_set("x", "C", [A.swap("s", 1e6, 5, "payer")])], "cva")
     assert "unknown counterparty 'C'" in native.app_call("hipAnalyse", bad, engine)["error"]
+    spec = dict(A.base_spec(), rhos=[-0.5, 0.5])
     for analysis in ("collateral", "wrongWayRisk"):
-        assert "unsupported" in native.app_call("hipAnalyse", A.book(A.base_spec(), sets, analysis), engine)
+        b = analyse(engine, A.book(spec, sets, analysis))
+        assert len(b["nettingSets"]) == 3 and b["counterparties"] == ["A", "B"]
+    col = analyse(engine, A.book(spec, sets, "collateral"))
+    assert [v["name"] for v in col["total"]["variants"]][:2] == ["no CSA", "specified CSA"]
+    assert col["total"]["variants"][0]["eepeVsNoCsa"] == 1.0
+    wwr = analyse(engine, A.book(spec, sets, "wrongWayRisk"))
+    for k, t in enumerate(wwr["total"]["results"]):
+        assert t["pathwiseCva"] == pytest.approx(
+            sum(s["result"]["results"][k]["pathwiseCva"] for s in wwr["nettingSets"]), rel=1e-12)
+    assert "unsupported" in native.app_call("hipAnalyse", A.book(spec, sets, "hedging"), engine)

@@ -1079,6 +1079,84 @@ bool perTradeFits(const Env& env) {
   return true;
 }
 
+struct CsaVariant {
+  std::string name;
+  bool enabled;
+  ccr_collateral csa;
+};
+
+// csaVariants (analyses.cpp:598-620): no CSA, the specified one, zero thresholds and MTA, a
+// 20-day margin period, one-way, posted in a government bond (2% haircut) and, with assets, in
+// the first asset (15% haircut).
+std::vector<CsaVariant> csaVariants(const Value& spec, const Env& env) {
+  ccr_collateral c{};
+  if (has(spec, "csa")) {
+    c = parseCsa(spec["csa"]);
+  } else {
+    c.threshold_own = kInf;
+    c.margin_period_of_risk = 10.0 / 250.0;
+  }
+  ccr_collateral zero = c, mpr20 = c, oneWay = c, bond = c, asset = c;
+  zero.threshold_counterparty = zero.threshold_own = zero.minimum_transfer_amount = 0.0;
+  mpr20.margin_period_of_risk = 20.0 / 250.0;
+  oneWay.threshold_own = kInf;
+  bond.type = CCR_COLLATERAL_BOND;
+  bond.haircut = 0.02;
+  bond.bond_maturity = std::max(10.0, num(spec["grid"], "horizon", 10.0));
+  std::vector<CsaVariant> v = {{"no CSA", false, c},          {"specified CSA", true, c},
+                               {"zero threshold & MTA", true, zero}, {"MPR 20 days", true, mpr20},
+                               {"one-way (we never post)", true, oneWay}, {"bond collateral (2% haircut)", true, bond}};
+  if (!env.assets.empty()) {
+    asset.type = CCR_COLLATERAL_ASSET;
+    asset.haircut = 0.15;
+    asset.asset_index = 0;
+    v.push_back({env.assets[0].name + " collateral (15% haircut)", true, asset});
+  }
+  return v;
+}
+
+SetData variantSet(const Env& env, const CsaVariant& v) {
+  SetData s = nettingSet(env, v.name, false);
+  s.hasCsa = v.enabled;
+  s.csa = v.csa;
+  return s;
+}
+
+Value collateralResult(const std::vector<CsaVariant>& variants, const std::vector<SetResult>& res) {  // :1351-1359
+  const double noCsa = eepe(res[0].profile, 1.0);
+  Value list = Value::array();
+  for (std::size_t i = 0; i < variants.size(); ++i) {
+    Value x = Value::object();
+    x.set("name", variants[i].name);
+    x.set("csa", csaToJs(variants[i].enabled, variants[i].csa));
+    x.set("profile", profileToJs(res[i].profile));
+    x.set("eepeVsNoCsa", noCsa > 0.0 ? eepe(res[i].profile, 1.0) / noCsa : kNaN);
+    list.push(x);
+  }
+  Value out = Value::object();
+  out.set("variants", list);
+  return out;
+}
+
+// One rho of the wrong-way-risk sweep (analyses.cpp:1360-1383) for a netting set's result.
+Value wrongWayPoint(double rho, const CreditData& cp, const SetResult& r) {
+  Value x = Value::object();
+  x.set("rho", rho);
+  const double independent = unilateralCva(r.profile.times, r.profile.dee, cp.curve, cp.recovery);
+  const double pw = pathwiseCva(r.profile, cp.recovery);
+  x.set("pathwiseCva", pw);
+  x.set("independentCva", independent);
+  x.set("multiplier", pw / independent);
+  x.setNumbers("times", r.profile.times);
+  x.setNumbers("discountedEe", r.profile.dee);
+  x.setNumbers("conditionalDiscountedEe", conditionalDee(r.profile));
+  return x;
+}
+
+std::vector<double> wrongWayRhos(const Value& spec) {  // analyses.cpp:622-624
+  return has(spec, "rhos") ? spec["rhos"].asNumbers("rhos") : std::vector<double>{-0.6, -0.3, 0.0, 0.3, 0.6};
+}
+
 Value analyseSingle(ccr_hip_ctx* ctx, const Value& spec, const std::string& analysis) {
   const Env env = parseEnv(spec);
   if (env.trades.empty()) throw std::invalid_argument("ExposureEngine::run: empty netting set");
@@ -1125,50 +1203,12 @@ Value analyseSingle(ccr_hip_ctx* ctx, const Value& spec, const std::string& anal
   }
   if (analysis == "collateral") {
     // csaVariants (analyses.cpp:598-620), all variants in one plan (same scenarios).
-    ccr_collateral c = has(spec, "csa") ? parseCsa(spec["csa"]) : ccr_collateral{};
-    if (!has(spec, "csa")) {
-      c.threshold_own = kInf;
-      c.margin_period_of_risk = 10.0 / 250.0;
-    }
-    ccr_collateral zero = c, mpr20 = c, oneWay = c, bond = c, asset = c;
-    zero.threshold_counterparty = zero.threshold_own = zero.minimum_transfer_amount = 0.0;
-    mpr20.margin_period_of_risk = 20.0 / 250.0;
-    oneWay.threshold_own = kInf;
-    bond.type = CCR_COLLATERAL_BOND;
-    bond.haircut = 0.02;
-    bond.bond_maturity = std::max(10.0, num(spec["grid"], "horizon", 10.0));
-    std::vector<std::pair<std::string, std::pair<bool, ccr_collateral>>> variants = {
-        {"no CSA", {false, c}},         {"specified CSA", {true, c}}, {"zero threshold & MTA", {true, zero}},
-        {"MPR 20 days", {true, mpr20}}, {"one-way (we never post)", {true, oneWay}},
-        {"bond collateral (2% haircut)", {true, bond}}};
-    if (!env.assets.empty()) {
-      asset.type = CCR_COLLATERAL_ASSET;
-      asset.haircut = 0.15;
-      asset.asset_index = 0;
-      variants.push_back({env.assets[0].name + " collateral (15% haircut)", {true, asset}});
-    }
+    const auto variants = csaVariants(spec, env);
     std::vector<SetData> sets;
-    for (const auto& v : variants) {
-      SetData s = nettingSet(env, v.first, false);
-      s.hasCsa = v.second.first;
-      s.csa = v.second.second;
-      sets.push_back(std::move(s));
-    }
+    for (const auto& v : variants) sets.push_back(variantSet(env, v));
     RunOptions o;
     const auto res = runSets(ctx, env, model, sets, o);
-    const double noCsa = eepe(res[0].profile, 1.0);
-    Value list = Value::array();
-    for (std::size_t i = 0; i < res.size(); ++i) {
-      Value x = Value::object();
-      x.set("name", variants[i].first);
-      x.set("csa", csaToJs(variants[i].second.first, variants[i].second.second));
-      x.set("profile", profileToJs(res[i].profile));
-      x.set("eepeVsNoCsa", noCsa > 0.0 ? eepe(res[i].profile, 1.0) / noCsa : kNaN);
-      list.push(x);
-    }
-    Value out = Value::object();
-    out.set("variants", list);
-    return out;
+    return collateralResult(variants, res);
   }
   if (analysis == "wrongWayRisk") {
     // analyses.cpp:761-769 and :1360-1383: the counterparty intensity's correlation with one
@@ -1178,32 +1218,23 @@ Value analyseSingle(ccr_hip_ctx* ctx, const Value& spec, const std::string& anal
     const int driver = static_cast<int>(num(spec, "wwrFactor", env.assets.empty() ? 0 : 1));
     const int creditFactor = 1 + static_cast<int>(env.assets.size()) + env.counterparty;
     if (driver < 0 || driver >= nF || creditFactor >= nF) throw std::invalid_argument("wrong-way risk: factor out of range");
-    const std::vector<double> rhos = has(spec, "rhos") ? spec["rhos"].asNumbers("rhos")
-                                                       : std::vector<double>{-0.6, -0.3, 0.0, 0.3, 0.6};
+    const std::vector<double> rhos = wrongWayRhos(spec);
     const CreditData& cp = env.cpty();
     Value list = Value::array();
     for (double rho : rhos) {
       std::vector<double> corr = env.correlation;
       corr[static_cast<std::size_t>(driver * nF + creditFactor)] = rho;
       corr[static_cast<std::size_t>(creditFactor * nF + driver)] = rho;
-      Value x = Value::object();
-      x.set("rho", rho);
       try {
         const Model m(env, corr);
         RunOptions o;
-        const SetResult r = runSets(ctx, env, m, {nettingSet(env, "base", true)}, o)[0];
-        const double independent = unilateralCva(r.profile.times, r.profile.dee, cp.curve, cp.recovery);
-        const double pw = pathwiseCva(r.profile, cp.recovery);
-        x.set("pathwiseCva", pw);
-        x.set("independentCva", independent);
-        x.set("multiplier", pw / independent);
-        x.setNumbers("times", r.profile.times);
-        x.setNumbers("discountedEe", r.profile.dee);
-        x.setNumbers("conditionalDiscountedEe", conditionalDee(r.profile));
+        list.push(wrongWayPoint(rho, cp, runSets(ctx, env, m, {nettingSet(env, "base", true)}, o)[0]));
       } catch (const std::exception& e) {
+        Value x = Value::object();
+        x.set("rho", rho);
         x.set("error", std::string(e.what()));
+        list.push(x);
       }
-      list.push(x);
     }
     Value out = Value::object();
     out.set("results", list);
@@ -1223,7 +1254,8 @@ struct BookPart {
 };
 
 // A book (spec.scope = "book", spec.book.nettingSets = [{id, counterparty, csa, trades}]) of
-// "exposure", "cva", "xva" or "allocation", compiled as one book plan: every netting set on the
+// "exposure", "cva", "xva", "allocation", "collateral" (every set's CSA variants as sets of one
+// plan) or "wrongWayRisk" (one plan per rho), compiled as book plans: every netting set on the
 // union grids and the same scenarios (bookParts / compileBook, analyses.cpp:1671-1766), each
 // with its own counterparty's simulated intensity (the generator holds every credit that is
 // some set's counterparty, each its own factor; the reference's book gives each set a
@@ -1235,7 +1267,7 @@ Value analyseBook(ccr_hip_ctx* ctx, const Value& spec) {
                                                      "xva",      "wrongWayRisk", "hedging", "instruments"};
   if (std::find(portfolio.begin(), portfolio.end(), analysis) == portfolio.end())
     throw std::invalid_argument("the " + analysis + " analysis does not run on a portfolio");
-  if (analysis != "exposure" && analysis != "cva" && analysis != "xva" && analysis != "allocation")
+  if (analysis == "hedging" || analysis == "instruments")
     throw std::domain_error("the " + analysis + " analysis of a portfolio book is not provided by this front door");
   const Value& nets = spec["book"]["nettingSets"];
   if (nets.size() == 0) throw std::invalid_argument("the portfolio has no netting sets");
@@ -1299,6 +1331,7 @@ Value analyseBook(ccr_hip_ctx* ctx, const Value& spec) {
   }
   const Env market = parseEnv(base);
   std::vector<Env> envs;
+  std::vector<Value> partSpecs;
   for (std::size_t i = 0; i < parts.size(); ++i) {
     const Value& ns = nets.at(i);
     Value s = base;
@@ -1310,54 +1343,139 @@ Value analyseBook(ccr_hip_ctx* ctx, const Value& spec) {
     const auto at = std::find(used.begin(), used.end(), parts[i].credit);
     s.set("counterparty", static_cast<double>(at == used.end() ? 0 : at - used.begin()));
     envs.push_back(parseEnv(s));
+    partSpecs.push_back(s);
   }
   const Model model(market, market.correlation);
-  std::vector<SetData> sets;
-  for (std::size_t i = 0; i < parts.size(); ++i) sets.push_back(nettingSet(envs[i], parts[i].id, true));
   RunOptions o;
   o.book = true;
-  bool perTrade = analysis == "allocation" || analysis == "xva";
-  for (const auto& e : envs) perTrade = perTrade && perTradeFits(e);
-  if (analysis == "allocation" && !perTrade)
-    throw std::domain_error("per-trade statistics cover at most 16 trades without Bermudan swaptions per netting set");
-  if (analysis == "allocation")
-    for (auto& s : sets) s.hasCsa = false;  // the uncollateralised sets
-  RunOptions first = o;
-  first.perTrade = perTrade;
-  std::vector<SetResult> res = runSets(ctx, market, model, sets, first);
-  if (analysis == "xva") {
-    // Book plans carry the IM proxy, not the nested IM.
-    RunOptions m = o;
-    m.imProxyQuantile = num(spec, "imQuantile", 0.99);
-    m.imMpr = num(spec, "imMprDays", 10.0) / 250.0;
-    const auto im = runSets(ctx, market, model, sets, m);
-    for (std::size_t i = 0; i < res.size(); ++i) res[i].imProxy = im[i].imProxy;
-  }
+  std::vector<SetResult> res;  // per netting set, its first job's result (the book totals' profile)
   std::vector<Value> rs;
-  for (std::size_t i = 0; i < parts.size(); ++i) {
+  std::vector<std::vector<CsaVariant>> variants;        // collateral
+  std::vector<std::vector<SetResult>> variantResults;  // collateral: [set][variant]
+  std::vector<double> rhos;                            // wrong-way risk
+  std::vector<std::vector<SetResult>> rhoResults;      // wrong-way risk: [rho][set] (empty: failed)
+  std::vector<std::string> rhoErrors;
+  int driver = 0;
+  auto wrap = [&](std::size_t i, const std::function<Value()>& f) {
     try {
-      const Env& e = envs[i];
-      if (analysis == "exposure") rs.push_back(exposureResult(e, res[i]));
-      else if (analysis == "cva") rs.push_back(cvaResult(spec, e, res[i]));
-      else if (analysis == "xva") rs.push_back(xvaResult(spec, e, res[i]));
-      else rs.push_back(allocationResult(spec, e, res[i]));
+      rs.push_back(f());
     } catch (const std::exception& ex) {
       throw std::invalid_argument("netting set '" + parts[i].id + "': " + ex.what());
     }
-  }
-  // sumProfiles (analyses.cpp:1769-1784): the additive fields; PFE does not add.
-  Profile total = res[0].profile;
-  for (std::size_t i = 1; i < res.size(); ++i) {
-    const Profile& p = res[i].profile;
-    if (p.times.size() != total.times.size()) throw std::runtime_error("book: the netting sets' grids differ");
-    for (std::size_t k = 0; k < p.times.size(); ++k) {
-      total.ev[k] += p.ev[k];
-      total.ee[k] += p.ee[k];
-      total.ene[k] += p.ene[k];
-      total.dee[k] += p.dee[k];
-      total.dene[k] += p.dene[k];
+  };
+  if (analysis == "collateral") {
+    // Every set's CSA variants as netting sets of one book plan.
+    std::vector<SetData> sets;
+    for (std::size_t i = 0; i < parts.size(); ++i) {
+      variants.push_back(csaVariants(partSpecs[i], envs[i]));
+      for (const auto& v : variants.back()) sets.push_back(variantSet(envs[i], v));
+    }
+    const auto all = runSets(ctx, market, model, sets, o);
+    std::size_t off = 0;
+    for (std::size_t i = 0; i < parts.size(); ++i) {
+      variantResults.emplace_back(all.begin() + static_cast<std::ptrdiff_t>(off),
+                                  all.begin() + static_cast<std::ptrdiff_t>(off + variants[i].size()));
+      off += variants[i].size();
+      res.push_back(variantResults.back()[0]);
+      wrap(i, [&] { return collateralResult(variants[i], variantResults[i]); });
+    }
+  } else if (analysis == "wrongWayRisk") {
+    // One book plan per rho: every simulated counterparty intensity correlated at rho with the
+    // driver (spec.wwrFactor; default the first asset, else the rate).
+    if (market.credits.empty()) throw std::invalid_argument("wrong-way risk needs a stochastic counterparty intensity");
+    const int nF = market.factors(), nA = static_cast<int>(market.assets.size());
+    driver = static_cast<int>(num(spec, "wwrFactor", market.assets.empty() ? 0 : 1));
+    if (driver < 0 || driver >= 1 + nA) throw std::invalid_argument("wrong-way risk: driver factor out of range");
+    rhos = wrongWayRhos(spec);
+    std::vector<SetData> sets;
+    for (std::size_t i = 0; i < parts.size(); ++i) sets.push_back(nettingSet(envs[i], parts[i].id, true));
+    for (double rho : rhos) {
+      std::vector<double> corr = market.correlation;
+      for (int c = 1 + nA; c < nF; ++c) {
+        corr[static_cast<std::size_t>(driver * nF + c)] = rho;
+        corr[static_cast<std::size_t>(c * nF + driver)] = rho;
+      }
+      try {
+        const Model m(market, corr);
+        rhoResults.push_back(runSets(ctx, market, m, sets, o));
+        rhoErrors.emplace_back();
+      } catch (const std::exception& e) {
+        rhoResults.emplace_back();
+        rhoErrors.emplace_back(e.what());
+      }
+    }
+    for (std::size_t i = 0; i < parts.size(); ++i) {
+      wrap(i, [&] {
+        const CreditData& cp = envs[i].cpty();
+        Value list = Value::array();
+        for (std::size_t k = 0; k < rhos.size(); ++k) {
+          if (!rhoErrors[k].empty()) {
+            Value x = Value::object();
+            x.set("rho", rhos[k]);
+            x.set("error", rhoErrors[k]);
+            list.push(x);
+          } else {
+            list.push(wrongWayPoint(rhos[k], cp, rhoResults[k][i]));
+          }
+        }
+        Value out = Value::object();
+        out.set("results", list);
+        out.set("driver", static_cast<double>(driver));
+        return out;
+      });
+    }
+    for (std::size_t k = 0; k < rhos.size() && res.empty(); ++k)
+      if (rhoErrors[k].empty()) res = rhoResults[k];
+    if (res.empty()) throw std::invalid_argument(rhoErrors.empty() ? "no correlations" : rhoErrors[0]);
+  } else {
+    std::vector<SetData> sets;
+    for (std::size_t i = 0; i < parts.size(); ++i) sets.push_back(nettingSet(envs[i], parts[i].id, true));
+    bool perTrade = analysis == "allocation" || analysis == "xva";
+    for (const auto& e : envs) perTrade = perTrade && perTradeFits(e);
+    if (analysis == "allocation" && !perTrade)
+      throw std::domain_error("per-trade statistics cover at most 16 trades without Bermudan swaptions per netting set");
+    if (analysis == "allocation")
+      for (auto& st : sets) st.hasCsa = false;  // the uncollateralised sets
+    RunOptions first = o;
+    first.perTrade = perTrade;
+    res = runSets(ctx, market, model, sets, first);
+    if (analysis == "xva") {
+      // Book plans carry the IM proxy, not the nested IM.
+      RunOptions m = o;
+      m.imProxyQuantile = num(spec, "imQuantile", 0.99);
+      m.imMpr = num(spec, "imMprDays", 10.0) / 250.0;
+      const auto im = runSets(ctx, market, model, sets, m);
+      for (std::size_t i = 0; i < res.size(); ++i) res[i].imProxy = im[i].imProxy;
+    }
+    for (std::size_t i = 0; i < parts.size(); ++i) {
+      const Env& e = envs[i];
+      wrap(i, [&] {
+        if (analysis == "exposure") return exposureResult(e, res[i]);
+        if (analysis == "cva") return cvaResult(spec, e, res[i]);
+        if (analysis == "xva") return xvaResult(spec, e, res[i]);
+        return allocationResult(spec, e, res[i]);
+      });
     }
   }
+  // sumProfiles (analyses.cpp:1769-1784): the additive fields; PFE does not add.
+  auto sumProfiles = [](const std::vector<const Profile*>& ps) {
+    Profile t = *ps[0];
+    for (std::size_t i = 1; i < ps.size(); ++i) {
+      const Profile& p = *ps[i];
+      if (p.times.size() != t.times.size()) throw std::runtime_error("book: the netting sets' grids differ");
+      for (std::size_t k = 0; k < p.times.size(); ++k) {
+        t.ev[k] += p.ev[k];
+        t.ee[k] += p.ee[k];
+        t.ene[k] += p.ene[k];
+        t.dee[k] += p.dee[k];
+        t.dene[k] += p.dene[k];
+      }
+    }
+    return t;
+  };
+  std::vector<const Profile*> firsts;
+  for (const auto& r : res) firsts.push_back(&r.profile);
+  const Profile total = sumProfiles(firsts);
   auto numAt = [](const Value& o, const char* a, const char* b = nullptr) {
     const Value& v = b ? o[a][b] : o[a];
     return v.present() ? v.asNumber(a) : 0.0;
@@ -1427,6 +1545,15 @@ Value analyseBook(ccr_hip_ctx* ctx, const Value& spec) {
       s.set("pathwiseCva", numAt(r, "pathwiseCva"));
     } else if (analysis == "xva") {
       for (const char* k : {"cva", "dva", "fca", "fba", "fva", "mva", "kva", "total"}) s.set(k, numAt(r["xva"], k));
+    } else if (analysis == "collateral") {
+      const Value& v = r["variants"];
+      s.set("eepeNoCsa", numAt(v.at(0)["profile"], "eepe1y"));
+      s.set("eepeCsa", numAt(v.at(1)["profile"], "eepe1y"));
+    } else if (analysis == "wrongWayRisk") {
+      const Value& v = r["results"];
+      s.set("independentCva", numAt(v.at(0), "independentCva"));
+      s.set("pathwiseCvaLow", numAt(v.at(0), "pathwiseCva"));
+      s.set("pathwiseCvaHigh", numAt(v.at(v.size() - 1), "pathwiseCva"));
     } else {
       for (const char* k : {"totalCva", "standaloneCvaSum", "incrementalCvaSum", "nettingSaving"}) s.set(k, numAt(r, k));
     }
@@ -1486,10 +1613,50 @@ Value analyseBook(ccr_hip_ctx* ctx, const Value& spec) {
     tot.setNumbers("totalEe", sumArrays("totalEe"));
     tot.set("trades", concatTrades({"marginalCva", "marginalFca", "marginalKva", "incrementalCva", "standaloneCva"}));
   }
-  Value out = Value::object();
+  if (analysis == "collateral") {
+    // Per variant, the sum over the netting sets (bookResult, :1954-1970).
+    Value list = Value::array();
+    std::vector<const Profile*> none;
+    for (const auto& v : variantResults) none.push_back(&v[0].profile);
+    const double eepe0 = eepe(sumProfiles(none), 1.0);
+    for (std::size_t v = 0; v < variants[0].size(); ++v) {
+      std::vector<const Profile*> ps;
+      for (const auto& vr : variantResults) ps.push_back(&vr[v].profile);
+      const Profile p = sumProfiles(ps);
+      Value x = Value::object();
+      x.set("name", variants[0][v].name);
+      x.set("profile", totalProfileToJs(p));
+      x.set("eepeVsNoCsa", eepe0 > 0.0 ? eepe(p, 1.0) / eepe0 : kNaN);
+      list.push(x);
+    }
+    tot.set("variants", list);
+  } else if (analysis == "wrongWayRisk") {
+    Value list = Value::array();  // :1971-1984
+    for (std::size_t k = 0; k < rhos.size(); ++k) {
+      double pw = 0.0, ind = 0.0;
+      for (const auto& r : rs) {
+        pw += numAt(r["results"].at(k), "pathwiseCva");
+        ind += numAt(r["results"].at(k), "independentCva");
+      }
+      Value x = Value::object();
+      x.set("rho", rhos[k]);
+      x.set("pathwiseCva", pw);
+      x.set("independentCva", ind);
+      x.set("multiplier", ind != 0.0 ? pw / ind : kNaN);
+      list.push(x);
+    }
+    tot.set("results", list);
+  }
+  Value out = Value::object();  // :2011-2022
   out.set("scope", "book");
   out.set("analysis", analysis);
-  out.set("name", str(spec["book"], "name", "book"));
+  out.set("name", str(spec["book"], "name", "portfolio"));
+  out.setNumbers("times", total.times);
+  out.set("numPaths", static_cast<double>(res[0].numPaths));
+  out.set("simulationDates", static_cast<double>(res[0].simulationDates));
+  Value names = Value::array();
+  for (const auto& c : cps) names.push(Value::string(c));
+  out.set("counterparties", names);
   out.set("nettingSets", setsOut);
   out.set("total", tot);
   return out;

@@ -163,7 +163,7 @@ void runEngine(HostPlan& h, PathArgs<double>& a, const std::vector<std::vector<s
     const double* rp = eng.rfit.data() + static_cast<size_t>(rec) * kBasisStride;
     out[0] = rp[1];
     out[1] = rp[2];
-    for (int i = 0; i < 4; ++i) out[2 + i] = rp[6 + i];
+    for (int i = 0; i < 4; ++i) out[2 + i] = i <= static_cast<int>(rp[5]) ? rp[6 + i] : 0.0;
     out[6] = rp[5];
   }
 }

@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(256) amc_csolve_kernel(AmcArgs a) {
   regSolve(rp, a.gram + 10 * m, xty, xty[0], coef);
   out[0] = rp.center;
   out[1] = rp.scale;
-  for (int i = 0; i < 4; ++i) out[2 + i] = coef[i];
+  for (int i = 0; i < 4; ++i) out[2 + i] = i <= rp.deg ? coef[i] : 0.0;  // zeros above the degree
   out[6] = static_cast<double>(rp.deg);
 }
 
@@ -745,7 +745,7 @@ __global__ void __launch_bounds__(256) amc_rcont_kernel(AmcArgs a) {
   const double* rp = a.rfit + static_cast<int64_t>(rec) * kBasisStride;
   out[0] = rp[1];
   out[1] = rp[2];
-  for (int i = 0; i < 4; ++i) out[2 + i] = rp[6 + i];
+  for (int i = 0; i < 4; ++i) out[2 + i] = i <= static_cast<int>(rp[5]) ? rp[6 + i] : 0.0;  // zeros above the degree
   out[6] = rp[5];
 }
 

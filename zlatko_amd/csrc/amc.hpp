@@ -102,6 +102,13 @@ __host__ __device__ inline double regPredictZ(double z, int deg, const double* c
   for (int d = deg < 0 ? 0 : deg; d >= 0; --d) r = r * z + coef[d];
   return r;
 }
+// The same for coefficients stored as a cubic with zeros above the degree (the valuation's
+// continuation records): the zero leading steps of Horner's rule leave the next coefficient
+// exactly (0 z + c = c), so the value is that of the degree's own recurrence, with no loop over
+// a per-record degree.
+__host__ __device__ inline double regPredictCubic(double z, const double* coef) {
+  return ((coef[3] * z + coef[2]) * z + coef[1]) * z + coef[0];
+}
 // The same from the state x: z = (x - center) / scale.
 __host__ __device__ inline double regPredict(double x, double center, double scale, int deg, const double* coef) {
   return regPredictZ((x - center) / scale, deg, coef);

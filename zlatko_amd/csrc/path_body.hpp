@@ -215,7 +215,7 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
         z = (xd - zCenter) / zScale;
         haveZ = true;
       }
-      const double pred = regPredictZ(z, static_cast<int>(c[6]), coef);
+      const double pred = regPredictCubic(z, coef);  // the record's coefficients above its degree are 0
       v = static_cast<Real>(pred > 0.0 ? pred : 0.0);
     }
     if (bd.has_swap && !allLanes(!exercised)) {

@@ -268,9 +268,9 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
       const int lb = h.clive_rng[2 * static_cast<size_t>(d)], le = h.clive_rng[2 * static_cast<size_t>(d) + 1];
       bermv[static_cast<size_t>(d) * P + p] =
           h.basis == kBasisSwapRateAnnuity
-              ? bermudanSum<double, true>(S, j, p, P, h.bdates.data(), cont.data(), exidx.data(), states.data(), bb,
+              ? bermudanSum<double, true>(S, j, p, P, cont.data(), exidx.data(), states.data(), bb,
                                           ff, a.poly + S.poly_off, h.clive.data(), lb, le, view)
-              : bermudanSum<double, false>(S, j, p, P, h.bdates.data(), cont.data(), exidx.data(), states.data(), bb,
+              : bermudanSum<double, false>(S, j, p, P, cont.data(), exidx.data(), states.data(), bb,
                                            ff, a.poly + S.poly_off, h.clive.data(), lb, le, view);
     }
   }

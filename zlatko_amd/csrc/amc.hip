@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(kAmcChunk) amc_value_kernel(AmcArgs a, const R
     eng.rfit = toConst(static_cast<const double*>(a.rfit));
     eng.saterms = toConst(a.saterms);
   }
-  const Real v = bermudanSum<Real, kSa>(S, j, p, a.paths, toConst(a.bdates), toConst(a.cont), a.exidx,
+  const Real v = bermudanSum<Real, kSa>(S, j, p, a.paths, toConst(a.cont), a.exidx,
                                    a.states, toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off),
                                    toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off), toConst(polyOf<Real>(a)) + S.poly_off,
                                    toConst(a.clive), toConst(a.clive_rng)[2 * d], toConst(a.clive_rng)[2 * d + 1], eng);

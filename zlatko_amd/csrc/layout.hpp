@@ -169,19 +169,23 @@ struct BermDev {
   int32_t ev_off;     // first of ne x P exercise values / strategy values (x P)
 };
 
-// One swaption that can be nonzero on a continuation date: its global index, its date record
-// (BermDate / continuation fit, date_rec + j), its exercise count and its position, so the
-// valuation reads them in one load instead of a chain through the swaption's record.
-struct LiveRec {
-  int32_t b, rec, ne, pad;
-  double position;
-};
-
 struct BermDate {
   int32_t next_k;     // first exercise strictly after t_j (ne: none left)
   int32_t has_swap;   // the underlying's value terms are present (physical settlement, exercisable)
   TermRange swap;     // value of the underlying swap at t_j
 };
+
+// One swaption that can be nonzero on a continuation date: its global index, its date record
+// (continuation fit, date_rec + j), its exercise count and position, and a copy of that date's
+// BermDate, so the valuation reads everything static about it from one record (the per-record
+// chain of dependent scalar loads was the valuation kernel's bound).
+struct LiveRec {
+  int32_t b, rec, ne, next_k;
+  int32_t has_swap, pad;
+  double position;
+  TermRange swap;
+};
+static_assert(sizeof(LiveRec) == 72, "LiveRec layout");
 
 // One tile of the continuation GEMM: nd dates of one epoch (their continuation-date indices are
 // tcd[d_off ..]) x nb swaptions (global indices gberm[col_off ..]) with the strategy column of

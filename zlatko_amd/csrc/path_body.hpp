@@ -173,8 +173,7 @@ __host__ __device__ inline Real evalRange(const CCR_CONST TermRange& tr, Real x,
 // swaptions live[lbeg, lend) are visited: the plan lists, per continuation date, those that can
 // be nonzero on some path (the others add an exact 0).
 template <typename Real, bool kSa = false>
-__host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, int64_t p, int64_t paths,
-                                            cptr<BermDate> bdates, cptr<double> cont,
+__host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, int64_t p, int64_t paths, cptr<double> cont,
                                             const int16_t* exidx, const double* states, cptr<Real> bonds,
                                             cptr<Real> fixed, cptr<Real> poly, cptr<LiveRec> live, int lbeg,
                                             int lend, const BasisView& eng) {
@@ -188,14 +187,29 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
   double zCenter = 0.0, zScale = 0.0, z = 0.0;
   bool haveZ = false;
   for (int l = lbeg; l < lend; ++l) {
+    // The swaption, its date record and that date's BermDate: the first 32 bytes of the live
+    // record in two 16-byte scalar loads (member-wise reads become a chain of small loads).
+#if defined(__HIP_DEVICE_COMPILE__)
+    LiveRec lr;
+    {
+      cptr<int4> raw = reinterpret_cast<cptr<int4>>(live + l);
+      const int4 h0 = raw[0], h1 = raw[1];
+      lr.b = h0.x;
+      lr.rec = h0.y;
+      lr.ne = h0.z;
+      lr.next_k = h0.w;
+      lr.has_swap = h1.x;
+      lr.position = __hiloint2double(h1.w, h1.z);
+    }
+#else
     const LiveRec lr = live[l];
-    const CCR_CONST BermDate& bd = bdates[lr.rec];
-    const bool exercised = exidx[static_cast<int64_t>(lr.b) * paths + p] < bd.next_k;
+#endif
+    const bool exercised = exidx[static_cast<int64_t>(lr.b) * paths + p] < lr.next_k;
     Real v = 0;
     if constexpr (kSa) {
       // The continuation fit of this (swaption, date) on the swap rate and annuity of the tail
       // entered at the next exercise.
-      const int rid = bd.next_k < lr.ne ? eng.rcont[lr.rec] : -1;
+      const int rid = lr.next_k < lr.ne ? eng.rcont[lr.rec] : -1;
       if (rid >= 0) {
         const CCR_CONST RegRec& rr = eng.rrecs[rid];
         auto noFix = [](int) { return Real(0); };
@@ -206,7 +220,7 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
         const double pred = basisPredict(kBasisSwapRateAnnuity, rp, F / A, A);
         v = static_cast<Real>(pred > 0.0 ? pred : 0.0);
       }
-    } else if (bd.next_k < lr.ne) {
+    } else if (lr.next_k < lr.ne) {
       cptr<double> c = cont + static_cast<int64_t>(lr.rec) * kContStride;
       const double coef[4] = {c[2], c[3], c[4], c[5]};
       if (!haveZ || c[0] != zCenter || c[1] != zScale) {
@@ -218,8 +232,8 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
       const double pred = regPredictCubic(z, coef);  // the record's coefficients above its degree are 0
       v = static_cast<Real>(pred > 0.0 ? pred : 0.0);
     }
-    if (bd.has_swap && !allLanes(!exercised)) {
-      const Real sv = evalRange<Real>(bd.swap, x, bonds, fixed, poly, fixing);
+    if (lr.has_swap && !allLanes(!exercised)) {
+      const Real sv = evalRange<Real>(live[l].swap, x, bonds, fixed, poly, fixing);
       if (exercised) v = sv;
     } else if (exercised) {
       v = 0;

@@ -1180,6 +1180,12 @@ HostPlan buildPlan(const ccr_scenario_generator& gen, const ccr_simulation_confi
   plan.stream_len = blocksPaths * maxStride;
   for (int s = 0; s < num_sets; ++s)
     plan.sets[static_cast<std::size_t>(s)].rows_off = static_cast<int64_t>(s) * plan.max_rep * P;
+  for (LiveRec& lr : plan.clive) {  // each live record carries its date's BermDate
+    const BermDate& bd = plan.bdates[static_cast<std::size_t>(lr.rec)];
+    lr.next_k = bd.next_k;
+    lr.has_swap = bd.has_swap;
+    lr.swap = bd.swap;
+  }
   if (plan.clive.empty()) plan.clive.push_back(LiveRec{});  // a valid table even when no swaption is ever live
   // One zero coupon record past the end: the path kernel reads coupons two at a time.
   plan.fixed.insert(plan.fixed.end(), 4, 0.0);

@@ -266,12 +266,18 @@ void runAmc(HostPlan& h, PathArgs<double>& a, std::vector<double>& states, std::
       const double* bb = a.bonds + 2 * static_cast<int64_t>(S.bond_off);
       const double* ff = a.fixed + 4 * static_cast<int64_t>(S.fixed_off);
       const int lb = h.clive_rng[2 * static_cast<size_t>(d)], le = h.clive_rng[2 * static_cast<size_t>(d) + 1];
-      bermv[static_cast<size_t>(d) * P + p] =
-          h.basis == kBasisSwapRateAnnuity
-              ? bermudanSum<double, true>(S, j, p, P, cont.data(), exidx.data(), states.data(), bb,
-                                          ff, a.poly + S.poly_off, h.clive.data(), lb, le, view)
-              : bermudanSum<double, false>(S, j, p, P, cont.data(), exidx.data(), states.data(), bb,
-                                           ff, a.poly + S.poly_off, h.clive.data(), lb, le, view);
+      if (h.basis == kBasisSwapRateAnnuity) {
+        bermv[static_cast<size_t>(d) * P + p] = bermudanSum<double, true>(
+            S, j, p, P, cont.data(), exidx.data(), states.data(), bb, ff, a.poly + S.poly_off, h.clive.data(), lb, le,
+            view);
+      } else {
+        // The state basis as the GPU values it (bermudanSumN; one path per call on the host).
+        const int64_t one[1] = {p};
+        double v[1];
+        bermudanSumN<double, 1>(S, j, one, P, cont.data(), exidx.data(), states.data(), bb, ff, a.poly + S.poly_off,
+                                h.clive.data(), lb, le, v);
+        bermv[static_cast<size_t>(d) * P + p] = v[0];
+      }
     }
   }
   a.bermv = bermv.data();

@@ -468,6 +468,43 @@ __global__ void __launch_bounds__(kAmcChunk) amc_value_kernel(AmcArgs a, const R
   if (p0 < a.paths) a.bermv[static_cast<int64_t>(d) * a.paths + p0] = static_cast<double>(v);
 }
 
+// The state-basis valuation with kValuePaths paths per thread (bermudanSumN): a workgroup covers
+// kValuePaths x kAmcChunk paths of one continuation date, thread t the paths t + i kAmcChunk.
+constexpr int kValuePaths = 2;
+template <typename Real>
+__global__ void __launch_bounds__(kAmcChunk) amc_value_pair_kernel(AmcArgs a, const Real* bonds, const Real* fixed) {
+  const int chunk = static_cast<int>(blockIdx.x);
+  const int64_t first = static_cast<int64_t>(chunk) * (kValuePaths * kAmcChunk) + threadIdx.x;
+  if (chunk * kValuePaths * kAmcChunk >= a.paths) return;
+  const int d = static_cast<int>(blockIdx.y);
+  const CCR_CONST SetDev& S = toConst(a.sets)[toConst(a.cdates)[2 * d]];
+  const int j = toConst(a.cdates)[2 * d + 1];
+  int64_t p[kValuePaths];
+#pragma unroll
+  for (int i = 0; i < kValuePaths; ++i) {
+    const int64_t q = first + static_cast<int64_t>(i) * kAmcChunk;
+    p[i] = q < a.paths ? q : a.paths - 1;  // lanes past the last path replay it and store nothing
+  }
+  Real v[kValuePaths];
+  bermudanSumN<Real, kValuePaths>(S, j, p, a.paths, toConst(a.cont), a.exidx, a.states,
+                                  toConst(bonds) + 2 * static_cast<int64_t>(S.bond_off),
+                                  toConst(fixed) + 4 * static_cast<int64_t>(S.fixed_off),
+                                  toConst(polyOf<Real>(a)) + S.poly_off, toConst(a.clive), toConst(a.clive_rng)[2 * d],
+                                  toConst(a.clive_rng)[2 * d + 1], v);
+#pragma unroll
+  for (int i = 0; i < kValuePaths; ++i) {
+    const int64_t q = first + static_cast<int64_t>(i) * kAmcChunk;
+    if (q < a.paths) a.bermv[static_cast<int64_t>(d) * a.paths + q] = static_cast<double>(v[i]);
+  }
+}
+
+// Grid of amc_value_pair_kernel: its path chunks (a multiple of 8, so the workgroups of one chunk
+// land on one XCD and read the chunk's exercise indices from its L2) x the continuation dates.
+inline dim3 valuePairGrid(const AmcArgs& a) {
+  const int64_t chunks = (a.paths + kValuePaths * kAmcChunk - 1) / (kValuePaths * kAmcChunk);
+  return dim3(static_cast<unsigned>((chunks + 7) / 8 * 8), static_cast<unsigned>(a.ncdates));
+}
+
 // ---------------------------------------------------------------- regression engine (basis choice)
 // Records (layout.hpp RegRec) are regressed in the reference's order: exercise k of every
 // swaption at once, k from the last exercise down, then every continuation record at once. Per
@@ -843,8 +880,7 @@ hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int
       hipLaunchKernelGGL((amc_value_kernel<Real, true>), dim3((a.nchunks + 7) / 8 * 8, a.ncdates), blk, 0, stream,
                          a, bonds, fixed);
     else if (a.ncdates > 0)
-      hipLaunchKernelGGL((amc_value_kernel<Real, false>), dim3((a.nchunks + 7) / 8 * 8, a.ncdates), blk, 0, stream,
-                         a, bonds, fixed);
+      hipLaunchKernelGGL((amc_value_pair_kernel<Real>), valuePairGrid(a), blk, 0, stream, a, bonds, fixed);
     return last();
   }
   for (int k = max_exercises - 1; k >= 0; --k) {
@@ -869,8 +905,7 @@ hipError_t launchAmc(const AmcArgs& a, const Real* bonds, const Real* fixed, int
   }
   hipLaunchKernelGGL(amc_csolve_kernel, dim3((a.nbdates + 255) / 256), dim3(256), 0, stream, a);
   if (a.ncdates > 0)
-    hipLaunchKernelGGL((amc_value_kernel<Real, false>), dim3((a.nchunks + 7) / 8 * 8, a.ncdates), blk, 0, stream, a,
-                       bonds, fixed);
+    hipLaunchKernelGGL((amc_value_pair_kernel<Real>), valuePairGrid(a), blk, 0, stream, a, bonds, fixed);
   return last();
 }
 

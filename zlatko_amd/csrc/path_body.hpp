@@ -243,6 +243,88 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
   return V;
 }
 
+// bermudanSum (state basis) for NP paths per thread, p[0..NP): the live records, the exercise
+// decisions' scalar work and the date's standardisation are shared by the thread's paths, which
+// halves the scalar instructions per path of the valuation kernel and gives each record NP
+// independent exercise-index loads and predictions in flight. Each path's value is the one
+// bermudanSum computes; the uniform choices (evaluate the underlying, power series or
+// exponential terms) are taken over the wavefront's lanes for each of its NP path slots.
+template <typename Real, int NP>
+__host__ __device__ inline void bermudanSumN(const CCR_CONST SetDev& S, int j, const int64_t (&p)[NP], int64_t paths,
+                                             cptr<double> cont, const int16_t* exidx, const double* states,
+                                             cptr<Real> bonds, cptr<Real> fixed, cptr<Real> poly, cptr<LiveRec> live,
+                                             int lbeg, int lend, Real (&V)[NP]) {
+  const double* xrow = states + S.state_off;
+  double xd[NP], z[NP];
+  Real x[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    xd[i] = xrow[static_cast<int64_t>(j) * paths + p[i]];
+    x[i] = static_cast<Real>(xd[i]);
+    z[i] = 0.0;
+    V[i] = 0;
+  }
+  double zCenter = 0.0, zScale = 0.0;
+  bool haveZ = false;
+  for (int l = lbeg; l < lend; ++l) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    LiveRec lr;
+    {
+      cptr<int4> raw = reinterpret_cast<cptr<int4>>(live + l);
+      const int4 h0 = raw[0], h1 = raw[1];
+      lr.b = h0.x;
+      lr.rec = h0.y;
+      lr.ne = h0.z;
+      lr.next_k = h0.w;
+      lr.has_swap = h1.x;
+      lr.position = __hiloint2double(h1.w, h1.z);
+    }
+#else
+    const LiveRec lr = live[l];
+#endif
+    bool exercised[NP], any = false;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      exercised[i] = exidx[static_cast<int64_t>(lr.b) * paths + p[i]] < lr.next_k;
+      any = any || exercised[i];
+    }
+    Real v[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) v[i] = 0;
+    if (lr.next_k < lr.ne) {
+      cptr<double> c = cont + static_cast<int64_t>(lr.rec) * kContStride;
+      const double coef[4] = {c[2], c[3], c[4], c[5]};
+      if (!haveZ || c[0] != zCenter || c[1] != zScale) {
+        zCenter = c[0];
+        zScale = c[1];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) z[i] = (xd[i] - zCenter) / zScale;
+        haveZ = true;
+      }
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        const double pred = regPredictCubic(z[i], coef);
+        v[i] = static_cast<Real>(pred > 0.0 ? pred : 0.0);
+      }
+    }
+    if (lr.has_swap && !allLanes(!any)) {
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        const int64_t pi = p[i];
+        auto fixing = [&](int k) { return static_cast<Real>(xrow[static_cast<int64_t>(k) * paths + pi]); };
+        const Real sv = evalRange<Real>(live[l].swap, x[i], bonds, fixed, poly, fixing);
+        if (exercised[i]) v[i] = sv;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NP; ++i)
+        if (exercised[i]) v[i] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) V[i] += static_cast<Real>(lr.position) * v[i];
+  }
+}
+
 // Model shapes the path kernel is specialised for; everything else runs the generic shape.
 template <int NR_, int NA_, int NC_, bool Exact_>
 struct Shape {

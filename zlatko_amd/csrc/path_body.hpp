@@ -245,8 +245,8 @@ __host__ __device__ inline Real bermudanSum(const CCR_CONST SetDev& S, int j, in
 
 // bermudanSum (state basis) for NP paths per thread, p[0..NP): the live records, the exercise
 // decisions' scalar work and the date's standardisation are shared by the thread's paths, which
-// halves the scalar instructions per path of the valuation kernel and gives each record NP
-// independent exercise-index loads and predictions in flight. Each path's value is the one
+// divides the scalar instructions per path of the valuation kernel by NP and gives each record
+// NP independent exercise-index loads and predictions in flight. Each path's value is the one
 // bermudanSum computes; the uniform choices (evaluate the underlying, power series or
 // exponential terms) are taken over the wavefront's lanes for each of its NP path slots.
 template <typename Real, int NP>

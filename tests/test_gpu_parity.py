@@ -25,6 +25,8 @@ SMALL = {
     "two_leg": dict(two_leg=True),
     "all": dict(csa=True, fx=True, credits=True, two_leg=True),
     "options": dict(fx=True, csa=True, credits=True, options=6),
+    "csa_bond": dict(csa=True, credits=True, security="bond"),
+    "csa_asset": dict(csa=True, fx=True, credits=True, security="asset"),
 }
 
 
@@ -523,13 +525,14 @@ def test_xva_allocation_of_per_trade_contributions_adds_up():
     e.close()
 
 
-def test_book_matches_oracle_and_totals_add_up_fp64():
+@pytest.mark.parametrize("secured", [False, True])
+def test_book_matches_oracle_and_totals_add_up_fp64(secured):
     """Portfolio book: every netting set on the union grids with pathwise identical scenarios
     (E[D] bit-identical across sets on the GPU), profiles within 1e-10 of the oracle's book, and
-    ccr_book_totals per counterparty equal to the sums of the sets' additive fields."""
-    import ctypes as C
+    ccr_book_totals per counterparty equal to the sums of the sets' additive fields. secured: two
+    of the sets post a zero bond and the FX asset as collateral."""
     from zlatko_amd import native
-    spec = W.book_case(num_paths=3000)
+    spec = W.book_case(num_paths=3000, secured=secured)
     e = native.Engine()
     e.set_book(True)
     plan = e.compile(spec)
@@ -588,6 +591,28 @@ def test_book_kernel_group_sizes_give_identical_results(precision):
     last group is partial for 8 ... and exact for the others."""
     from zlatko_amd import native
     spec = _book_spec(777)
+    got = {}
+    for group in (1, 2, 4, 8):
+        e = native.Engine()
+        e.set_book(True)
+        e.set_book_group(group)
+        p = e.compile(spec, precision)
+        got[group] = p.execute(rows=True)
+        p.close()
+        e.close()
+    for group in (2, 4, 8):
+        for a, b in zip(got[group], got[1]):
+            for f in abi.PROFILE_FIELDS:
+                assert np.array_equal(a.get(f), b.get(f)), (group, f)
+            assert np.array_equal(a.exposure_rows(spec.num_paths), b.exposure_rows(spec.num_paths)), group
+
+
+@pytest.mark.parametrize("precision", [abi.CCR_FP64, abi.CCR_FP32])
+def test_book_kernel_group_sizes_agree_on_security_collateral(precision):
+    """Bond and FX-asset collateral in the book kernel: 1, 2, 4 or 8 sets per workgroup give the
+    same profiles and exposure rows bit for bit."""
+    from zlatko_amd import native
+    spec = W.book_case(num_paths=513, secured=True)
     got = {}
     for group in (1, 2, 4, 8):
         e = native.Engine()

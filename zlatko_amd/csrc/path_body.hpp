@@ -786,7 +786,9 @@ __host__ __device__ __forceinline__ Real collateralAndReport(const Market<Real, 
   if (S.has_csa) {
     Real price = 1;
     if (S.sec_type == CCR_COLLATERAL_BOND) {
-      price = dr[kDateBond] * expr(-dr[kDateBond + 1] * mk.x[0]);
+      // expTerm's FP64 polynomial reads its coefficients from SGPRs: the library exp keeps them
+      // in VGPRs, which the book kernel's unrolled set loop hoists and spills for every set.
+      price = dr[kDateBond] * expTerm(-dr[kDateBond + 1] * mk.x[0]);
     } else if (S.sec_type == CCR_COLLATERAL_ASSET) {
 #pragma unroll
       for (int i = 0; i < NA; ++i)
@@ -1158,7 +1160,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 #pragma clang fp contract(off)
       Real price = 1;
       if (S.sec_type == CCR_COLLATERAL_BOND) {
-        price = dr[kDateBond] * expr(-dr[kDateBond + 1] * x[0]);
+        price = dr[kDateBond] * expTerm(-dr[kDateBond + 1] * x[0]);
       } else if (S.sec_type == CCR_COLLATERAL_ASSET) {
 #pragma unroll
         for (int i = 0; i < NA; ++i)

@@ -21,9 +21,9 @@ from typing import List
 
 import numpy as np
 
-from .abi import (CCR_OPTION_CALL, CCR_OPTION_PUT, CCR_TRADE_ASSET_FORWARD, CCR_TRADE_BERMUDAN_SWAPTION,
-                  CCR_TRADE_EUROPEAN_OPTION, CCR_TRADE_SWAP, Asset, Cir, Collateral,
-                  CreditCurve, Curve, HullWhite, Model, NettingSet, Spec, Trade, swap_schedule, uniform_grid)
+from .abi import (CCR_COLLATERAL_ASSET, CCR_COLLATERAL_BOND, CCR_OPTION_CALL, CCR_OPTION_PUT,
+                  CCR_TRADE_ASSET_FORWARD, CCR_TRADE_BERMUDAN_SWAPTION, CCR_TRADE_EUROPEAN_OPTION, CCR_TRADE_SWAP,
+                  Asset, Cir, Collateral, CreditCurve, Curve, HullWhite, Model, NettingSet, Spec, Trade, swap_schedule, uniform_grid)
 
 MASK64 = (1 << 64) - 1
 
@@ -205,8 +205,10 @@ def c5(num_paths: int = 20000, dates: int = 120, num_sets: int = 500, largest: i
 
 def small_reference_case(num_paths: int = 64, dates: int = 12, csa: bool = False, fx: bool = False,
                          credits: bool = False, two_leg: bool = False, seed: int = 7, bermudans: int = 0,
-                         cash_settled: bool = False, options: int = 0) -> Spec:
-    """Small reference-representable netting set for parity against a build of the reference."""
+                         cash_settled: bool = False, options: int = 0, security: str = "cash") -> Spec:
+    """Small reference-representable netting set for parity against a build of the reference.
+    security (with csa): the collateral posted -- "cash", "bond" (a 6y zero bond at a 2%
+    haircut) or "asset" (the FX asset at 8%; needs fx)."""
     rng = SplitMix64(11)
     curve = Curve([0.5, 2.0, 5.0, 10.0], [0.020, 0.025, 0.030, 0.033])
     model = Model(HullWhite(curve, 0.05, 0.01))
@@ -246,15 +248,22 @@ def small_reference_case(num_paths: int = 64, dates: int = 12, csa: bool = False
     ns = NettingSet(ts)
     if csa:
         ns.collateral = Collateral(threshold_counterparty=2e4, threshold_own=5e4, mta=1e4, mpr=10.0 / 250.0)
+        if security == "bond":
+            ns.collateral.type, ns.collateral.haircut, ns.collateral.bond_maturity = CCR_COLLATERAL_BOND, 0.02, 6.0
+        elif security == "asset":
+            ns.collateral.type, ns.collateral.haircut, ns.collateral.asset_index = CCR_COLLATERAL_ASSET, 0.08, 0
+        elif security != "cash":
+            raise ValueError(f"unknown collateral security {security!r}")
     if credits:
         ns.counterparty_credit, ns.own_credit = 0, 1
     return Spec(model, [ns], uniform_grid(5.0, dates), num_paths, seed, True, 0.95, "small")
 
 
-def book_case(num_paths: int = 256, dates: int = 10, seed: int = 7) -> Spec:
+def book_case(num_paths: int = 256, dates: int = 10, seed: int = 7, secured: bool = False) -> Spec:
     """A small portfolio book: three netting sets on different date sets (a CSA set of swaps, an
     uncollateralised set of swaps and FX forwards, an uncollateralised set of FX options), the
-    first two facing counterparty credit 0 and the third credit 1."""
+    first two facing counterparty credit 0 and the third credit 1. secured: the second and third
+    sets are under CSAs too, collateralised with a zero bond and with the FX asset."""
     rng = SplitMix64(23)
     curve = Curve([0.5, 2.0, 5.0, 10.0], [0.020, 0.025, 0.030, 0.033])
     model = Model(HullWhite(curve, 0.05, 0.01))
@@ -276,4 +285,9 @@ def book_case(num_paths: int = 256, dates: int = 10, seed: int = 7) -> Spec:
                           rng.uniform(0.95, 1.25), asset=0, maturity=rng.uniform(0.5, 3.0),
                           option_type=CCR_OPTION_PUT if i % 2 else CCR_OPTION_CALL) for i in range(3)])
     c.counterparty_credit, c.own_credit = 1, 0
+    if secured:
+        b.collateral = Collateral(threshold_counterparty=1e4, threshold_own=3e4, mta=5e3, mpr=10.0 / 250.0,
+                                  type=CCR_COLLATERAL_BOND, haircut=0.03, bond_maturity=7.0)
+        c.collateral = Collateral(threshold_counterparty=1e4, threshold_own=2e4, mta=5e3, mpr=10.0 / 250.0,
+                                  type=CCR_COLLATERAL_ASSET, haircut=0.1, asset_index=0)
     return Spec(model, [a, b, c], uniform_grid(5.0, dates), num_paths, seed, True, 0.95, "book")

@@ -705,7 +705,7 @@ template <typename Real>
 hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStream_t stream) {
   if (args.group > 1) {
     const size_t lds = static_cast<size_t>(args.slots + args.group * (args.ring + args.price_rows)) * block * sizeof(Real) +
-                       (!groupUnrolled(args.group) ? static_cast<size_t>(3 * args.group) * block * sizeof(double) : 0);
+                       (!groupUnrolled(args.group) ? static_cast<size_t>(kSetStateRows * args.group) * block * sizeof(double) : 0);
     return withShape(args.nrates, args.nassets, args.ncredits, [&](auto shape) {
       return launchGroupShape<Real, decltype(shape)>(args, block, lds, stream);
     });
@@ -725,10 +725,10 @@ hipError_t launchPath(const PathArgs<Real>& args, int blocks, int block, hipStre
 template <typename Real>
 hipError_t pathOccupancy(int nrates, int nassets, int ncredits, bool options, int ldsRows, int block, int* wgPerCu,
                          bool trades, int group) {
-  // Group kernels: ldsRows Real rows plus, where the set loop is not unrolled, three double
-  // rows per set.
+  // Group kernels: ldsRows Real rows plus, where the set loop is not unrolled, kSetStateRows
+  // double rows per set.
   const size_t lds = static_cast<size_t>(ldsRows) * block * sizeof(Real) +
-                     (!groupUnrolled(group) ? static_cast<size_t>(3 * group) * block * sizeof(double) : 0);
+                     (!groupUnrolled(group) ? static_cast<size_t>(kSetStateRows * group) * block * sizeof(double) : 0);
   if (group > 1)
     return withShape(nrates, nassets, ncredits, [&](auto shape) -> hipError_t {
       using Sh = decltype(shape);

@@ -31,6 +31,8 @@ constexpr int kNestedMaxSave = 16;
 // and discounted-exposure state in registers instead of LDS rows: four sets per workgroup (two
 // sets fit five waves per SIMD with the state in LDS; eight would hold too much code).
 constexpr bool groupUnrolled(int group) { return group == 4; }
+// Double rows per set holding that state when the set loop is rolled: units, previous V - C.
+constexpr int kSetStateRows = 2;
 constexpr int kMaxSub = 8;        // CIR++ sub-steps per grid step
 constexpr int kStats = 9;         // E[V-C], EE, ENE, EE*, ENE*, E[D], E[Q], CVA incr., DVA incr.
 constexpr int kMaxBlock = 256;    // launch bound of the path kernel (threads per workgroup)

@@ -364,6 +364,7 @@ struct Market {
   Real lnD, lnDc, D;  // FP32: Kahan sum of int r; FP64: running product
   Real zn[Sh::kPre];
   double prevQ[Sh::NC > 0 ? Sh::NC : 1];  // each credit's survival on the previous reporting date
+  double prevD;                            // the deflator on the previous reporting date
 };
 
 template <typename Real, typename Sh>
@@ -397,6 +398,7 @@ __host__ __device__ __forceinline__ void marketInit(Market<Real, Sh>& mk, const 
     mk.Qs[c] = 1;
     mk.prevQ[c] = 1.0;
   }
+  mk.prevD = 1.0;
   mk.lnD = 0;
   mk.lnDc = 0;
   mk.D = 1;
@@ -750,25 +752,28 @@ __host__ __device__ __forceinline__ Real valueRecord(const Market<Real, Sh>& mk,
 }
 
 // Per-path state a netting set carries from date to date: CSA collateral units and the previous
-// reporting date's discounted exposures (pathwise CVA / DVA trapezoids; the survivals of that
-// date are the market's, Market::prevQ).
+// reporting date's V - C, whose positive and negative parts times that date's deflator
+// (Market::prevD) are its discounted exposures (pathwise CVA / DVA trapezoids; the survivals of
+// that date are the market's, Market::prevQ). One double per set instead of two: the deflator
+// is shared by the sets of a lane.
 template <typename Real>
 struct SetPath {
   Real units;
-  double prevDvp, prevDvn;
+  double prevV;
 };
 template <typename Real>
 __host__ __device__ __forceinline__ void setPathInit(SetPath<Real>& st) {
   st.units = 0;
-  st.prevDvp = 0;
-  st.prevDvn = 0;
+  st.prevV = 0;
 }
 
-// After the sets' reports of a reporting date: its survivals become the previous ones.
+// After the sets' reports of a reporting date: its survivals and deflator become the previous
+// ones.
 template <typename Real, typename Sh>
 __host__ __device__ __forceinline__ void marketReported(Market<Real, Sh>& mk) {
 #pragma unroll
   for (int c = 0; c < Sh::NC; ++c) mk.prevQ[c] = static_cast<double>(mk.Qs[c]);
+  mk.prevD = static_cast<double>(mk.D);
 }
 
 // Collateral (collateral.cpp:10-47, the margin-call value read from the set's ring) and, on a
@@ -838,10 +843,14 @@ __host__ __device__ __forceinline__ Real collateralAndReport(const Market<Real, 
   st9[4] = d * vn;
   st9[5] = d;
   st9[6] = qc;
-  st9[7] = rr > 0 ? 0.5 * (st.prevDvp + dvp) * (pqc - qc) : 0.0;
-  st9[8] = rr > 0 ? 0.5 * (st.prevDvn + dvn) * (pqo - qo) : 0.0;
-  st.prevDvp = dvp;
-  st.prevDvn = dvn;
+  // The previous date's discounted exposures, recomputed from its V - C and deflator by the same
+  // operations that formed dvp and dvn then (so bit for bit the same values).
+  const double pv = st.prevV;
+  const double pvp = pv > 0.0 ? pv : 0.0, pvn = pv < 0.0 ? pv : 0.0;
+  const double pdvp = mk.prevD * pvp, pdvn = mk.prevD * (-pvn);
+  st9[7] = rr > 0 ? 0.5 * (pdvp + dvp) * (pqc - qc) : 0.0;
+  st9[8] = rr > 0 ? 0.5 * (pdvn + dvn) * (pqo - qo) : 0.0;
+  st.prevV = v;
   emit(rr, st9, vp);
   return C;
 }
@@ -1225,7 +1234,7 @@ __host__ __device__ inline void simulatePath(cptr<GlobalDev> gp, cptr<SetDev> sp
 // set's lists -- a book's lists are the same) and each set's margin-call ring and price rows (rows
 // slots + k (ring + priceRows) ...). Each set's collateral units and previous discounted
 // exposures stay in registers when the set loop is unrolled (groupUnrolled(G)); otherwise the
-// loop stays rolled and they live in `state` (a double column, rows 3k .. 3k + 2), since
+// loop stays rolled and they live in `state` (a double column, rows 2k and 2k + 1), since
 // registers indexed by a loop counter would sit in scratch memory.
 // sp[g], g < ns: the group's sets; emit(g, rep, stats, positive_exposure).
 template <typename Real, typename Sh, int G, typename Emit>
@@ -1249,9 +1258,8 @@ __host__ __device__ __forceinline__ void simulateGroup(cptr<GlobalDev> gp, cptr<
   for (int k = 0; k < (kRegs ? G : 1); ++k) setPathInit(regs[k]);
   if constexpr (!kRegs) {
     for (int k = 0; k < ns; ++k) {
-      state[(3 * k) * stride] = 0.0;
-      state[(3 * k + 1) * stride] = 0.0;
-      state[(3 * k + 2) * stride] = 0.0;
+      state[(kSetStateRows * k) * stride] = 0.0;
+      state[(kSetStateRows * k + 1) * stride] = 0.0;
     }
   }
   for (int j = 0; j < S0.num_dates; ++j) {
@@ -1283,13 +1291,11 @@ __host__ __device__ __forceinline__ void simulateGroup(cptr<GlobalDev> gp, cptr<
     } else {
       for (int k = 0; k < ns; ++k) {
         SetPath<Real> st;
-        st.units = static_cast<Real>(state[(3 * k) * stride]);
-        st.prevDvp = state[(3 * k + 1) * stride];
-        st.prevDvn = state[(3 * k + 2) * stride];
+        st.units = static_cast<Real>(state[(kSetStateRows * k) * stride]);
+        st.prevV = state[(kSetStateRows * k + 1) * stride];
         setStep(k, st);
-        state[(3 * k) * stride] = static_cast<double>(st.units);
-        state[(3 * k + 1) * stride] = st.prevDvp;
-        state[(3 * k + 2) * stride] = st.prevDvn;
+        state[(kSetStateRows * k) * stride] = static_cast<double>(st.units);
+        state[(kSetStateRows * k + 1) * stride] = st.prevV;
       }
     }
     // A book's sets report on the same dates (the book's reporting grid).

@@ -118,22 +118,15 @@ ccr_path_kernel(PathArgs<Real> a) {
 // run together on one XCD and share its L2. Each set's statistics go to its own wave partials and
 // exposure rows as in ccr_path_kernel. LDS per path: the shared fixing slots, then each set's
 // margin-call ring and price rows, then (rolled set loops) its state rows.
-// Waves per SIMD the book kernel's registers are held to. Rolled set loops (per-set state in
-// LDS): five (96 VGPRs). The unrolled four-set loop needs about 126 VGPRs to run without
-// spilling, four waves; held to six waves (80 VGPRs, about 170 bytes of spills per lane, mostly
-// per-set table pointers reloaded once per set and date) the FP64 kernel is faster still: the
-// 500-set book's path kernel took 362 ms at four waves, 344 ms at five, 337 ms at six and
-// 342 ms at seven. The FP32 kernel fits five waves without spilling (93 VGPRs) and is no faster
-// beyond that. With the LDS state, four sets of the 3-currency CSA book (3 fixing rows + per
-// set 2 ring rows and 3 state rows of 8 bytes per path) allowed only three waves per SIMD.
-template <typename Real, int G>
-constexpr int groupWaveHint() {
-  if (!groupUnrolled(G)) return 5;
-  return std::is_same<Real, double>::value ? 6 : 5;
-}
+// The book kernel's registers are held to five waves per SIMD (96 VGPRs). The unrolled four-set
+// loop keeps each set's state (collateral units, previous V - C: two values per set) in
+// registers and fits that with a few bytes of spills in FP64 (FP32: 88 VGPRs, none); the
+// 500-set book's FP64 path kernel took 317 ms at five waves, 320 ms at six (80 VGPRs, 76 bytes
+// of spills per lane) and 322 ms at seven. Rolled set loops keep the state in LDS rows.
+constexpr int kGroupWaves = 5;
 
 template <typename Real, typename Sh, int G>
-__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(groupWaveHint<Real, G>())))
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(kGroupWaves)))
 ccr_group_kernel(PathArgs<Real> a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const CCR_CONST GlobalDev& g = *toConst(a.g);

@@ -429,6 +429,28 @@ __device__ inline void appendKeys(bool take, K k, K* keys, unsigned int* count) 
   }
 }
 
+// appendKeys for N keys per lane at once (those with take[u], in u order): one wavefront scan of
+// the lanes' counts and one LDS atomic per wavefront instead of one per key. Call with the whole
+// wave active.
+template <int kCap, int N, typename K>
+__device__ inline void appendKeysN(const bool (&take)[N], const K (&k)[N], K* keys, unsigned int* count) {
+  unsigned int n = 0;
+#pragma unroll
+  for (int u = 0; u < N; ++u) n += take[u] ? 1u : 0u;
+  if (__ballot(n != 0) == 0) return;
+  const unsigned int incl = waveInclusiveScan(n);
+  const unsigned int total = __shfl(incl, 63);
+  unsigned int base = 0;
+  if ((threadIdx.x & 63) == 0) base = atomicAdd(count, total);
+  unsigned int pos = __shfl(base, 0) + incl - n;
+#pragma unroll
+  for (int u = 0; u < N; ++u)
+    if (take[u]) {
+      if (pos < static_cast<unsigned int>(kCap)) keys[pos] = k[u];
+      ++pos;
+    }
+}
+
 // Finds the bin of hist[0, kPfeBins) holding rank `rank` (0-based, below the histogram's total):
 // each thread owns kPfeBins / kThreads consecutive bins, a workgroup prefix sum of the per-thread counts names the
 // owner, which scans its bins. Leaves the bin and the rank within it in *bin / *binRank; ends with
@@ -537,24 +559,33 @@ __global__ void __launch_bounds__(pfeThreads<kPfeKeys>()) ccr_pfe_kernel(const S
   {
     unsigned int below = 0, eqA = 0, eqB = 0;
     K minAbove = ~K(0);
-    for (int base = 0; base < num_paths; base += kPfeThreads * kPfeUnroll) {
+    // Full spans of kPfeThreads x kPfeUnroll keys need no bounds tests; the last, partial span
+    // marks the keys past the row ~0 and counts them nowhere.
+    constexpr int kSpan = kPfeThreads * kPfeUnroll;
+    const int fullEnd = num_paths / kSpan * kSpan;
+    auto visit = [&](int base, auto tailTag) {
+      constexpr bool kTail = decltype(tailTag)::value;
       K kk[kPfeUnroll];
+      bool take[kPfeUnroll];
 #pragma unroll
       for (int u = 0; u < kPfeUnroll; ++u) {
         const int i = base + u * kPfeThreads + tid;
-        kk[u] = i < num_paths ? keyOf<Real>(row[i]) : ~K(0);  // ~0: past the row, counted nowhere
+        kk[u] = (!kTail || i < num_paths) ? keyOf<Real>(row[i]) : ~K(0);
       }
 #pragma unroll
       for (int u = 0; u < kPfeUnroll; ++u) {
         const K k = kk[u];
-        const bool valid = k != ~K(0);
+        const bool valid = !kTail || k != ~K(0);
         below += (valid && k < a) ? 1u : 0u;
         eqA += (valid && k == a) ? 1u : 0u;
         eqB += (valid && twoEnds && k == b) ? 1u : 0u;
         if (valid && k > b && k < minAbove) minAbove = k;
-        appendKeys<kPfeKeys>(valid && k > a && k < b, k, keys, &sOpen);
+        take[u] = valid && k > a && k < b;
       }
-    }
+      appendKeysN<kPfeKeys>(take, kk, keys, &sOpen);
+    };
+    for (int base = 0; base < fullEnd; base += kSpan) visit(base, std::false_type{});
+    if (fullEnd < num_paths) visit(fullEnd, std::true_type{});
     below = waveTotal(below);
     eqA = waveTotal(eqA);
     eqB = waveTotal(eqB);

@@ -558,7 +558,7 @@ def main():
             "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": None,
             "algorithmic_bytes": path_bytes,
-            "kernel": "ccr_path_kernel (all waves of one step)",
+            "kernel": ("ccr_group_kernel" if book else "ccr_path_kernel") + " (all waves of one step)",
         },
     }
     if book:

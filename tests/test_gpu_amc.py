@@ -102,6 +102,17 @@ def test_c4_with_bermudans_fp64(engine):
     _check(spec, engine.run(spec), oracle_lib.run_oracle(spec, threads=8))
 
 
+def test_c4_full_dates_fp64(engine):
+    """Config 4's netting set on its full grid (120 monthly dates; 196 profile dates with the
+    Bermudans' exercise and fixing dates): two-leg swaps, FX forwards and 10 Bermudans under CIR++
+    counterparty and own intensities with 0.3 wrong-way correlation, at 4,000 paths. Profile, pathwise CVA / DVA
+    increments and PFE against the FP64 oracle, scale-relative and pointwise."""
+    spec = W.c4(num_paths=4000, dates=120)
+    outs = engine.run(spec)
+    assert outs[0].c.num_dates == 196
+    _check(spec, outs, oracle_lib.run_oracle(spec, threads=8))
+
+
 def test_bermudans_under_csa_fp64(engine):
     """Bermudans inside a collateralised set next to plain swaps (margin-call ring + AMC)."""
     spec = W.small_reference_case(num_paths=1500, csa=True, fx=True, bermudans=3)

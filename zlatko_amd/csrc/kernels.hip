@@ -373,7 +373,7 @@ constexpr int pfeThreads() {
 constexpr int kPfeMaxWaves = 16;
 constexpr int kPfeSample = 2048; // smallest sample of a row longer than the key buffer
 constexpr int kPfeBins = 4096;   // 12-bit digits of the radix fallback (histogram aliases the keys)
-constexpr int kPfeUnroll = 4;    // row elements in flight per thread
+constexpr int kPfeUnroll = 8;    // row elements in flight per thread
 
 // Ascending bitonic sort of keys[0, n2), n2 a power of two, by the whole workgroup: each step
 // visits the n2 / 2 compare-exchange pairs once.
@@ -499,7 +499,9 @@ __device__ inline void pfeFindBin(const unsigned int* hist, long long rank, unsi
 // Otherwise (a sample miss, or too many keys between the splitters: large P with a central
 // quantile) an MSB radix select over 12-bit digits narrows the prefix until its keys fit in LDS.
 template <typename Real, int kPfeKeys>
-__global__ void __launch_bounds__(pfeThreads<kPfeKeys>()) ccr_pfe_kernel(const SetDev* sets, const Real* rows, int max_rep,
+// Eight row elements in flight per lane, held to 64 VGPRs (eight waves per SIMD) so that two
+// workgroups of the 8,192-key buffer share a CU.
+__global__ void __launch_bounds__(pfeThreads<kPfeKeys>()) __attribute__((amdgpu_waves_per_eu(8))) ccr_pfe_kernel(const SetDev* sets, const Real* rows, int max_rep,
                                                               int num_paths, double level, double* out,
                                                               int which) {
   using K = typename KeyOf<Real>::T;
